@@ -1,0 +1,194 @@
+"""ctypes wrapper of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker.  The product package never
+imports this module.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+DIRICHLET, NEUMANN, PERIODIC, ADJACENTED, NULL = 0, 1, 2, 3, 4
+XNEG, XPOS, YNEG, YPOS = 0, 1, 2, 3
+SCHUR_UNIT_SOLVES, SCHUR_SPECTRAL = 0, 1
+
+_lib = None
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def build():
+    """Compile liboracle.so with the Makefile next to this file."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        L.or_dft.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp]
+        L.or_r2c.argtypes = [ctypes.c_int, _dp, _dp]
+        L.or_transform.argtypes = [ctypes.c_int] * 5 + [_dp, _dp]
+        L.or_x_diag.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp]
+        L.or_chasing.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp]
+        L.or_chasing.restype = None
+        L.or_chasing_singular.argtypes = [ctypes.c_int, _dp, _dp]
+        L.or_chasing_singular.restype = None
+        L.or_sub_solve.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _ip, _dp]
+        L.or_cat_create.restype = ctypes.c_void_p
+        L.or_cat_create.argtypes = [ctypes.c_int, _ip, _ip, _dp, _dp, _ip, ctypes.c_int, _ip, ctypes.c_int,
+                                    ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.or_cat_root.argtypes = [ctypes.c_void_p]
+        L.or_cat_tree.argtypes = [ctypes.c_void_p, _ip, _ip, _ip]
+        L.or_cat_solve.argtypes = [ctypes.c_void_p, ctypes.POINTER(_dp), ctypes.POINTER(_dp), _ip, _dp]
+        L.or_cat_schur.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _ip]
+        L.or_cat_destroy.argtypes = [ctypes.c_void_p]
+        L.or_cat_destroy.restype = None
+        L.or_gmres_mock.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                    _dp, _ip]
+        L.or_num_threads.argtypes = []
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def dft(x, sign=-1):
+    x = np.asarray(x, dtype=np.complex128)
+    xi = np.ascontiguousarray(x.view(np.float64))
+    out = np.empty_like(xi)
+    assert lib().or_dft(len(x), sign, _p(xi), _p(out)) == 0
+    return out.view(np.complex128)
+
+
+def r2c(x):
+    x = _f64(x)
+    out = np.empty(2 * (len(x) // 2 + 1))
+    assert lib().or_r2c(len(x), _p(x), _p(out)) == 0
+    return out.view(np.complex128)
+
+
+def transform(bc_neg, bc_pos, field, inverse=False):
+    f = _f64(np.atleast_2d(field))
+    out = np.empty_like(f)
+    rc = lib().or_transform(bc_neg, bc_pos, int(inverse), f.shape[0], f.shape[1], _p(f), _p(out))
+    assert rc == 0, "unsupported boundary pair"
+    return out
+
+
+def x_diag(ny, hx, hy, bc_neg, bc_pos):
+    out = np.empty(ny)
+    assert lib().or_x_diag(ny, hx, hy, bc_neg, bc_pos, _p(out)) == 0
+    return out
+
+
+def chasing(diag, f, bc_start=DIRICHLET, bc_end=DIRICHLET):
+    f = _f64(f)
+    x = np.empty_like(f)
+    lib().or_chasing(float(diag), len(f), bc_start, bc_end, _p(f), _p(x))
+    return x
+
+
+def chasing_singular(f):
+    f = _f64(f)
+    x = np.empty_like(f)
+    lib().or_chasing_singular(len(f), _p(f), _p(x))
+    return x
+
+
+def sub_solve(f, hx, hy, bc):
+    """PoissonSolver2D::solve of an already hx^2-scaled RHS (returns a new array)."""
+    f = _f64(f).copy()
+    b = (ctypes.c_int * 4)(*bc)
+    assert lib().or_sub_solve(f.shape[0], f.shape[1], hx, hy, b, _p(f)) == 0
+    return f
+
+
+class Composite:
+    """ConcatPoissonSolver2D restated on the CPU.
+
+    domains: list of (nx, ny, hx, hy, (bc_xneg, bc_xpos, bc_yneg, bc_ypos))
+    conn: list of (a, dir, b) as Geometry2D::connect(a, dir, b).
+    """
+
+    def __init__(self, domains, conn, m=20, tol=1e-8, tol_is_relative=False, maxiter=100,
+                 schur_mode=SCHUR_UNIT_SOLVES):
+        self.domains = domains
+        nd = len(domains)
+        self.nd = nd
+        nx = (ctypes.c_int * nd)(*[d[0] for d in domains])
+        ny = (ctypes.c_int * nd)(*[d[1] for d in domains])
+        hx = (ctypes.c_double * nd)(*[d[2] for d in domains])
+        hy = (ctypes.c_double * nd)(*[d[3] for d in domains])
+        bc = (ctypes.c_int * (4 * nd))(*[t for d in domains for t in d[4]])
+        flat = [v for c in conn for v in c]
+        cc = (ctypes.c_int * max(1, len(flat)))(*flat)
+        self.h = lib().or_cat_create(nd, nx, ny, hx, hy, bc, len(conn), cc, m, tol, int(tol_is_relative),
+                                     maxiter, schur_mode)
+        if not self.h:
+            raise ValueError("or_cat_create failed")
+
+    def root(self):
+        return lib().or_cat_root(self.h)
+
+    def tree(self):
+        lv = (ctypes.c_int * self.nd)()
+        pa = (ctypes.c_int * self.nd)()
+        pl = (ctypes.c_int * self.nd)()
+        lib().or_cat_tree(self.h, lv, pa, pl)
+        return list(lv), list(pa), list(pl)
+
+    def schur(self, child):
+        cn = ctypes.c_int(0)
+        assert lib().or_cat_schur(self.h, child, None, ctypes.byref(cn)) == 0
+        out = np.empty((cn.value, cn.value))
+        lib().or_cat_schur(self.h, child, _p(out), ctypes.byref(cn))
+        return out
+
+    def solve(self, fields, buffers=None):
+        """fields: list of raw RHS arrays; buffers: dict (dom, face) -> ghost data.
+        Returns (solutions, gmres_iterations, final_residual)."""
+        fs = [_f64(f).copy() for f in fields]
+        fptr = (_dp * self.nd)(*[_p(f) for f in fs])
+        keep = []
+        bptr = (_dp * (4 * self.nd))()
+        for (d, face), g in (buffers or {}).items():
+            g = _f64(g)
+            keep.append(g)
+            bptr[4 * d + face] = _p(g)
+        it = ctypes.c_int(0)
+        res = ctypes.c_double(0)
+        assert lib().or_cat_solve(self.h, fptr, bptr, ctypes.byref(it), ctypes.byref(res)) == 0
+        return fs, it.value, res.value
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_cat_destroy(self.h)
+            self.h = None
+
+
+def gmres_mock(ainv, S, b, m, tol, maxiter):
+    n = len(b)
+    b = _f64(b).copy()
+    resvec = np.zeros(maxiter + 1)
+    nres = ctypes.c_int(0)
+    rc = lib().or_gmres_mock(n, _p(_f64(ainv)), _p(_f64(S)), _p(b), m, tol, maxiter, _p(resvec),
+                             ctypes.byref(nres))
+    return b, resvec[:nres.value], rc
+
+
+def num_threads():
+    return lib().or_num_threads()

@@ -1,0 +1,229 @@
+"""Pins the CPU oracle against what does not depend on it: numpy's FFT,
+scipy's DST/DCT, dense linear algebra, the assembled sparse system and the
+golden fixtures written from it (tests/golden/gen_golden.py)."""
+import itertools
+import os
+
+import numpy as np
+import pytest
+import scipy.fft
+
+import discrete
+import oracle as O
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+D, N, P, A = 0, 1, 2, 3
+
+
+def rel(a, b):
+    a = np.concatenate([np.ravel(x) for x in a]) if isinstance(a, list) else np.ravel(a)
+    b = np.concatenate([np.ravel(x) for x in b]) if isinstance(b, list) else np.ravel(b)
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 8, 12, 41, 64, 100, 257, 1025, 2050, 683, 4098])
+def test_dft_matches_numpy(n):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    ref = np.fft.fft(x)
+    assert np.abs(O.dft(x) - ref).max() <= 2e-15 * np.abs(ref).max() * max(1, np.log2(n))
+    assert np.abs(O.dft(x, +1) - np.fft.ifft(x) * n).max() <= 2e-15 * np.abs(ref).max() * max(1, np.log2(n))
+    xr = rng.standard_normal(n)
+    assert np.allclose(O.r2c(xr), np.fft.rfft(xr), rtol=0, atol=1e-13 * np.abs(np.fft.rfft(xr)).max())
+
+
+def basis(bcp, ny, inverse=False):
+    """Phi with Phi[k][n] = transform(e_n)[k]."""
+    return O.transform(bcp[0], bcp[1], np.eye(ny), inverse=inverse).T
+
+
+def y_operator(bcp, ny):
+    M = -2 * np.eye(ny) + np.eye(ny, k=1) + np.eye(ny, k=-1)
+    if bcp[0] == N:
+        M[0, 0] += 1
+    if bcp[1] == N:
+        M[-1, -1] += 1
+    return M
+
+
+@pytest.mark.parametrize("bcp", [(D, D), (N, N), (D, N), (N, D), (A, D), (D, A), (A, N), (N, A)])
+@pytest.mark.parametrize("ny", [2, 5, 16, 33, 64])
+def test_transform_is_orthonormal_eigenbasis(bcp, ny):
+    Phi = basis(bcp, ny)
+    PhiT = basis(bcp, ny, inverse=True)
+    I = np.eye(ny)
+    assert np.abs(Phi @ Phi.T - I).max() < 1e-13
+    assert np.abs(PhiT @ Phi - I).max() < 1e-13
+    # Phi M_y Phi^T = diag(lambda - 2) with the reference's per-mode order; the
+    # eigenvalues come from or_x_diag with hx = hy (x_diag + 2 = lambda - 2).
+    lam = O.x_diag(ny, 1.0, 1.0, *bcp) + 2.0
+    assert np.abs(Phi @ y_operator(bcp, ny) @ Phi.T - np.diag(lam)).max() < 1e-12
+    # the eigenvalues themselves are the operator's spectrum
+    assert np.allclose(np.sort(lam), np.sort(np.linalg.eigvalsh(y_operator(bcp, ny))), atol=1e-12)
+
+
+def test_dd_is_scipy_dst1_and_nn_is_dct2():
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((4, 37))
+    assert np.allclose(O.transform(D, D, x), scipy.fft.dst(x, type=1, norm="ortho", axis=1), atol=1e-14)
+    c = scipy.fft.dct(x, type=2, norm="ortho", axis=1)
+    got = O.transform(N, N, x)
+    # mode j <-> DCT frequency N-1-j (mean mode last); signs are the reference's own
+    assert np.allclose(np.abs(got[:, ::-1]), np.abs(c), atol=1e-13)
+
+
+def test_x_diag_scaling():
+    hx, hy = 0.3, 0.5
+    lam = O.x_diag(9, 1.0, 1.0, D, N) + 2.0
+    assert np.allclose(O.x_diag(9, hx, hy, D, N), -2.0 + (hx / hy) ** 2 * lam)
+
+
+@pytest.mark.parametrize("bs,be", [(D, D), (N, D), (D, N), (N, N), (A, N)])
+def test_chasing_matches_dense(bs, be):
+    rng = np.random.default_rng(5)
+    n, d = 17, -2.3
+    T = d * np.eye(n) + np.eye(n, k=1) + np.eye(n, k=-1)
+    if bs == N:
+        T[0, 0] += 1
+    if be == N:
+        T[-1, -1] += 1
+    f = rng.standard_normal(n)
+    assert np.allclose(O.chasing(d, f, bs, be), np.linalg.solve(T, f), rtol=1e-13, atol=1e-13)
+
+
+def test_chasing_singular_gauge():
+    rng = np.random.default_rng(6)
+    n = 12
+    T = -2 * np.eye(n) + np.eye(n, k=1) + np.eye(n, k=-1)
+    T[0, 0] += 1
+    T[-1, -1] += 1
+    f = rng.standard_normal(n)
+    f -= f.mean()  # compatible right-hand side
+    x = O.chasing_singular(f)
+    assert x[0] == 0.0
+    assert np.abs(T @ x - f).max() < 1e-12
+
+
+def test_poisson_3x4_golden():
+    g = np.load(os.path.join(GOLD, "poisson_3x4.npz"))
+    for bc, u in zip(g["bc"], g["u"]):
+        got = O.sub_solve(g["f"] * float(g["hx"]) ** 2, float(g["hx"]), float(g["hy"]), tuple(int(b) for b in bc))
+        assert rel(got, u) < 1e-13, bc
+
+
+@pytest.mark.parametrize("bc", [b for b in itertools.product([D, N, A], repeat=4) if b != (N, N, N, N)])
+def test_sub_solve_matches_sparse(bc):
+    rng = np.random.default_rng(sum(bc))
+    nx, ny, hx, hy = 9, 7, 0.3, 0.5
+    f = rng.standard_normal((nx, ny))
+    u = O.sub_solve(hx * hx * f, hx, hy, bc)
+    assert rel(u, discrete.solve([(nx, ny, hx, hy, bc)], [], [f])[0]) < 1e-13
+
+
+def test_sub_solve_all_neumann_gauge():
+    rng = np.random.default_rng(7)
+    nx, ny = 6, 5
+    f = rng.standard_normal((nx, ny))
+    f -= f.mean()
+    u = O.sub_solve(f, 1.0, 1.0, (N, N, N, N))
+    M, rhs, _ = discrete.assemble([(nx, ny, 1.0, 1.0, (N, N, N, N))], [], [f])
+    assert np.abs(M @ u.ravel() - rhs).max() < 1e-12
+    assert abs(u[0].sum()) < 1e-12  # zero-mode gauge x0 = 0 on row i = 0
+
+
+def load_domains(g, bcs=None, h=1.0):
+    sizes = [tuple(int(v) for v in s) for s in g["sizes"]]
+    if bcs is None:
+        bcs = [(D, D, D, D)] * len(sizes)
+    return [(a, b, h, h, tuple(int(t) for t in bc)) for (a, b), bc in zip(sizes, bcs)]
+
+
+def test_concat_six_golden_and_tree():
+    g = np.load(os.path.join(GOLD, "concat_six.npz"))
+    doms = load_domains(g)
+    conn = [tuple(int(v) for v in c) for c in g["conn"]]
+    fs = [g["f%d" % k] for k in range(6)]
+    cat = O.Composite(doms, conn, tol=1e-13, tol_is_relative=True)
+    assert cat.root() == 0  # two centres (T1, T3): the lower index wins
+    level, parent, _ = cat.tree()
+    assert level == [0, 1, 1, 1, 1, 2] and parent[5] == 2
+    us, iters, _ = cat.solve(fs)
+    assert rel(us, [g["u%d" % k] for k in range(6)]) < 1e-11
+    # default reference parameters (absolute tol 1e-8) still land close
+    us2, _, res = O.Composite(doms, conn).solve(fs)
+    assert res < 1e-8 and rel(us2, [g["u%d" % k] for k in range(6)]) < 1e-6
+
+
+def cross32():
+    g = np.load(os.path.join(GOLD, "cross_32.npz"))
+    doms = load_domains(g, g["bcs"], float(g["h"]))
+    conn = [tuple(int(v) for v in c) for c in g["conn"]]
+    fs = [g["f%d" % k] for k in range(5)]
+    bufs = {}
+    for k in g.files:
+        if k.startswith("g"):
+            d, face = k[1:].split("_")
+            bufs[(int(d), int(face))] = g[k]
+    return g, doms, conn, fs, bufs
+
+
+@pytest.mark.parametrize("mode", [O.SCHUR_UNIT_SOLVES, O.SCHUR_SPECTRAL])
+def test_cross32_golden(mode):
+    g, doms, conn, fs, bufs = cross32()
+    cat = O.Composite(doms, conn, tol=1e-13, tol_is_relative=True, schur_mode=mode)
+    us, iters, _ = cat.solve(fs, bufs)
+    assert rel(us, [g["u%d" % k] for k in range(5)]) < 1e-11
+
+
+def test_schur_spectral_equals_unit_solves_and_is_symmetric():
+    _, doms, conn, _, _ = cross32()
+    c0 = O.Composite(doms, conn, schur_mode=O.SCHUR_UNIT_SOLVES)
+    c1 = O.Composite(doms, conn, schur_mode=O.SCHUR_SPECTRAL)
+    for k in range(1, 5):
+        S0, S1 = c0.schur(k), c1.schur(k)
+        assert np.abs(S0 - S1).max() < 1e-14
+        assert np.abs(S0 - S0.T).max() < 1e-14
+
+
+def _krylov_min(B, c, x0, m):
+    """argmin over x0 + span{r0, B r0, ..., B^(m-1) r0} of ||c - B x||: the
+    defining property of one GMRES(m) cycle, computed by least squares."""
+    r0 = c - B @ x0
+    K = [r0]
+    for _ in range(m - 1):
+        K.append(B @ K[-1])
+    K = np.array(K).T
+    z, *_ = np.linalg.lstsq(B @ K, r0, rcond=None)
+    return x0 + K @ z
+
+
+@pytest.mark.parametrize("loc", [0, 1, 2, 3])
+def test_gmres_mock_semantics(loc):
+    """gmres_reference.cpp: root 3x5, neighbour of width 4 at `loc`, mock
+    solver multiplying f(i,j) by i*ny+j+1, m=2, tol=1e-3, maxIter=2."""
+    nx, ny = 3, 5
+    bnx, bny = (4, ny) if loc < 2 else (nx, 4)
+    scale = (np.arange(nx * ny) + 1.0)
+    bscale = (np.arange(bnx * bny) + 1.0).reshape(bnx, bny)
+    n = nx * ny
+    S = np.zeros((n, n))
+    idx = np.arange(n).reshape(nx, ny)
+    if loc == 0:
+        line, bline = idx[0, :], bscale[bnx - 1, :]
+    elif loc == 1:
+        line, bline = idx[nx - 1, :], bscale[0, :]
+    elif loc == 2:
+        line, bline = idx[:, 0], bscale[:, bny - 1]
+    else:
+        line, bline = idx[:, ny - 1], bscale[:, 0]
+    # SchurMat2D with the mock: value(j,i) = delta_ij * mock scale on the child line
+    for a, b in zip(line, bline):
+        S[a, a] = b
+    b = scale.copy()  # p_root(i,j) = i*ny+j+1
+    x, resvec, rc = O.gmres_mock(scale, S, b, m=2, tol=1e-3, maxiter=2)
+    Bm = np.eye(n) - np.diag(scale) @ S
+    c = scale * b
+    x1 = _krylov_min(Bm, c, c.copy(), 2)
+    x2 = _krylov_min(Bm, c, x1, 2)
+    assert np.allclose(resvec[0], np.linalg.norm(c - Bm @ c))
+    assert np.allclose(x, x2, rtol=1e-10, atol=1e-10 * np.abs(x2).max())
