@@ -1,0 +1,165 @@
+"""MI355X-native Focalors_Poisson solve path.
+
+Python face of the C ABI (include/focalors_poisson.h), mirroring the
+reference's solver classes:
+
+  PoissonSolver2D(ctx, nx, ny, hx, hy, bc)   pe/poisson/poisson_solver2d.h:10-34
+      .solve(f)                              in place, f = hx^2-scaled RHS
+  ConcatPoissonSolver2D(ctx, domains, conn, m, tol, max_iter)
+                                             pe/concat/concat_solver2d.h:18-56
+      .solve(fields, ghost)                  in place, raw RHS -> solution
+
+Device memory is torch tensors on a HIP device (torch is the allocator and
+stream provider only); every computation runs in libfocalors_poisson.so.
+"""
+import ctypes
+
+from . import _capi
+from ._capi import FP_F32_REFINE, FP_F64, FPError, SolveInfo  # noqa: F401
+
+DIRICHLET, NEUMANN, PERIODIC, ADJACENTED, NULL = 0, 1, 2, 3, 4
+XNEG, XPOS, YNEG, YPOS = 0, 1, 2, 3
+
+
+def _stream_ptr(stream):
+    import torch
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+class Context:
+    """fp_ctx on one HIP device."""
+
+    def __init__(self, device=0):
+        self._h = ctypes.c_void_p()
+        _capi.check(_capi.lib().fp_ctx_create(int(device), ctypes.byref(self._h)))
+        self.device = device
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and h.value:
+            _capi.lib().fp_ctx_destroy(h)
+            self._h = None
+
+
+def _check_field(t, dtype, n):
+    import torch
+    if not t.is_cuda:
+        raise ValueError("fields must be device tensors")
+    if t.dtype != dtype:
+        raise TypeError("expected %s field, got %s" % (dtype, t.dtype))
+    if not t.is_contiguous():
+        raise ValueError("fields must be contiguous")
+    if t.numel() % n:
+        raise ValueError("field size %d is not a multiple of nx*ny = %d" % (t.numel(), n))
+    return t.numel() // n
+
+
+class PoissonSolver2D:
+    """fp_sub: the subdomain fast solver (transform in y, tridiagonal in x)."""
+
+    def __init__(self, ctx, nx, ny, hx, hy, bc):
+        self.ctx, self.nx, self.ny, self.hx, self.hy, self.bc = ctx, nx, ny, hx, hy, tuple(bc)
+        self._h = ctypes.c_void_p()
+        b = (ctypes.c_ubyte * 4)(*bc)
+        _capi.check(_capi.lib().fp_sub_create(ctx._h, nx, ny, float(hx), float(hy), b, ctypes.byref(self._h)))
+
+    def solve(self, f, stream=None):
+        """In-place solve of one or more hx^2-scaled right-hand sides
+        (tensor of shape (..., nx, ny), float64 or float32)."""
+        import torch
+        n = self.nx * self.ny
+        if f.dtype == torch.float64:
+            nrhs = _check_field(f, torch.float64, n)
+            fn = _capi.lib().fp_sub_solve
+        else:
+            nrhs = _check_field(f, torch.float32, n)
+            fn = _capi.lib().fp_sub_solve_f32
+        _capi.check(fn(self._h, ctypes.c_void_p(f.data_ptr()), nrhs, n, _stream_ptr(stream)))
+        return f
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and h.value:
+            _capi.lib().fp_sub_destroy(h)
+            self._h = None
+
+
+def solve_batched(ctx, solvers, fields, stream=None):
+    """Many independent subdomain solves in one call (fp_sub_solve_batched)."""
+    import torch
+    k = len(solvers)
+    if k == 0:
+        return
+    f64 = fields[0].dtype == torch.float64
+    subs = (ctypes.c_void_p * k)(*[s._h.value for s in solvers])
+    ptrs = (ctypes.c_void_p * k)(*[f.data_ptr() for f in fields])
+    nr = (ctypes.c_int * k)(*[_check_field(f, fields[0].dtype, s.nx * s.ny) for s, f in zip(solvers, fields)])
+    fn = _capi.lib().fp_sub_solve_batched if f64 else _capi.lib().fp_sub_solve_batched_f32
+    _capi.check(fn(ctx._h, k, subs, ptrs, nr, _stream_ptr(stream)))
+
+
+class ConcatPoissonSolver2D:
+    """fp_cat: composite geometry solve (tree sweep + interface GMRES).
+
+    domains: list of (nx, ny, hx, hy, (bc_xneg, bc_xpos, bc_yneg, bc_ypos))
+    conn:    list of (a, dir, b) = Geometry2D::connect(a, dir, b)
+    """
+
+    def __init__(self, ctx, domains, conn, m=20, tol=1e-8, tol_is_relative=False, max_iter=100,
+                 precision=FP_F64):
+        self.ctx, self.domains, self.conn = ctx, list(domains), list(conn)
+        nd = len(domains)
+        self._keep = [
+            (ctypes.c_int * nd)(*[d[0] for d in domains]), (ctypes.c_int * nd)(*[d[1] for d in domains]),
+            (ctypes.c_double * nd)(*[d[2] for d in domains]), (ctypes.c_double * nd)(*[d[3] for d in domains]),
+            (ctypes.c_ubyte * (4 * nd))(*[t for d in domains for t in d[4]]),
+            (ctypes.c_int * max(1, 3 * len(conn)))(*[v for c in conn for v in c]),
+        ]
+        k = self._keep
+        desc = _capi.GeometryDesc(nd, k[0], k[1], k[2], k[3], k[4], len(conn), k[5])
+        self._h = ctypes.c_void_p()
+        _capi.check(_capi.lib().fp_cat_create(ctx._h, ctypes.byref(desc), int(m), float(tol), int(tol_is_relative),
+                                              int(max_iter), int(precision), ctypes.byref(self._h)))
+        self.root = _capi.lib().fp_cat_root(self._h)
+
+    def _ptrs(self, fields, ghost):
+        import torch
+        nd = len(self.domains)
+        if len(fields) != nd:
+            raise ValueError("need one field tensor per domain")
+        nrhs = None
+        for d, f in enumerate(fields):
+            r = _check_field(f, torch.float64, self.domains[d][0] * self.domains[d][1])
+            if nrhs is None:
+                nrhs = r
+            elif r != nrhs:
+                raise ValueError("all domains need the same number of right-hand sides")
+        fp = (ctypes.c_void_p * nd)(*[f.data_ptr() for f in fields])
+        gp = (ctypes.c_void_p * (4 * nd))()
+        for (d, face), g in (ghost or {}).items():
+            if not g.is_cuda or g.dtype != torch.float64 or not g.is_contiguous():
+                raise ValueError("ghost data must be contiguous float64 device tensors")
+            flen = self.domains[d][1] if face < 2 else self.domains[d][0]
+            if g.numel() != flen * nrhs:
+                raise ValueError("ghost data of domain %d face %d needs %d values" % (d, face, flen * nrhs))
+            gp[4 * d + face] = g.data_ptr()
+        return fp, gp, nrhs
+
+    def solve(self, fields, ghost=None, stream=None, allow_not_converged=False):
+        """In-place composite solve; returns the GMRES statistics dict."""
+        fp, gp, nrhs = self._ptrs(fields, ghost)
+        info = _capi.SolveInfo()
+        allow = (_capi.FP_E_NOT_CONVERGED,) if allow_not_converged else ()
+        st = _capi.check(_capi.lib().fp_cat_solve(self._h, fp, gp, nrhs, _stream_ptr(stream), ctypes.byref(info)),
+                         allow)
+        d = info.as_dict()
+        d["status"] = st
+        return d
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and h.value:
+            _capi.lib().fp_cat_destroy(h)
+            self._h = None

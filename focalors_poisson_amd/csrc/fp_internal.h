@@ -1,0 +1,122 @@
+// Internal declarations of the MI355X solve path: device plans, kernel
+// launchers and the error type that the C ABI turns into status codes.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "focalors_poisson.h"
+
+namespace fp {
+
+// Boundary codes (PDEBoundaryType, base/location_boundary.h:62-69) and faces
+// (LocationType, base/location_boundary.h:8-16).
+enum Bc : int { DIRICHLET = 0, NEUMANN = 1, PERIODIC = 2, ADJACENTED = 3, BC_NULL = 4 };
+enum Face : int { XNEG = 0, XPOS = 1, YNEG = 2, YPOS = 3 };
+
+// y-transform kinds, chosen by the (yneg, ypos) pair exactly as
+// PoissonSolver2DBase::create_fft (pe/poisson/poisson_solver2d_base.cpp:64-91).
+enum TKind : int { T_DD = 0, T_NN = 1, T_DN = 2, T_ND = 3 };
+
+inline bool dir_like(int b) { return b == DIRICHLET || b == ADJACENTED; }
+
+struct Error : std::runtime_error {
+    int status;
+    Error(int s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+#define FP_HIP(call)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (call);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            throw ::fp::Error(e_ == hipErrorOutOfMemory ? FP_E_NOMEM : FP_E_HIP,                       \
+                              std::string(#call) + ": " + hipGetErrorString(e_));                      \
+    } while (0)
+
+#define FP_REQUIRE(cond, msg)                                                                          \
+    do {                                                                                               \
+        if (!(cond)) throw ::fp::Error(FP_E_INVALID, msg);                                             \
+    } while (0)
+
+// Owning device buffer.
+template <typename T>
+struct DevBuf {
+    T*     p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t count) { alloc(count); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        if (count) FP_HIP(hipMalloc(&p, count * sizeof(T)));
+        n = count;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void upload(const T* h, size_t count) { FP_HIP(hipMemcpy(p, h, count * sizeof(T), hipMemcpyHostToDevice)); }
+};
+
+constexpr int kMaxRadices = 24;
+
+// A batched y transform of one subdomain size and boundary pair: the packed
+// complex FFT of length L (two field rows per FFT) with its radix schedule
+// and tables.  T is the arithmetic type.
+template <typename T>
+struct TransformPlan {
+    int        kind = T_DD;
+    int        N    = 0;  // transform length = ny
+    int        L    = 0;  // complex FFT length
+    int        nrad = 0;
+    int        rad[kMaxRadices] = {};
+    DevBuf<T>  W;         // 2L: W[k] = exp(-2 pi i k / L)
+    DevBuf<T>  aux;       // 2N: NN phase exp(-i pi m / 2N) or ND (sin, cos) of 2N pi (j+1)/(2N+1)
+    T          c0 = 0, c1 = 0;
+    void build(int kind, int N);
+};
+
+// Per-subdomain plan: PoissonSolver2D's precomputed state on the device.
+struct SubPlan {
+    int    nx = 0, ny = 0;
+    double hx = 0, hy = 0;
+    int    bc[4] = {0, 0, 0, 0};
+    int    kind = T_DD;
+    bool   no_dirichlet = false;
+    std::vector<double>   x_diag;   // host copy, reference order
+    TransformPlan<double> t64;
+    TransformPlan<float>  t32;
+    DevBuf<double>        pinv64;   // nx*ny: reciprocal Thomas pivots per (row, mode)
+    DevBuf<float>         pinv32;
+    int                   dev = 0;
+    void build(int nx, int ny, double hx, double hy, const int bc[4]);
+    void ensure_f32();
+};
+
+// ---- kernel launchers (fp_transform.hip / fp_tridiag.hip) ----
+// y transform of `nrhs` fields of one subdomain, in place.  inverse = 0: the
+// reference's transform; 1: transform_transpose.
+template <typename T>
+void launch_ytransform(const TransformPlan<T>& tp, T* data, long long ld, int nx, int nrhs, int inverse,
+                       hipStream_t s);
+// Tridiagonal solves along x for every mode, in place.
+template <typename T>
+void launch_tridiag(const SubPlan& sp, const T* pinv, T* data, long long ld, int nrhs, hipStream_t s);
+
+// Full subdomain solve of nrhs fields (transform, sweep, inverse transform).
+template <typename T>
+void sub_solve(SubPlan& sp, T* data, long long ld, int nrhs, hipStream_t s);
+
+}  // namespace fp

@@ -1,0 +1,722 @@
+// Host orchestration of the MI355X solve path and its C ABI.
+//
+//   SubPlan        PoissonSolver2D state (pe/poisson/poisson_solver2d.cpp:37-60)
+//   sub_solve      PoissonSolver2D::solve (poisson_solver2d.cpp:62-90)
+//   Node / Cat     ConcatPoissonSolver2D + GMRESSolver2D + SchurMat2D_*
+//                  (pe/concat/concat_solver2d.cpp, gmres_solver2d.cpp,
+//                  schur_mat2d.cpp), with the Krylov iteration moved from
+//                  full root fields to the interface lines of the root.
+#include "fp_vec.h"
+
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+
+namespace fp {
+
+// ---------------------------------------------------------------------------
+// subdomain plan
+
+static int transform_kind(int neg, int pos)
+{
+    if (neg == PERIODIC || pos == PERIODIC) throw Error(FP_E_UNSUPPORTED, "periodic faces are not supported");
+    if (neg == BC_NULL || pos == BC_NULL || neg < 0 || neg > 4 || pos < 0 || pos > 4)
+        throw Error(FP_E_INVALID, "invalid boundary type");
+    if (neg == NEUMANN && pos == NEUMANN) return T_NN;
+    if (dir_like(neg) && dir_like(pos)) return T_DD;
+    if (dir_like(neg) && pos == NEUMANN) return T_DN;
+    return T_ND;
+}
+
+void SubPlan::build(int nx_, int ny_, double hx_, double hy_, const int bc_[4])
+{
+    FP_REQUIRE(nx_ >= 2 && ny_ >= 1, "subdomain needs nx >= 2 and ny >= 1");
+    FP_REQUIRE(hx_ > 0 && hy_ > 0 && std::isfinite(hx_) && std::isfinite(hy_), "spacings must be positive");
+    for (int f = 0; f < 4; f++) {
+        if (bc_[f] == PERIODIC) throw Error(FP_E_UNSUPPORTED, "periodic faces are not supported");
+        FP_REQUIRE(bc_[f] == DIRICHLET || bc_[f] == NEUMANN || bc_[f] == ADJACENTED, "invalid boundary type");
+    }
+    nx = nx_; ny = ny_; hx = hx_; hy = hy_;
+    for (int f = 0; f < 4; f++) bc[f] = bc_[f];
+    kind         = transform_kind(bc[YNEG], bc[YPOS]);
+    no_dirichlet = !(dir_like(bc[0]) || dir_like(bc[1]) || dir_like(bc[2]) || dir_like(bc[3]));
+    FP_HIP(hipGetDevice(&dev));
+    // PoissonSolver2DBase::cal_lambda + PoissonSolver2D::init
+    // (poisson_solver2d_base.cpp:28-62, poisson_solver2d.cpp:43-51)
+    const double PI = 3.14159265358979323846;
+    x_diag.resize(ny);
+    for (int j = 0; j < ny; j++) {
+        const int i = j + 1;
+        double    lam;
+        if (kind == T_NN) lam = -2.0 * std::cos(PI / ny * i);
+        else if (kind == T_DD) lam = 2.0 * std::cos(PI / (ny + 1) * i);
+        else lam = -2.0 * std::cos(2.0 * PI * i / (2 * ny + 1));
+        x_diag[j] = -2.0 + hx * hx / hy / hy * (lam - 2.0);
+    }
+    // reciprocal Thomas pivots (the reference's c vector,
+    // chasing_method_base.cpp:33-46), one column per mode
+    std::vector<double> p((size_t)nx * ny, 0.0);
+    const bool          ns = bc[XNEG] == NEUMANN, ne = bc[XPOS] == NEUMANN;
+    for (int j = 0; j < ny; j++) {
+        if (no_dirichlet && j == ny - 1) continue;  // singular mode: recurrence, no pivots
+        const double d = x_diag[j];
+        double       c = ns ? 1.0 / (d + 1.0) : 1.0 / d;
+        p[j]           = c;
+        for (int i = 1; i < nx - 1; i++) {
+            c                     = 1.0 / (d - c);
+            p[(size_t)i * ny + j] = c;
+        }
+        p[(size_t)(nx - 1) * ny + j] = 1.0 / ((ne ? d + 1.0 : d) - c);
+    }
+    pinv64.alloc(p.size());
+    pinv64.upload(p.data(), p.size());
+    t64.build(kind, ny);
+}
+
+void SubPlan::ensure_f32()
+{
+    if (pinv32.p) return;
+    std::vector<double> p((size_t)nx * ny);
+    FP_HIP(hipMemcpy(p.data(), pinv64.p, p.size() * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<float> q(p.begin(), p.end());
+    pinv32.alloc(q.size());
+    pinv32.upload(q.data(), q.size());
+    t32.build(kind, ny);
+}
+
+template <typename T>
+static const TransformPlan<T>& tplan(const SubPlan& sp);
+template <>
+const TransformPlan<double>& tplan<double>(const SubPlan& sp) { return sp.t64; }
+template <>
+const TransformPlan<float>& tplan<float>(const SubPlan& sp) { return sp.t32; }
+template <typename T>
+static const T* pinv_of(const SubPlan& sp);
+template <>
+const double* pinv_of<double>(const SubPlan& sp) { return sp.pinv64.p; }
+template <>
+const float* pinv_of<float>(const SubPlan& sp) { return sp.pinv32.p; }
+
+template <typename T>
+void sub_solve(SubPlan& sp, T* data, long long ld, int nrhs, hipStream_t s)
+{
+    if (nrhs <= 0) return;
+    if constexpr (sizeof(T) == 4) sp.ensure_f32();
+    launch_ytransform<T>(tplan<T>(sp), data, ld, sp.nx, nrhs, 0, s);
+    launch_tridiag<T>(sp, pinv_of<T>(sp), data, ld, nrhs, s);
+    launch_ytransform<T>(tplan<T>(sp), data, ld, sp.nx, nrhs, 1, s);
+}
+template void sub_solve<double>(SubPlan&, double*, long long, int, hipStream_t);
+template void sub_solve<float>(SubPlan&, float*, long long, int, hipStream_t);
+
+// ---------------------------------------------------------------------------
+// composite solver
+
+struct ChildLink {
+    int            dom = -1, loc = 0, cn = 0, off = 0;
+    DevBuf<double> S;      // cn x cn, value(i, j) row-major
+    DevBuf<int>    ccell;  // child's adjacent line cells
+};
+
+struct Node {
+    int                    dom = -1;
+    SubPlan                plan;
+    std::vector<ChildLink> ch;
+    int                    K = 0;
+    DevBuf<int>            cell, ucell, uk1, uk2;
+    int                    nu = 0;
+    // GMRES workspace, sized for ws_nrhs systems
+    int            ws_nrhs = 0, ws_m = 0, nblk = 1;
+    DevBuf<double> V, Wv, Tv, X, C, H, Hacc, cs, sn, g, Y, part, part2, tolabs, resid, G;
+    DevBuf<int>    state;
+    int*           h_state = nullptr;
+    double*        h_resid = nullptr;
+    ~Node()
+    {
+        if (h_state) (void)hipHostFree(h_state);
+        if (h_resid) (void)hipHostFree(h_resid);
+    }
+};
+
+}  // namespace fp
+
+struct fp_ctx_s {
+    int device = 0;
+};
+
+struct fp_sub_s {
+    fp_ctx_s*   ctx = nullptr;
+    fp::SubPlan plan;
+};
+
+struct fp_cat_s {
+    fp_ctx_s*                              ctx = nullptr;
+    int                                    nd  = 0;
+    std::vector<int>                       nx, ny, bc, adj, child, parent, parent_loc, level;
+    std::vector<double>                    hx, hy;
+    int                                    root = 0, nlev = 0;
+    std::vector<std::vector<int>>          levels;
+    std::vector<std::unique_ptr<fp::Node>> nodes;
+    std::vector<fp::DevBuf<double>>        temp;
+    int                                    temp_nrhs = 0;
+    int                                    m = 20, tol_rel = 0, maxiter = 100, precision = FP_F64;
+    double                                 tol = 1e-8;
+    int                                    check_every = 4;
+    // statistics of the last solve
+    int    last_steps = 0, last_restarts = 0, last_status = 0;
+    double last_res = 0.0, last_rel = 0.0;
+};
+
+namespace fp {
+
+static long long dom_n(const fp_cat_s* c, int d) { return (long long)c->nx[d] * c->ny[d]; }
+
+// root-side line cells of the face `loc` of an nx x ny field
+static std::vector<int> line_cells(int loc, int nx, int ny)
+{
+    std::vector<int> v;
+    if (loc == XNEG) for (int j = 0; j < ny; j++) v.push_back(j);
+    else if (loc == XPOS) for (int j = 0; j < ny; j++) v.push_back((nx - 1) * ny + j);
+    else if (loc == YNEG) for (int i = 0; i < nx; i++) v.push_back(i * ny);
+    else for (int i = 0; i < nx; i++) v.push_back(i * ny + ny - 1);
+    return v;
+}
+
+// child-side adjacent line of a child sitting at the parent's side `loc`
+// (SchurMat2D_{xneg,xpos,yneg,ypos}::construct, schur_mat2d.cpp:5-106)
+static std::vector<int> child_cells(int loc, int bnx, int bny)
+{
+    std::vector<int> v;
+    if (loc == XNEG) for (int j = 0; j < bny; j++) v.push_back((bnx - 1) * bny + j);
+    else if (loc == XPOS) for (int j = 0; j < bny; j++) v.push_back(j);
+    else if (loc == YNEG) for (int i = 0; i < bnx; i++) v.push_back(i * bny + bny - 1);
+    else for (int i = 0; i < bnx; i++) v.push_back(i * bny);
+    return v;
+}
+
+static void node_solve(fp_cat_s* c, int d, double* f, int nrhs, hipStream_t s);
+
+static void ensure_ws(fp_cat_s* c, Node& nd, int nrhs)
+{
+    if (nd.ws_nrhs >= nrhs && nd.ws_m == c->m) return;
+    const int       K = nd.K, m = c->m;
+    const long long n = dom_n(c, nd.dom);
+    nd.nblk           = std::max(1, std::min(64, (K + 511) / 512));
+    nd.V.alloc((size_t)(m + 1) * nrhs * K);
+    nd.Wv.alloc((size_t)nrhs * K);
+    nd.Tv.alloc((size_t)nrhs * K);
+    nd.X.alloc((size_t)nrhs * K);
+    nd.C.alloc((size_t)nrhs * K);
+    nd.H.alloc((size_t)nrhs * (m + 1) * m);
+    nd.Hacc.alloc((size_t)nrhs * (m + 1));
+    nd.cs.alloc((size_t)nrhs * m);
+    nd.sn.alloc((size_t)nrhs * m);
+    nd.g.alloc((size_t)nrhs * (m + 1));
+    nd.Y.alloc((size_t)nrhs * m);
+    nd.part.alloc((size_t)nrhs * nd.nblk);
+    nd.part2.alloc((size_t)nrhs * (m + 1) * nd.nblk);
+    nd.tolabs.alloc(nrhs);
+    nd.resid.alloc(nrhs);
+    nd.state.alloc((size_t)nrhs * 4);
+    nd.G.alloc((size_t)nrhs * n);
+    if (nd.h_state) (void)hipHostFree(nd.h_state);
+    if (nd.h_resid) (void)hipHostFree(nd.h_resid);
+    FP_HIP(hipHostMalloc((void**)&nd.h_state, sizeof(int) * 4 * nrhs));
+    FP_HIP(hipHostMalloc((void**)&nd.h_resid, sizeof(double) * nrhs));
+    nd.ws_nrhs = nrhs;
+    nd.ws_m    = m;
+}
+
+static GmresDev gdev(fp_cat_s* c, Node& nd, int nrhs)
+{
+    GmresDev g{};
+    g.K = nd.K; g.nrhs = nrhs; g.m = c->m; g.nblk = nd.nblk;
+    g.V = nd.V.p; g.Wv = nd.Wv.p; g.X = nd.X.p; g.C = nd.C.p; g.H = nd.H.p; g.Hacc = nd.Hacc.p;
+    g.cs = nd.cs.p; g.sn = nd.sn.p; g.g = nd.g.p; g.Y = nd.Y.p; g.part = nd.part.p; g.part2 = nd.part2.p;
+    g.tolabs = nd.tolabs.p; g.resid = nd.resid.p; g.state = nd.state.p;
+    return g;
+}
+
+// out = v - R A^-1 E S v  (GMRESSolver2D::Afun on interface vectors)
+static void matvec(fp_cat_s* c, Node& nd, const double* v, double* out, int nrhs, hipStream_t s)
+{
+    const long long n = dom_n(c, nd.dom);
+    for (auto& L : nd.ch) launch_schur_gemv(L.S.p, L.cn, v, nd.K, L.off, nd.Tv.p, nd.K, L.off, nrhs, s);
+    launch_fill<double>(nd.G.p, n * nrhs, 0.0, s);
+    launch_scatter(nd.G.p, n, nd.ucell.p, nd.uk1.p, nd.uk2.p, nd.nu, nd.Tv.p, nd.K, nrhs, s);
+    sub_solve<double>(nd.plan, nd.G.p, n, nrhs, s);
+    launch_gather(nd.G.p, n, nd.cell.p, nd.K, v, out, nd.K, 1.0, -1.0, nrhs, s);
+}
+
+static void read_state(Node& nd, int nrhs, hipStream_t s)
+{
+    FP_HIP(hipMemcpyAsync(nd.h_state, nd.state.p, sizeof(int) * 4 * nrhs, hipMemcpyDeviceToHost, s));
+    FP_HIP(hipMemcpyAsync(nd.h_resid, nd.resid.p, sizeof(double) * nrhs, hipMemcpyDeviceToHost, s));
+    FP_HIP(hipStreamSynchronize(s));
+}
+
+// GMRESSolver2D::solve (gmres_solver2d.cpp:123-252) on interface unknowns:
+// b <- A^-1 b; y0 = R b; GMRES on (I - R A^-1 E S) y = R b; then
+// u = A^-1 b + A^-1 E S y.
+static void gmres_solve(fp_cat_s* c, Node& nd, double* b, int nrhs, hipStream_t s)
+{
+    ensure_ws(c, nd, nrhs);
+    const long long n = dom_n(c, nd.dom);
+    const int       K = nd.K, m = c->m;
+    sub_solve<double>(nd.plan, b, n, nrhs, s);
+    launch_gather(b, n, nd.cell.p, K, nullptr, nd.C.p, K, 0.0, 1.0, nrhs, s);
+    FP_HIP(hipMemcpyAsync(nd.X.p, nd.C.p, sizeof(double) * K * nrhs, hipMemcpyDeviceToDevice, s));
+    const GmresDev g = gdev(c, nd, nrhs);
+    gm_init(g, c->tol, c->tol_rel, s);
+    read_state(nd, nrhs, s);
+    std::vector<double> rhsnorm(nd.h_resid, nd.h_resid + nrhs);
+    int                 restarts = 0, status = 0;
+    for (int outer = 0; outer < c->maxiter; outer++) {
+        matvec(c, nd, nd.X.p, nd.Wv.p, nrhs, s);
+        gm_resid(g, s);
+        read_state(nd, nrhs, s);
+        bool all_done = true;
+        for (int r = 0; r < nrhs; r++)
+            if (nd.h_state[4 * r + 2] == 0) all_done = false;
+        if (all_done) break;
+        const long long ldv = (long long)nrhs * K;
+        for (int j = 0; j < m; j++) {
+            matvec(c, nd, nd.V.p + j * ldv, nd.Wv.p, nrhs, s);
+            gm_arnoldi(g, j, s);
+            if ((j + 1) % c->check_every == 0 || j == m - 1) {
+                read_state(nd, nrhs, s);
+                bool any = false;
+                for (int r = 0; r < nrhs; r++)
+                    if (nd.h_state[4 * r + 0]) any = true;
+                if (!any) break;
+            }
+        }
+        gm_finish_cycle(g, s);
+        restarts++;
+    }
+    read_state(nd, nrhs, s);
+    int    steps = 0;
+    double res = 0.0, rel = 0.0;
+    for (int r = 0; r < nrhs; r++) {
+        steps = std::max(steps, nd.h_state[4 * r + 3]);
+        res   = std::max(res, nd.h_resid[r]);
+        rel   = std::max(rel, rhsnorm[r] > 0 ? nd.h_resid[r] / rhsnorm[r] : nd.h_resid[r]);
+        if (nd.h_state[4 * r + 2] == 2) status = FP_E_NUMERIC;
+        else if (nd.h_state[4 * r + 2] != 1 && status == 0) status = FP_E_NOT_CONVERGED;
+    }
+    // u = z0 + A^-1 E S y
+    for (auto& L : nd.ch) launch_schur_gemv(L.S.p, L.cn, nd.X.p, K, L.off, nd.Tv.p, K, L.off, nrhs, s);
+    launch_fill<double>(nd.G.p, n * nrhs, 0.0, s);
+    launch_scatter(nd.G.p, n, nd.ucell.p, nd.uk1.p, nd.uk2.p, nd.nu, nd.Tv.p, K, nrhs, s);
+    sub_solve<double>(nd.plan, nd.G.p, n, nrhs, s);
+    launch_axpby(nd.G.p, b, n * nrhs, 1.0, 1.0, s);
+    if (nd.dom == c->root) {
+        c->last_steps    = steps;
+        c->last_restarts = restarts;
+        c->last_res      = res;
+        c->last_rel      = rel;
+        c->last_status   = status;
+    }
+}
+
+static void node_solve(fp_cat_s* c, int d, double* f, int nrhs, hipStream_t s)
+{
+    Node& nd = *c->nodes[d];
+    if (nd.ch.empty()) sub_solve<double>(nd.plan, f, dom_n(c, d), nrhs, s);
+    else gmres_solve(c, nd, f, nrhs, s);
+}
+
+// GMRESSolver2D::schur_mat_construct (gmres_solver2d.cpp:39-89): S of each
+// child from unit sources on the child's adjacent line, solved in batches.
+static void build_schur(fp_cat_s* c, Node& nd, ChildLink& L, hipStream_t s)
+{
+    const int       ch = L.dom, cn = L.cn;
+    const long long nc = dom_n(c, ch);
+    L.S.alloc((size_t)cn * cn);
+    const long long budget = 1LL << 28;  // doubles of scratch (2 GiB)
+    int             chunk  = (int)std::max<long long>(1, std::min<long long>(cn, budget / nc));
+    DevBuf<double>  buf((size_t)chunk * nc);
+    for (int i0 = 0; i0 < cn; i0 += chunk) {
+        const int cnt = std::min(chunk, cn - i0);
+        launch_fill<double>(buf.p, nc * cnt, 0.0, s);
+        launch_set_units(buf.p, nc, L.ccell.p, i0, cnt, s);
+        node_solve(c, ch, buf.p, cnt, s);
+        launch_schur_collect(buf.p, nc, L.ccell.p, cn, i0, cnt, L.S.p, s);
+    }
+    FP_HIP(hipStreamSynchronize(s));
+}
+
+static void build_node(fp_cat_s* c, int d, hipStream_t s)
+{
+    auto nd = std::make_unique<Node>();
+    nd->dom = d;
+    int bc[4];
+    for (int f = 0; f < 4; f++) bc[f] = c->bc[4 * d + f];
+    nd->plan.build(c->nx[d], c->ny[d], c->hx[d], c->hy[d], bc);
+    std::vector<int>                 cells;
+    std::map<int, std::vector<int>>  users;
+    for (int loc = 0; loc < 4; loc++) {
+        const int chd = c->child[4 * d + loc];
+        if (chd < 0) continue;
+        ChildLink L;
+        L.dom = chd;
+        L.loc = loc;
+        L.cn  = (loc == XNEG || loc == XPOS) ? c->ny[chd] : c->nx[chd];
+        L.off = (int)cells.size();
+        std::vector<int> rc = line_cells(loc, c->nx[d], c->ny[d]);
+        FP_REQUIRE((int)rc.size() == L.cn, "connected faces must have the same number of cells");
+        for (int t = 0; t < L.cn; t++) {
+            users[rc[t]].push_back((int)cells.size());
+            cells.push_back(rc[t]);
+        }
+        std::vector<int> cc = child_cells(loc, c->nx[chd], c->ny[chd]);
+        L.ccell.alloc(cc.size());
+        L.ccell.upload(cc.data(), cc.size());
+        nd->ch.push_back(std::move(L));
+    }
+    nd->K = (int)cells.size();
+    if (nd->K) {
+        nd->cell.alloc(cells.size());
+        nd->cell.upload(cells.data(), cells.size());
+        std::vector<int> uc, k1, k2;
+        for (auto& kv : users) {
+            uc.push_back(kv.first);
+            k1.push_back(kv.second[0]);
+            k2.push_back(kv.second.size() > 1 ? kv.second[1] : -1);
+        }
+        nd->nu = (int)uc.size();
+        nd->ucell.alloc(uc.size()); nd->ucell.upload(uc.data(), uc.size());
+        nd->uk1.alloc(k1.size()); nd->uk1.upload(k1.data(), k1.size());
+        nd->uk2.alloc(k2.size()); nd->uk2.upload(k2.data(), k2.size());
+    }
+    c->nodes[d] = std::move(nd);
+    for (auto& L : c->nodes[d]->ch) build_schur(c, *c->nodes[d], L, s);
+}
+
+// TreeUtils::findOptimalRoot / buildTreeMapFromRoot / BFS
+// (base/domain/geometry_tree.hpp:22-200).  A two-centre tie is broken by the
+// lower domain index (the reference compares Domain2DUniform addresses).
+static void build_tree(fp_cat_s* c)
+{
+    const int nd = c->nd;
+    c->root      = 0;
+    if (nd > 1) {
+        std::vector<int> deg(nd, 0), q;
+        for (int a = 0; a < nd; a++)
+            for (int l = 0; l < 4; l++)
+                if (c->adj[4 * a + l] >= 0) deg[a]++;
+        for (int a = 0; a < nd; a++)
+            if (deg[a] <= 1) q.push_back(a);
+        size_t qh = 0;
+        int    remaining = nd;
+        while (remaining > 2) {
+            const int cnt = (int)(q.size() - qh);
+            if (cnt == 0) break;
+            remaining -= cnt;
+            for (int i = 0; i < cnt; i++) {
+                const int leaf = q[qh++];
+                for (int l = 0; l < 4; l++) {
+                    const int nb = c->adj[4 * leaf + l];
+                    if (nb >= 0 && --deg[nb] == 1) q.push_back(nb);
+                }
+            }
+        }
+        if (qh < q.size()) {
+            c->root = q[qh];
+            if (qh + 1 < q.size() && q[qh + 1] < c->root) c->root = q[qh + 1];
+        }
+    }
+    c->child.assign(4 * nd, -1);
+    c->parent.assign(nd, -1);
+    c->parent_loc.assign(nd, -1);
+    c->level.assign(nd, -1);
+    std::vector<int> order{c->root};
+    c->level[c->root] = 0;
+    for (size_t h = 0; h < order.size(); h++) {
+        const int a = order[h];
+        for (int l = 0; l < 4; l++) {
+            const int b = c->adj[4 * a + l];
+            if (b < 0 || b == c->parent[a] || c->level[b] >= 0) continue;
+            c->child[4 * a + l] = b;
+            c->parent[b]        = a;
+            c->parent_loc[b]    = l ^ 1;
+            c->level[b]         = c->level[a] + 1;
+            order.push_back(b);
+        }
+    }
+    FP_REQUIRE((int)order.size() == nd, "geometry is not single connected");
+    c->nlev = 0;
+    for (int i = 0; i < nd; i++) c->nlev = std::max(c->nlev, c->level[i] + 1);
+    c->levels.assign(c->nlev, {});
+    for (int a : order) c->levels[c->level[a]].push_back(a);
+}
+
+// ConcatPoissonSolver2D::solve (pe/concat/concat_solver2d.cpp:101-157)
+static void cat_solve(fp_cat_s* c, double* const* f, const double* const* ghost, int nrhs, hipStream_t s)
+{
+    const int nd = c->nd;
+    if (c->temp_nrhs < nrhs) {
+        c->temp.clear();
+        c->temp.resize(nd);
+        for (int d = 0; d < nd; d++)
+            if (d != c->root) c->temp[d].alloc((size_t)dom_n(c, d) * nrhs);
+        c->temp_nrhs = nrhs;
+    }
+    for (int d = 0; d < nd; d++) {
+        AsmFaces<double> fc{};
+        const double     hx = c->hx[d], hy = c->hy[d];
+        for (int face = 0; face < 4; face++) {
+            const int t = c->bc[4 * d + face];
+            fc.g[face]  = nullptr;
+            fc.a[face]  = 0.0;
+            const double* gp = ghost ? ghost[4 * d + face] : nullptr;
+            if (!gp) continue;
+            if (t == DIRICHLET) fc.a[face] = face < 2 ? -1.0 / hx / hx : -1.0 / hy / hy;
+            else if (t == NEUMANN) {
+                const double h = face < 2 ? hx : hy;
+                fc.a[face]     = (face == XNEG || face == YNEG) ? 1.0 / h : -1.0 / h;
+            } else continue;
+            fc.g[face] = gp;
+        }
+        launch_assemble<double>(f[d], c->nx[d], c->ny[d], dom_n(c, d), nrhs, hx * hx, fc, s);
+    }
+    for (int l = c->nlev - 1; l >= 1; l--) {
+        for (int d : c->levels[l]) {
+            const long long n = dom_n(c, d);
+            FP_HIP(hipMemcpyAsync(c->temp[d].p, f[d], sizeof(double) * n * nrhs, hipMemcpyDeviceToDevice, s));
+            for (int loc = 0; loc < 4; loc++) {
+                const int ch = c->child[4 * d + loc];
+                if (ch < 0) continue;
+                launch_bond_add<double>(c->temp[d].p, c->nx[d], c->ny[d], n, c->temp[ch].p, c->nx[ch], c->ny[ch],
+                                        dom_n(c, ch), loc, -1.0, nrhs, s);
+                launch_bond_add<double>(f[d], c->nx[d], c->ny[d], n, c->temp[ch].p, c->nx[ch], c->ny[ch],
+                                        dom_n(c, ch), loc, -1.0, nrhs, s);
+            }
+        }
+        for (int d : c->levels[l]) node_solve(c, d, c->temp[d].p, nrhs, s);
+    }
+    const int r = c->root;
+    for (int loc = 0; loc < 4; loc++) {
+        const int ch = c->child[4 * r + loc];
+        if (ch < 0) continue;
+        launch_bond_add<double>(f[r], c->nx[r], c->ny[r], dom_n(c, r), c->temp[ch].p, c->nx[ch], c->ny[ch],
+                                dom_n(c, ch), loc, -1.0, nrhs, s);
+    }
+    c->last_steps = c->last_restarts = c->last_status = 0;
+    c->last_res = c->last_rel = 0.0;
+    node_solve(c, r, f[r], nrhs, s);
+    for (int l = 1; l < c->nlev; l++) {
+        for (int d : c->levels[l]) {
+            const int p = c->parent[d];
+            launch_bond_add<double>(f[d], c->nx[d], c->ny[d], dom_n(c, d), f[p], c->nx[p], c->ny[p], dom_n(c, p),
+                                    c->parent_loc[d], -1.0, nrhs, s);
+        }
+        for (int d : c->levels[l]) node_solve(c, d, f[d], nrhs, s);
+    }
+}
+
+}  // namespace fp
+
+// ---------------------------------------------------------------------------
+// C ABI
+
+namespace {
+thread_local std::string g_last_error;
+
+template <typename F>
+int guarded(F&& fn)
+{
+    try {
+        fn();
+        return FP_OK;
+    } catch (const fp::Error& e) {
+        g_last_error = e.what();
+        return e.status;
+    } catch (const std::bad_alloc& e) {
+        g_last_error = "host allocation failed";
+        return FP_E_NOMEM;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return FP_E_INVALID;
+    }
+}
+}  // namespace
+
+extern "C" {
+
+int fp_version(void) { return 1; }
+
+const char* fp_status_str(int st)
+{
+    switch (st) {
+        case FP_OK: return "ok";
+        case FP_E_INVALID: return "invalid argument";
+        case FP_E_NOMEM: return "out of memory";
+        case FP_E_HIP: return "HIP runtime error";
+        case FP_E_NOT_CONVERGED: return "GMRES did not reach the tolerance";
+        case FP_E_NUMERIC: return "non-finite value in the iteration";
+        case FP_E_UNSUPPORTED: return "unsupported configuration";
+        default: return "unknown status";
+    }
+}
+
+const char* fp_last_error(void) { return g_last_error.c_str(); }
+
+int fp_ctx_create(int device, fp_ctx* out)
+{
+    return guarded([&] {
+        FP_REQUIRE(out != nullptr, "null output handle");
+        int n = 0;
+        FP_HIP(hipGetDeviceCount(&n));
+        FP_REQUIRE(device >= 0 && device < n, "no such HIP device");
+        FP_HIP(hipSetDevice(device));
+        auto* c   = new fp_ctx_s();
+        c->device = device;
+        *out      = c;
+    });
+}
+
+void fp_ctx_destroy(fp_ctx ctx) { delete ctx; }
+
+int fp_sub_create(fp_ctx ctx, int nx, int ny, double hx, double hy, const unsigned char bc[4], fp_sub* out)
+{
+    return guarded([&] {
+        FP_REQUIRE(ctx && out && bc, "null argument");
+        FP_HIP(hipSetDevice(ctx->device));
+        auto s = std::make_unique<fp_sub_s>();
+        s->ctx = ctx;
+        int b[4] = {bc[0], bc[1], bc[2], bc[3]};
+        s->plan.build(nx, ny, hx, hy, b);
+        *out = s.release();
+    });
+}
+
+void fp_sub_destroy(fp_sub sub) { delete sub; }
+
+int fp_sub_solve(fp_sub sub, double* d_f, int nrhs, long long ld, void* stream)
+{
+    return guarded([&] {
+        FP_REQUIRE(sub && d_f && nrhs >= 0, "null argument");
+        FP_REQUIRE(ld >= (long long)sub->plan.nx * sub->plan.ny || nrhs <= 1, "ld smaller than nx*ny");
+        FP_HIP(hipSetDevice(sub->ctx->device));
+        fp::sub_solve<double>(sub->plan, d_f, ld, nrhs, (hipStream_t)stream);
+    });
+}
+
+int fp_sub_solve_f32(fp_sub sub, float* d_f, int nrhs, long long ld, void* stream)
+{
+    return guarded([&] {
+        FP_REQUIRE(sub && d_f && nrhs >= 0, "null argument");
+        FP_REQUIRE(ld >= (long long)sub->plan.nx * sub->plan.ny || nrhs <= 1, "ld smaller than nx*ny");
+        FP_HIP(hipSetDevice(sub->ctx->device));
+        fp::sub_solve<float>(sub->plan, d_f, ld, nrhs, (hipStream_t)stream);
+    });
+}
+
+int fp_sub_solve_batched(fp_ctx ctx, int nprob, const fp_sub* subs, double* const* d_f, const int* nrhs, void* stream)
+{
+    return guarded([&] {
+        FP_REQUIRE(ctx && (nprob == 0 || (subs && d_f && nrhs)), "null argument");
+        FP_HIP(hipSetDevice(ctx->device));
+        for (int k = 0; k < nprob; k++) {
+            FP_REQUIRE(subs[k] && d_f[k], "null problem");
+            fp::sub_solve<double>(subs[k]->plan, d_f[k], (long long)subs[k]->plan.nx * subs[k]->plan.ny, nrhs[k],
+                                  (hipStream_t)stream);
+        }
+    });
+}
+
+int fp_sub_solve_batched_f32(fp_ctx ctx, int nprob, const fp_sub* subs, float* const* d_f, const int* nrhs,
+                             void* stream)
+{
+    return guarded([&] {
+        FP_REQUIRE(ctx && (nprob == 0 || (subs && d_f && nrhs)), "null argument");
+        FP_HIP(hipSetDevice(ctx->device));
+        for (int k = 0; k < nprob; k++) {
+            FP_REQUIRE(subs[k] && d_f[k], "null problem");
+            fp::sub_solve<float>(subs[k]->plan, d_f[k], (long long)subs[k]->plan.nx * subs[k]->plan.ny, nrhs[k],
+                                 (hipStream_t)stream);
+        }
+    });
+}
+
+int fp_cat_create(fp_ctx ctx, const fp_geometry_desc* g, int m, double tol, int tol_is_relative, int max_iter,
+                  int precision, fp_cat* out)
+{
+    return guarded([&] {
+        FP_REQUIRE(ctx && g && out, "null argument");
+        FP_REQUIRE(g->nd >= 1 && g->nx && g->ny && g->hx && g->hy && g->bc, "empty geometry");
+        FP_REQUIRE(m >= 1 && m <= 1000, "m must be in [1, 1000]");
+        FP_REQUIRE(max_iter >= 1 && tol >= 0.0, "invalid GMRES parameters");
+        FP_REQUIRE(precision == FP_F64, "composite solve supports FP_F64");
+        FP_HIP(hipSetDevice(ctx->device));
+        auto c       = std::make_unique<fp_cat_s>();
+        c->ctx       = ctx;
+        c->nd        = g->nd;
+        c->m         = m;
+        c->tol       = tol;
+        c->tol_rel   = tol_is_relative ? 1 : 0;
+        c->maxiter   = max_iter;
+        c->precision = precision;
+        const int nd = g->nd;
+        c->nx.assign(g->nx, g->nx + nd);
+        c->ny.assign(g->ny, g->ny + nd);
+        c->hx.assign(g->hx, g->hx + nd);
+        c->hy.assign(g->hy, g->hy + nd);
+        c->bc.assign(g->bc, g->bc + 4 * nd);
+        c->adj.assign(4 * nd, -1);
+        for (int k = 0; k < g->nconn; k++) {
+            const int a = g->conn[3 * k], dir = g->conn[3 * k + 1], b = g->conn[3 * k + 2];
+            FP_REQUIRE(a >= 0 && a < nd && b >= 0 && b < nd && a != b && dir >= 0 && dir < 4, "bad connection");
+            c->adj[4 * a + dir]       = b;
+            c->adj[4 * b + (dir ^ 1)] = a;
+            c->bc[4 * a + dir]        = fp::ADJACENTED;
+            c->bc[4 * b + (dir ^ 1)]  = fp::ADJACENTED;
+        }
+        fp::build_tree(c.get());
+        c->nodes.resize(nd);
+        // construct_solver_map: deepest level first, root last (concat_solver2d.cpp:88-99)
+        hipStream_t s = nullptr;
+        for (int l = c->nlev - 1; l >= 0; l--)
+            for (int d : c->levels[l]) fp::build_node(c.get(), d, s);
+        FP_HIP(hipDeviceSynchronize());
+        *out = c.release();
+    });
+}
+
+void fp_cat_destroy(fp_cat cat) { delete cat; }
+
+int fp_cat_root(fp_cat cat) { return cat ? cat->root : -1; }
+
+int fp_cat_solve(fp_cat cat, double* const* d_fields, const double* const* d_ghost, int nrhs, void* stream,
+                 fp_solve_info* info)
+{
+    int st = guarded([&] {
+        FP_REQUIRE(cat && d_fields && nrhs >= 1, "null argument");
+        for (int d = 0; d < cat->nd; d++) FP_REQUIRE(d_fields[d], "null field");
+        FP_HIP(hipSetDevice(cat->ctx->device));
+        fp::cat_solve(cat, d_fields, d_ghost, nrhs, (hipStream_t)stream);
+        if (info) {
+            info->iterations      = cat->last_steps;
+            info->restarts        = cat->last_restarts;
+            info->residual        = cat->last_res;
+            info->rel_residual    = cat->last_rel;
+            info->refine_steps    = 0;
+            info->refine_residual = 0.0;
+        }
+    });
+    if (st == FP_OK && cat && cat->last_status) st = cat->last_status;
+    return st;
+}
+
+int fp_cat_vjp(fp_cat cat, const double* const* d_gu, double* const* d_gf, double* const* d_gghost, int nrhs,
+               void* stream, fp_solve_info* info)
+{
+    (void)cat; (void)d_gu; (void)d_gf; (void)d_gghost; (void)nrhs; (void)stream; (void)info;
+    g_last_error = "fp_cat_vjp is not built yet";
+    return FP_E_UNSUPPORTED;
+}
+
+}  // extern "C"

@@ -1,0 +1,376 @@
+// Batched y transforms of the subdomain solver (DST-I, DCT-II/III and the
+// two quarter-wave pairs), as packed complex FFTs staged in LDS.
+//
+// Reference semantics: PoissonFFT2D_{DD,NN,DN,ND}::transform and
+// ::transform_transpose (pe/poisson/poisson_fft2d.cpp:37-57, 105-133,
+// 180-205, 242-262).  The reference runs one FFTW r2c per row (two for the
+// NN/ND inverse); here two rows a, b of the field form one complex sequence
+// z = ext(a) + i ext(b) of length L, one FFT of length L is done in LDS, and
+// the two rows' coefficients are split out of the spectrum:
+//   DD     : odd extension, L = 2N+2, spectrum purely imaginary per row
+//   NN fwd : half-sample even extension, L = 2N, phase exp(-i pi m / 2N)
+//   NN inv : Hermitian extension of the phased coefficients, inverse FFT, L = 2N
+//   DN, ND : L = 2N+1, split by Z(m) +- conj Z(L-m)
+// Each output equals the reference's coefficient (same signs and mode order),
+// so transform results are directly comparable with the reference's.
+//
+// The FFT is a mixed-radix Stockham autosort: radix 4/2/3/5 butterflies in
+// registers, any other prime by a two-phase in-LDS DFT.  One workgroup owns
+// one row pair of one right-hand side: it loads both rows (coalesced), runs
+// all passes between two LDS buffers and writes the coefficients back in place.
+#include "fp_internal.h"
+
+#include <cmath>
+
+namespace fp {
+
+template <typename T>
+struct alignas(2 * sizeof(T)) C2 {
+    T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ C2<T> cadd(C2<T> a, C2<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T>
+__device__ __forceinline__ C2<T> csub(C2<T> a, C2<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename T>
+__device__ __forceinline__ C2<T> cmul(C2<T> a, C2<T> b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+// a * conj(b)
+template <typename T>
+__device__ __forceinline__ C2<T> cmulc(C2<T> a, C2<T> b) { return {a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y}; }
+template <typename T>
+__device__ __forceinline__ C2<T> cscale(C2<T> a, T s) { return {a.x * s, a.y * s}; }
+// multiply by -i (FWD) or +i (!FWD)
+template <bool FWD, typename T>
+__device__ __forceinline__ C2<T> mul_mi(C2<T> a) { return FWD ? C2<T>{a.y, -a.x} : C2<T>{-a.y, a.x}; }
+
+template <typename T>
+struct YArgs {
+    T*           data;
+    long long    ld;
+    int          nx, N, L, kind, inverse, nrad;
+    int          rad[kMaxRadices];
+    const C2<T>* W;
+    const C2<T>* aux;
+    T            c0, c1;
+};
+
+template <bool FWD, typename T>
+__device__ __forceinline__ C2<T> twiddle(const C2<T>* W, int t, C2<T> v)
+{
+    const C2<T> w = W[t];
+    return FWD ? cmul(v, w) : cmulc(v, w);
+}
+
+// One Stockham pass with a register butterfly of radix R.
+template <int R, bool FWD, int NT, typename T>
+__device__ void pass_reg(const C2<T>* __restrict__ src, C2<T>* __restrict__ dst, const C2<T>* W, int L, int Ns)
+{
+    const int nb = L / R, stride = L / (Ns * R);
+    for (int jb = threadIdx.x; jb < nb; jb += NT) {
+        const int k = jb % Ns;
+        C2<T>     v[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            v[r] = src[jb + r * nb];
+            if (r > 0) v[r] = twiddle<FWD>(W, r * k * stride, v[r]);
+        }
+        C2<T> o[R];
+        if constexpr (R == 2) {
+            o[0] = cadd(v[0], v[1]);
+            o[1] = csub(v[0], v[1]);
+        } else if constexpr (R == 4) {
+            const C2<T> t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+            const C2<T> t2 = cadd(v[1], v[3]), t3 = mul_mi<FWD>(csub(v[1], v[3]));
+            o[0] = cadd(t0, t2);
+            o[2] = csub(t0, t2);
+            o[1] = cadd(t1, t3);
+            o[3] = csub(t1, t3);
+        } else if constexpr (R == 3) {
+            const T     h = T(0.86602540378443864676372317075294);
+            const C2<T> s = cadd(v[1], v[2]), d = csub(v[1], v[2]);
+            const C2<T> m = {v[0].x - T(0.5) * s.x, v[0].y - T(0.5) * s.y};
+            const C2<T> e = mul_mi<FWD>(cscale(d, h));
+            o[0] = cadd(v[0], s);
+            o[1] = cadd(m, e);
+            o[2] = csub(m, e);
+        } else if constexpr (R == 5) {
+            const T     c1 = T(0.30901699437494742410229341718282), c2 = T(-0.80901699437494742410229341718282);
+            const T     s1 = T(0.95105651629515357211643933337938), s2 = T(0.58778525229247312916870595463907);
+            const C2<T> a1 = cadd(v[1], v[4]), b1 = csub(v[1], v[4]);
+            const C2<T> a2 = cadd(v[2], v[3]), b2 = csub(v[2], v[3]);
+            const C2<T> m1 = {v[0].x + c1 * a1.x + c2 * a2.x, v[0].y + c1 * a1.y + c2 * a2.y};
+            const C2<T> m2 = {v[0].x + c2 * a1.x + c1 * a2.x, v[0].y + c2 * a1.y + c1 * a2.y};
+            const C2<T> n1 = mul_mi<FWD>(C2<T>{s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y});
+            const C2<T> n2 = mul_mi<FWD>(C2<T>{s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y});
+            o[0] = {v[0].x + a1.x + a2.x, v[0].y + a1.y + a2.y};
+            o[1] = cadd(m1, n1);
+            o[4] = csub(m1, n1);
+            o[2] = cadd(m2, n2);
+            o[3] = csub(m2, n2);
+        }
+        const int idxD = (jb / Ns) * Ns * R + k;
+#pragma unroll
+        for (int r = 0; r < R; r++) dst[idxD + r * Ns] = o[r];
+    }
+}
+
+// Stockham pass of any radix R: twiddle in place, then each output is an
+// R-term sum read from LDS.
+template <bool FWD, int NT, typename T>
+__device__ void pass_generic(C2<T>* __restrict__ src, C2<T>* __restrict__ dst, const C2<T>* W, int L, int Ns,
+                             int R)
+{
+    const int nb = L / R, stride = L / (Ns * R);
+    for (int idx = threadIdx.x; idx < L; idx += NT) {
+        const int r = idx / nb, jb = idx - r * nb, k = jb % Ns;
+        if (r > 0) src[idx] = twiddle<FWD>(W, r * k * stride, src[idx]);
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < L; idx += NT) {
+        const int q = idx / nb, jb = idx - q * nb, k = jb % Ns;
+        C2<T>     acc = src[jb];
+        int       t   = 0;
+        for (int r = 1; r < R; r++) {
+            t += q * nb;
+            if (t >= L) t -= L;
+            acc = cadd(acc, twiddle<FWD>(W, t, src[jb + r * nb]));
+        }
+        dst[(jb / Ns) * Ns * R + k + q * Ns] = acc;
+    }
+}
+
+template <bool FWD, int NT, typename T>
+__device__ C2<T>* run_fft(const YArgs<T>& a, C2<T>* A, C2<T>* B)
+{
+    C2<T>* src = A;
+    C2<T>* dst = B;
+    int    Ns  = 1;
+    for (int s = 0; s < a.nrad; s++) {
+        const int R = a.rad[s];
+        switch (R) {
+            case 2: pass_reg<2, FWD, NT>(src, dst, a.W, a.L, Ns); break;
+            case 3: pass_reg<3, FWD, NT>(src, dst, a.W, a.L, Ns); break;
+            case 4: pass_reg<4, FWD, NT>(src, dst, a.W, a.L, Ns); break;
+            case 5: pass_reg<5, FWD, NT>(src, dst, a.W, a.L, Ns); break;
+            default: pass_generic<FWD, NT>(src, dst, a.W, a.L, Ns, R); break;
+        }
+        __syncthreads();
+        C2<T>* t = src;
+        src      = dst;
+        dst      = t;
+        Ns *= R;
+    }
+    return src;
+}
+
+template <typename T>
+__device__ __forceinline__ T sgn_pow(int p) { return (p & 1) ? T(-1) : T(1); }
+
+template <int NT, typename T>
+__global__ __launch_bounds__(NT) void k_ytransform(YArgs<T> a)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C2<T>* A  = reinterpret_cast<C2<T>*>(smem_raw);
+    C2<T>* B  = A + a.L;
+    T*     y0 = reinterpret_cast<T*>(B + a.L);  // NN inverse: y_0 of rows a, b
+
+    const int  N = a.N, L = a.L;
+    const int  ra = 2 * blockIdx.x, rb = ra + 1;
+    const bool hb = rb < a.nx;
+    T*         pa = a.data + (long long)blockIdx.y * a.ld + (long long)ra * N;
+    T*         pb = pa + N;
+
+    // ---- load: build the packed extended sequence in A ----
+    if (a.kind == T_DD) {
+        for (int p = threadIdx.x; p < L; p += NT) {
+            C2<T> z = {T(0), T(0)};
+            if (p >= 1 && p <= N) z = {pa[p - 1], hb ? pb[p - 1] : T(0)};
+            else if (p >= N + 2) z = {-pa[L - 1 - p], hb ? -pb[L - 1 - p] : T(0)};
+            A[p] = z;
+        }
+    } else if (a.kind == T_NN && !a.inverse) {
+        for (int p = threadIdx.x; p < L; p += NT) {
+            const int n = p < N ? p : 2 * N - 1 - p;
+            A[p]        = {pa[n], hb ? pb[n] : T(0)};
+        }
+    } else if (a.kind == T_NN) {
+        for (int p = threadIdx.x; p < L; p += NT) {
+            C2<T> z = {T(0), T(0)};
+            if (p != N) {
+                const int   m  = p < N ? p : 2 * N - p;
+                const T     s  = m == 0 ? a.c1 : a.c0;  // sqrt(1/N) or -sqrt(2/N)
+                const T     ya = s * pa[N - 1 - m], yb = hb ? s * pb[N - 1 - m] : T(0);
+                const C2<T> e  = a.aux[m];              // exp(-i pi m / 2N); Y_m uses conj(e)
+                if (p < N) z = {ya * e.x + yb * e.y, -ya * e.y + yb * e.x};
+                else z = {ya * e.x - yb * e.y, ya * e.y + yb * e.x};
+                if (p == 0) {
+                    y0[0] = ya;
+                    y0[1] = yb;
+                }
+            }
+            A[p] = z;
+        }
+    } else if (a.kind == T_ND && a.inverse) {
+        for (int p = threadIdx.x; p < L; p += NT) {
+            C2<T> z = {T(0), T(0)};
+            if (p >= 1 && p <= N) z = {pa[p - 1], hb ? pb[p - 1] : T(0)};
+            A[p] = z;
+        }
+    } else {  // DN both ways, ND forward
+        for (int p = threadIdx.x; p < L; p += NT) {
+            C2<T> z = {T(0), T(0)};
+            if (p >= 1 && p <= N) {
+                const T s = sgn_pow<T>(p) * a.c0;
+                z         = {s * pa[p - 1], hb ? s * pb[p - 1] : T(0)};
+            }
+            A[p] = z;
+        }
+    }
+    __syncthreads();
+
+    const bool   fwd = !(a.kind == T_NN && a.inverse);
+    const C2<T>* Z   = fwd ? run_fft<true, NT>(a, A, B) : run_fft<false, NT>(a, A, B);
+
+    // ---- store: split the two rows' coefficients ----
+    for (int j = threadIdx.x; j < N; j += NT) {
+        T oa, ob;
+        if (a.kind == T_DD) {
+            const C2<T> z = Z[j + 1];
+            oa            = -a.c0 * z.y;
+            ob            = a.c0 * z.x;
+        } else if (a.kind == T_NN && !a.inverse) {
+            const int   m = N - 1 - j;
+            const C2<T> w = cmul(Z[m], a.aux[m]);
+            const T     s = (j == N - 1) ? a.c1 : a.c0;
+            oa            = s * w.x;
+            ob            = s * w.y;
+        } else if (a.kind == T_NN) {
+            const C2<T> z = Z[j];
+            oa            = T(0.5) * (z.x + y0[0]);
+            ob            = T(0.5) * (z.y + y0[1]);
+        } else if (a.kind == T_DN) {
+            const C2<T> zm = Z[j + 1], zc = Z[L - 1 - j];
+            const T     s  = sgn_pow<T>(j) * T(0.5);
+            oa             = s * (zm.y - zc.y);
+            ob             = s * (zc.x - zm.x);
+        } else if (!a.inverse) {  // ND forward
+            const C2<T> zm = Z[j + 1], zc = Z[L - 1 - j], sc = a.aux[j];
+            const T     rea = T(0.5) * (zm.x + zc.x), ima = T(0.5) * (zm.y - zc.y);
+            const T     reb = T(0.5) * (zm.y + zc.y), imb = T(0.5) * (zc.x - zm.x);
+            oa              = sc.x * rea - sc.y * ima;
+            ob              = sc.x * reb - sc.y * imb;
+        } else {  // ND inverse
+            const int   k  = N - j;
+            const C2<T> zk = Z[k], zc = Z[L - k];
+            const T     s  = sgn_pow<T>(j + 1) * a.c0 * T(0.5);
+            oa             = s * (zk.y - zc.y);
+            ob             = s * (zc.x - zk.x);
+        }
+        pa[j] = oa;
+        if (hb) pb[j] = ob;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+
+static void factor_radices(int L, int* rad, int& nrad)
+{
+    nrad  = 0;
+    int n = L;
+    while (n % 4 == 0) { rad[nrad++] = 4; n /= 4; }
+    while (n % 2 == 0) { rad[nrad++] = 2; n /= 2; }
+    while (n % 3 == 0) { rad[nrad++] = 3; n /= 3; }
+    while (n % 5 == 0) { rad[nrad++] = 5; n /= 5; }
+    for (int p = 7; n > 1; p += 2)
+        while (n % p == 0) {
+            if (nrad >= kMaxRadices) throw Error(FP_E_INVALID, "transform length has too many factors");
+            rad[nrad++] = p;
+            n /= p;
+        }
+}
+
+template <typename T>
+void TransformPlan<T>::build(int k, int n)
+{
+    kind = k;
+    N    = n;
+    if (kind == T_DD) L = 2 * N + 2;
+    else if (kind == T_NN) L = 2 * N;
+    else L = 2 * N + 1;
+    factor_radices(L, rad, nrad);
+    const long double PI = 3.141592653589793238462643383279502884L;
+    std::vector<T>    w(2 * (size_t)L), x(2 * (size_t)N, T(0));
+    for (int q = 0; q < L; q++) {
+        const long double ang = -2.0L * PI * (long double)q / (long double)L;
+        w[2 * q]              = (T)cosl(ang);
+        w[2 * q + 1]          = (T)sinl(ang);
+    }
+    if (kind == T_NN) {
+        for (int m = 0; m < N; m++) {
+            const long double ang = -PI * (long double)m / (2.0L * N);
+            x[2 * m]              = (T)cosl(ang);
+            x[2 * m + 1]          = (T)sinl(ang);
+        }
+        c0 = (T)(-std::sqrt(2.0 / N) * 0.5);
+        c1 = (T)(std::sqrt(1.0 / N) * 0.5);
+    } else if (kind == T_ND) {
+        for (int j = 0; j < N; j++) {
+            // theta_j = 2 N pi (j+1) / (2N+1), reduced modulo 2 pi on integers
+            const long long   q   = (2LL * N * (j + 1)) % (2LL * (2 * N + 1));
+            const long double ang = PI * (long double)q / (long double)(2 * N + 1);
+            x[2 * j]              = (T)sinl(ang);
+            x[2 * j + 1]          = (T)cosl(ang);
+        }
+        c0 = (T)std::sqrt(4.0 / (2.0 * N + 1.0));
+    } else if (kind == T_DN) {
+        c0 = (T)std::sqrt(4.0 / (2.0 * N + 1.0));
+    } else {
+        c0 = (T)(std::sqrt(2.0 / (N + 1.0)) * 0.5);
+    }
+    W.alloc(2 * (size_t)L);
+    W.upload(w.data(), w.size());
+    aux.alloc(2 * (size_t)N);
+    aux.upload(x.data(), x.size());
+}
+
+template struct TransformPlan<double>;
+template struct TransformPlan<float>;
+
+template <typename T>
+void launch_ytransform(const TransformPlan<T>& tp, T* data, long long ld, int nx, int nrhs, int inverse,
+                       hipStream_t s)
+{
+    constexpr int NT = 256;
+    YArgs<T>      a{};
+    a.data    = data;
+    a.ld      = ld;
+    a.nx      = nx;
+    a.N       = tp.N;
+    a.L       = tp.L;
+    a.kind    = tp.kind;
+    a.inverse = inverse;
+    a.nrad    = tp.nrad;
+    for (int i = 0; i < tp.nrad; i++) a.rad[i] = tp.rad[i];
+    a.W   = reinterpret_cast<const C2<T>*>(tp.W.p);
+    a.aux = reinterpret_cast<const C2<T>*>(tp.aux.p);
+    a.c0  = tp.c0;
+    a.c1  = tp.c1;
+    const size_t lds = 2 * (size_t)tp.L * sizeof(C2<T>) + 4 * sizeof(T);
+    if (lds > 160 * 1024) throw Error(FP_E_INVALID, "transform length exceeds the LDS of one workgroup");
+    static bool attr_set = false;
+    if (lds > 64 * 1024 && !attr_set) {
+        FP_HIP(hipFuncSetAttribute((const void*)k_ytransform<NT, T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   160 * 1024));
+        attr_set = true;
+    }
+    dim3 grid((nx + 1) / 2, nrhs);
+    hipLaunchKernelGGL((k_ytransform<NT, T>), grid, dim3(NT), lds, s, a);
+    FP_HIP(hipGetLastError());
+}
+
+template void launch_ytransform<double>(const TransformPlan<double>&, double*, long long, int, int, int, hipStream_t);
+template void launch_ytransform<float>(const TransformPlan<float>&, float*, long long, int, int, int, hipStream_t);
+
+}  // namespace fp

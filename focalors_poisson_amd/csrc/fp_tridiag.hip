@@ -1,0 +1,182 @@
+// Tridiagonal solves along x, one system per y mode, for the subdomain solver.
+//
+// Reference semantics: ChasingMethod2D::chasing over the transposed buffer
+// with chasing_standard_without_precompute per mode and the singular
+// recurrence for the last mode of a domain with no Dirichlet-like face
+// (pe/poisson/chasing_method2d.cpp:67-88, chasing_method_base.cpp:24-53,146-154).
+//
+// Layout: the transformed field stays row-major [i][j]; lane = mode j, so each
+// row access is coalesced and the reference's two transposes
+// (poisson_solver2d.cpp:74,78) disappear.
+//
+// Parallel Thomas: the LU factor of tridiag(1, d_j, 1) with the Neumann end
+// pivots is precomputed per (row, mode) as the reciprocal pivot p_i (the
+// reference's c_i).  The forward sweep y_i = p_i (f_i - y_{i-1}) and the back
+// substitution x_i = y_i - p_i x_{i+1} are first-order affine recurrences, so
+// each thread owns a chunk of CH rows of one mode, composes its chunk's affine
+// map, the chunk maps are chained in LDS, and every thread then replays its
+// chunk from the correct start value.  The whole column stays in registers:
+// one read of f and p, one write of x.  The singular zero mode (x_0 = 0,
+// x_{i+1} = f_i - x_{i-1} + 2 x_i) is two prefix sums and uses the same chain.
+#include "fp_internal.h"
+
+namespace fp {
+
+template <typename T, int CH>
+__global__ __launch_bounds__(1024) void k_tridiag(T* __restrict__ data, long long ld, int nx, int ny,
+                                                  const T* __restrict__ pinv, int P, int MB, int singular_last)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* SA = reinterpret_cast<T*>(smem_raw);
+    T* SB = SA + P * MB;
+    T* SY = SB + P * MB;
+
+    const int  tid = threadIdx.x;
+    const int  jl  = tid % MB;
+    const int  c   = tid / MB;
+    const int  j   = blockIdx.x * MB + jl;
+    const bool act = (c < P) && (j < ny);
+    T*         f   = data + (long long)blockIdx.y * ld;
+    const int  i0  = c * CH;
+
+    T fv[CH], pv[CH];
+#pragma unroll
+    for (int k = 0; k < CH; k++) {
+        const int  i  = i0 + k;
+        const bool ok = act && i < nx;
+        fv[k]         = ok ? f[(long long)i * ny + j] : T(0);
+        pv[k]         = ok ? pinv[(long long)i * ny + j] : T(0);
+    }
+    const bool sing = singular_last && (j == ny - 1);
+
+    // forward: chunk map y_out = A y_in + B
+    T A = T(1), B = T(0);
+    if (!sing) {
+#pragma unroll
+        for (int k = 0; k < CH; k++)
+            if (i0 + k < nx) {
+                A = -pv[k] * A;
+                B = pv[k] * (fv[k] - B);
+            }
+    } else {
+#pragma unroll
+        for (int k = 0; k < CH; k++)
+            if (i0 + k < nx) B += fv[k];
+    }
+    if (c < P) {
+        SA[c * MB + jl] = A;
+        SB[c * MB + jl] = B;
+    }
+    __syncthreads();
+    if (c == 0 && j < ny) {
+        T y = T(0);
+        for (int q = 0; q < P; q++) {
+            SY[q * MB + jl] = y;
+            y               = SA[q * MB + jl] * y + SB[q * MB + jl];
+        }
+    }
+    __syncthreads();
+    T y = (c < P) ? SY[c * MB + jl] : T(0);
+    if (!sing) {
+#pragma unroll
+        for (int k = 0; k < CH; k++)
+            if (i0 + k < nx) {
+                y     = pv[k] * (fv[k] - y);
+                fv[k] = y;
+            }
+    } else {
+#pragma unroll
+        for (int k = 0; k < CH; k++)
+            if (i0 + k < nx) {
+                y     = y + fv[k];  // d_i = d_{i-1} + f_i
+                fv[k] = y;
+            }
+    }
+    __syncthreads();
+
+    // backward: chunk map x_{i0} = A x_{i0+CH} + B  (x_nx = 0)
+    A = T(1);
+    B = T(0);
+    if (!sing) {
+#pragma unroll
+        for (int k = CH - 1; k >= 0; k--)
+            if (i0 + k < nx) {
+                B = fv[k] - pv[k] * B;
+                A = -pv[k] * A;
+            }
+    } else {
+#pragma unroll
+        for (int k = 0; k < CH; k++)
+            if (i0 + k < nx) B += fv[k];
+    }
+    if (c < P) {
+        SA[c * MB + jl] = A;
+        SB[c * MB + jl] = B;
+    }
+    __syncthreads();
+    if (c == 0 && j < ny) {
+        T x = T(0);
+        if (!sing) {
+            for (int q = P - 1; q >= 0; q--) {
+                SY[q * MB + jl] = x;
+                x               = SA[q * MB + jl] * x + SB[q * MB + jl];
+            }
+        } else {
+            for (int q = 0; q < P; q++) {  // x at chunk start = sum of earlier d
+                SY[q * MB + jl] = x;
+                x += SB[q * MB + jl];
+            }
+        }
+    }
+    __syncthreads();
+    T x = (c < P) ? SY[c * MB + jl] : T(0);
+    if (!sing) {
+#pragma unroll
+        for (int k = CH - 1; k >= 0; k--)
+            if (i0 + k < nx) {
+                x     = fv[k] - pv[k] * x;
+                fv[k] = x;
+            }
+    } else {
+#pragma unroll
+        for (int k = 0; k < CH; k++)
+            if (i0 + k < nx) {
+                const T d = fv[k];
+                fv[k]     = x;  // x_i = sum_{q<i} d_q
+                x += d;
+            }
+    }
+#pragma unroll
+    for (int k = 0; k < CH; k++) {
+        const int i = i0 + k;
+        if (act && i < nx) f[(long long)i * ny + j] = fv[k];
+    }
+}
+
+template <typename T, int CH>
+static void launch_tridiag_ch(const SubPlan& sp, const T* pinv, T* data, long long ld, int nrhs, hipStream_t s)
+{
+    const int P = (sp.nx + CH - 1) / CH;
+    int       MB = 1;
+    while (MB * 2 * P <= 1024 && MB < 64) MB *= 2;
+    if (MB * P > 1024) throw Error(FP_E_INVALID, "nx too large for the tridiagonal kernel");
+    const size_t lds = 3 * (size_t)P * MB * sizeof(T);
+    dim3         grid((sp.ny + MB - 1) / MB, nrhs);
+    hipLaunchKernelGGL((k_tridiag<T, CH>), grid, dim3(MB * P), lds, s, data, ld, sp.nx, sp.ny, pinv, P, MB,
+                       sp.no_dirichlet ? 1 : 0);
+    FP_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_tridiag(const SubPlan& sp, const T* pinv, T* data, long long ld, int nrhs, hipStream_t s)
+{
+    // chunk length: 16 rows per thread up to nx = 1024, 32 above (register budget)
+    if (sp.nx <= 1024) launch_tridiag_ch<T, 16>(sp, pinv, data, ld, nrhs, s);
+    else if (sp.nx <= 32768) launch_tridiag_ch<T, 32>(sp, pinv, data, ld, nrhs, s);
+    else throw Error(FP_E_INVALID, "nx above 32768 is not supported");
+}
+
+template void launch_tridiag<double>(const SubPlan&, const double*, double*, long long, int, hipStream_t);
+template void launch_tridiag<float>(const SubPlan&, const float*, float*, long long, int, hipStream_t);
+
+}  // namespace fp

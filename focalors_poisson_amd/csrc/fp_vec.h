@@ -1,0 +1,59 @@
+// Declarations of the composite-solve vector kernels (fp_vec.hip).
+#pragma once
+
+#include "fp_internal.h"
+
+#include <algorithm>
+
+namespace fp {
+
+template <typename T>
+struct AsmFaces {
+    const T* g[4];  // ghost data per face (nrhs arrays back to back) or null
+    T        a[4];  // coefficient per face
+};
+
+template <typename T>
+void launch_assemble(T* f, int nx, int ny, long long ld, int nrhs, T hx2, const AsmFaces<T>& fc, hipStream_t s);
+template <typename T>
+void launch_bond_add(T* f, int nx, int ny, long long ldf, const T* nb, int nbx, int nby, long long ldn, int loc, T a,
+                     int nrhs, hipStream_t s);
+template <typename T>
+void launch_fill(T* p, long long n, T v, hipStream_t s);
+void launch_axpby(const double* x, double* y, long long n, double a, double b, hipStream_t s);
+void launch_gather(const double* G, long long ldG, const int* cell, int K, const double* v, double* out, long long ldo,
+                   double a, double b, int nrhs, hipStream_t s);
+void launch_scatter(double* G, long long ldG, const int* ucell, const int* k1, const int* k2, int nu, const double* w,
+                    long long ldw, int nrhs, hipStream_t s);
+void launch_schur_gemv(const double* S, int cn, const double* x, long long ldx, int xoff, double* y, long long ldy,
+                       int yoff, int nrhs, hipStream_t s);
+void launch_schur_collect(const double* buf, long long ldb, const int* cell, int cn, int i0, int cnt, double* S,
+                          hipStream_t s);
+void launch_set_units(double* buf, long long ldb, const int* cell, int i0, int cnt, hipStream_t s);
+
+// Device state of a batched interface GMRES (all pointers device memory).
+struct GmresDev {
+    int     K, nrhs, m, nblk;
+    double* V;      // (m+1) x nrhs x K
+    double* Wv;     // nrhs x K
+    double* X;      // nrhs x K
+    double* C;      // nrhs x K (right-hand side)
+    double* H;      // nrhs x (m+1) x m
+    double* Hacc;   // nrhs x (m+1)
+    double* cs;     // nrhs x m
+    double* sn;     // nrhs x m
+    double* g;      // nrhs x (m+1)
+    double* Y;      // nrhs x m
+    double* part;   // nrhs x nblk
+    double* part2;  // nrhs x (m+1) x nblk
+    double* tolabs; // nrhs
+    double* resid;  // nrhs
+    int*    state;  // nrhs x 4: active, valid columns, status (1 converged, 2 numeric), steps
+};
+
+void gm_init(const GmresDev& g, double tol, int rel, hipStream_t s);
+void gm_resid(const GmresDev& g, hipStream_t s);
+void gm_arnoldi(const GmresDev& g, int j, hipStream_t s);
+void gm_finish_cycle(const GmresDev& g, hipStream_t s);
+
+}  // namespace fp

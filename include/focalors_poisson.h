@@ -1,0 +1,138 @@
+/*
+ * focalors_poisson.h -- C ABI of the MI355X-native Focalors_Poisson solve path.
+ *
+ * The entry points replace, one for one, the reference's solver interface for
+ * the 2-D composite Poisson solve (reference files, relative to the reference
+ * root):
+ *
+ *   fp_sub_create / fp_sub_solve*   PoissonSolver2D(nx, ny, hx, hy, bc x4) and
+ *                                   PoissonSolver2D::solve(field2&)
+ *                                   (pe/poisson/poisson_solver2d.h:14-24,
+ *                                    pe/poisson/poisson_solver2d.cpp:3-90)
+ *   fp_cat_create                   ConcatPoissonSolver2D(Variable2D*) plus
+ *                                   set_parameter(m, tol, maxIter)
+ *                                   (pe/concat/concat_solver2d.h:18-56,
+ *                                    pe/concat/concat_solver2d.cpp:7-99)
+ *   fp_cat_solve                    ConcatPoissonSolver2D::solve()
+ *                                   (pe/concat/concat_solver2d.cpp:101-223)
+ *   fp_cat_vjp                      no reference counterpart: adjoint of
+ *                                   fp_cat_solve for gradients
+ *
+ * Conventions
+ *   - Boundary codes are PDEBoundaryType values (base/location_boundary.h:62-69):
+ *     Dirichlet 0, Neumann 1, Periodic 2, Adjacented 3, Null 4.  Face order is
+ *     LocationType (base/location_boundary.h:8-16): XNegative, XPositive,
+ *     YNegative, YPositive.
+ *   - Fields are device arrays, row-major [i][j] with j (y) contiguous, as
+ *     field2 (base/field/field2.cpp:304-314).  nrhs fields of one domain are
+ *     stored back to back, `ld` elements apart (ld >= nx*ny).
+ *   - The caller owns every device buffer.  Handles own plans, tables, Schur
+ *     blocks and scratch.  Work is enqueued on the given HIP stream (NULL =
+ *     the null stream); results are in place.
+ *   - No exception crosses the ABI; every call returns an fp_status code.
+ *     fp_last_error() gives the message of the last failure on this thread.
+ *   - One handle per host thread at a time: a handle holds scratch memory and
+ *     is not reentrant (as the reference's solver objects).
+ */
+#ifndef FOCALORS_POISSON_H
+#define FOCALORS_POISSON_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fp_ctx_s* fp_ctx;
+typedef struct fp_sub_s* fp_sub;
+typedef struct fp_cat_s* fp_cat;
+
+typedef enum fp_status {
+    FP_OK               = 0,
+    FP_E_INVALID        = 1, /* bad argument: size, boundary pair, geometry */
+    FP_E_NOMEM          = 2, /* device or host allocation failed */
+    FP_E_HIP            = 3, /* HIP runtime error */
+    FP_E_NOT_CONVERGED  = 4, /* GMRES stopped at max_iter above tolerance (results still written) */
+    FP_E_NUMERIC        = 5, /* NaN/Inf met in the iteration */
+    FP_E_UNSUPPORTED    = 6  /* valid in the reference but not provided here (periodic faces) */
+} fp_status;
+
+typedef enum fp_precision {
+    FP_F64        = 0, /* transforms, tridiagonal sweeps and Krylov work in FP64 */
+    FP_F32_REFINE = 1  /* FP32 transforms and sweeps, FP64 residual and iterative refinement */
+} fp_precision;
+
+typedef struct fp_solve_info {
+    int    iterations;     /* GMRES iterations (Arnoldi steps) of the root, max over right-hand sides */
+    int    restarts;       /* GMRES restart cycles */
+    double residual;       /* final interface residual norm, max over right-hand sides */
+    double rel_residual;   /* residual / initial preconditioned right-hand-side norm */
+    int    refine_steps;   /* iterative-refinement steps (FP_F32_REFINE only) */
+    double refine_residual;/* final relative FP64 residual of the refinement */
+} fp_solve_info;
+
+/* Geometry of a composite problem: domains and Geometry2D::connect calls
+ * (base/domain/geometry2d.cpp:47-72).  bc holds the user-set face types per
+ * domain; faces named in a connection become Adjacented, as
+ * Variable2D::set_geometry does (base/domain/variable2d.cpp:25-36). */
+typedef struct fp_geometry_desc {
+    int                  nd;
+    const int*           nx;
+    const int*           ny;
+    const double*        hx;
+    const double*        hy;
+    const unsigned char* bc;    /* nd*4 face types */
+    int                  nconn;
+    const int*           conn;  /* nconn*3: (a, dir, b) = connect(a, dir, b) */
+} fp_geometry_desc;
+
+/* ---- context ----------------------------------------------------------- */
+int         fp_ctx_create(int device, fp_ctx* out);
+void        fp_ctx_destroy(fp_ctx ctx);
+const char* fp_status_str(int status);
+const char* fp_last_error(void);
+int         fp_version(void);
+
+/* ---- one subdomain: PoissonSolver2D ------------------------------------ */
+/* PoissonSolver2D(nx, ny, hx, hy, bc_xneg, bc_xpos, bc_yneg, bc_ypos),
+ * pe/poisson/poisson_solver2d.cpp:3-21. */
+int fp_sub_create(fp_ctx ctx, int nx, int ny, double hx, double hy, const unsigned char bc[4], fp_sub* out);
+void fp_sub_destroy(fp_sub sub);
+/* PoissonSolver2D::solve(field2& f), pe/poisson/poisson_solver2d.cpp:62-90:
+ * d_f holds the hx^2-scaled right-hand side on entry and the solution on exit. */
+int fp_sub_solve(fp_sub sub, double* d_f, int nrhs, long long ld, void* stream);
+/* The same solve on FP32 fields (FP32 transforms and sweeps). */
+int fp_sub_solve_f32(fp_sub sub, float* d_f, int nrhs, long long ld, void* stream);
+/* Many independent subdomain solves of different sizes in one call: the
+ * reference's loop over independent PoissonSolver2D objects.  d_f[k] holds
+ * nrhs[k] fields of subs[k] (ld = nx*ny). */
+int fp_sub_solve_batched(fp_ctx ctx, int nprob, const fp_sub* subs, double* const* d_f, const int* nrhs,
+                         void* stream);
+int fp_sub_solve_batched_f32(fp_ctx ctx, int nprob, const fp_sub* subs, float* const* d_f, const int* nrhs,
+                             void* stream);
+
+/* ---- composite geometry: ConcatPoissonSolver2D ------------------------- */
+/* ConcatPoissonSolver2D(Variable2D*) + set_parameter(m, tol, maxIter):
+ * builds the tree (geometry_tree.hpp:22-200), the subdomain plans and the
+ * dense Schur blocks of every non-leaf (schur_mat2d.cpp:5-123) on the device.
+ * tol_is_relative = 0 keeps the reference's absolute tolerance
+ * (gmres_solver2d.cpp:155,221); 1 scales it by the norm of the preconditioned
+ * interface right-hand side.  precision: fp_precision. */
+int fp_cat_create(fp_ctx ctx, const fp_geometry_desc* g, int m, double tol, int tol_is_relative, int max_iter,
+                  int precision, fp_cat* out);
+void fp_cat_destroy(fp_cat cat);
+int  fp_cat_root(fp_cat cat);
+/* ConcatPoissonSolver2D::solve(): d_fields[d] holds nrhs raw right-hand sides
+ * of domain d (overwritten with the solution); d_ghost[4*d+face] holds nrhs
+ * ghost-centre boundary data arrays of that face back to back (the
+ * Variable2D::buffer_map values, variable2d.cpp:345-416), or NULL for none. */
+int fp_cat_solve(fp_cat cat, double* const* d_fields, const double* const* d_ghost, int nrhs, void* stream,
+                 fp_solve_info* info);
+/* Adjoint of fp_cat_solve for L(u): given dL/du in d_gu[d] (nrhs fields per
+ * domain), writes dL/df into d_gf[d] and dL/dg (ghost data of Dirichlet and
+ * Neumann faces) into d_gghost[4*d+face] (NULL entries are skipped). */
+int fp_cat_vjp(fp_cat cat, const double* const* d_gu, double* const* d_gf, double* const* d_gghost, int nrhs,
+               void* stream, fp_solve_info* info);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,140 @@
+"""Subdomain solver parity on the device against the CPU oracle
+(PoissonSolver2D::solve, pe/poisson/poisson_solver2d.cpp:62-90).
+FP64 bar: relative L2 <= 1e-12.  FP32 bar: relative L2 <= 2e-5."""
+import itertools
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from gpu_util import banner, device, rel, to_dev
+
+pytestmark = pytest.mark.gpu
+D, N, A = 0, 1, 3
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    device()
+    import focalors_poisson_amd as fpa
+    return fpa.Context(0)
+
+
+def gpu_solve(ctx, f, hx, hy, bc, dtype=None):
+    import torch
+    import focalors_poisson_amd as fpa
+    s = fpa.PoissonSolver2D(ctx, f.shape[-2], f.shape[-1], hx, hy, bc)
+    t = to_dev(f, dtype)
+    s.solve(t)
+    torch.cuda.synchronize()
+    return t.double().cpu().numpy()
+
+
+def test_poisson_3x4_golden(ctx):
+    g = np.load(os.path.join(GOLD, "poisson_3x4.npz"))
+    hx, hy = float(g["hx"]), float(g["hy"])
+    for bc, u in zip(g["bc"], g["u"]):
+        banner("poisson_3x4 bc=%s" % (tuple(bc),))
+        got = gpu_solve(ctx, g["f"] * hx * hx, hx, hy, tuple(int(b) for b in bc))
+        assert rel(got, u) < 1e-12, bc
+
+
+@pytest.mark.parametrize("bc", [b for b in itertools.product([D, N, A], repeat=4) if b != (N, N, N, N)])
+def test_all_boundary_pairs_small(ctx, bc):
+    rng = np.random.default_rng(sum(bc) + 7)
+    nx, ny, hx, hy = 9, 7, 0.3, 0.5
+    f = rng.standard_normal((nx, ny)) * hx * hx
+    banner("small bc=%s" % (bc,))
+    assert rel(gpu_solve(ctx, f, hx, hy, bc), O.sub_solve(f, hx, hy, bc)) < 1e-12
+
+
+SIZES = [(2, 1), (3, 2), (5, 41), (17, 64), (33, 100), (64, 257), (40, 512), (65, 1024), (1024, 37)]
+
+
+@pytest.mark.parametrize("nx,ny", SIZES)
+@pytest.mark.parametrize("bc", [(D, D, D, D), (N, D, N, N), (D, N, D, N), (A, N, N, D)])
+def test_sizes(ctx, nx, ny, bc):
+    rng = np.random.default_rng(nx * 1000 + ny)
+    h = 1.0 / max(nx, ny)
+    f = rng.standard_normal((nx, ny)) * h * h
+    banner("sizes nx=%d ny=%d bc=%s" % (nx, ny, bc))
+    assert rel(gpu_solve(ctx, f, h, h, bc), O.sub_solve(f, h, h, bc)) < 1e-12
+
+
+@pytest.mark.parametrize("nx,ny,bc", [(256, 256, (D, D, D, D)), (512, 1024, (D, A, N, N)),
+                                      (1024, 512, (N, D, A, N)), (1024, 1024, (A, A, A, A))])
+def test_config_shapes(ctx, nx, ny, bc):
+    rng = np.random.default_rng(nx + ny)
+    h = 1.0 / 1024
+    f = rng.standard_normal((nx, ny)) * h * h
+    banner("config shape nx=%d ny=%d bc=%s" % (nx, ny, bc))
+    assert rel(gpu_solve(ctx, f, h, h, bc), O.sub_solve(f, h, h, bc)) < 1e-12
+
+
+def test_all_neumann_singular_mode(ctx):
+    rng = np.random.default_rng(11)
+    nx, ny = 24, 16
+    f = rng.standard_normal((nx, ny))
+    f -= f.mean()
+    banner("singular NNNN")
+    assert rel(gpu_solve(ctx, f, 1.0, 1.0, (N, N, N, N)), O.sub_solve(f, 1.0, 1.0, (N, N, N, N))) < 1e-12
+
+
+def test_multi_rhs_and_ld(ctx):
+    import torch
+    import focalors_poisson_amd as fpa
+    rng = np.random.default_rng(3)
+    nx, ny, nrhs = 48, 30, 5
+    f = rng.standard_normal((nrhs, nx, ny))
+    s = fpa.PoissonSolver2D(ctx, nx, ny, 1.0 / 48, 1.0 / 48, (D, N, N, D))
+    t = to_dev(f)
+    banner("multi rhs")
+    s.solve(t)
+    torch.cuda.synchronize()
+    got = t.cpu().numpy()
+    for r in range(nrhs):
+        assert rel(got[r], O.sub_solve(f[r], 1.0 / 48, 1.0 / 48, (D, N, N, D))) < 1e-12
+
+
+@pytest.mark.parametrize("nx,ny,bc", [(64, 64, (D, D, D, D)), (100, 77, (N, D, D, N)), (256, 512, (D, N, N, N))])
+def test_fp32_solve(ctx, nx, ny, bc):
+    import torch
+    rng = np.random.default_rng(5)
+    h = 1.0 / max(nx, ny)
+    f = rng.standard_normal((nx, ny)) * h * h
+    banner("fp32 nx=%d ny=%d" % (nx, ny))
+    got = gpu_solve(ctx, f, h, h, bc, dtype=torch.float32)
+    assert rel(got, O.sub_solve(f.astype(np.float32).astype(np.float64), h, h, bc)) < 2e-5
+
+
+def test_batched_uneven(ctx):
+    import torch
+    import focalors_poisson_amd as fpa
+    from focalors_poisson_amd.cases import batched_sizes
+    sizes = batched_sizes(24, seed=2, lo=8, hi=96)
+    rng = np.random.default_rng(9)
+    solvers, fields, hosts = [], [], []
+    for nx, ny in sizes:
+        h = 1.0 / max(nx, ny)
+        solvers.append(fpa.PoissonSolver2D(ctx, nx, ny, h, h, (D, D, D, D)))
+        f = rng.uniform(-1, 1, (nx, ny)) * h * h
+        hosts.append(f)
+        fields.append(to_dev(f))
+    banner("batched uneven 24")
+    fpa.solve_batched(ctx, solvers, fields)
+    torch.cuda.synchronize()
+    for (nx, ny), f, t in zip(sizes, hosts, fields):
+        h = 1.0 / max(nx, ny)
+        assert rel(t.cpu().numpy(), O.sub_solve(f, h, h, (D, D, D, D))) < 1e-12
+
+
+def test_errors_are_status_codes(ctx):
+    import focalors_poisson_amd as fpa
+    with pytest.raises(fpa.FPError) as e:
+        fpa.PoissonSolver2D(ctx, 1, 8, 1.0, 1.0, (D, D, D, D))
+    assert e.value.status == 1
+    with pytest.raises(fpa.FPError) as e:
+        fpa.PoissonSolver2D(ctx, 8, 8, 1.0, 1.0, (2, 2, D, D))
+    assert e.value.status == 6
