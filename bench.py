@@ -1,0 +1,160 @@
+#!/usr/bin/env python3
+"""Cross-geometry composite Poisson solves per second on one MI355X.
+
+Workload: the cross of 5 subdomains (centre 1024 x 1024, arms 512 x 1024 and
+1024 x 512, h = 1/1024, mixed Dirichlet/Neumann outer faces), seeded RHS
+sin(2 pi x) cos(pi y) + 1e-3 N(0,1).  One step = one composite solve
+(ConcatPoissonSolver2D::solve equivalent): refill the RHS from a resident copy,
+boundary assembly, 4 arm solves, interface GMRES on the centre, 4 arm solves.
+Inputs are resident in HBM before the timed region.  The GPU solution is
+checked against the CPU oracle's solution of the same problem (rel L2).
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=1024, help="centre size (arms n/2)")
+    ap.add_argument("--m", type=int, default=60, help="GMRES restart length on the interface")
+    ap.add_argument("--tol", type=float, default=1e-12, help="relative GMRES tolerance on the interface")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--kernel-reps", type=int, default=20)
+    return ap.parse_args()
+
+
+def time_kernel(fn, reps, torch):
+    """Average duration (ms) of fn() over reps launches, HIP events on the
+    current stream (the stream the library launches on)."""
+    fn()
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(reps):
+        fn()
+    e1.record(s)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py runs on one GPU: this build has no multi-GPU path (the composite solve would shard by "
+              "subdomain with interface all-gathers; see DESIGN.md). Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import focalors_poisson_amd as fpa
+    from focalors_poisson_amd.cases import cross
+
+    dev = torch.device("cuda:0")
+    n = a.n
+    doms, conn, fields, _ = cross(n, seed=1)
+    ctx = fpa.Context(0)
+    t0 = time.perf_counter()
+    cat = fpa.ConcatPoissonSolver2D(ctx, doms, conn, m=a.m, tol=a.tol, tol_is_relative=True, max_iter=100)
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t0
+
+    pristine = [torch.from_numpy(f[0]).to(dev) for f in fields]
+    work = [torch.empty_like(p) for p in pristine]
+    info = {}
+
+    def step():
+        for w, p in zip(work, pristine):
+            w.copy_(p)
+        info.update(cat.solve(work))
+
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms = (t1 - t0) / a.steps * 1e3
+    gpu_u = [w.cpu().numpy() for w in work]
+    cells = sum(d[0] * d[1] for d in doms)
+
+    # ---- per-kernel timing on the centre subdomain (the GMRES matvec's solve) ----
+    nx, ny = doms[0][0], doms[0][1]
+    sub = fpa.PoissonSolver2D(ctx, nx, ny, doms[0][2], doms[0][3], (3, 3, 3, 3))
+    buf = torch.randn(nx, ny, dtype=torch.float64, device=dev)
+    fbytes = 8.0 * nx * ny
+    kern = {}
+    for name, fn, nbytes in [
+        ("ytransform_fwd", lambda: sub.transform(buf), 2 * fbytes),
+        ("ytransform_inv", lambda: sub.transform(buf, inverse=True), 2 * fbytes),
+        ("tridiag", lambda: sub.chasing(buf), 2 * fbytes),
+    ]:
+        t = time_kernel(fn, a.kernel_reps, torch)
+        kern[name] = {"ms": round(t, 5), "bytes": nbytes, "gbs": round(nbytes / (t * 1e-3) / 1e9, 1)}
+    dom_name = max(kern, key=lambda k: kern[k]["ms"])
+    dom = kern[dom_name]
+
+    # ---- CPU baseline + accuracy: the oracle on the host cores ----
+    cpu = None
+    rel_l2 = None
+    if not a.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        oc = O.Composite(doms, conn, m=20, tol=a.tol, tol_is_relative=True, maxiter=100,
+                         schur_mode=O.SCHUR_SPECTRAL)
+        t0 = time.perf_counter()
+        ref, it, _ = oc.solve([f[0] for f in fields])
+        t_cpu = time.perf_counter() - t0
+        x = np.concatenate([u.ravel() for u in gpu_u])
+        y = np.concatenate([r.ravel() for r in ref])
+        rel_l2 = float(np.linalg.norm(x - y) / np.linalg.norm(y))
+        cpu = {"value": round(1.0 / t_cpu, 4), "unit": "cross solves/s", "cores": O.num_threads(),
+               "kind": "port",
+               "sample": "one full %dx%d-centre cross solve on the host (construction excluded), reference "
+                         "algorithm: full-field GMRES(m=20) with dense Schur blocks to relative %.0e, %d "
+                         "iterations, %.2f s" % (n, n, a.tol, it, t_cpu)}
+
+    out = {
+        "metric": "cross-geometry solves/s at 1e-10 rel L2: FP64 vs FP32+refine; HBM GB/s",
+        "value": round(1000.0 / ms, 3),
+        "unit": "cross solves/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "cross %d: centre %dx%d, arms %dx%d / %dx%d, FP64, interface GMRES(m=%d) to "
+                               "relative %.0e" % (n, n, n, n // 2, n, n, n // 2, a.m, a.tol),
+                   "cells": cells, "gmres_iterations": info.get("iterations"),
+                   "gmres_restarts": info.get("restarts"), "rel_residual": info.get("rel_residual"),
+                   "rel_l2_vs_oracle": rel_l2, "construct_s": round(t_build, 3)},
+        "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": dom["gbs"], "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": round(dom["gbs"] / PEAK_HBM_GBS, 4), "traffic": None},
+        "kernels": kern,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

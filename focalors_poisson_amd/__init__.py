@@ -79,6 +79,24 @@ class PoissonSolver2D:
         _capi.check(fn(self._h, ctypes.c_void_p(f.data_ptr()), nrhs, n, _stream_ptr(stream)))
         return f
 
+    def transform(self, f, inverse=False, stream=None):
+        """PoissonFFT2D::transform / transform_transpose along y, in place (float64)."""
+        import torch
+        n = self.nx * self.ny
+        nrhs = _check_field(f, torch.float64, n)
+        _capi.check(_capi.lib().fp_sub_transform(self._h, ctypes.c_void_p(f.data_ptr()), nrhs, n, int(inverse),
+                                                 _stream_ptr(stream)))
+        return f
+
+    def chasing(self, f, stream=None):
+        """ChasingMethod2D::chasing along x on a transformed field, in place (float64)."""
+        import torch
+        n = self.nx * self.ny
+        nrhs = _check_field(f, torch.float64, n)
+        _capi.check(_capi.lib().fp_sub_chasing(self._h, ctypes.c_void_p(f.data_ptr()), nrhs, n,
+                                               _stream_ptr(stream)))
+        return f
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h and h.value:

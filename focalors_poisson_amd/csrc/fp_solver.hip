@@ -605,6 +605,27 @@ int fp_sub_solve(fp_sub sub, double* d_f, int nrhs, long long ld, void* stream)
     });
 }
 
+int fp_sub_transform(fp_sub sub, double* d_f, int nrhs, long long ld, int inverse, void* stream)
+{
+    return guarded([&] {
+        FP_REQUIRE(sub && d_f && nrhs >= 0, "null argument");
+        FP_REQUIRE(ld >= (long long)sub->plan.nx * sub->plan.ny || nrhs <= 1, "ld smaller than nx*ny");
+        FP_HIP(hipSetDevice(sub->ctx->device));
+        if (nrhs) fp::launch_ytransform<double>(sub->plan.t64, d_f, ld, sub->plan.nx, nrhs, inverse ? 1 : 0,
+                                                (hipStream_t)stream);
+    });
+}
+
+int fp_sub_chasing(fp_sub sub, double* d_f, int nrhs, long long ld, void* stream)
+{
+    return guarded([&] {
+        FP_REQUIRE(sub && d_f && nrhs >= 0, "null argument");
+        FP_REQUIRE(ld >= (long long)sub->plan.nx * sub->plan.ny || nrhs <= 1, "ld smaller than nx*ny");
+        FP_HIP(hipSetDevice(sub->ctx->device));
+        if (nrhs) fp::launch_tridiag<double>(sub->plan, sub->plan.pinv64.p, d_f, ld, nrhs, (hipStream_t)stream);
+    });
+}
+
 int fp_sub_solve_f32(fp_sub sub, float* d_f, int nrhs, long long ld, void* stream)
 {
     return guarded([&] {

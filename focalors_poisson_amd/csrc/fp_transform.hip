@@ -241,8 +241,8 @@ __global__ __launch_bounds__(NT) void k_ytransform(YArgs<T> a)
             ob            = a.c0 * z.x;
         } else if (a.kind == T_NN && !a.inverse) {
             const int   m = N - 1 - j;
-            const C2<T> w = cmul(Z[m], a.aux[m]);
-            const T     s = (j == N - 1) ? a.c1 : a.c0;
+            const C2<T> w = cmul(Z[m], a.aux[m]);  // 2 C_a(m) + 2i C_b(m)
+            const T     s = T(0.5) * ((j == N - 1) ? a.c1 : a.c0);
             oa            = s * w.x;
             ob            = s * w.y;
         } else if (a.kind == T_NN) {
@@ -313,8 +313,8 @@ void TransformPlan<T>::build(int k, int n)
             x[2 * m]              = (T)cosl(ang);
             x[2 * m + 1]          = (T)sinl(ang);
         }
-        c0 = (T)(-std::sqrt(2.0 / N) * 0.5);
-        c1 = (T)(std::sqrt(1.0 / N) * 0.5);
+        c0 = (T)(-std::sqrt(2.0 / N));
+        c1 = (T)std::sqrt(1.0 / N);
     } else if (kind == T_ND) {
         for (int j = 0; j < N; j++) {
             // theta_j = 2 N pi (j+1) / (2N+1), reduced modulo 2 pi on integers

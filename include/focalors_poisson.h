@@ -99,6 +99,13 @@ void fp_sub_destroy(fp_sub sub);
 /* PoissonSolver2D::solve(field2& f), pe/poisson/poisson_solver2d.cpp:62-90:
  * d_f holds the hx^2-scaled right-hand side on entry and the solution on exit. */
 int fp_sub_solve(fp_sub sub, double* d_f, int nrhs, long long ld, void* stream);
+/* The solve's inner plugins on their own, for callers that drive the steps
+ * themselves: PoissonFFT2D::transform (inverse = 0) / transform_transpose
+ * (inverse = 1) along y (pe/poisson/poisson_fft2d.h:8-16), and
+ * ChasingMethod2D::chasing along x on the transformed field
+ * (pe/poisson/chasing_method2d.h:10-35; no transposes: mode j stays column j). */
+int fp_sub_transform(fp_sub sub, double* d_f, int nrhs, long long ld, int inverse, void* stream);
+int fp_sub_chasing(fp_sub sub, double* d_f, int nrhs, long long ld, void* stream);
 /* The same solve on FP32 fields (FP32 transforms and sweeps). */
 int fp_sub_solve_f32(fp_sub sub, float* d_f, int nrhs, long long ld, void* stream);
 /* Many independent subdomain solves of different sizes in one call: the

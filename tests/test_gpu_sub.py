@@ -138,3 +138,23 @@ def test_errors_are_status_codes(ctx):
     with pytest.raises(fpa.FPError) as e:
         fpa.PoissonSolver2D(ctx, 8, 8, 1.0, 1.0, (2, 2, D, D))
     assert e.value.status == 6
+
+
+@pytest.mark.parametrize("bcp", [(D, D), (N, N), (D, N), (N, D)])
+@pytest.mark.parametrize("ny", [5, 64, 100, 1024])
+def test_transforms_match_reference_coefficients(ctx, bcp, ny):
+    """PoissonFFT2D_*::transform / transform_transpose coefficient for
+    coefficient (same signs and mode order as the reference)."""
+    import torch
+    import focalors_poisson_amd as fpa
+    rng = np.random.default_rng(ny)
+    nx = 7
+    f = rng.standard_normal((nx, ny))
+    s = fpa.PoissonSolver2D(ctx, nx, ny, 1.0, 1.0, (D, D) + bcp)
+    for inv in (False, True):
+        t = to_dev(f)
+        banner("transform bc=%s ny=%d inverse=%d" % (bcp, ny, inv))
+        s.transform(t, inverse=inv)
+        torch.cuda.synchronize()
+        ref = O.transform(bcp[0], bcp[1], f, inverse=inv)
+        assert np.abs(t.cpu().numpy() - ref).max() < 1e-12 * max(1.0, np.abs(ref).max())
