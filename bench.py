@@ -30,7 +30,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=1024, help="centre size (arms n/2)")
-    ap.add_argument("--m", type=int, default=60, help="GMRES restart length on the interface")
+    ap.add_argument("--m", type=int, default=200, help="GMRES restart length on the interface")
     ap.add_argument("--tol", type=float, default=1e-12, help="relative GMRES tolerance on the interface")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=20)

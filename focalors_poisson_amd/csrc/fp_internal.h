@@ -115,6 +115,23 @@ void launch_ytransform(const TransformPlan<T>& tp, T* data, long long ld, int nx
 template <typename T>
 void launch_tridiag(const SubPlan& sp, const T* pinv, T* data, long long ld, int nrhs, hipStream_t s);
 
+// Arguments of the line-sourced tridiagonal sweep (interface operator).
+template <typename T>
+struct LinesArgs {
+    const T*  lines;    // per rhs: Xh0 (ny) | Xh1 (ny) | a (nx) | b (nx)
+    long long ldl;      // 2 ny + 2 nx
+    const T*  phi;      // phi0 (ny) | phi1 (ny)
+    const T*  pinv;     // nx*ny reciprocal pivots
+    T*        xu;       // per rhs: coefficient rows 0 and nx-1 (2 ny)
+    T*        colpart;  // per rhs, per workgroup: 2 x nx column partial sums
+    int       nx, ny;
+    int       xrows, ycols;
+};
+template <typename T>
+int launch_tridiag_lines(const SubPlan& sp, const LinesArgs<T>& a, int nrhs, hipStream_t s);
+template <typename T>
+int lines_workgroups(const SubPlan& sp);
+
 // Full subdomain solve of nrhs fields (transform, sweep, inverse transform).
 template <typename T>
 void sub_solve(SubPlan& sp, T* data, long long ld, int nrhs, hipStream_t s);

@@ -9,6 +9,7 @@
 #include "fp_vec.h"
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -130,6 +131,12 @@ struct Node {
     // GMRES workspace, sized for ws_nrhs systems
     int            ws_nrhs = 0, ws_m = 0, nblk = 1;
     DevBuf<double> V, Wv, Tv, X, C, H, Hacc, cs, sn, g, Y, part, part2, tolabs, resid, G;
+    // structured interface operator R A^-1 E of this node's subdomain
+    int            face_off[4] = {-1, -1, -1, -1};
+    int            has_x = 0, has_y = 0, nwg = 0;
+    long long      ldl = 0;
+    DevBuf<double> phi, LB, XU, CP;
+    DevBuf<int>    kface, kpos;
     DevBuf<int>    state;
     int*           h_state = nullptr;
     double*        h_resid = nullptr;
@@ -203,7 +210,7 @@ static void ensure_ws(fp_cat_s* c, Node& nd, int nrhs)
     if (nd.ws_nrhs >= nrhs && nd.ws_m == c->m) return;
     const int       K = nd.K, m = c->m;
     const long long n = dom_n(c, nd.dom);
-    nd.nblk           = std::max(1, std::min(64, (K + 511) / 512));
+    nd.nblk           = (K + 63) / 64;
     nd.V.alloc((size_t)(m + 1) * nrhs * K);
     nd.Wv.alloc((size_t)nrhs * K);
     nd.Tv.alloc((size_t)nrhs * K);
@@ -216,11 +223,16 @@ static void ensure_ws(fp_cat_s* c, Node& nd, int nrhs)
     nd.g.alloc((size_t)nrhs * (m + 1));
     nd.Y.alloc((size_t)nrhs * m);
     nd.part.alloc((size_t)nrhs * nd.nblk);
-    nd.part2.alloc((size_t)nrhs * (m + 1) * nd.nblk);
+    nd.part2.alloc((size_t)nrhs * (m + 1));
     nd.tolabs.alloc(nrhs);
     nd.resid.alloc(nrhs);
     nd.state.alloc((size_t)nrhs * 4);
     nd.G.alloc((size_t)nrhs * n);
+    nd.LB.alloc((size_t)nrhs * nd.ldl);
+    launch_fill<double>(nd.LB.p, (long long)nrhs * nd.ldl, 0.0, nullptr);
+    nd.XU.alloc((size_t)nrhs * 2 * c->ny[nd.dom]);
+    nd.CP.alloc((size_t)nrhs * nd.nwg * 2 * c->nx[nd.dom]);
+    FP_HIP(hipDeviceSynchronize());
     if (nd.h_state) (void)hipHostFree(nd.h_state);
     if (nd.h_resid) (void)hipHostFree(nd.h_resid);
     FP_HIP(hipHostMalloc((void**)&nd.h_state, sizeof(int) * 4 * nrhs));
@@ -239,9 +251,61 @@ static GmresDev gdev(fp_cat_s* c, Node& nd, int nrhs)
     return g;
 }
 
-// out = v - R A^-1 E S v  (GMRESSolver2D::Afun on interface vectors)
+// out = v - R A^-1 E S v  (GMRESSolver2D::Afun on interface vectors), with
+// R A^-1 E applied through the line-sourced sweep: no full root field is
+// formed, transformed or read.
+static void matvec_lines(fp_cat_s* c, Node& nd, const double* v, double* out, int nrhs, hipStream_t s)
+{
+    const int      nx = c->nx[nd.dom], ny = c->ny[nd.dom];
+    SchurLinesArgs ga{};
+    int            rows = 0;
+    for (auto& L : nd.ch) {
+        SchurChild& sc = ga.ch[ga.nch++];
+        sc.S    = L.S.p;
+        sc.cn   = L.cn;
+        sc.voff = L.off;
+        sc.lboff = L.loc == XNEG ? 0 : L.loc == XPOS ? ny : L.loc == YNEG ? 2 * ny : 2 * ny + nx;
+        sc.row0 = rows;
+        rows += L.cn;
+    }
+    ga.total_rows = rows;
+    ga.v          = v;
+    ga.ldv        = nd.K;
+    ga.lb         = nd.LB.p;
+    ga.ldl        = nd.ldl;
+    launch_schur_lines(ga, nrhs, s);
+    if (nd.has_x) launch_ytransform<double>(nd.plan.t64, nd.LB.p, nd.ldl, 2, nrhs, 0, s);
+    LinesArgs<double> la{};
+    la.lines   = nd.LB.p;
+    la.ldl     = nd.ldl;
+    la.phi     = nd.phi.p;
+    la.pinv    = nd.plan.pinv64.p;
+    la.xu      = nd.XU.p;
+    la.colpart = nd.CP.p;
+    la.nx      = nx;
+    la.ny      = ny;
+    la.xrows   = nd.has_x;
+    la.ycols   = nd.has_y;
+    launch_tridiag_lines<double>(nd.plan, la, nrhs, s);
+    if (nd.has_x) launch_ytransform<double>(nd.plan.t64, nd.XU.p, 2LL * ny, 2, nrhs, 1, s);
+    launch_lines_assemble(v, out, nd.K, nd.kface.p, nd.kpos.p, nd.XU.p, ny, nd.CP.p, nd.nwg, nx, nrhs, s);
+}
+
+static bool full_field_matvec()
+{
+    static const bool v = [] {
+        const char* e = std::getenv("FP_MATVEC");
+        return e && std::strcmp(e, "full") == 0;
+    }();
+    return v;
+}
+
 static void matvec(fp_cat_s* c, Node& nd, const double* v, double* out, int nrhs, hipStream_t s)
 {
+    if (!full_field_matvec()) {
+        matvec_lines(c, nd, v, out, nrhs, s);
+        return;
+    }
     const long long n = dom_n(c, nd.dom);
     for (auto& L : nd.ch) launch_schur_gemv(L.S.p, L.cn, v, nd.K, L.off, nd.Tv.p, nd.K, L.off, nrhs, s);
     launch_fill<double>(nd.G.p, n * nrhs, 0.0, s);
@@ -378,6 +442,29 @@ static void build_node(fp_cat_s* c, int d, hipStream_t s)
     }
     nd->K = (int)cells.size();
     if (nd->K) {
+        const int nx = c->nx[d], ny = c->ny[d];
+        std::vector<int> kf, kp;
+        for (auto& L : nd->ch) {
+            nd->face_off[L.loc] = L.off;
+            if (L.loc == XNEG || L.loc == XPOS) nd->has_x = 1;
+            else nd->has_y = 1;
+            for (int t = 0; t < L.cn; t++) {
+                kf.push_back(L.loc);
+                kp.push_back(t);
+            }
+        }
+        nd->kface.alloc(kf.size()); nd->kface.upload(kf.data(), kf.size());
+        nd->kpos.alloc(kp.size()); nd->kpos.upload(kp.data(), kp.size());
+        nd->ldl = 2LL * ny + 2LL * nx;
+        nd->nwg = lines_workgroups<double>(nd->plan);
+        // phi0 / phi1: transform of the unit vectors e_0 and e_{ny-1}
+        std::vector<double> e(2 * (size_t)ny, 0.0);
+        e[0]               = 1.0;
+        e[ny + (ny - 1)]   = 1.0;
+        nd->phi.alloc(e.size());
+        nd->phi.upload(e.data(), e.size());
+        launch_ytransform<double>(nd->plan.t64, nd->phi.p, ny, 2, 1, 0, s);
+        FP_HIP(hipStreamSynchronize(s));
         nd->cell.alloc(cells.size());
         nd->cell.upload(cells.data(), cells.size());
         std::vector<int> uc, k1, k2;

@@ -153,6 +153,136 @@ __global__ __launch_bounds__(1024) void k_tridiag(T* __restrict__ data, long lon
     }
 }
 
+// Line-sourced variant for the interface operator R A^-1 E of a root whose
+// right-hand side is zero except on its interface lines.  The y transform of
+// such a field is rank-limited: row i transforms to
+//   [i == 0] Xh0(k) + [i == nx-1] Xh1(k) + phi0(k) a_i + phi1(k) b_i
+// where Xh0/Xh1 are the transformed x-lines (rows 0 and nx-1), a/b the y-line
+// data (columns 0 and ny-1) and phi0/phi1 the transform of the unit vectors
+// e_0/e_{ny-1}.  The right-hand side is generated in registers, the sweep is
+// the same chained-chunk Thomas, and only what the restriction needs leaves
+// the kernel: rows 0 and nx-1 of the solution's coefficients, and the column
+// sums sum_k phi0(k) u(i,k), sum_k phi1(k) u(i,k) (= u(i,0), u(i,ny-1) after
+// the inverse transform), reduced over this workgroup's modes.
+template <typename T, int CH>
+__global__ __launch_bounds__(1024) void k_tridiag_lines(LinesArgs<T> a, int P, int MB)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* SA = reinterpret_cast<T*>(smem_raw);
+    T* SB = SA + P * MB;
+    T* SY = SB + P * MB;
+
+    const int  nx = a.nx, ny = a.ny;
+    const int  tid = threadIdx.x;
+    const int  jl  = tid % MB;
+    const int  c   = tid / MB;
+    const int  j   = blockIdx.x * MB + jl;
+    const bool act = (c < P) && (j < ny);
+    const int  r   = blockIdx.y;
+    const T*   lb  = a.lines + (long long)r * a.ldl;
+    const T*   xh0 = lb;
+    const T*   xh1 = lb + ny;
+    const T*   av  = lb + 2 * ny;
+    const T*   bv  = av + nx;
+    const T    p0  = act ? a.phi[j] : T(0);
+    const T    p1  = act ? a.phi[ny + j] : T(0);
+    const int  i0  = c * CH;
+
+    T fv[CH], pv[CH];
+#pragma unroll
+    for (int k = 0; k < CH; k++) {
+        const int  i  = i0 + k;
+        const bool ok = act && i < nx;
+        T          v  = T(0);
+        if (ok) {
+            v = p0 * av[i] + p1 * bv[i];
+            if (i == 0) v += xh0[j];
+            if (i == nx - 1) v += xh1[j];
+        }
+        fv[k] = v;
+        pv[k] = ok ? a.pinv[(long long)i * ny + j] : T(0);
+    }
+    T A = T(1), B = T(0);
+#pragma unroll
+    for (int k = 0; k < CH; k++)
+        if (i0 + k < nx) {
+            A = -pv[k] * A;
+            B = pv[k] * (fv[k] - B);
+        }
+    if (c < P) {
+        SA[c * MB + jl] = A;
+        SB[c * MB + jl] = B;
+    }
+    __syncthreads();
+    if (c == 0 && j < ny) {
+        T y = T(0);
+        for (int q = 0; q < P; q++) {
+            SY[q * MB + jl] = y;
+            y               = SA[q * MB + jl] * y + SB[q * MB + jl];
+        }
+    }
+    __syncthreads();
+    T y = (c < P) ? SY[c * MB + jl] : T(0);
+#pragma unroll
+    for (int k = 0; k < CH; k++)
+        if (i0 + k < nx) {
+            y     = pv[k] * (fv[k] - y);
+            fv[k] = y;
+        }
+    __syncthreads();
+    A = T(1);
+    B = T(0);
+#pragma unroll
+    for (int k = CH - 1; k >= 0; k--)
+        if (i0 + k < nx) {
+            B = fv[k] - pv[k] * B;
+            A = -pv[k] * A;
+        }
+    if (c < P) {
+        SA[c * MB + jl] = A;
+        SB[c * MB + jl] = B;
+    }
+    __syncthreads();
+    if (c == 0 && j < ny) {
+        T x = T(0);
+        for (int q = P - 1; q >= 0; q--) {
+            SY[q * MB + jl] = x;
+            x               = SA[q * MB + jl] * x + SB[q * MB + jl];
+        }
+    }
+    __syncthreads();
+    T x = (c < P) ? SY[c * MB + jl] : T(0);
+#pragma unroll
+    for (int k = CH - 1; k >= 0; k--)
+        if (i0 + k < nx) {
+            x     = fv[k] - pv[k] * x;
+            fv[k] = x;
+        }
+    if (a.xrows && act) {
+        if (i0 == 0) a.xu[(long long)r * 2 * ny + j] = fv[0];
+#pragma unroll
+        for (int k = 0; k < CH; k++)
+            if (i0 + k == nx - 1) a.xu[(long long)r * 2 * ny + ny + j] = fv[k];
+    }
+    if (a.ycols) {
+        // reduce phi0 u, phi1 u over the MB modes of this chunk (lanes c*MB .. c*MB+MB-1)
+        T* out = a.colpart + ((long long)r * gridDim.x + blockIdx.x) * 2 * nx;
+#pragma unroll
+        for (int k = 0; k < CH; k++) {
+            T s0 = p0 * fv[k], s1 = p1 * fv[k];
+            for (int off = MB / 2; off > 0; off >>= 1) {
+                s0 += __shfl_xor(s0, off, 64);
+                s1 += __shfl_xor(s1, off, 64);
+            }
+            const int i = i0 + k;
+            if (jl == 0 && c < P && i < nx) {
+                out[i]      = s0;
+                out[nx + i] = s1;
+            }
+        }
+    }
+}
+
 template <typename T, int CH>
 static void launch_tridiag_ch(const SubPlan& sp, const T* pinv, T* data, long long ld, int nrhs, hipStream_t s)
 {
@@ -175,6 +305,41 @@ void launch_tridiag(const SubPlan& sp, const T* pinv, T* data, long long ld, int
     else if (sp.nx <= 32768) launch_tridiag_ch<T, 32>(sp, pinv, data, ld, nrhs, s);
     else throw Error(FP_E_INVALID, "nx above 32768 is not supported");
 }
+
+template <typename T, int CH>
+static int launch_lines_ch(const SubPlan& sp, const LinesArgs<T>& a, int nrhs, hipStream_t s)
+{
+    const int P = (sp.nx + CH - 1) / CH;
+    int       MB = 1;
+    while (MB * 2 * P <= 1024 && MB < 64) MB *= 2;
+    if (MB * P > 1024) throw Error(FP_E_INVALID, "nx too large for the tridiagonal kernel");
+    const size_t lds = 3 * (size_t)P * MB * sizeof(T);
+    dim3         grid((sp.ny + MB - 1) / MB, nrhs);
+    hipLaunchKernelGGL((k_tridiag_lines<T, CH>), grid, dim3(MB * P), lds, s, a, P, MB);
+    FP_HIP(hipGetLastError());
+    return grid.x;
+}
+
+template <typename T>
+int lines_workgroups(const SubPlan& sp)
+{
+    const int CH = sp.nx <= 1024 ? 16 : 32;
+    const int P  = (sp.nx + CH - 1) / CH;
+    int       MB = 1;
+    while (MB * 2 * P <= 1024 && MB < 64) MB *= 2;
+    return (sp.ny + MB - 1) / MB;
+}
+template int lines_workgroups<double>(const SubPlan&);
+
+template <typename T>
+int launch_tridiag_lines(const SubPlan& sp, const LinesArgs<T>& a, int nrhs, hipStream_t s)
+{
+    FP_REQUIRE(!sp.no_dirichlet, "line operator needs a Dirichlet-like face");
+    if (sp.nx <= 1024) return launch_lines_ch<T, 16>(sp, a, nrhs, s);
+    if (sp.nx <= 32768) return launch_lines_ch<T, 32>(sp, a, nrhs, s);
+    throw Error(FP_E_INVALID, "nx above 32768 is not supported");
+}
+template int launch_tridiag_lines<double>(const SubPlan&, const LinesArgs<double>&, int, hipStream_t);
 
 template void launch_tridiag<double>(const SubPlan&, const double*, double*, long long, int, hipStream_t);
 template void launch_tridiag<float>(const SubPlan&, const float*, float*, long long, int, hipStream_t);

@@ -27,13 +27,28 @@ void launch_scatter(double* G, long long ldG, const int* ucell, const int* k1, c
                     long long ldw, int nrhs, hipStream_t s);
 void launch_schur_gemv(const double* S, int cn, const double* x, long long ldx, int xoff, double* y, long long ldy,
                        int yoff, int nrhs, hipStream_t s);
+struct SchurChild {
+    const double* S;
+    int           cn, voff, lboff, row0;
+};
+struct SchurLinesArgs {
+    SchurChild    ch[4];
+    int           nch, total_rows;
+    const double* v;
+    long long     ldv;
+    double*       lb;
+    long long     ldl;
+};
+void launch_schur_lines(const SchurLinesArgs& a, int nrhs, hipStream_t s);
+void launch_lines_assemble(const double* v, double* out, int K, const int* kface, const int* kpos, const double* xu,
+                           int ny, const double* colpart, int nwg, int nx, int nrhs, hipStream_t s);
 void launch_schur_collect(const double* buf, long long ldb, const int* cell, int cn, int i0, int cnt, double* S,
                           hipStream_t s);
 void launch_set_units(double* buf, long long ldb, const int* cell, int i0, int cnt, hipStream_t s);
 
 // Device state of a batched interface GMRES (all pointers device memory).
 struct GmresDev {
-    int     K, nrhs, m, nblk;
+    int     K, nrhs, m, nblk;  // nblk = ceil(K / 64)
     double* V;      // (m+1) x nrhs x K
     double* Wv;     // nrhs x K
     double* X;      // nrhs x K
@@ -44,8 +59,8 @@ struct GmresDev {
     double* sn;     // nrhs x m
     double* g;      // nrhs x (m+1)
     double* Y;      // nrhs x m
-    double* part;   // nrhs x nblk
-    double* part2;  // nrhs x (m+1) x nblk
+    double* part;   // nrhs x nblk partial sums
+    double* part2;  // nrhs x (m+1): dot products of the current pass
     double* tolabs; // nrhs
     double* resid;  // nrhs
     int*    state;  // nrhs x 4: active, valid columns, status (1 converged, 2 numeric), steps

@@ -7,6 +7,12 @@ namespace fp {
 
 constexpr int VB = 256;  // threads per block of the vector kernels
 
+__device__ __forceinline__ double wave_reduce_sum(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
 // ---- boundary assembly + h^2 scaling ----------------------------------------
 // ConcatPoissonSolver2D::boundary_assembly and the hx^2 scaling
 // (pe/concat/concat_solver2d.cpp:110-116,159-223):
@@ -160,6 +166,62 @@ void launch_schur_gemv(const double* S, int cn, const double* x, long long ldx, 
     FP_HIP(hipGetLastError());
 }
 
+// All children's Schur products of one root in one launch, written straight
+// into the root's line buffer (x-lines as rows 0/1, y-lines as a/b):
+// one wavefront per output row, 4 rows per block.
+__global__ __launch_bounds__(256) void k_schur_lines(SchurLinesArgs a)
+{
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int row  = blockIdx.x * 4 + wave;
+    if (row >= a.total_rows) return;
+    int c = 0;
+    while (c + 1 < a.nch && row >= a.ch[c + 1].row0) c++;
+    const SchurChild& C  = a.ch[c];
+    const int         i  = row - C.row0;
+    const long long   r  = blockIdx.y;
+    const double*     Si = C.S + (long long)i * C.cn;
+    const double*     xr = a.v + r * a.ldv + C.voff;
+    double            acc = 0.0;
+#pragma unroll 4
+    for (int j = lane; j < C.cn; j += 64) acc += Si[j] * xr[j];
+    acc = wave_reduce_sum(acc);
+    if (lane == 0) a.lb[r * a.ldl + C.lboff + i] = acc;
+}
+void launch_schur_lines(const SchurLinesArgs& a, int nrhs, hipStream_t s)
+{
+    if (a.total_rows <= 0) return;
+    hipLaunchKernelGGL(k_schur_lines, dim3((a.total_rows + 3) / 4, nrhs), dim3(256), 0, s, a);
+    FP_HIP(hipGetLastError());
+}
+
+// out = v - t, where t is the restriction of the line-sourced solution:
+// x-lines from the inverse-transformed coefficient rows, y-lines from the
+// column sums of the workgroups (fixed order)
+__global__ void k_lines_assemble(const double* v, double* out, int K, const int* kface, const int* kpos,
+                                 const double* xu, int ny, const double* colpart, int nwg, int nx)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    const long long r = blockIdx.y;
+    const int       f = kface[k], p = kpos[k];
+    double          t;
+    if (f == XNEG) t = xu[r * 2 * ny + p];
+    else if (f == XPOS) t = xu[r * 2 * ny + ny + p];
+    else {
+        const double* cp = colpart + r * (long long)nwg * 2 * nx + (f == YNEG ? 0 : nx) + p;
+        t                = 0.0;
+        for (int w = 0; w < nwg; w++) t += cp[(long long)w * 2 * nx];
+    }
+    out[r * K + k] = v[r * K + k] - t;
+}
+void launch_lines_assemble(const double* v, double* out, int K, const int* kface, const int* kpos, const double* xu,
+                           int ny, const double* colpart, int nwg, int nx, int nrhs, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_lines_assemble, dim3((K + 255) / 256, nrhs), dim3(256), 0, s, v, out, K, kface, kpos, xu,
+                       ny, colpart, nwg, nx);
+    FP_HIP(hipGetLastError());
+}
+
 // S(j, i) = response at line position j to the unit source at position i0+q:
 // buf holds `cnt` child fields, field q solved for the unit at i0+q.
 __global__ void k_schur_collect(const double* buf, long long ldb, const int* cell, int cn, int i0, double* S)
@@ -193,148 +255,153 @@ void launch_set_units(double* buf, long long ldb, const int* cell, int i0, int c
 // GMRESSolver2D::solve (pe/concat/gmres_solver2d.cpp:123-252) restated on
 // interface vectors of length K, nrhs independent systems in lockstep with
 // per-system masks.  Orthogonalisation is classical Gram-Schmidt applied
-// twice (the reference uses modified Gram-Schmidt; both produce the same
-// Hessenberg matrix in exact arithmetic, CGS2 needs one reduction per pass
-// instead of one per basis vector).  Reductions are two-stage and
-// deterministic: per-block partial sums, then a fixed-order final sum.
+// twice (the reference uses modified Gram-Schmidt; both give the same
+// Hessenberg matrix in exact arithmetic, and CGS2 needs one reduction per
+// pass instead of one per basis vector).  All reductions have a fixed order
+// (no atomics), so results are reproducible run to run.
+//
+// Element kernels use one 64-lane wavefront per block and g.nblk = ceil(K/64)
+// blocks per system; a dot product is one block per basis vector.
 
-// sum of part[0..n) in a fixed order by one block
-__device__ double block_sum_fixed(const double* part, int n, double* sh)
+constexpr int GB = 64;
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// fixed-order sum of n values by one 64-lane block
+__device__ double sum_fixed64(const double* p, int n)
 {
     double acc = 0.0;
-    for (int q = threadIdx.x; q < n; q += blockDim.x) acc += part[q];
-    sh[threadIdx.x] = acc;
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-        __syncthreads();
-    }
-    const double r = sh[0];
-    __syncthreads();
-    return r;
+    for (int q = threadIdx.x; q < n; q += GB) acc += p[q];
+    return wave_sum(acc);
 }
 
-__device__ double block_reduce(double v, double* sh)
+// tolabs[r] = tol * ||C_r|| (relative) or tol (absolute)
+__global__ __launch_bounds__(GB) void k_gm_norm_part(GmresDev g)
 {
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-        __syncthreads();
+    const int       r   = blockIdx.y;
+    const long long K   = g.K;
+    const long long k   = blockIdx.x * (long long)GB + threadIdx.x;
+    const double    c   = k < K ? g.C[r * K + k] : 0.0;
+    const double    acc = wave_sum(c * c);
+    if (threadIdx.x == 0) g.part[(long long)r * g.nblk + blockIdx.x] = acc;
+}
+__global__ __launch_bounds__(GB) void k_gm_set_tol(GmresDev g, double tol, int rel)
+{
+    const int    r  = blockIdx.x;
+    const double nc = sqrt(sum_fixed64(g.part + (long long)r * g.nblk, g.nblk));
+    if (threadIdx.x == 0) {
+        g.tolabs[r]        = rel ? tol * nc : tol;
+        g.resid[r]         = nc;
+        g.state[r * 4 + 0] = 0;
+        g.state[r * 4 + 1] = 0;
+        g.state[r * 4 + 2] = 0;
+        g.state[r * 4 + 3] = 0;
     }
-    const double r = sh[0];
-    __syncthreads();
-    return r;
 }
 
-// R = C - BX into V0, partial norms
-__global__ __launch_bounds__(VB) void k_gm_resid(GmresDev g)
+// R = C - BX into V0 and partial norms
+__global__ __launch_bounds__(GB) void k_gm_resid(GmresDev g)
 {
-    __shared__ double sh[VB];
-    const int         r = blockIdx.y;
-    const long long   K = g.K;
-    double*           v0 = g.V + (long long)r * K;
-    double            acc = 0.0;
-    for (long long k = blockIdx.x * (long long)VB + threadIdx.x; k < K; k += (long long)gridDim.x * VB) {
-        const double t = g.C[r * K + k] - g.Wv[r * K + k];
-        v0[k]          = t;
-        acc += t * t;
+    const int       r = blockIdx.y;
+    const long long K = g.K;
+    const long long k = blockIdx.x * (long long)GB + threadIdx.x;
+    double          t = 0.0;
+    if (k < K) {
+        t                       = g.C[r * K + k] - g.Wv[r * K + k];
+        g.V[(long long)r * K + k] = t;
     }
-    acc = block_reduce(acc, sh);
+    const double acc = wave_sum(t * t);
     if (threadIdx.x == 0) g.part[(long long)r * g.nblk + blockIdx.x] = acc;
 }
 
-// beta = ||R||, V0 = R / beta, cycle state; convergence at cycle start
+// beta = ||R||, V0 = R / beta; convergence test at cycle start
 // (gmres_solver2d.cpp:148-168)
-__global__ __launch_bounds__(VB) void k_gm_cycle_start(GmresDev g)
+__global__ __launch_bounds__(GB) void k_gm_cycle_start(GmresDev g)
 {
-    __shared__ double sh[VB];
-    const int         r    = blockIdx.y;
-    const double      beta = sqrt(block_sum_fixed(g.part + (long long)r * g.nblk, g.nblk, sh));
-    const long long   K    = g.K;
-    const bool        conv = beta < g.tolabs[r];
+    const int       r    = blockIdx.y;
+    const double    beta = sqrt(sum_fixed64(g.part + (long long)r * g.nblk, g.nblk));
+    const long long K    = g.K;
     if (!isfinite(beta)) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             g.state[r * 4 + 0] = 0;
             g.state[r * 4 + 2] = 2;  // numeric failure
-            g.resid[r]          = beta;
+            g.resid[r]         = beta;
         }
         return;
     }
-    if (!conv) {
-        double* v0 = g.V + (long long)r * K;
-        for (long long k = blockIdx.x * (long long)VB + threadIdx.x; k < K; k += (long long)gridDim.x * VB)
-            v0[k] = v0[k] * (1.0 / beta);
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        g.resid[r] = beta;
+    const bool conv = beta < g.tolabs[r];
+    const long long k = blockIdx.x * (long long)GB + threadIdx.x;
+    if (!conv && k < K) g.V[(long long)r * K + k] *= (1.0 / beta);
+    if (blockIdx.x == 0) {
         const int m = g.m;
-        for (int q = 0; q <= m; q++) g.g[(long long)r * (m + 1) + q] = 0.0;
-        g.g[(long long)r * (m + 1)] = beta;
-        g.state[r * 4 + 0]          = conv ? 0 : 1;  // active in this cycle
-        g.state[r * 4 + 1]          = 0;             // valid columns
-        if (conv) g.state[r * 4 + 2] = 1;            // converged
+        for (int q = threadIdx.x; q <= m; q += GB) g.g[(long long)r * (m + 1) + q] = q == 0 ? beta : 0.0;
+        if (threadIdx.x == 0) {
+            g.resid[r]         = beta;
+            g.state[r * 4 + 0] = conv ? 0 : 1;  // active in this cycle
+            g.state[r * 4 + 1] = 0;             // valid columns
+            if (conv) g.state[r * 4 + 2] = 1;   // converged
+        }
     }
 }
 
-// partial dots of V_i (i <= j) with w: part2[r][i][blk]
-__global__ __launch_bounds__(VB) void k_gm_dots(GmresDev g, int j)
+// Hd[r][i] = V_i . w, one block per basis vector
+__global__ __launch_bounds__(256) void k_gm_dots(GmresDev g, int j)
 {
-    __shared__ double sh[VB];
-    const int         r = blockIdx.y;
+    __shared__ double sh[4];
+    const int r = blockIdx.y, i = blockIdx.x;
     if (!g.state[r * 4 + 0]) return;
     const long long K   = g.K;
-    const long long ldv = (long long)g.nrhs * K;
-    for (int i = 0; i <= j; i++) {
-        const double* vi  = g.V + i * ldv + (long long)r * K;
-        double        acc = 0.0;
-        for (long long k = blockIdx.x * (long long)VB + threadIdx.x; k < K; k += (long long)gridDim.x * VB)
-            acc += vi[k] * g.Wv[r * K + k];
-        acc = block_reduce(acc, sh);
-        if (threadIdx.x == 0) g.part2[((long long)r * (g.m + 1) + i) * g.nblk + blockIdx.x] = acc;
-    }
+    const double*   vi  = g.V + (long long)i * g.nrhs * K + (long long)r * K;
+    const double*   w   = g.Wv + (long long)r * K;
+    double          acc = 0.0;
+    for (long long k = threadIdx.x; k < K; k += 256) acc += vi[k] * w[k];
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) g.part2[(long long)r * (g.m + 1) + i] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
 }
 
-// h_i = sum part2; w -= sum_i h_i V_i; Hacc += h (pass 1 sets, pass 2 adds);
-// partial norms of the new w
-__global__ __launch_bounds__(VB) void k_gm_update(GmresDev g, int j, int pass)
+// w -= sum_i h_i V_i; Hacc (= h on pass 0, += h on pass 1); partial norms
+__global__ __launch_bounds__(GB) void k_gm_update(GmresDev g, int j, int pass)
 {
-    __shared__ double sh[VB];
     __shared__ double hs[1024];
-    const int         r = blockIdx.y;
+    const int r = blockIdx.y;
     if (!g.state[r * 4 + 0]) return;
     const int m = g.m;
-    for (int i = 0; i <= j; i++) {
-        const double h = block_sum_fixed(g.part2 + ((long long)r * (m + 1) + i) * g.nblk, g.nblk, sh);
-        if (threadIdx.x == 0) hs[i] = h;
-    }
+    for (int i = threadIdx.x; i <= j; i += GB) hs[i] = g.part2[(long long)r * (m + 1) + i];
     __syncthreads();
-    if (blockIdx.x == 0 && threadIdx.x <= j) {
+    if (blockIdx.x == 0) {
         double* H = g.Hacc + (long long)r * (m + 1);
-        H[threadIdx.x] = (pass == 0 ? 0.0 : H[threadIdx.x]) + hs[threadIdx.x];
+        for (int i = threadIdx.x; i <= j; i += GB) H[i] = (pass == 0 ? 0.0 : H[i]) + hs[i];
     }
     const long long K   = g.K;
     const long long ldv = (long long)g.nrhs * K;
-    double          acc = 0.0;
-    for (long long k = blockIdx.x * (long long)VB + threadIdx.x; k < K; k += (long long)gridDim.x * VB) {
-        double w = g.Wv[r * K + k];
-        for (int i = 0; i <= j; i++) w -= hs[i] * g.V[i * ldv + (long long)r * K + k];
+    const long long k   = blockIdx.x * (long long)GB + threadIdx.x;
+    double          w   = 0.0;
+    if (k < K) {
+        w                 = g.Wv[r * K + k];
+        const double* vk  = g.V + (long long)r * K + k;
+        for (int i = 0; i <= j; i++) w -= hs[i] * vk[i * ldv];
         g.Wv[r * K + k] = w;
-        acc += w * w;
     }
-    acc = block_reduce(acc, sh);
+    const double acc = wave_sum(w * w);
     if (threadIdx.x == 0) g.part[(long long)r * g.nblk + blockIdx.x] = acc;
 }
 
 // h_{j+1,j} = ||w||; breakdown check; V_{j+1}; Givens update of column j and
 // of g; convergence estimate |g_{j+1}| < tol (gmres_solver2d.cpp:189-225)
-__global__ __launch_bounds__(VB) void k_gm_finish(GmresDev g, int j)
+__global__ __launch_bounds__(GB) void k_gm_finish(GmresDev g, int j)
 {
-    __shared__ double sh[VB];
-    const int         r = blockIdx.y;
+    __shared__ double col[1026];
+    __shared__ double scs[1024], ssn[1024];
+    const int r = blockIdx.y;
     if (!g.state[r * 4 + 0]) return;
     const int       m   = g.m;
-    const double    hn  = sqrt(block_sum_fixed(g.part + (long long)r * g.nblk, g.nblk, sh));
+    const double    hn  = sqrt(sum_fixed64(g.part + (long long)r * g.nblk, g.nblk));
     const long long K   = g.K;
     const long long ldv = (long long)g.nrhs * K;
     if (!isfinite(hn)) {
@@ -348,128 +415,138 @@ __global__ __launch_bounds__(VB) void k_gm_finish(GmresDev g, int j)
         if (blockIdx.x == 0 && threadIdx.x == 0) g.state[r * 4 + 0] = 0;
         return;
     }
-    double* vn = g.V + (j + 1) * ldv + (long long)r * K;
-    for (long long k = blockIdx.x * (long long)VB + threadIdx.x; k < K; k += (long long)gridDim.x * VB)
-        vn[k] = g.Wv[r * K + k] * (1.0 / hn);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        double*       H  = g.H + (long long)r * (m + 1) * m;
-        double*       cs = g.cs + (long long)r * m;
-        double*       sn = g.sn + (long long)r * m;
-        double*       gg = g.g + (long long)r * (m + 1);
-        const double* ha = g.Hacc + (long long)r * (m + 1);
-        for (int i = 0; i <= j; i++) H[i * m + j] = ha[i];
-        H[(j + 1) * m + j] = hn;
+    const long long k = blockIdx.x * (long long)GB + threadIdx.x;
+    if (k < K) g.V[(j + 1) * ldv + (long long)r * K + k] = g.Wv[r * K + k] * (1.0 / hn);
+    if (blockIdx.x != 0) return;
+    double*       H  = g.H + (long long)r * (m + 1) * m;
+    double*       cs = g.cs + (long long)r * m;
+    double*       sn = g.sn + (long long)r * m;
+    double*       gg = g.g + (long long)r * (m + 1);
+    const double* ha = g.Hacc + (long long)r * (m + 1);
+    for (int i = threadIdx.x; i <= j; i += GB) col[i] = ha[i];
+    for (int i = threadIdx.x; i < j; i += GB) {
+        scs[i] = cs[i];
+        ssn[i] = sn[i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        col[j + 1] = hn;
         for (int i = 0; i < j; i++) {
-            const double t     = cs[i] * H[i * m + j] + sn[i] * H[(i + 1) * m + j];
-            H[(i + 1) * m + j] = -sn[i] * H[i * m + j] + cs[i] * H[(i + 1) * m + j];
-            H[i * m + j]       = t;
+            const double t = scs[i] * col[i] + ssn[i] * col[i + 1];
+            col[i + 1]     = -ssn[i] * col[i] + scs[i] * col[i + 1];
+            col[i]         = t;
         }
-        const double hjj = H[j * m + j], hj1 = H[(j + 1) * m + j];
+        const double hjj = col[j], hj1 = col[j + 1];
         const double den = hypot(hjj, hj1);
-        cs[j]            = hjj / den;
-        sn[j]            = hj1 / den;
-        H[j * m + j]       = cs[j] * H[j * m + j] + sn[j] * H[(j + 1) * m + j];
-        H[(j + 1) * m + j] = 0.0;
-        const double t     = cs[j] * gg[j];
-        gg[j + 1]          = -sn[j] * gg[j];
-        gg[j]              = t;
+        const double c = hjj / den, sg = hj1 / den;
+        cs[j]          = c;
+        sn[j]          = sg;
+        col[j]         = c * hjj + sg * hj1;
+        col[j + 1]     = 0.0;
+        const double t = c * gg[j];
+        gg[j + 1]      = -sg * gg[j];
+        gg[j]          = t;
         g.state[r * 4 + 1] = j + 1;
         g.resid[r]         = fabs(gg[j + 1]);
         if (fabs(gg[j + 1]) < g.tolabs[r]) g.state[r * 4 + 0] = 0;
         g.state[r * 4 + 3] += 1;  // Arnoldi steps taken
     }
+    __syncthreads();
+    for (int i = threadIdx.x; i <= j + 1; i += GB) H[i * m + j] = col[i];
 }
 
-// back substitution for y (gmres_solver2d.cpp:228-236), one thread per system
-__global__ void k_gm_backsub(GmresDev g)
+// back substitution for y (gmres_solver2d.cpp:228-236): blocks of 32 columns
+// from the right, the 32x32 diagonal block solved in LDS, the columns above
+// updated in parallel.  One block of 256 threads per system.
+__global__ __launch_bounds__(256) void k_gm_backsub(GmresDev g)
 {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= g.nrhs) return;
+    __shared__ double y[1024];
+    __shared__ double D[32][33];
+    const int     r  = blockIdx.x;
     const int     m  = g.m;
     const int     jr = g.state[r * 4 + 1];
     const double* H  = g.H + (long long)r * (m + 1) * m;
     const double* gg = g.g + (long long)r * (m + 1);
-    double*       y  = g.Y + (long long)r * m;
-    for (int i = 0; i < m; i++) y[i] = 0.0;
-    for (int i = jr - 1; i >= 0; i--) {
-        double acc = gg[i];
-        for (int k = i + 1; k < jr; k++) acc -= H[i * m + k] * y[k];
-        y[i] = acc / H[i * m + i];
+    for (int i = threadIdx.x; i < m; i += 256) y[i] = i < jr ? gg[i] : 0.0;
+    __syncthreads();
+    for (int c1 = jr; c1 > 0; c1 -= 32) {
+        const int c0 = c1 > 32 ? c1 - 32 : 0;
+        const int nb = c1 - c0;
+        for (int q = threadIdx.x; q < nb * nb; q += 256) {
+            const int a = q / nb, b = q % nb;
+            D[a][b] = H[(long long)(c0 + a) * m + c0 + b];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int a = nb - 1; a >= 0; a--) {
+                double acc = y[c0 + a];
+                for (int b = a + 1; b < nb; b++) acc -= D[a][b] * y[c0 + b];
+                y[c0 + a] = acc / D[a][a];
+            }
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < c0; t += 256) {
+            double acc = 0.0;
+            for (int b = 0; b < nb; b++) acc += H[(long long)t * m + c0 + b] * y[c0 + b];
+            y[t] -= acc;
+        }
+        __syncthreads();
     }
+    double* yo = g.Y + (long long)r * m;
+    for (int i = threadIdx.x; i < m; i += 256) yo[i] = y[i];
 }
 
 // x += V(:, 0:jr) y (gmres_solver2d.cpp:238-241)
-__global__ __launch_bounds__(VB) void k_gm_update_x(GmresDev g)
+__global__ __launch_bounds__(GB) void k_gm_update_x(GmresDev g)
 {
-    const int       r   = blockIdx.y;
-    const int       jr  = g.state[r * 4 + 1];
+    __shared__ double ys[1024];
+    const int r  = blockIdx.y;
+    const int jr = g.state[r * 4 + 1];
+    for (int i = threadIdx.x; i < jr; i += GB) ys[i] = g.Y[(long long)r * g.m + i];
+    __syncthreads();
     const long long K   = g.K;
     const long long ldv = (long long)g.nrhs * K;
-    const double*   y   = g.Y + (long long)r * g.m;
-    for (long long k = blockIdx.x * (long long)VB + threadIdx.x; k < K; k += (long long)gridDim.x * VB) {
-        double x = g.X[r * K + k];
-        for (int i = 0; i < jr; i++) x += y[i] * g.V[i * ldv + (long long)r * K + k];
-        g.X[r * K + k] = x;
-    }
+    const long long k   = blockIdx.x * (long long)GB + threadIdx.x;
+    if (k >= K) return;
+    double        x  = g.X[r * K + k];
+    const double* vk = g.V + (long long)r * K + k;
+    for (int i = 0; i < jr; i++) x += ys[i] * vk[i * ldv];
+    g.X[r * K + k] = x;
 }
 
-// tolabs[r] = tol * ||C_r|| (relative) or tol (absolute)
-__global__ __launch_bounds__(VB) void k_gm_norm_part(GmresDev g)
-{
-    __shared__ double sh[VB];
-    const int         r   = blockIdx.y;
-    const long long   K   = g.K;
-    double            acc = 0.0;
-    for (long long k = blockIdx.x * (long long)VB + threadIdx.x; k < K; k += (long long)gridDim.x * VB)
-        acc += g.C[r * K + k] * g.C[r * K + k];
-    acc = block_reduce(acc, sh);
-    if (threadIdx.x == 0) g.part[(long long)r * g.nblk + blockIdx.x] = acc;
-}
-__global__ __launch_bounds__(VB) void k_gm_set_tol(GmresDev g, double tol, int rel)
-{
-    __shared__ double sh[VB];
-    const int         r  = blockIdx.x;
-    const double      nc = sqrt(block_sum_fixed(g.part + (long long)r * g.nblk, g.nblk, sh));
-    if (threadIdx.x == 0) {
-        g.tolabs[r]        = rel ? tol * nc : tol;
-        g.resid[r]         = nc;
-        g.state[r * 4 + 0] = 0;
-        g.state[r * 4 + 1] = 0;
-        g.state[r * 4 + 2] = 0;
-        g.state[r * 4 + 3] = 0;
-    }
-}
 void gm_init(const GmresDev& g, double tol, int rel, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_gm_norm_part, dim3(g.nblk, g.nrhs), dim3(VB), 0, s, g);
+    hipLaunchKernelGGL(k_gm_norm_part, dim3(g.nblk, g.nrhs), dim3(GB), 0, s, g);
     FP_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_gm_set_tol, dim3(g.nrhs), dim3(VB), 0, s, g, tol, rel);
+    hipLaunchKernelGGL(k_gm_set_tol, dim3(g.nrhs), dim3(GB), 0, s, g, tol, rel);
     FP_HIP(hipGetLastError());
 }
 
 void gm_resid(const GmresDev& g, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_gm_resid, dim3(g.nblk, g.nrhs), dim3(VB), 0, s, g);
+    hipLaunchKernelGGL(k_gm_resid, dim3(g.nblk, g.nrhs), dim3(GB), 0, s, g);
     FP_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_gm_cycle_start, dim3(g.nblk, g.nrhs), dim3(VB), 0, s, g);
+    hipLaunchKernelGGL(k_gm_cycle_start, dim3(g.nblk, g.nrhs), dim3(GB), 0, s, g);
     FP_HIP(hipGetLastError());
 }
+
 void gm_arnoldi(const GmresDev& g, int j, hipStream_t s)
 {
     for (int pass = 0; pass < 2; pass++) {
-        hipLaunchKernelGGL(k_gm_dots, dim3(g.nblk, g.nrhs), dim3(VB), 0, s, g, j);
+        hipLaunchKernelGGL(k_gm_dots, dim3(j + 1, g.nrhs), dim3(256), 0, s, g, j);
         FP_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_gm_update, dim3(g.nblk, g.nrhs), dim3(VB), 0, s, g, j, pass);
+        hipLaunchKernelGGL(k_gm_update, dim3(g.nblk, g.nrhs), dim3(GB), 0, s, g, j, pass);
         FP_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_gm_finish, dim3(g.nblk, g.nrhs), dim3(VB), 0, s, g, j);
+    hipLaunchKernelGGL(k_gm_finish, dim3(g.nblk, g.nrhs), dim3(GB), 0, s, g, j);
     FP_HIP(hipGetLastError());
 }
+
 void gm_finish_cycle(const GmresDev& g, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_gm_backsub, dim3((g.nrhs + 63) / 64), dim3(64), 0, s, g);
+    hipLaunchKernelGGL(k_gm_backsub, dim3(g.nrhs), dim3(256), 0, s, g);
     FP_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_gm_update_x, dim3(g.nblk, g.nrhs), dim3(VB), 0, s, g);
+    hipLaunchKernelGGL(k_gm_update_x, dim3(g.nblk, g.nrhs), dim3(GB), 0, s, g);
     FP_HIP(hipGetLastError());
 }
 
