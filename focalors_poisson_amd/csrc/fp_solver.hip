@@ -118,6 +118,7 @@ template void sub_solve<float>(SubPlan&, float*, long long, int, hipStream_t);
 struct ChildLink {
     int            dom = -1, loc = 0, cn = 0, off = 0;
     DevBuf<double> S;      // cn x cn, value(i, j) row-major
+    DevBuf<double> St;     // x-line child: Phi S Phi^T in the parent's y basis
     DevBuf<int>    ccell;  // child's adjacent line cells
 };
 
@@ -133,7 +134,8 @@ struct Node {
     DevBuf<double> V, Wv, Tv, X, C, H, Hacc, cs, sn, g, Y, part, part2, tolabs, resid, G;
     // structured interface operator R A^-1 E of this node's subdomain
     int            face_off[4] = {-1, -1, -1, -1};
-    int            has_x = 0, has_y = 0, nwg = 0;
+    int            has_x = 0, has_y = 0, nwg = 0, nxl = 0;  // nxl: x-line segments (first in the vector)
+    bool           spectral_x = false;  // x-line unknowns held as y-transform coefficients
     long long      ldl = 0;
     DevBuf<double> phi, LB, XU, CP;
     DevBuf<int>    kface, kpos;
@@ -261,7 +263,7 @@ static void matvec_lines(fp_cat_s* c, Node& nd, const double* v, double* out, in
     int            rows = 0;
     for (auto& L : nd.ch) {
         SchurChild& sc = ga.ch[ga.nch++];
-        sc.S    = L.S.p;
+        sc.S    = (nd.spectral_x && L.St.p) ? L.St.p : L.S.p;
         sc.cn   = L.cn;
         sc.voff = L.off;
         sc.lboff = L.loc == XNEG ? 0 : L.loc == XPOS ? ny : L.loc == YNEG ? 2 * ny : 2 * ny + nx;
@@ -274,7 +276,7 @@ static void matvec_lines(fp_cat_s* c, Node& nd, const double* v, double* out, in
     ga.lb         = nd.LB.p;
     ga.ldl        = nd.ldl;
     launch_schur_lines(ga, nrhs, s);
-    if (nd.has_x) launch_ytransform<double>(nd.plan.t64, nd.LB.p, nd.ldl, 2, nrhs, 0, s);
+    if (nd.has_x && !nd.spectral_x) launch_ytransform<double>(nd.plan.t64, nd.LB.p, nd.ldl, 2, nrhs, 0, s);
     LinesArgs<double> la{};
     la.lines   = nd.LB.p;
     la.ldl     = nd.ldl;
@@ -287,7 +289,7 @@ static void matvec_lines(fp_cat_s* c, Node& nd, const double* v, double* out, in
     la.xrows   = nd.has_x;
     la.ycols   = nd.has_y;
     launch_tridiag_lines<double>(nd.plan, la, nrhs, s);
-    if (nd.has_x) launch_ytransform<double>(nd.plan.t64, nd.XU.p, 2LL * ny, 2, nrhs, 1, s);
+    if (nd.has_x && !nd.spectral_x) launch_ytransform<double>(nd.plan.t64, nd.XU.p, 2LL * ny, 2, nrhs, 1, s);
     launch_lines_assemble(v, out, nd.K, nd.kface.p, nd.kpos.p, nd.XU.p, ny, nd.CP.p, nd.nwg, nx, nrhs, s);
 }
 
@@ -331,6 +333,8 @@ static void gmres_solve(fp_cat_s* c, Node& nd, double* b, int nrhs, hipStream_t 
     const int       K = nd.K, m = c->m;
     sub_solve<double>(nd.plan, b, n, nrhs, s);
     launch_gather(b, n, nd.cell.p, K, nullptr, nd.C.p, K, 0.0, 1.0, nrhs, s);
+    // x-line unknowns to the root's y basis (orthogonal: GMRES norms unchanged)
+    if (nd.spectral_x) launch_ytransform<double>(nd.plan.t64, nd.C.p, K, nd.nxl, nrhs, 0, s);
     FP_HIP(hipMemcpyAsync(nd.X.p, nd.C.p, sizeof(double) * K * nrhs, hipMemcpyDeviceToDevice, s));
     const GmresDev g = gdev(c, nd, nrhs);
     gm_init(g, c->tol, c->tol_rel, s);
@@ -370,6 +374,7 @@ static void gmres_solve(fp_cat_s* c, Node& nd, double* b, int nrhs, hipStream_t 
         if (nd.h_state[4 * r + 2] == 2) status = FP_E_NUMERIC;
         else if (nd.h_state[4 * r + 2] != 1 && status == 0) status = FP_E_NOT_CONVERGED;
     }
+    if (nd.spectral_x) launch_ytransform<double>(nd.plan.t64, nd.X.p, K, nd.nxl, nrhs, 1, s);
     // u = z0 + A^-1 E S y
     for (auto& L : nd.ch) launch_schur_gemv(L.S.p, L.cn, nd.X.p, K, L.off, nd.Tv.p, K, L.off, nrhs, s);
     launch_fill<double>(nd.G.p, n * nrhs, 0.0, s);
@@ -479,7 +484,24 @@ static void build_node(fp_cat_s* c, int d, hipStream_t s)
         nd->uk2.alloc(k2.size()); nd->uk2.upload(k2.data(), k2.size());
     }
     c->nodes[d] = std::move(nd);
-    for (auto& L : c->nodes[d]->ch) build_schur(c, *c->nodes[d], L, s);
+    Node& N = *c->nodes[d];
+    for (auto& L : N.ch) build_schur(c, N, L, s);
+    N.spectral_x = N.has_x && !full_field_matvec();
+    for (auto& L : N.ch) {
+        if (L.loc != XNEG && L.loc != XPOS) continue;
+        N.nxl++;
+        if (!N.spectral_x) continue;
+        // St = Phi S Phi^T with the row transform R(M) = M Phi^T:
+        // R((R(S^T))^T) = (S^T Phi^T)^T Phi^T = Phi S Phi^T
+        const int      cn = L.cn;
+        DevBuf<double> t1((size_t)cn * cn);
+        L.St.alloc((size_t)cn * cn);
+        launch_transpose(L.S.p, t1.p, cn, s);
+        launch_ytransform<double>(N.plan.t64, t1.p, cn, cn, 1, 0, s);
+        launch_transpose(t1.p, L.St.p, cn, s);
+        launch_ytransform<double>(N.plan.t64, L.St.p, cn, cn, 1, 0, s);
+        FP_HIP(hipStreamSynchronize(s));
+    }
 }
 
 // TreeUtils::findOptimalRoot / buildTreeMapFromRoot / BFS

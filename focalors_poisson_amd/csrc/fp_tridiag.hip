@@ -22,6 +22,54 @@
 
 namespace fp {
 
+// Start values of the CH-row chunks of each mode: the chunk maps
+// v -> SA v + SB (LDS, chunk-major) are chained in application order and the
+// exclusive prefix, applied to 0, is written to SY.  backward = 1 chains the
+// chunks from the last row upwards (back substitution); mode sing_j always
+// chains forwards (the two prefix sums of the singular recurrence).
+// With P <= 64 one wavefront scans one mode with a log-depth affine scan;
+// longer chains fall back to one thread per mode.
+template <typename T>
+__device__ void chain_chunks(const T* SA, const T* SB, T* SY, int P, int MB, int j0, int ny, int backward,
+                             int sing_j)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+    if (P <= 64) {
+        for (int jm = wave; jm < MB; jm += nwave) {
+            const int j = j0 + jm;
+            if (j >= ny) continue;  // uniform over the wavefront
+            const bool bw = backward && j != sing_j;
+            const int  c  = bw ? P - 1 - lane : lane;
+            T          A = T(1), B = T(0);
+            if (lane < P) {
+                A = SA[c * MB + jm];
+                B = SB[c * MB + jm];
+            }
+            for (int d = 1; d < 64; d <<= 1) {
+                const T Ap = __shfl_up(A, d, 64), Bp = __shfl_up(B, d, 64);
+                if (lane >= d) {
+                    B = A * Bp + B;
+                    A = A * Ap;
+                }
+            }
+            T ex = __shfl_up(B, 1, 64);
+            if (lane == 0) ex = T(0);
+            if (lane < P) SY[c * MB + jm] = ex;
+        }
+    } else {
+        const int jm = threadIdx.x;
+        if (jm < MB && j0 + jm < ny) {
+            const bool bw = backward && (j0 + jm) != sing_j;
+            T          v  = T(0);
+            for (int q = 0; q < P; q++) {
+                const int c = bw ? P - 1 - q : q;
+                SY[c * MB + jm] = v;
+                v               = SA[c * MB + jm] * v + SB[c * MB + jm];
+            }
+        }
+    }
+}
+
 template <typename T, int CH>
 __global__ __launch_bounds__(1024) void k_tridiag(T* __restrict__ data, long long ld, int nx, int ny,
                                                   const T* __restrict__ pinv, int P, int MB, int singular_last)
@@ -48,6 +96,7 @@ __global__ __launch_bounds__(1024) void k_tridiag(T* __restrict__ data, long lon
         pv[k]         = ok ? pinv[(long long)i * ny + j] : T(0);
     }
     const bool sing = singular_last && (j == ny - 1);
+    const int  SINGJ = singular_last ? ny - 1 : -1;
 
     // forward: chunk map y_out = A y_in + B
     T A = T(1), B = T(0);
@@ -68,13 +117,7 @@ __global__ __launch_bounds__(1024) void k_tridiag(T* __restrict__ data, long lon
         SB[c * MB + jl] = B;
     }
     __syncthreads();
-    if (c == 0 && j < ny) {
-        T y = T(0);
-        for (int q = 0; q < P; q++) {
-            SY[q * MB + jl] = y;
-            y               = SA[q * MB + jl] * y + SB[q * MB + jl];
-        }
-    }
+    chain_chunks(SA, SB, SY, P, MB, blockIdx.x * MB, ny, 0, SINGJ);
     __syncthreads();
     T y = (c < P) ? SY[c * MB + jl] : T(0);
     if (!sing) {
@@ -114,20 +157,7 @@ __global__ __launch_bounds__(1024) void k_tridiag(T* __restrict__ data, long lon
         SB[c * MB + jl] = B;
     }
     __syncthreads();
-    if (c == 0 && j < ny) {
-        T x = T(0);
-        if (!sing) {
-            for (int q = P - 1; q >= 0; q--) {
-                SY[q * MB + jl] = x;
-                x               = SA[q * MB + jl] * x + SB[q * MB + jl];
-            }
-        } else {
-            for (int q = 0; q < P; q++) {  // x at chunk start = sum of earlier d
-                SY[q * MB + jl] = x;
-                x += SB[q * MB + jl];
-            }
-        }
-    }
+    chain_chunks(SA, SB, SY, P, MB, blockIdx.x * MB, ny, 1, SINGJ);
     __syncthreads();
     T x = (c < P) ? SY[c * MB + jl] : T(0);
     if (!sing) {
@@ -187,6 +217,7 @@ __global__ __launch_bounds__(1024) void k_tridiag_lines(LinesArgs<T> a, int P, i
     const T    p0  = act ? a.phi[j] : T(0);
     const T    p1  = act ? a.phi[ny + j] : T(0);
     const int  i0  = c * CH;
+    const int  SINGJ = -1;
 
     T fv[CH], pv[CH];
 #pragma unroll
@@ -214,13 +245,7 @@ __global__ __launch_bounds__(1024) void k_tridiag_lines(LinesArgs<T> a, int P, i
         SB[c * MB + jl] = B;
     }
     __syncthreads();
-    if (c == 0 && j < ny) {
-        T y = T(0);
-        for (int q = 0; q < P; q++) {
-            SY[q * MB + jl] = y;
-            y               = SA[q * MB + jl] * y + SB[q * MB + jl];
-        }
-    }
+    chain_chunks(SA, SB, SY, P, MB, blockIdx.x * MB, ny, 0, SINGJ);
     __syncthreads();
     T y = (c < P) ? SY[c * MB + jl] : T(0);
 #pragma unroll
@@ -243,13 +268,7 @@ __global__ __launch_bounds__(1024) void k_tridiag_lines(LinesArgs<T> a, int P, i
         SB[c * MB + jl] = B;
     }
     __syncthreads();
-    if (c == 0 && j < ny) {
-        T x = T(0);
-        for (int q = P - 1; q >= 0; q--) {
-            SY[q * MB + jl] = x;
-            x               = SA[q * MB + jl] * x + SB[q * MB + jl];
-        }
-    }
+    chain_chunks(SA, SB, SY, P, MB, blockIdx.x * MB, ny, 1, -1);
     __syncthreads();
     T x = (c < P) ? SY[c * MB + jl] : T(0);
 #pragma unroll
@@ -292,7 +311,7 @@ static void launch_tridiag_ch(const SubPlan& sp, const T* pinv, T* data, long lo
     if (MB * P > 1024) throw Error(FP_E_INVALID, "nx too large for the tridiagonal kernel");
     const size_t lds = 3 * (size_t)P * MB * sizeof(T);
     dim3         grid((sp.ny + MB - 1) / MB, nrhs);
-    hipLaunchKernelGGL((k_tridiag<T, CH>), grid, dim3(MB * P), lds, s, data, ld, sp.nx, sp.ny, pinv, P, MB,
+    hipLaunchKernelGGL((k_tridiag<T, CH>), grid, dim3((MB * P + 63) / 64 * 64), lds, s, data, ld, sp.nx, sp.ny, pinv, P, MB,
                        sp.no_dirichlet ? 1 : 0);
     FP_HIP(hipGetLastError());
 }
@@ -315,7 +334,7 @@ static int launch_lines_ch(const SubPlan& sp, const LinesArgs<T>& a, int nrhs, h
     if (MB * P > 1024) throw Error(FP_E_INVALID, "nx too large for the tridiagonal kernel");
     const size_t lds = 3 * (size_t)P * MB * sizeof(T);
     dim3         grid((sp.ny + MB - 1) / MB, nrhs);
-    hipLaunchKernelGGL((k_tridiag_lines<T, CH>), grid, dim3(MB * P), lds, s, a, P, MB);
+    hipLaunchKernelGGL((k_tridiag_lines<T, CH>), grid, dim3((MB * P + 63) / 64 * 64), lds, s, a, P, MB);
     FP_HIP(hipGetLastError());
     return grid.x;
 }

@@ -42,6 +42,7 @@ struct SchurLinesArgs {
 void launch_schur_lines(const SchurLinesArgs& a, int nrhs, hipStream_t s);
 void launch_lines_assemble(const double* v, double* out, int K, const int* kface, const int* kpos, const double* xu,
                            int ny, const double* colpart, int nwg, int nx, int nrhs, hipStream_t s);
+void launch_transpose(const double* in, double* out, int n, hipStream_t s);
 void launch_schur_collect(const double* buf, long long ldb, const int* cell, int cn, int i0, int cnt, double* S,
                           hipStream_t s);
 void launch_set_units(double* buf, long long ldb, const int* cell, int i0, int cnt, hipStream_t s);

@@ -209,16 +209,45 @@ __global__ void k_lines_assemble(const double* v, double* out, int K, const int*
     else if (f == XPOS) t = xu[r * 2 * ny + ny + p];
     else {
         const double* cp = colpart + r * (long long)nwg * 2 * nx + (f == YNEG ? 0 : nx) + p;
-        t                = 0.0;
-        for (int w = 0; w < nwg; w++) t += cp[(long long)w * 2 * nx];
+        double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
+        int    w  = 0;
+        for (; w + 4 <= nwg; w += 4) {
+            t0 += cp[(long long)w * 2 * nx];
+            t1 += cp[(long long)(w + 1) * 2 * nx];
+            t2 += cp[(long long)(w + 2) * 2 * nx];
+            t3 += cp[(long long)(w + 3) * 2 * nx];
+        }
+        for (; w < nwg; w++) t0 += cp[(long long)w * 2 * nx];
+        t = (t0 + t1) + (t2 + t3);
     }
     out[r * K + k] = v[r * K + k] - t;
 }
 void launch_lines_assemble(const double* v, double* out, int K, const int* kface, const int* kpos, const double* xu,
                            int ny, const double* colpart, int nwg, int nx, int nrhs, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_lines_assemble, dim3((K + 255) / 256, nrhs), dim3(256), 0, s, v, out, K, kface, kpos, xu,
+    hipLaunchKernelGGL(k_lines_assemble, dim3((K + 63) / 64, nrhs), dim3(64), 0, s, v, out, K, kface, kpos, xu,
                        ny, colpart, nwg, nx);
+    FP_HIP(hipGetLastError());
+}
+
+// out = in^T for an n x n matrix (32x32 tiles through LDS)
+__global__ void k_transpose(const double* in, double* out, int n)
+{
+    __shared__ double t[32][33];
+    const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+    for (int q = threadIdx.y; q < 32; q += blockDim.y) {
+        const int r = by + q, c = bx + threadIdx.x;
+        if (r < n && c < n) t[q][threadIdx.x] = in[(long long)r * n + c];
+    }
+    __syncthreads();
+    for (int q = threadIdx.y; q < 32; q += blockDim.y) {
+        const int r = bx + q, c = by + threadIdx.x;
+        if (r < n && c < n) out[(long long)r * n + c] = t[threadIdx.x][q];
+    }
+}
+void launch_transpose(const double* in, double* out, int n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_transpose, dim3((n + 31) / 32, (n + 31) / 32), dim3(32, 8), 0, s, in, out, n);
     FP_HIP(hipGetLastError());
 }
 
