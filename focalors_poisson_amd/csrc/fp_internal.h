@@ -98,8 +98,10 @@ struct SubPlan {
     std::vector<double>   x_diag;   // host copy, reference order
     TransformPlan<double> t64;
     TransformPlan<float>  t32;
-    DevBuf<double>        pinv64;   // nx*ny: reciprocal Thomas pivots per (row, mode)
+    DevBuf<double>        pinv64;   // ny*ldp, mode-major: reciprocal Thomas pivots (row i of mode j at j*ldp+i)
     DevBuf<float>         pinv32;
+    int                   ldp = 0;  // nx rounded up to 32, zero padded
+    int                   mb_field = 16, mb_lines = 4;  // modes per workgroup of the two sweeps
     int                   dev = 0;
     void build(int nx, int ny, double hx, double hy, const int bc[4]);
     void ensure_f32();
@@ -121,7 +123,8 @@ struct LinesArgs {
     const T*  lines;    // per rhs: Xh0 (ny) | Xh1 (ny) | a (nx) | b (nx)
     long long ldl;      // 2 ny + 2 nx
     const T*  phi;      // phi0 (ny) | phi1 (ny)
-    const T*  pinv;     // nx*ny reciprocal pivots
+    const T*  pinv;     // mode-major reciprocal pivots, ldp per mode
+    int       ldp;
     T*        xu;       // per rhs: coefficient rows 0 and nx-1 (2 ny)
     T*        colpart;  // per rhs, per workgroup: 2 x nx column partial sums
     int       nx, ny;

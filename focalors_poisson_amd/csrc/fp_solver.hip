@@ -58,18 +58,20 @@ void SubPlan::build(int nx_, int ny_, double hx_, double hy_, const int bc_[4])
     }
     // reciprocal Thomas pivots (the reference's c vector,
     // chasing_method_base.cpp:33-46), one column per mode
-    std::vector<double> p((size_t)nx * ny, 0.0);
+    ldp = (nx + 31) / 32 * 32;
+    std::vector<double> p((size_t)ldp * ny, 0.0);
     const bool          ns = bc[XNEG] == NEUMANN, ne = bc[XPOS] == NEUMANN;
     for (int j = 0; j < ny; j++) {
         if (no_dirichlet && j == ny - 1) continue;  // singular mode: recurrence, no pivots
-        const double d = x_diag[j];
-        double       c = ns ? 1.0 / (d + 1.0) : 1.0 / d;
-        p[j]           = c;
+        const double d  = x_diag[j];
+        double*      pj = p.data() + (size_t)j * ldp;
+        double       c  = ns ? 1.0 / (d + 1.0) : 1.0 / d;
+        pj[0]           = c;
         for (int i = 1; i < nx - 1; i++) {
-            c                     = 1.0 / (d - c);
-            p[(size_t)i * ny + j] = c;
+            c     = 1.0 / (d - c);
+            pj[i] = c;
         }
-        p[(size_t)(nx - 1) * ny + j] = 1.0 / ((ne ? d + 1.0 : d) - c);
+        pj[nx - 1] = 1.0 / ((ne ? d + 1.0 : d) - c);
     }
     pinv64.alloc(p.size());
     pinv64.upload(p.data(), p.size());
@@ -79,7 +81,7 @@ void SubPlan::build(int nx_, int ny_, double hx_, double hy_, const int bc_[4])
 void SubPlan::ensure_f32()
 {
     if (pinv32.p) return;
-    std::vector<double> p((size_t)nx * ny);
+    std::vector<double> p((size_t)ldp * ny);
     FP_HIP(hipMemcpy(p.data(), pinv64.p, p.size() * sizeof(double), hipMemcpyDeviceToHost));
     std::vector<float> q(p.begin(), p.end());
     pinv32.alloc(q.size());
@@ -136,6 +138,8 @@ struct Node {
     int            face_off[4] = {-1, -1, -1, -1};
     int            has_x = 0, has_y = 0, nwg = 0, nxl = 0;  // nxl: x-line segments (first in the vector)
     bool           spectral_x = false;  // x-line unknowns held as y-transform coefficients
+    bool           precond    = false;  // right preconditioner: explicit inverse of the interface operator
+    DevBuf<double> Minv, Zv;
     long long      ldl = 0;
     DevBuf<double> phi, LB, XU, CP;
     DevBuf<int>    kface, kpos;
@@ -173,6 +177,7 @@ struct fp_cat_s {
     int                                    m = 20, tol_rel = 0, maxiter = 100, precision = FP_F64;
     double                                 tol = 1e-8;
     int                                    check_every = 4;
+    int                                    precond_kind = 1;  // 0: none, 1: dense inverse of the interface operator
     // statistics of the last solve
     int    last_steps = 0, last_restarts = 0, last_status = 0;
     double last_res = 0.0, last_rel = 0.0;
@@ -233,6 +238,7 @@ static void ensure_ws(fp_cat_s* c, Node& nd, int nrhs)
     nd.LB.alloc((size_t)nrhs * nd.ldl);
     launch_fill<double>(nd.LB.p, (long long)nrhs * nd.ldl, 0.0, nullptr);
     nd.XU.alloc((size_t)nrhs * 2 * c->ny[nd.dom]);
+    nd.Zv.alloc((size_t)nrhs * K);
     nd.CP.alloc((size_t)nrhs * nd.nwg * 2 * c->nx[nd.dom]);
     FP_HIP(hipDeviceSynchronize());
     if (nd.h_state) (void)hipHostFree(nd.h_state);
@@ -256,7 +262,7 @@ static GmresDev gdev(fp_cat_s* c, Node& nd, int nrhs)
 // out = v - R A^-1 E S v  (GMRESSolver2D::Afun on interface vectors), with
 // R A^-1 E applied through the line-sourced sweep: no full root field is
 // formed, transformed or read.
-static void matvec_lines(fp_cat_s* c, Node& nd, const double* v, double* out, int nrhs, hipStream_t s)
+static SchurLinesArgs schur_lines_args(fp_cat_s* c, Node& nd, const double* v)
 {
     const int      nx = c->nx[nd.dom], ny = c->ny[nd.dom];
     SchurLinesArgs ga{};
@@ -275,13 +281,22 @@ static void matvec_lines(fp_cat_s* c, Node& nd, const double* v, double* out, in
     ga.ldv        = nd.K;
     ga.lb         = nd.LB.p;
     ga.ldl        = nd.ldl;
-    launch_schur_lines(ga, nrhs, s);
+    return ga;
+}
+
+// R A^-1 E on the line buffer LB (filled by the caller), then out = v - that
+// (v == nullptr: unit vectors e_{k0+r})
+static void lines_solve_assemble(fp_cat_s* c, Node& nd, const double* v, double* out, int nrhs, hipStream_t s,
+                                 int k0 = 0)
+{
+    const int nx = c->nx[nd.dom], ny = c->ny[nd.dom];
     if (nd.has_x && !nd.spectral_x) launch_ytransform<double>(nd.plan.t64, nd.LB.p, nd.ldl, 2, nrhs, 0, s);
     LinesArgs<double> la{};
     la.lines   = nd.LB.p;
     la.ldl     = nd.ldl;
     la.phi     = nd.phi.p;
     la.pinv    = nd.plan.pinv64.p;
+    la.ldp     = nd.plan.ldp;
     la.xu      = nd.XU.p;
     la.colpart = nd.CP.p;
     la.nx      = nx;
@@ -290,7 +305,13 @@ static void matvec_lines(fp_cat_s* c, Node& nd, const double* v, double* out, in
     la.ycols   = nd.has_y;
     launch_tridiag_lines<double>(nd.plan, la, nrhs, s);
     if (nd.has_x && !nd.spectral_x) launch_ytransform<double>(nd.plan.t64, nd.XU.p, 2LL * ny, 2, nrhs, 1, s);
-    launch_lines_assemble(v, out, nd.K, nd.kface.p, nd.kpos.p, nd.XU.p, ny, nd.CP.p, nd.nwg, nx, nrhs, s);
+    launch_lines_assemble(v, out, nd.K, nd.kface.p, nd.kpos.p, nd.XU.p, ny, nd.CP.p, nd.nwg, nx, nrhs, s, k0);
+}
+
+static void matvec_lines(fp_cat_s* c, Node& nd, const double* v, double* out, int nrhs, hipStream_t s)
+{
+    launch_schur_lines(schur_lines_args(c, nd, v), nrhs, s);
+    lines_solve_assemble(c, nd, v, out, nrhs, s);
 }
 
 static bool full_field_matvec()
@@ -314,6 +335,42 @@ static void matvec(fp_cat_s* c, Node& nd, const double* v, double* out, int nrhs
     launch_scatter(nd.G.p, n, nd.ucell.p, nd.uk1.p, nd.uk2.p, nd.nu, nd.Tv.p, nd.K, nrhs, s);
     sub_solve<double>(nd.plan, nd.G.p, n, nrhs, s);
     launch_gather(nd.G.p, n, nd.cell.p, nd.K, v, out, nd.K, 1.0, -1.0, nrhs, s);
+}
+
+// w = B M v with M the explicit inverse of the interface operator B
+static void op(fp_cat_s* c, Node& nd, const double* v, double* out, int nrhs, hipStream_t s)
+{
+    if (!nd.precond) {
+        matvec(c, nd, v, out, nrhs, s);
+        return;
+    }
+    launch_dense_gemv(nd.Minv.p, nd.K, v, nd.Zv.p, 1.0, 0.0, nrhs, s);
+    matvec(c, nd, nd.Zv.p, out, nrhs, s);
+}
+
+static void ensure_ws(fp_cat_s* c, Node& nd, int nrhs);
+
+// B = I - R A^-1 E S column by column (unit interface vectors through the
+// line-sourced sweep, 256 at a time), then inverted in place on the device.
+static void build_interface_inverse(fp_cat_s* c, Node& nd, hipStream_t s)
+{
+    const int K = nd.K;
+    if (!nd.spectral_x || K == 0) return;
+    const int chunk = std::min(K, 256);
+    ensure_ws(c, nd, chunk);
+    DevBuf<double> Bt((size_t)K * K);
+    SchurLinesArgs ga = schur_lines_args(c, nd, nullptr);
+    for (int k0 = 0; k0 < K; k0 += chunk) {
+        const int cnt = std::min(chunk, K - k0);
+        launch_fill<double>(nd.LB.p, (long long)cnt * nd.ldl, 0.0, s);
+        launch_lines_from_units(ga, k0, cnt, s);
+        lines_solve_assemble(c, nd, nullptr, Bt.p + (long long)k0 * K, cnt, s, k0);
+    }
+    launch_fill<double>(nd.LB.p, (long long)nd.ws_nrhs * nd.ldl, 0.0, s);
+    nd.Minv.alloc((size_t)K * K);
+    launch_transpose(Bt.p, nd.Minv.p, K, s);
+    gauss_jordan_invert(nd.Minv.p, K, s);
+    nd.precond = true;
 }
 
 static void read_state(Node& nd, int nrhs, hipStream_t s)
@@ -351,7 +408,7 @@ static void gmres_solve(fp_cat_s* c, Node& nd, double* b, int nrhs, hipStream_t 
         if (all_done) break;
         const long long ldv = (long long)nrhs * K;
         for (int j = 0; j < m; j++) {
-            matvec(c, nd, nd.V.p + j * ldv, nd.Wv.p, nrhs, s);
+            op(c, nd, nd.V.p + j * ldv, nd.Wv.p, nrhs, s);
             gm_arnoldi(g, j, s);
             if ((j + 1) % c->check_every == 0 || j == m - 1) {
                 read_state(nd, nrhs, s);
@@ -361,7 +418,15 @@ static void gmres_solve(fp_cat_s* c, Node& nd, double* b, int nrhs, hipStream_t 
                 if (!any) break;
             }
         }
-        gm_finish_cycle(g, s);
+        if (!nd.precond) gm_finish_cycle(g, s);
+        else {
+            // x += M (V y)
+            launch_fill<double>(nd.Zv.p, (long long)K * nrhs, 0.0, s);
+            GmresDev gz = g;
+            gz.X        = nd.Zv.p;
+            gm_finish_cycle(gz, s);
+            launch_dense_gemv(nd.Minv.p, K, nd.Zv.p, nd.X.p, 1.0, 1.0, nrhs, s);
+        }
         restarts++;
     }
     read_state(nd, nrhs, s);
@@ -502,6 +567,7 @@ static void build_node(fp_cat_s* c, int d, hipStream_t s)
         launch_ytransform<double>(N.plan.t64, L.St.p, cn, cn, 1, 0, s);
         FP_HIP(hipStreamSynchronize(s));
     }
+    if (c->precond_kind == 1 && N.K > 0 && N.K <= 16384) build_interface_inverse(c, N, s);
 }
 
 // TreeUtils::findOptimalRoot / buildTreeMapFromRoot / BFS
@@ -805,6 +871,7 @@ int fp_cat_create(fp_ctx ctx, const fp_geometry_desc* g, int m, double tol, int 
             c->bc[4 * a + dir]        = fp::ADJACENTED;
             c->bc[4 * b + (dir ^ 1)]  = fp::ADJACENTED;
         }
+        if (const char* e = std::getenv("FP_INTERFACE_PRECOND")) c->precond_kind = std::atoi(e);
         fp::build_tree(c.get());
         c->nodes.resize(nd);
         // construct_solver_map: deepest level first, root last (concat_solver2d.cpp:88-99)

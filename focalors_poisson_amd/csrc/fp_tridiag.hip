@@ -72,7 +72,8 @@ __device__ void chain_chunks(const T* SA, const T* SB, T* SY, int P, int MB, int
 
 template <typename T, int CH>
 __global__ __launch_bounds__(1024) void k_tridiag(T* __restrict__ data, long long ld, int nx, int ny,
-                                                  const T* __restrict__ pinv, int P, int MB, int singular_last)
+                                                  const T* __restrict__ pinv, int ldp, int P, int MB,
+                                                  int singular_last)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* SA = reinterpret_cast<T*>(smem_raw);
@@ -88,12 +89,15 @@ __global__ __launch_bounds__(1024) void k_tridiag(T* __restrict__ data, long lon
     const int  i0  = c * CH;
 
     T fv[CH], pv[CH];
+    // pivots are mode-major and zero-padded to ldp rows: one contiguous,
+    // aligned run of CH values per thread
+    const T* pp = reinterpret_cast<const T*>(__builtin_assume_aligned(pinv + (long long)(act ? j : 0) * ldp + i0, 16));
 #pragma unroll
     for (int k = 0; k < CH; k++) {
         const int  i  = i0 + k;
         const bool ok = act && i < nx;
         fv[k]         = ok ? f[(long long)i * ny + j] : T(0);
-        pv[k]         = ok ? pinv[(long long)i * ny + j] : T(0);
+        pv[k]         = act ? pp[k] : T(0);
     }
     const bool sing = singular_last && (j == ny - 1);
     const int  SINGJ = singular_last ? ny - 1 : -1;
@@ -231,7 +235,12 @@ __global__ __launch_bounds__(1024) void k_tridiag_lines(LinesArgs<T> a, int P, i
             if (i == nx - 1) v += xh1[j];
         }
         fv[k] = v;
-        pv[k] = ok ? a.pinv[(long long)i * ny + j] : T(0);
+    }
+    {
+        const T* pp = reinterpret_cast<const T*>(
+            __builtin_assume_aligned(a.pinv + (long long)(act ? j : 0) * a.ldp + i0, 16));
+#pragma unroll
+        for (int k = 0; k < CH; k++) pv[k] = act ? pp[k] : T(0);
     }
     T A = T(1), B = T(0);
 #pragma unroll
@@ -307,11 +316,12 @@ static void launch_tridiag_ch(const SubPlan& sp, const T* pinv, T* data, long lo
 {
     const int P = (sp.nx + CH - 1) / CH;
     int       MB = 1;
-    while (MB * 2 * P <= 1024 && MB < 64) MB *= 2;
+    while (MB * 2 * P <= 1024 && MB < sp.mb_field) MB *= 2;
     if (MB * P > 1024) throw Error(FP_E_INVALID, "nx too large for the tridiagonal kernel");
     const size_t lds = 3 * (size_t)P * MB * sizeof(T);
     dim3         grid((sp.ny + MB - 1) / MB, nrhs);
-    hipLaunchKernelGGL((k_tridiag<T, CH>), grid, dim3((MB * P + 63) / 64 * 64), lds, s, data, ld, sp.nx, sp.ny, pinv, P, MB,
+    hipLaunchKernelGGL((k_tridiag<T, CH>), grid, dim3((MB * P + 63) / 64 * 64), lds, s, data, ld, sp.nx, sp.ny, pinv,
+                       sp.ldp, P, MB,
                        sp.no_dirichlet ? 1 : 0);
     FP_HIP(hipGetLastError());
 }
@@ -330,7 +340,7 @@ static int launch_lines_ch(const SubPlan& sp, const LinesArgs<T>& a, int nrhs, h
 {
     const int P = (sp.nx + CH - 1) / CH;
     int       MB = 1;
-    while (MB * 2 * P <= 1024 && MB < 64) MB *= 2;
+    while (MB * 2 * P <= 1024 && MB < sp.mb_lines) MB *= 2;
     if (MB * P > 1024) throw Error(FP_E_INVALID, "nx too large for the tridiagonal kernel");
     const size_t lds = 3 * (size_t)P * MB * sizeof(T);
     dim3         grid((sp.ny + MB - 1) / MB, nrhs);
@@ -345,7 +355,7 @@ int lines_workgroups(const SubPlan& sp)
     const int CH = sp.nx <= 1024 ? 16 : 32;
     const int P  = (sp.nx + CH - 1) / CH;
     int       MB = 1;
-    while (MB * 2 * P <= 1024 && MB < 64) MB *= 2;
+    while (MB * 2 * P <= 1024 && MB < sp.mb_lines) MB *= 2;
     return (sp.ny + MB - 1) / MB;
 }
 template int lines_workgroups<double>(const SubPlan&);

@@ -40,8 +40,13 @@ struct SchurLinesArgs {
     long long     ldl;
 };
 void launch_schur_lines(const SchurLinesArgs& a, int nrhs, hipStream_t s);
+// v == nullptr: v is the unit vector e_{k0 + r} of right-hand side r
 void launch_lines_assemble(const double* v, double* out, int K, const int* kface, const int* kpos, const double* xu,
-                           int ny, const double* colpart, int nwg, int nx, int nrhs, hipStream_t s);
+                           int ny, const double* colpart, int nwg, int nx, int nrhs, hipStream_t s, int k0 = 0);
+void launch_lines_from_units(const SchurLinesArgs& a, int k0, int cnt, hipStream_t s);
+void launch_dense_gemv(const double* A, int n, const double* x, double* y, double alpha, double beta, int nrhs,
+                       hipStream_t s);
+void gauss_jordan_invert(double* A, int n, hipStream_t s);
 void launch_transpose(const double* in, double* out, int n, hipStream_t s);
 void launch_schur_collect(const double* buf, long long ldb, const int* cell, int cn, int i0, int cnt, double* S,
                           hipStream_t s);

@@ -198,7 +198,7 @@ void launch_schur_lines(const SchurLinesArgs& a, int nrhs, hipStream_t s)
 // x-lines from the inverse-transformed coefficient rows, y-lines from the
 // column sums of the workgroups (fixed order)
 __global__ void k_lines_assemble(const double* v, double* out, int K, const int* kface, const int* kpos,
-                                 const double* xu, int ny, const double* colpart, int nwg, int nx)
+                                 const double* xu, int ny, const double* colpart, int nwg, int nx, int k0)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= K) return;
@@ -220,13 +220,14 @@ __global__ void k_lines_assemble(const double* v, double* out, int K, const int*
         for (; w < nwg; w++) t0 += cp[(long long)w * 2 * nx];
         t = (t0 + t1) + (t2 + t3);
     }
-    out[r * K + k] = v[r * K + k] - t;
+    const double vv = v ? v[r * K + k] : (k == k0 + (int)r ? 1.0 : 0.0);
+    out[r * K + k]  = vv - t;
 }
 void launch_lines_assemble(const double* v, double* out, int K, const int* kface, const int* kpos, const double* xu,
-                           int ny, const double* colpart, int nwg, int nx, int nrhs, hipStream_t s)
+                           int ny, const double* colpart, int nwg, int nx, int nrhs, hipStream_t s, int k0)
 {
     hipLaunchKernelGGL(k_lines_assemble, dim3((K + 63) / 64, nrhs), dim3(64), 0, s, v, out, K, kface, kpos, xu,
-                       ny, colpart, nwg, nx);
+                       ny, colpart, nwg, nx, k0);
     FP_HIP(hipGetLastError());
 }
 
@@ -249,6 +250,149 @@ void launch_transpose(const double* in, double* out, int n, hipStream_t s)
 {
     hipLaunchKernelGGL(k_transpose, dim3((n + 31) / 32, (n + 31) / 32), dim3(32, 8), 0, s, in, out, n);
     FP_HIP(hipGetLastError());
+}
+
+// ---- dense interface preconditioner ---------------------------------------------
+// y = alpha * A x + beta * y for a dense n x n row-major A, one wavefront per
+// row, nrhs right-hand sides (x, y: nrhs x n)
+__global__ __launch_bounds__(256) void k_dense_gemv(const double* __restrict__ A, int n, const double* x, double* y,
+                                                    double alpha, double beta)
+{
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i    = blockIdx.x * 4 + wave;
+    if (i >= n) return;
+    const long long r   = blockIdx.y;
+    const double*   Ai  = A + (long long)i * n;
+    const double*   xr  = x + r * n;
+    double          a0 = 0.0, a1 = 0.0;
+    int             j  = lane;
+    for (; j + 64 < n; j += 128) {
+        a0 += Ai[j] * xr[j];
+        a1 += Ai[j + 64] * xr[j + 64];
+    }
+    if (j < n) a0 += Ai[j] * xr[j];
+    const double acc = wave_reduce_sum(a0 + a1);
+    if (lane == 0) y[r * n + i] = alpha * acc + (beta == 0.0 ? 0.0 : beta * y[r * n + i]);
+}
+void launch_dense_gemv(const double* A, int n, const double* x, double* y, double alpha, double beta, int nrhs,
+                       hipStream_t s)
+{
+    hipLaunchKernelGGL(k_dense_gemv, dim3((n + 3) / 4, nrhs), dim3(256), 0, s, A, n, x, y, alpha, beta);
+    FP_HIP(hipGetLastError());
+}
+
+// line buffers of unit interface vectors k0 .. k0+cnt-1: the Schur product of a
+// unit vector is one column of that child's block
+__global__ void k_lines_from_units(SchurLinesArgs a, int k0, int cnt)
+{
+    const int q = blockIdx.y;
+    const int k = k0 + q;
+    int       c = 0;
+    while (c + 1 < a.nch && k >= a.ch[c + 1].voff) c++;
+    const SchurChild& C = a.ch[c];
+    const int         p = k - C.voff;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < C.cn; i += gridDim.x * blockDim.x)
+        a.lb[(long long)q * a.ldl + C.lboff + i] = C.S[(long long)i * C.cn + p];
+}
+void launch_lines_from_units(const SchurLinesArgs& a, int k0, int cnt, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_lines_from_units, dim3(4, cnt), dim3(256), 0, s, a, k0, cnt);
+    FP_HIP(hipGetLastError());
+}
+
+// Gauss-Jordan inversion in place with partial pivoting (columns unscrambled at
+// the end).  Step k: pivot search, row swap and scaling of row k by one block,
+// then the rank-1 elimination of every other row.
+__global__ __launch_bounds__(1024) void k_gj_pivot(double* A, int n, int k, int* perm, double* fcol, int* bad)
+{
+    __shared__ double bv[1024];
+    __shared__ int    bi[1024];
+    double            best = -1.0;
+    int               arg  = k;
+    for (int i = k + threadIdx.x; i < n; i += 1024) {
+        const double v = fabs(A[(long long)i * n + k]);
+        if (v > best) {
+            best = v;
+            arg  = i;
+        }
+    }
+    bv[threadIdx.x] = best;
+    bi[threadIdx.x] = arg;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            const double o = bv[threadIdx.x + w];
+            const int    oi = bi[threadIdx.x + w];
+            if (o > bv[threadIdx.x] || (o == bv[threadIdx.x] && oi < bi[threadIdx.x])) {
+                bv[threadIdx.x] = o;
+                bi[threadIdx.x] = oi;
+            }
+        }
+        __syncthreads();
+    }
+    const int p = bi[0];
+    if (threadIdx.x == 0) {
+        perm[k] = p;
+        if (!(bv[0] > 0.0) || !isfinite(bv[0])) *bad = 1;
+    }
+    double* rk = A + (long long)k * n;
+    double* rp = A + (long long)p * n;
+    if (p != k)
+        for (int j = threadIdx.x; j < n; j += 1024) {
+            const double t = rk[j];
+            rk[j]          = rp[j];
+            rp[j]          = t;
+        }
+    __syncthreads();
+    const double piv = rk[k];
+    __syncthreads();
+    const double inv = 1.0 / piv;
+    for (int j = threadIdx.x; j < n; j += 1024) rk[j] = (j == k ? 1.0 : rk[j]) * inv;
+    for (int i = threadIdx.x; i < n; i += 1024) fcol[i] = (i == k) ? 0.0 : A[(long long)i * n + k];
+}
+
+__global__ void k_gj_eliminate(double* A, int n, int k, const double* fcol)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y;
+    if (j >= n || i == k) return;
+    const double f  = fcol[i];
+    const double rk = A[(long long)k * n + j];
+    double*      a  = A + (long long)i * n + j;
+    *a              = (j == k ? 0.0 : *a) - f * rk;
+}
+
+__global__ void k_gj_unscramble(double* A, int n, const int* perm)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double* row = A + (long long)i * n;
+    for (int k = n - 1; k >= 0; k--) {
+        const int p = perm[k];
+        if (p != k) {
+            const double t = row[k];
+            row[k]         = row[p];
+            row[p]         = t;
+        }
+    }
+}
+
+void gauss_jordan_invert(double* A, int n, hipStream_t s)
+{
+    DevBuf<int>    perm(n), bad(1);
+    DevBuf<double> fcol(n);
+    FP_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), s));
+    for (int k = 0; k < n; k++) {
+        hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(1024), 0, s, A, n, k, perm.p, fcol.p, bad.p);
+        hipLaunchKernelGGL(k_gj_eliminate, dim3((n + 255) / 256, n), dim3(256), 0, s, A, n, k, fcol.p);
+    }
+    FP_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_gj_unscramble, dim3((n + 255) / 256), dim3(256), 0, s, A, n, perm.p);
+    FP_HIP(hipGetLastError());
+    int h_bad = 0;
+    FP_HIP(hipMemcpyAsync(&h_bad, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    FP_HIP(hipStreamSynchronize(s));
+    if (h_bad) throw Error(FP_E_NUMERIC, "interface operator is singular");
 }
 
 // S(j, i) = response at line position j to the unit source at position i0+q:
@@ -440,12 +584,13 @@ __global__ __launch_bounds__(GB) void k_gm_finish(GmresDev g, int j)
         }
         return;
     }
-    if (hn < 1e-12) {  // breakdown: stop this cycle with j columns
-        if (blockIdx.x == 0 && threadIdx.x == 0) g.state[r * 4 + 0] = 0;
-        return;
-    }
-    const long long k = blockIdx.x * (long long)GB + threadIdx.x;
-    if (k < K) g.V[(j + 1) * ldv + (long long)r * K + k] = g.Wv[r * K + k] * (1.0 / hn);
+    // h_{j+1,j} < 1e-12 (the reference's breakdown threshold): the Krylov
+    // space is invariant.  Column j is kept (the reference drops it, which
+    // stalls when the operator is close to the identity, e.g. preconditioned)
+    // and the cycle ends with a zero least-squares residual.
+    const bool      breakdown = hn < 1e-12;
+    const long long k         = blockIdx.x * (long long)GB + threadIdx.x;
+    if (!breakdown && k < K) g.V[(j + 1) * ldv + (long long)r * K + k] = g.Wv[r * K + k] * (1.0 / hn);
     if (blockIdx.x != 0) return;
     double*       H  = g.H + (long long)r * (m + 1) * m;
     double*       cs = g.cs + (long long)r * m;
@@ -477,7 +622,7 @@ __global__ __launch_bounds__(GB) void k_gm_finish(GmresDev g, int j)
         gg[j]          = t;
         g.state[r * 4 + 1] = j + 1;
         g.resid[r]         = fabs(gg[j + 1]);
-        if (fabs(gg[j + 1]) < g.tolabs[r]) g.state[r * 4 + 0] = 0;
+        if (breakdown || fabs(gg[j + 1]) < g.tolabs[r]) g.state[r * 4 + 0] = 0;
         g.state[r * 4 + 3] += 1;  // Arnoldi steps taken
     }
     __syncthreads();
