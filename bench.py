@@ -32,6 +32,7 @@ def parse():
     ap.add_argument("--n", type=int, default=1024, help="centre size (arms n/2)")
     ap.add_argument("--m", type=int, default=200, help="GMRES restart length on the interface")
     ap.add_argument("--tol", type=float, default=1e-12, help="relative GMRES tolerance on the interface")
+    ap.add_argument("--oracle-tol", type=float, default=1e-13, help="relative GMRES tolerance of the CPU oracle")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=20)
     return ap.parse_args()
@@ -116,7 +117,9 @@ def main():
     if not a.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
-        oc = O.Composite(doms, conn, m=20, tol=a.tol, tol_is_relative=True, maxiter=100,
+        # reference semantics, GMRES(20); 1e-13 relative keeps the oracle's own
+        # error near 1e-11 at this size, so the comparison resolves 1e-10
+        oc = O.Composite(doms, conn, m=20, tol=a.oracle_tol, tol_is_relative=True, maxiter=200,
                          schur_mode=O.SCHUR_SPECTRAL)
         t0 = time.perf_counter()
         ref, it, _ = oc.solve([f[0] for f in fields])
@@ -128,7 +131,7 @@ def main():
                "kind": "port",
                "sample": "one full %dx%d-centre cross solve on the host (construction excluded), reference "
                          "algorithm: full-field GMRES(m=20) with dense Schur blocks to relative %.0e, %d "
-                         "iterations, %.2f s" % (n, n, a.tol, it, t_cpu)}
+                         "iterations, %.2f s" % (n, n, a.oracle_tol, it, t_cpu)}
 
     out = {
         "metric": "cross-geometry solves/s at 1e-10 rel L2: FP64 vs FP32+refine; HBM GB/s",

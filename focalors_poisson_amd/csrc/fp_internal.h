@@ -82,7 +82,8 @@ struct TransformPlan {
     int        L    = 0;  // complex FFT length
     int        nrad = 0;
     int        rad[kMaxRadices] = {};
-    DevBuf<T>  W;         // 2L: W[k] = exp(-2 pi i k / L)
+    int        p1   = 0;  // odd prime > 7 with a register codelet (0: none)
+    DevBuf<T>  W;         // twiddles: 64 of W^m, then L/64+1 of W^(64m), W = exp(-2 pi i / L)
     DevBuf<T>  aux;       // 2N: NN phase exp(-i pi m / 2N) or ND (sin, cos) of 2N pi (j+1)/(2N+1)
     T          c0 = 0, c1 = 0;
     void build(int kind, int N);

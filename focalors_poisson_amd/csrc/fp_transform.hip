@@ -18,6 +18,7 @@
 // registers, any other prime by a two-phase in-LDS DFT.  One workgroup owns
 // one row pair of one right-hand side: it loads both rows (coalesced), runs
 // all passes between two LDS buffers and writes the coefficients back in place.
+#include "fp_dft_tables.h"
 #include "fp_internal.h"
 
 #include <cmath>
@@ -55,16 +56,24 @@ struct YArgs {
     T            c0, c1;
 };
 
+// Twiddle factors exp(-2 pi i t / L) from two small LDS tables:
+// W^t = Whi[t >> 6] * Wlo[t & 63]  (error of one extra rounded product)
+template <typename T>
+struct Tw {
+    const C2<T>* lo;  // 64 entries: W^m
+    const C2<T>* hi;  // L/64 + 1 entries: W^(64 m)
+};
+
 template <bool FWD, typename T>
-__device__ __forceinline__ C2<T> twiddle(const C2<T>* W, int t, C2<T> v)
+__device__ __forceinline__ C2<T> twiddle(const Tw<T> W, int t, C2<T> v)
 {
-    const C2<T> w = W[t];
+    const C2<T> w = cmul(W.hi[t >> 6], W.lo[t & 63]);
     return FWD ? cmul(v, w) : cmulc(v, w);
 }
 
 // One Stockham pass with a register butterfly of radix R.
 template <int R, bool FWD, int NT, typename T>
-__device__ void pass_reg(const C2<T>* __restrict__ src, C2<T>* __restrict__ dst, const C2<T>* W, int L, int Ns)
+__device__ void pass_reg(const C2<T>* __restrict__ src, C2<T>* __restrict__ dst, const Tw<T> W, int L, int Ns)
 {
     const int nb = L / R, stride = L / (Ns * R);
     for (int jb = threadIdx.x; jb < nb; jb += NT) {
@@ -73,7 +82,7 @@ __device__ void pass_reg(const C2<T>* __restrict__ src, C2<T>* __restrict__ dst,
 #pragma unroll
         for (int r = 0; r < R; r++) {
             v[r] = src[jb + r * nb];
-            if (r > 0) v[r] = twiddle<FWD>(W, r * k * stride, v[r]);
+            if (r > 0 && Ns > 1) v[r] = twiddle<FWD>(W, r * k * stride, v[r]);
         }
         C2<T> o[R];
         if constexpr (R == 2) {
@@ -86,6 +95,26 @@ __device__ void pass_reg(const C2<T>* __restrict__ src, C2<T>* __restrict__ dst,
             o[2] = csub(t0, t2);
             o[1] = cadd(t1, t3);
             o[3] = csub(t1, t3);
+        } else if constexpr (R == 8) {
+            const T     h  = T(0.70710678118654752440084436210485);
+            const C2<T> a0 = cadd(v[0], v[4]), a1 = csub(v[0], v[4]);
+            const C2<T> a2 = cadd(v[2], v[6]), a3 = mul_mi<FWD>(csub(v[2], v[6]));
+            const C2<T> a4 = cadd(v[1], v[5]), a5 = csub(v[1], v[5]);
+            const C2<T> a6 = cadd(v[3], v[7]), a7 = mul_mi<FWD>(csub(v[3], v[7]));
+            const C2<T> b0 = cadd(a0, a2), b2 = csub(a0, a2), b1 = cadd(a1, a3), b3 = csub(a1, a3);
+            C2<T>       c0 = cadd(a4, a6), c2 = csub(a4, a6), c1 = cadd(a5, a7), c3 = csub(a5, a7);
+            // c1 *= w8, c2 *= w8^2 = -+i, c3 *= w8^3  (w8 = exp(-+ i pi / 4))
+            c1 = FWD ? C2<T>{h * (c1.x + c1.y), h * (c1.y - c1.x)} : C2<T>{h * (c1.x - c1.y), h * (c1.y + c1.x)};
+            c2 = mul_mi<FWD>(c2);
+            c3 = FWD ? C2<T>{h * (c3.y - c3.x), -h * (c3.x + c3.y)} : C2<T>{-h * (c3.x + c3.y), h * (c3.x - c3.y)};
+            o[0] = cadd(b0, c0);
+            o[4] = csub(b0, c0);
+            o[1] = cadd(b1, c1);
+            o[5] = csub(b1, c1);
+            o[2] = cadd(b2, c2);
+            o[6] = csub(b2, c2);
+            o[3] = cadd(b3, c3);
+            o[7] = csub(b3, c3);
         } else if constexpr (R == 3) {
             const T     h = T(0.86602540378443864676372317075294);
             const C2<T> s = cadd(v[1], v[2]), d = csub(v[1], v[2]);
@@ -115,10 +144,98 @@ __device__ void pass_reg(const C2<T>* __restrict__ src, C2<T>* __restrict__ dst,
     }
 }
 
+// One odd-prime butterfly, the share of thread group g: outputs q = QB..QE-1
+// and R-q, plus X_0 for group 0.  With a_r = x_r + x_{R-r}, b_r = x_r - x_{R-r}:
+//   X_q = x_0 + A_q -+ i B_q,  X_{R-q} = x_0 + A_q +- i B_q,
+//   A_q = sum_r a_r cos(2 pi q r / R),  B_q = sum_r b_r sin(2 pi q r / R).
+// The inputs stream through registers one pair at a time; only the group's
+// accumulators stay live.  q and r are compile-time, so the cosines are
+// immediates.
+template <int R, int G, int g, bool FWD, bool TW, typename T>
+__device__ __forceinline__ void odd_group(const C2<T>* __restrict__ src, C2<T>* __restrict__ dst, const Tw<T> W,
+                                          int jb, int nb, int k, int stride, int base, int Ns)
+{
+    constexpr int H  = (R - 1) / 2;
+    constexpr int QP = (H + G - 1) / G;
+    constexpr int QB = 1 + g * QP;
+    constexpr int QE = (QB + QP < H + 1) ? QB + QP : H + 1;
+    constexpr int NQ = QE - QB;
+    const C2<T>   v0 = src[jb];
+    T             ax[NQ], ay[NQ], bx[NQ], by[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        ax[q] = v0.x;
+        ay[q] = v0.y;
+        bx[q] = T(0);
+        by[q] = T(0);
+    }
+    T x0x = v0.x, x0y = v0.y;
+#pragma unroll
+    for (int r = 1; r <= H; r++) {
+        C2<T> x1 = src[jb + r * nb], x2 = src[jb + (R - r) * nb];
+        if constexpr (TW) {
+            x1 = twiddle<FWD>(W, r * k * stride, x1);
+            x2 = twiddle<FWD>(W, (R - r) * k * stride, x2);
+        }
+        const C2<T> a = cadd(x1, x2), b = csub(x1, x2);
+        if constexpr (g == 0) {
+            x0x += a.x;
+            x0y += a.y;
+        }
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+            const T c  = T(DftTab<R>::c[((QB + q) * r) % R]);
+            const T sn = T(DftTab<R>::s[((QB + q) * r) % R]);
+            ax[q] += a.x * c;
+            ay[q] += a.y * c;
+            bx[q] += b.x * sn;
+            by[q] += b.y * sn;
+        }
+        // one input pair in flight at a time: bounds the live registers to the
+        // accumulators (other waves hide the LDS latency)
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    if constexpr (g == 0) dst[base] = C2<T>{x0x, x0y};
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        const C2<T> xq  = FWD ? C2<T>{ax[q] + by[q], ay[q] - bx[q]} : C2<T>{ax[q] - by[q], ay[q] + bx[q]};
+        const C2<T> xrq = FWD ? C2<T>{ax[q] - by[q], ay[q] + bx[q]} : C2<T>{ax[q] + by[q], ay[q] - bx[q]};
+        dst[base + (QB + q) * Ns]     = xq;
+        dst[base + (R - QB - q) * Ns] = xrq;
+    }
+}
+
+// One Stockham pass of odd prime radix R: G threads share a butterfly, each
+// computing about five of its (q, R-q) output pairs.
+template <int R, bool FWD, int NT, bool TW, typename T>
+__device__ void pass_odd_tw(const C2<T>* __restrict__ src, C2<T>* __restrict__ dst, const Tw<T> W, int L, int Ns)
+{
+    constexpr int H = (R - 1) / 2;
+    constexpr int G = (H + 4) / 5;
+    const int     nb = L / R, stride = L / (Ns * R);
+    for (int t = threadIdx.x; t < nb * G; t += NT) {
+        const int jb = t / G, grp = t - jb * G, k = jb % Ns;
+        const int base = (jb / Ns) * Ns * R + k;
+        if (grp == 0) odd_group<R, G, 0, FWD, TW>(src, dst, W, jb, nb, k, stride, base, Ns);
+        if constexpr (G > 1) if (grp == 1) odd_group<R, G, 1, FWD, TW>(src, dst, W, jb, nb, k, stride, base, Ns);
+        if constexpr (G > 2) if (grp == 2) odd_group<R, G, 2, FWD, TW>(src, dst, W, jb, nb, k, stride, base, Ns);
+        if constexpr (G > 3) if (grp == 3) odd_group<R, G, 3, FWD, TW>(src, dst, W, jb, nb, k, stride, base, Ns);
+        if constexpr (G > 4) if (grp == 4) odd_group<R, G, 4, FWD, TW>(src, dst, W, jb, nb, k, stride, base, Ns);
+    }
+}
+
+// the first pass (Ns = 1) has unit twiddles: a separate instantiation skips them
+template <int R, bool FWD, int NT, typename T>
+__device__ void pass_odd(const C2<T>* __restrict__ src, C2<T>* __restrict__ dst, const Tw<T> W, int L, int Ns)
+{
+    if (Ns == 1) pass_odd_tw<R, FWD, NT, false>(src, dst, W, L, Ns);
+    else pass_odd_tw<R, FWD, NT, true>(src, dst, W, L, Ns);
+}
+
 // Stockham pass of any radix R: twiddle in place, then each output is an
 // R-term sum read from LDS.
 template <bool FWD, int NT, typename T>
-__device__ void pass_generic(C2<T>* __restrict__ src, C2<T>* __restrict__ dst, const C2<T>* W, int L, int Ns,
+__device__ void pass_generic(C2<T>* __restrict__ src, C2<T>* __restrict__ dst, const Tw<T> W, int L, int Ns,
                              int R)
 {
     const int nb = L / R, stride = L / (Ns * R);
@@ -140,21 +257,25 @@ __device__ void pass_generic(C2<T>* __restrict__ src, C2<T>* __restrict__ dst, c
     }
 }
 
-template <bool FWD, int NT, typename T>
-__device__ C2<T>* run_fft(const YArgs<T>& a, C2<T>* A, C2<T>* B)
+// Radix dispatch: 2, 3, 4, 5, 7, 8 in every kernel, one larger odd prime P1
+// per kernel instantiation (keeping one codelet's registers per kernel), any
+// other factor through the generic pass.
+template <int P1, bool FWD, int NT, typename T>
+__device__ C2<T>* run_fft(const YArgs<T>& a, const Tw<T> W, C2<T>* A, C2<T>* B)
 {
     C2<T>* src = A;
     C2<T>* dst = B;
     int    Ns  = 1;
     for (int s = 0; s < a.nrad; s++) {
         const int R = a.rad[s];
-        switch (R) {
-            case 2: pass_reg<2, FWD, NT>(src, dst, a.W, a.L, Ns); break;
-            case 3: pass_reg<3, FWD, NT>(src, dst, a.W, a.L, Ns); break;
-            case 4: pass_reg<4, FWD, NT>(src, dst, a.W, a.L, Ns); break;
-            case 5: pass_reg<5, FWD, NT>(src, dst, a.W, a.L, Ns); break;
-            default: pass_generic<FWD, NT>(src, dst, a.W, a.L, Ns, R); break;
-        }
+        if (R == 2) pass_reg<2, FWD, NT>(src, dst, W, a.L, Ns);
+        else if (R == 3) pass_reg<3, FWD, NT>(src, dst, W, a.L, Ns);
+        else if (R == 4) pass_reg<4, FWD, NT>(src, dst, W, a.L, Ns);
+        else if (R == 5) pass_reg<5, FWD, NT>(src, dst, W, a.L, Ns);
+        else if (R == 8) pass_reg<8, FWD, NT>(src, dst, W, a.L, Ns);
+        else if (R == 7) pass_odd<7, FWD, NT>(src, dst, W, a.L, Ns);
+        else if (P1 > 7 && R == P1) pass_odd<(P1 > 7 ? P1 : 11), FWD, NT>(src, dst, W, a.L, Ns);
+        else pass_generic<FWD, NT>(src, dst, W, a.L, Ns, R);
         __syncthreads();
         C2<T>* t = src;
         src      = dst;
@@ -167,13 +288,18 @@ __device__ C2<T>* run_fft(const YArgs<T>& a, C2<T>* A, C2<T>* B)
 template <typename T>
 __device__ __forceinline__ T sgn_pow(int p) { return (p & 1) ? T(-1) : T(1); }
 
-template <int NT, typename T>
+template <int NT, int P1, typename T>
 __global__ __launch_bounds__(NT) void k_ytransform(YArgs<T> a)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    C2<T>* A  = reinterpret_cast<C2<T>*>(smem_raw);
-    C2<T>* B  = A + a.L;
-    T*     y0 = reinterpret_cast<T*>(B + a.L);  // NN inverse: y_0 of rows a, b
+    C2<T>* A   = reinterpret_cast<C2<T>*>(smem_raw);
+    C2<T>* B   = A + a.L;
+    C2<T>* tlo = B + a.L;           // 64 twiddles W^m
+    C2<T>* thi = tlo + 64;          // L/64 + 1 twiddles W^(64m)
+    const int nhi = a.L / 64 + 1;
+    T*     y0  = reinterpret_cast<T*>(thi + nhi);  // NN inverse: y_0 of rows a, b
+    for (int q = threadIdx.x; q < 64 + nhi; q += NT) tlo[q] = a.W[q];
+    const Tw<T> W{tlo, thi};
 
     const int  N = a.N, L = a.L;
     const int  ra = 2 * blockIdx.x, rb = ra + 1;
@@ -230,7 +356,7 @@ __global__ __launch_bounds__(NT) void k_ytransform(YArgs<T> a)
     __syncthreads();
 
     const bool   fwd = !(a.kind == T_NN && a.inverse);
-    const C2<T>* Z   = fwd ? run_fft<true, NT>(a, A, B) : run_fft<false, NT>(a, A, B);
+    const C2<T>* Z   = fwd ? run_fft<P1, true, NT>(a, W, A, B) : run_fft<P1, false, NT>(a, W, A, B);
 
     // ---- store: split the two rows' coefficients ----
     for (int j = threadIdx.x; j < N; j += NT) {
@@ -275,20 +401,33 @@ __global__ __launch_bounds__(NT) void k_ytransform(YArgs<T> a)
 // ---------------------------------------------------------------------------
 // host side
 
+// Radix schedule: the odd primes above 5 first (their passes then run with
+// Ns = 1, i.e. without twiddle factors), then 8, 4, 2, 3, 5.
 static void factor_radices(int L, int* rad, int& nrad)
 {
     nrad  = 0;
     int n = L;
+    {
+        int m = L, big[kMaxRadices], nb = 0;
+        for (int p : {2, 3, 5})
+            while (m % p == 0) m /= p;
+        for (int p = 7; m > 1; p += 2)
+            while (m % p == 0) {
+                if (nb >= kMaxRadices) throw Error(FP_E_INVALID, "transform length has too many factors");
+                big[nb++] = p;
+                m /= p;
+            }
+        for (int i = nb - 1; i >= 0; i--) {
+            rad[nrad++] = big[i];
+            n /= big[i];
+        }
+    }
+    while (n % 8 == 0) { rad[nrad++] = 8; n /= 8; }
     while (n % 4 == 0) { rad[nrad++] = 4; n /= 4; }
     while (n % 2 == 0) { rad[nrad++] = 2; n /= 2; }
     while (n % 3 == 0) { rad[nrad++] = 3; n /= 3; }
     while (n % 5 == 0) { rad[nrad++] = 5; n /= 5; }
-    for (int p = 7; n > 1; p += 2)
-        while (n % p == 0) {
-            if (nrad >= kMaxRadices) throw Error(FP_E_INVALID, "transform length has too many factors");
-            rad[nrad++] = p;
-            n /= p;
-        }
+    if (n != 1) throw Error(FP_E_INVALID, "radix schedule does not cover the transform length");
 }
 
 template <typename T>
@@ -300,10 +439,19 @@ void TransformPlan<T>::build(int k, int n)
     else if (kind == T_NN) L = 2 * N;
     else L = 2 * N + 1;
     factor_radices(L, rad, nrad);
-    const long double PI = 3.141592653589793238462643383279502884L;
-    std::vector<T>    w(2 * (size_t)L), x(2 * (size_t)N, T(0));
-    for (int q = 0; q < L; q++) {
-        const long double ang = -2.0L * PI * (long double)q / (long double)L;
+    p1 = 0;
+    for (int i = 0; i < nrad; i++) {
+        const int r = rad[i];
+        if ((r == 11 || r == 13 || r == 17 || r == 19 || r == 23 || r == 29 || r == 31 || r == 37 || r == 41 ||
+             r == 43 || r == 47) && r > p1)
+            p1 = r;
+    }
+    const long double PI  = 3.141592653589793238462643383279502884L;
+    const int         nhi = L / 64 + 1;
+    std::vector<T>    w(2 * (size_t)(64 + nhi)), x(2 * (size_t)N, T(0));
+    for (int q = 0; q < 64 + nhi; q++) {
+        const long long   t   = q < 64 ? q : 64LL * (q - 64);
+        const long double ang = -2.0L * PI * (long double)(t % L) / (long double)L;
         w[2 * q]              = (T)cosl(ang);
         w[2 * q + 1]          = (T)sinl(ang);
     }
@@ -329,7 +477,7 @@ void TransformPlan<T>::build(int k, int n)
     } else {
         c0 = (T)(std::sqrt(2.0 / (N + 1.0)) * 0.5);
     }
-    W.alloc(2 * (size_t)L);
+    W.alloc(w.size());
     W.upload(w.data(), w.size());
     aux.alloc(2 * (size_t)N);
     aux.upload(x.data(), x.size());
@@ -357,16 +505,36 @@ void launch_ytransform(const TransformPlan<T>& tp, T* data, long long ld, int nx
     a.aux = reinterpret_cast<const C2<T>*>(tp.aux.p);
     a.c0  = tp.c0;
     a.c1  = tp.c1;
-    const size_t lds = 2 * (size_t)tp.L * sizeof(C2<T>) + 4 * sizeof(T);
+    const size_t lds = (2 * (size_t)tp.L + 64 + tp.L / 64 + 1) * sizeof(C2<T>) + 4 * sizeof(T);
     if (lds > 160 * 1024) throw Error(FP_E_INVALID, "transform length exceeds the LDS of one workgroup");
-    static bool attr_set = false;
-    if (lds > 64 * 1024 && !attr_set) {
-        FP_HIP(hipFuncSetAttribute((const void*)k_ytransform<NT, T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   160 * 1024));
-        attr_set = true;
-    }
     dim3 grid((nx + 1) / 2, nrhs);
-    hipLaunchKernelGGL((k_ytransform<NT, T>), grid, dim3(NT), lds, s, a);
+    switch (tp.p1) {
+#define FP_P1_CASE(P)                                                                                            \
+    case P: {                                                                                                    \
+        static bool attr_set = false;                                                                            \
+        if (lds > 64 * 1024 && !attr_set) {                                                                      \
+            FP_HIP(hipFuncSetAttribute((const void*)k_ytransform<NT, P, T>,                                      \
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));                 \
+            attr_set = true;                                                                                     \
+        }                                                                                                        \
+        hipLaunchKernelGGL((k_ytransform<NT, P, T>), grid, dim3(NT), lds, s, a);                                 \
+        break;                                                                                                   \
+    }
+        FP_P1_CASE(0)
+        FP_P1_CASE(11)
+        FP_P1_CASE(13)
+        FP_P1_CASE(17)
+        FP_P1_CASE(19)
+        FP_P1_CASE(23)
+        FP_P1_CASE(29)
+        FP_P1_CASE(31)
+        FP_P1_CASE(37)
+        FP_P1_CASE(41)
+        FP_P1_CASE(43)
+        FP_P1_CASE(47)
+#undef FP_P1_CASE
+        default: throw Error(FP_E_INVALID, "no transform kernel for this radix schedule");
+    }
     FP_HIP(hipGetLastError());
 }
 
