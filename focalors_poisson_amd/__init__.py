@@ -79,6 +79,21 @@ class PoissonSolver2D:
         _capi.check(fn(self._h, ctypes.c_void_p(f.data_ptr()), nrhs, n, _stream_ptr(stream)))
         return f
 
+    def solve_refine(self, f, tol=1e-12, max_steps=30, stream=None, allow_not_converged=False):
+        """FP32 inner solves + FP64 residual/iterative refinement, in place on a
+        float64 field; returns the solve statistics dict."""
+        import torch
+        n = self.nx * self.ny
+        nrhs = _check_field(f, torch.float64, n)
+        info = _capi.SolveInfo()
+        allow = (_capi.FP_E_NOT_CONVERGED,) if allow_not_converged else ()
+        st = _capi.check(_capi.lib().fp_sub_solve_refine(self._h, ctypes.c_void_p(f.data_ptr()), nrhs, n, float(tol),
+                                                         int(max_steps), _stream_ptr(stream), ctypes.byref(info)),
+                         allow)
+        d = info.as_dict()
+        d["status"] = st
+        return d
+
     def transform(self, f, inverse=False, stream=None):
         """PoissonFFT2D::transform / transform_transpose along y, in place (float64)."""
         import torch
@@ -175,6 +190,26 @@ class ConcatPoissonSolver2D:
         d = info.as_dict()
         d["status"] = st
         return d
+
+    def vjp(self, grad_u, faces=(), stream=None):
+        """Adjoint of solve: returns (dL/df per domain, {(d, face): dL/dg}) for
+        dL/du = grad_u (one tensor per domain, same shapes as the fields)."""
+        import torch
+        nd = len(self.domains)
+        gu = [g.contiguous() for g in grad_u]
+        up, _, nrhs = self._ptrs(gu, None)
+        gf = [torch.empty_like(g) for g in gu]
+        fp = (ctypes.c_void_p * nd)(*[t.data_ptr() for t in gf])
+        gp = (ctypes.c_void_p * (4 * nd))()
+        gg = {}
+        for d, face in faces:
+            flen = self.domains[d][1] if face < 2 else self.domains[d][0]
+            t = torch.empty(nrhs * flen, dtype=torch.float64, device=gu[0].device)
+            gg[(d, face)] = t
+            gp[4 * d + face] = t.data_ptr()
+        info = _capi.SolveInfo()
+        _capi.check(_capi.lib().fp_cat_vjp(self._h, up, fp, gp, nrhs, _stream_ptr(stream), ctypes.byref(info)))
+        return gf, gg
 
     def __del__(self):
         h = getattr(self, "_h", None)

@@ -77,6 +77,8 @@ def lib():
     L.fp_sub_transform.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_longlong, ctypes.c_int, _vp]
     L.fp_sub_chasing.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_longlong, _vp]
     L.fp_sub_solve_f32.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_longlong, _vp]
+    L.fp_sub_solve_refine.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_longlong, ctypes.c_double, ctypes.c_int, _vp,
+                                      ctypes.POINTER(SolveInfo)]
     L.fp_sub_solve_batched.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _ip, _vp]
     L.fp_sub_solve_batched_f32.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _ip, _vp]
     L.fp_cat_create.argtypes = [_vp, ctypes.POINTER(GeometryDesc), ctypes.c_int, ctypes.c_double, ctypes.c_int,

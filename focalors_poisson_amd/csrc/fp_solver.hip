@@ -6,6 +6,7 @@
 //                  (pe/concat/concat_solver2d.cpp, gmres_solver2d.cpp,
 //                  schur_mat2d.cpp), with the Krylov iteration moved from
 //                  full root fields to the interface lines of the root.
+#include "fp_refine.h"
 #include "fp_vec.h"
 
 #include <cmath>
@@ -811,6 +812,28 @@ int fp_sub_solve_f32(fp_sub sub, float* d_f, int nrhs, long long ld, void* strea
     });
 }
 
+int fp_sub_solve_refine(fp_sub sub, double* d_f, int nrhs, long long ld, double tol, int max_steps, void* stream,
+                        fp_solve_info* info)
+{
+    int extra = FP_OK;
+    int st    = guarded([&] {
+        FP_REQUIRE(sub && d_f && nrhs >= 1, "null argument");
+        FP_REQUIRE(ld >= (long long)sub->plan.nx * sub->plan.ny || nrhs <= 1, "ld smaller than nx*ny");
+        FP_REQUIRE(tol >= 0.0, "negative tolerance");
+        FP_HIP(hipSetDevice(sub->ctx->device));
+        const long long     ldd = nrhs <= 1 ? (long long)sub->plan.nx * sub->plan.ny : ld;
+        const fp::RefineStats r = fp::refine_solve(sub->plan, d_f, ldd, nrhs, tol, max_steps, (hipStream_t)stream);
+        if (info) {
+            *info                 = fp_solve_info{};
+            info->refine_steps    = r.steps;
+            info->refine_residual = r.correction;
+        }
+        if (r.numeric) extra = FP_E_NUMERIC;
+        else if (r.correction > tol) extra = FP_E_NOT_CONVERGED;
+    });
+    return st != FP_OK ? st : extra;
+}
+
 int fp_sub_solve_batched(fp_ctx ctx, int nprob, const fp_sub* subs, double* const* d_f, const int* nrhs, void* stream)
 {
     return guarded([&] {
@@ -908,12 +931,61 @@ int fp_cat_solve(fp_cat cat, double* const* d_fields, const double* const* d_gho
     return st;
 }
 
+// Adjoint of fp_cat_solve.  The solve is u = A^-1 b with b_d = hx_d^2 (f_d + C_d g_d):
+// A is the assembled scaled 5-point operator (symmetric: the interface
+// coupling is 1 in both directions), C_d maps ghost data onto the face lines
+// (Dirichlet -1/h^2, Neumann +-1/h).  Hence lambda = A^-1 dL/du (one more
+// composite solve), dL/df_d = hx_d^2 lambda_d and
+// dL/dg_{d,face} = hx_d^2 coef(face) lambda_d on that face's line.
 int fp_cat_vjp(fp_cat cat, const double* const* d_gu, double* const* d_gf, double* const* d_gghost, int nrhs,
                void* stream, fp_solve_info* info)
 {
-    (void)cat; (void)d_gu; (void)d_gf; (void)d_gghost; (void)nrhs; (void)stream; (void)info;
-    g_last_error = "fp_cat_vjp is not built yet";
-    return FP_E_UNSUPPORTED;
+    int st = guarded([&] {
+        FP_REQUIRE(cat && d_gu && d_gf && nrhs >= 1, "null argument");
+        FP_HIP(hipSetDevice(cat->ctx->device));
+        hipStream_t                     s  = (hipStream_t)stream;
+        const int                       nd = cat->nd;
+        std::vector<fp::DevBuf<double>> lam(nd);
+        std::vector<double*>            lp(nd);
+        for (int d = 0; d < nd; d++) {
+            FP_REQUIRE(d_gu[d] && d_gf[d], "null field");
+            const long long n = fp::dom_n(cat, d);
+            lam[d].alloc((size_t)n * nrhs);
+            lp[d] = lam[d].p;
+            // the solve multiplies by hx^2 after assembly: pre-divide so that b = dL/du
+            fp::launch_axpby(d_gu[d], lam[d].p, n * nrhs, 1.0 / (cat->hx[d] * cat->hx[d]), 0.0, s);
+        }
+        fp::cat_solve(cat, lp.data(), nullptr, nrhs, s);
+        for (int d = 0; d < nd; d++) {
+            const long long n   = fp::dom_n(cat, d);
+            const double    hx2 = cat->hx[d] * cat->hx[d], hx = cat->hx[d], hy = cat->hy[d];
+            fp::launch_axpby(lam[d].p, d_gf[d], n * nrhs, hx2, 0.0, s);
+            for (int face = 0; face < 4; face++) {
+                double* out = d_gghost ? d_gghost[4 * d + face] : nullptr;
+                if (!out) continue;
+                const int t = cat->bc[4 * d + face];
+                double    coef;
+                if (t == fp::DIRICHLET) coef = face < 2 ? -1.0 / hx / hx : -1.0 / hy / hy;
+                else if (t == fp::NEUMANN) {
+                    const double h = face < 2 ? hx : hy;
+                    coef           = (face == fp::XNEG || face == fp::YNEG) ? 1.0 / h : -1.0 / h;
+                } else
+                    coef = 0.0;  // Adjacented / Null faces take no ghost data
+                fp::launch_face_extract(lam[d].p, cat->nx[d], cat->ny[d], n, face, hx2 * coef, out, nrhs, s);
+            }
+        }
+        FP_HIP(hipStreamSynchronize(s));
+        if (info) {
+            info->iterations      = cat->last_steps;
+            info->restarts        = cat->last_restarts;
+            info->residual        = cat->last_res;
+            info->rel_residual    = cat->last_rel;
+            info->refine_steps    = 0;
+            info->refine_residual = 0.0;
+        }
+    });
+    if (st == FP_OK && cat && cat->last_status) st = cat->last_status;
+    return st;
 }
 
 }  // extern "C"

@@ -20,6 +20,8 @@ void launch_bond_add(T* f, int nx, int ny, long long ldf, const T* nb, int nbx, 
                      int nrhs, hipStream_t s);
 template <typename T>
 void launch_fill(T* p, long long n, T v, hipStream_t s);
+void launch_face_extract(const double* f, int nx, int ny, long long ld, int face, double a, double* out, int nrhs,
+                         hipStream_t s);
 void launch_axpby(const double* x, double* y, long long n, double a, double b, hipStream_t s);
 void launch_gather(const double* G, long long ldG, const int* cell, int K, const double* v, double* out, long long ldo,
                    double a, double b, int nrhs, hipStream_t s);

@@ -44,6 +44,28 @@ void launch_assemble(T* f, int nx, int ny, long long ld, int nrhs, T hx2, const 
 }
 template void launch_assemble<double>(double*, int, int, long long, int, double, const AsmFaces<double>&, hipStream_t);
 
+// out[r][t] = a * f[r][line cell t] for one face of an nx x ny field
+__global__ void k_face_extract(const double* f, int nx, int ny, long long ld, int face, double a, double* out)
+{
+    const int       t   = blockIdx.x * blockDim.x + threadIdx.x;
+    const long long r   = blockIdx.y;
+    const int       len = face < 2 ? ny : nx;
+    if (t >= len) return;
+    long long c;
+    if (face == XNEG) c = t;
+    else if (face == XPOS) c = (long long)(nx - 1) * ny + t;
+    else if (face == YNEG) c = (long long)t * ny;
+    else c = (long long)t * ny + ny - 1;
+    out[r * len + t] = a * f[r * ld + c];
+}
+void launch_face_extract(const double* f, int nx, int ny, long long ld, int face, double a, double* out, int nrhs,
+                         hipStream_t s)
+{
+    const int len = face < 2 ? ny : nx;
+    hipLaunchKernelGGL(k_face_extract, dim3((len + 255) / 256, nrhs), dim3(256), 0, s, f, nx, ny, ld, face, a, out);
+    FP_HIP(hipGetLastError());
+}
+
 // ---- line exchange -----------------------------------------------------------
 // field2::bond_add(location, a, neighbour) (base/field/field2.cpp:257-302):
 // this side's line += a * the neighbour's adjacent line.

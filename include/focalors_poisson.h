@@ -108,6 +108,15 @@ int fp_sub_transform(fp_sub sub, double* d_f, int nrhs, long long ld, int invers
 int fp_sub_chasing(fp_sub sub, double* d_f, int nrhs, long long ld, void* stream);
 /* The same solve on FP32 fields (FP32 transforms and sweeps). */
 int fp_sub_solve_f32(fp_sub sub, float* d_f, int nrhs, long long ld, void* stream);
+/* Mixed precision (no reference counterpart): FP32 transforms and sweeps as
+ * the inner solver, FP64 residual of the discrete operator and FP64
+ * iterative refinement.  Same in/out contract as fp_sub_solve.  Stops when
+ * the last FP64 correction is below tol relative to the solution (2-norms,
+ * every right-hand side) or after max_steps inner solves; info->refine_steps
+ * and info->refine_residual (that relative correction) report the outcome.
+ * Returns FP_E_NOT_CONVERGED if max_steps ended it above tol. */
+int fp_sub_solve_refine(fp_sub sub, double* d_f, int nrhs, long long ld, double tol, int max_steps, void* stream,
+                        fp_solve_info* info);
 /* Many independent subdomain solves of different sizes in one call: the
  * reference's loop over independent PoissonSolver2D objects.  d_f[k] holds
  * nrhs[k] fields of subs[k] (ld = nx*ny). */
@@ -133,9 +142,11 @@ int  fp_cat_root(fp_cat cat);
  * Variable2D::buffer_map values, variable2d.cpp:345-416), or NULL for none. */
 int fp_cat_solve(fp_cat cat, double* const* d_fields, const double* const* d_ghost, int nrhs, void* stream,
                  fp_solve_info* info);
-/* Adjoint of fp_cat_solve for L(u): given dL/du in d_gu[d] (nrhs fields per
- * domain), writes dL/df into d_gf[d] and dL/dg (ghost data of Dirichlet and
- * Neumann faces) into d_gghost[4*d+face] (NULL entries are skipped). */
+/* Adjoint of fp_cat_solve for a loss L(u) (no reference counterpart): given
+ * dL/du in d_gu[d] (nrhs fields per domain), writes dL/df into d_gf[d] and
+ * dL/dg for the ghost data of every face into d_gghost[4*d+face] (zero for
+ * Adjacented faces; NULL entries are skipped).  d_gghost may be NULL.
+ * Costs one composite solve: the assembled operator is symmetric. */
 int fp_cat_vjp(fp_cat cat, const double* const* d_gu, double* const* d_gf, double* const* d_gghost, int nrhs,
                void* stream, fp_solve_info* info);
 

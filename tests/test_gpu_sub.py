@@ -158,3 +158,33 @@ def test_transforms_match_reference_coefficients(ctx, bcp, ny):
         torch.cuda.synchronize()
         ref = O.transform(bcp[0], bcp[1], f, inverse=inv)
         assert np.abs(t.cpu().numpy() - ref).max() < 1e-12 * max(1.0, np.abs(ref).max())
+
+
+@pytest.mark.parametrize("nx,ny,bc", [(64, 64, (D, D, D, D)), (200, 131, (N, D, A, N)), (512, 1024, (D, A, N, N)),
+                                      (1024, 1024, (A, A, A, A))])
+def test_fp32_refine_reaches_fp64_oracle(ctx, nx, ny, bc):
+    """FP32 inner solves + FP64 refinement: every step's error vs the FP64
+    oracle is printed; the final one meets the FP64 bar of 1e-10."""
+    import torch
+    import focalors_poisson_amd as fpa
+    rng = np.random.default_rng(nx + 3 * ny)
+    h = 1.0 / max(nx, ny)
+    f = rng.standard_normal((nx, ny)) * h * h
+    ref = O.sub_solve(f, h, h, bc)
+    s = fpa.PoissonSolver2D(ctx, nx, ny, h, h, bc)
+    errs = []
+    for steps in (1, 2, 3, 6):
+        t = to_dev(f)
+        info = s.solve_refine(t, tol=0.0, max_steps=steps, allow_not_converged=True)
+        torch.cuda.synchronize()
+        errs.append(rel(t.cpu().numpy(), ref))
+        banner("refine nx=%d ny=%d steps=%d rel err %.3e (correction %.2e)" % (nx, ny, steps, errs[-1],
+                                                                            info["refine_residual"]))
+    assert errs[0] < 1e-3
+    assert errs[1] < errs[0]
+    t = to_dev(f)
+    info = s.solve_refine(t, tol=1e-13, max_steps=40)
+    torch.cuda.synchronize()
+    err = rel(t.cpu().numpy(), ref)
+    banner("refine nx=%d ny=%d converged in %d steps, rel err %.3e" % (nx, ny, info["refine_steps"], err))
+    assert err < 1e-10 and info["refine_residual"] <= 1e-13
