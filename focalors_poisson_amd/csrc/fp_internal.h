@@ -71,6 +71,8 @@ struct DevBuf {
 };
 
 constexpr int kMaxRadices = 24;
+// subdomain problems per batched launch (descriptors travel as kernel arguments)
+constexpr int kMaxBatch = 8;
 
 // A batched y transform of one subdomain size and boundary pair: the packed
 // complex FFT of length L (two field rows per FFT) with its radix schedule
@@ -118,6 +120,28 @@ void launch_ytransform(const TransformPlan<T>& tp, T* data, long long ld, int nx
 template <typename T>
 void launch_tridiag(const SubPlan& sp, const T* pinv, T* data, long long ld, int nrhs, hipStream_t s);
 
+// Batched forms: the jobs of several subdomains (any sizes and boundary
+// pairs) in as few launches as their kernels allow.
+template <typename T>
+struct TransformJob {
+    const TransformPlan<T>* plan;
+    T*                      data;
+    long long               ld;
+    int                     nx, nrhs;
+};
+template <typename T>
+void launch_ytransform_batch(const TransformJob<T>* jobs, int njobs, int inverse, hipStream_t s);
+template <typename T>
+struct TridiagJob {
+    const SubPlan* sp;
+    const T*       pinv;
+    T*             data;
+    long long      ld;
+    int            nrhs;
+};
+template <typename T>
+void launch_tridiag_batch(const TridiagJob<T>* jobs, int njobs, hipStream_t s);
+
 // Arguments of the line-sourced tridiagonal sweep (interface operator).
 template <typename T>
 struct LinesArgs {
@@ -139,5 +163,10 @@ int lines_workgroups(const SubPlan& sp);
 // Full subdomain solve of nrhs fields (transform, sweep, inverse transform).
 template <typename T>
 void sub_solve(SubPlan& sp, T* data, long long ld, int nrhs, hipStream_t s);
+// Several subdomain solves as one batch per phase: problem k solves nrhs[k]
+// fields of plans[k] at data[k] (ld[k] apart).
+template <typename T>
+void sub_solve_batch(SubPlan* const* plans, T* const* data, const long long* ld, const int* nrhs, int n,
+                     hipStream_t s);
 
 }  // namespace fp

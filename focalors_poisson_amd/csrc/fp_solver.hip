@@ -115,6 +115,29 @@ void sub_solve(SubPlan& sp, T* data, long long ld, int nrhs, hipStream_t s)
 template void sub_solve<double>(SubPlan&, double*, long long, int, hipStream_t);
 template void sub_solve<float>(SubPlan&, float*, long long, int, hipStream_t);
 
+template <typename T>
+void sub_solve_batch(SubPlan* const* plans, T* const* data, const long long* ld, const int* nrhs, int n,
+                     hipStream_t s)
+{
+    std::vector<TransformJob<T>> tj;
+    std::vector<TridiagJob<T>>   rj;
+    tj.reserve(n);
+    rj.reserve(n);
+    for (int k = 0; k < n; k++) {
+        if (nrhs[k] <= 0) continue;
+        SubPlan& sp = *plans[k];
+        if constexpr (sizeof(T) == 4) sp.ensure_f32();
+        tj.push_back({&tplan<T>(sp), data[k], ld[k], sp.nx, nrhs[k]});
+        rj.push_back({&sp, pinv_of<T>(sp), data[k], ld[k], nrhs[k]});
+    }
+    if (tj.empty()) return;
+    launch_ytransform_batch<T>(tj.data(), (int)tj.size(), 0, s);
+    launch_tridiag_batch<T>(rj.data(), (int)rj.size(), s);
+    launch_ytransform_batch<T>(tj.data(), (int)tj.size(), 1, s);
+}
+template void sub_solve_batch<double>(SubPlan* const*, double* const*, const long long*, const int*, int, hipStream_t);
+template void sub_solve_batch<float>(SubPlan* const*, float* const*, const long long*, const int*, int, hipStream_t);
+
 // ---------------------------------------------------------------------------
 // composite solver
 
@@ -134,7 +157,7 @@ struct Node {
     int                    nu = 0;
     // GMRES workspace, sized for ws_nrhs systems
     int            ws_nrhs = 0, ws_m = 0, nblk = 1;
-    DevBuf<double> V, Wv, Tv, X, C, H, Hacc, cs, sn, g, Y, part, part2, tolabs, resid, G;
+    DevBuf<double> V, Wv, Tv, X, C, H, Hacc, cs, sn, g, Y, part, part2, tolabs, resid, rnorm, G;
     // structured interface operator R A^-1 E of this node's subdomain
     int            face_off[4] = {-1, -1, -1, -1};
     int            has_x = 0, has_y = 0, nwg = 0, nxl = 0;  // nxl: x-line segments (first in the vector)
@@ -146,7 +169,7 @@ struct Node {
     DevBuf<int>    kface, kpos;
     DevBuf<int>    state;
     int*           h_state = nullptr;
-    double*        h_resid = nullptr;
+    double*        h_resid = nullptr;  // nrhs residual norms, then nrhs initial norms
     ~Node()
     {
         if (h_state) (void)hipHostFree(h_state);
@@ -234,6 +257,7 @@ static void ensure_ws(fp_cat_s* c, Node& nd, int nrhs)
     nd.part2.alloc((size_t)nrhs * (m + 1));
     nd.tolabs.alloc(nrhs);
     nd.resid.alloc(nrhs);
+    nd.rnorm.alloc(nrhs);
     nd.state.alloc((size_t)nrhs * 4);
     nd.G.alloc((size_t)nrhs * n);
     nd.LB.alloc((size_t)nrhs * nd.ldl);
@@ -245,7 +269,7 @@ static void ensure_ws(fp_cat_s* c, Node& nd, int nrhs)
     if (nd.h_state) (void)hipHostFree(nd.h_state);
     if (nd.h_resid) (void)hipHostFree(nd.h_resid);
     FP_HIP(hipHostMalloc((void**)&nd.h_state, sizeof(int) * 4 * nrhs));
-    FP_HIP(hipHostMalloc((void**)&nd.h_resid, sizeof(double) * nrhs));
+    FP_HIP(hipHostMalloc((void**)&nd.h_resid, sizeof(double) * 2 * nrhs));
     nd.ws_nrhs = nrhs;
     nd.ws_m    = m;
 }
@@ -256,7 +280,7 @@ static GmresDev gdev(fp_cat_s* c, Node& nd, int nrhs)
     g.K = nd.K; g.nrhs = nrhs; g.m = c->m; g.nblk = nd.nblk;
     g.V = nd.V.p; g.Wv = nd.Wv.p; g.X = nd.X.p; g.C = nd.C.p; g.H = nd.H.p; g.Hacc = nd.Hacc.p;
     g.cs = nd.cs.p; g.sn = nd.sn.p; g.g = nd.g.p; g.Y = nd.Y.p; g.part = nd.part.p; g.part2 = nd.part2.p;
-    g.tolabs = nd.tolabs.p; g.resid = nd.resid.p; g.state = nd.state.p;
+    g.tolabs = nd.tolabs.p; g.resid = nd.resid.p; g.rnorm = nd.rnorm.p; g.state = nd.state.p;
     return g;
 }
 
@@ -378,6 +402,7 @@ static void read_state(Node& nd, int nrhs, hipStream_t s)
 {
     FP_HIP(hipMemcpyAsync(nd.h_state, nd.state.p, sizeof(int) * 4 * nrhs, hipMemcpyDeviceToHost, s));
     FP_HIP(hipMemcpyAsync(nd.h_resid, nd.resid.p, sizeof(double) * nrhs, hipMemcpyDeviceToHost, s));
+    FP_HIP(hipMemcpyAsync(nd.h_resid + nrhs, nd.rnorm.p, sizeof(double) * nrhs, hipMemcpyDeviceToHost, s));
     FP_HIP(hipStreamSynchronize(s));
 }
 
@@ -396,13 +421,13 @@ static void gmres_solve(fp_cat_s* c, Node& nd, double* b, int nrhs, hipStream_t 
     FP_HIP(hipMemcpyAsync(nd.X.p, nd.C.p, sizeof(double) * K * nrhs, hipMemcpyDeviceToDevice, s));
     const GmresDev g = gdev(c, nd, nrhs);
     gm_init(g, c->tol, c->tol_rel, s);
-    read_state(nd, nrhs, s);
-    std::vector<double> rhsnorm(nd.h_resid, nd.h_resid + nrhs);
-    int                 restarts = 0, status = 0;
+    int  restarts = 0, status = 0;
+    bool fresh    = false;  // host copy of the state is current
     for (int outer = 0; outer < c->maxiter; outer++) {
         matvec(c, nd, nd.X.p, nd.Wv.p, nrhs, s);
         gm_resid(g, s);
         read_state(nd, nrhs, s);
+        fresh         = true;
         bool all_done = true;
         for (int r = 0; r < nrhs; r++)
             if (nd.h_state[4 * r + 2] == 0) all_done = false;
@@ -411,7 +436,9 @@ static void gmres_solve(fp_cat_s* c, Node& nd, double* b, int nrhs, hipStream_t 
         for (int j = 0; j < m; j++) {
             op(c, nd, nd.V.p + j * ldv, nd.Wv.p, nrhs, s);
             gm_arnoldi(g, j, s);
-            if ((j + 1) % c->check_every == 0 || j == m - 1) {
+            // the first step is checked at once (an exact preconditioner
+            // converges there), later ones every check_every steps
+            if (j == 0 || (j + 1) % c->check_every == 0 || j == m - 1) {
                 read_state(nd, nrhs, s);
                 bool any = false;
                 for (int r = 0; r < nrhs; r++)
@@ -428,15 +455,17 @@ static void gmres_solve(fp_cat_s* c, Node& nd, double* b, int nrhs, hipStream_t 
             gm_finish_cycle(gz, s);
             launch_dense_gemv(nd.Minv.p, K, nd.Zv.p, nd.X.p, 1.0, 1.0, nrhs, s);
         }
+        fresh = false;
         restarts++;
     }
-    read_state(nd, nrhs, s);
+    if (!fresh) read_state(nd, nrhs, s);
     int    steps = 0;
     double res = 0.0, rel = 0.0;
     for (int r = 0; r < nrhs; r++) {
+        const double r0 = nd.h_resid[nrhs + r];
         steps = std::max(steps, nd.h_state[4 * r + 3]);
         res   = std::max(res, nd.h_resid[r]);
-        rel   = std::max(rel, rhsnorm[r] > 0 ? nd.h_resid[r] / rhsnorm[r] : nd.h_resid[r]);
+        rel   = std::max(rel, r0 > 0 ? nd.h_resid[r] / r0 : nd.h_resid[r]);
         if (nd.h_state[4 * r + 2] == 2) status = FP_E_NUMERIC;
         else if (nd.h_state[4 * r + 2] != 1 && status == 0) status = FP_E_NOT_CONVERGED;
     }
@@ -461,6 +490,26 @@ static void node_solve(fp_cat_s* c, int d, double* f, int nrhs, hipStream_t s)
     Node& nd = *c->nodes[d];
     if (nd.ch.empty()) sub_solve<double>(nd.plan, f, dom_n(c, d), nrhs, s);
     else gmres_solve(c, nd, f, nrhs, s);
+}
+
+// The solves of one tree level: the leaves of the level in one batch (one
+// launch per solve phase for all of them), then each non-leaf's GMRES.
+static void level_solve(fp_cat_s* c, const std::vector<int>& doms, double* const* f, int nrhs, hipStream_t s)
+{
+    std::vector<SubPlan*>  plans;
+    std::vector<double*>   data;
+    std::vector<long long> ld;
+    std::vector<int>       nr;
+    for (int d : doms)
+        if (c->nodes[d]->ch.empty()) {
+            plans.push_back(&c->nodes[d]->plan);
+            data.push_back(f[d]);
+            ld.push_back(dom_n(c, d));
+            nr.push_back(nrhs);
+        }
+    if (!plans.empty()) sub_solve_batch<double>(plans.data(), data.data(), ld.data(), nr.data(), (int)plans.size(), s);
+    for (int d : doms)
+        if (!c->nodes[d]->ch.empty()) gmres_solve(c, *c->nodes[d], f[d], nrhs, s);
 }
 
 // GMRESSolver2D::schur_mat_construct (gmres_solver2d.cpp:39-89): S of each
@@ -671,7 +720,9 @@ static void cat_solve(fp_cat_s* c, double* const* f, const double* const* ghost,
                                         dom_n(c, ch), loc, -1.0, nrhs, s);
             }
         }
-        for (int d : c->levels[l]) node_solve(c, d, c->temp[d].p, nrhs, s);
+        std::vector<double*> tp(c->nd, nullptr);
+        for (int d : c->levels[l]) tp[d] = c->temp[d].p;
+        level_solve(c, c->levels[l], tp.data(), nrhs, s);
     }
     const int r = c->root;
     for (int loc = 0; loc < 4; loc++) {
@@ -689,7 +740,7 @@ static void cat_solve(fp_cat_s* c, double* const* f, const double* const* ghost,
             launch_bond_add<double>(f[d], c->nx[d], c->ny[d], dom_n(c, d), f[p], c->nx[p], c->ny[p], dom_n(c, p),
                                     c->parent_loc[d], -1.0, nrhs, s);
         }
-        for (int d : c->levels[l]) node_solve(c, d, f[d], nrhs, s);
+        level_solve(c, c->levels[l], f, nrhs, s);
     }
 }
 
@@ -717,6 +768,23 @@ int guarded(F&& fn)
         g_last_error = e.what();
         return FP_E_INVALID;
     }
+}
+}  // namespace
+
+namespace {
+template <typename T>
+void solve_batched(fp_ctx ctx, int nprob, const fp_sub* subs, T* const* d_f, const int* nrhs, void* stream)
+{
+    FP_REQUIRE(ctx && nprob >= 0 && (nprob == 0 || (subs && d_f && nrhs)), "null argument");
+    FP_HIP(hipSetDevice(ctx->device));
+    std::vector<fp::SubPlan*> plans(nprob);
+    std::vector<long long>    ld(nprob);
+    for (int k = 0; k < nprob; k++) {
+        FP_REQUIRE(subs[k] && d_f[k] && nrhs[k] >= 0, "null problem");
+        plans[k] = &subs[k]->plan;
+        ld[k]    = (long long)subs[k]->plan.nx * subs[k]->plan.ny;
+    }
+    fp::sub_solve_batch<T>(plans.data(), d_f, ld.data(), nrhs, nprob, (hipStream_t)stream);
 }
 }  // namespace
 
@@ -836,29 +904,13 @@ int fp_sub_solve_refine(fp_sub sub, double* d_f, int nrhs, long long ld, double 
 
 int fp_sub_solve_batched(fp_ctx ctx, int nprob, const fp_sub* subs, double* const* d_f, const int* nrhs, void* stream)
 {
-    return guarded([&] {
-        FP_REQUIRE(ctx && (nprob == 0 || (subs && d_f && nrhs)), "null argument");
-        FP_HIP(hipSetDevice(ctx->device));
-        for (int k = 0; k < nprob; k++) {
-            FP_REQUIRE(subs[k] && d_f[k], "null problem");
-            fp::sub_solve<double>(subs[k]->plan, d_f[k], (long long)subs[k]->plan.nx * subs[k]->plan.ny, nrhs[k],
-                                  (hipStream_t)stream);
-        }
-    });
+    return guarded([&] { solve_batched<double>(ctx, nprob, subs, d_f, nrhs, stream); });
 }
 
 int fp_sub_solve_batched_f32(fp_ctx ctx, int nprob, const fp_sub* subs, float* const* d_f, const int* nrhs,
                              void* stream)
 {
-    return guarded([&] {
-        FP_REQUIRE(ctx && (nprob == 0 || (subs && d_f && nrhs)), "null argument");
-        FP_HIP(hipSetDevice(ctx->device));
-        for (int k = 0; k < nprob; k++) {
-            FP_REQUIRE(subs[k] && d_f[k], "null problem");
-            fp::sub_solve<float>(subs[k]->plan, d_f[k], (long long)subs[k]->plan.nx * subs[k]->plan.ny, nrhs[k],
-                                 (hipStream_t)stream);
-        }
-    });
+    return guarded([&] { solve_batched<float>(ctx, nprob, subs, d_f, nrhs, stream); });
 }
 
 int fp_cat_create(fp_ctx ctx, const fp_geometry_desc* g, int m, double tol, int tol_is_relative, int max_iter,

@@ -21,6 +21,7 @@
 #include "fp_dft_tables.h"
 #include "fp_internal.h"
 
+#include <algorithm>
 #include <cmath>
 
 namespace fp {
@@ -54,6 +55,16 @@ struct YArgs {
     const C2<T>* W;
     const C2<T>* aux;
     T            c0, c1;
+};
+
+// Up to kMaxBatch transforms of different subdomains in one launch, passed
+// by value: workgroups start[q] .. start[q+1]-1 belong to problem q, one per
+// (row pair, right-hand side).
+template <typename T>
+struct YBatch {
+    int      nprob;
+    int      start[kMaxBatch + 1];
+    YArgs<T> p[kMaxBatch];
 };
 
 // Twiddle factors exp(-2 pi i t / L) from two small LDS tables:
@@ -288,9 +299,19 @@ __device__ C2<T>* run_fft(const YArgs<T>& a, const Tw<T> W, C2<T>* A, C2<T>* B)
 template <typename T>
 __device__ __forceinline__ T sgn_pow(int p) { return (p & 1) ? T(-1) : T(1); }
 
+// values per thread and row held in registers by the load and store phases
+constexpr int VPT = 8;
+
 template <int NT, int P1, typename T>
-__global__ __launch_bounds__(NT) void k_ytransform(YArgs<T> a)
+__global__ __launch_bounds__(NT) void k_ytransform(YBatch<T> bt)
 {
+    int q = 0;
+    while (q + 1 < bt.nprob && (int)blockIdx.x >= bt.start[q + 1]) q++;
+    const YArgs<T>& a      = bt.p[q];
+    const int       npairs = (a.nx + 1) >> 1;
+    const int       local  = (int)blockIdx.x - bt.start[q];
+    const int       rhs    = local / npairs, pair = local - rhs * npairs;
+
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C2<T>* A   = reinterpret_cast<C2<T>*>(smem_raw);
     C2<T>* B   = A + a.L;
@@ -302,55 +323,61 @@ __global__ __launch_bounds__(NT) void k_ytransform(YArgs<T> a)
     const Tw<T> W{tlo, thi};
 
     const int  N = a.N, L = a.L;
-    const int  ra = 2 * blockIdx.x, rb = ra + 1;
+    const int  ra = 2 * pair, rb = ra + 1;
     const bool hb = rb < a.nx;
-    T*         pa = a.data + (long long)blockIdx.y * a.ld + (long long)ra * N;
+    T*         pa = a.data + (long long)rhs * a.ld + (long long)ra * N;
     T*         pb = pa + N;
 
     // ---- load: build the packed extended sequence in A ----
-    if (a.kind == T_DD) {
-        for (int p = threadIdx.x; p < L; p += NT) {
-            C2<T> z = {T(0), T(0)};
-            if (p >= 1 && p <= N) z = {pa[p - 1], hb ? pb[p - 1] : T(0)};
-            else if (p >= N + 2) z = {-pa[L - 1 - p], hb ? -pb[L - 1 - p] : T(0)};
-            A[p] = z;
+    // The rows are read in chunks of VPT values per thread with every global
+    // load of a chunk issued before the first LDS write, so the HBM latency
+    // is paid once per chunk instead of once per element.
+    const int kind = a.kind, inv = a.inverse;
+    if (kind == T_DD) {
+        if (threadIdx.x == 0) A[0] = {T(0), T(0)};
+        if (threadIdx.x == 1) A[N + 1] = {T(0), T(0)};
+    } else if (kind == T_NN && inv) {
+        if (threadIdx.x == 0) A[N] = {T(0), T(0)};
+    } else if (kind != T_NN) {
+        if (threadIdx.x == 0) A[0] = {T(0), T(0)};
+        for (int p = N + 1 + threadIdx.x; p < L; p += NT) A[p] = {T(0), T(0)};
+    }
+    for (int base = 0; base < N; base += VPT * NT) {
+        T va[VPT], vb[VPT];
+#pragma unroll
+        for (int k = 0; k < VPT; k++) {
+            const int n = base + k * NT + threadIdx.x;
+            va[k]       = n < N ? pa[n] : T(0);
+            vb[k]       = (n < N && hb) ? pb[n] : T(0);
         }
-    } else if (a.kind == T_NN && !a.inverse) {
-        for (int p = threadIdx.x; p < L; p += NT) {
-            const int n = p < N ? p : 2 * N - 1 - p;
-            A[p]        = {pa[n], hb ? pb[n] : T(0)};
-        }
-    } else if (a.kind == T_NN) {
-        for (int p = threadIdx.x; p < L; p += NT) {
-            C2<T> z = {T(0), T(0)};
-            if (p != N) {
-                const int   m  = p < N ? p : 2 * N - p;
+#pragma unroll
+        for (int k = 0; k < VPT; k++) {
+            const int n = base + k * NT + threadIdx.x;
+            if (n >= N) break;
+            const T x = va[k], y = vb[k];
+            if (kind == T_DD) {
+                A[n + 1]     = {x, y};
+                A[L - 1 - n] = {-x, -y};
+            } else if (kind == T_NN && !inv) {
+                A[n]             = {x, y};
+                A[2 * N - 1 - n] = {x, y};
+            } else if (kind == T_NN) {
+                const int   m  = N - 1 - n;
                 const T     s  = m == 0 ? a.c1 : a.c0;  // sqrt(1/N) or -sqrt(2/N)
-                const T     ya = s * pa[N - 1 - m], yb = hb ? s * pb[N - 1 - m] : T(0);
+                const T     ya = s * x, yb = s * y;
                 const C2<T> e  = a.aux[m];              // exp(-i pi m / 2N); Y_m uses conj(e)
-                if (p < N) z = {ya * e.x + yb * e.y, -ya * e.y + yb * e.x};
-                else z = {ya * e.x - yb * e.y, ya * e.y + yb * e.x};
-                if (p == 0) {
+                A[m]           = {ya * e.x + yb * e.y, -ya * e.y + yb * e.x};
+                if (m > 0) A[2 * N - m] = {ya * e.x - yb * e.y, ya * e.y + yb * e.x};
+                else {
                     y0[0] = ya;
                     y0[1] = yb;
                 }
+            } else if (kind == T_ND && inv) {
+                A[n + 1] = {x, y};
+            } else {  // DN both ways, ND forward
+                const T s = sgn_pow<T>(n + 1) * a.c0;
+                A[n + 1]  = {s * x, s * y};
             }
-            A[p] = z;
-        }
-    } else if (a.kind == T_ND && a.inverse) {
-        for (int p = threadIdx.x; p < L; p += NT) {
-            C2<T> z = {T(0), T(0)};
-            if (p >= 1 && p <= N) z = {pa[p - 1], hb ? pb[p - 1] : T(0)};
-            A[p] = z;
-        }
-    } else {  // DN both ways, ND forward
-        for (int p = threadIdx.x; p < L; p += NT) {
-            C2<T> z = {T(0), T(0)};
-            if (p >= 1 && p <= N) {
-                const T s = sgn_pow<T>(p) * a.c0;
-                z         = {s * pa[p - 1], hb ? s * pb[p - 1] : T(0)};
-            }
-            A[p] = z;
         }
     }
     __syncthreads();
@@ -359,42 +386,61 @@ __global__ __launch_bounds__(NT) void k_ytransform(YArgs<T> a)
     const C2<T>* Z   = fwd ? run_fft<P1, true, NT>(a, W, A, B) : run_fft<P1, false, NT>(a, W, A, B);
 
     // ---- store: split the two rows' coefficients ----
-    for (int j = threadIdx.x; j < N; j += NT) {
-        T oa, ob;
-        if (a.kind == T_DD) {
-            const C2<T> z = Z[j + 1];
-            oa            = -a.c0 * z.y;
-            ob            = a.c0 * z.x;
-        } else if (a.kind == T_NN && !a.inverse) {
-            const int   m = N - 1 - j;
-            const C2<T> w = cmul(Z[m], a.aux[m]);  // 2 C_a(m) + 2i C_b(m)
-            const T     s = T(0.5) * ((j == N - 1) ? a.c1 : a.c0);
-            oa            = s * w.x;
-            ob            = s * w.y;
-        } else if (a.kind == T_NN) {
-            const C2<T> z = Z[j];
-            oa            = T(0.5) * (z.x + y0[0]);
-            ob            = T(0.5) * (z.y + y0[1]);
-        } else if (a.kind == T_DN) {
-            const C2<T> zm = Z[j + 1], zc = Z[L - 1 - j];
-            const T     s  = sgn_pow<T>(j) * T(0.5);
-            oa             = s * (zm.y - zc.y);
-            ob             = s * (zc.x - zm.x);
-        } else if (!a.inverse) {  // ND forward
-            const C2<T> zm = Z[j + 1], zc = Z[L - 1 - j], sc = a.aux[j];
-            const T     rea = T(0.5) * (zm.x + zc.x), ima = T(0.5) * (zm.y - zc.y);
-            const T     reb = T(0.5) * (zm.y + zc.y), imb = T(0.5) * (zc.x - zm.x);
-            oa              = sc.x * rea - sc.y * ima;
-            ob              = sc.x * reb - sc.y * imb;
-        } else {  // ND inverse
-            const int   k  = N - j;
-            const C2<T> zk = Z[k], zc = Z[L - k];
-            const T     s  = sgn_pow<T>(j + 1) * a.c0 * T(0.5);
-            oa             = s * (zk.y - zc.y);
-            ob             = s * (zc.x - zk.x);
+    for (int base = 0; base < N; base += VPT * NT) {
+        C2<T> sc[VPT];
+        if (kind == T_NN && !inv) {
+#pragma unroll
+            for (int k = 0; k < VPT; k++) {
+                const int j = base + k * NT + threadIdx.x;
+                sc[k]       = j < N ? a.aux[N - 1 - j] : C2<T>{T(0), T(0)};
+            }
+        } else if (kind == T_ND && !inv) {
+#pragma unroll
+            for (int k = 0; k < VPT; k++) {
+                const int j = base + k * NT + threadIdx.x;
+                sc[k]       = j < N ? a.aux[j] : C2<T>{T(0), T(0)};
+            }
         }
-        pa[j] = oa;
-        if (hb) pb[j] = ob;
+#pragma unroll
+        for (int k = 0; k < VPT; k++) {
+            const int j = base + k * NT + threadIdx.x;
+            if (j >= N) break;
+            T oa, ob;
+            if (kind == T_DD) {
+                const C2<T> z = Z[j + 1];
+                oa            = -a.c0 * z.y;
+                ob            = a.c0 * z.x;
+            } else if (kind == T_NN && !inv) {
+                const int   m = N - 1 - j;
+                const C2<T> w = cmul(Z[m], sc[k]);  // 2 C_a(m) + 2i C_b(m)
+                const T     s = T(0.5) * ((j == N - 1) ? a.c1 : a.c0);
+                oa            = s * w.x;
+                ob            = s * w.y;
+            } else if (kind == T_NN) {
+                const C2<T> z = Z[j];
+                oa            = T(0.5) * (z.x + y0[0]);
+                ob            = T(0.5) * (z.y + y0[1]);
+            } else if (kind == T_DN) {
+                const C2<T> zm = Z[j + 1], zc = Z[L - 1 - j];
+                const T     s  = sgn_pow<T>(j) * T(0.5);
+                oa             = s * (zm.y - zc.y);
+                ob             = s * (zc.x - zm.x);
+            } else if (!inv) {  // ND forward
+                const C2<T> zm = Z[j + 1], zc = Z[L - 1 - j];
+                const T     rea = T(0.5) * (zm.x + zc.x), ima = T(0.5) * (zm.y - zc.y);
+                const T     reb = T(0.5) * (zm.y + zc.y), imb = T(0.5) * (zc.x - zm.x);
+                oa              = sc[k].x * rea - sc[k].y * ima;
+                ob              = sc[k].x * reb - sc[k].y * imb;
+            } else {  // ND inverse
+                const int   kk = N - j;
+                const C2<T> zk = Z[kk], zc = Z[L - kk];
+                const T     s  = sgn_pow<T>(j + 1) * a.c0 * T(0.5);
+                oa             = s * (zk.y - zc.y);
+                ob             = s * (zc.x - zk.x);
+            }
+            pa[j] = oa;
+            if (hb) pb[j] = ob;
+        }
     }
 }
 
@@ -487,14 +533,13 @@ template struct TransformPlan<double>;
 template struct TransformPlan<float>;
 
 template <typename T>
-void launch_ytransform(const TransformPlan<T>& tp, T* data, long long ld, int nx, int nrhs, int inverse,
-                       hipStream_t s)
+static YArgs<T> yargs(const TransformJob<T>& j, int inverse)
 {
-    constexpr int NT = 256;
-    YArgs<T>      a{};
-    a.data    = data;
-    a.ld      = ld;
-    a.nx      = nx;
+    const TransformPlan<T>& tp = *j.plan;
+    YArgs<T>                a{};
+    a.data    = j.data;
+    a.ld      = j.ld;
+    a.nx      = j.nx;
     a.N       = tp.N;
     a.L       = tp.L;
     a.kind    = tp.kind;
@@ -505,10 +550,37 @@ void launch_ytransform(const TransformPlan<T>& tp, T* data, long long ld, int nx
     a.aux = reinterpret_cast<const C2<T>*>(tp.aux.p);
     a.c0  = tp.c0;
     a.c1  = tp.c1;
-    const size_t lds = (2 * (size_t)tp.L + 64 + tp.L / 64 + 1) * sizeof(C2<T>) + 4 * sizeof(T);
+    return a;
+}
+
+template <typename T>
+static size_t ytransform_lds(int L)
+{
+    return (2 * (size_t)L + 64 + L / 64 + 1) * sizeof(C2<T>) + 4 * sizeof(T);
+}
+
+// one launch of up to kMaxBatch jobs with the register codelet P1
+template <typename T>
+static void launch_group(const TransformJob<T>* const* jobs, int n, int P1, int inverse, hipStream_t s)
+{
+    constexpr int NT = 256;
+    YBatch<T>     bt{};
+    size_t        lds = 0;
+    int           blocks = 0;
+    bt.nprob           = n;
+    for (int q = 0; q < n; q++) {
+        const TransformJob<T>& j = *jobs[q];
+        bt.p[q]                  = yargs(j, inverse);
+        bt.start[q]              = blocks;
+        const long long nb       = (long long)((j.nx + 1) / 2) * j.nrhs;
+        if (blocks + nb > (1LL << 30)) throw Error(FP_E_INVALID, "too many transform workgroups in one launch");
+        blocks += (int)nb;
+        lds = std::max(lds, ytransform_lds<T>(j.plan->L));
+    }
+    bt.start[n] = blocks;
     if (lds > 160 * 1024) throw Error(FP_E_INVALID, "transform length exceeds the LDS of one workgroup");
-    dim3 grid((nx + 1) / 2, nrhs);
-    switch (tp.p1) {
+    if (blocks == 0) return;
+    switch (P1) {
 #define FP_P1_CASE(P)                                                                                            \
     case P: {                                                                                                    \
         static bool attr_set = false;                                                                            \
@@ -517,7 +589,7 @@ void launch_ytransform(const TransformPlan<T>& tp, T* data, long long ld, int nx
                                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));                 \
             attr_set = true;                                                                                     \
         }                                                                                                        \
-        hipLaunchKernelGGL((k_ytransform<NT, P, T>), grid, dim3(NT), lds, s, a);                                 \
+        hipLaunchKernelGGL((k_ytransform<NT, P, T>), dim3(blocks), dim3(NT), lds, s, bt);                        \
         break;                                                                                                   \
     }
         FP_P1_CASE(0)
@@ -538,6 +610,50 @@ void launch_ytransform(const TransformPlan<T>& tp, T* data, long long ld, int nx
     FP_HIP(hipGetLastError());
 }
 
+// Jobs are grouped by their large-prime codelet (jobs without one join the
+// first group) and launched kMaxBatch at a time.
+template <typename T>
+void launch_ytransform_batch(const TransformJob<T>* jobs, int njobs, int inverse, hipStream_t s)
+{
+    std::vector<int>                           keys;
+    std::vector<std::vector<const TransformJob<T>*>> groups;
+    std::vector<const TransformJob<T>*>        plain;
+    for (int k = 0; k < njobs; k++) {
+        if (jobs[k].nrhs <= 0 || jobs[k].nx <= 0) continue;
+        const int p1 = jobs[k].plan->p1;
+        if (p1 == 0) {
+            plain.push_back(&jobs[k]);
+            continue;
+        }
+        size_t g = 0;
+        while (g < keys.size() && keys[g] != p1) g++;
+        if (g == keys.size()) {
+            keys.push_back(p1);
+            groups.emplace_back();
+        }
+        groups[g].push_back(&jobs[k]);
+    }
+    if (groups.empty()) {
+        keys.push_back(0);
+        groups.emplace_back();
+    }
+    groups[0].insert(groups[0].end(), plain.begin(), plain.end());
+    for (size_t g = 0; g < groups.size(); g++)
+        for (size_t q0 = 0; q0 < groups[g].size(); q0 += kMaxBatch)
+            launch_group<T>(groups[g].data() + q0, (int)std::min<size_t>(kMaxBatch, groups[g].size() - q0), keys[g],
+                            inverse, s);
+}
+
+template <typename T>
+void launch_ytransform(const TransformPlan<T>& tp, T* data, long long ld, int nx, int nrhs, int inverse,
+                       hipStream_t s)
+{
+    const TransformJob<T> j{&tp, data, ld, nx, nrhs};
+    launch_ytransform_batch<T>(&j, 1, inverse, s);
+}
+
+template void launch_ytransform_batch<double>(const TransformJob<double>*, int, int, hipStream_t);
+template void launch_ytransform_batch<float>(const TransformJob<float>*, int, int, hipStream_t);
 template void launch_ytransform<double>(const TransformPlan<double>&, double*, long long, int, int, int, hipStream_t);
 template void launch_ytransform<float>(const TransformPlan<float>&, float*, long long, int, int, int, hipStream_t);
 

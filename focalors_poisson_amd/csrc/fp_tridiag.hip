@@ -20,6 +20,8 @@
 // x_{i+1} = f_i - x_{i-1} + 2 x_i) is two prefix sums and uses the same chain.
 #include "fp_internal.h"
 
+#include <algorithm>
+
 namespace fp {
 
 // Start values of the CH-row chunks of each mode: the chunk maps
@@ -70,11 +72,32 @@ __device__ void chain_chunks(const T* SA, const T* SB, T* SY, int P, int MB, int
     }
 }
 
+// One subdomain's sweep in a batched launch.
+template <typename T>
+struct TriProb {
+    T*        data;
+    long long ld;
+    const T*  pinv;
+    int       nx, ny, ldp, P, MB, singular_last, nblk;  // nblk: mode blocks per right-hand side
+};
+template <typename T>
+struct TriBatch {
+    int        nprob;
+    int        start[kMaxBatch + 1];
+    TriProb<T> p[kMaxBatch];
+};
+
 template <typename T, int CH>
-__global__ __launch_bounds__(1024) void k_tridiag(T* __restrict__ data, long long ld, int nx, int ny,
-                                                  const T* __restrict__ pinv, int ldp, int P, int MB,
-                                                  int singular_last)
+__global__ __launch_bounds__(1024) void k_tridiag(TriBatch<T> bt)
 {
+    int q = 0;
+    while (q + 1 < bt.nprob && (int)blockIdx.x >= bt.start[q + 1]) q++;
+    const TriProb<T>& pr  = bt.p[q];
+    const int         nx = pr.nx, ny = pr.ny, ldp = pr.ldp, P = pr.P, MB = pr.MB, singular_last = pr.singular_last;
+    const T* __restrict__ pinv = pr.pinv;
+    const int local = (int)blockIdx.x - bt.start[q];
+    const int rhs = local / pr.nblk, mblk = local - rhs * pr.nblk;
+
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* SA = reinterpret_cast<T*>(smem_raw);
     T* SB = SA + P * MB;
@@ -83,9 +106,9 @@ __global__ __launch_bounds__(1024) void k_tridiag(T* __restrict__ data, long lon
     const int  tid = threadIdx.x;
     const int  jl  = tid % MB;
     const int  c   = tid / MB;
-    const int  j   = blockIdx.x * MB + jl;
+    const int  j   = mblk * MB + jl;
     const bool act = (c < P) && (j < ny);
-    T*         f   = data + (long long)blockIdx.y * ld;
+    T*         f   = pr.data + (long long)rhs * pr.ld;
     const int  i0  = c * CH;
 
     T fv[CH], pv[CH];
@@ -121,7 +144,7 @@ __global__ __launch_bounds__(1024) void k_tridiag(T* __restrict__ data, long lon
         SB[c * MB + jl] = B;
     }
     __syncthreads();
-    chain_chunks(SA, SB, SY, P, MB, blockIdx.x * MB, ny, 0, SINGJ);
+    chain_chunks(SA, SB, SY, P, MB, mblk * MB, ny, 0, SINGJ);
     __syncthreads();
     T y = (c < P) ? SY[c * MB + jl] : T(0);
     if (!sing) {
@@ -161,7 +184,7 @@ __global__ __launch_bounds__(1024) void k_tridiag(T* __restrict__ data, long lon
         SB[c * MB + jl] = B;
     }
     __syncthreads();
-    chain_chunks(SA, SB, SY, P, MB, blockIdx.x * MB, ny, 1, SINGJ);
+    chain_chunks(SA, SB, SY, P, MB, mblk * MB, ny, 1, SINGJ);
     __syncthreads();
     T x = (c < P) ? SY[c * MB + jl] : T(0);
     if (!sing) {
@@ -311,28 +334,68 @@ __global__ __launch_bounds__(1024) void k_tridiag_lines(LinesArgs<T> a, int P, i
     }
 }
 
+static int tridiag_ch(int nx) { return nx <= 1024 ? 16 : 32; }
+
 template <typename T, int CH>
-static void launch_tridiag_ch(const SubPlan& sp, const T* pinv, T* data, long long ld, int nrhs, hipStream_t s)
+static void launch_tridiag_group(const TridiagJob<T>* const* jobs, int n, hipStream_t s)
 {
-    const int P = (sp.nx + CH - 1) / CH;
-    int       MB = 1;
-    while (MB * 2 * P <= 1024 && MB < sp.mb_field) MB *= 2;
-    if (MB * P > 1024) throw Error(FP_E_INVALID, "nx too large for the tridiagonal kernel");
-    const size_t lds = 3 * (size_t)P * MB * sizeof(T);
-    dim3         grid((sp.ny + MB - 1) / MB, nrhs);
-    hipLaunchKernelGGL((k_tridiag<T, CH>), grid, dim3((MB * P + 63) / 64 * 64), lds, s, data, ld, sp.nx, sp.ny, pinv,
-                       sp.ldp, P, MB,
-                       sp.no_dirichlet ? 1 : 0);
+    TriBatch<T> bt{};
+    int         blocks = 0, threads = 64;
+    size_t      lds = 0;
+    bt.nprob        = n;
+    for (int q = 0; q < n; q++) {
+        const TridiagJob<T>& jb = *jobs[q];
+        const SubPlan&       sp = *jb.sp;
+        const int            P  = (sp.nx + CH - 1) / CH;
+        int                  MB = 1;
+        while (MB * 2 * P <= 1024 && MB < sp.mb_field) MB *= 2;
+        if (MB * P > 1024) throw Error(FP_E_INVALID, "nx too large for the tridiagonal kernel");
+        TriProb<T>& pr   = bt.p[q];
+        pr.data          = jb.data;
+        pr.ld            = jb.ld;
+        pr.pinv          = jb.pinv;
+        pr.nx            = sp.nx;
+        pr.ny            = sp.ny;
+        pr.ldp           = sp.ldp;
+        pr.P             = P;
+        pr.MB            = MB;
+        pr.singular_last = sp.no_dirichlet ? 1 : 0;
+        pr.nblk          = (sp.ny + MB - 1) / MB;
+        bt.start[q]      = blocks;
+        const long long nb = (long long)pr.nblk * jb.nrhs;
+        if (blocks + nb > (1LL << 30)) throw Error(FP_E_INVALID, "too many tridiagonal workgroups in one launch");
+        blocks += (int)nb;
+        threads = std::max(threads, (MB * P + 63) / 64 * 64);
+        lds     = std::max(lds, 3 * (size_t)P * MB * sizeof(T));
+    }
+    bt.start[n] = blocks;
+    if (blocks == 0) return;
+    hipLaunchKernelGGL((k_tridiag<T, CH>), dim3(blocks), dim3(threads), lds, s, bt);
     FP_HIP(hipGetLastError());
+}
+
+// chunk length: 16 rows per thread up to nx = 1024, 32 above (register
+// budget); jobs are grouped by it and launched kMaxBatch at a time
+template <typename T>
+void launch_tridiag_batch(const TridiagJob<T>* jobs, int njobs, hipStream_t s)
+{
+    std::vector<const TridiagJob<T>*> g16, g32;
+    for (int k = 0; k < njobs; k++) {
+        if (jobs[k].nrhs <= 0) continue;
+        if (jobs[k].sp->nx > 32768) throw Error(FP_E_INVALID, "nx above 32768 is not supported");
+        (tridiag_ch(jobs[k].sp->nx) == 16 ? g16 : g32).push_back(&jobs[k]);
+    }
+    for (size_t q0 = 0; q0 < g16.size(); q0 += kMaxBatch)
+        launch_tridiag_group<T, 16>(g16.data() + q0, (int)std::min<size_t>(kMaxBatch, g16.size() - q0), s);
+    for (size_t q0 = 0; q0 < g32.size(); q0 += kMaxBatch)
+        launch_tridiag_group<T, 32>(g32.data() + q0, (int)std::min<size_t>(kMaxBatch, g32.size() - q0), s);
 }
 
 template <typename T>
 void launch_tridiag(const SubPlan& sp, const T* pinv, T* data, long long ld, int nrhs, hipStream_t s)
 {
-    // chunk length: 16 rows per thread up to nx = 1024, 32 above (register budget)
-    if (sp.nx <= 1024) launch_tridiag_ch<T, 16>(sp, pinv, data, ld, nrhs, s);
-    else if (sp.nx <= 32768) launch_tridiag_ch<T, 32>(sp, pinv, data, ld, nrhs, s);
-    else throw Error(FP_E_INVALID, "nx above 32768 is not supported");
+    const TridiagJob<T> j{&sp, pinv, data, ld, nrhs};
+    launch_tridiag_batch<T>(&j, 1, s);
 }
 
 template <typename T, int CH>
@@ -371,6 +434,8 @@ int launch_tridiag_lines(const SubPlan& sp, const LinesArgs<T>& a, int nrhs, hip
 template int launch_tridiag_lines<double>(const SubPlan&, const LinesArgs<double>&, int, hipStream_t);
 
 template void launch_tridiag<double>(const SubPlan&, const double*, double*, long long, int, hipStream_t);
+template void launch_tridiag_batch<double>(const TridiagJob<double>*, int, hipStream_t);
+template void launch_tridiag_batch<float>(const TridiagJob<float>*, int, hipStream_t);
 template void launch_tridiag<float>(const SubPlan&, const float*, float*, long long, int, hipStream_t);
 
 }  // namespace fp

@@ -67,6 +67,7 @@ struct GmresDev {
     double* sn;     // nrhs x m
     double* g;      // nrhs x (m+1)
     double* Y;      // nrhs x m
+    double* rnorm;  // nrhs: norm of the right-hand side C at GMRES start
     double* part;   // nrhs x nblk partial sums
     double* part2;  // nrhs x (m+1): dot products of the current pass
     double* tolabs; // nrhs

@@ -491,6 +491,7 @@ __global__ __launch_bounds__(GB) void k_gm_set_tol(GmresDev g, double tol, int r
     if (threadIdx.x == 0) {
         g.tolabs[r]        = rel ? tol * nc : tol;
         g.resid[r]         = nc;
+        g.rnorm[r]         = nc;
         g.state[r * 4 + 0] = 0;
         g.state[r * 4 + 1] = 0;
         g.state[r * 4 + 2] = 0;

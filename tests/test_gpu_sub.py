@@ -130,6 +130,37 @@ def test_batched_uneven(ctx):
         assert rel(t.cpu().numpy(), O.sub_solve(f, h, h, (D, D, D, D))) < 1e-12
 
 
+@pytest.mark.parametrize("f32", [False, True])
+def test_batched_mixed_kinds(ctx, f32):
+    """Batches that span every transform kind, several large-prime codelets,
+    both tridiagonal chunk lengths (nx above 1024), several right-hand sides
+    per problem and more problems than one launch takes."""
+    import torch
+    import focalors_poisson_amd as fpa
+    rng = np.random.default_rng(17 + f32)
+    pairs = [(D, D), (N, N), (D, N), (N, D), (A, N), (N, A)]
+    sizes = [(40, 22), (1100, 37), (64, 28), (33, 100), (1030, 64), (12, 255), (70, 45), (96, 96),
+             (20, 511), (48, 130), (9, 18), (300, 60), (17, 200), (64, 1024)]
+    solvers, fields, hosts, specs = [], [], [], []
+    for k, (nx, ny) in enumerate(sizes):
+        bc = (D if k % 3 else N, A if k % 2 else D) + pairs[k % len(pairs)]
+        h = 1.0 / max(nx, ny)
+        nrhs = 1 + k % 3
+        f = rng.uniform(-1, 1, (nrhs, nx, ny)) * h * h
+        solvers.append(fpa.PoissonSolver2D(ctx, nx, ny, h, h, bc))
+        fields.append(to_dev(f, torch.float32 if f32 else None))
+        hosts.append(f)
+        specs.append((h, bc))
+    banner("batched mixed kinds f32=%d" % f32)
+    fpa.solve_batched(ctx, solvers, fields)
+    torch.cuda.synchronize()
+    bar = 2e-5 if f32 else 1e-12
+    for f, t, (h, bc) in zip(hosts, fields, specs):
+        got = t.double().cpu().numpy()
+        for r in range(f.shape[0]):
+            assert rel(got[r], O.sub_solve(f[r], h, h, bc)) < bar, (f.shape, bc)
+
+
 def test_errors_are_status_codes(ctx):
     import focalors_poisson_amd as fpa
     with pytest.raises(fpa.FPError) as e:
