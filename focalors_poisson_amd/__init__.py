@@ -157,6 +157,11 @@ class ConcatPoissonSolver2D:
                                               int(max_iter), int(precision), ctypes.byref(self._h)))
         self.root = _capi.lib().fp_cat_root(self._h)
 
+    def set_refinement(self, tol=1e-12, max_steps=10):
+        """FP_F32_REFINE handles: target estimated relative error and sweep cap
+        of the iterative refinement (fp_cat_set_refinement)."""
+        _capi.check(_capi.lib().fp_cat_set_refinement(self._h, float(tol), int(max_steps)))
+
     def _ptrs(self, fields, ghost):
         import torch
         nd = len(self.domains)

@@ -122,12 +122,18 @@ void launch_tridiag(const SubPlan& sp, const T* pinv, T* data, long long ld, int
 
 // Batched forms: the jobs of several subdomains (any sizes and boundary
 // pairs) in as few launches as their kernels allow.
+// data/ld: the output (and, with src == null, the input: in place).  A
+// separate src/lds is read instead when given; src64/dst64 mark buffers of
+// doubles (FP32 transforms reading or writing FP64 fields).
 template <typename T>
 struct TransformJob {
     const TransformPlan<T>* plan;
-    T*                      data;
+    void*                   data;
     long long               ld;
     int                     nx, nrhs;
+    const void*             src   = nullptr;
+    long long               lds   = 0;
+    int                     src64 = 0, dst64 = 0;
 };
 template <typename T>
 void launch_ytransform_batch(const TransformJob<T>* jobs, int njobs, int inverse, hipStream_t s);

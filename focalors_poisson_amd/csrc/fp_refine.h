@@ -34,4 +34,29 @@ void        launch_to_f32(const double* in, long long ldi, float* out, long long
 void        launch_refine_step(const RefineArgs& a, int nrhs, hipStream_t s);
 RefineStats refine_solve(SubPlan& sp, double* f, long long ld, int nrhs, double tol, int max_steps, hipStream_t s);
 
+// FP64 residual of the assembled composite operator on one subdomain:
+//   r = b - A u - (sum of the neighbours' adjacent lines)
+// A the hx^2-scaled 5-point operator of the subdomain (ghost 0 on Dirichlet
+// and Adjacented faces, ghost = cell on Neumann faces); the coupling
+// coefficient 1 is the one ConcatPoissonSolver2D::solve applies
+// (pe/concat/concat_solver2d.cpp:127-128,140,150).
+struct CatResidualArgs {
+    const double* u;
+    const double* b;
+    double*       r;
+    long long     ldu, ldb, ldr;
+    int           nx, ny;
+    double        ry;           // hx^2 / hy^2
+    int           neu[4];       // Neumann faces
+    const double* nb[4];        // neighbour field on Adjacented faces, else null
+    int           nbx[4], nby[4];
+    long long     ldn[4];
+};
+void launch_cat_residual(const CatResidualArgs& a, int nrhs, hipStream_t s);
+// u += d (d may be null: u unchanged) over nx*ny values per right-hand side;
+// writes per-block partial sums (|d|^2, |u|^2) to part[(r * nblk + blk) * 2]
+constexpr int kNormBlocks = 256;
+void launch_update_norms(double* u, long long ldu, const double* d, long long ldd, long long n, int nrhs,
+                         double* part, hipStream_t s);
+
 }  // namespace fp

@@ -167,4 +167,84 @@ RefineStats refine_solve(SubPlan& sp, double* f, long long ld, int nrhs, double 
     return st;
 }
 
+// ---- composite refinement ------------------------------------------------------
+
+__global__ __launch_bounds__(RB) void k_cat_residual(CatResidualArgs a)
+{
+    const int       nx = a.nx, ny = a.ny;
+    const long long n  = (long long)nx * ny;
+    const long long r  = blockIdx.y;
+    const double*   u  = a.u + r * a.ldu;
+    const double*   b  = a.b + r * a.ldb;
+    double*         rr = a.r + r * a.ldr;
+    for (long long q = blockIdx.x * (long long)RB + threadIdx.x; q < n; q += (long long)gridDim.x * RB) {
+        const int    i = (int)(q / ny), j = (int)(q - (long long)i * ny);
+        const double c = u[q];
+        double       ax = -2.0 * c, ay = -2.0 * c, cpl = 0.0;
+        if (i > 0) ax += u[q - ny];
+        else if (a.neu[XNEG]) ax += c;
+        else if (a.nb[XNEG]) cpl += a.nb[XNEG][r * a.ldn[XNEG] + (long long)(a.nbx[XNEG] - 1) * a.nby[XNEG] + j];
+        if (i < nx - 1) ax += u[q + ny];
+        else if (a.neu[XPOS]) ax += c;
+        else if (a.nb[XPOS]) cpl += a.nb[XPOS][r * a.ldn[XPOS] + j];
+        if (j > 0) ay += u[q - 1];
+        else if (a.neu[YNEG]) ay += c;
+        else if (a.nb[YNEG]) cpl += a.nb[YNEG][r * a.ldn[YNEG] + (long long)i * a.nby[YNEG] + a.nby[YNEG] - 1];
+        if (j < ny - 1) ay += u[q + 1];
+        else if (a.neu[YPOS]) ay += c;
+        else if (a.nb[YPOS]) cpl += a.nb[YPOS][r * a.ldn[YPOS] + (long long)i * a.nby[YPOS]];
+        rr[q] = b[q] - cpl - (ax + a.ry * ay);
+    }
+}
+
+void launch_cat_residual(const CatResidualArgs& a, int nrhs, hipStream_t s)
+{
+    const long long n = (long long)a.nx * a.ny;
+    hipLaunchKernelGGL(k_cat_residual, dim3(refine_blocks(n), nrhs), dim3(RB), 0, s, a);
+    FP_HIP(hipGetLastError());
+}
+
+__global__ __launch_bounds__(RB) void k_update_norms(double* u, long long ldu, const double* d, long long ldd,
+                                                     long long n, double* part)
+{
+    __shared__ double sd[RB / 64], su[RB / 64];
+    const long long r  = blockIdx.y;
+    double*         ur = u + r * ldu;
+    const double*   dr = d ? d + r * ldd : nullptr;
+    double          pd = 0.0, pu = 0.0;
+    for (long long q = blockIdx.x * (long long)RB + threadIdx.x; q < n; q += (long long)gridDim.x * RB) {
+        double v = ur[q];
+        if (dr) {
+            const double dd = dr[q];
+            v += dd;
+            ur[q] = v;
+            pd += dd * dd;
+        }
+        pu += v * v;
+    }
+    pd = wsum(pd);
+    pu = wsum(pu);
+    if ((threadIdx.x & 63) == 0) {
+        sd[threadIdx.x >> 6] = pd;
+        su[threadIdx.x >> 6] = pu;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double td = 0.0, tu = 0.0;
+        for (int w = 0; w < RB / 64; w++) {
+            td += sd[w];
+            tu += su[w];
+        }
+        part[(r * gridDim.x + blockIdx.x) * 2 + 0] = td;
+        part[(r * gridDim.x + blockIdx.x) * 2 + 1] = tu;
+    }
+}
+
+void launch_update_norms(double* u, long long ldu, const double* d, long long ldd, long long n, int nrhs,
+                         double* part, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_update_norms, dim3(kNormBlocks, nrhs), dim3(RB), 0, s, u, ldu, d, ldd, n, part);
+    FP_HIP(hipGetLastError());
+}
+
 }  // namespace fp

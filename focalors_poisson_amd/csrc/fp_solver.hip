@@ -127,7 +127,14 @@ void sub_solve_batch(SubPlan* const* plans, T* const* data, const long long* ld,
         if (nrhs[k] <= 0) continue;
         SubPlan& sp = *plans[k];
         if constexpr (sizeof(T) == 4) sp.ensure_f32();
-        tj.push_back({&tplan<T>(sp), data[k], ld[k], sp.nx, nrhs[k]});
+        TransformJob<T> j{};
+        j.plan  = &tplan<T>(sp);
+        j.data  = data[k];
+        j.ld    = ld[k];
+        j.nx    = sp.nx;
+        j.nrhs  = nrhs[k];
+        j.dst64 = sizeof(T) == 8;
+        tj.push_back(j);
         rj.push_back({&sp, pinv_of<T>(sp), data[k], ld[k], nrhs[k]});
     }
     if (tj.empty()) return;
@@ -137,6 +144,50 @@ void sub_solve_batch(SubPlan* const* plans, T* const* data, const long long* ld,
 }
 template void sub_solve_batch<double>(SubPlan* const*, double* const*, const long long*, const int*, int, hipStream_t);
 template void sub_solve_batch<float>(SubPlan* const*, float* const*, const long long*, const int*, int, hipStream_t);
+
+// FP32 transforms and sweeps on FP64 fields: the forward transform reads the
+// FP64 field and writes FP32 coefficients to scratch[k], the sweep runs there,
+// and the inverse transform writes the FP64 field again (32 bytes per cell
+// instead of 48, no separate conversion passes).
+static void sub_solve_batch_mixed(SubPlan* const* plans, double* const* data, const long long* ld,
+                                  float* const* scratch, const int* nrhs, int n, hipStream_t s)
+{
+    std::vector<TransformJob<float>> fw, iv;
+    std::vector<TridiagJob<float>>   rj;
+    for (int k = 0; k < n; k++) {
+        if (nrhs[k] <= 0) continue;
+        SubPlan& sp = *plans[k];
+        sp.ensure_f32();
+        const long long nn = (long long)sp.nx * sp.ny;
+        TransformJob<float> a{};
+        a.plan  = &sp.t32;
+        a.data  = scratch[k];
+        a.ld    = nn;
+        a.nx    = sp.nx;
+        a.nrhs  = nrhs[k];
+        a.src   = data[k];
+        a.lds   = ld[k];
+        a.src64 = 1;
+        a.dst64 = 0;
+        fw.push_back(a);
+        TransformJob<float> b{};
+        b.plan  = &sp.t32;
+        b.data  = data[k];
+        b.ld    = ld[k];
+        b.nx    = sp.nx;
+        b.nrhs  = nrhs[k];
+        b.src   = scratch[k];
+        b.lds   = nn;
+        b.src64 = 0;
+        b.dst64 = 1;
+        iv.push_back(b);
+        rj.push_back({&sp, sp.pinv32.p, scratch[k], nn, nrhs[k]});
+    }
+    if (fw.empty()) return;
+    launch_ytransform_batch<float>(fw.data(), (int)fw.size(), 0, s);
+    launch_tridiag_batch<float>(rj.data(), (int)rj.size(), s);
+    launch_ytransform_batch<float>(iv.data(), (int)iv.size(), 1, s);
+}
 
 // ---------------------------------------------------------------------------
 // composite solver
@@ -201,6 +252,21 @@ struct fp_cat_s {
     int                                    m = 20, tol_rel = 0, maxiter = 100, precision = FP_F64;
     double                                 tol = 1e-8;
     int                                    check_every = 4;
+    // FP32 + refinement (precision FP_F32_REFINE)
+    bool                                   mixed = false;  // sweep solves in FP32 arithmetic
+    double                                 refine_tol = 1e-12;
+    int                                    refine_max = 10;
+    int                                    ws32_nrhs  = 0;
+    std::vector<fp::DevBuf<float>>         s32;            // per domain FP32 scratch
+    std::vector<fp::DevBuf<double>>        rb, rw;         // per domain: assembled RHS, residual / correction
+    fp::DevBuf<double>                     rpart;
+    double*                                h_rpart = nullptr;
+    int                                    last_refine_steps = 0;
+    double                                 last_refine_est   = 0.0;
+    ~fp_cat_s()
+    {
+        if (h_rpart) (void)hipHostFree(h_rpart);
+    }
     int                                    precond_kind = 1;  // 0: none, 1: dense inverse of the interface operator
     // statistics of the last solve
     int    last_steps = 0, last_restarts = 0, last_status = 0;
@@ -210,6 +276,28 @@ struct fp_cat_s {
 namespace fp {
 
 static long long dom_n(const fp_cat_s* c, int d) { return (long long)c->nx[d] * c->ny[d]; }
+
+// The subdomain solves of a sweep on domains doms[0..n): one batch per solve
+// phase, in FP64, or in FP32 arithmetic on the FP64 fields while a
+// refinement runs its inner sweeps.
+static void dom_solve_batch(fp_cat_s* c, const int* doms, double* const* data, int n, int nrhs, hipStream_t s)
+{
+    std::vector<SubPlan*>  plans(n);
+    std::vector<long long> ld(n);
+    std::vector<int>       nr(n, nrhs);
+    std::vector<float*>    sc(n);
+    for (int k = 0; k < n; k++) {
+        plans[k] = &c->nodes[doms[k]]->plan;
+        ld[k]    = dom_n(c, doms[k]);
+        sc[k]    = c->mixed ? c->s32[doms[k]].p : nullptr;
+    }
+    if (c->mixed) sub_solve_batch_mixed(plans.data(), data, ld.data(), sc.data(), nr.data(), n, s);
+    else sub_solve_batch<double>(plans.data(), data, ld.data(), nr.data(), n, s);
+}
+static void dom_solve(fp_cat_s* c, int d, double* data, int nrhs, hipStream_t s)
+{
+    dom_solve_batch(c, &d, &data, 1, nrhs, s);
+}
 
 // root-side line cells of the face `loc` of an nx x ny field
 static std::vector<int> line_cells(int loc, int nx, int ny)
@@ -234,7 +322,6 @@ static std::vector<int> child_cells(int loc, int bnx, int bny)
     return v;
 }
 
-static void node_solve(fp_cat_s* c, int d, double* f, int nrhs, hipStream_t s);
 
 static void ensure_ws(fp_cat_s* c, Node& nd, int nrhs)
 {
@@ -414,7 +501,7 @@ static void gmres_solve(fp_cat_s* c, Node& nd, double* b, int nrhs, hipStream_t 
     ensure_ws(c, nd, nrhs);
     const long long n = dom_n(c, nd.dom);
     const int       K = nd.K, m = c->m;
-    sub_solve<double>(nd.plan, b, n, nrhs, s);
+    dom_solve(c, nd.dom, b, nrhs, s);
     launch_gather(b, n, nd.cell.p, K, nullptr, nd.C.p, K, 0.0, 1.0, nrhs, s);
     // x-line unknowns to the root's y basis (orthogonal: GMRES norms unchanged)
     if (nd.spectral_x) launch_ytransform<double>(nd.plan.t64, nd.C.p, K, nd.nxl, nrhs, 0, s);
@@ -474,7 +561,7 @@ static void gmres_solve(fp_cat_s* c, Node& nd, double* b, int nrhs, hipStream_t 
     for (auto& L : nd.ch) launch_schur_gemv(L.S.p, L.cn, nd.X.p, K, L.off, nd.Tv.p, K, L.off, nrhs, s);
     launch_fill<double>(nd.G.p, n * nrhs, 0.0, s);
     launch_scatter(nd.G.p, n, nd.ucell.p, nd.uk1.p, nd.uk2.p, nd.nu, nd.Tv.p, K, nrhs, s);
-    sub_solve<double>(nd.plan, nd.G.p, n, nrhs, s);
+    dom_solve(c, nd.dom, nd.G.p, nrhs, s);
     launch_axpby(nd.G.p, b, n * nrhs, 1.0, 1.0, s);
     if (nd.dom == c->root) {
         c->last_steps    = steps;
@@ -485,31 +572,26 @@ static void gmres_solve(fp_cat_s* c, Node& nd, double* b, int nrhs, hipStream_t 
     }
 }
 
-static void node_solve(fp_cat_s* c, int d, double* f, int nrhs, hipStream_t s)
-{
-    Node& nd = *c->nodes[d];
-    if (nd.ch.empty()) sub_solve<double>(nd.plan, f, dom_n(c, d), nrhs, s);
-    else gmres_solve(c, nd, f, nrhs, s);
-}
-
 // The solves of one tree level: the leaves of the level in one batch (one
 // launch per solve phase for all of them), then each non-leaf's GMRES.
 static void level_solve(fp_cat_s* c, const std::vector<int>& doms, double* const* f, int nrhs, hipStream_t s)
 {
-    std::vector<SubPlan*>  plans;
-    std::vector<double*>   data;
-    std::vector<long long> ld;
-    std::vector<int>       nr;
+    std::vector<int>     leaves;
+    std::vector<double*> data;
     for (int d : doms)
         if (c->nodes[d]->ch.empty()) {
-            plans.push_back(&c->nodes[d]->plan);
+            leaves.push_back(d);
             data.push_back(f[d]);
-            ld.push_back(dom_n(c, d));
-            nr.push_back(nrhs);
         }
-    if (!plans.empty()) sub_solve_batch<double>(plans.data(), data.data(), ld.data(), nr.data(), (int)plans.size(), s);
+    if (!leaves.empty()) dom_solve_batch(c, leaves.data(), data.data(), (int)leaves.size(), nrhs, s);
     for (int d : doms)
         if (!c->nodes[d]->ch.empty()) gmres_solve(c, *c->nodes[d], f[d], nrhs, s);
+}
+
+static void node_solve(fp_cat_s* c, int d, double* f, int nrhs, hipStream_t s)
+{
+    if (c->nodes[d]->ch.empty()) dom_solve(c, d, f, nrhs, s);
+    else gmres_solve(c, *c->nodes[d], f, nrhs, s);
 }
 
 // GMRESSolver2D::schur_mat_construct (gmres_solver2d.cpp:39-89): S of each
@@ -678,18 +760,11 @@ static void build_tree(fp_cat_s* c)
     for (int a : order) c->levels[c->level[a]].push_back(a);
 }
 
-// ConcatPoissonSolver2D::solve (pe/concat/concat_solver2d.cpp:101-157)
-static void cat_solve(fp_cat_s* c, double* const* f, const double* const* ghost, int nrhs, hipStream_t s)
+// Boundary assembly and hx^2 scaling of every domain, in place
+// (pe/concat/concat_solver2d.cpp:110-116,159-223).
+static void cat_assemble(fp_cat_s* c, double* const* f, const double* const* ghost, int nrhs, hipStream_t s)
 {
-    const int nd = c->nd;
-    if (c->temp_nrhs < nrhs) {
-        c->temp.clear();
-        c->temp.resize(nd);
-        for (int d = 0; d < nd; d++)
-            if (d != c->root) c->temp[d].alloc((size_t)dom_n(c, d) * nrhs);
-        c->temp_nrhs = nrhs;
-    }
-    for (int d = 0; d < nd; d++) {
+    for (int d = 0; d < c->nd; d++) {
         AsmFaces<double> fc{};
         const double     hx = c->hx[d], hy = c->hy[d];
         for (int face = 0; face < 4; face++) {
@@ -706,6 +781,20 @@ static void cat_solve(fp_cat_s* c, double* const* f, const double* const* ghost,
             fc.g[face] = gp;
         }
         launch_assemble<double>(f[d], c->nx[d], c->ny[d], dom_n(c, d), nrhs, hx * hx, fc, s);
+    }
+}
+
+// The tree sweep of ConcatPoissonSolver2D::solve (concat_solver2d.cpp:118-157)
+// on assembled, scaled right-hand sides: u = A^-1 b in place.
+static void cat_sweep(fp_cat_s* c, double* const* f, int nrhs, hipStream_t s)
+{
+    const int nd = c->nd;
+    if (c->temp_nrhs < nrhs) {
+        c->temp.clear();
+        c->temp.resize(nd);
+        for (int d = 0; d < nd; d++)
+            if (d != c->root) c->temp[d].alloc((size_t)dom_n(c, d) * nrhs);
+        c->temp_nrhs = nrhs;
     }
     for (int l = c->nlev - 1; l >= 1; l--) {
         for (int d : c->levels[l]) {
@@ -742,6 +831,143 @@ static void cat_solve(fp_cat_s* c, double* const* f, const double* const* ghost,
         }
         level_solve(c, c->levels[l], f, nrhs, s);
     }
+}
+
+static void ensure_refine_ws(fp_cat_s* c, int nrhs)
+{
+    if (c->ws32_nrhs >= nrhs) return;
+    const int nd = c->nd;
+    c->s32.clear();
+    c->rb.clear();
+    c->rw.clear();
+    c->s32.resize(nd);
+    c->rb.resize(nd);
+    c->rw.resize(nd);
+    for (int d = 0; d < nd; d++) {
+        const size_t n = (size_t)dom_n(c, d) * nrhs;
+        c->s32[d].alloc(n);
+        c->rb[d].alloc(n);
+        c->rw[d].alloc(n);
+    }
+    c->rpart.alloc((size_t)2 * 2 * nd * nrhs * kNormBlocks);
+    if (c->h_rpart) (void)hipHostFree(c->h_rpart);
+    c->h_rpart = nullptr;
+    FP_HIP(hipHostMalloc((void**)&c->h_rpart, sizeof(double) * c->rpart.n));
+    c->ws32_nrhs = nrhs;
+}
+
+// r_d = b_d - A_d u_d - (neighbour lines) for every domain
+static void cat_residual(fp_cat_s* c, double* const* u, int nrhs, hipStream_t s)
+{
+    for (int d = 0; d < c->nd; d++) {
+        CatResidualArgs a{};
+        const long long n = dom_n(c, d);
+        a.u   = u[d];
+        a.b   = c->rb[d].p;
+        a.r   = c->rw[d].p;
+        a.ldu = a.ldb = a.ldr = n;
+        a.nx  = c->nx[d];
+        a.ny  = c->ny[d];
+        a.ry  = c->hx[d] * c->hx[d] / (c->hy[d] * c->hy[d]);
+        for (int face = 0; face < 4; face++) {
+            a.neu[face] = c->bc[4 * d + face] == NEUMANN;
+            const int nb = c->adj[4 * d + face];
+            a.nb[face]   = nb >= 0 ? u[nb] : nullptr;
+            a.nbx[face]  = nb >= 0 ? c->nx[nb] : 0;
+            a.nby[face]  = nb >= 0 ? c->ny[nb] : 0;
+            a.ldn[face]  = nb >= 0 ? dom_n(c, nb) : 0;
+        }
+        launch_cat_residual(a, nrhs, s);
+    }
+}
+
+// FP32 + refinement of the composite solve (no reference counterpart):
+//   u_1 = S32(b);  u_{k+1} = u_k + S32(b - A u_k)
+// with S32 the tree sweep whose subdomain solves run in FP32 arithmetic (the
+// interface GMRES, Schur products and this residual stay FP64).  With
+// d_k = u_k - u_{k-1} and rho = |d_k| / |d_{k-1}| the error of u_k is
+// estimated as rho / (1 - rho) |d_k| (contraction of the iteration); the loop
+// stops when that estimate, or |d_k| itself, is at most tol |u_k| for every
+// right-hand side, or after refine_max sweeps.
+static void cat_refine(fp_cat_s* c, double* const* f, int nrhs, hipStream_t s)
+{
+    const int nd = c->nd;
+    ensure_refine_ws(c, nrhs);
+    for (int d = 0; d < nd; d++)
+        FP_HIP(hipMemcpyAsync(c->rb[d].p, f[d], sizeof(double) * dom_n(c, d) * nrhs, hipMemcpyDeviceToDevice, s));
+    const size_t half = (size_t)2 * nd * nrhs * kNormBlocks;  // doubles per norm set
+    auto         norms = [&](double* const* u, const double* const* dlt, size_t slot) {
+        for (int d = 0; d < nd; d++)
+            launch_update_norms(u[d], dom_n(c, d), dlt ? dlt[d] : nullptr, dom_n(c, d), dom_n(c, d), nrhs,
+                                c->rpart.p + slot * half + (size_t)2 * d * nrhs * kNormBlocks, s);
+    };
+    auto sums = [&](size_t slot, int r, double& dd, double& uu) {
+        dd = uu = 0.0;
+        for (int d = 0; d < nd; d++) {
+            const double* p = c->h_rpart + slot * half + (size_t)2 * d * nrhs * kNormBlocks + (size_t)2 * r * kNormBlocks;
+            for (int k = 0; k < kNormBlocks; k++) {
+                dd += p[2 * k];
+                uu += p[2 * k + 1];
+            }
+        }
+    };
+    std::vector<double*>       w(nd);
+    std::vector<const double*> wc(nd);
+    for (int d = 0; d < nd; d++) wc[d] = w[d] = c->rw[d].p;
+    c->mixed = true;
+    std::vector<double> prev(nrhs, 0.0);
+    int                 steps = 0;
+    double              est   = INFINITY;
+    try {
+        cat_sweep(c, f, nrhs, s);  // u_1 = d_1
+        steps = 1;
+        norms(f, nullptr, 0);      // |u_1|^2 in slot 0
+        for (int k = 2; k <= c->refine_max; k++) {
+            cat_residual(c, f, nrhs, s);
+            cat_sweep(c, w.data(), nrhs, s);
+            norms(f, wc.data(), 1);  // u += d, |d|^2, |u|^2 in slot 1
+            steps = k;
+            FP_HIP(hipMemcpyAsync(c->h_rpart, c->rpart.p, sizeof(double) * 2 * half, hipMemcpyDeviceToHost, s));
+            FP_HIP(hipStreamSynchronize(s));
+            double worst = 0.0;
+            bool   bad   = false;
+            for (int r = 0; r < nrhs; r++) {
+                double dd, uu, d0, u0;
+                if (k == 2) {
+                    sums(0, r, d0, u0);
+                    prev[r] = std::sqrt(u0);  // |d_1| = |u_1|
+                }
+                sums(1, r, dd, uu);
+                const double dn = std::sqrt(dd), un = std::sqrt(uu);
+                const double ratio = un > 0 ? dn / un : dn;
+                const double rho   = prev[r] > 0 ? dn / prev[r] : 0.0;
+                double       e     = rho < 1.0 ? std::min(ratio, rho / (1.0 - rho) * ratio) : ratio;
+                if (!std::isfinite(e)) bad = true;
+                worst   = std::max(worst, e);
+                prev[r] = dn;
+            }
+            est = bad ? INFINITY : worst;
+            if (bad || est <= c->refine_tol) break;
+        }
+    } catch (...) {
+        c->mixed = false;
+        throw;
+    }
+    c->mixed             = false;
+    c->last_refine_steps = steps;
+    c->last_refine_est   = est;
+    if (!std::isfinite(est)) c->last_status = FP_E_NUMERIC;
+    else if (est > c->refine_tol && c->last_status == 0) c->last_status = FP_E_NOT_CONVERGED;
+}
+
+// ConcatPoissonSolver2D::solve (pe/concat/concat_solver2d.cpp:101-157)
+static void cat_solve(fp_cat_s* c, double* const* f, const double* const* ghost, int nrhs, hipStream_t s)
+{
+    cat_assemble(c, f, ghost, nrhs, s);
+    c->last_refine_steps = 0;
+    c->last_refine_est   = 0.0;
+    if (c->precision == FP_F32_REFINE) cat_refine(c, f, nrhs, s);
+    else cat_sweep(c, f, nrhs, s);
 }
 
 }  // namespace fp
@@ -921,7 +1147,7 @@ int fp_cat_create(fp_ctx ctx, const fp_geometry_desc* g, int m, double tol, int 
         FP_REQUIRE(g->nd >= 1 && g->nx && g->ny && g->hx && g->hy && g->bc, "empty geometry");
         FP_REQUIRE(m >= 1 && m <= 1000, "m must be in [1, 1000]");
         FP_REQUIRE(max_iter >= 1 && tol >= 0.0, "invalid GMRES parameters");
-        FP_REQUIRE(precision == FP_F64, "composite solve supports FP_F64");
+        FP_REQUIRE(precision == FP_F64 || precision == FP_F32_REFINE, "unknown precision");
         FP_HIP(hipSetDevice(ctx->device));
         auto c       = std::make_unique<fp_cat_s>();
         c->ctx       = ctx;
@@ -975,12 +1201,22 @@ int fp_cat_solve(fp_cat cat, double* const* d_fields, const double* const* d_gho
             info->restarts        = cat->last_restarts;
             info->residual        = cat->last_res;
             info->rel_residual    = cat->last_rel;
-            info->refine_steps    = 0;
-            info->refine_residual = 0.0;
+            info->refine_steps    = cat->last_refine_steps;
+            info->refine_residual = cat->last_refine_est;
         }
     });
     if (st == FP_OK && cat && cat->last_status) st = cat->last_status;
     return st;
+}
+
+int fp_cat_set_refinement(fp_cat cat, double tol, int max_steps)
+{
+    return guarded([&] {
+        FP_REQUIRE(cat, "null argument");
+        FP_REQUIRE(tol > 0.0 && max_steps >= 2, "refinement needs tol > 0 and max_steps >= 2");
+        cat->refine_tol = tol;
+        cat->refine_max = max_steps;
+    });
 }
 
 // Adjoint of fp_cat_solve.  The solve is u = A^-1 b with b_d = hx_d^2 (f_d + C_d g_d):
@@ -1032,8 +1268,8 @@ int fp_cat_vjp(fp_cat cat, const double* const* d_gu, double* const* d_gf, doubl
             info->restarts        = cat->last_restarts;
             info->residual        = cat->last_res;
             info->rel_residual    = cat->last_rel;
-            info->refine_steps    = 0;
-            info->refine_residual = 0.0;
+            info->refine_steps    = cat->last_refine_steps;
+            info->refine_residual = cat->last_refine_est;
         }
     });
     if (st == FP_OK && cat && cat->last_status) st = cat->last_status;

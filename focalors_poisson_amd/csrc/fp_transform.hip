@@ -46,10 +46,15 @@ __device__ __forceinline__ C2<T> cscale(C2<T> a, T s) { return {a.x * s, a.y * s
 template <bool FWD, typename T>
 __device__ __forceinline__ C2<T> mul_mi(C2<T> a) { return FWD ? C2<T>{a.y, -a.x} : C2<T>{-a.y, a.x}; }
 
+// src/dst: the rows read and the coefficients written (the same buffer for an
+// in-place transform).  src64/dst64 = 1: that buffer holds doubles (an FP32
+// transform then reads or writes FP64 fields directly); 0: it holds T.
 template <typename T>
 struct YArgs {
-    T*           data;
-    long long    ld;
+    const void*  src;
+    void*        dst;
+    long long    lds, ldd;
+    int          src64, dst64;
     int          nx, N, L, kind, inverse, nrad;
     int          rad[kMaxRadices];
     const C2<T>* W;
@@ -325,8 +330,16 @@ __global__ __launch_bounds__(NT) void k_ytransform(YBatch<T> bt)
     const int  N = a.N, L = a.L;
     const int  ra = 2 * pair, rb = ra + 1;
     const bool hb = rb < a.nx;
-    T*         pa = a.data + (long long)rhs * a.ld + (long long)ra * N;
-    T*         pb = pa + N;
+    const long long ia = (long long)rhs * a.lds + (long long)ra * N;  // row a in src
+    const long long oa0 = (long long)rhs * a.ldd + (long long)ra * N;  // row a in dst
+    const bool s64 = sizeof(T) == 8 || a.src64, d64 = sizeof(T) == 8 || a.dst64;
+    auto ld_src = [&](long long q) -> T {
+        return s64 ? (T)static_cast<const double*>(a.src)[q] : static_cast<const T*>(a.src)[q];
+    };
+    auto st_dst = [&](long long q, T v) {
+        if (d64) static_cast<double*>(a.dst)[q] = (double)v;
+        else static_cast<T*>(a.dst)[q] = v;
+    };
 
     // ---- load: build the packed extended sequence in A ----
     // The rows are read in chunks of VPT values per thread with every global
@@ -347,8 +360,8 @@ __global__ __launch_bounds__(NT) void k_ytransform(YBatch<T> bt)
 #pragma unroll
         for (int k = 0; k < VPT; k++) {
             const int n = base + k * NT + threadIdx.x;
-            va[k]       = n < N ? pa[n] : T(0);
-            vb[k]       = (n < N && hb) ? pb[n] : T(0);
+            va[k]       = n < N ? ld_src(ia + n) : T(0);
+            vb[k]       = (n < N && hb) ? ld_src(ia + N + n) : T(0);
         }
 #pragma unroll
         for (int k = 0; k < VPT; k++) {
@@ -438,8 +451,8 @@ __global__ __launch_bounds__(NT) void k_ytransform(YBatch<T> bt)
                 oa             = s * (zk.y - zc.y);
                 ob             = s * (zc.x - zk.x);
             }
-            pa[j] = oa;
-            if (hb) pb[j] = ob;
+            st_dst(oa0 + j, oa);
+            if (hb) st_dst(oa0 + N + j, ob);
         }
     }
 }
@@ -537,8 +550,12 @@ static YArgs<T> yargs(const TransformJob<T>& j, int inverse)
 {
     const TransformPlan<T>& tp = *j.plan;
     YArgs<T>                a{};
-    a.data    = j.data;
-    a.ld      = j.ld;
+    a.src     = j.src ? j.src : j.data;
+    a.dst     = j.data;
+    a.lds     = j.src ? j.lds : j.ld;
+    a.ldd     = j.ld;
+    a.src64   = j.src ? j.src64 : (sizeof(T) == 8);
+    a.dst64   = j.dst64;
     a.nx      = j.nx;
     a.N       = tp.N;
     a.L       = tp.L;
@@ -648,7 +665,13 @@ template <typename T>
 void launch_ytransform(const TransformPlan<T>& tp, T* data, long long ld, int nx, int nrhs, int inverse,
                        hipStream_t s)
 {
-    const TransformJob<T> j{&tp, data, ld, nx, nrhs};
+    TransformJob<T> j{};
+    j.plan  = &tp;
+    j.data  = data;
+    j.ld    = ld;
+    j.nx    = nx;
+    j.nrhs  = nrhs;
+    j.dst64 = sizeof(T) == 8;
     launch_ytransform_batch<T>(&j, 1, inverse, s);
 }
 

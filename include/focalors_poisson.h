@@ -134,6 +134,15 @@ int fp_sub_solve_batched_f32(fp_ctx ctx, int nprob, const fp_sub* subs, float* c
  * interface right-hand side.  precision: fp_precision. */
 int fp_cat_create(fp_ctx ctx, const fp_geometry_desc* g, int m, double tol, int tol_is_relative, int max_iter,
                   int precision, fp_cat* out);
+/* FP_F32_REFINE handles only (no reference counterpart): the refinement runs
+ * sweeps with FP32 subdomain solves on the FP64 residual of the assembled
+ * composite operator until the estimated relative error of the solution
+ * (contraction of successive corrections) or the last relative correction is
+ * at most tol for every right-hand side, or max_steps sweeps were done
+ * (defaults 1e-12, 10).  fp_solve_info reports refine_steps (sweeps) and
+ * refine_residual (that estimate); FP_E_NOT_CONVERGED if max_steps ended it
+ * above tol. */
+int fp_cat_set_refinement(fp_cat cat, double tol, int max_steps);
 void fp_cat_destroy(fp_cat cat);
 int  fp_cat_root(fp_cat cat);
 /* ConcatPoissonSolver2D::solve(): d_fields[d] holds nrhs raw right-hand sides

@@ -1,7 +1,9 @@
 """Composite solve parity on the device (ConcatPoissonSolver2D::solve,
 pe/concat/concat_solver2d.cpp:101-157): golden fixtures from the assembled
 sparse system and the CPU oracle.  Interface GMRES to relative 1e-13; bar
-relative L2 <= 1e-10 against the exact discrete solution."""
+relative L2 <= 1e-10 against the exact discrete solution, for the FP64 path
+and for FP32 sweeps with FP64 iterative refinement (FP_F32_REFINE, target
+estimated relative error 1e-12)."""
 import os
 
 import numpy as np
@@ -22,10 +24,11 @@ def ctx():
     return fpa.Context(0)
 
 
-def run(ctx, doms, conn, fields, ghost=None, m=40, tol=1e-13, rel_tol=True, max_iter=200):
+def run(ctx, doms, conn, fields, ghost=None, m=40, tol=1e-13, rel_tol=True, max_iter=200, precision=0):
     import torch
     import focalors_poisson_amd as fpa
-    cat = fpa.ConcatPoissonSolver2D(ctx, doms, conn, m=m, tol=tol, tol_is_relative=rel_tol, max_iter=max_iter)
+    cat = fpa.ConcatPoissonSolver2D(ctx, doms, conn, m=m, tol=tol, tol_is_relative=rel_tol, max_iter=max_iter,
+                                    precision=precision)
     ts = [to_dev(f) for f in fields]
     gs = {k: to_dev(v) for k, v in (ghost or {}).items()}
     info = cat.solve(ts, gs)
@@ -33,7 +36,8 @@ def run(ctx, doms, conn, fields, ghost=None, m=40, tol=1e-13, rel_tol=True, max_
     return [t.cpu().numpy() for t in ts], info, cat
 
 
-def test_cross32_golden(ctx):
+@pytest.mark.parametrize("precision", [0, 1])
+def test_cross32_golden(ctx, precision):
     g = np.load(os.path.join(GOLD, "cross_32.npz"))
     h = float(g["h"])
     doms = [(int(a), int(b), h, h, tuple(int(t) for t in bc)) for (a, b), bc in zip(g["sizes"], g["bcs"])]
@@ -44,20 +48,21 @@ def test_cross32_golden(ctx):
         if k.startswith("g"):
             d, face = k[1:].split("_")
             ghost[(int(d), int(face))] = g[k]
-    banner("cross32 golden")
-    us, info, cat = run(ctx, doms, conn, fs, ghost)
+    banner("cross32 golden precision=%d" % precision)
+    us, info, cat = run(ctx, doms, conn, fs, ghost, precision=precision)
     assert cat.root == 0
     assert rel(us, [g["u%d" % k] for k in range(5)]) < 1e-10, info
 
 
-def test_concat_six_golden(ctx):
+@pytest.mark.parametrize("precision", [0, 1])
+def test_concat_six_golden(ctx, precision):
     g = np.load(os.path.join(GOLD, "concat_six.npz"))
     sizes = [tuple(int(v) for v in s) for s in g["sizes"]]
     doms = [(a, b, 1.0, 1.0, (D, D, D, D)) for a, b in sizes]
     conn = [tuple(int(v) for v in c) for c in g["conn"]]
     fs = [g["f%d" % k] for k in range(6)]
-    banner("concat six golden (depth-2 tree)")
-    us, info, cat = run(ctx, doms, conn, fs)
+    banner("concat six golden (depth-2 tree) precision=%d" % precision)
+    us, info, cat = run(ctx, doms, conn, fs, precision=precision)
     assert rel(us, [g["u%d" % k] for k in range(6)]) < 1e-10, info
 
 
@@ -103,3 +108,42 @@ def test_not_converged_status(ctx):
     with pytest.raises(fpa.FPError) as e:
         cat.solve([to_dev(f[0]) for f in fields])
     assert e.value.status == 4
+
+
+@pytest.mark.parametrize("n", [32, 128, 256])
+def test_cross_f32_refine_vs_oracle(ctx, n):
+    """FP32 sweeps + FP64 refinement reach the FP64 oracle solution."""
+    from focalors_poisson_amd.cases import cross
+    doms, conn, fields, _ = cross(n, seed=1)
+    fs = [f[0] for f in fields]
+    banner("cross f32+refine n=%d" % n)
+    us, info, _ = run(ctx, doms, conn, fs, precision=1)
+    ref, _, _ = O.Composite(doms, conn, tol=1e-14, tol_is_relative=True, maxiter=400,
+                            schur_mode=O.SCHUR_SPECTRAL).solve(fs)
+    assert rel(us, ref) < 1e-10, info
+    assert info["refine_steps"] >= 2 and info["refine_residual"] <= 1e-12, info
+
+
+def test_cross_multi_rhs_f32_refine(ctx):
+    from focalors_poisson_amd.cases import cross
+    n, nrhs = 32, 3
+    doms, conn, fields, _ = cross(n, seed=200, nrhs=nrhs)
+    banner("cross multi rhs f32+refine")
+    us, info, _ = run(ctx, doms, conn, fields, precision=1)
+    cat = O.Composite(doms, conn, tol=1e-14, tol_is_relative=True, maxiter=400)
+    for r in range(nrhs):
+        ref, _, _ = cat.solve([f[r] for f in fields])
+        assert rel([u[r] for u in us], ref) < 1e-10
+
+
+def test_refine_cap_reports_not_converged(ctx):
+    import focalors_poisson_amd as fpa
+    from focalors_poisson_amd.cases import cross
+    doms, conn, fields, _ = cross(32, seed=1)
+    banner("refine cap status")
+    cat = fpa.ConcatPoissonSolver2D(ctx, doms, conn, m=40, tol=1e-13, tol_is_relative=True, precision=1)
+    cat.set_refinement(tol=1e-30, max_steps=2)
+    info = cat.solve([to_dev(f[0]) for f in fields], allow_not_converged=True)
+    assert info["status"] == 4 and info["refine_steps"] == 2
+    with pytest.raises(fpa.FPError):
+        cat.set_refinement(tol=0.0)
