@@ -84,12 +84,19 @@ struct TransformPlan {
     int        L    = 0;  // complex FFT length
     int        nrad = 0;
     int        rad[kMaxRadices] = {};
-    int        p1   = 0;  // odd prime > 7 with a register codelet (0: none)
-    DevBuf<T>  W;         // twiddles: 64 of W^m, then L/64+1 of W^(64m), W = exp(-2 pi i / L)
+    int        engine = 1;  // 1: single-buffer FFT, 0: two-buffer fallback (prime factors above 47)
+    int        ncs  = 0;  // cos/sin entries of the odd-prime passes
+    DevBuf<T>  W;         // twiddles: 64 of W^m, then L/64+1 of W^(64m), W = exp(-2 pi i / L), then
+                          // (cos, sin)(2 pi m / R) of every odd-prime pass R
     DevBuf<T>  aux;       // 2N: NN phase exp(-i pi m / 2N) or ND (sin, cos) of 2N pi (j+1)/(2N+1)
     T          c0 = 0, c1 = 0;
+    int        nt = 256;  // threads per workgroup (tuned)
     void build(int kind, int N);
 };
+// Times the plan's transform on an nx-row field and keeps the fastest
+// workgroup size (measure = false: the default).
+template <typename T>
+void tune_transform(TransformPlan<T>& tp, int nx, bool measure);
 
 // Per-subdomain plan: PoissonSolver2D's precomputed state on the device.
 struct SubPlan {
@@ -101,13 +108,15 @@ struct SubPlan {
     std::vector<double>   x_diag;   // host copy, reference order
     TransformPlan<double> t64;
     TransformPlan<float>  t32;
-    DevBuf<double>        pinv64;   // ny*ldp, mode-major: reciprocal Thomas pivots (row i of mode j at j*ldp+i)
-    DevBuf<float>         pinv32;
-    int                   ldp = 0;  // nx rounded up to 32, zero padded
-    int                   mb_field = 16, mb_lines = 4;  // modes per workgroup of the two sweeps
+    DevBuf<double>        xd;       // x_diag on the device
+    DevBuf<double>        pst;      // ny*tri_p: Thomas pivot preceding each row chunk (mode-major)
+    int                   tri_ch = 16, tri_p = 0;        // rows per chunk of the x sweep, chunks per mode
+    int                   mb_field = 4, mb_lines = 4;    // modes per workgroup of the two sweeps
     int                   dev = 0;
     void build(int nx, int ny, double hx, double hy, const int bc[4]);
     void ensure_f32();
+    void set_chunks(int ch);
+    void tune_tridiag(bool measure);
 };
 
 // ---- kernel launchers (fp_transform.hip / fp_tridiag.hip) ----
@@ -118,7 +127,7 @@ void launch_ytransform(const TransformPlan<T>& tp, T* data, long long ld, int nx
                        hipStream_t s);
 // Tridiagonal solves along x for every mode, in place.
 template <typename T>
-void launch_tridiag(const SubPlan& sp, const T* pinv, T* data, long long ld, int nrhs, hipStream_t s);
+void launch_tridiag(const SubPlan& sp, T* data, long long ld, int nrhs, hipStream_t s);
 
 // Batched forms: the jobs of several subdomains (any sizes and boundary
 // pairs) in as few launches as their kernels allow.
@@ -140,7 +149,6 @@ void launch_ytransform_batch(const TransformJob<T>* jobs, int njobs, int inverse
 template <typename T>
 struct TridiagJob {
     const SubPlan* sp;
-    const T*       pinv;
     T*             data;
     long long      ld;
     int            nrhs;
@@ -154,8 +162,9 @@ struct LinesArgs {
     const T*  lines;    // per rhs: Xh0 (ny) | Xh1 (ny) | a (nx) | b (nx)
     long long ldl;      // 2 ny + 2 nx
     const T*  phi;      // phi0 (ny) | phi1 (ny)
-    const T*  pinv;     // mode-major reciprocal pivots, ldp per mode
-    int       ldp;
+    const double* xd;   // filled by the launcher from the plan: x_diag,
+    const double* pst;  // chunk-start pivots,
+    int       ns, ne;   // Neumann x ends
     T*        xu;       // per rhs: coefficient rows 0 and nx-1 (2 ny)
     T*        colpart;  // per rhs, per workgroup: 2 x nx column partial sums
     int       nx, ny;

@@ -57,37 +57,21 @@ void SubPlan::build(int nx_, int ny_, double hx_, double hy_, const int bc_[4])
         else lam = -2.0 * std::cos(2.0 * PI * i / (2 * ny + 1));
         x_diag[j] = -2.0 + hx * hx / hy / hy * (lam - 2.0);
     }
+    xd.alloc(ny);
+    xd.upload(x_diag.data(), ny);
     // reciprocal Thomas pivots (the reference's c vector,
-    // chasing_method_base.cpp:33-46), one column per mode
-    ldp = (nx + 31) / 32 * 32;
-    std::vector<double> p((size_t)ldp * ny, 0.0);
-    const bool          ns = bc[XNEG] == NEUMANN, ne = bc[XPOS] == NEUMANN;
-    for (int j = 0; j < ny; j++) {
-        if (no_dirichlet && j == ny - 1) continue;  // singular mode: recurrence, no pivots
-        const double d  = x_diag[j];
-        double*      pj = p.data() + (size_t)j * ldp;
-        double       c  = ns ? 1.0 / (d + 1.0) : 1.0 / d;
-        pj[0]           = c;
-        for (int i = 1; i < nx - 1; i++) {
-            c     = 1.0 / (d - c);
-            pj[i] = c;
-        }
-        pj[nx - 1] = 1.0 / ((ne ? d + 1.0 : d) - c);
-    }
-    pinv64.alloc(p.size());
-    pinv64.upload(p.data(), p.size());
+    // chasing_method_base.cpp:33-46) are regenerated inside the sweep from
+    // one stored pivot per row chunk; the chunk schedule is tuned here
+    tune_tridiag(true);
     t64.build(kind, ny);
+    tune_transform(t64, nx, true);
 }
 
 void SubPlan::ensure_f32()
 {
-    if (pinv32.p) return;
-    std::vector<double> p((size_t)ldp * ny);
-    FP_HIP(hipMemcpy(p.data(), pinv64.p, p.size() * sizeof(double), hipMemcpyDeviceToHost));
-    std::vector<float> q(p.begin(), p.end());
-    pinv32.alloc(q.size());
-    pinv32.upload(q.data(), q.size());
+    if (t32.L) return;
     t32.build(kind, ny);
+    tune_transform(t32, nx, true);
 }
 
 template <typename T>
@@ -96,12 +80,6 @@ template <>
 const TransformPlan<double>& tplan<double>(const SubPlan& sp) { return sp.t64; }
 template <>
 const TransformPlan<float>& tplan<float>(const SubPlan& sp) { return sp.t32; }
-template <typename T>
-static const T* pinv_of(const SubPlan& sp);
-template <>
-const double* pinv_of<double>(const SubPlan& sp) { return sp.pinv64.p; }
-template <>
-const float* pinv_of<float>(const SubPlan& sp) { return sp.pinv32.p; }
 
 template <typename T>
 void sub_solve(SubPlan& sp, T* data, long long ld, int nrhs, hipStream_t s)
@@ -109,7 +87,7 @@ void sub_solve(SubPlan& sp, T* data, long long ld, int nrhs, hipStream_t s)
     if (nrhs <= 0) return;
     if constexpr (sizeof(T) == 4) sp.ensure_f32();
     launch_ytransform<T>(tplan<T>(sp), data, ld, sp.nx, nrhs, 0, s);
-    launch_tridiag<T>(sp, pinv_of<T>(sp), data, ld, nrhs, s);
+    launch_tridiag<T>(sp, data, ld, nrhs, s);
     launch_ytransform<T>(tplan<T>(sp), data, ld, sp.nx, nrhs, 1, s);
 }
 template void sub_solve<double>(SubPlan&, double*, long long, int, hipStream_t);
@@ -135,7 +113,7 @@ void sub_solve_batch(SubPlan* const* plans, T* const* data, const long long* ld,
         j.nrhs  = nrhs[k];
         j.dst64 = sizeof(T) == 8;
         tj.push_back(j);
-        rj.push_back({&sp, pinv_of<T>(sp), data[k], ld[k], nrhs[k]});
+        rj.push_back({&sp, data[k], ld[k], nrhs[k]});
     }
     if (tj.empty()) return;
     launch_ytransform_batch<T>(tj.data(), (int)tj.size(), 0, s);
@@ -181,7 +159,7 @@ static void sub_solve_batch_mixed(SubPlan* const* plans, double* const* data, co
         b.src64 = 0;
         b.dst64 = 1;
         iv.push_back(b);
-        rj.push_back({&sp, sp.pinv32.p, scratch[k], nn, nrhs[k]});
+        rj.push_back({&sp, scratch[k], nn, nrhs[k]});
     }
     if (fw.empty()) return;
     launch_ytransform_batch<float>(fw.data(), (int)fw.size(), 0, s);
@@ -407,8 +385,6 @@ static void lines_solve_assemble(fp_cat_s* c, Node& nd, const double* v, double*
     la.lines   = nd.LB.p;
     la.ldl     = nd.ldl;
     la.phi     = nd.phi.p;
-    la.pinv    = nd.plan.pinv64.p;
-    la.ldp     = nd.plan.ldp;
     la.xu      = nd.XU.p;
     la.colpart = nd.CP.p;
     la.nx      = nx;
@@ -1092,7 +1068,7 @@ int fp_sub_chasing(fp_sub sub, double* d_f, int nrhs, long long ld, void* stream
         FP_REQUIRE(sub && d_f && nrhs >= 0, "null argument");
         FP_REQUIRE(ld >= (long long)sub->plan.nx * sub->plan.ny || nrhs <= 1, "ld smaller than nx*ny");
         FP_HIP(hipSetDevice(sub->ctx->device));
-        if (nrhs) fp::launch_tridiag<double>(sub->plan, sub->plan.pinv64.p, d_f, ld, nrhs, (hipStream_t)stream);
+        if (nrhs) fp::launch_tridiag<double>(sub->plan, d_f, ld, nrhs, (hipStream_t)stream);
     });
 }
 

@@ -14,15 +14,29 @@
 // Each output equals the reference's coefficient (same signs and mode order),
 // so transform results are directly comparable with the reference's.
 //
-// The FFT is a mixed-radix Stockham autosort: radix 4/2/3/5 butterflies in
-// registers, any other prime by a two-phase in-LDS DFT.  One workgroup owns
-// one row pair of one right-hand side: it loads both rows (coalesced), runs
-// all passes between two LDS buffers and writes the coefficients back in place.
-#include "fp_dft_tables.h"
+// One workgroup owns one row pair of one right-hand side: it loads both rows
+// (coalesced), runs the FFT in LDS and writes the coefficients back in place.
+// Two FFT engines share those load/store phases:
+//
+//   k_ytransform_sb  (every length whose prime factors are at most 47): a
+//     mixed-radix Stockham FFT on ONE LDS buffer of L values.  Each pass reads
+//     all of its inputs into registers (radix 2/3/4/5/8 butterflies, or the
+//     accumulators of an odd-prime butterfly), the workgroup synchronises, and
+//     the outputs overwrite the buffer.  Half the LDS of a two-buffer FFT, so
+//     twice the workgroups per CU.  Twiddles: one two-level table lookup per
+//     butterfly, the other powers by multiplication.  Odd primes 7..47 run a
+//     symmetric-pair DFT split over groups of threads (a group is a whole
+//     wavefront, so no divergence), its cosines scalar loads from a table.
+//   k_ytransform_gen (fallback for larger prime factors or lengths whose
+//     passes do not fit the per-thread register budget): two LDS buffers,
+//     any prime through an R-term in-LDS sum per output.
 #include "fp_internal.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
+#include <map>
+#include <mutex>
 
 namespace fp {
 
@@ -57,7 +71,8 @@ struct YArgs {
     int          src64, dst64;
     int          nx, N, L, kind, inverse, nrad;
     int          rad[kMaxRadices];
-    const C2<T>* W;
+    const C2<T>* W;    // 64 + L/64 + 1 twiddles, then cos/sin tables of the odd-prime passes
+    int          ncs;  // entries of those tables
     const C2<T>* aux;
     T            c0, c1;
 };
@@ -80,17 +95,265 @@ struct Tw {
     const C2<T>* hi;  // L/64 + 1 entries: W^(64 m)
 };
 
+template <typename T>
+__device__ __forceinline__ C2<T> troot(const Tw<T> W, int t)
+{
+    return cmul(W.hi[t >> 6], W.lo[t & 63]);
+}
+
 template <bool FWD, typename T>
 __device__ __forceinline__ C2<T> twiddle(const Tw<T> W, int t, C2<T> v)
 {
-    const C2<T> w = cmul(W.hi[t >> 6], W.lo[t & 63]);
+    const C2<T> w = troot(W, t);
     return FWD ? cmul(v, w) : cmulc(v, w);
 }
 
+// Radix-R butterfly in registers, R in {2, 3, 4, 5, 8}.
+template <int R, bool FWD, typename T>
+__device__ __forceinline__ void bfly(const C2<T> (&v)[R], C2<T> (&o)[R])
+{
+    if constexpr (R == 2) {
+        o[0] = cadd(v[0], v[1]);
+        o[1] = csub(v[0], v[1]);
+    } else if constexpr (R == 4) {
+        const C2<T> t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+        const C2<T> t2 = cadd(v[1], v[3]), t3 = mul_mi<FWD>(csub(v[1], v[3]));
+        o[0] = cadd(t0, t2);
+        o[2] = csub(t0, t2);
+        o[1] = cadd(t1, t3);
+        o[3] = csub(t1, t3);
+    } else if constexpr (R == 8) {
+        const T     h  = T(0.70710678118654752440084436210485);
+        const C2<T> a0 = cadd(v[0], v[4]), a1 = csub(v[0], v[4]);
+        const C2<T> a2 = cadd(v[2], v[6]), a3 = mul_mi<FWD>(csub(v[2], v[6]));
+        const C2<T> a4 = cadd(v[1], v[5]), a5 = csub(v[1], v[5]);
+        const C2<T> a6 = cadd(v[3], v[7]), a7 = mul_mi<FWD>(csub(v[3], v[7]));
+        const C2<T> b0 = cadd(a0, a2), b2 = csub(a0, a2), b1 = cadd(a1, a3), b3 = csub(a1, a3);
+        C2<T>       c0 = cadd(a4, a6), c2 = csub(a4, a6), c1 = cadd(a5, a7), c3 = csub(a5, a7);
+        // c1 *= w8, c2 *= w8^2 = -+i, c3 *= w8^3  (w8 = exp(-+ i pi / 4))
+        c1 = FWD ? C2<T>{h * (c1.x + c1.y), h * (c1.y - c1.x)} : C2<T>{h * (c1.x - c1.y), h * (c1.y + c1.x)};
+        c2 = mul_mi<FWD>(c2);
+        c3 = FWD ? C2<T>{h * (c3.y - c3.x), -h * (c3.x + c3.y)} : C2<T>{-h * (c3.x + c3.y), h * (c3.x - c3.y)};
+        o[0] = cadd(b0, c0);
+        o[4] = csub(b0, c0);
+        o[1] = cadd(b1, c1);
+        o[5] = csub(b1, c1);
+        o[2] = cadd(b2, c2);
+        o[6] = csub(b2, c2);
+        o[3] = cadd(b3, c3);
+        o[7] = csub(b3, c3);
+    } else if constexpr (R == 3) {
+        const T     h = T(0.86602540378443864676372317075294);
+        const C2<T> s = cadd(v[1], v[2]), d = csub(v[1], v[2]);
+        const C2<T> m = {v[0].x - T(0.5) * s.x, v[0].y - T(0.5) * s.y};
+        const C2<T> e = mul_mi<FWD>(cscale(d, h));
+        o[0] = cadd(v[0], s);
+        o[1] = cadd(m, e);
+        o[2] = csub(m, e);
+    } else if constexpr (R == 5) {
+        const T     c1 = T(0.30901699437494742410229341718282), c2 = T(-0.80901699437494742410229341718282);
+        const T     s1 = T(0.95105651629515357211643933337938), s2 = T(0.58778525229247312916870595463907);
+        const C2<T> a1 = cadd(v[1], v[4]), b1 = csub(v[1], v[4]);
+        const C2<T> a2 = cadd(v[2], v[3]), b2 = csub(v[2], v[3]);
+        const C2<T> m1 = {v[0].x + c1 * a1.x + c2 * a2.x, v[0].y + c1 * a1.y + c2 * a2.y};
+        const C2<T> m2 = {v[0].x + c2 * a1.x + c1 * a2.x, v[0].y + c2 * a1.y + c1 * a2.y};
+        const C2<T> n1 = mul_mi<FWD>(C2<T>{s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y});
+        const C2<T> n2 = mul_mi<FWD>(C2<T>{s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y});
+        o[0] = {v[0].x + a1.x + a2.x, v[0].y + a1.y + a2.y};
+        o[1] = cadd(m1, n1);
+        o[4] = csub(m1, n1);
+        o[2] = cadd(m2, n2);
+        o[3] = csub(m2, n2);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// single-buffer engine
+
+// LDS position of sequence element i: one padding slot after every 2^kZpad
+// elements spreads the strided butterfly writes of the early passes over the
+// LDS banks.
+#ifndef FP_ZPAD
+#define FP_ZPAD 4
+#endif
+constexpr int kZpad = FP_ZPAD;
+template <int PAD>
+__device__ __forceinline__ int zp(int i)
+{
+    if constexpr (PAD == 0) return i;
+    else return i + (i >> PAD);
+}
+
+// butterflies of radix R one thread holds in registers in one pass
+// (16 complex values at most)
+constexpr int kEPT = 16;
+template <int R>
+constexpr int sb_bpt()
+{
+    return (kEPT + R - 1) / R;
+}
+// odd-prime butterflies: output pairs per thread group, groups per thread
+constexpr int kNQ  = 5;
+constexpr int kIPT = 2;
+constexpr int kMaxOddPrime = 47;
+
+// One Stockham pass of radix R on the single buffer Z: read every input,
+// synchronise, twiddle + butterfly, write, synchronise.  Twiddles of a
+// butterfly are w^r, r = 1..R-1, with w = W^(k * stride) from the table and
+// its powers by repeated products (depth <= 3 multiplications).
+template <int R, bool FWD, typename T>
+__device__ __forceinline__ void pass_reg_sb(C2<T>* __restrict__ Z, const Tw<T> W, int L, int Ns)
+{
+    constexpr int BPT = sb_bpt<R>();
+    const int     NT  = blockDim.x;
+    const int     nb = L / R, stride = L / (Ns * R);
+    C2<T>         v[BPT][R];
+#pragma unroll
+    for (int b = 0; b < BPT; b++) {
+        const int jb = threadIdx.x + b * NT;
+        if (jb < nb) {
+#pragma unroll
+            for (int r = 0; r < R; r++) v[b][r] = Z[zp<kZpad>(jb + r * nb)];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < BPT; b++) {
+        const int jb = threadIdx.x + b * NT;
+        if (jb < nb) {
+            const int k = jb % Ns;
+            if (Ns > 1) {
+                C2<T> pw[R];
+                pw[1] = troot(W, k * stride);
+#pragma unroll
+                for (int r = 2; r < R; r++) pw[r] = (r & 1) ? cmul(pw[r - 1], pw[1]) : cmul(pw[r / 2], pw[r / 2]);
+#pragma unroll
+                for (int r = 1; r < R; r++) v[b][r] = FWD ? cmul(v[b][r], pw[r]) : cmulc(v[b][r], pw[r]);
+            }
+            C2<T> o[R];
+            bfly<R, FWD>(v[b], o);
+            const int idxD = (jb / Ns) * Ns * R + k;
+#pragma unroll
+            for (int r = 0; r < R; r++) Z[zp<kZpad>(idxD + r * Ns)] = o[r];
+        }
+    }
+    __syncthreads();
+}
+
+// One Stockham pass of an odd prime radix R (7..47) on the single buffer.
+// With a_r = x_r + x_{R-r}, b_r = x_r - x_{R-r} (twiddled inputs):
+//   X_q = x_0 + A_q -+ i B_q,  X_{R-q} = x_0 + A_q +- i B_q,
+//   A_q = sum_r a_r cos(2 pi q r / R),  B_q = sum_r b_r sin(2 pi q r / R).
+// The H = (R-1)/2 output pairs of a butterfly are split over G groups of kNQ
+// pairs; work item t = (group, butterfly) with the butterflies padded to a
+// multiple of 64, so one wavefront is one group.  The group index is then
+// wavefront-uniform (readfirstlane), and so are the cosines a wavefront
+// needs: a table of (cos, sin)(2 pi q r / R) with r-major rows of G*kNQ
+// values in global memory, read by contiguous scalar loads (scalar cache),
+// not LDS traffic.  Items a thread owns keep their accumulators in registers
+// across the synchronisation.
+template <bool FWD, typename T>
+__device__ __forceinline__ void pass_odd_sb(C2<T>* __restrict__ Z, const Tw<T> W, int L, int Ns, int R,
+                                            const C2<T>* __restrict__ csg)
+{
+    const int NT = blockDim.x;
+    const int H = (R - 1) >> 1, G = (H + kNQ - 1) / kNQ;
+    const int nb = L / R, stride = L / (Ns * R), nbp = (nb + 63) & ~63;
+    T         ax[kIPT][kNQ], ay[kIPT][kNQ], bx[kIPT][kNQ], by[kIPT][kNQ];
+    C2<T>     x0[kIPT];
+#pragma unroll
+    for (int it = 0; it < kIPT; it++) {
+        const int t   = threadIdx.x + it * NT;
+        const int grp = __builtin_amdgcn_readfirstlane(t / nbp);
+        const int jb  = t - grp * nbp;
+        if (grp < G && jb < nb) {
+            const int   QB = 1 + grp * kNQ;
+            const int   k  = jb % Ns;
+            const C2<T> v0 = Z[zp<kZpad>(jb)];
+            x0[it]         = v0;
+#pragma unroll
+            for (int q = 0; q < kNQ; q++) {
+                ax[it][q] = v0.x;
+                ay[it][q] = v0.y;
+                bx[it][q] = T(0);
+                by[it][q] = T(0);
+            }
+            // row r of the table holds (cos, sin)(2 pi q r / R) for q = 1..G*kNQ
+            const C2<T>* crow = csg + (QB - 1);
+            const int    ldc  = G * kNQ;
+#pragma unroll 2
+            for (int r = 1; r <= H; r++) {
+                C2<T> x1 = Z[zp<kZpad>(jb + r * nb)], x2 = Z[zp<kZpad>(jb + (R - r) * nb)];
+                if (Ns > 1) {
+                    x1 = twiddle<FWD>(W, r * k * stride, x1);
+                    x2 = twiddle<FWD>(W, (R - r) * k * stride, x2);
+                }
+                const C2<T> a = cadd(x1, x2), b = csub(x1, x2);
+                x0[it] = cadd(x0[it], a);
+                const C2<T>* cr = crow + (r - 1) * ldc;
+#pragma unroll
+                for (int q = 0; q < kNQ; q++) {
+                    const C2<T> w = cr[q];
+                    ax[it][q] += a.x * w.x;
+                    ay[it][q] += a.y * w.x;
+                    bx[it][q] += b.x * w.y;
+                    by[it][q] += b.y * w.y;
+                }
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < kIPT; it++) {
+        const int t = threadIdx.x + it * NT;
+        const int grp = t / nbp, jb = t - grp * nbp;
+        if (grp < G && jb < nb) {
+            const int QB   = 1 + grp * kNQ;
+            const int k    = jb % Ns;
+            const int base = (jb / Ns) * Ns * R + k;
+            if (grp == 0) Z[zp<kZpad>(base)] = x0[it];
+#pragma unroll
+            for (int q = 0; q < kNQ; q++) {
+                if (QB + q > H) break;
+                const C2<T> xq  = FWD ? C2<T>{ax[it][q] + by[it][q], ay[it][q] - bx[it][q]}
+                                      : C2<T>{ax[it][q] - by[it][q], ay[it][q] + bx[it][q]};
+                const C2<T> xrq = FWD ? C2<T>{ax[it][q] - by[it][q], ay[it][q] + bx[it][q]}
+                                      : C2<T>{ax[it][q] + by[it][q], ay[it][q] - bx[it][q]};
+                Z[zp<kZpad>(base + (QB + q) * Ns)]     = xq;
+                Z[zp<kZpad>(base + (R - QB - q) * Ns)] = xrq;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+template <bool FWD, typename T>
+__device__ __forceinline__ void run_fft_sb(const YArgs<T>& a, const Tw<T> W, C2<T>* Z, const C2<T>* cs)
+{
+    int Ns = 1, co = 0;
+    for (int s = 0; s < a.nrad; s++) {
+        const int R = a.rad[s];
+        if (R == 2) pass_reg_sb<2, FWD>(Z, W, a.L, Ns);
+        else if (R == 3) pass_reg_sb<3, FWD>(Z, W, a.L, Ns);
+        else if (R == 4) pass_reg_sb<4, FWD>(Z, W, a.L, Ns);
+        else if (R == 5) pass_reg_sb<5, FWD>(Z, W, a.L, Ns);
+        else if (R == 8) pass_reg_sb<8, FWD>(Z, W, a.L, Ns);
+        else {
+            pass_odd_sb<FWD>(Z, W, a.L, Ns, R, cs + co);
+            co += ((R - 1) / 2) * (((R - 1) / 2 + kNQ - 1) / kNQ * kNQ);
+        }
+        Ns *= R;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// two-buffer fallback engine
+
 // One Stockham pass with a register butterfly of radix R.
-template <int R, bool FWD, int NT, typename T>
+template <int R, bool FWD, typename T>
 __device__ void pass_reg(const C2<T>* __restrict__ src, C2<T>* __restrict__ dst, const Tw<T> W, int L, int Ns)
 {
+    const int NT = blockDim.x;
     const int nb = L / R, stride = L / (Ns * R);
     for (int jb = threadIdx.x; jb < nb; jb += NT) {
         const int k = jb % Ns;
@@ -101,159 +364,20 @@ __device__ void pass_reg(const C2<T>* __restrict__ src, C2<T>* __restrict__ dst,
             if (r > 0 && Ns > 1) v[r] = twiddle<FWD>(W, r * k * stride, v[r]);
         }
         C2<T> o[R];
-        if constexpr (R == 2) {
-            o[0] = cadd(v[0], v[1]);
-            o[1] = csub(v[0], v[1]);
-        } else if constexpr (R == 4) {
-            const C2<T> t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
-            const C2<T> t2 = cadd(v[1], v[3]), t3 = mul_mi<FWD>(csub(v[1], v[3]));
-            o[0] = cadd(t0, t2);
-            o[2] = csub(t0, t2);
-            o[1] = cadd(t1, t3);
-            o[3] = csub(t1, t3);
-        } else if constexpr (R == 8) {
-            const T     h  = T(0.70710678118654752440084436210485);
-            const C2<T> a0 = cadd(v[0], v[4]), a1 = csub(v[0], v[4]);
-            const C2<T> a2 = cadd(v[2], v[6]), a3 = mul_mi<FWD>(csub(v[2], v[6]));
-            const C2<T> a4 = cadd(v[1], v[5]), a5 = csub(v[1], v[5]);
-            const C2<T> a6 = cadd(v[3], v[7]), a7 = mul_mi<FWD>(csub(v[3], v[7]));
-            const C2<T> b0 = cadd(a0, a2), b2 = csub(a0, a2), b1 = cadd(a1, a3), b3 = csub(a1, a3);
-            C2<T>       c0 = cadd(a4, a6), c2 = csub(a4, a6), c1 = cadd(a5, a7), c3 = csub(a5, a7);
-            // c1 *= w8, c2 *= w8^2 = -+i, c3 *= w8^3  (w8 = exp(-+ i pi / 4))
-            c1 = FWD ? C2<T>{h * (c1.x + c1.y), h * (c1.y - c1.x)} : C2<T>{h * (c1.x - c1.y), h * (c1.y + c1.x)};
-            c2 = mul_mi<FWD>(c2);
-            c3 = FWD ? C2<T>{h * (c3.y - c3.x), -h * (c3.x + c3.y)} : C2<T>{-h * (c3.x + c3.y), h * (c3.x - c3.y)};
-            o[0] = cadd(b0, c0);
-            o[4] = csub(b0, c0);
-            o[1] = cadd(b1, c1);
-            o[5] = csub(b1, c1);
-            o[2] = cadd(b2, c2);
-            o[6] = csub(b2, c2);
-            o[3] = cadd(b3, c3);
-            o[7] = csub(b3, c3);
-        } else if constexpr (R == 3) {
-            const T     h = T(0.86602540378443864676372317075294);
-            const C2<T> s = cadd(v[1], v[2]), d = csub(v[1], v[2]);
-            const C2<T> m = {v[0].x - T(0.5) * s.x, v[0].y - T(0.5) * s.y};
-            const C2<T> e = mul_mi<FWD>(cscale(d, h));
-            o[0] = cadd(v[0], s);
-            o[1] = cadd(m, e);
-            o[2] = csub(m, e);
-        } else if constexpr (R == 5) {
-            const T     c1 = T(0.30901699437494742410229341718282), c2 = T(-0.80901699437494742410229341718282);
-            const T     s1 = T(0.95105651629515357211643933337938), s2 = T(0.58778525229247312916870595463907);
-            const C2<T> a1 = cadd(v[1], v[4]), b1 = csub(v[1], v[4]);
-            const C2<T> a2 = cadd(v[2], v[3]), b2 = csub(v[2], v[3]);
-            const C2<T> m1 = {v[0].x + c1 * a1.x + c2 * a2.x, v[0].y + c1 * a1.y + c2 * a2.y};
-            const C2<T> m2 = {v[0].x + c2 * a1.x + c1 * a2.x, v[0].y + c2 * a1.y + c1 * a2.y};
-            const C2<T> n1 = mul_mi<FWD>(C2<T>{s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y});
-            const C2<T> n2 = mul_mi<FWD>(C2<T>{s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y});
-            o[0] = {v[0].x + a1.x + a2.x, v[0].y + a1.y + a2.y};
-            o[1] = cadd(m1, n1);
-            o[4] = csub(m1, n1);
-            o[2] = cadd(m2, n2);
-            o[3] = csub(m2, n2);
-        }
+        bfly<R, FWD>(v, o);
         const int idxD = (jb / Ns) * Ns * R + k;
 #pragma unroll
         for (int r = 0; r < R; r++) dst[idxD + r * Ns] = o[r];
     }
 }
 
-// One odd-prime butterfly, the share of thread group g: outputs q = QB..QE-1
-// and R-q, plus X_0 for group 0.  With a_r = x_r + x_{R-r}, b_r = x_r - x_{R-r}:
-//   X_q = x_0 + A_q -+ i B_q,  X_{R-q} = x_0 + A_q +- i B_q,
-//   A_q = sum_r a_r cos(2 pi q r / R),  B_q = sum_r b_r sin(2 pi q r / R).
-// The inputs stream through registers one pair at a time; only the group's
-// accumulators stay live.  q and r are compile-time, so the cosines are
-// immediates.
-template <int R, int G, int g, bool FWD, bool TW, typename T>
-__device__ __forceinline__ void odd_group(const C2<T>* __restrict__ src, C2<T>* __restrict__ dst, const Tw<T> W,
-                                          int jb, int nb, int k, int stride, int base, int Ns)
-{
-    constexpr int H  = (R - 1) / 2;
-    constexpr int QP = (H + G - 1) / G;
-    constexpr int QB = 1 + g * QP;
-    constexpr int QE = (QB + QP < H + 1) ? QB + QP : H + 1;
-    constexpr int NQ = QE - QB;
-    const C2<T>   v0 = src[jb];
-    T             ax[NQ], ay[NQ], bx[NQ], by[NQ];
-#pragma unroll
-    for (int q = 0; q < NQ; q++) {
-        ax[q] = v0.x;
-        ay[q] = v0.y;
-        bx[q] = T(0);
-        by[q] = T(0);
-    }
-    T x0x = v0.x, x0y = v0.y;
-#pragma unroll
-    for (int r = 1; r <= H; r++) {
-        C2<T> x1 = src[jb + r * nb], x2 = src[jb + (R - r) * nb];
-        if constexpr (TW) {
-            x1 = twiddle<FWD>(W, r * k * stride, x1);
-            x2 = twiddle<FWD>(W, (R - r) * k * stride, x2);
-        }
-        const C2<T> a = cadd(x1, x2), b = csub(x1, x2);
-        if constexpr (g == 0) {
-            x0x += a.x;
-            x0y += a.y;
-        }
-#pragma unroll
-        for (int q = 0; q < NQ; q++) {
-            const T c  = T(DftTab<R>::c[((QB + q) * r) % R]);
-            const T sn = T(DftTab<R>::s[((QB + q) * r) % R]);
-            ax[q] += a.x * c;
-            ay[q] += a.y * c;
-            bx[q] += b.x * sn;
-            by[q] += b.y * sn;
-        }
-        // one input pair in flight at a time: bounds the live registers to the
-        // accumulators (other waves hide the LDS latency)
-        __builtin_amdgcn_sched_barrier(0);
-    }
-    if constexpr (g == 0) dst[base] = C2<T>{x0x, x0y};
-#pragma unroll
-    for (int q = 0; q < NQ; q++) {
-        const C2<T> xq  = FWD ? C2<T>{ax[q] + by[q], ay[q] - bx[q]} : C2<T>{ax[q] - by[q], ay[q] + bx[q]};
-        const C2<T> xrq = FWD ? C2<T>{ax[q] - by[q], ay[q] + bx[q]} : C2<T>{ax[q] + by[q], ay[q] - bx[q]};
-        dst[base + (QB + q) * Ns]     = xq;
-        dst[base + (R - QB - q) * Ns] = xrq;
-    }
-}
-
-// One Stockham pass of odd prime radix R: G threads share a butterfly, each
-// computing about five of its (q, R-q) output pairs.
-template <int R, bool FWD, int NT, bool TW, typename T>
-__device__ void pass_odd_tw(const C2<T>* __restrict__ src, C2<T>* __restrict__ dst, const Tw<T> W, int L, int Ns)
-{
-    constexpr int H = (R - 1) / 2;
-    constexpr int G = (H + 4) / 5;
-    const int     nb = L / R, stride = L / (Ns * R);
-    for (int t = threadIdx.x; t < nb * G; t += NT) {
-        const int jb = t / G, grp = t - jb * G, k = jb % Ns;
-        const int base = (jb / Ns) * Ns * R + k;
-        if (grp == 0) odd_group<R, G, 0, FWD, TW>(src, dst, W, jb, nb, k, stride, base, Ns);
-        if constexpr (G > 1) if (grp == 1) odd_group<R, G, 1, FWD, TW>(src, dst, W, jb, nb, k, stride, base, Ns);
-        if constexpr (G > 2) if (grp == 2) odd_group<R, G, 2, FWD, TW>(src, dst, W, jb, nb, k, stride, base, Ns);
-        if constexpr (G > 3) if (grp == 3) odd_group<R, G, 3, FWD, TW>(src, dst, W, jb, nb, k, stride, base, Ns);
-        if constexpr (G > 4) if (grp == 4) odd_group<R, G, 4, FWD, TW>(src, dst, W, jb, nb, k, stride, base, Ns);
-    }
-}
-
-// the first pass (Ns = 1) has unit twiddles: a separate instantiation skips them
-template <int R, bool FWD, int NT, typename T>
-__device__ void pass_odd(const C2<T>* __restrict__ src, C2<T>* __restrict__ dst, const Tw<T> W, int L, int Ns)
-{
-    if (Ns == 1) pass_odd_tw<R, FWD, NT, false>(src, dst, W, L, Ns);
-    else pass_odd_tw<R, FWD, NT, true>(src, dst, W, L, Ns);
-}
-
 // Stockham pass of any radix R: twiddle in place, then each output is an
 // R-term sum read from LDS.
-template <bool FWD, int NT, typename T>
+template <bool FWD, typename T>
 __device__ void pass_generic(C2<T>* __restrict__ src, C2<T>* __restrict__ dst, const Tw<T> W, int L, int Ns,
                              int R)
 {
+    const int NT = blockDim.x;
     const int nb = L / R, stride = L / (Ns * R);
     for (int idx = threadIdx.x; idx < L; idx += NT) {
         const int r = idx / nb, jb = idx - r * nb, k = jb % Ns;
@@ -273,25 +397,20 @@ __device__ void pass_generic(C2<T>* __restrict__ src, C2<T>* __restrict__ dst, c
     }
 }
 
-// Radix dispatch: 2, 3, 4, 5, 7, 8 in every kernel, one larger odd prime P1
-// per kernel instantiation (keeping one codelet's registers per kernel), any
-// other factor through the generic pass.
-template <int P1, bool FWD, int NT, typename T>
-__device__ C2<T>* run_fft(const YArgs<T>& a, const Tw<T> W, C2<T>* A, C2<T>* B)
+template <bool FWD, typename T>
+__device__ C2<T>* run_fft_gen(const YArgs<T>& a, const Tw<T> W, C2<T>* A, C2<T>* B)
 {
     C2<T>* src = A;
     C2<T>* dst = B;
     int    Ns  = 1;
     for (int s = 0; s < a.nrad; s++) {
         const int R = a.rad[s];
-        if (R == 2) pass_reg<2, FWD, NT>(src, dst, W, a.L, Ns);
-        else if (R == 3) pass_reg<3, FWD, NT>(src, dst, W, a.L, Ns);
-        else if (R == 4) pass_reg<4, FWD, NT>(src, dst, W, a.L, Ns);
-        else if (R == 5) pass_reg<5, FWD, NT>(src, dst, W, a.L, Ns);
-        else if (R == 8) pass_reg<8, FWD, NT>(src, dst, W, a.L, Ns);
-        else if (R == 7) pass_odd<7, FWD, NT>(src, dst, W, a.L, Ns);
-        else if (P1 > 7 && R == P1) pass_odd<(P1 > 7 ? P1 : 11), FWD, NT>(src, dst, W, a.L, Ns);
-        else pass_generic<FWD, NT>(src, dst, W, a.L, Ns, R);
+        if (R == 2) pass_reg<2, FWD>(src, dst, W, a.L, Ns);
+        else if (R == 3) pass_reg<3, FWD>(src, dst, W, a.L, Ns);
+        else if (R == 4) pass_reg<4, FWD>(src, dst, W, a.L, Ns);
+        else if (R == 5) pass_reg<5, FWD>(src, dst, W, a.L, Ns);
+        else if (R == 8) pass_reg<8, FWD>(src, dst, W, a.L, Ns);
+        else pass_generic<FWD>(src, dst, W, a.L, Ns, R);
         __syncthreads();
         C2<T>* t = src;
         src      = dst;
@@ -301,67 +420,64 @@ __device__ C2<T>* run_fft(const YArgs<T>& a, const Tw<T> W, C2<T>* A, C2<T>* B)
     return src;
 }
 
+// ---------------------------------------------------------------------------
+// load and store phases (both engines)
+
 template <typename T>
 __device__ __forceinline__ T sgn_pow(int p) { return (p & 1) ? T(-1) : T(1); }
 
 // values per thread and row held in registers by the load and store phases
 constexpr int VPT = 8;
 
-template <int NT, int P1, typename T>
-__global__ __launch_bounds__(NT) void k_ytransform(YBatch<T> bt)
+template <typename T>
+struct RowPair {
+    long long ia, oa;  // row a in src / dst
+    bool      hb;      // row b exists
+};
+
+template <typename T>
+__device__ __forceinline__ RowPair<T> row_pair(const YArgs<T>& a, int local)
 {
-    int q = 0;
-    while (q + 1 < bt.nprob && (int)blockIdx.x >= bt.start[q + 1]) q++;
-    const YArgs<T>& a      = bt.p[q];
-    const int       npairs = (a.nx + 1) >> 1;
-    const int       local  = (int)blockIdx.x - bt.start[q];
-    const int       rhs    = local / npairs, pair = local - rhs * npairs;
+    const int  npairs = (a.nx + 1) >> 1;
+    const int  rhs = local / npairs, pair = local - rhs * npairs;
+    const int  ra = 2 * pair;
+    RowPair<T> rp;
+    rp.hb = ra + 1 < a.nx;
+    rp.ia = (long long)rhs * a.lds + (long long)ra * a.N;
+    rp.oa = (long long)rhs * a.ldd + (long long)ra * a.N;
+    return rp;
+}
 
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    C2<T>* A   = reinterpret_cast<C2<T>*>(smem_raw);
-    C2<T>* B   = A + a.L;
-    C2<T>* tlo = B + a.L;           // 64 twiddles W^m
-    C2<T>* thi = tlo + 64;          // L/64 + 1 twiddles W^(64m)
-    const int nhi = a.L / 64 + 1;
-    T*     y0  = reinterpret_cast<T*>(thi + nhi);  // NN inverse: y_0 of rows a, b
-    for (int q = threadIdx.x; q < 64 + nhi; q += NT) tlo[q] = a.W[q];
-    const Tw<T> W{tlo, thi};
-
+// Builds the packed extended sequence of the row pair in A (length L).
+template <int PAD, typename T>
+__device__ __forceinline__ void load_rows(const YArgs<T>& a, const RowPair<T>& rp, C2<T>* A, T* y0)
+{
+    const int  NT = blockDim.x;
     const int  N = a.N, L = a.L;
-    const int  ra = 2 * pair, rb = ra + 1;
-    const bool hb = rb < a.nx;
-    const long long ia = (long long)rhs * a.lds + (long long)ra * N;  // row a in src
-    const long long oa0 = (long long)rhs * a.ldd + (long long)ra * N;  // row a in dst
-    const bool s64 = sizeof(T) == 8 || a.src64, d64 = sizeof(T) == 8 || a.dst64;
-    auto ld_src = [&](long long q) -> T {
-        return s64 ? (T)static_cast<const double*>(a.src)[q] : static_cast<const T*>(a.src)[q];
+    const bool s64 = sizeof(T) == 8 || a.src64;
+    auto       ld_src = [&](long long q) -> T {
+        return s64 ? (T) static_cast<const double*>(a.src)[q] : static_cast<const T*>(a.src)[q];
     };
-    auto st_dst = [&](long long q, T v) {
-        if (d64) static_cast<double*>(a.dst)[q] = (double)v;
-        else static_cast<T*>(a.dst)[q] = v;
-    };
-
-    // ---- load: build the packed extended sequence in A ----
     // The rows are read in chunks of VPT values per thread with every global
     // load of a chunk issued before the first LDS write, so the HBM latency
     // is paid once per chunk instead of once per element.
     const int kind = a.kind, inv = a.inverse;
     if (kind == T_DD) {
-        if (threadIdx.x == 0) A[0] = {T(0), T(0)};
-        if (threadIdx.x == 1) A[N + 1] = {T(0), T(0)};
+        if (threadIdx.x == 0) A[zp<PAD>(0)] = {T(0), T(0)};
+        if (threadIdx.x == 1) A[zp<PAD>(N + 1)] = {T(0), T(0)};
     } else if (kind == T_NN && inv) {
-        if (threadIdx.x == 0) A[N] = {T(0), T(0)};
+        if (threadIdx.x == 0) A[zp<PAD>(N)] = {T(0), T(0)};
     } else if (kind != T_NN) {
-        if (threadIdx.x == 0) A[0] = {T(0), T(0)};
-        for (int p = N + 1 + threadIdx.x; p < L; p += NT) A[p] = {T(0), T(0)};
+        if (threadIdx.x == 0) A[zp<PAD>(0)] = {T(0), T(0)};
+        for (int p = N + 1 + threadIdx.x; p < L; p += NT) A[zp<PAD>(p)] = {T(0), T(0)};
     }
     for (int base = 0; base < N; base += VPT * NT) {
         T va[VPT], vb[VPT];
 #pragma unroll
         for (int k = 0; k < VPT; k++) {
             const int n = base + k * NT + threadIdx.x;
-            va[k]       = n < N ? ld_src(ia + n) : T(0);
-            vb[k]       = (n < N && hb) ? ld_src(ia + N + n) : T(0);
+            va[k]       = n < N ? ld_src(rp.ia + n) : T(0);
+            vb[k]       = (n < N && rp.hb) ? ld_src(rp.ia + N + n) : T(0);
         }
 #pragma unroll
         for (int k = 0; k < VPT; k++) {
@@ -369,36 +485,44 @@ __global__ __launch_bounds__(NT) void k_ytransform(YBatch<T> bt)
             if (n >= N) break;
             const T x = va[k], y = vb[k];
             if (kind == T_DD) {
-                A[n + 1]     = {x, y};
-                A[L - 1 - n] = {-x, -y};
+                A[zp<PAD>(n + 1)]     = {x, y};
+                A[zp<PAD>(L - 1 - n)] = {-x, -y};
             } else if (kind == T_NN && !inv) {
-                A[n]             = {x, y};
-                A[2 * N - 1 - n] = {x, y};
+                A[zp<PAD>(n)]             = {x, y};
+                A[zp<PAD>(2 * N - 1 - n)] = {x, y};
             } else if (kind == T_NN) {
                 const int   m  = N - 1 - n;
                 const T     s  = m == 0 ? a.c1 : a.c0;  // sqrt(1/N) or -sqrt(2/N)
                 const T     ya = s * x, yb = s * y;
                 const C2<T> e  = a.aux[m];              // exp(-i pi m / 2N); Y_m uses conj(e)
-                A[m]           = {ya * e.x + yb * e.y, -ya * e.y + yb * e.x};
-                if (m > 0) A[2 * N - m] = {ya * e.x - yb * e.y, ya * e.y + yb * e.x};
+                A[zp<PAD>(m)]           = {ya * e.x + yb * e.y, -ya * e.y + yb * e.x};
+                if (m > 0) A[zp<PAD>(2 * N - m)] = {ya * e.x - yb * e.y, ya * e.y + yb * e.x};
                 else {
                     y0[0] = ya;
                     y0[1] = yb;
                 }
             } else if (kind == T_ND && inv) {
-                A[n + 1] = {x, y};
+                A[zp<PAD>(n + 1)] = {x, y};
             } else {  // DN both ways, ND forward
                 const T s = sgn_pow<T>(n + 1) * a.c0;
-                A[n + 1]  = {s * x, s * y};
+                A[zp<PAD>(n + 1)]  = {s * x, s * y};
             }
         }
     }
-    __syncthreads();
+}
 
-    const bool   fwd = !(a.kind == T_NN && a.inverse);
-    const C2<T>* Z   = fwd ? run_fft<P1, true, NT>(a, W, A, B) : run_fft<P1, false, NT>(a, W, A, B);
-
-    // ---- store: split the two rows' coefficients ----
+// Splits the two rows' coefficients out of the spectrum Z and stores them.
+template <int PAD, typename T>
+__device__ __forceinline__ void store_rows(const YArgs<T>& a, const RowPair<T>& rp, const C2<T>* Z, const T* y0)
+{
+    const int  NT = blockDim.x;
+    const int  N = a.N, L = a.L;
+    const int  kind = a.kind, inv = a.inverse;
+    const bool d64 = sizeof(T) == 8 || a.dst64;
+    auto       st_dst = [&](long long q, T v) {
+        if (d64) static_cast<double*>(a.dst)[q] = (double)v;
+        else static_cast<T*>(a.dst)[q] = v;
+    };
     for (int base = 0; base < N; base += VPT * NT) {
         C2<T> sc[VPT];
         if (kind == T_NN && !inv) {
@@ -420,48 +544,110 @@ __global__ __launch_bounds__(NT) void k_ytransform(YBatch<T> bt)
             if (j >= N) break;
             T oa, ob;
             if (kind == T_DD) {
-                const C2<T> z = Z[j + 1];
+                const C2<T> z = Z[zp<PAD>(j + 1)];
                 oa            = -a.c0 * z.y;
                 ob            = a.c0 * z.x;
             } else if (kind == T_NN && !inv) {
                 const int   m = N - 1 - j;
-                const C2<T> w = cmul(Z[m], sc[k]);  // 2 C_a(m) + 2i C_b(m)
+                const C2<T> w = cmul(Z[zp<PAD>(m)], sc[k]);  // 2 C_a(m) + 2i C_b(m)
                 const T     s = T(0.5) * ((j == N - 1) ? a.c1 : a.c0);
                 oa            = s * w.x;
                 ob            = s * w.y;
             } else if (kind == T_NN) {
-                const C2<T> z = Z[j];
+                const C2<T> z = Z[zp<PAD>(j)];
                 oa            = T(0.5) * (z.x + y0[0]);
                 ob            = T(0.5) * (z.y + y0[1]);
             } else if (kind == T_DN) {
-                const C2<T> zm = Z[j + 1], zc = Z[L - 1 - j];
+                const C2<T> zm = Z[zp<PAD>(j + 1)], zc = Z[zp<PAD>(L - 1 - j)];
                 const T     s  = sgn_pow<T>(j) * T(0.5);
                 oa             = s * (zm.y - zc.y);
                 ob             = s * (zc.x - zm.x);
             } else if (!inv) {  // ND forward
-                const C2<T> zm = Z[j + 1], zc = Z[L - 1 - j];
+                const C2<T> zm = Z[zp<PAD>(j + 1)], zc = Z[zp<PAD>(L - 1 - j)];
                 const T     rea = T(0.5) * (zm.x + zc.x), ima = T(0.5) * (zm.y - zc.y);
                 const T     reb = T(0.5) * (zm.y + zc.y), imb = T(0.5) * (zc.x - zm.x);
                 oa              = sc[k].x * rea - sc[k].y * ima;
                 ob              = sc[k].x * reb - sc[k].y * imb;
             } else {  // ND inverse
                 const int   kk = N - j;
-                const C2<T> zk = Z[kk], zc = Z[L - kk];
+                const C2<T> zk = Z[zp<PAD>(kk)], zc = Z[zp<PAD>(L - kk)];
                 const T     s  = sgn_pow<T>(j + 1) * a.c0 * T(0.5);
                 oa             = s * (zk.y - zc.y);
                 ob             = s * (zc.x - zk.x);
             }
-            st_dst(oa0 + j, oa);
-            if (hb) st_dst(oa0 + N + j, ob);
+            st_dst(rp.oa + j, oa);
+            if (rp.hb) st_dst(rp.oa + N + j, ob);
         }
     }
+}
+
+template <typename T>
+__device__ __forceinline__ const YArgs<T>& find_problem(const YBatch<T>& bt, int& local)
+{
+    int q = 0;
+    while (q + 1 < bt.nprob && (int)blockIdx.x >= bt.start[q + 1]) q++;
+    local = (int)blockIdx.x - bt.start[q];
+    return bt.p[q];
+}
+
+// threads per workgroup: chosen per plan (128, 256 or 512)
+constexpr int kMaxNT = 512;
+
+template <typename T>
+__global__ __launch_bounds__(kMaxNT) void k_ytransform_sb(YBatch<T> bt)
+{
+    int             local;
+    const YArgs<T>& a  = find_problem(bt, local);
+    const RowPair<T> rp = row_pair(a, local);
+    const int       NT = blockDim.x;
+
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C2<T>*    Z   = reinterpret_cast<C2<T>*>(smem_raw);
+    C2<T>*    tlo = Z + zp<kZpad>(a.L) + 1;  // 64 twiddles W^m
+    const int nhi = a.L / 64 + 1;
+    C2<T>*    thi = tlo + 64;           // L/64 + 1 twiddles W^(64m)
+    T*        y0  = reinterpret_cast<T*>(thi + nhi);  // NN inverse: y_0 of rows a, b
+    const C2<T>* cs = a.W + 64 + nhi;   // cos/sin tables of the odd-prime passes (global)
+    for (int q = threadIdx.x; q < 64 + nhi; q += NT) tlo[q] = a.W[q];
+    const Tw<T> W{tlo, thi};
+
+    load_rows<kZpad>(a, rp, Z, y0);
+    __syncthreads();
+    if (!(a.kind == T_NN && a.inverse)) run_fft_sb<true>(a, W, Z, cs);
+    else run_fft_sb<false>(a, W, Z, cs);
+    store_rows<kZpad>(a, rp, Z, y0);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kMaxNT) void k_ytransform_gen(YBatch<T> bt)
+{
+    int             local;
+    const YArgs<T>& a  = find_problem(bt, local);
+    const RowPair<T> rp = row_pair(a, local);
+    const int       NT = blockDim.x;
+
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C2<T>*    A   = reinterpret_cast<C2<T>*>(smem_raw);
+    C2<T>*    B   = A + a.L;
+    C2<T>*    tlo = B + a.L;
+    C2<T>*    thi = tlo + 64;
+    const int nhi = a.L / 64 + 1;
+    T*        y0  = reinterpret_cast<T*>(thi + nhi);
+    for (int q = threadIdx.x; q < 64 + nhi; q += NT) tlo[q] = a.W[q];
+    const Tw<T> W{tlo, thi};
+
+    load_rows<0>(a, rp, A, y0);
+    __syncthreads();
+    const bool   fwd = !(a.kind == T_NN && a.inverse);
+    const C2<T>* Z   = fwd ? run_fft_gen<true>(a, W, A, B) : run_fft_gen<false>(a, W, A, B);
+    store_rows<0>(a, rp, Z, y0);
 }
 
 // ---------------------------------------------------------------------------
 // host side
 
-// Radix schedule: the odd primes above 5 first (their passes then run with
-// Ns = 1, i.e. without twiddle factors), then 8, 4, 2, 3, 5.
+// Radix schedule: the odd primes above 5 first, largest first (their passes
+// then run with Ns = 1, i.e. without twiddle factors), then 8, 4, 2, 3, 5.
 static void factor_radices(int L, int* rad, int& nrad)
 {
     nrad  = 0;
@@ -489,6 +675,43 @@ static void factor_radices(int L, int* rad, int& nrad)
     if (n != 1) throw Error(FP_E_INVALID, "radix schedule does not cover the transform length");
 }
 
+// Whether the single-buffer engine runs this schedule with NT threads: every
+// pass's butterflies (or odd-prime work items) fit the per-thread registers.
+static int sb_bpt_rt(int R)
+{
+    switch (R) {
+        case 2: return sb_bpt<2>();
+        case 3: return sb_bpt<3>();
+        case 4: return sb_bpt<4>();
+        case 5: return sb_bpt<5>();
+        case 8: return sb_bpt<8>();
+        default: return 0;
+    }
+}
+static bool sb_fits(const int* rad, int nrad, int L, int NT)
+{
+    for (int s = 0; s < nrad; s++) {
+        const int R = rad[s], nb = L / R;
+        if (R <= 8 && R != 7) {
+            if ((nb + NT - 1) / NT > sb_bpt_rt(R)) return false;
+        } else {
+            if (R > kMaxOddPrime) return false;
+            const int H = (R - 1) / 2, G = (H + kNQ - 1) / kNQ, nbp = (nb + 63) / 64 * 64;
+            if ((nbp * G + NT - 1) / NT > kIPT) return false;
+        }
+    }
+    return true;
+}
+
+template <typename T>
+static size_t lds_bytes(const TransformPlan<T>& tp)
+{
+    const size_t nhi = tp.L / 64 + 1;
+    const size_t zl  = (size_t)tp.L + (kZpad ? ((size_t)tp.L >> kZpad) : 0) + 1;
+    if (tp.engine == 1) return (zl + 64 + nhi) * sizeof(C2<T>) + 4 * sizeof(T);
+    return (2 * (size_t)tp.L + 64 + nhi) * sizeof(C2<T>) + 4 * sizeof(T);
+}
+
 template <typename T>
 void TransformPlan<T>::build(int k, int n)
 {
@@ -498,22 +721,31 @@ void TransformPlan<T>::build(int k, int n)
     else if (kind == T_NN) L = 2 * N;
     else L = 2 * N + 1;
     factor_radices(L, rad, nrad);
-    p1 = 0;
-    for (int i = 0; i < nrad; i++) {
-        const int r = rad[i];
-        if ((r == 11 || r == 13 || r == 17 || r == 19 || r == 23 || r == 29 || r == 31 || r == 37 || r == 41 ||
-             r == 43 || r == 47) && r > p1)
-            p1 = r;
-    }
     const long double PI  = 3.141592653589793238462643383279502884L;
     const int         nhi = L / 64 + 1;
-    std::vector<T>    w(2 * (size_t)(64 + nhi)), x(2 * (size_t)N, T(0));
+    // odd-prime cos/sin tables, one per odd-prime pass in schedule order
+    std::vector<long double> cs;
+    ncs = 0;
+    for (int s = 0; s < nrad; s++) {
+        const int R = rad[s];
+        if (R <= 8 && R != 7) continue;
+        const int H = (R - 1) / 2, QW = (H + kNQ - 1) / kNQ * kNQ;
+        for (int r = 1; r <= H; r++)
+            for (int q = 1; q <= QW; q++) {
+                const int m = (int)(((long long)q * r) % R);
+                cs.push_back(q <= H ? cosl(2.0L * PI * m / R) : 0.0L);
+                cs.push_back(q <= H ? sinl(2.0L * PI * m / R) : 0.0L);
+            }
+        ncs += H * QW;
+    }
+    std::vector<T> w(2 * (size_t)(64 + nhi + ncs)), x(2 * (size_t)N, T(0));
     for (int q = 0; q < 64 + nhi; q++) {
         const long long   t   = q < 64 ? q : 64LL * (q - 64);
         const long double ang = -2.0L * PI * (long double)(t % L) / (long double)L;
         w[2 * q]              = (T)cosl(ang);
         w[2 * q + 1]          = (T)sinl(ang);
     }
+    for (int q = 0; q < 2 * ncs; q++) w[2 * (size_t)(64 + nhi) + q] = (T)cs[q];
     if (kind == T_NN) {
         for (int m = 0; m < N; m++) {
             const long double ang = -PI * (long double)m / (2.0L * N);
@@ -536,6 +768,22 @@ void TransformPlan<T>::build(int k, int n)
     } else {
         c0 = (T)(std::sqrt(2.0 / (N + 1.0)) * 0.5);
     }
+    // engine: single buffer at 256 threads when it fits (the tuner may change
+    // the thread count among the sizes that fit), else the two-buffer fallback
+    engine = 0;
+    nt     = 256;
+    for (int t : {256, 512, 128})
+        if (sb_fits(rad, nrad, L, t)) {
+            engine = 1;
+            nt     = t;
+            break;
+        }
+    if (lds_bytes(*this) > 160 * 1024) {
+        // the fallback needs twice the LDS: keep the single buffer if it fits
+        if (engine == 0) throw Error(FP_E_INVALID, "transform length exceeds the LDS of one workgroup");
+    }
+    if (engine == 1 && lds_bytes(*this) > 160 * 1024)
+        throw Error(FP_E_INVALID, "transform length exceeds the LDS of one workgroup");
     W.alloc(w.size());
     W.upload(w.data(), w.size());
     aux.alloc(2 * (size_t)N);
@@ -564,27 +812,21 @@ static YArgs<T> yargs(const TransformJob<T>& j, int inverse)
     a.nrad    = tp.nrad;
     for (int i = 0; i < tp.nrad; i++) a.rad[i] = tp.rad[i];
     a.W   = reinterpret_cast<const C2<T>*>(tp.W.p);
+    a.ncs = tp.engine == 1 ? tp.ncs : 0;
     a.aux = reinterpret_cast<const C2<T>*>(tp.aux.p);
     a.c0  = tp.c0;
     a.c1  = tp.c1;
     return a;
 }
 
+// one launch of up to kMaxBatch jobs of one engine and thread count
 template <typename T>
-static size_t ytransform_lds(int L)
+static void launch_group(const TransformJob<T>* const* jobs, int n, int engine, int NT, int inverse, hipStream_t s)
 {
-    return (2 * (size_t)L + 64 + L / 64 + 1) * sizeof(C2<T>) + 4 * sizeof(T);
-}
-
-// one launch of up to kMaxBatch jobs with the register codelet P1
-template <typename T>
-static void launch_group(const TransformJob<T>* const* jobs, int n, int P1, int inverse, hipStream_t s)
-{
-    constexpr int NT = 256;
-    YBatch<T>     bt{};
-    size_t        lds = 0;
-    int           blocks = 0;
-    bt.nprob           = n;
+    YBatch<T> bt{};
+    size_t    lds    = 0;
+    int       blocks = 0;
+    bt.nprob         = n;
     for (int q = 0; q < n; q++) {
         const TransformJob<T>& j = *jobs[q];
         bt.p[q]                  = yargs(j, inverse);
@@ -592,73 +834,38 @@ static void launch_group(const TransformJob<T>* const* jobs, int n, int P1, int 
         const long long nb       = (long long)((j.nx + 1) / 2) * j.nrhs;
         if (blocks + nb > (1LL << 30)) throw Error(FP_E_INVALID, "too many transform workgroups in one launch");
         blocks += (int)nb;
-        lds = std::max(lds, ytransform_lds<T>(j.plan->L));
+        lds = std::max(lds, lds_bytes(*j.plan));
     }
     bt.start[n] = blocks;
     if (lds > 160 * 1024) throw Error(FP_E_INVALID, "transform length exceeds the LDS of one workgroup");
     if (blocks == 0) return;
-    switch (P1) {
-#define FP_P1_CASE(P)                                                                                            \
-    case P: {                                                                                                    \
-        static bool attr_set = false;                                                                            \
-        if (lds > 64 * 1024 && !attr_set) {                                                                      \
-            FP_HIP(hipFuncSetAttribute((const void*)k_ytransform<NT, P, T>,                                      \
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));                 \
-            attr_set = true;                                                                                     \
-        }                                                                                                        \
-        hipLaunchKernelGGL((k_ytransform<NT, P, T>), dim3(blocks), dim3(NT), lds, s, bt);                        \
-        break;                                                                                                   \
+    const void* fn = engine == 1 ? (const void*)k_ytransform_sb<T> : (const void*)k_ytransform_gen<T>;
+    static bool attr_set[2] = {false, false};
+    if (lds > 64 * 1024 && !attr_set[engine]) {
+        FP_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        attr_set[engine] = true;
     }
-        FP_P1_CASE(0)
-        FP_P1_CASE(11)
-        FP_P1_CASE(13)
-        FP_P1_CASE(17)
-        FP_P1_CASE(19)
-        FP_P1_CASE(23)
-        FP_P1_CASE(29)
-        FP_P1_CASE(31)
-        FP_P1_CASE(37)
-        FP_P1_CASE(41)
-        FP_P1_CASE(43)
-        FP_P1_CASE(47)
-#undef FP_P1_CASE
-        default: throw Error(FP_E_INVALID, "no transform kernel for this radix schedule");
-    }
+    if (engine == 1) hipLaunchKernelGGL(k_ytransform_sb<T>, dim3(blocks), dim3(NT), lds, s, bt);
+    else hipLaunchKernelGGL(k_ytransform_gen<T>, dim3(blocks), dim3(NT), lds, s, bt);
     FP_HIP(hipGetLastError());
 }
 
-// Jobs are grouped by their large-prime codelet (jobs without one join the
-// first group) and launched kMaxBatch at a time.
+// Jobs are grouped by (engine, threads per workgroup) and launched kMaxBatch
+// at a time.
 template <typename T>
 void launch_ytransform_batch(const TransformJob<T>* jobs, int njobs, int inverse, hipStream_t s)
 {
-    std::vector<int>                           keys;
-    std::vector<std::vector<const TransformJob<T>*>> groups;
-    std::vector<const TransformJob<T>*>        plain;
+    std::map<std::pair<int, int>, std::vector<const TransformJob<T>*>> groups;
     for (int k = 0; k < njobs; k++) {
         if (jobs[k].nrhs <= 0 || jobs[k].nx <= 0) continue;
-        const int p1 = jobs[k].plan->p1;
-        if (p1 == 0) {
-            plain.push_back(&jobs[k]);
-            continue;
-        }
-        size_t g = 0;
-        while (g < keys.size() && keys[g] != p1) g++;
-        if (g == keys.size()) {
-            keys.push_back(p1);
-            groups.emplace_back();
-        }
-        groups[g].push_back(&jobs[k]);
+        groups[{jobs[k].plan->engine, jobs[k].plan->nt}].push_back(&jobs[k]);
     }
-    if (groups.empty()) {
-        keys.push_back(0);
-        groups.emplace_back();
+    for (auto& kv : groups) {
+        auto& g = kv.second;
+        for (size_t q0 = 0; q0 < g.size(); q0 += kMaxBatch)
+            launch_group<T>(g.data() + q0, (int)std::min<size_t>(kMaxBatch, g.size() - q0), kv.first.first,
+                            kv.first.second, inverse, s);
     }
-    groups[0].insert(groups[0].end(), plain.begin(), plain.end());
-    for (size_t g = 0; g < groups.size(); g++)
-        for (size_t q0 = 0; q0 < groups[g].size(); q0 += kMaxBatch)
-            launch_group<T>(groups[g].data() + q0, (int)std::min<size_t>(kMaxBatch, groups[g].size() - q0), keys[g],
-                            inverse, s);
 }
 
 template <typename T>
@@ -674,6 +881,59 @@ void launch_ytransform(const TransformPlan<T>& tp, T* data, long long ld, int nx
     j.dst64 = sizeof(T) == 8;
     launch_ytransform_batch<T>(&j, 1, inverse, s);
 }
+
+// Threads per workgroup of a plan's transform, chosen by timing the forward
+// transform of an nx-row scratch field (one right-hand side) with every
+// workgroup size its engine supports.  Cached per (kind, N, nx, precision)
+// for the process.
+static std::mutex                        g_tt_mu;
+static std::map<std::array<int, 4>, int> g_tt_cache;
+
+template <typename T>
+void tune_transform(TransformPlan<T>& tp, int nx, bool measure)
+{
+    std::vector<int> cand;
+    for (int t : {128, 256, 512})
+        if (tp.engine == 0 ? t == 256 : sb_fits(tp.rad, tp.nrad, tp.L, t)) cand.push_back(t);
+    if (!measure || cand.size() < 2 || (long long)nx * tp.N < (1LL << 16)) return;
+    const std::array<int, 4> key{tp.kind, tp.N, nx, (int)sizeof(T)};
+    {
+        std::lock_guard<std::mutex> lk(g_tt_mu);
+        auto                        it = g_tt_cache.find(key);
+        if (it != g_tt_cache.end()) {
+            tp.nt = it->second;
+            return;
+        }
+    }
+    DevBuf<T> buf((size_t)nx * tp.N);
+    FP_HIP(hipMemset(buf.p, 0, (size_t)nx * tp.N * sizeof(T)));
+    hipEvent_t e0, e1;
+    FP_HIP(hipEventCreate(&e0));
+    FP_HIP(hipEventCreate(&e1));
+    float best    = 1e30f;
+    int   nt_best = tp.nt;
+    for (int nt : cand) {
+        tp.nt = nt;
+        launch_ytransform<T>(tp, buf.p, (long long)nx * tp.N, nx, 1, 0, nullptr);
+        FP_HIP(hipEventRecord(e0, nullptr));
+        for (int r = 0; r < 5; r++) launch_ytransform<T>(tp, buf.p, (long long)nx * tp.N, nx, 1, 0, nullptr);
+        FP_HIP(hipEventRecord(e1, nullptr));
+        FP_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        FP_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+            best    = ms;
+            nt_best = nt;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    tp.nt = nt_best;
+    std::lock_guard<std::mutex> lk(g_tt_mu);
+    g_tt_cache[key] = nt_best;
+}
+template void tune_transform<double>(TransformPlan<double>&, int, bool);
+template void tune_transform<float>(TransformPlan<float>&, int, bool);
 
 template void launch_ytransform_batch<double>(const TransformJob<double>*, int, int, hipStream_t);
 template void launch_ytransform_batch<float>(const TransformJob<float>*, int, int, hipStream_t);

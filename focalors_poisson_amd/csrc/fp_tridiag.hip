@@ -9,76 +9,103 @@
 // row access is coalesced and the reference's two transposes
 // (poisson_solver2d.cpp:74,78) disappear.
 //
-// Parallel Thomas: the LU factor of tridiag(1, d_j, 1) with the Neumann end
-// pivots is precomputed per (row, mode) as the reciprocal pivot p_i (the
-// reference's c_i).  The forward sweep y_i = p_i (f_i - y_{i-1}) and the back
-// substitution x_i = y_i - p_i x_{i+1} are first-order affine recurrences, so
-// each thread owns a chunk of CH rows of one mode, composes its chunk's affine
-// map, the chunk maps are chained in LDS, and every thread then replays its
-// chunk from the correct start value.  The whole column stays in registers:
-// one read of f and p, one write of x.  The singular zero mode (x_0 = 0,
+// Parallel Thomas: the LU factor of tridiag(1, b_i, 1) (b_i = the mode's
+// diagonal d_j, plus 1 on a Neumann end row) has the reciprocal pivots
+//   p_0 = 1 / b_0,   p_i = 1 / (b_i - p_{i-1})       (the reference's c_i).
+// The forward sweep y_i = p_i (f_i - y_{i-1}) and the back substitution
+// x_i = y_i - p_i x_{i+1} are first-order affine recurrences, so each thread
+// owns a chunk of CH rows of one mode, composes its chunk's affine map, the
+// chunk maps are chained in LDS, and every thread then replays its chunk from
+// the correct start value.  The pivots are not read from memory: each thread
+// regenerates its CH pivots from the pivot that precedes its chunk (a table
+// of one value per chunk and mode, 1/CH of the field) with the same IEEE
+// divisions as the host's recurrence, so they are bit-identical to it.  The
+// kernel therefore moves one read and one write of the field and nothing
+// else of its size.  The singular zero mode (x_0 = 0,
 // x_{i+1} = f_i - x_{i-1} + 2 x_i) is two prefix sums and uses the same chain.
 #include "fp_internal.h"
 
 #include <algorithm>
+#include <map>
+#include <mutex>
 
 namespace fp {
 
-// Start values of the CH-row chunks of each mode: the chunk maps
-// v -> SA v + SB (LDS, chunk-major) are chained in application order and the
-// exclusive prefix, applied to 0, is written to SY.  backward = 1 chains the
-// chunks from the last row upwards (back substitution); mode sing_j always
-// chains forwards (the two prefix sums of the singular recurrence).
-// With P <= 64 one wavefront scans one mode with a log-depth affine scan;
-// longer chains fall back to one thread per mode.
+// Start values of the chunks of each mode: the chunk maps v -> SA v + SB
+// (LDS, chunk-major) are chained in application order and the exclusive
+// prefix, applied to 0, is written to SY.  backward = 1 chains the chunks from
+// the last row upwards (back substitution); mode sing_j always chains
+// forwards (the two prefix sums of the singular recurrence).  One wavefront
+// per mode: lane l composes q = ceil(P/64) consecutive chunk maps, a log-depth
+// shuffle scan chains the lanes, and each lane replays its q chunks.
 template <typename T>
 __device__ void chain_chunks(const T* SA, const T* SB, T* SY, int P, int MB, int j0, int ny, int backward,
                              int sing_j)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
-    if (P <= 64) {
-        for (int jm = wave; jm < MB; jm += nwave) {
-            const int j = j0 + jm;
-            if (j >= ny) continue;  // uniform over the wavefront
-            const bool bw = backward && j != sing_j;
-            const int  c  = bw ? P - 1 - lane : lane;
-            T          A = T(1), B = T(0);
-            if (lane < P) {
-                A = SA[c * MB + jm];
-                B = SB[c * MB + jm];
-            }
-            for (int d = 1; d < 64; d <<= 1) {
-                const T Ap = __shfl_up(A, d, 64), Bp = __shfl_up(B, d, 64);
-                if (lane >= d) {
-                    B = A * Bp + B;
-                    A = A * Ap;
-                }
-            }
-            T ex = __shfl_up(B, 1, 64);
-            if (lane == 0) ex = T(0);
-            if (lane < P) SY[c * MB + jm] = ex;
+    const int q  = (P + 63) >> 6;
+    const int p0 = lane * q, p1 = min(P, p0 + q);
+    for (int jm = wave; jm < MB; jm += nwave) {
+        if (j0 + jm >= ny) continue;  // uniform over the wavefront
+        const bool bw = backward && (j0 + jm) != sing_j;
+        T          A = T(1), B = T(0);
+        for (int p = p0; p < p1; p++) {
+            const int c = bw ? P - 1 - p : p;
+            const T   a = SA[c * MB + jm];
+            B           = a * B + SB[c * MB + jm];
+            A           = a * A;
         }
-    } else {
-        const int jm = threadIdx.x;
-        if (jm < MB && j0 + jm < ny) {
-            const bool bw = backward && (j0 + jm) != sing_j;
-            T          v  = T(0);
-            for (int q = 0; q < P; q++) {
-                const int c = bw ? P - 1 - q : q;
-                SY[c * MB + jm] = v;
-                v               = SA[c * MB + jm] * v + SB[c * MB + jm];
+        for (int d = 1; d < 64; d <<= 1) {
+            const T Ap = __shfl_up(A, d, 64), Bp = __shfl_up(B, d, 64);
+            if (lane >= d) {
+                B = A * Bp + B;
+                A = A * Ap;
             }
         }
+        T v = __shfl_up(B, 1, 64);
+        if (lane == 0) v = T(0);
+        for (int p = p0; p < p1; p++) {
+            const int c     = bw ? P - 1 - p : p;
+            SY[c * MB + jm] = v;
+            v               = SA[c * MB + jm] * v + SB[c * MB + jm];
+        }
+    }
+}
+
+// The dispatcher deals workgroups to the 8 XCDs round robin.  Renumbering the
+// grid so that consecutive logical blocks (neighbouring modes, which share
+// cache lines of every row) run on one XCD keeps those lines in one L2.
+__device__ __forceinline__ int xcd_block(int b, int nblocks)
+{
+    if (nblocks < 64 || (nblocks & 7) != 0) return b;
+    return (b & 7) * (nblocks >> 3) + (b >> 3);
+}
+
+// The CH pivots of rows i0 .. i0+CH-1 of one mode from the pivot before the
+// chunk (0 before row 0), by the host's recurrence.
+template <typename T, int CH>
+__device__ __forceinline__ void chunk_pivots(T (&pv)[CH], double d, double p, int i0, int nx, int ns, int ne,
+                                             bool act)
+{
+#pragma unroll
+    for (int k = 0; k < CH; k++) {
+        const int i = i0 + k;
+        if (act && i < nx) {
+            const double b = ((i == 0 && ns) || (i == nx - 1 && ne)) ? d + 1.0 : d;
+            p              = 1.0 / (b - p);
+        }
+        pv[k] = (T)p;
     }
 }
 
 // One subdomain's sweep in a batched launch.
 template <typename T>
 struct TriProb {
-    T*        data;
-    long long ld;
-    const T*  pinv;
-    int       nx, ny, ldp, P, MB, singular_last, nblk;  // nblk: mode blocks per right-hand side
+    T*            data;
+    long long     ld;
+    const double* xd;   // x_diag (ny)
+    const double* pst;  // chunk-start pivots, P per mode
+    int           nx, ny, P, MB, singular_last, nblk, ns, ne;  // nblk: mode blocks per right-hand side
 };
 template <typename T>
 struct TriBatch {
@@ -87,15 +114,21 @@ struct TriBatch {
     TriProb<T> p[kMaxBatch];
 };
 
-template <typename T, int CH>
-__global__ __launch_bounds__(1024) void k_tridiag(TriBatch<T> bt)
+template <int CH>
+constexpr int tri_threads()
 {
-    int q = 0;
-    while (q + 1 < bt.nprob && (int)blockIdx.x >= bt.start[q + 1]) q++;
-    const TriProb<T>& pr  = bt.p[q];
-    const int         nx = pr.nx, ny = pr.ny, ldp = pr.ldp, P = pr.P, MB = pr.MB, singular_last = pr.singular_last;
-    const T* __restrict__ pinv = pr.pinv;
-    const int local = (int)blockIdx.x - bt.start[q];
+    return CH >= 32 ? 512 : 1024;
+}
+
+template <typename T, int CH>
+__global__ __launch_bounds__(tri_threads<CH>()) void k_tridiag(TriBatch<T> bt)
+{
+    const int blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
+    int       q   = 0;
+    while (q + 1 < bt.nprob && blk >= bt.start[q + 1]) q++;
+    const TriProb<T>& pr = bt.p[q];
+    const int nx = pr.nx, ny = pr.ny, P = pr.P, MB = pr.MB, singular_last = pr.singular_last;
+    const int local = blk - bt.start[q];
     const int rhs = local / pr.nblk, mblk = local - rhs * pr.nblk;
 
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -112,18 +145,19 @@ __global__ __launch_bounds__(1024) void k_tridiag(TriBatch<T> bt)
     const int  i0  = c * CH;
 
     T fv[CH], pv[CH];
-    // pivots are mode-major and zero-padded to ldp rows: one contiguous,
-    // aligned run of CH values per thread
-    const T* pp = reinterpret_cast<const T*>(__builtin_assume_aligned(pinv + (long long)(act ? j : 0) * ldp + i0, 16));
 #pragma unroll
     for (int k = 0; k < CH; k++) {
         const int  i  = i0 + k;
         const bool ok = act && i < nx;
         fv[k]         = ok ? f[(long long)i * ny + j] : T(0);
-        pv[k]         = act ? pp[k] : T(0);
     }
-    const bool sing = singular_last && (j == ny - 1);
+    const bool sing  = singular_last && (j == ny - 1);
     const int  SINGJ = singular_last ? ny - 1 : -1;
+    if (!sing) {
+        const double d  = act ? pr.xd[j] : 0.0;
+        const double p0 = act ? pr.pst[(long long)j * P + c] : 0.0;
+        chunk_pivots<T, CH>(pv, d, p0, i0, nx, pr.ns, pr.ne, act);
+    }
 
     // forward: chunk map y_out = A y_in + B
     T A = T(1), B = T(0);
@@ -222,7 +256,7 @@ __global__ __launch_bounds__(1024) void k_tridiag(TriBatch<T> bt)
 // sums sum_k phi0(k) u(i,k), sum_k phi1(k) u(i,k) (= u(i,0), u(i,ny-1) after
 // the inverse transform), reduced over this workgroup's modes.
 template <typename T, int CH>
-__global__ __launch_bounds__(1024) void k_tridiag_lines(LinesArgs<T> a, int P, int MB)
+__global__ __launch_bounds__(tri_threads<CH>()) void k_tridiag_lines(LinesArgs<T> a, int P, int MB)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* SA = reinterpret_cast<T*>(smem_raw);
@@ -244,7 +278,6 @@ __global__ __launch_bounds__(1024) void k_tridiag_lines(LinesArgs<T> a, int P, i
     const T    p0  = act ? a.phi[j] : T(0);
     const T    p1  = act ? a.phi[ny + j] : T(0);
     const int  i0  = c * CH;
-    const int  SINGJ = -1;
 
     T fv[CH], pv[CH];
 #pragma unroll
@@ -260,10 +293,9 @@ __global__ __launch_bounds__(1024) void k_tridiag_lines(LinesArgs<T> a, int P, i
         fv[k] = v;
     }
     {
-        const T* pp = reinterpret_cast<const T*>(
-            __builtin_assume_aligned(a.pinv + (long long)(act ? j : 0) * a.ldp + i0, 16));
-#pragma unroll
-        for (int k = 0; k < CH; k++) pv[k] = act ? pp[k] : T(0);
+        const double d   = act ? a.xd[j] : 0.0;
+        const double pp0 = act ? a.pst[(long long)j * P + c] : 0.0;
+        chunk_pivots<T, CH>(pv, d, pp0, i0, nx, a.ns, a.ne, act);
     }
     T A = T(1), B = T(0);
 #pragma unroll
@@ -277,7 +309,7 @@ __global__ __launch_bounds__(1024) void k_tridiag_lines(LinesArgs<T> a, int P, i
         SB[c * MB + jl] = B;
     }
     __syncthreads();
-    chain_chunks(SA, SB, SY, P, MB, blockIdx.x * MB, ny, 0, SINGJ);
+    chain_chunks(SA, SB, SY, P, MB, blockIdx.x * MB, ny, 0, -1);
     __syncthreads();
     T y = (c < P) ? SY[c * MB + jl] : T(0);
 #pragma unroll
@@ -334,7 +366,19 @@ __global__ __launch_bounds__(1024) void k_tridiag_lines(LinesArgs<T> a, int P, i
     }
 }
 
-static int tridiag_ch(int nx) { return nx <= 1024 ? 16 : 32; }
+// ---------------------------------------------------------------------------
+// host side
+
+static int tri_max_threads(int ch) { return ch >= 32 ? 512 : 1024; }
+
+// modes per workgroup: the largest power of two up to mb_cap whose
+// workgroup (MB x P threads) fits the kernel's thread limit
+static int tri_mb(int P, int ch, int mb_cap)
+{
+    int MB = 1;
+    while (MB * 2 * P <= tri_max_threads(ch) && MB < mb_cap) MB *= 2;
+    return MB;
+}
 
 template <typename T, int CH>
 static void launch_tridiag_group(const TridiagJob<T>* const* jobs, int n, hipStream_t s)
@@ -346,21 +390,22 @@ static void launch_tridiag_group(const TridiagJob<T>* const* jobs, int n, hipStr
     for (int q = 0; q < n; q++) {
         const TridiagJob<T>& jb = *jobs[q];
         const SubPlan&       sp = *jb.sp;
-        const int            P  = (sp.nx + CH - 1) / CH;
-        int                  MB = 1;
-        while (MB * 2 * P <= 1024 && MB < sp.mb_field) MB *= 2;
-        if (MB * P > 1024) throw Error(FP_E_INVALID, "nx too large for the tridiagonal kernel");
+        const int            P  = sp.tri_p;
+        const int            MB = tri_mb(P, CH, sp.mb_field);
+        if (MB * P > tri_max_threads(CH)) throw Error(FP_E_INVALID, "nx too large for the tridiagonal kernel");
         TriProb<T>& pr   = bt.p[q];
         pr.data          = jb.data;
         pr.ld            = jb.ld;
-        pr.pinv          = jb.pinv;
+        pr.xd            = sp.xd.p;
+        pr.pst           = sp.pst.p;
         pr.nx            = sp.nx;
         pr.ny            = sp.ny;
-        pr.ldp           = sp.ldp;
         pr.P             = P;
         pr.MB            = MB;
         pr.singular_last = sp.no_dirichlet ? 1 : 0;
         pr.nblk          = (sp.ny + MB - 1) / MB;
+        pr.ns            = sp.bc[XNEG] == NEUMANN;
+        pr.ne            = sp.bc[XPOS] == NEUMANN;
         bt.start[q]      = blocks;
         const long long nb = (long long)pr.nblk * jb.nrhs;
         if (blocks + nb > (1LL << 30)) throw Error(FP_E_INVALID, "too many tridiagonal workgroups in one launch");
@@ -374,17 +419,18 @@ static void launch_tridiag_group(const TridiagJob<T>* const* jobs, int n, hipStr
     FP_HIP(hipGetLastError());
 }
 
-// chunk length: 16 rows per thread up to nx = 1024, 32 above (register
-// budget); jobs are grouped by it and launched kMaxBatch at a time
+// jobs are grouped by their plan's chunk length and launched kMaxBatch at a time
 template <typename T>
 void launch_tridiag_batch(const TridiagJob<T>* jobs, int njobs, hipStream_t s)
 {
-    std::vector<const TridiagJob<T>*> g16, g32;
+    std::vector<const TridiagJob<T>*> g8, g16, g32;
     for (int k = 0; k < njobs; k++) {
         if (jobs[k].nrhs <= 0) continue;
-        if (jobs[k].sp->nx > 32768) throw Error(FP_E_INVALID, "nx above 32768 is not supported");
-        (tridiag_ch(jobs[k].sp->nx) == 16 ? g16 : g32).push_back(&jobs[k]);
+        const int ch = jobs[k].sp->tri_ch;
+        (ch == 8 ? g8 : ch == 16 ? g16 : g32).push_back(&jobs[k]);
     }
+    for (size_t q0 = 0; q0 < g8.size(); q0 += kMaxBatch)
+        launch_tridiag_group<T, 8>(g8.data() + q0, (int)std::min<size_t>(kMaxBatch, g8.size() - q0), s);
     for (size_t q0 = 0; q0 < g16.size(); q0 += kMaxBatch)
         launch_tridiag_group<T, 16>(g16.data() + q0, (int)std::min<size_t>(kMaxBatch, g16.size() - q0), s);
     for (size_t q0 = 0; q0 < g32.size(); q0 += kMaxBatch)
@@ -392,19 +438,17 @@ void launch_tridiag_batch(const TridiagJob<T>* jobs, int njobs, hipStream_t s)
 }
 
 template <typename T>
-void launch_tridiag(const SubPlan& sp, const T* pinv, T* data, long long ld, int nrhs, hipStream_t s)
+void launch_tridiag(const SubPlan& sp, T* data, long long ld, int nrhs, hipStream_t s)
 {
-    const TridiagJob<T> j{&sp, pinv, data, ld, nrhs};
+    const TridiagJob<T> j{&sp, data, ld, nrhs};
     launch_tridiag_batch<T>(&j, 1, s);
 }
 
 template <typename T, int CH>
 static int launch_lines_ch(const SubPlan& sp, const LinesArgs<T>& a, int nrhs, hipStream_t s)
 {
-    const int P = (sp.nx + CH - 1) / CH;
-    int       MB = 1;
-    while (MB * 2 * P <= 1024 && MB < sp.mb_lines) MB *= 2;
-    if (MB * P > 1024) throw Error(FP_E_INVALID, "nx too large for the tridiagonal kernel");
+    const int    P   = sp.tri_p;
+    const int    MB  = tri_mb(P, CH, sp.mb_lines);
     const size_t lds = 3 * (size_t)P * MB * sizeof(T);
     dim3         grid((sp.ny + MB - 1) / MB, nrhs);
     hipLaunchKernelGGL((k_tridiag_lines<T, CH>), grid, dim3((MB * P + 63) / 64 * 64), lds, s, a, P, MB);
@@ -415,10 +459,7 @@ static int launch_lines_ch(const SubPlan& sp, const LinesArgs<T>& a, int nrhs, h
 template <typename T>
 int lines_workgroups(const SubPlan& sp)
 {
-    const int CH = sp.nx <= 1024 ? 16 : 32;
-    const int P  = (sp.nx + CH - 1) / CH;
-    int       MB = 1;
-    while (MB * 2 * P <= 1024 && MB < sp.mb_lines) MB *= 2;
+    const int MB = tri_mb(sp.tri_p, sp.tri_ch, sp.mb_lines);
     return (sp.ny + MB - 1) / MB;
 }
 template int lines_workgroups<double>(const SubPlan&);
@@ -427,15 +468,122 @@ template <typename T>
 int launch_tridiag_lines(const SubPlan& sp, const LinesArgs<T>& a, int nrhs, hipStream_t s)
 {
     FP_REQUIRE(!sp.no_dirichlet, "line operator needs a Dirichlet-like face");
-    if (sp.nx <= 1024) return launch_lines_ch<T, 16>(sp, a, nrhs, s);
-    if (sp.nx <= 32768) return launch_lines_ch<T, 32>(sp, a, nrhs, s);
-    throw Error(FP_E_INVALID, "nx above 32768 is not supported");
+    LinesArgs<T> b = a;
+    b.xd           = sp.xd.p;
+    b.pst          = sp.pst.p;
+    b.ns           = sp.bc[XNEG] == NEUMANN;
+    b.ne           = sp.bc[XPOS] == NEUMANN;
+    if (sp.tri_ch == 8) return launch_lines_ch<T, 8>(sp, b, nrhs, s);
+    if (sp.tri_ch == 16) return launch_lines_ch<T, 16>(sp, b, nrhs, s);
+    return launch_lines_ch<T, 32>(sp, b, nrhs, s);
 }
 template int launch_tridiag_lines<double>(const SubPlan&, const LinesArgs<double>&, int, hipStream_t);
 
-template void launch_tridiag<double>(const SubPlan&, const double*, double*, long long, int, hipStream_t);
+// ---- chunk schedule and its autotuning ----
+
+// Host pivots of every mode (the reference's recurrence) sampled at the chunk
+// starts of chunk length ch: pst[j*P + c] = p_{c*ch - 1} (0 for c = 0).
+void SubPlan::set_chunks(int ch)
+{
+    tri_ch = ch;
+    tri_p  = (nx + ch - 1) / ch;
+    std::vector<double> t((size_t)ny * tri_p, 0.0);
+    const bool          ns = bc[XNEG] == NEUMANN, ne = bc[XPOS] == NEUMANN;
+    for (int j = 0; j < ny; j++) {
+        if (no_dirichlet && j == ny - 1) continue;  // singular mode: recurrence, no pivots
+        const double d = x_diag[j];
+        double       p = 0.0;
+        for (int i = 0; i < nx; i++) {
+            if (i % ch == 0) t[(size_t)j * tri_p + i / ch] = p;
+            const double b = ((i == 0 && ns) || (i == nx - 1 && ne)) ? d + 1.0 : d;
+            p              = 1.0 / (b - p);
+        }
+    }
+    pst.alloc(t.size());
+    pst.upload(t.data(), t.size());
+}
+
+// Candidate chunk lengths for nx: the chain needs at least one thread per
+// chunk (P <= the kernel's thread limit).
+static std::vector<int> chunk_candidates(int nx)
+{
+    std::vector<int> v;
+    for (int ch : {8, 16, 32})
+        if ((nx + ch - 1) / ch <= tri_max_threads(ch)) v.push_back(ch);
+    return v;
+}
+
+// Default schedule without timing: the smallest chunk whose chain fits one
+// wavefront per 64 chunks, and 4 modes per workgroup.
+static int default_chunk(int nx)
+{
+    for (int ch : {16, 32})
+        if ((nx + ch - 1) / ch <= 64) return ch;
+    return 32;
+}
+
+struct TriTune {
+    int ch, mb;
+};
+static std::mutex                               g_tune_mu;
+static std::map<std::pair<int, int>, TriTune>   g_tri_tuned;
+
+// Chooses (chunk length, modes per workgroup) for the field sweep of this
+// plan by timing every candidate on a scratch field of the plan's size (one
+// right-hand side).  Results are cached per (nx, ny) for the process.
+void SubPlan::tune_tridiag(bool measure)
+{
+    if (!measure || (long long)nx * ny < (1LL << 16)) {
+        set_chunks(default_chunk(nx));
+        mb_field = 4;
+        return;
+    }
+    const std::pair<int, int> key(nx, ny);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto                        it = g_tri_tuned.find(key);
+        if (it != g_tri_tuned.end()) {
+            set_chunks(it->second.ch);
+            mb_field = it->second.mb;
+            return;
+        }
+    }
+    DevBuf<double> buf((size_t)nx * ny);
+    FP_HIP(hipMemset(buf.p, 0, (size_t)nx * ny * sizeof(double)));
+    hipEvent_t e0, e1;
+    FP_HIP(hipEventCreate(&e0));
+    FP_HIP(hipEventCreate(&e1));
+    float  best_ms = 1e30f;
+    TriTune best{default_chunk(nx), 4};
+    for (int ch : chunk_candidates(nx)) {
+        set_chunks(ch);
+        for (int mb : {1, 2, 4, 8, 16}) {
+            if (tri_mb(tri_p, ch, mb) != mb) continue;
+            mb_field = mb;
+            launch_tridiag<double>(*this, buf.p, (long long)nx * ny, 1, nullptr);
+            FP_HIP(hipEventRecord(e0, nullptr));
+            for (int r = 0; r < 5; r++) launch_tridiag<double>(*this, buf.p, (long long)nx * ny, 1, nullptr);
+            FP_HIP(hipEventRecord(e1, nullptr));
+            FP_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            FP_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best_ms) {
+                best_ms = ms;
+                best    = {ch, mb};
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    set_chunks(best.ch);
+    mb_field = best.mb;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tri_tuned[key] = best;
+}
+
+template void launch_tridiag<double>(const SubPlan&, double*, long long, int, hipStream_t);
 template void launch_tridiag_batch<double>(const TridiagJob<double>*, int, hipStream_t);
 template void launch_tridiag_batch<float>(const TridiagJob<float>*, int, hipStream_t);
-template void launch_tridiag<float>(const SubPlan&, const float*, float*, long long, int, hipStream_t);
+template void launch_tridiag<float>(const SubPlan&, float*, long long, int, hipStream_t);
 
 }  // namespace fp
