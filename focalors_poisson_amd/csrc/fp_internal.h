@@ -143,6 +143,14 @@ struct TransformJob {
     const void*             src   = nullptr;
     long long               lds   = 0;
     int                     src64 = 0, dst64 = 0;
+    // forward transforms of raw right-hand sides: boundary assembly and the
+    // hx^2 scaling fused into the load (ghost data g[face], nrhs arrays back
+    // to back, times ga[face]; null faces take none)
+    int                     asmb  = 0;
+    double                  hx2   = 1.0;
+    const double*           g[4]  = {nullptr, nullptr, nullptr, nullptr};
+    double                  ga[4] = {0.0, 0.0, 0.0, 0.0};
+    int                     nt    = 0;  // threads per workgroup (0: the plan's)
 };
 template <typename T>
 void launch_ytransform_batch(const TransformJob<T>* jobs, int njobs, int inverse, hipStream_t s);
@@ -172,6 +180,33 @@ struct LinesArgs {
 };
 template <typename T>
 int launch_tridiag_lines(const SubPlan& sp, const LinesArgs<T>& a, int nrhs, hipStream_t s);
+
+// General sweep of the composite solve (fp_tridiag.hip, k_sweep): right-hand
+// side = (fin ? f : 0) + lines(lbp) - lines(lbm), line buffers packed per
+// right-hand side as [x0 (ny) | x1 (ny) | y0 (nx) | y1 (nx)] (x-lines as
+// transform coefficients, y-lines physical); outputs: the field (fout, in
+// place), solved rows 0 / nx-1 (xo0 / xo1, ldxo apart per right-hand side)
+// and per-(rhs, mode block) partial column sums cp (2 nx each; the blocks per
+// right-hand side are sweep_blocks(sp)).
+template <typename T>
+struct SweepJob {
+    const SubPlan* sp;
+    int            nrhs;
+    T*             f     = nullptr;
+    long long      ld    = 0;
+    const T*       lbp   = nullptr;
+    const T*       lbm   = nullptr;
+    long long      ldl   = 0;
+    const T*       phi   = nullptr;
+    T*             xo0   = nullptr;
+    T*             xo1   = nullptr;
+    long long      ldxo  = 0;
+    T*             cp    = nullptr;
+    int            fin   = 1, fout = 1;
+};
+template <typename T>
+void launch_sweep_batch(const SweepJob<T>* jobs, int njobs, hipStream_t s);
+int sweep_blocks(const SubPlan& sp);
 template <typename T>
 int lines_workgroups(const SubPlan& sp);
 

@@ -194,7 +194,7 @@ struct Node {
     bool           precond    = false;  // right preconditioner: explicit inverse of the interface operator
     DevBuf<double> Minv, Zv;
     long long      ldl = 0;
-    DevBuf<double> phi, LB, XU, CP;
+    DevBuf<double> phi, LB, XU, CP, YL;
     DevBuf<int>    kface, kpos;
     DevBuf<int>    state;
     int*           h_state = nullptr;
@@ -204,6 +204,17 @@ struct Node {
         if (h_state) (void)hipHostFree(h_state);
         if (h_resid) (void)hipHostFree(h_resid);
     }
+};
+
+// Workspace of the two-level composite sweep (cat_sweep_fast), per child
+// link of the root in link order.
+struct FastWS {
+    int                         nrhs = 0;
+    DevBuf<double>              LBc;  // root: children's adjacent lines in the root's basis (packed lines)
+    std::vector<DevBuf<double>> LBr;  // the root's adjacent line in the child's basis (packed lines)
+    std::vector<DevBuf<double>> CPc;  // partial column sums of the child's upward sweep
+    std::vector<DevBuf<double>> Xs;   // the child's solved adjacent row (child's basis)
+    std::vector<DevBuf<double>> Xp;   // that row in physical space
 };
 
 }  // namespace fp
@@ -246,6 +257,7 @@ struct fp_cat_s {
         if (h_rpart) (void)hipHostFree(h_rpart);
     }
     int                                    precond_kind = 1;  // 0: none, 1: dense inverse of the interface operator
+    fp::FastWS                             fast;
     // statistics of the last solve
     int    last_steps = 0, last_restarts = 0, last_status = 0;
     double last_res = 0.0, last_rel = 0.0;
@@ -329,7 +341,8 @@ static void ensure_ws(fp_cat_s* c, Node& nd, int nrhs)
     launch_fill<double>(nd.LB.p, (long long)nrhs * nd.ldl, 0.0, nullptr);
     nd.XU.alloc((size_t)nrhs * 2 * c->ny[nd.dom]);
     nd.Zv.alloc((size_t)nrhs * K);
-    nd.CP.alloc((size_t)nrhs * nd.nwg * 2 * c->nx[nd.dom]);
+    nd.CP.alloc((size_t)nrhs * std::max(nd.nwg, sweep_blocks(nd.plan)) * 2 * c->nx[nd.dom]);
+    nd.YL.alloc((size_t)nrhs * 2 * c->nx[nd.dom]);
     FP_HIP(hipDeviceSynchronize());
     if (nd.h_state) (void)hipHostFree(nd.h_state);
     if (nd.h_resid) (void)hipHostFree(nd.h_resid);
@@ -393,7 +406,8 @@ static void lines_solve_assemble(fp_cat_s* c, Node& nd, const double* v, double*
     la.ycols   = nd.has_y;
     launch_tridiag_lines<double>(nd.plan, la, nrhs, s);
     if (nd.has_x && !nd.spectral_x) launch_ytransform<double>(nd.plan.t64, nd.XU.p, 2LL * ny, 2, nrhs, 1, s);
-    launch_lines_assemble(v, out, nd.K, nd.kface.p, nd.kpos.p, nd.XU.p, ny, nd.CP.p, nd.nwg, nx, nrhs, s, k0);
+    if (nd.has_y) launch_colsum(nd.CP.p, nd.nwg, nx, nd.YL.p, 2LL * nx, nrhs, s);
+    launch_lines_assemble(v, out, nd.K, nd.kface.p, nd.kpos.p, nd.XU.p, ny, nd.YL.p, nx, nrhs, s, k0);
 }
 
 static void matvec_lines(fp_cat_s* c, Node& nd, const double* v, double* out, int nrhs, hipStream_t s)
@@ -469,19 +483,17 @@ static void read_state(Node& nd, int nrhs, hipStream_t s)
     FP_HIP(hipStreamSynchronize(s));
 }
 
-// GMRESSolver2D::solve (gmres_solver2d.cpp:123-252) on interface unknowns:
-// b <- A^-1 b; y0 = R b; GMRES on (I - R A^-1 E S) y = R b; then
-// u = A^-1 b + A^-1 E S y.
-static void gmres_solve(fp_cat_s* c, Node& nd, double* b, int nrhs, hipStream_t s)
+// GMRESSolver2D::solve (gmres_solver2d.cpp:123-252) on the interface
+// unknowns of node nd: C holds the right-hand side R A^-1 b' (x-line unknowns
+// as y-transform coefficients when spectral_x); X receives the solution y of
+// (I - R A^-1 E S) y = C.  With the explicit-inverse preconditioner the
+// iteration starts from x0 = M C (the reference starts from its right-hand
+// side); the first residual check then usually ends it.
+static void gmres_iterate(fp_cat_s* c, Node& nd, int nrhs, hipStream_t s)
 {
-    ensure_ws(c, nd, nrhs);
-    const long long n = dom_n(c, nd.dom);
-    const int       K = nd.K, m = c->m;
-    dom_solve(c, nd.dom, b, nrhs, s);
-    launch_gather(b, n, nd.cell.p, K, nullptr, nd.C.p, K, 0.0, 1.0, nrhs, s);
-    // x-line unknowns to the root's y basis (orthogonal: GMRES norms unchanged)
-    if (nd.spectral_x) launch_ytransform<double>(nd.plan.t64, nd.C.p, K, nd.nxl, nrhs, 0, s);
-    FP_HIP(hipMemcpyAsync(nd.X.p, nd.C.p, sizeof(double) * K * nrhs, hipMemcpyDeviceToDevice, s));
+    const int K = nd.K, m = c->m;
+    if (nd.precond) launch_dense_gemv(nd.Minv.p, K, nd.C.p, nd.X.p, 1.0, 0.0, nrhs, s);
+    else FP_HIP(hipMemcpyAsync(nd.X.p, nd.C.p, sizeof(double) * K * nrhs, hipMemcpyDeviceToDevice, s));
     const GmresDev g = gdev(c, nd, nrhs);
     gm_init(g, c->tol, c->tol_rel, s);
     int  restarts = 0, status = 0;
@@ -532,13 +544,6 @@ static void gmres_solve(fp_cat_s* c, Node& nd, double* b, int nrhs, hipStream_t 
         if (nd.h_state[4 * r + 2] == 2) status = FP_E_NUMERIC;
         else if (nd.h_state[4 * r + 2] != 1 && status == 0) status = FP_E_NOT_CONVERGED;
     }
-    if (nd.spectral_x) launch_ytransform<double>(nd.plan.t64, nd.X.p, K, nd.nxl, nrhs, 1, s);
-    // u = z0 + A^-1 E S y
-    for (auto& L : nd.ch) launch_schur_gemv(L.S.p, L.cn, nd.X.p, K, L.off, nd.Tv.p, K, L.off, nrhs, s);
-    launch_fill<double>(nd.G.p, n * nrhs, 0.0, s);
-    launch_scatter(nd.G.p, n, nd.ucell.p, nd.uk1.p, nd.uk2.p, nd.nu, nd.Tv.p, K, nrhs, s);
-    dom_solve(c, nd.dom, nd.G.p, nrhs, s);
-    launch_axpby(nd.G.p, b, n * nrhs, 1.0, 1.0, s);
     if (nd.dom == c->root) {
         c->last_steps    = steps;
         c->last_restarts = restarts;
@@ -546,6 +551,27 @@ static void gmres_solve(fp_cat_s* c, Node& nd, double* b, int nrhs, hipStream_t 
         c->last_rel      = rel;
         c->last_status   = status;
     }
+}
+
+// General tree node: b <- A^-1 b; C = R b; GMRES; then
+// u = A^-1 b + A^-1 E S y (full-field form).
+static void gmres_solve(fp_cat_s* c, Node& nd, double* b, int nrhs, hipStream_t s)
+{
+    ensure_ws(c, nd, nrhs);
+    const long long n = dom_n(c, nd.dom);
+    const int       K = nd.K;
+    dom_solve(c, nd.dom, b, nrhs, s);
+    launch_gather(b, n, nd.cell.p, K, nullptr, nd.C.p, K, 0.0, 1.0, nrhs, s);
+    // x-line unknowns to the root's y basis (orthogonal: GMRES norms unchanged)
+    if (nd.spectral_x) launch_ytransform<double>(nd.plan.t64, nd.C.p, K, nd.nxl, nrhs, 0, s);
+    gmres_iterate(c, nd, nrhs, s);
+    if (nd.spectral_x) launch_ytransform<double>(nd.plan.t64, nd.X.p, K, nd.nxl, nrhs, 1, s);
+    // u = z0 + A^-1 E S y
+    for (auto& L : nd.ch) launch_schur_gemv(L.S.p, L.cn, nd.X.p, K, L.off, nd.Tv.p, K, L.off, nrhs, s);
+    launch_fill<double>(nd.G.p, n * nrhs, 0.0, s);
+    launch_scatter(nd.G.p, n, nd.ucell.p, nd.uk1.p, nd.uk2.p, nd.nu, nd.Tv.p, K, nrhs, s);
+    dom_solve(c, nd.dom, nd.G.p, nrhs, s);
+    launch_axpby(nd.G.p, b, n * nrhs, 1.0, 1.0, s);
 }
 
 // The solves of one tree level: the leaves of the level in one batch (one
@@ -619,6 +645,19 @@ static void build_node(fp_cat_s* c, int d, hipStream_t s)
         nd->ch.push_back(std::move(L));
     }
     nd->K = (int)cells.size();
+    {
+        // phi0 / phi1: transform of the unit vectors e_0 and e_{ny-1}; ldl:
+        // packed line buffers [x0 (ny) | x1 (ny) | y0 (nx) | y1 (nx)]
+        const int ny = c->ny[d];
+        nd->ldl      = 2LL * ny + 2LL * c->nx[d];
+        std::vector<double> e(2 * (size_t)ny, 0.0);
+        e[0]             = 1.0;
+        e[ny + (ny - 1)] = 1.0;
+        nd->phi.alloc(e.size());
+        nd->phi.upload(e.data(), e.size());
+        launch_ytransform<double>(nd->plan.t64, nd->phi.p, ny, 2, 1, 0, s);
+        FP_HIP(hipStreamSynchronize(s));
+    }
     if (nd->K) {
         const int nx = c->nx[d], ny = c->ny[d];
         std::vector<int> kf, kp;
@@ -633,16 +672,7 @@ static void build_node(fp_cat_s* c, int d, hipStream_t s)
         }
         nd->kface.alloc(kf.size()); nd->kface.upload(kf.data(), kf.size());
         nd->kpos.alloc(kp.size()); nd->kpos.upload(kp.data(), kp.size());
-        nd->ldl = 2LL * ny + 2LL * nx;
         nd->nwg = lines_workgroups<double>(nd->plan);
-        // phi0 / phi1: transform of the unit vectors e_0 and e_{ny-1}
-        std::vector<double> e(2 * (size_t)ny, 0.0);
-        e[0]               = 1.0;
-        e[ny + (ny - 1)]   = 1.0;
-        nd->phi.alloc(e.size());
-        nd->phi.upload(e.data(), e.size());
-        launch_ytransform<double>(nd->plan.t64, nd->phi.p, ny, 2, 1, 0, s);
-        FP_HIP(hipStreamSynchronize(s));
         nd->cell.alloc(cells.size());
         nd->cell.upload(cells.data(), cells.size());
         std::vector<int> uc, k1, k2;
@@ -738,25 +768,33 @@ static void build_tree(fp_cat_s* c)
 
 // Boundary assembly and hx^2 scaling of every domain, in place
 // (pe/concat/concat_solver2d.cpp:110-116,159-223).
+// Ghost-data coefficient of each face of domain d: Dirichlet -1/h^2, Neumann
+// +1/h (negative faces) or -1/h (positive faces); none for Adjacented faces
+// or missing data.
+static void face_terms(const fp_cat_s* c, int d, const double* const* ghost, const double* g[4], double a[4])
+{
+    const double hx = c->hx[d], hy = c->hy[d];
+    for (int face = 0; face < 4; face++) {
+        const int t = c->bc[4 * d + face];
+        g[face]     = nullptr;
+        a[face]     = 0.0;
+        const double* gp = ghost ? ghost[4 * d + face] : nullptr;
+        if (!gp) continue;
+        if (t == DIRICHLET) a[face] = face < 2 ? -1.0 / hx / hx : -1.0 / hy / hy;
+        else if (t == NEUMANN) {
+            const double h = face < 2 ? hx : hy;
+            a[face]        = (face == XNEG || face == YNEG) ? 1.0 / h : -1.0 / h;
+        } else continue;
+        g[face] = gp;
+    }
+}
+
 static void cat_assemble(fp_cat_s* c, double* const* f, const double* const* ghost, int nrhs, hipStream_t s)
 {
     for (int d = 0; d < c->nd; d++) {
         AsmFaces<double> fc{};
-        const double     hx = c->hx[d], hy = c->hy[d];
-        for (int face = 0; face < 4; face++) {
-            const int t = c->bc[4 * d + face];
-            fc.g[face]  = nullptr;
-            fc.a[face]  = 0.0;
-            const double* gp = ghost ? ghost[4 * d + face] : nullptr;
-            if (!gp) continue;
-            if (t == DIRICHLET) fc.a[face] = face < 2 ? -1.0 / hx / hx : -1.0 / hy / hy;
-            else if (t == NEUMANN) {
-                const double h = face < 2 ? hx : hy;
-                fc.a[face]     = (face == XNEG || face == YNEG) ? 1.0 / h : -1.0 / h;
-            } else continue;
-            fc.g[face] = gp;
-        }
-        launch_assemble<double>(f[d], c->nx[d], c->ny[d], dom_n(c, d), nrhs, hx * hx, fc, s);
+        face_terms(c, d, ghost, fc.g, fc.a);
+        launch_assemble<double>(f[d], c->nx[d], c->ny[d], dom_n(c, d), nrhs, c->hx[d] * c->hx[d], fc, s);
     }
 }
 
@@ -861,10 +899,10 @@ static void cat_residual(fp_cat_s* c, double* const* u, int nrhs, hipStream_t s)
 //   u_1 = S32(b);  u_{k+1} = u_k + S32(b - A u_k)
 // with S32 the tree sweep whose subdomain solves run in FP32 arithmetic (the
 // interface GMRES, Schur products and this residual stay FP64).  With
-// d_k = u_k - u_{k-1} and rho = |d_k| / |d_{k-1}| the error of u_k is
-// estimated as rho / (1 - rho) |d_k| (contraction of the iteration); the loop
-// stops when that estimate, or |d_k| itself, is at most tol |u_k| for every
-// right-hand side, or after refine_max sweeps.
+// d_k = u_k - u_{k-1} and rho = |d_k| / |d_{k-1}| (k >= 3) the error of u_k
+// is estimated as rho / (1 - rho) |d_k| (contraction of the iteration); the
+// loop stops when that estimate, or |d_k| itself, is at most tol |u_k| for
+// every right-hand side, or after refine_max sweeps.
 static void cat_refine(fp_cat_s* c, double* const* f, int nrhs, hipStream_t s)
 {
     const int nd = c->nd;
@@ -916,7 +954,10 @@ static void cat_refine(fp_cat_s* c, double* const* f, int nrhs, hipStream_t s)
                 sums(1, r, dd, uu);
                 const double dn = std::sqrt(dd), un = std::sqrt(uu);
                 const double ratio = un > 0 ? dn / un : dn;
-                const double rho   = prev[r] > 0 ? dn / prev[r] : 0.0;
+                // d_1 = u_1 is the whole solution, so |d_2| / |d_1| is the inner
+                // solve's accuracy, not the contraction: the contraction is
+                // first seen in |d_3| / |d_2|
+                const double rho   = (k > 2 && prev[r] > 0) ? dn / prev[r] : 1.0;
                 double       e     = rho < 1.0 ? std::min(ratio, rho / (1.0 - rho) * ratio) : ratio;
                 if (!std::isfinite(e)) bad = true;
                 worst   = std::max(worst, e);
@@ -936,14 +977,270 @@ static void cat_refine(fp_cat_s* c, double* const* f, int nrhs, hipStream_t s)
     else if (est > c->refine_tol && c->last_status == 0) c->last_status = FP_E_NOT_CONVERGED;
 }
 
+// ---- two-level composite sweep in transform space ----
+//
+// For a root with leaf children (every geometry of depth two: the cross, an
+// L, a T, one lone domain) the tree sweep of concat_solver2d.cpp:118-157 is
+// run on transform coefficients, and every interface coupling enters as a
+// line source of the x sweep (k_sweep) instead of a full-field operation:
+//   1. every domain's raw right-hand side is assembled, scaled and
+//      transformed in place (one batched launch, assembly fused into the
+//      load): F_d;
+//   2. the children's upward solves A_c^-1 F_c are swept with only the line
+//      next to the root written out (x-lines: the solved row, taken to the
+//      root's basis by two one-row transforms unless the bases agree;
+//      y-lines: column sums);
+//   3. the root's sweep of F_r minus those lines writes only its interface
+//      lines: the GMRES right-hand side R A_r^-1 b';
+//   4. after GMRES, the root's sweep of F_r - child lines + E S y writes the
+//      whole coefficient field, which is transformed back in place;
+//   5. the root's line next to each child (in the child's basis for x-lines)
+//      is subtracted in the children's final sweeps, which write their fields
+//      for the batched inverse transforms.
+// Every field is read and written in place: two transforms and two sweeps per
+// domain, no copies, fills or full-field scatters.
+
+static bool same_basis(const SubPlan& a, const SubPlan& b)
+{
+    return a.kind == b.kind && a.ny == b.ny;
+}
+
+static bool fast_ok(const fp_cat_s* c)
+{
+    if (c->mixed || c->nlev > 2) return false;
+    const Node& R = *c->nodes[c->root];
+    return R.ch.empty() || !R.has_x || R.spectral_x;
+}
+
+static void ensure_fast_ws(fp_cat_s* c, int nrhs)
+{
+    FastWS& w = c->fast;
+    if (w.nrhs >= nrhs) return;
+    Node&        R  = *c->nodes[c->root];
+    const size_t nc = R.ch.size();
+    w.LBc.alloc((size_t)nrhs * R.ldl);
+    launch_fill<double>(w.LBc.p, (long long)nrhs * R.ldl, 0.0, nullptr);
+    w.LBr.clear();
+    w.CPc.clear();
+    w.Xs.clear();
+    w.Xp.clear();
+    w.LBr.resize(nc);
+    w.CPc.resize(nc);
+    w.Xs.resize(nc);
+    w.Xp.resize(nc);
+    for (size_t k = 0; k < nc; k++) {
+        const Node& C = *c->nodes[R.ch[k].dom];
+        w.LBr[k].alloc((size_t)nrhs * C.ldl);
+        launch_fill<double>(w.LBr[k].p, (long long)nrhs * C.ldl, 0.0, nullptr);
+        w.CPc[k].alloc((size_t)nrhs * sweep_blocks(C.plan) * 2 * c->nx[C.dom]);
+        w.Xs[k].alloc((size_t)nrhs * c->ny[C.dom]);
+        w.Xp[k].alloc((size_t)nrhs * c->ny[C.dom]);
+    }
+    FP_HIP(hipDeviceSynchronize());
+    w.nrhs = nrhs;
+}
+
+static TransformJob<double> row_job(const TransformPlan<double>* plan, const double* src, long long lds, double* dst,
+                                    long long ldd, int nrhs)
+{
+    TransformJob<double> j{};
+    j.plan  = plan;
+    j.data  = dst;
+    j.ld    = ldd;
+    j.nx    = 1;
+    j.nrhs  = nrhs;
+    j.src   = src;
+    j.lds   = lds;
+    j.src64 = 1;
+    j.dst64 = 1;
+    return j;
+}
+
+static void cat_sweep_fast(fp_cat_s* c, double* const* f, const double* const* ghost, int nrhs, hipStream_t s)
+{
+    const int nd = c->nd, root = c->root;
+    Node&     R  = *c->nodes[root];
+    c->last_steps = c->last_restarts = c->last_status = 0;
+    c->last_res = c->last_rel = 0.0;
+    // 1. assembly + forward transforms of every domain
+    std::vector<TransformJob<double>> fw(nd);
+    for (int d = 0; d < nd; d++) {
+        TransformJob<double>& j = fw[d];
+        j       = {};
+        j.plan  = &c->nodes[d]->plan.t64;
+        j.data  = f[d];
+        j.ld    = dom_n(c, d);
+        j.nx    = c->nx[d];
+        j.nrhs  = nrhs;
+        j.dst64 = 1;
+        j.asmb  = 1;
+        j.hx2   = c->hx[d] * c->hx[d];
+        face_terms(c, d, ghost, j.g, j.ga);
+    }
+    launch_ytransform_batch<double>(fw.data(), nd, 0, s);
+    const int       nxr = c->nx[root], nyr = c->ny[root];
+    const long long nr  = dom_n(c, root);
+    if (R.ch.empty()) {
+        launch_tridiag<double>(R.plan, f[root], nr, nrhs, s);
+        launch_ytransform<double>(R.plan.t64, f[root], nr, nxr, nrhs, 1, s);
+        return;
+    }
+    ensure_ws(c, R, nrhs);
+    ensure_fast_ws(c, nrhs);
+    FastWS&      w  = c->fast;
+    const size_t nc = R.ch.size();
+
+    // 2. children's upward solves, only the line next to the root written
+    std::vector<SweepJob<double>>     up(nc);
+    std::vector<TransformJob<double>> inv_rows, fwd_rows;
+    for (size_t k = 0; k < nc; k++) {
+        const ChildLink&  L = R.ch[k];
+        const Node&       C = *c->nodes[L.dom];
+        const int         nyc = c->ny[L.dom];
+        SweepJob<double>& j = up[k];
+        j.sp   = &C.plan;
+        j.nrhs = nrhs;
+        j.f    = f[L.dom];
+        j.ld   = dom_n(c, L.dom);
+        j.phi  = C.phi.p;
+        j.fin  = 1;
+        j.fout = 0;
+        if (L.loc == XNEG || L.loc == XPOS) {
+            // root face XNEG (row 0) meets the child's row nx-1, XPOS its row 0
+            const bool same = same_basis(C.plan, R.plan);
+            double*    slot = w.LBc.p + (L.loc == XNEG ? 0 : nyr);
+            double*    dst  = same ? slot : w.Xs[k].p;
+            j.ldxo          = same ? R.ldl : nyc;
+            if (L.loc == XNEG) j.xo1 = dst;
+            else j.xo0 = dst;
+            if (!same) {
+                inv_rows.push_back(row_job(&C.plan.t64, w.Xs[k].p, nyc, w.Xp[k].p, nyc, nrhs));
+                fwd_rows.push_back(row_job(&R.plan.t64, w.Xp[k].p, nyc, slot, R.ldl, nrhs));
+            }
+        } else
+            j.cp = w.CPc[k].p;
+    }
+    launch_sweep_batch<double>(up.data(), (int)nc, s);
+    for (size_t k = 0; k < nc; k++) {
+        const ChildLink& L = R.ch[k];
+        if (L.loc != YNEG && L.loc != YPOS) continue;
+        // root face YNEG (column 0) meets the child's column ny-1 (phi1 sums), YPOS its column 0
+        double* slot = w.LBc.p + 2LL * nyr + (L.loc == YNEG ? 0 : nxr);
+        launch_colsum(w.CPc[k].p, sweep_blocks(c->nodes[L.dom]->plan), c->nx[L.dom], slot, R.ldl, nrhs, s,
+                      L.loc == YNEG ? 1 : 0);
+    }
+    if (!inv_rows.empty()) {
+        launch_ytransform_batch<double>(inv_rows.data(), (int)inv_rows.size(), 1, s);
+        launch_ytransform_batch<double>(fwd_rows.data(), (int)fwd_rows.size(), 0, s);
+    }
+
+    // 3. the root's interface lines of A_r^-1 b' (the GMRES right-hand side)
+    {
+        SweepJob<double> j{};
+        j.sp   = &R.plan;
+        j.nrhs = nrhs;
+        j.f    = f[root];
+        j.ld   = nr;
+        j.lbm  = w.LBc.p;
+        j.ldl  = R.ldl;
+        j.phi  = R.phi.p;
+        j.xo0  = R.XU.p;
+        j.xo1  = R.XU.p + nyr;
+        j.ldxo = 2LL * nyr;
+        j.cp   = R.has_y ? R.CP.p : nullptr;
+        j.fin  = 1;
+        j.fout = 0;
+        launch_sweep_batch<double>(&j, 1, s);
+        if (R.has_y) launch_colsum(R.CP.p, sweep_blocks(R.plan), nxr, R.YL.p, 2LL * nxr, nrhs, s);
+        launch_lines_assemble(nullptr, R.C.p, R.K, R.kface.p, R.kpos.p, R.XU.p, nyr, R.YL.p, nxr, nrhs, s, 0, 1);
+    }
+    gmres_iterate(c, R, nrhs, s);
+
+    // 4. the root's solution: sweep of F_r - child lines + E S y, then back to physical space
+    launch_schur_lines(schur_lines_args(c, R, R.X.p), nrhs, s);
+    {
+        SweepJob<double> j{};
+        j.sp   = &R.plan;
+        j.nrhs = nrhs;
+        j.f    = f[root];
+        j.ld   = nr;
+        j.lbp  = R.LB.p;
+        j.lbm  = w.LBc.p;
+        j.ldl  = R.ldl;
+        j.phi  = R.phi.p;
+        j.fin  = 1;
+        j.fout = 1;
+        launch_sweep_batch<double>(&j, 1, s);
+    }
+    launch_ytransform<double>(R.plan.t64, f[root], nr, nxr, nrhs, 1, s);
+
+    // 5. the root's lines into the children's right-hand sides, then their final solves
+    std::vector<TransformJob<double>> rows;
+    LineCopyBatch                     lc;
+    for (size_t k = 0; k < nc; k++) {
+        const ChildLink& L = R.ch[k];
+        const Node&      C = *c->nodes[L.dom];
+        const int        nxc = c->nx[L.dom], nyc = c->ny[L.dom];
+        if (L.loc == XNEG || L.loc == XPOS) {
+            // root row 0 (XNEG) -> child's row nx-1 (slot x1); root row nx-1 (XPOS) -> child's row 0 (x0)
+            const double* src = f[root] + (L.loc == XNEG ? 0 : (long long)(nxr - 1) * nyr);
+            rows.push_back(row_job(&C.plan.t64, src, nr, w.LBr[k].p + (L.loc == XNEG ? nyc : 0), C.ldl, nrhs));
+        } else {
+            // root column 0 (YNEG) -> child's column ny-1 (slot y1); root column ny-1 (YPOS) -> child's column 0 (y0)
+            FP_REQUIRE(lc.n < 8, "too many y-line children");
+            LineCopy& q = lc.c[lc.n++];
+            q.src       = f[root] + (L.loc == YNEG ? 0 : nyr - 1);
+            q.lds       = nr;
+            q.stride    = nyr;
+            q.len       = nxr;
+            q.dst       = w.LBr[k].p + 2LL * nyc + (L.loc == YNEG ? nxc : 0);
+            q.ldd       = C.ldl;
+            q.coef      = 1.0;
+        }
+    }
+    if (!rows.empty()) launch_ytransform_batch<double>(rows.data(), (int)rows.size(), 0, s);
+    launch_line_copy(lc, nrhs, s);
+    std::vector<SweepJob<double>>     dn(nc);
+    std::vector<TransformJob<double>> iv(nc);
+    for (size_t k = 0; k < nc; k++) {
+        const int   cd = R.ch[k].dom;
+        const Node& C  = *c->nodes[cd];
+        SweepJob<double>& j = dn[k];
+        j.sp   = &C.plan;
+        j.nrhs = nrhs;
+        j.f    = f[cd];
+        j.ld   = dom_n(c, cd);
+        j.lbm  = w.LBr[k].p;
+        j.ldl  = C.ldl;
+        j.phi  = C.phi.p;
+        j.fin  = 1;
+        j.fout = 1;
+        iv[k]       = {};
+        iv[k].plan  = &C.plan.t64;
+        iv[k].data  = f[cd];
+        iv[k].ld    = dom_n(c, cd);
+        iv[k].nx    = c->nx[cd];
+        iv[k].nrhs  = nrhs;
+        iv[k].dst64 = 1;
+    }
+    launch_sweep_batch<double>(dn.data(), (int)nc, s);
+    launch_ytransform_batch<double>(iv.data(), (int)nc, 1, s);
+}
+
 // ConcatPoissonSolver2D::solve (pe/concat/concat_solver2d.cpp:101-157)
 static void cat_solve(fp_cat_s* c, double* const* f, const double* const* ghost, int nrhs, hipStream_t s)
 {
-    cat_assemble(c, f, ghost, nrhs, s);
     c->last_refine_steps = 0;
     c->last_refine_est   = 0.0;
-    if (c->precision == FP_F32_REFINE) cat_refine(c, f, nrhs, s);
-    else cat_sweep(c, f, nrhs, s);
+    if (c->precision == FP_F32_REFINE) {
+        cat_assemble(c, f, ghost, nrhs, s);
+        cat_refine(c, f, nrhs, s);
+    } else if (fast_ok(c))
+        cat_sweep_fast(c, f, ghost, nrhs, s);
+    else {
+        cat_assemble(c, f, ghost, nrhs, s);
+        cat_sweep(c, f, nrhs, s);
+    }
 }
 
 }  // namespace fp

@@ -75,6 +75,11 @@ struct YArgs {
     int          ncs;  // entries of those tables
     const C2<T>* aux;
     T            c0, c1;
+    // fused boundary assembly and scaling of a raw right-hand side on load
+    int           asmb;
+    T             hx2;
+    const double* g[4];
+    T             ga[4];
 };
 
 // Up to kMaxBatch transforms of different subdomains in one launch, passed
@@ -433,6 +438,7 @@ template <typename T>
 struct RowPair {
     long long ia, oa;  // row a in src / dst
     bool      hb;      // row b exists
+    int       ra, rhs; // row a, right-hand side
 };
 
 template <typename T>
@@ -442,10 +448,26 @@ __device__ __forceinline__ RowPair<T> row_pair(const YArgs<T>& a, int local)
     const int  rhs = local / npairs, pair = local - rhs * npairs;
     const int  ra = 2 * pair;
     RowPair<T> rp;
-    rp.hb = ra + 1 < a.nx;
-    rp.ia = (long long)rhs * a.lds + (long long)ra * a.N;
+    rp.hb  = ra + 1 < a.nx;
+    rp.ra  = ra;
+    rp.rhs = rhs;
+    rp.ia  = (long long)rhs * a.lds + (long long)ra * a.N;
     rp.oa = (long long)rhs * a.ldd + (long long)ra * a.N;
     return rp;
+}
+
+// ConcatPoissonSolver2D::boundary_assembly and the hx^2 scaling
+// (pe/concat/concat_solver2d.cpp:110-116,159-223) of value v at row i,
+// column n: Dirichlet f += -g / h^2, Neumann f += +-g / h on the face line,
+// then f *= hx^2 (the coefficients a.ga carry sign and spacing).
+template <typename T>
+__device__ __forceinline__ T assemble_at(const YArgs<T>& a, int rhs, int i, int n, T v)
+{
+    if (i == 0 && a.g[XNEG]) v += a.ga[XNEG] * (T)a.g[XNEG][(long long)rhs * a.N + n];
+    if (i == a.nx - 1 && a.g[XPOS]) v += a.ga[XPOS] * (T)a.g[XPOS][(long long)rhs * a.N + n];
+    if (n == 0 && a.g[YNEG]) v += a.ga[YNEG] * (T)a.g[YNEG][(long long)rhs * a.nx + i];
+    if (n == a.N - 1 && a.g[YPOS]) v += a.ga[YPOS] * (T)a.g[YPOS][(long long)rhs * a.nx + i];
+    return a.hx2 * v;
 }
 
 // Builds the packed extended sequence of the row pair in A (length L).
@@ -483,7 +505,11 @@ __device__ __forceinline__ void load_rows(const YArgs<T>& a, const RowPair<T>& r
         for (int k = 0; k < VPT; k++) {
             const int n = base + k * NT + threadIdx.x;
             if (n >= N) break;
-            const T x = va[k], y = vb[k];
+            T x = va[k], y = vb[k];
+            if (a.asmb) {
+                x = assemble_at(a, rp.rhs, rp.ra, n, x);
+                if (rp.hb) y = assemble_at(a, rp.rhs, rp.ra + 1, n, y);
+            }
             if (kind == T_DD) {
                 A[zp<PAD>(n + 1)]     = {x, y};
                 A[zp<PAD>(L - 1 - n)] = {-x, -y};
@@ -816,6 +842,12 @@ static YArgs<T> yargs(const TransformJob<T>& j, int inverse)
     a.aux = reinterpret_cast<const C2<T>*>(tp.aux.p);
     a.c0  = tp.c0;
     a.c1  = tp.c1;
+    a.asmb = j.asmb;
+    a.hx2  = (T)j.hx2;
+    for (int f = 0; f < 4; f++) {
+        a.g[f]  = j.g[f];
+        a.ga[f] = (T)j.ga[f];
+    }
     return a;
 }
 
@@ -858,7 +890,7 @@ void launch_ytransform_batch(const TransformJob<T>* jobs, int njobs, int inverse
     std::map<std::pair<int, int>, std::vector<const TransformJob<T>*>> groups;
     for (int k = 0; k < njobs; k++) {
         if (jobs[k].nrhs <= 0 || jobs[k].nx <= 0) continue;
-        groups[{jobs[k].plan->engine, jobs[k].plan->nt}].push_back(&jobs[k]);
+        groups[{jobs[k].plan->engine, jobs[k].nt ? jobs[k].nt : jobs[k].plan->nt}].push_back(&jobs[k]);
     }
     for (auto& kv : groups) {
         auto& g = kv.second;

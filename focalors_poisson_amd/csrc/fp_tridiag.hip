@@ -366,6 +366,170 @@ __global__ __launch_bounds__(tri_threads<CH>()) void k_tridiag_lines(LinesArgs<T
     }
 }
 
+// General sweep of the composite solve (two-level trees).  The right-hand
+// side of mode j, row i is
+//   fin ? F(i,j) : 0
+//   + sum over the line buffers (coefficient +1 for lbp, -1 for lbm) of
+//     [i == 0] X0(j) + [i == nx-1] X1(j) + phi0(j) Y0(i) + phi1(j) Y1(i)
+// where X0/X1 are x-lines in this domain's y basis (transform coefficients of
+// rows 0 / nx-1) and Y0/Y1 physical y-lines (columns 0 / ny-1), entering
+// through phi0/phi1, the transforms of the unit vectors e_0 / e_{ny-1}.
+// After the chained-chunk Thomas sweep the solution's coefficients go to any
+// of: the field (fout, in place), rows 0 / nx-1 (xo0 / xo1), and the
+// per-workgroup partial sums sum_j phi0(j) U(i,j), sum_j phi1(j) U(i,j) of
+// this workgroup's modes (cp), whose total over the workgroups is U at
+// columns 0 / ny-1 in physical space.  So a solve whose input or output is
+// only a few lines never reads or writes the rest of the field.
+template <typename T>
+struct SweepProb {
+    T*            f;
+    long long     ld;
+    const T*      lbp;
+    const T*      lbm;
+    long long     ldl;
+    const T*      phi;
+    T*            xo0;
+    T*            xo1;
+    long long     ldxo;
+    T*            cp;
+    const double* xd;
+    const double* pst;
+    int           nx, ny, P, MB, nblk, ns, ne, fin, fout;
+};
+template <typename T>
+struct SweepBatch {
+    int          nprob;
+    int          start[kMaxBatch + 1];
+    SweepProb<T> p[kMaxBatch];
+};
+
+template <typename T>
+__device__ __forceinline__ T line_rhs(const T* lb, int nx, int ny, int i, int j, T p0, T p1)
+{
+    T v = p0 * lb[2 * ny + i] + p1 * lb[2 * ny + nx + i];
+    if (i == 0) v += lb[j];
+    if (i == nx - 1) v += lb[ny + j];
+    return v;
+}
+
+template <typename T, int CH>
+__global__ __launch_bounds__(tri_threads<CH>()) void k_sweep(SweepBatch<T> bt)
+{
+    const int blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
+    int       q   = 0;
+    while (q + 1 < bt.nprob && blk >= bt.start[q + 1]) q++;
+    const SweepProb<T>& pr = bt.p[q];
+    const int nx = pr.nx, ny = pr.ny, P = pr.P, MB = pr.MB;
+    const int local = blk - bt.start[q];
+    const int r = local / pr.nblk, mblk = local - r * pr.nblk;
+
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* SA = reinterpret_cast<T*>(smem_raw);
+    T* SB = SA + P * MB;
+    T* SY = SB + P * MB;
+
+    const int  tid = threadIdx.x;
+    const int  jl  = tid % MB;
+    const int  c   = tid / MB;
+    const int  j   = mblk * MB + jl;
+    const bool act = (c < P) && (j < ny);
+    const int  i0  = c * CH;
+    T*         f   = pr.f ? pr.f + (long long)r * pr.ld : nullptr;
+    const T*   lp  = pr.lbp ? pr.lbp + (long long)r * pr.ldl : nullptr;
+    const T*   lm  = pr.lbm ? pr.lbm + (long long)r * pr.ldl : nullptr;
+    const T    p0  = act ? pr.phi[j] : T(0);
+    const T    p1  = act ? pr.phi[ny + j] : T(0);
+
+    T fv[CH], pv[CH];
+#pragma unroll
+    for (int k = 0; k < CH; k++) {
+        const int  i  = i0 + k;
+        const bool ok = act && i < nx;
+        T          v  = (ok && pr.fin) ? f[(long long)i * ny + j] : T(0);
+        if (ok && lp) v += line_rhs(lp, nx, ny, i, j, p0, p1);
+        if (ok && lm) v -= line_rhs(lm, nx, ny, i, j, p0, p1);
+        fv[k] = v;
+    }
+    {
+        const double d   = act ? pr.xd[j] : 0.0;
+        const double pp0 = act ? pr.pst[(long long)j * P + c] : 0.0;
+        chunk_pivots<T, CH>(pv, d, pp0, i0, nx, pr.ns, pr.ne, act);
+    }
+    T A = T(1), B = T(0);
+#pragma unroll
+    for (int k = 0; k < CH; k++)
+        if (i0 + k < nx) {
+            A = -pv[k] * A;
+            B = pv[k] * (fv[k] - B);
+        }
+    if (c < P) {
+        SA[c * MB + jl] = A;
+        SB[c * MB + jl] = B;
+    }
+    __syncthreads();
+    chain_chunks(SA, SB, SY, P, MB, mblk * MB, ny, 0, -1);
+    __syncthreads();
+    T y = (c < P) ? SY[c * MB + jl] : T(0);
+#pragma unroll
+    for (int k = 0; k < CH; k++)
+        if (i0 + k < nx) {
+            y     = pv[k] * (fv[k] - y);
+            fv[k] = y;
+        }
+    __syncthreads();
+    A = T(1);
+    B = T(0);
+#pragma unroll
+    for (int k = CH - 1; k >= 0; k--)
+        if (i0 + k < nx) {
+            B = fv[k] - pv[k] * B;
+            A = -pv[k] * A;
+        }
+    if (c < P) {
+        SA[c * MB + jl] = A;
+        SB[c * MB + jl] = B;
+    }
+    __syncthreads();
+    chain_chunks(SA, SB, SY, P, MB, mblk * MB, ny, 1, -1);
+    __syncthreads();
+    T x = (c < P) ? SY[c * MB + jl] : T(0);
+#pragma unroll
+    for (int k = CH - 1; k >= 0; k--)
+        if (i0 + k < nx) {
+            x     = fv[k] - pv[k] * x;
+            fv[k] = x;
+        }
+    if (act) {
+        if (pr.fout) {
+#pragma unroll
+            for (int k = 0; k < CH; k++)
+                if (i0 + k < nx) f[(long long)(i0 + k) * ny + j] = fv[k];
+        }
+        if (pr.xo0 && i0 == 0) pr.xo0[(long long)r * pr.ldxo + j] = fv[0];
+        if (pr.xo1) {
+#pragma unroll
+            for (int k = 0; k < CH; k++)
+                if (i0 + k == nx - 1) pr.xo1[(long long)r * pr.ldxo + j] = fv[k];
+        }
+    }
+    if (pr.cp) {
+        T* out = pr.cp + ((long long)r * pr.nblk + mblk) * 2 * nx;
+#pragma unroll
+        for (int k = 0; k < CH; k++) {
+            T s0 = p0 * fv[k], s1 = p1 * fv[k];
+            for (int off = MB / 2; off > 0; off >>= 1) {
+                s0 += __shfl_xor(s0, off, 64);
+                s1 += __shfl_xor(s1, off, 64);
+            }
+            const int i = i0 + k;
+            if (jl == 0 && c < P && i < nx) {
+                out[i]      = s0;
+                out[nx + i] = s1;
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // host side
 
@@ -478,6 +642,75 @@ int launch_tridiag_lines(const SubPlan& sp, const LinesArgs<T>& a, int nrhs, hip
     return launch_lines_ch<T, 32>(sp, b, nrhs, s);
 }
 template int launch_tridiag_lines<double>(const SubPlan&, const LinesArgs<double>&, int, hipStream_t);
+
+template <typename T, int CH>
+static void launch_sweep_group(const SweepJob<T>* const* jobs, int n, hipStream_t s)
+{
+    SweepBatch<T> bt{};
+    int           blocks = 0, threads = 64;
+    size_t        lds = 0;
+    bt.nprob          = n;
+    for (int q = 0; q < n; q++) {
+        const SweepJob<T>& jb = *jobs[q];
+        const SubPlan&     sp = *jb.sp;
+        FP_REQUIRE(!sp.no_dirichlet, "the composite sweep needs a Dirichlet-like face");
+        const int P  = sp.tri_p;
+        const int MB = tri_mb(P, CH, sp.mb_field);
+        SweepProb<T>& pr = bt.p[q];
+        pr.f    = jb.f;
+        pr.ld   = jb.ld;
+        pr.lbp  = jb.lbp;
+        pr.lbm  = jb.lbm;
+        pr.ldl  = jb.ldl;
+        pr.phi  = jb.phi;
+        pr.xo0  = jb.xo0;
+        pr.xo1  = jb.xo1;
+        pr.ldxo = jb.ldxo;
+        pr.cp   = jb.cp;
+        pr.xd   = sp.xd.p;
+        pr.pst  = sp.pst.p;
+        pr.nx   = sp.nx;
+        pr.ny   = sp.ny;
+        pr.P    = P;
+        pr.MB   = MB;
+        pr.nblk = (sp.ny + MB - 1) / MB;
+        pr.ns   = sp.bc[XNEG] == NEUMANN;
+        pr.ne   = sp.bc[XPOS] == NEUMANN;
+        pr.fin  = jb.fin;
+        pr.fout = jb.fout;
+        bt.start[q] = blocks;
+        blocks += pr.nblk * jb.nrhs;
+        threads = std::max(threads, (MB * P + 63) / 64 * 64);
+        lds     = std::max(lds, 3 * (size_t)P * MB * sizeof(T));
+    }
+    bt.start[n] = blocks;
+    if (blocks == 0) return;
+    hipLaunchKernelGGL((k_sweep<T, CH>), dim3(blocks), dim3(threads), lds, s, bt);
+    FP_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_sweep_batch(const SweepJob<T>* jobs, int njobs, hipStream_t s)
+{
+    std::vector<const SweepJob<T>*> g8, g16, g32;
+    for (int k = 0; k < njobs; k++) {
+        if (jobs[k].nrhs <= 0) continue;
+        const int ch = jobs[k].sp->tri_ch;
+        (ch == 8 ? g8 : ch == 16 ? g16 : g32).push_back(&jobs[k]);
+    }
+    for (size_t q0 = 0; q0 < g8.size(); q0 += kMaxBatch)
+        launch_sweep_group<T, 8>(g8.data() + q0, (int)std::min<size_t>(kMaxBatch, g8.size() - q0), s);
+    for (size_t q0 = 0; q0 < g16.size(); q0 += kMaxBatch)
+        launch_sweep_group<T, 16>(g16.data() + q0, (int)std::min<size_t>(kMaxBatch, g16.size() - q0), s);
+    for (size_t q0 = 0; q0 < g32.size(); q0 += kMaxBatch)
+        launch_sweep_group<T, 32>(g32.data() + q0, (int)std::min<size_t>(kMaxBatch, g32.size() - q0), s);
+}
+template void launch_sweep_batch<double>(const SweepJob<double>*, int, hipStream_t);
+
+int sweep_blocks(const SubPlan& sp)
+{
+    return (sp.ny + tri_mb(sp.tri_p, sp.tri_ch, sp.mb_field) - 1) / tri_mb(sp.tri_p, sp.tri_ch, sp.mb_field);
+}
 
 // ---- chunk schedule and its autotuning ----
 

@@ -42,9 +42,27 @@ struct SchurLinesArgs {
     long long     ldl;
 };
 void launch_schur_lines(const SchurLinesArgs& a, int nrhs, hipStream_t s);
-// v == nullptr: v is the unit vector e_{k0 + r} of right-hand side r
+// partial column sums of a line-sourced sweep (nblk blocks per rhs) -> y-lines
+// (half = 0 / 1: only the column-0 / column-(ny-1) sums, nx values)
+void launch_colsum(const double* cp, int nblk, int nx, double* yl, long long ldy, int nrhs, hipStream_t s,
+                   int half = -1);
+// mode 0: out = v - t (v == nullptr: the unit vector e_{k0 + r} of right-hand
+// side r); mode 1: out = t; t from the x rows xu and the y-lines yl (2 nx per rhs)
 void launch_lines_assemble(const double* v, double* out, int K, const int* kface, const int* kpos, const double* xu,
-                           int ny, const double* colpart, int nwg, int nx, int nrhs, hipStream_t s, int k0 = 0);
+                           int ny, const double* yl, int nx, int nrhs, hipStream_t s, int k0 = 0, int mode = 0);
+struct LineCopy {
+    const double* src;
+    long long     lds;
+    int           stride, len;
+    double*       dst;
+    long long     ldd;
+    double        coef;
+};
+struct LineCopyBatch {
+    int      n = 0;
+    LineCopy c[8];
+};
+void launch_line_copy(const LineCopyBatch& b, int nrhs, hipStream_t s);
 void launch_lines_from_units(const SchurLinesArgs& a, int k0, int cnt, hipStream_t s);
 void launch_dense_gemv(const double* A, int n, const double* x, double* y, double alpha, double beta, int nrhs,
                        hipStream_t s);

@@ -216,11 +216,37 @@ void launch_schur_lines(const SchurLinesArgs& a, int nrhs, hipStream_t s)
     FP_HIP(hipGetLastError());
 }
 
-// out = v - t, where t is the restriction of the line-sourced solution:
-// x-lines from the inverse-transformed coefficient rows, y-lines from the
-// column sums of the workgroups (fixed order)
+// Column sums of the line-sourced sweeps: yl[r][w*nx + i] (w = 0, 1) =
+// sum over the mode blocks b of cp[(r*nblk + b)*2nx + w*nx + i], one
+// wavefront per output in a fixed order (reproducible).  yl entries are ldy
+// apart per right-hand side.
+// half = 0 or 1: only that column's sums, written at yl[r * ldy + i].
+__global__ __launch_bounds__(256) void k_colsum(const double* __restrict__ cp, int nblk, int nx, double* yl,
+                                                long long ldy, int half)
+{
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int o    = blockIdx.x * 4 + wave;  // output index
+    if (o >= (half < 0 ? 2 : 1) * nx) return;
+    const long long r   = blockIdx.y;
+    const double*   src = cp + r * (long long)nblk * 2 * nx + o + (half > 0 ? nx : 0);
+    double          acc = 0.0;
+    for (int b = lane; b < nblk; b += 64) acc += src[(long long)b * 2 * nx];
+    acc = wave_reduce_sum(acc);
+    if (lane == 0) yl[r * ldy + o] = acc;
+}
+void launch_colsum(const double* cp, int nblk, int nx, double* yl, long long ldy, int nrhs, hipStream_t s, int half)
+{
+    const int nout = (half < 0 ? 2 : 1) * nx;
+    hipLaunchKernelGGL(k_colsum, dim3((nout + 3) / 4, nrhs), dim3(256), 0, s, cp, nblk, nx, yl, ldy, half);
+    FP_HIP(hipGetLastError());
+}
+
+// Restriction of a line-sourced solution to the interface unknowns: t[k] is
+// the x-line coefficient (rows 0 / nx-1 in xu) or the physical y-line value
+// (yl, from k_colsum) of unknown k.  mode 0: out = v - t (v == nullptr: the
+// unit vector e_{k0 + r}); mode 1: out = t.
 __global__ void k_lines_assemble(const double* v, double* out, int K, const int* kface, const int* kpos,
-                                 const double* xu, int ny, const double* colpart, int nwg, int nx, int k0)
+                                 const double* xu, int ny, const double* yl, int nx, int k0, int mode)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= K) return;
@@ -229,27 +255,37 @@ __global__ void k_lines_assemble(const double* v, double* out, int K, const int*
     double          t;
     if (f == XNEG) t = xu[r * 2 * ny + p];
     else if (f == XPOS) t = xu[r * 2 * ny + ny + p];
-    else {
-        const double* cp = colpart + r * (long long)nwg * 2 * nx + (f == YNEG ? 0 : nx) + p;
-        double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
-        int    w  = 0;
-        for (; w + 4 <= nwg; w += 4) {
-            t0 += cp[(long long)w * 2 * nx];
-            t1 += cp[(long long)(w + 1) * 2 * nx];
-            t2 += cp[(long long)(w + 2) * 2 * nx];
-            t3 += cp[(long long)(w + 3) * 2 * nx];
-        }
-        for (; w < nwg; w++) t0 += cp[(long long)w * 2 * nx];
-        t = (t0 + t1) + (t2 + t3);
+    else t = yl[r * 2 * nx + (f == YNEG ? 0 : nx) + p];
+    if (mode == 1) {
+        out[r * K + k] = t;
+        return;
     }
     const double vv = v ? v[r * K + k] : (k == k0 + (int)r ? 1.0 : 0.0);
     out[r * K + k]  = vv - t;
 }
 void launch_lines_assemble(const double* v, double* out, int K, const int* kface, const int* kpos, const double* xu,
-                           int ny, const double* colpart, int nwg, int nx, int nrhs, hipStream_t s, int k0)
+                           int ny, const double* yl, int nx, int nrhs, hipStream_t s, int k0, int mode)
 {
     hipLaunchKernelGGL(k_lines_assemble, dim3((K + 63) / 64, nrhs), dim3(64), 0, s, v, out, K, kface, kpos, xu,
-                       ny, colpart, nwg, nx, k0);
+                       ny, yl, nx, k0, mode);
+    FP_HIP(hipGetLastError());
+}
+
+// Batched strided line copies: dst[r*ldd + t] = coef * src[r*lds + t*stride],
+// t < len, for up to 8 lines (interface lines handed between subdomains).
+__global__ void k_line_copy(LineCopyBatch b)
+{
+    const LineCopy& c = b.c[blockIdx.z];
+    const long long r = blockIdx.y;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < c.len; t += gridDim.x * blockDim.x)
+        c.dst[r * c.ldd + t] = c.coef * c.src[r * c.lds + (long long)t * c.stride];
+}
+void launch_line_copy(const LineCopyBatch& b, int nrhs, hipStream_t s)
+{
+    if (b.n <= 0) return;
+    int maxlen = 0;
+    for (int q = 0; q < b.n; q++) maxlen = std::max(maxlen, b.c[q].len);
+    hipLaunchKernelGGL(k_line_copy, dim3((maxlen + VB - 1) / VB, nrhs, b.n), dim3(VB), 0, s, b);
     FP_HIP(hipGetLastError());
 }
 
