@@ -86,7 +86,7 @@ def main():
         info = {}
 
         def step():
-            for w, p in zip(work, pristine):
+            for w, p in zip(work, pristine):  # refill the in-place right-hand sides
                 w.copy_(p)
             info.update(cat.solve(work))
 

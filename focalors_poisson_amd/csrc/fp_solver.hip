@@ -175,6 +175,10 @@ struct ChildLink {
     DevBuf<double> S;      // cn x cn, value(i, j) row-major
     DevBuf<double> St;     // x-line child: Phi S Phi^T in the parent's y basis
     DevBuf<int>    ccell;  // child's adjacent line cells
+    // x-line children whose y basis differs from the parent's: Q = T_parent T_child^T
+    // maps the child's transform coefficients of a row to the parent's; Qt = Q^T
+    // the other way (the transforms are orthonormal)
+    DevBuf<double> Q, Qt;
 };
 
 struct Node {
@@ -199,6 +203,8 @@ struct Node {
     DevBuf<int>    state;
     int*           h_state = nullptr;
     double*        h_resid = nullptr;  // nrhs residual norms, then nrhs initial norms
+    int*           dh_state = nullptr; // the same host memory as the device addresses it
+    double*        dh_resid = nullptr;
     ~Node()
     {
         if (h_state) (void)hipHostFree(h_state);
@@ -214,7 +220,6 @@ struct FastWS {
     std::vector<DevBuf<double>> LBr;  // the root's adjacent line in the child's basis (packed lines)
     std::vector<DevBuf<double>> CPc;  // partial column sums of the child's upward sweep
     std::vector<DevBuf<double>> Xs;   // the child's solved adjacent row (child's basis)
-    std::vector<DevBuf<double>> Xp;   // that row in physical space
 };
 
 }  // namespace fp
@@ -312,6 +317,11 @@ static std::vector<int> child_cells(int loc, int bnx, int bny)
     return v;
 }
 
+// two subdomains' y transforms agree (same length and boundary pair)
+static bool same_basis(const SubPlan& a, const SubPlan& b)
+{
+    return a.kind == b.kind && a.ny == b.ny;
+}
 
 static void ensure_ws(fp_cat_s* c, Node& nd, int nrhs)
 {
@@ -346,8 +356,10 @@ static void ensure_ws(fp_cat_s* c, Node& nd, int nrhs)
     FP_HIP(hipDeviceSynchronize());
     if (nd.h_state) (void)hipHostFree(nd.h_state);
     if (nd.h_resid) (void)hipHostFree(nd.h_resid);
-    FP_HIP(hipHostMalloc((void**)&nd.h_state, sizeof(int) * 4 * nrhs));
-    FP_HIP(hipHostMalloc((void**)&nd.h_resid, sizeof(double) * 2 * nrhs));
+    FP_HIP(hipHostMalloc((void**)&nd.h_state, sizeof(int) * 4 * nrhs, hipHostMallocMapped | hipHostMallocCoherent));
+    FP_HIP(hipHostMalloc((void**)&nd.h_resid, sizeof(double) * 2 * nrhs, hipHostMallocMapped | hipHostMallocCoherent));
+    FP_HIP(hipHostGetDevicePointer((void**)&nd.dh_state, nd.h_state, 0));
+    FP_HIP(hipHostGetDevicePointer((void**)&nd.dh_resid, nd.h_resid, 0));
     nd.ws_nrhs = nrhs;
     nd.ws_m    = m;
 }
@@ -359,6 +371,7 @@ static GmresDev gdev(fp_cat_s* c, Node& nd, int nrhs)
     g.V = nd.V.p; g.Wv = nd.Wv.p; g.X = nd.X.p; g.C = nd.C.p; g.H = nd.H.p; g.Hacc = nd.Hacc.p;
     g.cs = nd.cs.p; g.sn = nd.sn.p; g.g = nd.g.p; g.Y = nd.Y.p; g.part = nd.part.p; g.part2 = nd.part2.p;
     g.tolabs = nd.tolabs.p; g.resid = nd.resid.p; g.rnorm = nd.rnorm.p; g.state = nd.state.p;
+    g.hstate = nd.dh_state; g.hres = nd.dh_resid;
     return g;
 }
 
@@ -477,9 +490,12 @@ static void build_interface_inverse(fp_cat_s* c, Node& nd, hipStream_t s)
 
 static void read_state(Node& nd, int nrhs, hipStream_t s)
 {
-    FP_HIP(hipMemcpyAsync(nd.h_state, nd.state.p, sizeof(int) * 4 * nrhs, hipMemcpyDeviceToHost, s));
-    FP_HIP(hipMemcpyAsync(nd.h_resid, nd.resid.p, sizeof(double) * nrhs, hipMemcpyDeviceToHost, s));
-    FP_HIP(hipMemcpyAsync(nd.h_resid + nrhs, nd.rnorm.p, sizeof(double) * nrhs, hipMemcpyDeviceToHost, s));
+    GmresDev g{};
+    g.nrhs  = nrhs;
+    g.state = nd.state.p;
+    g.resid = nd.resid.p;
+    g.rnorm = nd.rnorm.p;
+    gm_publish(g, nd.dh_state, nd.dh_resid, s);
     FP_HIP(hipStreamSynchronize(s));
 }
 
@@ -500,8 +516,8 @@ static void gmres_iterate(fp_cat_s* c, Node& nd, int nrhs, hipStream_t s)
     bool fresh    = false;  // host copy of the state is current
     for (int outer = 0; outer < c->maxiter; outer++) {
         matvec(c, nd, nd.X.p, nd.Wv.p, nrhs, s);
-        gm_resid(g, s);
-        read_state(nd, nrhs, s);
+        if (gm_resid(g, s)) FP_HIP(hipStreamSynchronize(s));
+        else read_state(nd, nrhs, s);
         fresh         = true;
         bool all_done = true;
         for (int r = 0; r < nrhs; r++)
@@ -703,6 +719,23 @@ static void build_node(fp_cat_s* c, int d, hipStream_t s)
         launch_ytransform<double>(N.plan.t64, t1.p, cn, cn, 1, 0, s);
         launch_transpose(t1.p, L.St.p, cn, s);
         launch_ytransform<double>(N.plan.t64, L.St.p, cn, cn, 1, 0, s);
+        FP_HIP(hipStreamSynchronize(s));
+    }
+    for (auto& L : N.ch) {
+        if (L.loc != XNEG && L.loc != XPOS) continue;
+        const Node& C = *c->nodes[L.dom];
+        if (same_basis(C.plan, N.plan)) continue;
+        // rows e_k -> child inverse transform -> parent forward transform: row k
+        // of the result is Q e_k, i.e. the result is Q^T (row-major)
+        const int           n = L.cn;
+        std::vector<double> eye((size_t)n * n, 0.0);
+        for (int k = 0; k < n; k++) eye[(size_t)k * n + k] = 1.0;
+        L.Qt.alloc(eye.size());
+        L.Qt.upload(eye.data(), eye.size());
+        launch_ytransform<double>(C.plan.t64, L.Qt.p, n, n, 1, 1, s);
+        launch_ytransform<double>(N.plan.t64, L.Qt.p, n, n, 1, 0, s);
+        L.Q.alloc(eye.size());
+        launch_transpose(L.Qt.p, L.Q.p, n, s);
         FP_HIP(hipStreamSynchronize(s));
     }
     if (c->precond_kind == 1 && N.K > 0 && N.K <= 16384) build_interface_inverse(c, N, s);
@@ -988,22 +1021,17 @@ static void cat_refine(fp_cat_s* c, double* const* f, int nrhs, hipStream_t s)
 //      load): F_d;
 //   2. the children's upward solves A_c^-1 F_c are swept with only the line
 //      next to the root written out (x-lines: the solved row, taken to the
-//      root's basis by two one-row transforms unless the bases agree;
+//      root's basis by the dense orthogonal map Q unless the bases agree;
 //      y-lines: column sums);
 //   3. the root's sweep of F_r minus those lines writes only its interface
 //      lines: the GMRES right-hand side R A_r^-1 b';
 //   4. after GMRES, the root's sweep of F_r - child lines + E S y writes the
-//      whole coefficient field, which is transformed back in place;
+//      whole coefficient field and its interface rows and column sums;
 //   5. the root's line next to each child (in the child's basis for x-lines)
-//      is subtracted in the children's final sweeps, which write their fields
-//      for the batched inverse transforms.
+//      is subtracted in the children's final sweeps, which write their
+//      fields; one batched launch transforms every domain back.
 // Every field is read and written in place: two transforms and two sweeps per
 // domain, no copies, fills or full-field scatters.
-
-static bool same_basis(const SubPlan& a, const SubPlan& b)
-{
-    return a.kind == b.kind && a.ny == b.ny;
-}
 
 static bool fast_ok(const fp_cat_s* c)
 {
@@ -1023,37 +1051,18 @@ static void ensure_fast_ws(fp_cat_s* c, int nrhs)
     w.LBr.clear();
     w.CPc.clear();
     w.Xs.clear();
-    w.Xp.clear();
     w.LBr.resize(nc);
     w.CPc.resize(nc);
     w.Xs.resize(nc);
-    w.Xp.resize(nc);
     for (size_t k = 0; k < nc; k++) {
         const Node& C = *c->nodes[R.ch[k].dom];
         w.LBr[k].alloc((size_t)nrhs * C.ldl);
         launch_fill<double>(w.LBr[k].p, (long long)nrhs * C.ldl, 0.0, nullptr);
         w.CPc[k].alloc((size_t)nrhs * sweep_blocks(C.plan) * 2 * c->nx[C.dom]);
         w.Xs[k].alloc((size_t)nrhs * c->ny[C.dom]);
-        w.Xp[k].alloc((size_t)nrhs * c->ny[C.dom]);
     }
     FP_HIP(hipDeviceSynchronize());
     w.nrhs = nrhs;
-}
-
-static TransformJob<double> row_job(const TransformPlan<double>* plan, const double* src, long long lds, double* dst,
-                                    long long ldd, int nrhs)
-{
-    TransformJob<double> j{};
-    j.plan  = plan;
-    j.data  = dst;
-    j.ld    = ldd;
-    j.nx    = 1;
-    j.nrhs  = nrhs;
-    j.src   = src;
-    j.lds   = lds;
-    j.src64 = 1;
-    j.dst64 = 1;
-    return j;
 }
 
 static void cat_sweep_fast(fp_cat_s* c, double* const* f, const double* const* ghost, int nrhs, hipStream_t s)
@@ -1063,7 +1072,7 @@ static void cat_sweep_fast(fp_cat_s* c, double* const* f, const double* const* g
     c->last_steps = c->last_restarts = c->last_status = 0;
     c->last_res = c->last_rel = 0.0;
     // 1. assembly + forward transforms of every domain
-    std::vector<TransformJob<double>> fw(nd);
+    std::vector<TransformJob<double>> fw(nd), iv(nd);
     for (int d = 0; d < nd; d++) {
         TransformJob<double>& j = fw[d];
         j       = {};
@@ -1073,6 +1082,7 @@ static void cat_sweep_fast(fp_cat_s* c, double* const* f, const double* const* g
         j.nx    = c->nx[d];
         j.nrhs  = nrhs;
         j.dst64 = 1;
+        iv[d]   = j;
         j.asmb  = 1;
         j.hx2   = c->hx[d] * c->hx[d];
         face_terms(c, d, ghost, j.g, j.ga);
@@ -1089,10 +1099,15 @@ static void cat_sweep_fast(fp_cat_s* c, double* const* f, const double* const* g
     ensure_fast_ws(c, nrhs);
     FastWS&      w  = c->fast;
     const size_t nc = R.ch.size();
+    FP_REQUIRE(nc <= 8, "too many children");
 
-    // 2. children's upward solves, only the line next to the root written
-    std::vector<SweepJob<double>>     up(nc);
-    std::vector<TransformJob<double>> inv_rows, fwd_rows;
+    // 2. children's upward solves, only the line next to the root written:
+    //    root face XNEG (row 0) meets the child's row nx-1, XPOS its row 0;
+    //    root face YNEG (column 0) meets the child's column ny-1 (phi1 sums),
+    //    YPOS its column 0 (phi0 sums)
+    std::vector<SweepJob<double>> up(nc);
+    ColsumBatch                   ycs;
+    GemvBatch                     qup;
     for (size_t k = 0; k < nc; k++) {
         const ChildLink&  L = R.ch[k];
         const Node&       C = *c->nodes[L.dom];
@@ -1106,33 +1121,21 @@ static void cat_sweep_fast(fp_cat_s* c, double* const* f, const double* const* g
         j.fin  = 1;
         j.fout = 0;
         if (L.loc == XNEG || L.loc == XPOS) {
-            // root face XNEG (row 0) meets the child's row nx-1, XPOS its row 0
-            const bool same = same_basis(C.plan, R.plan);
-            double*    slot = w.LBc.p + (L.loc == XNEG ? 0 : nyr);
-            double*    dst  = same ? slot : w.Xs[k].p;
-            j.ldxo          = same ? R.ldl : nyc;
+            double* slot = w.LBc.p + (L.loc == XNEG ? 0 : nyr);
+            double* dst  = L.Q.p ? w.Xs[k].p : slot;
+            j.ldxo       = L.Q.p ? nyc : R.ldl;
             if (L.loc == XNEG) j.xo1 = dst;
             else j.xo0 = dst;
-            if (!same) {
-                inv_rows.push_back(row_job(&C.plan.t64, w.Xs[k].p, nyc, w.Xp[k].p, nyc, nrhs));
-                fwd_rows.push_back(row_job(&R.plan.t64, w.Xp[k].p, nyc, slot, R.ldl, nrhs));
-            }
-        } else
+            if (L.Q.p) qup.j[qup.n++] = {L.Q.p, nyc, w.Xs[k].p, nyc, slot, R.ldl};
+        } else {
             j.cp = w.CPc[k].p;
+            ycs.j[ycs.n++] = {w.CPc[k].p, sweep_blocks(C.plan), c->nx[L.dom], L.loc == YNEG ? 1 : 0,
+                              w.LBc.p + 2LL * nyr + (L.loc == YNEG ? 0 : nxr), R.ldl};
+        }
     }
     launch_sweep_batch<double>(up.data(), (int)nc, s);
-    for (size_t k = 0; k < nc; k++) {
-        const ChildLink& L = R.ch[k];
-        if (L.loc != YNEG && L.loc != YPOS) continue;
-        // root face YNEG (column 0) meets the child's column ny-1 (phi1 sums), YPOS its column 0
-        double* slot = w.LBc.p + 2LL * nyr + (L.loc == YNEG ? 0 : nxr);
-        launch_colsum(w.CPc[k].p, sweep_blocks(c->nodes[L.dom]->plan), c->nx[L.dom], slot, R.ldl, nrhs, s,
-                      L.loc == YNEG ? 1 : 0);
-    }
-    if (!inv_rows.empty()) {
-        launch_ytransform_batch<double>(inv_rows.data(), (int)inv_rows.size(), 1, s);
-        launch_ytransform_batch<double>(fwd_rows.data(), (int)fwd_rows.size(), 0, s);
-    }
+    launch_colsum_batch(ycs, nrhs, s);
+    launch_gemv_batch(qup, nrhs, s);
 
     // 3. the root's interface lines of A_r^-1 b' (the GMRES right-hand side)
     {
@@ -1156,7 +1159,8 @@ static void cat_sweep_fast(fp_cat_s* c, double* const* f, const double* const* g
     }
     gmres_iterate(c, R, nrhs, s);
 
-    // 4. the root's solution: sweep of F_r - child lines + E S y, then back to physical space
+    // 4. the root's solution: sweep of F_r - child lines + E S y, written as
+    //    coefficients, with its interface rows and column sums for the children
     launch_schur_lines(schur_lines_args(c, R, R.X.p), nrhs, s);
     {
         SweepJob<double> j{};
@@ -1168,44 +1172,46 @@ static void cat_sweep_fast(fp_cat_s* c, double* const* f, const double* const* g
         j.lbm  = w.LBc.p;
         j.ldl  = R.ldl;
         j.phi  = R.phi.p;
+        j.xo0  = R.XU.p;
+        j.xo1  = R.XU.p + nyr;
+        j.ldxo = 2LL * nyr;
+        j.cp   = R.has_y ? R.CP.p : nullptr;
         j.fin  = 1;
         j.fout = 1;
         launch_sweep_batch<double>(&j, 1, s);
     }
-    launch_ytransform<double>(R.plan.t64, f[root], nr, nxr, nrhs, 1, s);
 
-    // 5. the root's lines into the children's right-hand sides, then their final solves
-    std::vector<TransformJob<double>> rows;
-    LineCopyBatch                     lc;
+    // 5. the root's line next to each child into the child's right-hand side:
+    //    x-lines from the solved rows (through Q^T when the bases differ),
+    //    y-lines from the column sums; then the children's final solves and
+    //    every domain's inverse transform in one batch
+    ColsumBatch   ydn;
+    GemvBatch     qdn;
+    LineCopyBatch lc;
     for (size_t k = 0; k < nc; k++) {
         const ChildLink& L = R.ch[k];
         const Node&      C = *c->nodes[L.dom];
         const int        nxc = c->nx[L.dom], nyc = c->ny[L.dom];
         if (L.loc == XNEG || L.loc == XPOS) {
             // root row 0 (XNEG) -> child's row nx-1 (slot x1); root row nx-1 (XPOS) -> child's row 0 (x0)
-            const double* src = f[root] + (L.loc == XNEG ? 0 : (long long)(nxr - 1) * nyr);
-            rows.push_back(row_job(&C.plan.t64, src, nr, w.LBr[k].p + (L.loc == XNEG ? nyc : 0), C.ldl, nrhs));
+            const double* src = R.XU.p + (L.loc == XNEG ? 0 : nyr);
+            double*       dst = w.LBr[k].p + (L.loc == XNEG ? nyc : 0);
+            if (L.Qt.p) qdn.j[qdn.n++] = {L.Qt.p, nyc, src, 2LL * nyr, dst, C.ldl};
+            else lc.c[lc.n++] = {src, 2LL * nyr, 1, nyc, dst, C.ldl, 1.0};
         } else {
             // root column 0 (YNEG) -> child's column ny-1 (slot y1); root column ny-1 (YPOS) -> child's column 0 (y0)
-            FP_REQUIRE(lc.n < 8, "too many y-line children");
-            LineCopy& q = lc.c[lc.n++];
-            q.src       = f[root] + (L.loc == YNEG ? 0 : nyr - 1);
-            q.lds       = nr;
-            q.stride    = nyr;
-            q.len       = nxr;
-            q.dst       = w.LBr[k].p + 2LL * nyc + (L.loc == YNEG ? nxc : 0);
-            q.ldd       = C.ldl;
-            q.coef      = 1.0;
+            ydn.j[ydn.n++] = {R.CP.p, sweep_blocks(R.plan), nxr, L.loc == YNEG ? 0 : 1,
+                              w.LBr[k].p + 2LL * nyc + (L.loc == YNEG ? nxc : 0), C.ldl};
         }
     }
-    if (!rows.empty()) launch_ytransform_batch<double>(rows.data(), (int)rows.size(), 0, s);
     launch_line_copy(lc, nrhs, s);
-    std::vector<SweepJob<double>>     dn(nc);
-    std::vector<TransformJob<double>> iv(nc);
+    launch_gemv_batch(qdn, nrhs, s);
+    launch_colsum_batch(ydn, nrhs, s);
+    std::vector<SweepJob<double>> dn(nc);
     for (size_t k = 0; k < nc; k++) {
-        const int   cd = R.ch[k].dom;
-        const Node& C  = *c->nodes[cd];
-        SweepJob<double>& j = dn[k];
+        const int         cd = R.ch[k].dom;
+        const Node&       C  = *c->nodes[cd];
+        SweepJob<double>& j  = dn[k];
         j.sp   = &C.plan;
         j.nrhs = nrhs;
         j.f    = f[cd];
@@ -1215,16 +1221,9 @@ static void cat_sweep_fast(fp_cat_s* c, double* const* f, const double* const* g
         j.phi  = C.phi.p;
         j.fin  = 1;
         j.fout = 1;
-        iv[k]       = {};
-        iv[k].plan  = &C.plan.t64;
-        iv[k].data  = f[cd];
-        iv[k].ld    = dom_n(c, cd);
-        iv[k].nx    = c->nx[cd];
-        iv[k].nrhs  = nrhs;
-        iv[k].dst64 = 1;
     }
     launch_sweep_batch<double>(dn.data(), (int)nc, s);
-    launch_ytransform_batch<double>(iv.data(), (int)nc, 1, s);
+    launch_ytransform_batch<double>(iv.data(), nd, 1, s);
 }
 
 // ConcatPoissonSolver2D::solve (pe/concat/concat_solver2d.cpp:101-157)

@@ -46,6 +46,30 @@ void launch_schur_lines(const SchurLinesArgs& a, int nrhs, hipStream_t s);
 // (half = 0 / 1: only the column-0 / column-(ny-1) sums, nx values)
 void launch_colsum(const double* cp, int nblk, int nx, double* yl, long long ldy, int nrhs, hipStream_t s,
                    int half = -1);
+struct ColsumJob {
+    const double* cp;
+    int           nblk, nx, half;
+    double*       yl;
+    long long     ldy;
+};
+struct ColsumBatch {
+    int       n = 0;
+    ColsumJob j[8];
+};
+void launch_colsum_batch(const ColsumBatch& b, int nrhs, hipStream_t s);
+struct GemvJob {
+    const double* A;  // n x n row-major
+    int           n;
+    const double* x;
+    long long     ldx;
+    double*       y;
+    long long     ldy;
+};
+struct GemvBatch {
+    int     n = 0;
+    GemvJob j[8];
+};
+void launch_gemv_batch(const GemvBatch& b, int nrhs, hipStream_t s);
 // mode 0: out = v - t (v == nullptr: the unit vector e_{k0 + r} of right-hand
 // side r); mode 1: out = t; t from the x rows xu and the y-lines yl (2 nx per rhs)
 void launch_lines_assemble(const double* v, double* out, int K, const int* kface, const int* kpos, const double* xu,
@@ -91,10 +115,13 @@ struct GmresDev {
     double* tolabs; // nrhs
     double* resid;  // nrhs
     int*    state;  // nrhs x 4: active, valid columns, status (1 converged, 2 numeric), steps
+    int*    hstate = nullptr;  // host-mapped copies written at cycle start (or null)
+    double* hres   = nullptr;  // nrhs residuals, then nrhs initial norms
 };
 
 void gm_init(const GmresDev& g, double tol, int rel, hipStream_t s);
-void gm_resid(const GmresDev& g, hipStream_t s);
+void gm_publish(const GmresDev& g, int* hstate, double* hres, hipStream_t s);
+bool gm_resid(const GmresDev& g, hipStream_t s);
 void gm_arnoldi(const GmresDev& g, int j, hipStream_t s);
 void gm_finish_cycle(const GmresDev& g, hipStream_t s);
 

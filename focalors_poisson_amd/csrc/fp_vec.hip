@@ -221,23 +221,58 @@ void launch_schur_lines(const SchurLinesArgs& a, int nrhs, hipStream_t s)
 // wavefront per output in a fixed order (reproducible).  yl entries are ldy
 // apart per right-hand side.
 // half = 0 or 1: only that column's sums, written at yl[r * ldy + i].
-__global__ __launch_bounds__(256) void k_colsum(const double* __restrict__ cp, int nblk, int nx, double* yl,
-                                                long long ldy, int half)
+// Up to 8 independent reductions per launch (blockIdx.z).
+__global__ __launch_bounds__(256) void k_colsum(ColsumBatch b)
 {
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int o    = blockIdx.x * 4 + wave;  // output index
-    if (o >= (half < 0 ? 2 : 1) * nx) return;
+    const ColsumJob& J    = b.j[blockIdx.z];
+    const int        wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int        o    = blockIdx.x * 4 + wave;  // output index
+    if (o >= (J.half < 0 ? 2 : 1) * J.nx) return;
     const long long r   = blockIdx.y;
-    const double*   src = cp + r * (long long)nblk * 2 * nx + o + (half > 0 ? nx : 0);
+    const double*   src = J.cp + r * (long long)J.nblk * 2 * J.nx + o + (J.half > 0 ? J.nx : 0);
     double          acc = 0.0;
-    for (int b = lane; b < nblk; b += 64) acc += src[(long long)b * 2 * nx];
+    for (int q = lane; q < J.nblk; q += 64) acc += src[(long long)q * 2 * J.nx];
     acc = wave_reduce_sum(acc);
-    if (lane == 0) yl[r * ldy + o] = acc;
+    if (lane == 0) J.yl[r * J.ldy + o] = acc;
+}
+void launch_colsum_batch(const ColsumBatch& b, int nrhs, hipStream_t s)
+{
+    if (b.n <= 0) return;
+    int nout = 0;
+    for (int q = 0; q < b.n; q++) nout = std::max(nout, (b.j[q].half < 0 ? 2 : 1) * b.j[q].nx);
+    hipLaunchKernelGGL(k_colsum, dim3((nout + 3) / 4, nrhs, b.n), dim3(256), 0, s, b);
+    FP_HIP(hipGetLastError());
 }
 void launch_colsum(const double* cp, int nblk, int nx, double* yl, long long ldy, int nrhs, hipStream_t s, int half)
 {
-    const int nout = (half < 0 ? 2 : 1) * nx;
-    hipLaunchKernelGGL(k_colsum, dim3((nout + 3) / 4, nrhs), dim3(256), 0, s, cp, nblk, nx, yl, ldy, half);
+    ColsumBatch b;
+    b.n    = 1;
+    b.j[0] = {cp, nblk, nx, half, yl, ldy};
+    launch_colsum_batch(b, nrhs, s);
+}
+
+// Batched strided dense products y[r*ldy + i] = sum_j A[i*n + j] x[r*ldx + j]
+// (basis changes of interface rows between subdomains), one wavefront per row.
+__global__ __launch_bounds__(256) void k_gemv_batch(GemvBatch b)
+{
+    const GemvJob& J    = b.j[blockIdx.z];
+    const int      wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int      i    = blockIdx.x * 4 + wave;
+    if (i >= J.n) return;
+    const long long r   = blockIdx.y;
+    const double*   Ai  = J.A + (long long)i * J.n;
+    const double*   xr  = J.x + r * J.ldx;
+    double          acc = 0.0;
+    for (int q = lane; q < J.n; q += 64) acc += Ai[q] * xr[q];
+    acc = wave_reduce_sum(acc);
+    if (lane == 0) J.y[r * J.ldy + i] = acc;
+}
+void launch_gemv_batch(const GemvBatch& b, int nrhs, hipStream_t s)
+{
+    if (b.n <= 0) return;
+    int nmax = 0;
+    for (int q = 0; q < b.n; q++) nmax = std::max(nmax, b.j[q].n);
+    hipLaunchKernelGGL(k_gemv_batch, dim3((nmax + 3) / 4, nrhs, b.n), dim3(256), 0, s, b);
     FP_HIP(hipGetLastError());
 }
 
@@ -747,20 +782,128 @@ __global__ __launch_bounds__(GB) void k_gm_update_x(GmresDev g)
     g.X[r * K + k] = x;
 }
 
+// state, residuals and initial norms to host-mapped memory (read by the host
+// after a stream synchronisation; no copy commands)
+__global__ void k_gm_publish(GmresDev g, int* hstate, double* hres)
+{
+    for (int q = threadIdx.x; q < 4 * g.nrhs; q += blockDim.x) hstate[q] = g.state[q];
+    for (int q = threadIdx.x; q < g.nrhs; q += blockDim.x) {
+        hres[q]          = g.resid[q];
+        hres[g.nrhs + q] = g.rnorm[q];
+    }
+}
+void gm_publish(const GmresDev& g, int* hstate, double* hres, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_gm_publish, dim3(1), dim3(256), 0, s, g, hstate, hres);
+    FP_HIP(hipGetLastError());
+}
+
+// One-workgroup forms of the cycle set-up for interface vectors that one
+// workgroup streams quickly (the usual case): a fixed-order block reduction
+// replaces the partial-sum kernel pair, and the cycle start also publishes
+// the state to host-mapped memory.
+constexpr int GW = 1024;
+
+__device__ double block_sum_fixed(double v, double* red)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    v = wave_sum(v);
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); q++) t += red[q];
+    __syncthreads();
+    return t;
+}
+
+__global__ __launch_bounds__(GW) void k_gm_init1(GmresDev g, double tol, int rel)
+{
+    __shared__ double red[GW / 64];
+    const int       r   = blockIdx.x;
+    const long long K   = g.K;
+    double          acc = 0.0;
+    for (long long k = threadIdx.x; k < K; k += GW) {
+        const double c = g.C[r * K + k];
+        acc += c * c;
+    }
+    const double nc = sqrt(block_sum_fixed(acc, red));
+    if (threadIdx.x == 0) {
+        g.tolabs[r]        = rel ? tol * nc : tol;
+        g.resid[r]         = nc;
+        g.rnorm[r]         = nc;
+        g.state[r * 4 + 0] = 0;
+        g.state[r * 4 + 1] = 0;
+        g.state[r * 4 + 2] = 0;
+        g.state[r * 4 + 3] = 0;
+    }
+}
+
+// R = C - BX, beta = ||R||, V0 = R / beta, convergence test at cycle start
+// (gmres_solver2d.cpp:148-168)
+__global__ __launch_bounds__(GW) void k_gm_start1(GmresDev g)
+{
+    __shared__ double red[GW / 64];
+    const int       r   = blockIdx.x;
+    const long long K   = g.K;
+    double          acc = 0.0;
+    for (long long k = threadIdx.x; k < K; k += GW) {
+        const double t = g.C[r * K + k] - g.Wv[r * K + k];
+        g.V[r * K + k] = t;
+        acc += t * t;
+    }
+    const double beta = sqrt(block_sum_fixed(acc, red));
+    if (!isfinite(beta)) {
+        if (threadIdx.x == 0) {
+            g.state[r * 4 + 0] = 0;
+            g.state[r * 4 + 2] = 2;  // numeric failure
+            g.resid[r]         = beta;
+        }
+    } else {
+        const bool conv = beta < g.tolabs[r];
+        if (!conv)
+            for (long long k = threadIdx.x; k < K; k += GW) g.V[r * K + k] *= (1.0 / beta);
+        const int m = g.m;
+        for (int q = threadIdx.x; q <= m; q += GW) g.g[(long long)r * (m + 1) + q] = q == 0 ? beta : 0.0;
+        if (threadIdx.x == 0) {
+            g.resid[r]         = beta;
+            g.state[r * 4 + 0] = conv ? 0 : 1;  // active in this cycle
+            g.state[r * 4 + 1] = 0;             // valid columns
+            if (conv) g.state[r * 4 + 2] = 1;   // converged
+        }
+    }
+    if (threadIdx.x == 0 && g.hstate) {
+        for (int q = 0; q < 4; q++) g.hstate[r * 4 + q] = g.state[r * 4 + q];
+        g.hres[r]          = g.resid[r];
+        g.hres[g.nrhs + r] = g.rnorm[r];
+    }
+}
+
 void gm_init(const GmresDev& g, double tol, int rel, hipStream_t s)
 {
+    if (g.K <= (1 << 20)) {
+        hipLaunchKernelGGL(k_gm_init1, dim3(g.nrhs), dim3(GW), 0, s, g, tol, rel);
+        FP_HIP(hipGetLastError());
+        return;
+    }
     hipLaunchKernelGGL(k_gm_norm_part, dim3(g.nblk, g.nrhs), dim3(GB), 0, s, g);
     FP_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_gm_set_tol, dim3(g.nrhs), dim3(GB), 0, s, g, tol, rel);
     FP_HIP(hipGetLastError());
 }
 
-void gm_resid(const GmresDev& g, hipStream_t s)
+// returns true when the state was also published to host-mapped memory
+bool gm_resid(const GmresDev& g, hipStream_t s)
 {
+    if (g.K <= (1 << 20)) {
+        hipLaunchKernelGGL(k_gm_start1, dim3(g.nrhs), dim3(GW), 0, s, g);
+        FP_HIP(hipGetLastError());
+        return g.hstate != nullptr;
+    }
     hipLaunchKernelGGL(k_gm_resid, dim3(g.nblk, g.nrhs), dim3(GB), 0, s, g);
     FP_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_gm_cycle_start, dim3(g.nblk, g.nrhs), dim3(GB), 0, s, g);
     FP_HIP(hipGetLastError());
+    return false;
 }
 
 void gm_arnoldi(const GmresDev& g, int j, hipStream_t s)
