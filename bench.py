@@ -100,7 +100,34 @@ def main():
         ms = (time.perf_counter() - t0) / a.steps * 1e3
         return {"ms": ms, "build_s": t_build, "info": dict(info), "u": [w.cpu().numpy() for w in work]}
 
+    def run_graph():
+        """FP64 solve captured once in a HIP graph and replayed per step (the
+        right-hand sides refilled by eager copies on the same stream)."""
+        cat = fpa.ConcatPoissonSolver2D(ctx, doms, conn, m=a.m, tol=a.tol, tol_is_relative=True, max_iter=100)
+        stream = torch.cuda.Stream()
+        work = [torch.empty_like(p) for p in pristine]
+        with torch.cuda.stream(stream):
+            cat.graph_capture(work, stream=stream)
+
+            def step(wait=False):
+                for w, p in zip(work, pristine):
+                    w.copy_(p)
+                return cat.graph_launch(stream=stream, wait=wait)
+
+            for _ in range(a.warmup):
+                step()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(a.steps):
+                step()
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t0) / a.steps * 1e3
+            info = step(wait=True)  # the residual check of a replay
+        torch.cuda.synchronize()
+        return {"ms": ms, "build_s": 0.0, "info": dict(info), "u": [w.cpu().numpy() for w in work]}
+
     v64 = run_variant(fpa.FP_F64)
+    vg = run_graph()
     v32 = run_variant(fpa.FP_F32_REFINE)
 
     # ---- per-kernel timing on the centre subdomain (the GMRES matvec's solve) ----
@@ -121,7 +148,7 @@ def main():
 
     # ---- CPU baseline + accuracy: the oracle on the host cores ----
     cpu = None
-    err = {"f64": None, "f32_refine": None}
+    err = {"f64": None, "f64_graph": None, "f32_refine": None}
     if not a.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
@@ -133,7 +160,7 @@ def main():
         ref, it, _ = oc.solve([f[0] for f in fields])
         t_cpu = time.perf_counter() - t0
         y = np.concatenate([r.ravel() for r in ref])
-        for key, v in (("f64", v64), ("f32_refine", v32)):
+        for key, v in (("f64", v64), ("f64_graph", vg), ("f32_refine", v32)):
             x = np.concatenate([u.ravel() for u in v["u"]])
             err[key] = float(np.linalg.norm(x - y) / np.linalg.norm(y))
         cpu = {"value": round(1.0 / t_cpu, 4), "unit": "cross solves/s", "cores": O.num_threads(),
@@ -149,8 +176,12 @@ def main():
                 "gmres_iterations": i.get("iterations"), "rel_residual": i.get("rel_residual"),
                 "refine_sweeps": i.get("refine_steps"), "refine_error_estimate": i.get("refine_residual")}
 
-    variants = {"f64": variant(v64, "f64"), "f32_refine": variant(v32, "f32_refine")}
-    main_v = v64
+    variants = {"f64": variant(v64, "f64"), "f64_graph": variant(vg, "f64_graph"),
+                "f32_refine": variant(v32, "f32_refine")}
+    # the headline is the faster FP64 form whose residual check passed
+    graph_ok = vg["info"].get("status") == 0
+    main_key = "f64_graph" if graph_ok and vg["ms"] < v64["ms"] else "f64"
+    main_v = vg if main_key == "f64_graph" else v64
     out = {
         "metric": "cross-geometry solves/s at 1e-10 rel L2: FP64 vs FP32+refine; HBM GB/s",
         "value": round(1000.0 / main_v["ms"], 3),
@@ -166,7 +197,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "cross %d: centre %dx%d, arms %dx%d / %dx%d, FP64, interface GMRES(m=%d) to "
                                "relative %.0e" % (n, n, n, n // 2, n, n, n // 2, a.m, a.tol),
-                   "cells": cells, "rel_l2_vs_oracle": err["f64"], "variants": variants},
+                   "form": main_key, "cells": cells, "rel_l2_vs_oracle": err[main_key], "variants": variants},
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": dom["gbs"], "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(dom["gbs"] / PEAK_HBM_GBS, 4), "traffic": None},
         "kernels": kern,

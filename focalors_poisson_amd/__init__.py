@@ -196,6 +196,29 @@ class ConcatPoissonSolver2D:
         d["status"] = st
         return d
 
+    def graph_capture(self, fields, ghost=None, stream=None):
+        """Records the FP64 solve of these fields (and ghost arrays) into a HIP
+        graph on `stream` (a non-default torch stream; fp_cat_graph_capture).
+        The tensors stay bound to the graph: refill them, then graph_launch()."""
+        fp, gp, nrhs = self._ptrs(fields, ghost)
+        if stream is None:
+            raise ValueError("graph capture needs an explicit, non-default stream")
+        self._graph_keep = (list(fields), dict(ghost or {}), fp, gp)
+        _capi.check(_capi.lib().fp_cat_graph_capture(self._h, fp, gp, nrhs, _stream_ptr(stream)))
+
+    def graph_launch(self, stream=None, wait=True, allow_not_converged=False):
+        """Replays the captured solve.  wait=True: synchronises and returns the
+        residual check as a statistics dict; wait=False: returns None at once."""
+        if not wait:
+            _capi.check(_capi.lib().fp_cat_graph_launch(self._h, _stream_ptr(stream), None))
+            return None
+        info = _capi.SolveInfo()
+        allow = (_capi.FP_E_NOT_CONVERGED,) if allow_not_converged else ()
+        st = _capi.check(_capi.lib().fp_cat_graph_launch(self._h, _stream_ptr(stream), ctypes.byref(info)), allow)
+        d = info.as_dict()
+        d["status"] = st
+        return d
+
     def vjp(self, grad_u, faces=(), stream=None):
         """Adjoint of solve: returns (dL/df per domain, {(d, face): dL/dg}) for
         dL/du = grad_u (one tensor per domain, same shapes as the fields)."""

@@ -260,9 +260,15 @@ struct fp_cat_s {
     ~fp_cat_s()
     {
         if (h_rpart) (void)hipHostFree(h_rpart);
+        if (gexec) (void)hipGraphExecDestroy(gexec);
     }
     int                                    precond_kind = 1;  // 0: none, 1: dense inverse of the interface operator
     fp::FastWS                             fast;
+    // captured solve (fp_cat_graph_capture): replays the two-level sweep with
+    // the GMRES start from the explicit inverse and no further Krylov steps
+    bool                                   graph_mode = false;
+    hipGraphExec_t                         gexec      = nullptr;
+    int                                    gnrhs      = 0;
     // statistics of the last solve
     int    last_steps = 0, last_restarts = 0, last_status = 0;
     double last_res = 0.0, last_rel = 0.0;
@@ -505,6 +511,8 @@ static void read_state(Node& nd, int nrhs, hipStream_t s)
 // (I - R A^-1 E S) y = C.  With the explicit-inverse preconditioner the
 // iteration starts from x0 = M C (the reference starts from its right-hand
 // side); the first residual check then usually ends it.
+static void gmres_stats(fp_cat_s* c, Node& nd, int nrhs, int restarts);
+
 static void gmres_iterate(fp_cat_s* c, Node& nd, int nrhs, hipStream_t s)
 {
     const int K = nd.K, m = c->m;
@@ -512,7 +520,14 @@ static void gmres_iterate(fp_cat_s* c, Node& nd, int nrhs, hipStream_t s)
     else FP_HIP(hipMemcpyAsync(nd.X.p, nd.C.p, sizeof(double) * K * nrhs, hipMemcpyDeviceToDevice, s));
     const GmresDev g = gdev(c, nd, nrhs);
     gm_init(g, c->tol, c->tol_rel, s);
-    int  restarts = 0, status = 0;
+    if (c->graph_mode) {
+        // captured: the residual check at the start is published to host
+        // memory and read after the replay; no host synchronisation here
+        matvec(c, nd, nd.X.p, nd.Wv.p, nrhs, s);
+        FP_REQUIRE(gm_resid(g, s), "graph capture needs the one-workgroup GMRES start");
+        return;
+    }
+    int  restarts = 0;
     bool fresh    = false;  // host copy of the state is current
     for (int outer = 0; outer < c->maxiter; outer++) {
         matvec(c, nd, nd.X.p, nd.Wv.p, nrhs, s);
@@ -550,7 +565,13 @@ static void gmres_iterate(fp_cat_s* c, Node& nd, int nrhs, hipStream_t s)
         restarts++;
     }
     if (!fresh) read_state(nd, nrhs, s);
-    int    steps = 0;
+    gmres_stats(c, nd, nrhs, restarts);
+}
+
+// Statistics of the last GMRES of node nd from the host copy of its state.
+static void gmres_stats(fp_cat_s* c, Node& nd, int nrhs, int restarts)
+{
+    int    steps = 0, status = 0;
     double res = 0.0, rel = 0.0;
     for (int r = 0; r < nrhs; r++) {
         const double r0 = nd.h_resid[nrhs + r];
@@ -1476,6 +1497,73 @@ int fp_cat_solve(fp_cat cat, double* const* d_fields, const double* const* d_gho
             info->refine_steps    = cat->last_refine_steps;
             info->refine_residual = cat->last_refine_est;
         }
+    });
+    if (st == FP_OK && cat && cat->last_status) st = cat->last_status;
+    return st;
+}
+
+int fp_cat_graph_capture(fp_cat cat, double* const* d_fields, const double* const* d_ghost, int nrhs, void* stream)
+{
+    return guarded([&] {
+        FP_REQUIRE(cat && d_fields && nrhs >= 1, "null argument");
+        FP_REQUIRE(stream, "graph capture needs a stream other than the null stream");
+        for (int d = 0; d < cat->nd; d++) FP_REQUIRE(d_fields[d], "null field");
+        FP_REQUIRE(cat->precision == FP_F64 && fp::fast_ok(cat),
+                   "graph capture covers the FP64 solve of a root with leaf children");
+        fp::Node& R = *cat->nodes[cat->root];
+        FP_REQUIRE(R.ch.empty() || R.precond, "graph capture needs the explicit-inverse interface preconditioner");
+        FP_HIP(hipSetDevice(cat->ctx->device));
+        hipStream_t s = (hipStream_t)stream;
+        // every allocation before the capture
+        if (!R.ch.empty()) {
+            fp::ensure_ws(cat, R, nrhs);
+            fp::ensure_fast_ws(cat, nrhs);
+        }
+        FP_HIP(hipDeviceSynchronize());
+        if (cat->gexec) {
+            (void)hipGraphExecDestroy(cat->gexec);
+            cat->gexec = nullptr;
+        }
+        hipGraph_t graph = nullptr;
+        FP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        cat->graph_mode = true;
+        try {
+            fp::cat_sweep_fast(cat, d_fields, d_ghost, nrhs, s);
+        } catch (...) {
+            cat->graph_mode = false;
+            (void)hipStreamEndCapture(s, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        cat->graph_mode = false;
+        FP_HIP(hipStreamEndCapture(s, &graph));
+        hipError_t e = hipGraphInstantiate(&cat->gexec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        FP_HIP(e);
+        cat->gnrhs = nrhs;
+    });
+}
+
+int fp_cat_graph_launch(fp_cat cat, void* stream, fp_solve_info* info)
+{
+    int st = guarded([&] {
+        FP_REQUIRE(cat && cat->gexec, "no captured solve");
+        FP_HIP(hipSetDevice(cat->ctx->device));
+        hipStream_t s = (hipStream_t)stream;
+        FP_HIP(hipGraphLaunch(cat->gexec, s));
+        cat->last_status = 0;
+        if (!info) return;
+        FP_HIP(hipStreamSynchronize(s));
+        fp::Node& R = *cat->nodes[cat->root];
+        cat->last_steps = cat->last_restarts = 0;
+        cat->last_res = cat->last_rel = 0.0;
+        if (!R.ch.empty()) fp::gmres_stats(cat, R, cat->gnrhs, 0);
+        info->iterations      = cat->last_steps;
+        info->restarts        = cat->last_restarts;
+        info->residual        = cat->last_res;
+        info->rel_residual    = cat->last_rel;
+        info->refine_steps    = 0;
+        info->refine_residual = 0.0;
     });
     if (st == FP_OK && cat && cat->last_status) st = cat->last_status;
     return st;

@@ -31,13 +31,53 @@
 
 namespace fp {
 
+// Lane permutations of the wavefront by DPP (no LDS traffic): lanes whose
+// source lane does not exist, or whose row is masked off, keep `old`.
+template <int CTRL, int RM>
+__device__ __forceinline__ double dpp(double old, double v)
+{
+    const unsigned long long o = (unsigned long long)__double_as_longlong(old);
+    const unsigned long long x = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp((int)(unsigned)o, (int)(unsigned)x, CTRL, RM, 0xf, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp((int)(unsigned)(o >> 32), (int)(unsigned)(x >> 32), CTRL,
+                                                             RM, 0xf, false);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+template <int CTRL, int RM>
+__device__ __forceinline__ float dpp(float old, float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), CTRL, RM, 0xf, false));
+}
+
+// x -> A x + B after the map of the earlier lanes (Ap, Bp)
+template <typename T>
+__device__ __forceinline__ void affine_after(T& A, T& B, T Ap, T Bp)
+{
+    B = A * Bp + B;
+    A = A * Ap;
+}
+
+// Inclusive scan of affine maps over the 64 lanes, lane 0 applied first:
+// row_shr 1, 2, 4, 8 within rows of 16 lanes, then row_bcast 15 and 31
+// across rows (the identity map fills lanes without a source).
+template <typename T>
+__device__ __forceinline__ void wave_affine_scan(T& A, T& B)
+{
+    affine_after(A, B, dpp<0x111, 0xf>(T(1), A), dpp<0x111, 0xf>(T(0), B));
+    affine_after(A, B, dpp<0x112, 0xf>(T(1), A), dpp<0x112, 0xf>(T(0), B));
+    affine_after(A, B, dpp<0x114, 0xf>(T(1), A), dpp<0x114, 0xf>(T(0), B));
+    affine_after(A, B, dpp<0x118, 0xf>(T(1), A), dpp<0x118, 0xf>(T(0), B));
+    affine_after(A, B, dpp<0x142, 0xa>(T(1), A), dpp<0x142, 0xa>(T(0), B));
+    affine_after(A, B, dpp<0x143, 0xc>(T(1), A), dpp<0x143, 0xc>(T(0), B));
+}
+
 // Start values of the chunks of each mode: the chunk maps v -> SA v + SB
 // (LDS, chunk-major) are chained in application order and the exclusive
 // prefix, applied to 0, is written to SY.  backward = 1 chains the chunks from
 // the last row upwards (back substitution); mode sing_j always chains
 // forwards (the two prefix sums of the singular recurrence).  One wavefront
 // per mode: lane l composes q = ceil(P/64) consecutive chunk maps, a log-depth
-// shuffle scan chains the lanes, and each lane replays its q chunks.
+// DPP scan chains the lanes, and each lane replays its q chunks.
 template <typename T>
 __device__ void chain_chunks(const T* SA, const T* SB, T* SY, int P, int MB, int j0, int ny, int backward,
                              int sing_j)
@@ -55,15 +95,8 @@ __device__ void chain_chunks(const T* SA, const T* SB, T* SY, int P, int MB, int
             B           = a * B + SB[c * MB + jm];
             A           = a * A;
         }
-        for (int d = 1; d < 64; d <<= 1) {
-            const T Ap = __shfl_up(A, d, 64), Bp = __shfl_up(B, d, 64);
-            if (lane >= d) {
-                B = A * Bp + B;
-                A = A * Ap;
-            }
-        }
-        T v = __shfl_up(B, 1, 64);
-        if (lane == 0) v = T(0);
+        wave_affine_scan(A, B);
+        T v = dpp<0x138, 0xf>(T(0), B);  // wave_shr 1: the exclusive prefix (0 on lane 0)
         for (int p = p0; p < p1; p++) {
             const int c     = bw ? P - 1 - p : p;
             SY[c * MB + jm] = v;

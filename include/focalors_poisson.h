@@ -151,6 +151,20 @@ int  fp_cat_root(fp_cat cat);
  * Variable2D::buffer_map values, variable2d.cpp:345-416), or NULL for none. */
 int fp_cat_solve(fp_cat cat, double* const* d_fields, const double* const* d_ghost, int nrhs, void* stream,
                  fp_solve_info* info);
+/* Captured composite solve (no reference counterpart; the reference has no
+ * device): records the whole FP64 solve of fp_cat_solve on these fields and
+ * ghost arrays into a HIP graph.  Covered: a root with leaf children (depth
+ * two, e.g. the cross) or one domain, with the explicit-inverse interface
+ * preconditioner.  The captured solve starts GMRES from that inverse's
+ * product and takes no further Krylov steps; its residual check is recorded
+ * too.  stream must not be the null stream.  Nothing is computed at capture. */
+int fp_cat_graph_capture(fp_cat cat, double* const* d_fields, const double* const* d_ghost, int nrhs, void* stream);
+/* Replays the captured solve on the captured buffers (the caller refills
+ * them in between).  info != NULL: waits for it and reports the residual
+ * check; FP_E_NOT_CONVERGED if the preconditioned start missed the tolerance
+ * (fp_cat_solve then iterates further).  info == NULL: returns at once. */
+int fp_cat_graph_launch(fp_cat cat, void* stream, fp_solve_info* info);
+
 /* Adjoint of fp_cat_solve for a loss L(u) (no reference counterpart): given
  * dL/du in d_gu[d] (nrhs fields per domain), writes dL/df into d_gf[d] and
  * dL/dg for the ghost data of every face into d_gghost[4*d+face] (zero for
