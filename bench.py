@@ -37,6 +37,9 @@ def parse():
                     help="target estimated relative error of the FP32+refine variant")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="only the per-kernel timing on the centre subdomain plus an 8 MiB device copy (the "
+                         "calibration kernel of the rocprofv3 counter runs); no composite solve")
     return ap.parse_args()
 
 
@@ -55,6 +58,39 @@ def time_kernel(fn, reps, torch):
     return e0.elapsed_time(e1) / reps
 
 
+def measured_traffic(kernel, n):
+    """HBM bytes per launch of `kernel` on the 1024^2 centre, from the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs recorded under profiles/ (null for
+    another kernel or size)."""
+    path = os.path.join(ROOT, "profiles", "centre1024_dd_traffic.json")
+    if n != 1024 or not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        rec = json.load(fh)
+    kinds = {"ytransform_fwd", "ytransform_inv"} if rec.get("kernel", "").startswith("ytransform") else {rec.get("kernel")}
+    return rec.get("traffic_bytes") if kernel in kinds else None
+
+
+def centre_kernels(fpa, ctx, doms, torch, reps):
+    """Average durations of the centre subdomain's kernels (1024 x 1024 DD
+    for the default cross): forward and inverse y transform (DST-I, packed
+    FFT of length 2N+2) and the x sweep, each launched alone on one field.
+    Algorithmic bytes: one read and one write of the field (16 B per cell)."""
+    nx, ny = doms[0][0], doms[0][1]
+    sub = fpa.PoissonSolver2D(ctx, nx, ny, doms[0][2], doms[0][3], (3, 3, 3, 3))
+    buf = torch.randn(nx, ny, dtype=torch.float64, device=torch.device("cuda:0"))
+    fbytes = 8.0 * nx * ny
+    kern = {}
+    for name, fn, nbytes in [
+        ("ytransform_fwd", lambda: sub.transform(buf), 2 * fbytes),
+        ("ytransform_inv", lambda: sub.transform(buf, inverse=True), 2 * fbytes),
+        ("tridiag", lambda: sub.chasing(buf), 2 * fbytes),
+    ]:
+        t = time_kernel(fn, reps, torch)
+        kern[name] = {"ms": round(t, 5), "bytes": nbytes, "gbs": round(nbytes / (t * 1e-3) / 1e9, 1)}
+    return kern
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -71,6 +107,14 @@ def main():
     n = a.n
     doms, conn, fields, _ = cross(n, seed=1)
     ctx = fpa.Context(0)
+    if a.kernels_only:
+        kern = centre_kernels(fpa, ctx, doms, torch, a.kernel_reps)
+        src = torch.randn(n * n, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        t = time_kernel(lambda: dst.copy_(src), a.kernel_reps, torch)
+        kern["copy_8MiB"] = {"ms": round(t, 5), "bytes": 16.0 * n * n, "gbs": round(16.0 * n * n / (t * 1e-3) / 1e9, 1)}
+        print(json.dumps({"kernels": kern}), flush=True)
+        return
     cells = sum(d[0] * d[1] for d in doms)
     pristine = [torch.from_numpy(f[0]).to(dev) for f in fields]
 
@@ -130,19 +174,8 @@ def main():
     vg = run_graph()
     v32 = run_variant(fpa.FP_F32_REFINE)
 
-    # ---- per-kernel timing on the centre subdomain (the GMRES matvec's solve) ----
-    nx, ny = doms[0][0], doms[0][1]
-    sub = fpa.PoissonSolver2D(ctx, nx, ny, doms[0][2], doms[0][3], (3, 3, 3, 3))
-    buf = torch.randn(nx, ny, dtype=torch.float64, device=dev)
-    fbytes = 8.0 * nx * ny
-    kern = {}
-    for name, fn, nbytes in [
-        ("ytransform_fwd", lambda: sub.transform(buf), 2 * fbytes),
-        ("ytransform_inv", lambda: sub.transform(buf, inverse=True), 2 * fbytes),
-        ("tridiag", lambda: sub.chasing(buf), 2 * fbytes),
-    ]:
-        t = time_kernel(fn, a.kernel_reps, torch)
-        kern[name] = {"ms": round(t, 5), "bytes": nbytes, "gbs": round(nbytes / (t * 1e-3) / 1e9, 1)}
+    # ---- per-kernel timing on the centre subdomain ----
+    kern = centre_kernels(fpa, ctx, doms, torch, a.kernel_reps)
     dom_name = max(kern, key=lambda k: kern[k]["ms"])
     dom = kern[dom_name]
 
@@ -199,7 +232,8 @@ def main():
                                "relative %.0e" % (n, n, n, n // 2, n, n, n // 2, a.m, a.tol),
                    "form": main_key, "cells": cells, "rel_l2_vs_oracle": err[main_key], "variants": variants},
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": dom["gbs"], "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": round(dom["gbs"] / PEAK_HBM_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(dom["gbs"] / PEAK_HBM_GBS, 4),
+                     "traffic": measured_traffic(dom_name, n)},
         "kernels": kern,
         "cpu_baseline": cpu,
     }
