@@ -73,8 +73,9 @@ def measured_traffic(kernel, n):
 
 def centre_kernels(fpa, ctx, doms, torch, reps):
     """Average durations of the centre subdomain's kernels (1024 x 1024 DD
-    for the default cross): forward and inverse y transform (DST-I, packed
-    FFT of length 2N+2) and the x sweep, each launched alone on one field.
+    for the default cross): forward and inverse y transform (DST-I, two rows
+    packed into one complex FFT of length N+1) and the x sweep, each launched
+    alone on one field.
     Algorithmic bytes: one read and one write of the field (16 B per cell)."""
     nx, ny = doms[0][0], doms[0][1]
     sub = fpa.PoissonSolver2D(ctx, nx, ny, doms[0][2], doms[0][3], (3, 3, 3, 3))

@@ -85,7 +85,8 @@ struct TransformPlan {
     int        nrad = 0;
     int        rad[kMaxRadices] = {};
     int        engine = 1;  // 1: single-buffer FFT, 0: two-buffer fallback (prime factors above 47)
-    int        ncs  = 0;  // cos/sin entries of the odd-prime passes
+    int        ncs  = 0;  // cos/sin entries of the odd-prime passes (FP32)
+    DevBuf<double> frag;  // FP64 odd-prime passes: matrix-core operand fragments
     DevBuf<T>  W;         // twiddles: 64 of W^m, then L/64+1 of W^(64m), W = exp(-2 pi i / L), then
                           // (cos, sin)(2 pi m / R) of every odd-prime pass R
     DevBuf<T>  aux;       // 2N: NN phase exp(-i pi m / 2N) or ND (sin, cos) of 2N pi (j+1)/(2N+1)

@@ -7,7 +7,10 @@
 // NN/ND inverse); here two rows a, b of the field form one complex sequence
 // z = ext(a) + i ext(b) of length L, one FFT of length L is done in LDS, and
 // the two rows' coefficients are split out of the spectrum:
-//   DD     : odd extension, L = 2N+2, spectrum purely imaginary per row
+//   DD     : L = N+1 (the half-length DST-I): with x_0 = 0, x_j = f(j-1),
+//            y_j = sin(pi j/L)(x_j + x_{L-j}) + (x_j - x_{L-j})/2 is FFT'd;
+//            even outputs X_2k = -Im F_k, odd outputs X_2k+1 = F_0/2 +
+//            Re(F_1 + .. + F_k) by a block prefix sum in LDS
 //   NN fwd : half-sample even extension, L = 2N, phase exp(-i pi m / 2N)
 //   NN inv : Hermitian extension of the phased coefficients, inverse FFT, L = 2N
 //   DN, ND : L = 2N+1, split by Z(m) +- conj Z(L-m)
@@ -26,7 +29,7 @@
 //     twice the workgroups per CU.  Twiddles: one two-level table lookup per
 //     butterfly, the other powers by multiplication.  Odd primes 7..47 run a
 //     symmetric-pair DFT split over groups of threads (a group is a whole
-//     wavefront, so no divergence), its cosines scalar loads from a table.
+//     wavefront, so no divergence), its cosines broadcast from an LDS table.
 //   k_ytransform_gen (fallback for larger prime factors or lengths whose
 //     passes do not fit the per-thread register budget): two LDS buffers,
 //     any prime through an R-term in-LDS sum per output.
@@ -72,7 +75,8 @@ struct YArgs {
     int          nx, N, L, kind, inverse, nrad;
     int          rad[kMaxRadices];
     const C2<T>* W;    // 64 + L/64 + 1 twiddles, then cos/sin tables of the odd-prime passes
-    int          ncs;  // entries of those tables
+    int          ncs;  // entries of those tables (staged in LDS; FP32 odd-prime passes)
+    const double* frag;  // FP64 odd-prime passes: matrix-core operand fragments of the DFT blocks
     const C2<T>* aux;
     T            c0, c1;
     // fused boundary assembly and scaling of a raw right-hand side on load
@@ -245,31 +249,40 @@ __device__ __forceinline__ void pass_reg_sb(C2<T>* __restrict__ Z, const Tw<T> W
     __syncthreads();
 }
 
+// Lanes given to the butterflies of one output group in the odd-prime pass:
+// whole wavefronts for 33 or more butterflies, else 32 or 16 lanes, so that
+// two or four groups share a wavefront (each cosine read then serves two or
+// four groups instead of one).
+__host__ __device__ __forceinline__ int odd_slots(int nb)
+{
+    return nb <= 16 ? 16 : nb <= 32 ? 32 : (nb + 63) & ~63;
+}
+
 // One Stockham pass of an odd prime radix R (7..47) on the single buffer.
 // With a_r = x_r + x_{R-r}, b_r = x_r - x_{R-r} (twiddled inputs):
 //   X_q = x_0 + A_q -+ i B_q,  X_{R-q} = x_0 + A_q +- i B_q,
 //   A_q = sum_r a_r cos(2 pi q r / R),  B_q = sum_r b_r sin(2 pi q r / R).
 // The H = (R-1)/2 output pairs of a butterfly are split over G groups of kNQ
 // pairs; work item t = (group, butterfly) with the butterflies padded to a
-// multiple of 64, so one wavefront is one group.  The group index is then
-// wavefront-uniform (readfirstlane), and so are the cosines a wavefront
-// needs: a table of (cos, sin)(2 pi q r / R) with r-major rows of G*kNQ
-// values in global memory, read by contiguous scalar loads (scalar cache),
-// not LDS traffic.  Items a thread owns keep their accumulators in registers
-// across the synchronisation.
+// multiple of 64 (or to 16 / 32 lanes for few butterflies, odd_slots), so a
+// wavefront holds one to four groups and reads the cosines of its groups,
+// (cos, sin)(2 pi q r / R) in a table of r-major rows of G*kNQ values staged
+// in LDS with the twiddles, at one to four addresses per read.  Items a
+// thread owns keep their accumulators in registers across the
+// synchronisation.
 template <bool FWD, typename T>
 __device__ __forceinline__ void pass_odd_sb(C2<T>* __restrict__ Z, const Tw<T> W, int L, int Ns, int R,
                                             const C2<T>* __restrict__ csg)
 {
     const int NT = blockDim.x;
     const int H = (R - 1) >> 1, G = (H + kNQ - 1) / kNQ;
-    const int nb = L / R, stride = L / (Ns * R), nbp = (nb + 63) & ~63;
+    const int nb = L / R, stride = L / (Ns * R), nbp = odd_slots(nb);
     T         ax[kIPT][kNQ], ay[kIPT][kNQ], bx[kIPT][kNQ], by[kIPT][kNQ];
     C2<T>     x0[kIPT];
 #pragma unroll
     for (int it = 0; it < kIPT; it++) {
         const int t   = threadIdx.x + it * NT;
-        const int grp = __builtin_amdgcn_readfirstlane(t / nbp);
+        const int grp = nbp >= 64 ? __builtin_amdgcn_readfirstlane(t / nbp) : t / nbp;
         const int jb  = t - grp * nbp;
         if (grp < G && jb < nb) {
             const int   QB = 1 + grp * kNQ;
@@ -286,7 +299,7 @@ __device__ __forceinline__ void pass_odd_sb(C2<T>* __restrict__ Z, const Tw<T> W
             // row r of the table holds (cos, sin)(2 pi q r / R) for q = 1..G*kNQ
             const C2<T>* crow = csg + (QB - 1);
             const int    ldc  = G * kNQ;
-#pragma unroll 2
+#pragma unroll 4
             for (int r = 1; r <= H; r++) {
                 C2<T> x1 = Z[zp<kZpad>(jb + r * nb)], x2 = Z[zp<kZpad>(jb + (R - r) * nb)];
                 if (Ns > 1) {
@@ -332,10 +345,113 @@ __device__ __forceinline__ void pass_odd_sb(C2<T>* __restrict__ Z, const Tw<T> W
     __syncthreads();
 }
 
+// FP64 odd-prime pass on the matrix cores.  With the same a_r, b_r as
+// pass_odd_sb, the H + 1 sums per butterfly and component are two real
+// matrix products,
+//   [S0; A_1 .. A_H] = Cm [a_1 .. a_H]^T,  [B_1 .. B_H] = Sm [b_1 .. b_H]^T,
+// Cm(q, r) = cos(2 pi q r / R) (row 0 all ones: S0 = sum_r a_r), Sm = sin,
+// whose columns are the butterflies' real and imaginary parts (column
+// n = 2 jb + component).  v_mfma_f64_16x16x4f64 tiles them: M = q in tiles of
+// 16 (two tiles cover R <= 61), K = r in steps of 4, N = columns in tiles of
+// 16.  A wavefront owns whole column tiles: per K step each lane loads the
+// x_r, x_{R-r} of its (r, column) (the B operand) and the wavefront issues
+// 2 x (M tiles) products against Cm/Sm fragments read from the plan (the A
+// operand, one double per lane).  The outputs pair the real and imaginary
+// columns of a butterfly (neighbouring lanes) to form X_q and X_{R-q}.
+// Operand layouts of the instruction (gfx950): A(i, k) in lane i + 16 k,
+// B(k, j) in lane j + 16 k, D(i, j) in lane j + 16 (i % 4), element i / 4.
+typedef double d4_t __attribute__((ext_vector_type(4)));
+constexpr int kMaxNTw = 2;  // column tiles per wavefront
+
+__host__ __device__ __forceinline__ int mfma_ksteps(int R) { return ((R - 1) / 2 + 3) / 4; }
+
+template <bool FWD>
+__device__ __forceinline__ void pass_odd_mfma(C2<double>* __restrict__ Z, const Tw<double> W, int L, int Ns, int R,
+                                              const double* __restrict__ frag)
+{
+    const int H = (R - 1) >> 1, MT = (H + 16) >> 4, KT = mfma_ksteps(R);
+    const int nb = L / R, stride = L / (Ns * R);
+    const int NTL  = (2 * nb + 15) >> 4;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int kq = lane >> 4, jn = lane & 15;
+    d4_t       Dc[kMaxNTw][2], Ds[kMaxNTw][2];
+    C2<double> x0v[kMaxNTw];
+#pragma unroll
+    for (int tt = 0; tt < kMaxNTw; tt++) {
+        const int nt = wave + tt * nw;
+        if (nt >= NTL) break;  // uniform over the wavefront
+        const int  n = nt * 16 + jn, jb = n >> 1, comp = n & 1;
+        const bool colok = jb < nb;
+        const int  k     = colok ? jb % Ns : 0;
+        x0v[tt]          = colok ? Z[zp<kZpad>(jb)] : C2<double>{0.0, 0.0};
+#pragma unroll
+        for (int mt = 0; mt < 2; mt++) {
+            Dc[tt][mt] = d4_t{0.0, 0.0, 0.0, 0.0};
+            Ds[tt][mt] = d4_t{0.0, 0.0, 0.0, 0.0};
+        }
+        for (int ks = 0; ks < KT; ks++) {
+            const int r  = 4 * ks + kq + 1;
+            double    av = 0.0, bv = 0.0;
+            if (colok && r <= H) {
+                C2<double> x1 = Z[zp<kZpad>(jb + r * nb)], x2 = Z[zp<kZpad>(jb + (R - r) * nb)];
+                if (Ns > 1) {
+                    x1 = twiddle<FWD>(W, r * k * stride, x1);
+                    x2 = twiddle<FWD>(W, (R - r) * k * stride, x2);
+                }
+                av = comp ? x1.y + x2.y : x1.x + x2.x;
+                bv = comp ? x1.y - x2.y : x1.x - x2.x;
+            }
+#pragma unroll
+            for (int mt = 0; mt < 2; mt++) {
+                if (mt >= MT) break;
+                const double ca = frag[((0 * 2 + mt) * KT + ks) * 64 + lane];
+                const double sa = frag[((1 * 2 + mt) * KT + ks) * 64 + lane];
+                Dc[tt][mt]      = __builtin_amdgcn_mfma_f64_16x16x4f64(ca, av, Dc[tt][mt], 0, 0, 0);
+                Ds[tt][mt]      = __builtin_amdgcn_mfma_f64_16x16x4f64(sa, bv, Ds[tt][mt], 0, 0, 0);
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int tt = 0; tt < kMaxNTw; tt++) {
+        const int nt = wave + tt * nw;
+        if (nt >= NTL) break;
+        const int  n = nt * 16 + jn, jb = n >> 1, comp = n & 1;
+        const bool colok = jb < nb;
+        const int  k     = colok ? jb % Ns : 0;
+        const int  base  = colok ? (jb / Ns) * Ns * R + k : 0;
+        const C2<double> x0 = x0v[tt];
+#pragma unroll
+        for (int mt = 0; mt < 2; mt++) {
+            if (mt >= MT) break;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const double c = Dc[tt][mt][v], sn = Ds[tt][mt][v];
+                const double co = __shfl_xor(c, 1, 64), so = __shfl_xor(sn, 1, 64);
+                const int    q  = 16 * mt + 4 * v + kq;
+                if (!colok || q > H) continue;
+                // (re, im) of A_q and B_q from this lane and its neighbour
+                const double Ar = comp ? co : c, Ai = comp ? c : co;
+                const double Br = comp ? so : sn, Bi = comp ? sn : so;
+                if (q == 0) {
+                    if (comp == 0) Z[zp<kZpad>(base)] = C2<double>{x0.x + Ar, x0.y + Ai};
+                } else if (comp == 0) {
+                    Z[zp<kZpad>(base + q * Ns)] = FWD ? C2<double>{x0.x + Ar + Bi, x0.y + Ai - Br}
+                                                      : C2<double>{x0.x + Ar - Bi, x0.y + Ai + Br};
+                } else {
+                    Z[zp<kZpad>(base + (R - q) * Ns)] = FWD ? C2<double>{x0.x + Ar - Bi, x0.y + Ai + Br}
+                                                            : C2<double>{x0.x + Ar + Bi, x0.y + Ai - Br};
+                }
+            }
+        }
+    }
+    __syncthreads();
+}
+
 template <bool FWD, typename T>
 __device__ __forceinline__ void run_fft_sb(const YArgs<T>& a, const Tw<T> W, C2<T>* Z, const C2<T>* cs)
 {
-    int Ns = 1, co = 0;
+    int Ns = 1, co = 0, fo = 0;
     for (int s = 0; s < a.nrad; s++) {
         const int R = a.rad[s];
         if (R == 2) pass_reg_sb<2, FWD>(Z, W, a.L, Ns);
@@ -343,7 +459,10 @@ __device__ __forceinline__ void run_fft_sb(const YArgs<T>& a, const Tw<T> W, C2<
         else if (R == 4) pass_reg_sb<4, FWD>(Z, W, a.L, Ns);
         else if (R == 5) pass_reg_sb<5, FWD>(Z, W, a.L, Ns);
         else if (R == 8) pass_reg_sb<8, FWD>(Z, W, a.L, Ns);
-        else {
+        else if constexpr (sizeof(T) == 8) {
+            pass_odd_mfma<FWD>(Z, W, a.L, Ns, R, a.frag + fo);
+            fo += 2 * 2 * mfma_ksteps(R) * 64;
+        } else {
             pass_odd_sb<FWD>(Z, W, a.L, Ns, R, cs + co);
             co += ((R - 1) / 2) * (((R - 1) / 2 + kNQ - 1) / kNQ * kNQ);
         }
@@ -486,7 +605,6 @@ __device__ __forceinline__ void load_rows(const YArgs<T>& a, const RowPair<T>& r
     const int kind = a.kind, inv = a.inverse;
     if (kind == T_DD) {
         if (threadIdx.x == 0) A[zp<PAD>(0)] = {T(0), T(0)};
-        if (threadIdx.x == 1) A[zp<PAD>(N + 1)] = {T(0), T(0)};
     } else if (kind == T_NN && inv) {
         if (threadIdx.x == 0) A[zp<PAD>(N)] = {T(0), T(0)};
     } else if (kind != T_NN) {
@@ -511,8 +629,7 @@ __device__ __forceinline__ void load_rows(const YArgs<T>& a, const RowPair<T>& r
                 if (rp.hb) y = assemble_at(a, rp.rhs, rp.ra + 1, n, y);
             }
             if (kind == T_DD) {
-                A[zp<PAD>(n + 1)]     = {x, y};
-                A[zp<PAD>(L - 1 - n)] = {-x, -y};
+                A[zp<PAD>(n + 1)] = {x, y};
             } else if (kind == T_NN && !inv) {
                 A[zp<PAD>(n)]             = {x, y};
                 A[zp<PAD>(2 * N - 1 - n)] = {x, y};
@@ -535,11 +652,57 @@ __device__ __forceinline__ void load_rows(const YArgs<T>& a, const RowPair<T>& r
             }
         }
     }
+    if (kind == T_DD) {
+        // y_j, y_{L-j} from x_j, x_{L-j}; each pair is one thread's
+        __syncthreads();
+        for (int j = 1 + threadIdx.x; 2 * j <= L; j += NT) {
+            const int   jm = L - j;
+            const T     s  = a.aux[j].x;  // sin(pi j / L)
+            const C2<T> xj = A[zp<PAD>(j)], xm = A[zp<PAD>(jm)];
+            if (j == jm) {
+                A[zp<PAD>(j)] = {T(2) * s * xj.x, T(2) * s * xj.y};
+            } else {
+                const C2<T> su = cadd(xj, xm), df = csub(xj, xm);
+                A[zp<PAD>(j)]  = {s * su.x + T(0.5) * df.x, s * su.y + T(0.5) * df.y};
+                A[zp<PAD>(jm)] = {s * su.x - T(0.5) * df.x, s * su.y - T(0.5) * df.y};
+            }
+        }
+    }
+}
+
+// Inclusive prefix sums of S[0..n) over the workgroup: a serial run per
+// thread chunk, a wavefront scan of the chunk totals, the wavefronts'
+// totals (wt, one per wavefront) added in order.  Summation depth is
+// logarithmic in n apart from the chunk (ceil(n / blockDim) terms).
+template <typename T>
+__device__ void block_scan(C2<T>* S, int n, C2<T>* wt)
+{
+    const int NT = blockDim.x, t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int cpt = (n + NT - 1) / NT;
+    const int b0 = min(n, t * cpt), b1 = min(n, b0 + cpt);
+    C2<T>     run = {T(0), T(0)};
+    for (int q = b0; q < b1; q++) {
+        run  = cadd(run, S[q]);
+        S[q] = run;
+    }
+    C2<T> v = run;
+    for (int d = 1; d < 64; d <<= 1) {
+        const C2<T> u = {__shfl_up(v.x, d, 64), __shfl_up(v.y, d, 64)};
+        if (lane >= d) v = cadd(v, u);
+    }
+    C2<T> ex = {__shfl_up(v.x, 1, 64), __shfl_up(v.y, 1, 64)};
+    if (lane == 0) ex = {T(0), T(0)};
+    if (lane == 63) wt[wave] = v;
+    __syncthreads();
+    for (int w = 0; w < wave; w++) ex = cadd(wt[w], ex);
+    for (int q = b0; q < b1; q++) S[q] = cadd(S[q], ex);
+    __syncthreads();
 }
 
 // Splits the two rows' coefficients out of the spectrum Z and stores them.
 template <int PAD, typename T>
-__device__ __forceinline__ void store_rows(const YArgs<T>& a, const RowPair<T>& rp, const C2<T>* Z, const T* y0)
+__device__ __forceinline__ void store_rows(const YArgs<T>& a, const RowPair<T>& rp, const C2<T>* Z, const T* y0,
+                                           C2<T>* S, C2<T>* wt)
 {
     const int  NT = blockDim.x;
     const int  N = a.N, L = a.L;
@@ -549,6 +712,37 @@ __device__ __forceinline__ void store_rows(const YArgs<T>& a, const RowPair<T>& 
         if (d64) static_cast<double*>(a.dst)[q] = (double)v;
         else static_cast<T*>(a.dst)[q] = v;
     };
+    if (kind == T_DD) {
+        // rows a, b: F_a(q) = (Z_q + conj Z_{L-q}) / 2, F_b(q) = (Z_q - conj Z_{L-q}) / 2i
+        const int Ko = (N + 1) / 2;  // odd outputs X_1, X_3, .. X_{2Ko-1}
+        for (int q = threadIdx.x; q < Ko; q += NT) {
+            const C2<T> zq = Z[zp<PAD>(q)];
+            if (q == 0) S[0] = {T(0.5) * zq.x, T(0.5) * zq.y};
+            else {
+                const C2<T> zm = Z[zp<PAD>(L - q)];
+                S[q]           = {T(0.5) * (zq.x + zm.x), T(0.5) * (zq.y + zm.y)};  // (Re F_a, Re F_b)
+            }
+        }
+        __syncthreads();
+        block_scan(S, Ko, wt);
+        for (int base = 0; base < N; base += VPT * NT) {
+#pragma unroll
+            for (int k = 0; k < VPT; k++) {
+                const int j = base + k * NT + threadIdx.x;
+                if (j >= N) break;
+                C2<T> X;
+                if (j & 1) {  // X_{j+1}, even: -Im F(k), k = (j+1)/2
+                    const int   kk = (j + 1) >> 1;
+                    const C2<T> zk = Z[zp<PAD>(kk)], zm = Z[zp<PAD>(L - kk)];
+                    X              = {T(0.5) * (zm.y - zk.y), T(0.5) * (zk.x - zm.x)};
+                } else
+                    X = S[j >> 1];
+                st_dst(rp.oa + j, a.c0 * X.x);
+                if (rp.hb) st_dst(rp.oa + N + j, a.c0 * X.y);
+            }
+        }
+        return;
+    }
     for (int base = 0; base < N; base += VPT * NT) {
         C2<T> sc[VPT];
         if (kind == T_NN && !inv) {
@@ -569,11 +763,7 @@ __device__ __forceinline__ void store_rows(const YArgs<T>& a, const RowPair<T>& 
             const int j = base + k * NT + threadIdx.x;
             if (j >= N) break;
             T oa, ob;
-            if (kind == T_DD) {
-                const C2<T> z = Z[zp<PAD>(j + 1)];
-                oa            = -a.c0 * z.y;
-                ob            = a.c0 * z.x;
-            } else if (kind == T_NN && !inv) {
+            if (kind == T_NN && !inv) {
                 const int   m = N - 1 - j;
                 const C2<T> w = cmul(Z[zp<PAD>(m)], sc[k]);  // 2 C_a(m) + 2i C_b(m)
                 const T     s = T(0.5) * ((j == N - 1) ? a.c1 : a.c0);
@@ -632,16 +822,17 @@ __global__ __launch_bounds__(kMaxNT) void k_ytransform_sb(YBatch<T> bt)
     C2<T>*    tlo = Z + zp<kZpad>(a.L) + 1;  // 64 twiddles W^m
     const int nhi = a.L / 64 + 1;
     C2<T>*    thi = tlo + 64;           // L/64 + 1 twiddles W^(64m)
-    T*        y0  = reinterpret_cast<T*>(thi + nhi);  // NN inverse: y_0 of rows a, b
-    const C2<T>* cs = a.W + 64 + nhi;   // cos/sin tables of the odd-prime passes (global)
-    for (int q = threadIdx.x; q < 64 + nhi; q += NT) tlo[q] = a.W[q];
+    C2<T>*    cs  = thi + nhi;          // cos/sin tables of the odd-prime passes
+    T*        y0  = reinterpret_cast<T*>(cs + a.ncs);  // NN inverse: y_0 of rows a, b
+    C2<T>*    S   = reinterpret_cast<C2<T>*>(y0 + 4);  // DD: prefix sums (N/2 + 1), then 8 wavefront totals
+    for (int q = threadIdx.x; q < 64 + nhi + a.ncs; q += NT) tlo[q] = a.W[q];
     const Tw<T> W{tlo, thi};
 
     load_rows<kZpad>(a, rp, Z, y0);
     __syncthreads();
     if (!(a.kind == T_NN && a.inverse)) run_fft_sb<true>(a, W, Z, cs);
     else run_fft_sb<false>(a, W, Z, cs);
-    store_rows<kZpad>(a, rp, Z, y0);
+    store_rows<kZpad>(a, rp, Z, y0, S, S + a.N / 2 + 1);
 }
 
 template <typename T>
@@ -659,6 +850,7 @@ __global__ __launch_bounds__(kMaxNT) void k_ytransform_gen(YBatch<T> bt)
     C2<T>*    thi = tlo + 64;
     const int nhi = a.L / 64 + 1;
     T*        y0  = reinterpret_cast<T*>(thi + nhi);
+    C2<T>*    S   = reinterpret_cast<C2<T>*>(y0 + 4);
     for (int q = threadIdx.x; q < 64 + nhi; q += NT) tlo[q] = a.W[q];
     const Tw<T> W{tlo, thi};
 
@@ -666,7 +858,7 @@ __global__ __launch_bounds__(kMaxNT) void k_ytransform_gen(YBatch<T> bt)
     __syncthreads();
     const bool   fwd = !(a.kind == T_NN && a.inverse);
     const C2<T>* Z   = fwd ? run_fft_gen<true>(a, W, A, B) : run_fft_gen<false>(a, W, A, B);
-    store_rows<0>(a, rp, Z, y0);
+    store_rows<0>(a, rp, Z, y0, S, S + a.N / 2 + 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -714,15 +906,18 @@ static int sb_bpt_rt(int R)
         default: return 0;
     }
 }
-static bool sb_fits(const int* rad, int nrad, int L, int NT)
+static bool sb_fits(const int* rad, int nrad, int L, int NT, bool mfma)
 {
     for (int s = 0; s < nrad; s++) {
         const int R = rad[s], nb = L / R;
         if (R <= 8 && R != 7) {
             if ((nb + NT - 1) / NT > sb_bpt_rt(R)) return false;
+        } else if (R > kMaxOddPrime) {
+            return false;
+        } else if (mfma) {
+            if ((2 * nb + 15) / 16 > kMaxNTw * (NT / 64)) return false;
         } else {
-            if (R > kMaxOddPrime) return false;
-            const int H = (R - 1) / 2, G = (H + kNQ - 1) / kNQ, nbp = (nb + 63) / 64 * 64;
+            const int H = (R - 1) / 2, G = (H + kNQ - 1) / kNQ, nbp = odd_slots(nb);
             if ((nbp * G + NT - 1) / NT > kIPT) return false;
         }
     }
@@ -734,8 +929,9 @@ static size_t lds_bytes(const TransformPlan<T>& tp)
 {
     const size_t nhi = tp.L / 64 + 1;
     const size_t zl  = (size_t)tp.L + (kZpad ? ((size_t)tp.L >> kZpad) : 0) + 1;
-    if (tp.engine == 1) return (zl + 64 + nhi) * sizeof(C2<T>) + 4 * sizeof(T);
-    return (2 * (size_t)tp.L + 64 + nhi) * sizeof(C2<T>) + 4 * sizeof(T);
+    const size_t sc  = tp.kind == T_DD ? (size_t)tp.N / 2 + 1 + 8 : 0;  // prefix sums + wavefront totals
+    if (tp.engine == 1) return (zl + 64 + nhi + tp.ncs + sc) * sizeof(C2<T>) + 4 * sizeof(T);
+    return (2 * (size_t)tp.L + 64 + nhi + sc) * sizeof(C2<T>) + 4 * sizeof(T);
 }
 
 template <typename T>
@@ -743,7 +939,7 @@ void TransformPlan<T>::build(int k, int n)
 {
     kind = k;
     N    = n;
-    if (kind == T_DD) L = 2 * N + 2;
+    if (kind == T_DD) L = N + 1;
     else if (kind == T_NN) L = 2 * N;
     else L = 2 * N + 1;
     factor_radices(L, rad, nrad);
@@ -764,7 +960,28 @@ void TransformPlan<T>::build(int k, int n)
             }
         ncs += H * QW;
     }
-    std::vector<T> w(2 * (size_t)(64 + nhi + ncs)), x(2 * (size_t)N, T(0));
+    // FP64: matrix-core fragments [cos/sin][M tile][K step][lane] of every odd-prime pass
+    std::vector<double> fr;
+    if (sizeof(T) == 8) {
+        ncs = 0;
+        cs.clear();
+        for (int s = 0; s < nrad; s++) {
+            const int R = rad[s];
+            if (R <= 8 && R != 7) continue;
+            const int H = (R - 1) / 2, KT = mfma_ksteps(R);
+            for (int f = 0; f < 2; f++)
+                for (int mt = 0; mt < 2; mt++)
+                    for (int ks = 0; ks < KT; ks++)
+                        for (int l = 0; l < 64; l++) {
+                            const int         q = 16 * mt + (l & 15), r = 4 * ks + (l >> 4) + 1;
+                            const long double ang = 2.0L * PI * (long double)(((long long)q * r) % R) / R;
+                            long double       v = 0.0L;
+                            if (q <= H && r <= H) v = f == 0 ? (q == 0 ? 1.0L : cosl(ang)) : (q == 0 ? 0.0L : sinl(ang));
+                            fr.push_back((double)v);
+                        }
+        }
+    }
+    std::vector<T> w(2 * (size_t)(64 + nhi + ncs)), x(2 * (size_t)(N + 1), T(0));
     for (int q = 0; q < 64 + nhi; q++) {
         const long long   t   = q < 64 ? q : 64LL * (q - 64);
         const long double ang = -2.0L * PI * (long double)(t % L) / (long double)L;
@@ -792,14 +1009,16 @@ void TransformPlan<T>::build(int k, int n)
     } else if (kind == T_DN) {
         c0 = (T)std::sqrt(4.0 / (2.0 * N + 1.0));
     } else {
-        c0 = (T)(std::sqrt(2.0 / (N + 1.0)) * 0.5);
+        // DD: sin(pi j / L), j < L, and the orthonormal scale sqrt(2 / (N+1))
+        for (int j = 0; j < L; j++) x[2 * j] = (T)sinl(PI * (long double)j / (long double)L);
+        c0 = (T)std::sqrt(2.0 / (N + 1.0));
     }
     // engine: single buffer at 256 threads when it fits (the tuner may change
     // the thread count among the sizes that fit), else the two-buffer fallback
     engine = 0;
     nt     = 256;
     for (int t : {256, 512, 128})
-        if (sb_fits(rad, nrad, L, t)) {
+        if (sb_fits(rad, nrad, L, t, sizeof(T) == 8)) {
             engine = 1;
             nt     = t;
             break;
@@ -812,7 +1031,9 @@ void TransformPlan<T>::build(int k, int n)
         throw Error(FP_E_INVALID, "transform length exceeds the LDS of one workgroup");
     W.alloc(w.size());
     W.upload(w.data(), w.size());
-    aux.alloc(2 * (size_t)N);
+    frag.alloc(fr.size());
+    if (!fr.empty()) frag.upload(fr.data(), fr.size());
+    aux.alloc(x.size());
     aux.upload(x.data(), x.size());
 }
 
@@ -838,7 +1059,8 @@ static YArgs<T> yargs(const TransformJob<T>& j, int inverse)
     a.nrad    = tp.nrad;
     for (int i = 0; i < tp.nrad; i++) a.rad[i] = tp.rad[i];
     a.W   = reinterpret_cast<const C2<T>*>(tp.W.p);
-    a.ncs = tp.engine == 1 ? tp.ncs : 0;
+    a.ncs  = tp.engine == 1 ? tp.ncs : 0;
+    a.frag = tp.frag.p;
     a.aux = reinterpret_cast<const C2<T>*>(tp.aux.p);
     a.c0  = tp.c0;
     a.c1  = tp.c1;
@@ -926,7 +1148,7 @@ void tune_transform(TransformPlan<T>& tp, int nx, bool measure)
 {
     std::vector<int> cand;
     for (int t : {128, 256, 512})
-        if (tp.engine == 0 ? t == 256 : sb_fits(tp.rad, tp.nrad, tp.L, t)) cand.push_back(t);
+        if (tp.engine == 0 ? t == 256 : sb_fits(tp.rad, tp.nrad, tp.L, t, sizeof(T) == 8)) cand.push_back(t);
     if (!measure || cand.size() < 2 || (long long)nx * tp.N < (1LL << 16)) return;
     const std::array<int, 4> key{tp.kind, tp.N, nx, (int)sizeof(T)};
     {
