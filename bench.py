@@ -37,6 +37,10 @@ def parse():
                     help="target estimated relative error of the FP32+refine variant")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--extras", action="store_true",
+                    help="measure the other workloads instead (cross x 64 right-hand sides, 512 batched uneven "
+                         "problems in FP64 and FP32, 4096^2 FP64 vs FP32+refine, adjoint fwd+bwd ratio); "
+                         "prints one JSON line")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the per-kernel timing on the centre subdomain plus an 8 MiB device copy (the "
                          "calibration kernel of the rocprofv3 counter runs); no composite solve")
@@ -92,6 +96,116 @@ def centre_kernels(fpa, ctx, doms, torch, reps):
     return kern
 
 
+def timed(fn, torch, reps=3):
+    """Mean wall time (s) of fn() over reps runs after one warm-up."""
+    fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / reps
+
+
+def extras(fpa, ctx, torch, np, n):
+    """The other workloads of the solve path, one measurement each."""
+    from focalors_poisson_amd import cases
+    dev = torch.device("cuda:0")
+    out = {}
+
+    # cross with 64 right-hand sides (seeds 100..163), FP64, one call
+    doms, conn, fields, _ = cases.cross(n, seed=100, nrhs=64)
+    cat = fpa.ConcatPoissonSolver2D(ctx, doms, conn, m=200, tol=1e-12, tol_is_relative=True, max_iter=100)
+    pristine = [torch.from_numpy(f).to(dev) for f in fields]
+    work = [torch.empty_like(p) for p in pristine]
+    info = {}
+
+    def multi():
+        for w, p in zip(work, pristine):
+            w.copy_(p)
+        info.update(cat.solve(work))
+
+    t = timed(multi, torch)
+    one = [p[:1].clone() for p in pristine]
+    cat.solve(one)
+    torch.cuda.synchronize()
+    same = max(float((w[:1] - o).norm() / o.norm()) for w, o in zip(work, one))
+    out["cross_x64_rhs"] = {"rhs_solves_per_s": round(64 / t, 1), "ms_per_call": round(t * 1e3, 3),
+                            "gmres_rel_residual": info.get("rel_residual"),
+                            "rel_diff_rhs0_vs_single_rhs_solve": same}
+    del cat, pristine, work, one, fields
+
+    # 512 independent Dirichlet problems, sides log-uniform in [64, 1024] (seed 2), RHS U(-1, 1) (seed 3)
+    sizes = cases.batched_sizes(512, seed=2)
+    rng = np.random.default_rng(3)
+    t0 = time.perf_counter()
+    subs = [fpa.PoissonSolver2D(ctx, nx, ny, 1.0 / max(nx, ny), 1.0 / max(nx, ny), (0, 0, 0, 0)) for nx, ny in sizes]
+    t_plan = time.perf_counter() - t0
+    rhs = [torch.from_numpy(rng.uniform(-1, 1, (nx, ny))).to(dev) for nx, ny in sizes]
+    cells = sum(nx * ny for nx, ny in sizes)
+    for prec, dt in (("f64", torch.float64), ("f32", torch.float32)):
+        src = [r.to(dt) for r in rhs]
+        wk = [torch.empty_like(r) for r in src]
+
+        def batch():
+            for w, r in zip(wk, src):
+                w.copy_(r)
+            fpa.solve_batched(ctx, subs, wk)
+
+        t = timed(batch, torch)
+        out["batched_512_" + prec] = {"problems_per_s": round(512 / t, 1), "ms_per_batch": round(t * 1e3, 3),
+                                      "cells": cells, "plan_build_s": round(t_plan, 2)}
+    del subs, rhs
+
+    # 4096^2 Dirichlet, multiscale RHS (seed 4): FP64 vs FP32 + FP64 refinement to 1e-10
+    N4 = 4096
+    h = 1.0 / N4
+    f = cases.multiscale_rhs(N4, N4, h, h, 0.0, 0.0, seed=4) * h * h
+    sub = fpa.PoissonSolver2D(ctx, N4, N4, h, h, (0, 0, 0, 0))
+    src = torch.from_numpy(f).to(dev)
+    u64 = torch.empty_like(src)
+
+    def s64():
+        u64.copy_(src)
+        sub.solve(u64)
+
+    t64 = timed(s64, torch)
+    u32 = torch.empty_like(src)
+    rinfo = {}
+
+    def s32():
+        u32.copy_(src)
+        rinfo.update(sub.solve_refine(u32, tol=1e-10, max_steps=10))
+
+    t32 = timed(s32, torch)
+    out["dirichlet_4096"] = {"f64_ms": round(t64 * 1e3, 3), "f32_refine_ms": round(t32 * 1e3, 3),
+                             "refine_steps": rinfo.get("refine_steps"),
+                             "refine_last_correction": rinfo.get("refine_residual"),
+                             "rel_l2_f32_refine_vs_f64": float((u32 - u64).norm() / u64.norm())}
+    del sub, src, u64, u32
+
+    # adjoint on the cross, batch of 32: (forward + backward) / forward time
+    doms, conn, fields, _ = cases.cross(n, seed=1, nrhs=32)
+    cat = fpa.ConcatPoissonSolver2D(ctx, doms, conn, m=200, tol=1e-12, tol_is_relative=True, max_iter=100)
+    fs = [torch.from_numpy(f).to(dev) for f in fields]
+    wk = [torch.empty_like(x) for x in fs]
+
+    def fwd():
+        for w, x in zip(wk, fs):
+            w.copy_(x)
+        cat.solve(wk)
+
+    def fwd_bwd():
+        fwd()
+        cat.vjp(wk)
+
+    tf = timed(fwd, torch)
+    tfb = timed(fwd_bwd, torch)
+    out["adjoint_cross_x32"] = {"forward_ms": round(tf * 1e3, 3), "forward_backward_ms": round(tfb * 1e3, 3),
+                                "ratio": round(tfb / tf, 3)}
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -108,6 +222,9 @@ def main():
     n = a.n
     doms, conn, fields, _ = cross(n, seed=1)
     ctx = fpa.Context(0)
+    if a.extras:
+        print(json.dumps({"extras": extras(fpa, ctx, torch, np, n)}), flush=True)
+        return
     if a.kernels_only:
         kern = centre_kernels(fpa, ctx, doms, torch, a.kernel_reps)
         src = torch.randn(n * n, dtype=torch.float64, device=dev)

@@ -77,6 +77,7 @@ struct YArgs {
     const C2<T>* W;    // 64 + L/64 + 1 twiddles, then cos/sin tables of the odd-prime passes
     int          ncs;  // entries of those tables (staged in LDS; FP32 odd-prime passes)
     const double* frag;  // FP64 odd-prime passes: matrix-core operand fragments of the DFT blocks
+    int           nfrag; // their count (doubles), staged in LDS
     const C2<T>* aux;
     T            c0, c1;
     // fused boundary assembly and scaling of a raw right-hand side on load
@@ -355,8 +356,8 @@ __device__ __forceinline__ void pass_odd_sb(C2<T>* __restrict__ Z, const Tw<T> W
 // 16 (two tiles cover R <= 61), K = r in steps of 4, N = columns in tiles of
 // 16.  A wavefront owns whole column tiles: per K step each lane loads the
 // x_r, x_{R-r} of its (r, column) (the B operand) and the wavefront issues
-// 2 x (M tiles) products against Cm/Sm fragments read from the plan (the A
-// operand, one double per lane).  The outputs pair the real and imaginary
+// 2 x (M tiles) products against Cm/Sm fragments staged in LDS at kernel
+// start (the A operand, one double per lane).  The outputs pair the real and imaginary
 // columns of a butterfly (neighbouring lanes) to form X_q and X_{R-q}.
 // Operand layouts of the instruction (gfx950): A(i, k) in lane i + 16 k,
 // B(k, j) in lane j + 16 k, D(i, j) in lane j + 16 (i % 4), element i / 4.
@@ -449,7 +450,8 @@ __device__ __forceinline__ void pass_odd_mfma(C2<double>* __restrict__ Z, const 
 }
 
 template <bool FWD, typename T>
-__device__ __forceinline__ void run_fft_sb(const YArgs<T>& a, const Tw<T> W, C2<T>* Z, const C2<T>* cs)
+__device__ __forceinline__ void run_fft_sb(const YArgs<T>& a, const Tw<T> W, C2<T>* Z, const C2<T>* cs,
+                                           const double* fragL)
 {
     int Ns = 1, co = 0, fo = 0;
     for (int s = 0; s < a.nrad; s++) {
@@ -460,7 +462,7 @@ __device__ __forceinline__ void run_fft_sb(const YArgs<T>& a, const Tw<T> W, C2<
         else if (R == 5) pass_reg_sb<5, FWD>(Z, W, a.L, Ns);
         else if (R == 8) pass_reg_sb<8, FWD>(Z, W, a.L, Ns);
         else if constexpr (sizeof(T) == 8) {
-            pass_odd_mfma<FWD>(Z, W, a.L, Ns, R, a.frag + fo);
+            pass_odd_mfma<FWD>(Z, W, a.L, Ns, R, fragL + fo);
             fo += 2 * 2 * mfma_ksteps(R) * 64;
         } else {
             pass_odd_sb<FWD>(Z, W, a.L, Ns, R, cs + co);
@@ -822,18 +824,22 @@ __global__ __launch_bounds__(kMaxNT) void k_ytransform_sb(YBatch<T> bt)
     C2<T>*    tlo = Z + zp<kZpad>(a.L) + 1;  // 64 twiddles W^m
     const int nhi = a.L / 64 + 1;
     C2<T>*    thi = tlo + 64;           // L/64 + 1 twiddles W^(64m)
-    C2<T>*    cs  = thi + nhi;          // cos/sin tables of the odd-prime passes
-    T*        y0  = reinterpret_cast<T*>(cs + a.ncs);  // NN inverse: y_0 of rows a, b
+    C2<T>*    cs  = thi + nhi;          // cos/sin tables of the odd-prime passes (FP32)
+    double*   fr  = reinterpret_cast<double*>(cs + a.ncs);  // matrix-core fragments (FP64)
+    T*        y0  = reinterpret_cast<T*>(fr + a.nfrag);     // NN inverse: y_0 of rows a, b
     C2<T>*    S   = reinterpret_cast<C2<T>*>(y0 + 4);  // DD: prefix sums (N/2 + 1), then 8 wavefront totals
     for (int q = threadIdx.x; q < 64 + nhi + a.ncs; q += NT) tlo[q] = a.W[q];
+    for (int q = threadIdx.x; q < a.nfrag; q += NT) fr[q] = a.frag[q];
     const Tw<T> W{tlo, thi};
 
     load_rows<kZpad>(a, rp, Z, y0);
     __syncthreads();
-    if (!(a.kind == T_NN && a.inverse)) run_fft_sb<true>(a, W, Z, cs);
-    else run_fft_sb<false>(a, W, Z, cs);
+    if (!(a.kind == T_NN && a.inverse)) run_fft_sb<true>(a, W, Z, cs, fr);
+    else run_fft_sb<false>(a, W, Z, cs, fr);
     store_rows<kZpad>(a, rp, Z, y0, S, S + a.N / 2 + 1);
 }
+
+__host__ __device__ __forceinline__ bool gen_scan_in_buffer(int L, int N) { return L >= N / 2 + 1 + 8; }
 
 template <typename T>
 __global__ __launch_bounds__(kMaxNT) void k_ytransform_gen(YBatch<T> bt)
@@ -858,7 +864,9 @@ __global__ __launch_bounds__(kMaxNT) void k_ytransform_gen(YBatch<T> bt)
     __syncthreads();
     const bool   fwd = !(a.kind == T_NN && a.inverse);
     const C2<T>* Z   = fwd ? run_fft_gen<true>(a, W, A, B) : run_fft_gen<false>(a, W, A, B);
-    store_rows<0>(a, rp, Z, y0, S, S + a.N / 2 + 1);
+    // DD prefix sums: in the buffer the FFT did not end in, when it is long enough
+    C2<T>* Sd = gen_scan_in_buffer(a.L, a.N) ? (Z == A ? B : A) : S;
+    store_rows<0>(a, rp, Z, y0, Sd, Sd + a.N / 2 + 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -930,8 +938,10 @@ static size_t lds_bytes(const TransformPlan<T>& tp)
     const size_t nhi = tp.L / 64 + 1;
     const size_t zl  = (size_t)tp.L + (kZpad ? ((size_t)tp.L >> kZpad) : 0) + 1;
     const size_t sc  = tp.kind == T_DD ? (size_t)tp.N / 2 + 1 + 8 : 0;  // prefix sums + wavefront totals
-    if (tp.engine == 1) return (zl + 64 + nhi + tp.ncs + sc) * sizeof(C2<T>) + 4 * sizeof(T);
-    return (2 * (size_t)tp.L + 64 + nhi + sc) * sizeof(C2<T>) + 4 * sizeof(T);
+    if (tp.engine == 1)
+        return (zl + 64 + nhi + tp.ncs + sc) * sizeof(C2<T>) + tp.frag.n * sizeof(double) + 4 * sizeof(T);
+    const size_t scg = gen_scan_in_buffer(tp.L, tp.N) ? 0 : sc;
+    return (2 * (size_t)tp.L + 64 + nhi + scg) * sizeof(C2<T>) + 4 * sizeof(T);
 }
 
 template <typename T>
@@ -1059,8 +1069,9 @@ static YArgs<T> yargs(const TransformJob<T>& j, int inverse)
     a.nrad    = tp.nrad;
     for (int i = 0; i < tp.nrad; i++) a.rad[i] = tp.rad[i];
     a.W   = reinterpret_cast<const C2<T>*>(tp.W.p);
-    a.ncs  = tp.engine == 1 ? tp.ncs : 0;
-    a.frag = tp.frag.p;
+    a.ncs   = tp.engine == 1 ? tp.ncs : 0;
+    a.frag  = tp.frag.p;
+    a.nfrag = tp.engine == 1 ? (int)tp.frag.n : 0;
     a.aux = reinterpret_cast<const C2<T>*>(tp.aux.p);
     a.c0  = tp.c0;
     a.c1  = tp.c1;

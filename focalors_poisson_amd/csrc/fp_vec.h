@@ -46,47 +46,65 @@ void launch_schur_lines(const SchurLinesArgs& a, int nrhs, hipStream_t s);
 // (half = 0 / 1: only the column-0 / column-(ny-1) sums, nx values)
 void launch_colsum(const double* cp, int nblk, int nx, double* yl, long long ldy, int nrhs, hipStream_t s,
                    int half = -1);
-struct ColsumJob {
-    const double* cp;
-    int           nblk, nx, half;
-    double*       yl;
-    long long     ldy;
+template <typename T>
+struct ColsumJobT {
+    const T*  cp;
+    int       nblk, nx, half;
+    T*        yl;
+    long long ldy;
 };
-struct ColsumBatch {
-    int       n = 0;
-    ColsumJob j[8];
+template <typename T>
+struct ColsumBatchT {
+    int           n = 0;
+    ColsumJobT<T> j[8];
 };
-void launch_colsum_batch(const ColsumBatch& b, int nrhs, hipStream_t s);
-struct GemvJob {
-    const double* A;  // n x n row-major
-    int           n;
-    const double* x;
-    long long     ldx;
-    double*       y;
-    long long     ldy;
+using ColsumJob   = ColsumJobT<double>;
+using ColsumBatch = ColsumBatchT<double>;
+template <typename T>
+void launch_colsum_batch(const ColsumBatchT<T>& b, int nrhs, hipStream_t s);
+template <typename T>
+struct GemvJobT {
+    const T*  A;  // n x n row-major
+    int       n;
+    const T*  x;
+    long long ldx;
+    T*        y;
+    long long ldy;
 };
-struct GemvBatch {
-    int     n = 0;
-    GemvJob j[8];
+template <typename T>
+struct GemvBatchT {
+    int         n = 0;
+    GemvJobT<T> j[8];
 };
-void launch_gemv_batch(const GemvBatch& b, int nrhs, hipStream_t s);
+using GemvJob   = GemvJobT<double>;
+using GemvBatch = GemvBatchT<double>;
+template <typename T>
+void launch_gemv_batch(const GemvBatchT<T>& b, int nrhs, hipStream_t s);
 // mode 0: out = v - t (v == nullptr: the unit vector e_{k0 + r} of right-hand
 // side r); mode 1: out = t; t from the x rows xu and the y-lines yl (2 nx per rhs)
-void launch_lines_assemble(const double* v, double* out, int K, const int* kface, const int* kpos, const double* xu,
-                           int ny, const double* yl, int nx, int nrhs, hipStream_t s, int k0 = 0, int mode = 0);
-struct LineCopy {
-    const double* src;
-    long long     lds;
-    int           stride, len;
-    double*       dst;
-    long long     ldd;
-    double        coef;
+template <typename TL>
+void launch_lines_assemble(const double* v, double* out, int K, const int* kface, const int* kpos, const TL* xu,
+                           int ny, const TL* yl, int nx, int nrhs, hipStream_t s, int k0 = 0, int mode = 0);
+template <typename T>
+struct LineCopyT {
+    const T*  src;
+    long long lds;
+    int       stride, len;
+    T*        dst;
+    long long ldd;
+    T         coef;
 };
-struct LineCopyBatch {
-    int      n = 0;
-    LineCopy c[8];
+template <typename T>
+struct LineCopyBatchT {
+    int          n = 0;
+    LineCopyT<T> c[8];
 };
-void launch_line_copy(const LineCopyBatch& b, int nrhs, hipStream_t s);
+using LineCopy      = LineCopyT<double>;
+using LineCopyBatch = LineCopyBatchT<double>;
+template <typename T>
+void launch_line_copy(const LineCopyBatchT<T>& b, int nrhs, hipStream_t s);
+// dst[q] = (float)src[q], q < n
+void launch_to_f32(const double* src, float* dst, long long n, hipStream_t s);
 void launch_lines_from_units(const SchurLinesArgs& a, int k0, int cnt, hipStream_t s);
 void launch_dense_gemv(const double* A, int n, const double* x, double* y, double alpha, double beta, int nrhs,
                        hipStream_t s);

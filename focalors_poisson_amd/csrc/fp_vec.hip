@@ -3,11 +3,57 @@
 // interface unknowns.
 #include "fp_vec.h"
 
+#include <rocblas/rocblas.h>
+
+#include <mutex>
+
 namespace fp {
+
+// ---- dense products of many right-hand sides: rocBLAS GEMM ----------------
+// With several right-hand sides the dense products (explicit inverse, Schur
+// blocks, basis maps) are matrix-matrix products; a GEMM reads each matrix
+// once instead of once per right-hand side.  One rocBLAS handle per device.
+constexpr int kGemmMinRhs = 4;
+
+static rocblas_handle blas_handle()
+{
+    static std::mutex     mu;
+    static rocblas_handle h[64] = {};
+    int                   dev   = 0;
+    FP_HIP(hipGetDevice(&dev));
+    FP_REQUIRE(dev >= 0 && dev < 64, "device index out of range");
+    std::lock_guard<std::mutex> lk(mu);
+    if (!h[dev] && rocblas_create_handle(&h[dev]) != rocblas_status_success)
+        throw Error(FP_E_HIP, "rocblas_create_handle failed");
+    return h[dev];
+}
+
+// Y[r] = alpha A X[r] + beta Y[r] for r < nrhs: A is m x k row-major (lda),
+// X[r] has k values (ldx apart), Y[r] m values (ldy apart).  In rocBLAS's
+// column-major terms: Y = alpha op(A^T) X + beta Y.
+static void dense_apply(const double* A, int m, int k, int lda, const double* X, long long ldx, double* Y,
+                        long long ldy, int nrhs, double alpha, double beta, hipStream_t s)
+{
+    rocblas_handle h = blas_handle();
+    if (rocblas_set_stream(h, s) != rocblas_status_success) throw Error(FP_E_HIP, "rocblas_set_stream failed");
+    const rocblas_status st = rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, m, nrhs, k,
+                                            &alpha, A, lda, X, (rocblas_int)ldx, &beta, Y, (rocblas_int)ldy);
+    if (st != rocblas_status_success) throw Error(FP_E_HIP, std::string("rocblas_dgemm: ") + rocblas_status_to_string(st));
+}
+static void dense_apply(const float* A, int m, int k, int lda, const float* X, long long ldx, float* Y, long long ldy,
+                        int nrhs, float alpha, float beta, hipStream_t s)
+{
+    rocblas_handle h = blas_handle();
+    if (rocblas_set_stream(h, s) != rocblas_status_success) throw Error(FP_E_HIP, "rocblas_set_stream failed");
+    const rocblas_status st = rocblas_sgemm(h, rocblas_operation_transpose, rocblas_operation_none, m, nrhs, k,
+                                            &alpha, A, lda, X, (rocblas_int)ldx, &beta, Y, (rocblas_int)ldy);
+    if (st != rocblas_status_success) throw Error(FP_E_HIP, std::string("rocblas_sgemm: ") + rocblas_status_to_string(st));
+}
 
 constexpr int VB = 256;  // threads per block of the vector kernels
 
-__device__ __forceinline__ double wave_reduce_sum(double v)
+template <typename T>
+__device__ __forceinline__ T wave_reduce_sum(T v)
 {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;
@@ -212,6 +258,13 @@ __global__ __launch_bounds__(256) void k_schur_lines(SchurLinesArgs a)
 void launch_schur_lines(const SchurLinesArgs& a, int nrhs, hipStream_t s)
 {
     if (a.total_rows <= 0) return;
+    if (nrhs >= kGemmMinRhs) {
+        for (int c = 0; c < a.nch; c++) {
+            const SchurChild& C = a.ch[c];
+            dense_apply(C.S, C.cn, C.cn, C.cn, a.v + C.voff, a.ldv, a.lb + C.lboff, a.ldl, nrhs, 1.0, 0.0, s);
+        }
+        return;
+    }
     hipLaunchKernelGGL(k_schur_lines, dim3((a.total_rows + 3) / 4, nrhs), dim3(256), 0, s, a);
     FP_HIP(hipGetLastError());
 }
@@ -222,27 +275,31 @@ void launch_schur_lines(const SchurLinesArgs& a, int nrhs, hipStream_t s)
 // apart per right-hand side.
 // half = 0 or 1: only that column's sums, written at yl[r * ldy + i].
 // Up to 8 independent reductions per launch (blockIdx.z).
-__global__ __launch_bounds__(256) void k_colsum(ColsumBatch b)
+template <typename T>
+__global__ __launch_bounds__(256) void k_colsum(ColsumBatchT<T> b)
 {
-    const ColsumJob& J    = b.j[blockIdx.z];
-    const int        wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int        o    = blockIdx.x * 4 + wave;  // output index
+    const ColsumJobT<T>& J    = b.j[blockIdx.z];
+    const int            wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int            o    = blockIdx.x * 4 + wave;  // output index
     if (o >= (J.half < 0 ? 2 : 1) * J.nx) return;
     const long long r   = blockIdx.y;
-    const double*   src = J.cp + r * (long long)J.nblk * 2 * J.nx + o + (J.half > 0 ? J.nx : 0);
-    double          acc = 0.0;
+    const T*        src = J.cp + r * (long long)J.nblk * 2 * J.nx + o + (J.half > 0 ? J.nx : 0);
+    T               acc = T(0);
     for (int q = lane; q < J.nblk; q += 64) acc += src[(long long)q * 2 * J.nx];
     acc = wave_reduce_sum(acc);
     if (lane == 0) J.yl[r * J.ldy + o] = acc;
 }
-void launch_colsum_batch(const ColsumBatch& b, int nrhs, hipStream_t s)
+template <typename T>
+void launch_colsum_batch(const ColsumBatchT<T>& b, int nrhs, hipStream_t s)
 {
     if (b.n <= 0) return;
     int nout = 0;
     for (int q = 0; q < b.n; q++) nout = std::max(nout, (b.j[q].half < 0 ? 2 : 1) * b.j[q].nx);
-    hipLaunchKernelGGL(k_colsum, dim3((nout + 3) / 4, nrhs, b.n), dim3(256), 0, s, b);
+    hipLaunchKernelGGL((k_colsum<T>), dim3((nout + 3) / 4, nrhs, b.n), dim3(256), 0, s, b);
     FP_HIP(hipGetLastError());
 }
+template void launch_colsum_batch<double>(const ColsumBatchT<double>&, int, hipStream_t);
+template void launch_colsum_batch<float>(const ColsumBatchT<float>&, int, hipStream_t);
 void launch_colsum(const double* cp, int nblk, int nx, double* yl, long long ldy, int nrhs, hipStream_t s, int half)
 {
     ColsumBatch b;
@@ -253,44 +310,56 @@ void launch_colsum(const double* cp, int nblk, int nx, double* yl, long long ldy
 
 // Batched strided dense products y[r*ldy + i] = sum_j A[i*n + j] x[r*ldx + j]
 // (basis changes of interface rows between subdomains), one wavefront per row.
-__global__ __launch_bounds__(256) void k_gemv_batch(GemvBatch b)
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemv_batch(GemvBatchT<T> b)
 {
-    const GemvJob& J    = b.j[blockIdx.z];
-    const int      wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int      i    = blockIdx.x * 4 + wave;
+    const GemvJobT<T>& J    = b.j[blockIdx.z];
+    const int          wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int          i    = blockIdx.x * 4 + wave;
     if (i >= J.n) return;
     const long long r   = blockIdx.y;
-    const double*   Ai  = J.A + (long long)i * J.n;
-    const double*   xr  = J.x + r * J.ldx;
-    double          acc = 0.0;
+    const T*        Ai  = J.A + (long long)i * J.n;
+    const T*        xr  = J.x + r * J.ldx;
+    T               acc = T(0);
     for (int q = lane; q < J.n; q += 64) acc += Ai[q] * xr[q];
     acc = wave_reduce_sum(acc);
     if (lane == 0) J.y[r * J.ldy + i] = acc;
 }
-void launch_gemv_batch(const GemvBatch& b, int nrhs, hipStream_t s)
+template <typename T>
+void launch_gemv_batch(const GemvBatchT<T>& b, int nrhs, hipStream_t s)
 {
     if (b.n <= 0) return;
+    if (nrhs >= kGemmMinRhs) {
+        for (int q = 0; q < b.n; q++) {
+            const GemvJobT<T>& J = b.j[q];
+            dense_apply(J.A, J.n, J.n, J.n, J.x, J.ldx, J.y, J.ldy, nrhs, T(1), T(0), s);
+        }
+        return;
+    }
     int nmax = 0;
     for (int q = 0; q < b.n; q++) nmax = std::max(nmax, b.j[q].n);
-    hipLaunchKernelGGL(k_gemv_batch, dim3((nmax + 3) / 4, nrhs, b.n), dim3(256), 0, s, b);
+    hipLaunchKernelGGL((k_gemv_batch<T>), dim3((nmax + 3) / 4, nrhs, b.n), dim3(256), 0, s, b);
     FP_HIP(hipGetLastError());
 }
+template void launch_gemv_batch<double>(const GemvBatchT<double>&, int, hipStream_t);
+template void launch_gemv_batch<float>(const GemvBatchT<float>&, int, hipStream_t);
 
 // Restriction of a line-sourced solution to the interface unknowns: t[k] is
 // the x-line coefficient (rows 0 / nx-1 in xu) or the physical y-line value
 // (yl, from k_colsum) of unknown k.  mode 0: out = v - t (v == nullptr: the
 // unit vector e_{k0 + r}); mode 1: out = t.
+template <typename TL>
 __global__ void k_lines_assemble(const double* v, double* out, int K, const int* kface, const int* kpos,
-                                 const double* xu, int ny, const double* yl, int nx, int k0, int mode)
+                                 const TL* xu, int ny, const TL* yl, int nx, int k0, int mode)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= K) return;
     const long long r = blockIdx.y;
     const int       f = kface[k], p = kpos[k];
     double          t;
-    if (f == XNEG) t = xu[r * 2 * ny + p];
-    else if (f == XPOS) t = xu[r * 2 * ny + ny + p];
-    else t = yl[r * 2 * nx + (f == YNEG ? 0 : nx) + p];
+    if (f == XNEG) t = (double)xu[r * 2 * ny + p];
+    else if (f == XPOS) t = (double)xu[r * 2 * ny + ny + p];
+    else t = (double)yl[r * 2 * nx + (f == YNEG ? 0 : nx) + p];
     if (mode == 1) {
         out[r * K + k] = t;
         return;
@@ -298,29 +367,51 @@ __global__ void k_lines_assemble(const double* v, double* out, int K, const int*
     const double vv = v ? v[r * K + k] : (k == k0 + (int)r ? 1.0 : 0.0);
     out[r * K + k]  = vv - t;
 }
-void launch_lines_assemble(const double* v, double* out, int K, const int* kface, const int* kpos, const double* xu,
-                           int ny, const double* yl, int nx, int nrhs, hipStream_t s, int k0, int mode)
+template <typename TL>
+void launch_lines_assemble(const double* v, double* out, int K, const int* kface, const int* kpos, const TL* xu,
+                           int ny, const TL* yl, int nx, int nrhs, hipStream_t s, int k0, int mode)
 {
-    hipLaunchKernelGGL(k_lines_assemble, dim3((K + 63) / 64, nrhs), dim3(64), 0, s, v, out, K, kface, kpos, xu,
+    hipLaunchKernelGGL((k_lines_assemble<TL>), dim3((K + 63) / 64, nrhs), dim3(64), 0, s, v, out, K, kface, kpos, xu,
                        ny, yl, nx, k0, mode);
     FP_HIP(hipGetLastError());
 }
+template void launch_lines_assemble<double>(const double*, double*, int, const int*, const int*, const double*, int,
+                                            const double*, int, int, hipStream_t, int, int);
+template void launch_lines_assemble<float>(const double*, double*, int, const int*, const int*, const float*, int,
+                                           const float*, int, int, hipStream_t, int, int);
 
 // Batched strided line copies: dst[r*ldd + t] = coef * src[r*lds + t*stride],
 // t < len, for up to 8 lines (interface lines handed between subdomains).
-__global__ void k_line_copy(LineCopyBatch b)
+template <typename T>
+__global__ void k_line_copy(LineCopyBatchT<T> b)
 {
-    const LineCopy& c = b.c[blockIdx.z];
-    const long long r = blockIdx.y;
+    const LineCopyT<T>& c = b.c[blockIdx.z];
+    const long long     r = blockIdx.y;
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < c.len; t += gridDim.x * blockDim.x)
         c.dst[r * c.ldd + t] = c.coef * c.src[r * c.lds + (long long)t * c.stride];
 }
-void launch_line_copy(const LineCopyBatch& b, int nrhs, hipStream_t s)
+template <typename T>
+void launch_line_copy(const LineCopyBatchT<T>& b, int nrhs, hipStream_t s)
 {
     if (b.n <= 0) return;
     int maxlen = 0;
     for (int q = 0; q < b.n; q++) maxlen = std::max(maxlen, b.c[q].len);
-    hipLaunchKernelGGL(k_line_copy, dim3((maxlen + VB - 1) / VB, nrhs, b.n), dim3(VB), 0, s, b);
+    hipLaunchKernelGGL((k_line_copy<T>), dim3((maxlen + VB - 1) / VB, nrhs, b.n), dim3(VB), 0, s, b);
+    FP_HIP(hipGetLastError());
+}
+template void launch_line_copy<double>(const LineCopyBatchT<double>&, int, hipStream_t);
+template void launch_line_copy<float>(const LineCopyBatchT<float>&, int, hipStream_t);
+
+__global__ void k_to_f32(const double* src, float* dst, long long n)
+{
+    for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x)
+        dst[q] = (float)src[q];
+}
+void launch_to_f32(const double* src, float* dst, long long n, hipStream_t s)
+{
+    if (n <= 0) return;
+    const int nb = (int)std::min<long long>((n + VB - 1) / VB, 1024);
+    hipLaunchKernelGGL(k_to_f32, dim3(nb), dim3(VB), 0, s, src, dst, n);
     FP_HIP(hipGetLastError());
 }
 
@@ -370,6 +461,10 @@ __global__ __launch_bounds__(256) void k_dense_gemv(const double* __restrict__ A
 void launch_dense_gemv(const double* A, int n, const double* x, double* y, double alpha, double beta, int nrhs,
                        hipStream_t s)
 {
+    if (nrhs >= kGemmMinRhs) {
+        dense_apply(A, n, n, n, x, n, y, n, nrhs, alpha, beta, s);
+        return;
+    }
     hipLaunchKernelGGL(k_dense_gemv, dim3((n + 3) / 4, nrhs), dim3(256), 0, s, A, n, x, y, alpha, beta);
     FP_HIP(hipGetLastError());
 }
