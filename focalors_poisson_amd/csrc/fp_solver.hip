@@ -179,6 +179,7 @@ struct ChildLink {
     // maps the child's transform coefficients of a row to the parent's; Qt = Q^T
     // the other way (the transforms are orthonormal)
     DevBuf<double> Q, Qt;
+    DevBuf<float>  Q32, Qt32;  // FP32 copies for the FP32 sweeps of a refinement
 };
 
 struct Node {
@@ -199,6 +200,7 @@ struct Node {
     DevBuf<double> Minv, Zv;
     long long      ldl = 0;
     DevBuf<double> phi, LB, XU, CP, YL;
+    DevBuf<float>  phi32;  // FP32 copy of phi for the FP32 sweeps of a refinement
     DevBuf<int>    kface, kpos;
     DevBuf<int>    state;
     int*           h_state = nullptr;
@@ -212,14 +214,18 @@ struct Node {
     }
 };
 
-// Workspace of the two-level composite sweep (cat_sweep_fast), per child
-// link of the root in link order.
-struct FastWS {
-    int                         nrhs = 0;
-    DevBuf<double>              LBc;  // root: children's adjacent lines in the root's basis (packed lines)
-    std::vector<DevBuf<double>> LBr;  // the root's adjacent line in the child's basis (packed lines)
-    std::vector<DevBuf<double>> CPc;  // partial column sums of the child's upward sweep
-    std::vector<DevBuf<double>> Xs;   // the child's solved adjacent row (child's basis)
+// Workspace of the two-level composite sweep (cat_sweep_fast_t) in one
+// precision, per child link of the root in link order.
+template <typename T>
+struct FastLines {
+    int                    nrhs = 0;
+    DevBuf<T>              LBc;  // root: children's adjacent lines in the root's basis (packed lines)
+    std::vector<DevBuf<T>> LBr;  // the root's adjacent line in the child's basis (packed lines)
+    std::vector<DevBuf<T>> CPc;  // partial column sums of the child's upward sweep
+    std::vector<DevBuf<T>> Xs;   // the child's solved adjacent row (child's basis)
+    // FP32 only (FP64 uses the root node's own): the root's rows, column
+    // partial sums, column sums and Schur lines
+    DevBuf<T> XU, CP, YL, LB;
 };
 
 }  // namespace fp
@@ -263,7 +269,8 @@ struct fp_cat_s {
         if (gexec) (void)hipGraphExecDestroy(gexec);
     }
     int                                    precond_kind = 1;  // 0: none, 1: dense inverse of the interface operator
-    fp::FastWS                             fast;
+    fp::FastLines<double>                  fast;
+    fp::FastLines<float>                   fast32;
     // captured solve (fp_cat_graph_capture): replays the two-level sweep with
     // the GMRES start from the explicit inverse and no further Krylov steps
     bool                                   graph_mode = false;
@@ -957,10 +964,22 @@ static void cat_residual(fp_cat_s* c, double* const* u, int nrhs, hipStream_t s)
 // is estimated as rho / (1 - rho) |d_k| (contraction of the iteration); the
 // loop stops when that estimate, or |d_k| itself, is at most tol |u_k| for
 // every right-hand side, or after refine_max sweeps.
+static bool fast_shape_ok(const fp_cat_s* c);
+template <typename T>
+static void cat_sweep_fast_t(fp_cat_s* c, double* const* f, const double* const* ghost, int nrhs, hipStream_t s);
+
 static void cat_refine(fp_cat_s* c, double* const* f, int nrhs, hipStream_t s)
 {
     const int nd = c->nd;
     ensure_refine_ws(c, nrhs);
+    // the inner sweep: the two-level sweep in transform space with FP32
+    // transforms and sweeps where the tree allows it, else the general sweep
+    // with FP32 subdomain solves (c->mixed)
+    const bool fast  = fast_shape_ok(c);
+    auto       sweep = [&](double* const* u) {
+        if (fast) cat_sweep_fast_t<float>(c, u, nullptr, nrhs, s);
+        else cat_sweep(c, u, nrhs, s);
+    };
     for (int d = 0; d < nd; d++)
         FP_HIP(hipMemcpyAsync(c->rb[d].p, f[d], sizeof(double) * dom_n(c, d) * nrhs, hipMemcpyDeviceToDevice, s));
     const size_t half = (size_t)2 * nd * nrhs * kNormBlocks;  // doubles per norm set
@@ -987,12 +1006,12 @@ static void cat_refine(fp_cat_s* c, double* const* f, int nrhs, hipStream_t s)
     int                 steps = 0;
     double              est   = INFINITY;
     try {
-        cat_sweep(c, f, nrhs, s);  // u_1 = d_1
+        sweep(f);  // u_1 = d_1
         steps = 1;
         norms(f, nullptr, 0);      // |u_1|^2 in slot 0
         for (int k = 2; k <= c->refine_max; k++) {
             cat_residual(c, f, nrhs, s);
-            cat_sweep(c, w.data(), nrhs, s);
+            sweep(w.data());
             norms(f, wc.data(), 1);  // u += d, |d|^2, |u|^2 in slot 1
             steps = k;
             FP_HIP(hipMemcpyAsync(c->h_rpart, c->rpart.p, sizeof(double) * 2 * half, hipMemcpyDeviceToHost, s));
@@ -1054,17 +1073,16 @@ static void cat_refine(fp_cat_s* c, double* const* f, int nrhs, hipStream_t s)
 // Every field is read and written in place: two transforms and two sweeps per
 // domain, no copies, fills or full-field scatters.
 
-static bool fast_ok(const fp_cat_s* c)
+static bool fast_shape_ok(const fp_cat_s* c)
 {
-    if (c->mixed || c->nlev > 2) return false;
+    if (c->nlev > 2) return false;
     const Node& R = *c->nodes[c->root];
     return R.ch.empty() || !R.has_x || R.spectral_x;
 }
+static bool fast_ok(const fp_cat_s* c) { return !c->mixed && fast_shape_ok(c); }
 
-static void ensure_fast_ws(fp_cat_s* c, int nrhs)
+static void alloc_fast_lines(fp_cat_s* c, FastLines<double>& w, int nrhs)
 {
-    FastWS& w = c->fast;
-    if (w.nrhs >= nrhs) return;
     Node&        R  = *c->nodes[c->root];
     const size_t nc = R.ch.size();
     w.LBc.alloc((size_t)nrhs * R.ldl);
@@ -1082,169 +1100,308 @@ static void ensure_fast_ws(fp_cat_s* c, int nrhs)
         w.CPc[k].alloc((size_t)nrhs * sweep_blocks(C.plan) * 2 * c->nx[C.dom]);
         w.Xs[k].alloc((size_t)nrhs * c->ny[C.dom]);
     }
+}
+static void alloc_fast_lines(fp_cat_s* c, FastLines<float>& w, int nrhs)
+{
+    Node&        R  = *c->nodes[c->root];
+    const size_t nc = R.ch.size();
+    // zero-filled through an FP64 fill of the same bytes
+    auto zero = [](DevBuf<float>& b, size_t n) {
+        b.alloc(n + 1);
+        launch_fill<double>(reinterpret_cast<double*>(b.p), (long long)(n + 1) / 2, 0.0, nullptr);
+    };
+    zero(w.LBc, (size_t)nrhs * R.ldl);
+    w.LBr.clear();
+    w.CPc.clear();
+    w.Xs.clear();
+    w.LBr.resize(nc);
+    w.CPc.resize(nc);
+    w.Xs.resize(nc);
+    for (size_t k = 0; k < nc; k++) {
+        const Node& C = *c->nodes[R.ch[k].dom];
+        zero(w.LBr[k], (size_t)nrhs * C.ldl);
+        w.CPc[k].alloc((size_t)nrhs * sweep_blocks(C.plan) * 2 * c->nx[C.dom]);
+        w.Xs[k].alloc((size_t)nrhs * c->ny[C.dom]);
+    }
+    zero(w.LB, (size_t)nrhs * R.ldl);
+    w.XU.alloc((size_t)nrhs * 2 * c->ny[R.dom]);
+    w.CP.alloc((size_t)nrhs * sweep_blocks(R.plan) * 2 * c->nx[R.dom]);
+    w.YL.alloc((size_t)nrhs * 2 * c->nx[R.dom]);
+    // FP32 copies of the phi tables and the orthogonal maps (nrhs-independent)
+    for (int d = 0; d < c->nd; d++) {
+        Node& N = *c->nodes[d];
+        N.plan.ensure_f32();
+        if (!N.phi32.p && N.phi.p) {
+            N.phi32.alloc(N.phi.n);
+            launch_to_f32(N.phi.p, N.phi32.p, (long long)N.phi.n, nullptr);
+        }
+    }
+    for (ChildLink& L : R.ch)
+        if (L.Q.p && !L.Q32.p) {
+            L.Q32.alloc(L.Q.n);
+            L.Qt32.alloc(L.Qt.n);
+            launch_to_f32(L.Q.p, L.Q32.p, (long long)L.Q.n, nullptr);
+            launch_to_f32(L.Qt.p, L.Qt32.p, (long long)L.Qt.n, nullptr);
+        }
+}
+
+template <typename T>
+static FastLines<T>& fast_lines(fp_cat_s* c);
+template <>
+FastLines<double>& fast_lines<double>(fp_cat_s* c) { return c->fast; }
+template <>
+FastLines<float>& fast_lines<float>(fp_cat_s* c) { return c->fast32; }
+
+template <typename T>
+static void ensure_fast_ws(fp_cat_s* c, int nrhs)
+{
+    FastLines<T>& w = fast_lines<T>(c);
+    if (w.nrhs >= nrhs) return;
+    alloc_fast_lines(c, w, nrhs);
     FP_HIP(hipDeviceSynchronize());
     w.nrhs = nrhs;
 }
 
-static void cat_sweep_fast(fp_cat_s* c, double* const* f, const double* const* ghost, int nrhs, hipStream_t s)
+// The buffers one precision of the two-level sweep works on.  FP64: the
+// fields themselves (transformed in place) and the nodes' own line buffers;
+// FP32 (the inner solves of a refinement, fields already assembled): per
+// domain FP32 coefficient scratch between an FP64-reading forward transform
+// and an FP64-writing inverse, and FP32 copies of every line buffer, phi table
+// and orthogonal map.
+template <typename T>
+struct FastView {
+    std::vector<T*>       fld;
+    std::vector<const T*> phi;
+    T *                   XU = nullptr, *CP = nullptr, *YL = nullptr;
+    const T*              LB = nullptr;
+    std::vector<const T*> Q, Qt;
+};
+static FastView<double> fast_view(fp_cat_s* c, double* const* f, double*)
 {
-    const int nd = c->nd, root = c->root;
-    Node&     R  = *c->nodes[root];
+    FastView<double> v;
+    Node&            R = *c->nodes[c->root];
+    for (int d = 0; d < c->nd; d++) {
+        v.fld.push_back(f[d]);
+        v.phi.push_back(c->nodes[d]->phi.p);
+    }
+    v.XU = R.XU.p;
+    v.CP = R.CP.p;
+    v.YL = R.YL.p;
+    v.LB = R.LB.p;
+    for (const ChildLink& L : R.ch) {
+        v.Q.push_back(L.Q.p);
+        v.Qt.push_back(L.Qt.p);
+    }
+    return v;
+}
+static FastView<float> fast_view(fp_cat_s* c, double* const*, float*)
+{
+    FastView<float> v;
+    Node&           R = *c->nodes[c->root];
+    for (int d = 0; d < c->nd; d++) {
+        v.fld.push_back(c->s32[d].p);
+        v.phi.push_back(c->nodes[d]->phi32.p);
+    }
+    v.XU = c->fast32.XU.p;
+    v.CP = c->fast32.CP.p;
+    v.YL = c->fast32.YL.p;
+    v.LB = c->fast32.LB.p;
+    for (const ChildLink& L : R.ch) {
+        v.Q.push_back(L.Q32.p);
+        v.Qt.push_back(L.Qt32.p);
+    }
+    return v;
+}
+
+template <typename T>
+static void cat_sweep_fast_t(fp_cat_s* c, double* const* f, const double* const* ghost, int nrhs, hipStream_t s)
+{
+    constexpr bool f64 = sizeof(T) == 8;
+    const int      nd = c->nd, root = c->root;
+    Node&          R  = *c->nodes[root];
     c->last_steps = c->last_restarts = c->last_status = 0;
     c->last_res = c->last_rel = 0.0;
-    // 1. assembly + forward transforms of every domain
-    std::vector<TransformJob<double>> fw(nd), iv(nd);
+    if (!R.ch.empty()) {
+        ensure_ws(c, R, nrhs);
+        ensure_fast_ws<T>(c, nrhs);
+    } else if constexpr (!f64)
+        R.plan.ensure_f32();
+    const FastView<T> v = fast_view(c, f, (T*)nullptr);
+    // 1. forward transforms of every domain (FP64: in place, with the
+    //    boundary assembly and scaling fused into the load; FP32: from the
+    //    assembled FP64 field into the FP32 scratch)
+    std::vector<TransformJob<T>> fw(nd), iv(nd);
     for (int d = 0; d < nd; d++) {
-        TransformJob<double>& j = fw[d];
-        j       = {};
-        j.plan  = &c->nodes[d]->plan.t64;
-        j.data  = f[d];
-        j.ld    = dom_n(c, d);
-        j.nx    = c->nx[d];
-        j.nrhs  = nrhs;
-        j.dst64 = 1;
-        iv[d]   = j;
-        j.asmb  = 1;
-        j.hx2   = c->hx[d] * c->hx[d];
-        face_terms(c, d, ghost, j.g, j.ga);
+        const long long  n = dom_n(c, d);
+        TransformJob<T>& a = fw[d];
+        TransformJob<T>& b = iv[d];
+        a       = {};
+        a.plan  = &tplan<T>(c->nodes[d]->plan);
+        a.nx    = c->nx[d];
+        a.nrhs  = nrhs;
+        a.dst64 = f64;
+        b       = a;
+        a.data  = v.fld[d];
+        a.ld    = n;
+        b.data  = f[d];
+        b.ld    = n;
+        b.dst64 = 1;
+        if constexpr (f64) {
+            a.asmb = 1;
+            a.hx2  = c->hx[d] * c->hx[d];
+            face_terms(c, d, ghost, a.g, a.ga);
+        } else {
+            a.src   = f[d];
+            a.lds   = n;
+            a.src64 = 1;
+            b.src   = v.fld[d];
+            b.lds   = n;
+        }
     }
-    launch_ytransform_batch<double>(fw.data(), nd, 0, s);
+    launch_ytransform_batch<T>(fw.data(), nd, 0, s);
     const int       nxr = c->nx[root], nyr = c->ny[root];
     const long long nr  = dom_n(c, root);
     if (R.ch.empty()) {
-        launch_tridiag<double>(R.plan, f[root], nr, nrhs, s);
-        launch_ytransform<double>(R.plan.t64, f[root], nr, nxr, nrhs, 1, s);
+        launch_tridiag<T>(R.plan, v.fld[root], nr, nrhs, s);
+        launch_ytransform_batch<T>(iv.data(), nd, 1, s);
         return;
     }
-    ensure_ws(c, R, nrhs);
-    ensure_fast_ws(c, nrhs);
-    FastWS&      w  = c->fast;
-    const size_t nc = R.ch.size();
+    FastLines<T>& w  = fast_lines<T>(c);
+    const size_t  nc = R.ch.size();
     FP_REQUIRE(nc <= 8, "too many children");
 
     // 2. children's upward solves, only the line next to the root written:
     //    root face XNEG (row 0) meets the child's row nx-1, XPOS its row 0;
     //    root face YNEG (column 0) meets the child's column ny-1 (phi1 sums),
     //    YPOS its column 0 (phi0 sums)
-    std::vector<SweepJob<double>> up(nc);
-    ColsumBatch                   ycs;
-    GemvBatch                     qup;
+    std::vector<SweepJob<T>> up(nc);
+    ColsumBatchT<T>          ycs;
+    GemvBatchT<T>            qup;
     for (size_t k = 0; k < nc; k++) {
-        const ChildLink&  L = R.ch[k];
-        const Node&       C = *c->nodes[L.dom];
-        const int         nyc = c->ny[L.dom];
-        SweepJob<double>& j = up[k];
+        const ChildLink& L   = R.ch[k];
+        const Node&      C   = *c->nodes[L.dom];
+        const int        nyc = c->ny[L.dom];
+        SweepJob<T>&     j   = up[k];
         j.sp   = &C.plan;
         j.nrhs = nrhs;
-        j.f    = f[L.dom];
+        j.f    = v.fld[L.dom];
         j.ld   = dom_n(c, L.dom);
-        j.phi  = C.phi.p;
+        j.phi  = v.phi[L.dom];
         j.fin  = 1;
         j.fout = 0;
         if (L.loc == XNEG || L.loc == XPOS) {
-            double* slot = w.LBc.p + (L.loc == XNEG ? 0 : nyr);
-            double* dst  = L.Q.p ? w.Xs[k].p : slot;
-            j.ldxo       = L.Q.p ? nyc : R.ldl;
+            T* slot = w.LBc.p + (L.loc == XNEG ? 0 : nyr);
+            T* dst  = v.Q[k] ? w.Xs[k].p : slot;
+            j.ldxo  = v.Q[k] ? nyc : R.ldl;
             if (L.loc == XNEG) j.xo1 = dst;
             else j.xo0 = dst;
-            if (L.Q.p) qup.j[qup.n++] = {L.Q.p, nyc, w.Xs[k].p, nyc, slot, R.ldl};
+            if (v.Q[k]) qup.j[qup.n++] = {v.Q[k], nyc, w.Xs[k].p, nyc, slot, R.ldl};
         } else {
             j.cp = w.CPc[k].p;
             ycs.j[ycs.n++] = {w.CPc[k].p, sweep_blocks(C.plan), c->nx[L.dom], L.loc == YNEG ? 1 : 0,
                               w.LBc.p + 2LL * nyr + (L.loc == YNEG ? 0 : nxr), R.ldl};
         }
     }
-    launch_sweep_batch<double>(up.data(), (int)nc, s);
+    launch_sweep_batch<T>(up.data(), (int)nc, s);
     launch_colsum_batch(ycs, nrhs, s);
     launch_gemv_batch(qup, nrhs, s);
 
     // 3. the root's interface lines of A_r^-1 b' (the GMRES right-hand side)
     {
-        SweepJob<double> j{};
+        SweepJob<T> j{};
         j.sp   = &R.plan;
         j.nrhs = nrhs;
-        j.f    = f[root];
+        j.f    = v.fld[root];
         j.ld   = nr;
         j.lbm  = w.LBc.p;
         j.ldl  = R.ldl;
-        j.phi  = R.phi.p;
-        j.xo0  = R.XU.p;
-        j.xo1  = R.XU.p + nyr;
+        j.phi  = v.phi[root];
+        j.xo0  = v.XU;
+        j.xo1  = v.XU + nyr;
         j.ldxo = 2LL * nyr;
-        j.cp   = R.has_y ? R.CP.p : nullptr;
+        j.cp   = R.has_y ? v.CP : nullptr;
         j.fin  = 1;
         j.fout = 0;
-        launch_sweep_batch<double>(&j, 1, s);
-        if (R.has_y) launch_colsum(R.CP.p, sweep_blocks(R.plan), nxr, R.YL.p, 2LL * nxr, nrhs, s);
-        launch_lines_assemble(nullptr, R.C.p, R.K, R.kface.p, R.kpos.p, R.XU.p, nyr, R.YL.p, nxr, nrhs, s, 0, 1);
+        launch_sweep_batch<T>(&j, 1, s);
+        if (R.has_y) {
+            ColsumBatchT<T> cs;
+            cs.j[cs.n++] = {v.CP, sweep_blocks(R.plan), nxr, -1, v.YL, 2LL * nxr};
+            launch_colsum_batch(cs, nrhs, s);
+        }
+        launch_lines_assemble<T>(nullptr, R.C.p, R.K, R.kface.p, R.kpos.p, v.XU, nyr, v.YL, nxr, nrhs, s, 0, 1);
     }
     gmres_iterate(c, R, nrhs, s);
 
     // 4. the root's solution: sweep of F_r - child lines + E S y, written as
     //    coefficients, with its interface rows and column sums for the children
     launch_schur_lines(schur_lines_args(c, R, R.X.p), nrhs, s);
+    if constexpr (!f64) launch_to_f32(R.LB.p, w.LB.p, (long long)nrhs * R.ldl, s);
     {
-        SweepJob<double> j{};
+        SweepJob<T> j{};
         j.sp   = &R.plan;
         j.nrhs = nrhs;
-        j.f    = f[root];
+        j.f    = v.fld[root];
         j.ld   = nr;
-        j.lbp  = R.LB.p;
+        j.lbp  = v.LB;
         j.lbm  = w.LBc.p;
         j.ldl  = R.ldl;
-        j.phi  = R.phi.p;
-        j.xo0  = R.XU.p;
-        j.xo1  = R.XU.p + nyr;
+        j.phi  = v.phi[root];
+        j.xo0  = v.XU;
+        j.xo1  = v.XU + nyr;
         j.ldxo = 2LL * nyr;
-        j.cp   = R.has_y ? R.CP.p : nullptr;
+        j.cp   = R.has_y ? v.CP : nullptr;
         j.fin  = 1;
         j.fout = 1;
-        launch_sweep_batch<double>(&j, 1, s);
+        launch_sweep_batch<T>(&j, 1, s);
     }
 
     // 5. the root's line next to each child into the child's right-hand side:
     //    x-lines from the solved rows (through Q^T when the bases differ),
     //    y-lines from the column sums; then the children's final solves and
     //    every domain's inverse transform in one batch
-    ColsumBatch   ydn;
-    GemvBatch     qdn;
-    LineCopyBatch lc;
+    ColsumBatchT<T>   ydn;
+    GemvBatchT<T>     qdn;
+    LineCopyBatchT<T> lc;
     for (size_t k = 0; k < nc; k++) {
-        const ChildLink& L = R.ch[k];
-        const Node&      C = *c->nodes[L.dom];
+        const ChildLink& L   = R.ch[k];
+        const Node&      C   = *c->nodes[L.dom];
         const int        nxc = c->nx[L.dom], nyc = c->ny[L.dom];
         if (L.loc == XNEG || L.loc == XPOS) {
             // root row 0 (XNEG) -> child's row nx-1 (slot x1); root row nx-1 (XPOS) -> child's row 0 (x0)
-            const double* src = R.XU.p + (L.loc == XNEG ? 0 : nyr);
-            double*       dst = w.LBr[k].p + (L.loc == XNEG ? nyc : 0);
-            if (L.Qt.p) qdn.j[qdn.n++] = {L.Qt.p, nyc, src, 2LL * nyr, dst, C.ldl};
-            else lc.c[lc.n++] = {src, 2LL * nyr, 1, nyc, dst, C.ldl, 1.0};
+            const T* src = v.XU + (L.loc == XNEG ? 0 : nyr);
+            T*       dst = w.LBr[k].p + (L.loc == XNEG ? nyc : 0);
+            if (v.Qt[k]) qdn.j[qdn.n++] = {v.Qt[k], nyc, src, 2LL * nyr, dst, C.ldl};
+            else lc.c[lc.n++] = {src, 2LL * nyr, 1, nyc, dst, C.ldl, T(1)};
         } else {
             // root column 0 (YNEG) -> child's column ny-1 (slot y1); root column ny-1 (YPOS) -> child's column 0 (y0)
-            ydn.j[ydn.n++] = {R.CP.p, sweep_blocks(R.plan), nxr, L.loc == YNEG ? 0 : 1,
+            ydn.j[ydn.n++] = {v.CP, sweep_blocks(R.plan), nxr, L.loc == YNEG ? 0 : 1,
                               w.LBr[k].p + 2LL * nyc + (L.loc == YNEG ? nxc : 0), C.ldl};
         }
     }
     launch_line_copy(lc, nrhs, s);
     launch_gemv_batch(qdn, nrhs, s);
     launch_colsum_batch(ydn, nrhs, s);
-    std::vector<SweepJob<double>> dn(nc);
+    std::vector<SweepJob<T>> dn(nc);
     for (size_t k = 0; k < nc; k++) {
-        const int         cd = R.ch[k].dom;
-        const Node&       C  = *c->nodes[cd];
-        SweepJob<double>& j  = dn[k];
+        const int    cd = R.ch[k].dom;
+        const Node&  C  = *c->nodes[cd];
+        SweepJob<T>& j  = dn[k];
         j.sp   = &C.plan;
         j.nrhs = nrhs;
-        j.f    = f[cd];
+        j.f    = v.fld[cd];
         j.ld   = dom_n(c, cd);
         j.lbm  = w.LBr[k].p;
         j.ldl  = C.ldl;
-        j.phi  = C.phi.p;
+        j.phi  = v.phi[cd];
         j.fin  = 1;
         j.fout = 1;
     }
-    launch_sweep_batch<double>(dn.data(), (int)nc, s);
-    launch_ytransform_batch<double>(iv.data(), nd, 1, s);
+    launch_sweep_batch<T>(dn.data(), (int)nc, s);
+    launch_ytransform_batch<T>(iv.data(), nd, 1, s);
+}
+static void cat_sweep_fast(fp_cat_s* c, double* const* f, const double* const* ghost, int nrhs, hipStream_t s)
+{
+    cat_sweep_fast_t<double>(c, f, ghost, nrhs, s);
 }
 
 // ConcatPoissonSolver2D::solve (pe/concat/concat_solver2d.cpp:101-157)
@@ -1517,7 +1674,7 @@ int fp_cat_graph_capture(fp_cat cat, double* const* d_fields, const double* cons
         // every allocation before the capture
         if (!R.ch.empty()) {
             fp::ensure_ws(cat, R, nrhs);
-            fp::ensure_fast_ws(cat, nrhs);
+            fp::ensure_fast_ws<double>(cat, nrhs);
         }
         FP_HIP(hipDeviceSynchronize());
         if (cat->gexec) {

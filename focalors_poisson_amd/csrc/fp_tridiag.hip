@@ -739,6 +739,7 @@ void launch_sweep_batch(const SweepJob<T>* jobs, int njobs, hipStream_t s)
         launch_sweep_group<T, 32>(g32.data() + q0, (int)std::min<size_t>(kMaxBatch, g32.size() - q0), s);
 }
 template void launch_sweep_batch<double>(const SweepJob<double>*, int, hipStream_t);
+template void launch_sweep_batch<float>(const SweepJob<float>*, int, hipStream_t);
 
 int sweep_blocks(const SubPlan& sp)
 {
