@@ -155,6 +155,10 @@ struct TransformJob {
 };
 template <typename T>
 void launch_ytransform_batch(const TransformJob<T>* jobs, int njobs, int inverse, hipStream_t s);
+// One thread count for a batch launched together (0: every job its own),
+// chosen by timing the forward batch; the jobs' fields are overwritten.
+template <typename T>
+int tune_batch_nt(std::vector<TransformJob<T>> jobs, hipStream_t s);
 template <typename T>
 struct TridiagJob {
     const SubPlan* sp;

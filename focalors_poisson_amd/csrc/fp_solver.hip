@@ -269,6 +269,9 @@ struct fp_cat_s {
         if (gexec) (void)hipGraphExecDestroy(gexec);
     }
     int                                    precond_kind = 1;  // 0: none, 1: dense inverse of the interface operator
+    // threads per workgroup shared by the batched transforms of all domains
+    // (0: each plan's own; -1: not chosen yet)
+    int                                    yt_nt64 = -1, yt_nt32 = -1;
     fp::FastLines<double>                  fast;
     fp::FastLines<float>                   fast32;
     // captured solve (fp_cat_graph_capture): replays the two-level sweep with
@@ -1213,6 +1216,36 @@ static FastView<float> fast_view(fp_cat_s* c, double* const*, float*)
     return v;
 }
 
+// The thread count shared by the batched transforms of every domain, so that
+// each direction is one launch whose domains run side by side: timed once per
+// solver and precision on zeroed scratch fields.
+template <typename T>
+static int batch_nt(fp_cat_s* c)
+{
+    int& v = sizeof(T) == 8 ? c->yt_nt64 : c->yt_nt32;
+    if (v >= 0) return v;
+    if (c->nd < 2) return v = 0;
+    std::vector<DevBuf<T>>       buf(c->nd);
+    std::vector<TransformJob<T>> jobs(c->nd);
+    for (int d = 0; d < c->nd; d++) {
+        SubPlan& sp = c->nodes[d]->plan;
+        if constexpr (sizeof(T) == 4) sp.ensure_f32();
+        const long long n = dom_n(c, d);
+        buf[d].alloc((size_t)n);
+        FP_HIP(hipMemset(buf[d].p, 0, sizeof(T) * n));
+        jobs[d]       = {};
+        jobs[d].plan  = &tplan<T>(sp);
+        jobs[d].data  = buf[d].p;
+        jobs[d].ld    = n;
+        jobs[d].nx    = c->nx[d];
+        jobs[d].nrhs  = 1;
+        jobs[d].dst64 = sizeof(T) == 8;
+    }
+    v = tune_batch_nt<T>(jobs, nullptr);
+    FP_HIP(hipDeviceSynchronize());
+    return v;
+}
+
 template <typename T>
 static void cat_sweep_fast_t(fp_cat_s* c, double* const* f, const double* const* ghost, int nrhs, hipStream_t s)
 {
@@ -1226,7 +1259,8 @@ static void cat_sweep_fast_t(fp_cat_s* c, double* const* f, const double* const*
         ensure_fast_ws<T>(c, nrhs);
     } else if constexpr (!f64)
         R.plan.ensure_f32();
-    const FastView<T> v = fast_view(c, f, (T*)nullptr);
+    const FastView<T> v  = fast_view(c, f, (T*)nullptr);
+    const int         nt = batch_nt<T>(c);
     // 1. forward transforms of every domain (FP64: in place, with the
     //    boundary assembly and scaling fused into the load; FP32: from the
     //    assembled FP64 field into the FP32 scratch)
@@ -1240,6 +1274,7 @@ static void cat_sweep_fast_t(fp_cat_s* c, double* const* f, const double* const*
         a.nx    = c->nx[d];
         a.nrhs  = nrhs;
         a.dst64 = f64;
+        a.nt    = nt;
         b       = a;
         a.data  = v.fld[d];
         a.ld    = n;
@@ -1629,6 +1664,10 @@ int fp_cat_create(fp_ctx ctx, const fp_geometry_desc* g, int m, double tol, int 
         hipStream_t s = nullptr;
         for (int l = c->nlev - 1; l >= 0; l--)
             for (int d : c->levels[l]) fp::build_node(c.get(), d, s);
+        if (nd > 1 && fp::fast_shape_ok(c.get())) {
+            if (precision == FP_F64) fp::batch_nt<double>(c.get());
+            else fp::batch_nt<float>(c.get());
+        }
         FP_HIP(hipDeviceSynchronize());
         *out = c.release();
     });

@@ -1197,6 +1197,47 @@ void tune_transform(TransformPlan<T>& tp, int nx, bool measure)
     std::lock_guard<std::mutex> lk(g_tt_mu);
     g_tt_cache[key] = nt_best;
 }
+// Threads per workgroup for a batch of transforms launched together (the
+// domains of a composite solve): every job of the single-buffer engine takes
+// the same count where it fits, so that jobs whose own tuned counts differ
+// share one launch and run side by side.  The forward batch is timed with
+// each candidate; 0 (every job its own count) is one of them.  The jobs must
+// point at valid fields.
+template <typename T>
+int tune_batch_nt(std::vector<TransformJob<T>> jobs, hipStream_t s)
+{
+    if (jobs.size() < 2) return 0;
+    hipEvent_t e0, e1;
+    FP_HIP(hipEventCreate(&e0));
+    FP_HIP(hipEventCreate(&e1));
+    float best    = 1e30f;
+    int   nt_best = 0;
+    for (int cand : {0, 128, 256, 512}) {
+        bool any = false;
+        for (auto& j : jobs) {
+            const TransformPlan<T>& tp = *j.plan;
+            j.nt = (cand && tp.engine == 1 && sb_fits(tp.rad, tp.nrad, tp.L, cand, sizeof(T) == 8)) ? cand : 0;
+            any |= j.nt != 0;
+        }
+        if (cand && !any) continue;
+        launch_ytransform_batch<T>(jobs.data(), (int)jobs.size(), 0, s);
+        FP_HIP(hipEventRecord(e0, s));
+        for (int r = 0; r < 5; r++) launch_ytransform_batch<T>(jobs.data(), (int)jobs.size(), 0, s);
+        FP_HIP(hipEventRecord(e1, s));
+        FP_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        FP_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+            best    = ms;
+            nt_best = cand;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return nt_best;
+}
+template int tune_batch_nt<double>(std::vector<TransformJob<double>>, hipStream_t);
+template int tune_batch_nt<float>(std::vector<TransformJob<float>>, hipStream_t);
 template void tune_transform<double>(TransformPlan<double>&, int, bool);
 template void tune_transform<float>(TransformPlan<float>&, int, bool);
 
