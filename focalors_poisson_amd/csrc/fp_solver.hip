@@ -199,7 +199,7 @@ struct Node {
     bool           precond    = false;  // right preconditioner: explicit inverse of the interface operator
     DevBuf<double> Minv, Zv;
     long long      ldl = 0;
-    DevBuf<double> phi, LB, XU, CP, YL;
+    DevBuf<double> phi, LB, XU, CP;
     DevBuf<float>  phi32;  // FP32 copy of phi for the FP32 sweeps of a refinement
     DevBuf<int>    kface, kpos;
     DevBuf<int>    state;
@@ -224,8 +224,8 @@ struct FastLines {
     std::vector<DevBuf<T>> CPc;  // partial column sums of the child's upward sweep
     std::vector<DevBuf<T>> Xs;   // the child's solved adjacent row (child's basis)
     // FP32 only (FP64 uses the root node's own): the root's rows, column
-    // partial sums, column sums and Schur lines
-    DevBuf<T> XU, CP, YL, LB;
+    // partial sums and Schur lines
+    DevBuf<T> XU, CP, LB;
 };
 
 }  // namespace fp
@@ -368,7 +368,6 @@ static void ensure_ws(fp_cat_s* c, Node& nd, int nrhs)
     nd.XU.alloc((size_t)nrhs * 2 * c->ny[nd.dom]);
     nd.Zv.alloc((size_t)nrhs * K);
     nd.CP.alloc((size_t)nrhs * std::max(nd.nwg, sweep_blocks(nd.plan)) * 2 * c->nx[nd.dom]);
-    nd.YL.alloc((size_t)nrhs * 2 * c->nx[nd.dom]);
     FP_HIP(hipDeviceSynchronize());
     if (nd.h_state) (void)hipHostFree(nd.h_state);
     if (nd.h_resid) (void)hipHostFree(nd.h_resid);
@@ -435,8 +434,7 @@ static void lines_solve_assemble(fp_cat_s* c, Node& nd, const double* v, double*
     la.ycols   = nd.has_y;
     launch_tridiag_lines<double>(nd.plan, la, nrhs, s);
     if (nd.has_x && !nd.spectral_x) launch_ytransform<double>(nd.plan.t64, nd.XU.p, 2LL * ny, 2, nrhs, 1, s);
-    if (nd.has_y) launch_colsum(nd.CP.p, nd.nwg, nx, nd.YL.p, 2LL * nx, nrhs, s);
-    launch_lines_assemble(v, out, nd.K, nd.kface.p, nd.kpos.p, nd.XU.p, ny, nd.YL.p, nx, nrhs, s, k0);
+    launch_lines_assemble(v, out, nd.K, nd.kface.p, nd.kpos.p, nd.XU.p, ny, nd.CP.p, nd.nwg, nx, nrhs, s, k0);
 }
 
 static void matvec_lines(fp_cat_s* c, Node& nd, const double* v, double* out, int nrhs, hipStream_t s)
@@ -1129,7 +1127,6 @@ static void alloc_fast_lines(fp_cat_s* c, FastLines<float>& w, int nrhs)
     zero(w.LB, (size_t)nrhs * R.ldl);
     w.XU.alloc((size_t)nrhs * 2 * c->ny[R.dom]);
     w.CP.alloc((size_t)nrhs * sweep_blocks(R.plan) * 2 * c->nx[R.dom]);
-    w.YL.alloc((size_t)nrhs * 2 * c->nx[R.dom]);
     // FP32 copies of the phi tables and the orthogonal maps (nrhs-independent)
     for (int d = 0; d < c->nd; d++) {
         Node& N = *c->nodes[d];
@@ -1175,7 +1172,7 @@ template <typename T>
 struct FastView {
     std::vector<T*>       fld;
     std::vector<const T*> phi;
-    T *                   XU = nullptr, *CP = nullptr, *YL = nullptr;
+    T *                   XU = nullptr, *CP = nullptr;
     const T*              LB = nullptr;
     std::vector<const T*> Q, Qt;
 };
@@ -1189,7 +1186,6 @@ static FastView<double> fast_view(fp_cat_s* c, double* const* f, double*)
     }
     v.XU = R.XU.p;
     v.CP = R.CP.p;
-    v.YL = R.YL.p;
     v.LB = R.LB.p;
     for (const ChildLink& L : R.ch) {
         v.Q.push_back(L.Q.p);
@@ -1207,7 +1203,6 @@ static FastView<float> fast_view(fp_cat_s* c, double* const*, float*)
     }
     v.XU = c->fast32.XU.p;
     v.CP = c->fast32.CP.p;
-    v.YL = c->fast32.YL.p;
     v.LB = c->fast32.LB.p;
     for (const ChildLink& L : R.ch) {
         v.Q.push_back(L.Q32.p);
@@ -1338,8 +1333,7 @@ static void cat_sweep_fast_t(fp_cat_s* c, double* const* f, const double* const*
         }
     }
     launch_sweep_batch<T>(up.data(), (int)nc, s);
-    launch_colsum_batch(ycs, nrhs, s);
-    launch_gemv_batch(qup, nrhs, s);
+    launch_exchange(ycs, qup, LineCopyBatchT<T>{}, nrhs, s);
 
     // 3. the root's interface lines of A_r^-1 b' (the GMRES right-hand side)
     {
@@ -1358,12 +1352,8 @@ static void cat_sweep_fast_t(fp_cat_s* c, double* const* f, const double* const*
         j.fin  = 1;
         j.fout = 0;
         launch_sweep_batch<T>(&j, 1, s);
-        if (R.has_y) {
-            ColsumBatchT<T> cs;
-            cs.j[cs.n++] = {v.CP, sweep_blocks(R.plan), nxr, -1, v.YL, 2LL * nxr};
-            launch_colsum_batch(cs, nrhs, s);
-        }
-        launch_lines_assemble<T>(nullptr, R.C.p, R.K, R.kface.p, R.kpos.p, v.XU, nyr, v.YL, nxr, nrhs, s, 0, 1);
+        launch_lines_assemble<T>(nullptr, R.C.p, R.K, R.kface.p, R.kpos.p, v.XU, nyr, v.CP, sweep_blocks(R.plan),
+                                 nxr, nrhs, s, 0, 1);
     }
     gmres_iterate(c, R, nrhs, s);
 
@@ -1413,9 +1403,7 @@ static void cat_sweep_fast_t(fp_cat_s* c, double* const* f, const double* const*
                               w.LBr[k].p + 2LL * nyc + (L.loc == YNEG ? nxc : 0), C.ldl};
         }
     }
-    launch_line_copy(lc, nrhs, s);
-    launch_gemv_batch(qdn, nrhs, s);
-    launch_colsum_batch(ydn, nrhs, s);
+    launch_exchange(ydn, qdn, lc, nrhs, s);
     std::vector<SweepJob<T>> dn(nc);
     for (size_t k = 0; k < nc; k++) {
         const int    cd = R.ch[k].dom;

@@ -81,10 +81,11 @@ using GemvBatch = GemvBatchT<double>;
 template <typename T>
 void launch_gemv_batch(const GemvBatchT<T>& b, int nrhs, hipStream_t s);
 // mode 0: out = v - t (v == nullptr: the unit vector e_{k0 + r} of right-hand
-// side r); mode 1: out = t; t from the x rows xu and the y-lines yl (2 nx per rhs)
+// side r); mode 1: out = t; t from the x rows xu and the column sums of the
+// partials cp (nblk blocks of 2 nx per rhs; unused without y-lines)
 template <typename TL>
 void launch_lines_assemble(const double* v, double* out, int K, const int* kface, const int* kpos, const TL* xu,
-                           int ny, const TL* yl, int nx, int nrhs, hipStream_t s, int k0 = 0, int mode = 0);
+                           int ny, const TL* cp, int nblk, int nx, int nrhs, hipStream_t s, int k0 = 0, int mode = 0);
 template <typename T>
 struct LineCopyT {
     const T*  src;
@@ -103,6 +104,12 @@ using LineCopy      = LineCopyT<double>;
 using LineCopyBatch = LineCopyBatchT<double>;
 template <typename T>
 void launch_line_copy(const LineCopyBatchT<T>& b, int nrhs, hipStream_t s);
+// The line exchanges of one step of the composite sweep in one launch: the
+// column sums, basis-change products and line copies of the three batches
+// (products go to the BLAS when nrhs reaches the GEMM threshold).
+template <typename T>
+void launch_exchange(const ColsumBatchT<T>& cs, const GemvBatchT<T>& gv, const LineCopyBatchT<T>& lc, int nrhs,
+                     hipStream_t s);
 // dst[q] = (float)src[q], q < n
 void launch_to_f32(const double* src, float* dst, long long n, hipStream_t s);
 void launch_lines_from_units(const SchurLinesArgs& a, int k0, int cnt, hipStream_t s);

@@ -346,20 +346,29 @@ template void launch_gemv_batch<float>(const GemvBatchT<float>&, int, hipStream_
 
 // Restriction of a line-sourced solution to the interface unknowns: t[k] is
 // the x-line coefficient (rows 0 / nx-1 in xu) or the physical y-line value
-// (yl, from k_colsum) of unknown k.  mode 0: out = v - t (v == nullptr: the
-// unit vector e_{k0 + r}); mode 1: out = t.
+// of unknown k, the latter the total of the sweep's per-workgroup partial
+// column sums (cp, nblk blocks of 2 nx per right-hand side), reduced here by
+// one wavefront per unknown.  mode 0: out = v - t (v == nullptr: the unit
+// vector e_{k0 + r}); mode 1: out = t.
 template <typename TL>
-__global__ void k_lines_assemble(const double* v, double* out, int K, const int* kface, const int* kpos,
-                                 const TL* xu, int ny, const TL* yl, int nx, int k0, int mode)
+__global__ __launch_bounds__(256) void k_lines_assemble(const double* v, double* out, int K, const int* kface,
+                                                        const int* kpos, const TL* xu, int ny, const TL* cp, int nblk,
+                                                        int nx, int k0, int mode)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (k >= K) return;
     const long long r = blockIdx.y;
     const int       f = kface[k], p = kpos[k];
     double          t;
-    if (f == XNEG) t = (double)xu[r * 2 * ny + p];
-    else if (f == XPOS) t = (double)xu[r * 2 * ny + ny + p];
-    else t = (double)yl[r * 2 * nx + (f == YNEG ? 0 : nx) + p];
+    if (f == XNEG || f == XPOS) {
+        t = (double)xu[r * 2 * ny + (f == XPOS ? ny : 0) + p];
+    } else {
+        const TL* src = cp + r * (long long)nblk * 2 * nx + (f == YNEG ? 0 : nx) + p;
+        TL        acc = TL(0);
+        for (int q = lane; q < nblk; q += 64) acc += src[(long long)q * 2 * nx];
+        t = (double)wave_reduce_sum(acc);
+    }
+    if (lane) return;
     if (mode == 1) {
         out[r * K + k] = t;
         return;
@@ -369,16 +378,16 @@ __global__ void k_lines_assemble(const double* v, double* out, int K, const int*
 }
 template <typename TL>
 void launch_lines_assemble(const double* v, double* out, int K, const int* kface, const int* kpos, const TL* xu,
-                           int ny, const TL* yl, int nx, int nrhs, hipStream_t s, int k0, int mode)
+                           int ny, const TL* cp, int nblk, int nx, int nrhs, hipStream_t s, int k0, int mode)
 {
-    hipLaunchKernelGGL((k_lines_assemble<TL>), dim3((K + 63) / 64, nrhs), dim3(64), 0, s, v, out, K, kface, kpos, xu,
-                       ny, yl, nx, k0, mode);
+    hipLaunchKernelGGL((k_lines_assemble<TL>), dim3((K + 3) / 4, nrhs), dim3(256), 0, s, v, out, K, kface, kpos, xu,
+                       ny, cp, nblk, nx, k0, mode);
     FP_HIP(hipGetLastError());
 }
 template void launch_lines_assemble<double>(const double*, double*, int, const int*, const int*, const double*, int,
-                                            const double*, int, int, hipStream_t, int, int);
+                                            const double*, int, int, int, hipStream_t, int, int);
 template void launch_lines_assemble<float>(const double*, double*, int, const int*, const int*, const float*, int,
-                                           const float*, int, int, hipStream_t, int, int);
+                                           const float*, int, int, int, hipStream_t, int, int);
 
 // Batched strided line copies: dst[r*ldd + t] = coef * src[r*lds + t*stride],
 // t < len, for up to 8 lines (interface lines handed between subdomains).
@@ -401,6 +410,88 @@ void launch_line_copy(const LineCopyBatchT<T>& b, int nrhs, hipStream_t s)
 }
 template void launch_line_copy<double>(const LineCopyBatchT<double>&, int, hipStream_t);
 template void launch_line_copy<float>(const LineCopyBatchT<float>&, int, hipStream_t);
+
+template <typename T>
+struct ExchangeBatch {
+    ColsumBatchT<T>   cs;
+    GemvBatchT<T>     gv;
+    LineCopyBatchT<T> lc;
+    int               start[25];  // first block of each job: colsums, products, copies
+    int               njobs;
+};
+template <typename T>
+__global__ __launch_bounds__(256) void k_exchange(ExchangeBatch<T> b)
+{
+    int q = 0;
+    while (q + 1 < b.njobs && (int)blockIdx.x >= b.start[q + 1]) q++;
+    const int       blk  = (int)blockIdx.x - b.start[q];
+    const int       wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const long long r    = blockIdx.y;
+    if (q < b.cs.n) {
+        const ColsumJobT<T>& J = b.cs.j[q];
+        const int            o = blk * 4 + wave;
+        if (o >= (J.half < 0 ? 2 : 1) * J.nx) return;
+        const T* src = J.cp + r * (long long)J.nblk * 2 * J.nx + o + (J.half > 0 ? J.nx : 0);
+        T        acc = T(0);
+        for (int k = lane; k < J.nblk; k += 64) acc += src[(long long)k * 2 * J.nx];
+        acc = wave_reduce_sum(acc);
+        if (lane == 0) J.yl[r * J.ldy + o] = acc;
+        return;
+    }
+    q -= b.cs.n;
+    if (q < b.gv.n) {
+        const GemvJobT<T>& J = b.gv.j[q];
+        const int          i = blk * 4 + wave;
+        if (i >= J.n) return;
+        const T* Ai  = J.A + (long long)i * J.n;
+        const T* xr  = J.x + r * J.ldx;
+        T        acc = T(0);
+        for (int k = lane; k < J.n; k += 64) acc += Ai[k] * xr[k];
+        acc = wave_reduce_sum(acc);
+        if (lane == 0) J.y[r * J.ldy + i] = acc;
+        return;
+    }
+    q -= b.gv.n;
+    const LineCopyT<T>& c = b.lc.c[q];
+    const int           t = blk * VB + threadIdx.x;
+    if (t < c.len) c.dst[r * c.ldd + t] = c.coef * c.src[r * c.lds + (long long)t * c.stride];
+}
+template <typename T>
+void launch_exchange(const ColsumBatchT<T>& cs, const GemvBatchT<T>& gv, const LineCopyBatchT<T>& lc, int nrhs,
+                     hipStream_t s)
+{
+    static_assert(VB == 256, "one block size for every exchange job");
+    ExchangeBatch<T> b{};
+    b.cs       = cs;
+    b.lc       = lc;
+    int blocks = 0, q = 0;
+    for (int k = 0; k < cs.n; k++, q++) {
+        b.start[q] = blocks;
+        blocks += ((cs.j[k].half < 0 ? 2 : 1) * cs.j[k].nx + 3) / 4;
+    }
+    if (nrhs >= kGemmMinRhs) {
+        launch_gemv_batch(gv, nrhs, s);
+    } else {
+        b.gv = gv;
+        for (int k = 0; k < gv.n; k++, q++) {
+            b.start[q] = blocks;
+            blocks += (gv.j[k].n + 3) / 4;
+        }
+    }
+    for (int k = 0; k < lc.n; k++, q++) {
+        b.start[q] = blocks;
+        blocks += (lc.c[k].len + VB - 1) / VB;
+    }
+    b.njobs    = q;
+    b.start[q] = blocks;
+    if (blocks == 0) return;
+    hipLaunchKernelGGL((k_exchange<T>), dim3(blocks, nrhs), dim3(VB), 0, s, b);
+    FP_HIP(hipGetLastError());
+}
+template void launch_exchange<double>(const ColsumBatchT<double>&, const GemvBatchT<double>&,
+                                      const LineCopyBatchT<double>&, int, hipStream_t);
+template void launch_exchange<float>(const ColsumBatchT<float>&, const GemvBatchT<float>&,
+                                     const LineCopyBatchT<float>&, int, hipStream_t);
 
 __global__ void k_to_f32(const double* src, float* dst, long long n)
 {
