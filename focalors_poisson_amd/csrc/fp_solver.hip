@@ -521,31 +521,44 @@ static void read_state(Node& nd, int nrhs, hipStream_t s)
 // side); the first residual check then usually ends it.
 static void gmres_stats(fp_cat_s* c, Node& nd, int nrhs, int restarts);
 
-static void gmres_iterate(fp_cat_s* c, Node& nd, int nrhs, hipStream_t s)
+// Returns true when the node's line buffer LB holds E S X for the final X
+// (the last operator application was the residual check of X by the
+// line-sourced matvec), so that the caller need not form it again.
+static bool gmres_iterate(fp_cat_s* c, Node& nd, int nrhs, hipStream_t s)
 {
     const int K = nd.K, m = c->m;
     if (nd.precond) launch_dense_gemv(nd.Minv.p, K, nd.C.p, nd.X.p, 1.0, 0.0, nrhs, s);
     else FP_HIP(hipMemcpyAsync(nd.X.p, nd.C.p, sizeof(double) * K * nrhs, hipMemcpyDeviceToDevice, s));
-    const GmresDev g = gdev(c, nd, nrhs);
-    gm_init(g, c->tol, c->tol_rel, s);
+    const GmresDev g     = gdev(c, nd, nrhs);
+    const bool     lines = !full_field_matvec() && (!nd.has_x || nd.spectral_x);
+    // the first residual check sets the tolerance in the same launch where
+    // the interface fits one workgroup
+    const bool fused = g.hstate && g.K <= (1 << 20);
+    if (!fused) gm_init(g, c->tol, c->tol_rel, s);
+    auto check = [&](bool first) {
+        matvec(c, nd, nd.X.p, nd.Wv.p, nrhs, s);
+        return (first && fused) ? gm_first(g, c->tol, c->tol_rel, s) : gm_resid(g, s);
+    };
     if (c->graph_mode) {
         // captured: the residual check at the start is published to host
         // memory and read after the replay; no host synchronisation here
-        matvec(c, nd, nd.X.p, nd.Wv.p, nrhs, s);
-        FP_REQUIRE(gm_resid(g, s), "graph capture needs the one-workgroup GMRES start");
-        return;
+        FP_REQUIRE(check(true), "graph capture needs the one-workgroup GMRES start");
+        return lines;
     }
     int  restarts = 0;
     bool fresh    = false;  // host copy of the state is current
+    bool lb_x     = false;
     for (int outer = 0; outer < c->maxiter; outer++) {
-        matvec(c, nd, nd.X.p, nd.Wv.p, nrhs, s);
-        if (gm_resid(g, s)) FP_HIP(hipStreamSynchronize(s));
+        if (check(outer == 0)) FP_HIP(hipStreamSynchronize(s));
         else read_state(nd, nrhs, s);
         fresh         = true;
         bool all_done = true;
         for (int r = 0; r < nrhs; r++)
             if (nd.h_state[4 * r + 2] == 0) all_done = false;
-        if (all_done) break;
+        if (all_done) {
+            lb_x = lines;
+            break;
+        }
         const long long ldv = (long long)nrhs * K;
         for (int j = 0; j < m; j++) {
             op(c, nd, nd.V.p + j * ldv, nd.Wv.p, nrhs, s);
@@ -574,6 +587,7 @@ static void gmres_iterate(fp_cat_s* c, Node& nd, int nrhs, hipStream_t s)
     }
     if (!fresh) read_state(nd, nrhs, s);
     gmres_stats(c, nd, nrhs, restarts);
+    return lb_x;
 }
 
 // Statistics of the last GMRES of node nd from the host copy of its state.
@@ -1355,11 +1369,13 @@ static void cat_sweep_fast_t(fp_cat_s* c, double* const* f, const double* const*
         launch_lines_assemble<T>(nullptr, R.C.p, R.K, R.kface.p, R.kpos.p, v.XU, nyr, v.CP, sweep_blocks(R.plan),
                                  nxr, nrhs, s, 0, 1);
     }
-    gmres_iterate(c, R, nrhs, s);
+    const bool lb_x = gmres_iterate(c, R, nrhs, s);
 
     // 4. the root's solution: sweep of F_r - child lines + E S y, written as
     //    coefficients, with its interface rows and column sums for the children
-    launch_schur_lines(schur_lines_args(c, R, R.X.p), nrhs, s);
+    //    (E S y is still in the line buffer when GMRES ended on the residual
+    //    check of y)
+    if (!lb_x) launch_schur_lines(schur_lines_args(c, R, R.X.p), nrhs, s);
     if constexpr (!f64) launch_to_f32(R.LB.p, w.LB.p, (long long)nrhs * R.ldl, s);
     {
         SweepJob<T> j{};

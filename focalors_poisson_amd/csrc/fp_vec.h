@@ -147,6 +147,10 @@ struct GmresDev {
 void gm_init(const GmresDev& g, double tol, int rel, hipStream_t s);
 void gm_publish(const GmresDev& g, int* hstate, double* hres, hipStream_t s);
 bool gm_resid(const GmresDev& g, hipStream_t s);
+// gm_init and the first gm_resid of a solve in one one-workgroup launch (the
+// state published like gm_resid's); false, with nothing launched, when K is
+// too large for one workgroup
+bool gm_first(const GmresDev& g, double tol, int rel, hipStream_t s);
 void gm_arnoldi(const GmresDev& g, int j, hipStream_t s);
 void gm_finish_cycle(const GmresDev& g, hipStream_t s);
 

@@ -1025,27 +1025,46 @@ __global__ __launch_bounds__(GW) void k_gm_init1(GmresDev g, double tol, int rel
 }
 
 // R = C - BX, beta = ||R||, V0 = R / beta, convergence test at cycle start
-// (gmres_solver2d.cpp:148-168)
-__global__ __launch_bounds__(GW) void k_gm_start1(GmresDev g)
+// (gmres_solver2d.cpp:148-168).  FIRST: the first check of a solve, which
+// also sets the tolerance from ||C|| and clears the state (k_gm_init1's work,
+// in the same pass over C).
+template <bool FIRST>
+__global__ __launch_bounds__(GW) void k_gm_start1(GmresDev g, double tol, int rel)
 {
     __shared__ double red[GW / 64];
     const int       r   = blockIdx.x;
     const long long K   = g.K;
-    double          acc = 0.0;
+    double          acc = 0.0, accc = 0.0;
     for (long long k = threadIdx.x; k < K; k += GW) {
-        const double t = g.C[r * K + k] - g.Wv[r * K + k];
+        const double c = g.C[r * K + k];
+        const double t = c - g.Wv[r * K + k];
         g.V[r * K + k] = t;
         acc += t * t;
+        if (FIRST) accc += c * c;
     }
     const double beta = sqrt(block_sum_fixed(acc, red));
+    double       tola;
+    if constexpr (FIRST) {
+        const double nc = sqrt(block_sum_fixed(accc, red));
+        tola            = rel ? tol * nc : tol;
+        if (threadIdx.x == 0) {
+            g.tolabs[r]        = tola;
+            g.rnorm[r]         = nc;
+            g.state[r * 4 + 2] = 0;
+            g.state[r * 4 + 3] = 0;
+        }
+    } else {
+        tola = g.tolabs[r];
+    }
     if (!isfinite(beta)) {
         if (threadIdx.x == 0) {
             g.state[r * 4 + 0] = 0;
+            g.state[r * 4 + 1] = 0;
             g.state[r * 4 + 2] = 2;  // numeric failure
             g.resid[r]         = beta;
         }
     } else {
-        const bool conv = beta < g.tolabs[r];
+        const bool conv = beta < tola;
         if (!conv)
             for (long long k = threadIdx.x; k < K; k += GW) g.V[r * K + k] *= (1.0 / beta);
         const int m = g.m;
@@ -1062,6 +1081,14 @@ __global__ __launch_bounds__(GW) void k_gm_start1(GmresDev g)
         g.hres[r]          = g.resid[r];
         g.hres[g.nrhs + r] = g.rnorm[r];
     }
+}
+
+bool gm_first(const GmresDev& g, double tol, int rel, hipStream_t s)
+{
+    if (g.K > (1 << 20)) return false;
+    hipLaunchKernelGGL(k_gm_start1<true>, dim3(g.nrhs), dim3(GW), 0, s, g, tol, rel);
+    FP_HIP(hipGetLastError());
+    return true;
 }
 
 void gm_init(const GmresDev& g, double tol, int rel, hipStream_t s)
@@ -1081,7 +1108,7 @@ void gm_init(const GmresDev& g, double tol, int rel, hipStream_t s)
 bool gm_resid(const GmresDev& g, hipStream_t s)
 {
     if (g.K <= (1 << 20)) {
-        hipLaunchKernelGGL(k_gm_start1, dim3(g.nrhs), dim3(GW), 0, s, g);
+        hipLaunchKernelGGL(k_gm_start1<false>, dim3(g.nrhs), dim3(GW), 0, s, g, 0.0, 0);
         FP_HIP(hipGetLastError());
         return g.hstate != nullptr;
     }
