@@ -120,6 +120,11 @@ struct SubPlan {
     void tune_tridiag(bool measure);
 };
 
+// Choices of the autotuners by key (fp_tune.cpp): process-wide, and shared
+// across processes through the file FP_TUNE_FILE names, if any.
+bool tune_lookup(const std::string& key, std::vector<int>& vals);
+void tune_store(const std::string& key, const std::vector<int>& vals);
+
 // ---- kernel launchers (fp_transform.hip / fp_tridiag.hip) ----
 // y transform of `nrhs` fields of one subdomain, in place.  inverse = 0: the
 // reference's transform; 1: transform_transpose.
@@ -190,7 +195,7 @@ int launch_tridiag_lines(const SubPlan& sp, const LinesArgs<T>& a, int nrhs, hip
 // side = (fin ? f : 0) + lines(lbp) - lines(lbm), line buffers packed per
 // right-hand side as [x0 (ny) | x1 (ny) | y0 (nx) | y1 (nx)] (x-lines as
 // transform coefficients, y-lines physical); outputs: the field (fout, in
-// place), solved rows 0 / nx-1 (xo0 / xo1, ldxo apart per right-hand side)
+// place or to fo), solved rows 0 / nx-1 (xo0 / xo1, ldxo apart per right-hand side)
 // and per-(rhs, mode block) partial column sums cp (2 nx each; the blocks per
 // right-hand side are sweep_blocks(sp)).
 template <typename T>
@@ -208,6 +213,7 @@ struct SweepJob {
     long long      ldxo  = 0;
     T*             cp    = nullptr;
     int            fin   = 1, fout = 1;
+    T*             fo    = nullptr;  // field output (fout) elsewhere than f, same ld
 };
 template <typename T>
 void launch_sweep_batch(const SweepJob<T>* jobs, int njobs, hipStream_t s);

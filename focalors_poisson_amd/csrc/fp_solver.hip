@@ -521,35 +521,37 @@ static void read_state(Node& nd, int nrhs, hipStream_t s)
 // side); the first residual check then usually ends it.
 static void gmres_stats(fp_cat_s* c, Node& nd, int nrhs, int restarts);
 
-// Returns true when the node's line buffer LB holds E S X for the final X
-// (the last operator application was the residual check of X by the
-// line-sourced matvec), so that the caller need not form it again.
-static bool gmres_iterate(fp_cat_s* c, Node& nd, int nrhs, hipStream_t s)
+// x0 = M C with the explicit-inverse preconditioner, else x0 = C; the
+// tolerance is set here unless the first residual check sets it (fused: the
+// interface fits one workgroup, see gm_first).
+static GmresDev gmres_begin(fp_cat_s* c, Node& nd, int nrhs, hipStream_t s, bool& fused)
 {
-    const int K = nd.K, m = c->m;
+    const int K = nd.K;
     if (nd.precond) launch_dense_gemv(nd.Minv.p, K, nd.C.p, nd.X.p, 1.0, 0.0, nrhs, s);
     else FP_HIP(hipMemcpyAsync(nd.X.p, nd.C.p, sizeof(double) * K * nrhs, hipMemcpyDeviceToDevice, s));
-    const GmresDev g     = gdev(c, nd, nrhs);
-    const bool     lines = !full_field_matvec() && (!nd.has_x || nd.spectral_x);
-    // the first residual check sets the tolerance in the same launch where
-    // the interface fits one workgroup
-    const bool fused = g.hstate && g.K <= (1 << 20);
+    const GmresDev g = gdev(c, nd, nrhs);
+    fused            = g.hstate && g.K <= (1 << 20);
     if (!fused) gm_init(g, c->tol, c->tol_rel, s);
-    auto check = [&](bool first) {
-        matvec(c, nd, nd.X.p, nd.Wv.p, nrhs, s);
-        return (first && fused) ? gm_first(g, c->tol, c->tol_rel, s) : gm_resid(g, s);
-    };
-    if (c->graph_mode) {
-        // captured: the residual check at the start is published to host
-        // memory and read after the replay; no host synchronisation here
-        FP_REQUIRE(check(true), "graph capture needs the one-workgroup GMRES start");
-        return lines;
-    }
-    int  restarts = 0;
-    bool fresh    = false;  // host copy of the state is current
-    bool lb_x     = false;
+    return g;
+}
+
+// The restarted iteration from the current X.  first_pending: no residual
+// check has run yet (the first one sets the tolerance when fused).  Returns
+// true when the node's line buffer LB holds E S X for the final X (the last
+// operator application was the residual check of X by the line-sourced
+// matvec), so that the caller need not form it again.
+static bool gmres_loop(fp_cat_s* c, Node& nd, int nrhs, hipStream_t s, const GmresDev& g, bool fused,
+                       bool first_pending)
+{
+    const int  K = nd.K, m = c->m;
+    const bool lines = !full_field_matvec() && (!nd.has_x || nd.spectral_x);
+    int        restarts = 0;
+    bool       fresh    = false;  // host copy of the state is current
+    bool       lb_x     = false;
     for (int outer = 0; outer < c->maxiter; outer++) {
-        if (check(outer == 0)) FP_HIP(hipStreamSynchronize(s));
+        matvec(c, nd, nd.X.p, nd.Wv.p, nrhs, s);
+        const bool pub = (first_pending && outer == 0 && fused) ? gm_first(g, c->tol, c->tol_rel, s) : gm_resid(g, s);
+        if (pub) FP_HIP(hipStreamSynchronize(s));
         else read_state(nd, nrhs, s);
         fresh         = true;
         bool all_done = true;
@@ -588,6 +590,21 @@ static bool gmres_iterate(fp_cat_s* c, Node& nd, int nrhs, hipStream_t s)
     if (!fresh) read_state(nd, nrhs, s);
     gmres_stats(c, nd, nrhs, restarts);
     return lb_x;
+}
+
+// Returns true when LB holds E S X for the final X (see gmres_loop).
+static bool gmres_iterate(fp_cat_s* c, Node& nd, int nrhs, hipStream_t s)
+{
+    bool           fused;
+    const GmresDev g = gmres_begin(c, nd, nrhs, s, fused);
+    if (c->graph_mode) {
+        // captured: the residual check at the start is published to host
+        // memory and read after the replay; no host synchronisation here
+        matvec(c, nd, nd.X.p, nd.Wv.p, nrhs, s);
+        FP_REQUIRE(fused && gm_first(g, c->tol, c->tol_rel, s), "graph capture needs the one-workgroup GMRES start");
+        return !full_field_matvec() && (!nd.has_x || nd.spectral_x);
+    }
+    return gmres_loop(c, nd, nrhs, s, g, fused, true);
 }
 
 // Statistics of the last GMRES of node nd from the host copy of its state.
@@ -718,7 +735,6 @@ static void build_node(fp_cat_s* c, int d, hipStream_t s)
         FP_HIP(hipStreamSynchronize(s));
     }
     if (nd->K) {
-        const int nx = c->nx[d], ny = c->ny[d];
         std::vector<int> kf, kp;
         for (auto& L : nd->ch) {
             nd->face_off[L.loc] = L.off;
@@ -1369,19 +1385,18 @@ static void cat_sweep_fast_t(fp_cat_s* c, double* const* f, const double* const*
         launch_lines_assemble<T>(nullptr, R.C.p, R.K, R.kface.p, R.kpos.p, v.XU, nyr, v.CP, sweep_blocks(R.plan),
                                  nxr, nrhs, s, 0, 1);
     }
-    const bool lb_x = gmres_iterate(c, R, nrhs, s);
-
     // 4. the root's solution: sweep of F_r - child lines + E S y, written as
     //    coefficients, with its interface rows and column sums for the children
-    //    (E S y is still in the line buffer when GMRES ended on the residual
-    //    check of y)
-    if (!lb_x) launch_schur_lines(schur_lines_args(c, R, R.X.p), nrhs, s);
-    if constexpr (!f64) launch_to_f32(R.LB.p, w.LB.p, (long long)nrhs * R.ldl, s);
-    {
+    //    (E S y is still in the line buffer when GMRES ended on a residual
+    //    check of y by the line-sourced matvec)
+    auto final_sweep = [&](bool schur, T* fo) {
+        if (schur) launch_schur_lines(schur_lines_args(c, R, R.X.p), nrhs, s);
+        if constexpr (!f64) launch_to_f32(R.LB.p, w.LB.p, (long long)nrhs * R.ldl, s);
         SweepJob<T> j{};
         j.sp   = &R.plan;
         j.nrhs = nrhs;
         j.f    = v.fld[root];
+        j.fo   = fo;
         j.ld   = nr;
         j.lbp  = v.LB;
         j.lbm  = w.LBc.p;
@@ -1394,7 +1409,40 @@ static void cat_sweep_fast_t(fp_cat_s* c, double* const* f, const double* const*
         j.fin  = 1;
         j.fout = 1;
         launch_sweep_batch<T>(&j, 1, s);
+    };
+    bool swept = false;
+    if constexpr (f64) {
+        if (R.precond) {
+            // FP64 with the explicit-inverse start x0 = M C: the final sweep
+            // runs on x0 at once, into the scratch field G so that F_r survives
+            // a restart, and doubles as the residual check.  Its interface
+            // values are R U_r = C + R A^-1 E S x0, so B x0 = x0 + C - R U_r
+            // and the first check costs no separate operator application.
+            bool           fused;
+            const GmresDev g = gmres_begin(c, R, nrhs, s, fused);
+            FP_REQUIRE(fused || !c->graph_mode, "graph capture needs the one-workgroup GMRES start");
+            if (fused) {
+                final_sweep(true, R.G.p);
+                launch_lines_assemble<double>(R.X.p, R.Wv.p, R.K, R.kface.p, R.kpos.p, R.XU.p, nyr, R.CP.p,
+                                              sweep_blocks(R.plan), nxr, nrhs, s, 0, 2, R.C.p);
+                gm_first(g, c->tol, c->tol_rel, s);
+                if (!c->graph_mode) {
+                    FP_HIP(hipStreamSynchronize(s));
+                    bool done = true;
+                    for (int r = 0; r < nrhs; r++)
+                        if (R.h_state[4 * r + 2] == 0) done = false;
+                    if (done) gmres_stats(c, R, nrhs, 0);
+                    else final_sweep(!gmres_loop(c, R, nrhs, s, g, fused, false), R.G.p);
+                }
+                iv[root].src = R.G.p;
+                iv[root].lds = nr;
+            } else {
+                final_sweep(!gmres_loop(c, R, nrhs, s, g, fused, true), nullptr);
+            }
+            swept = true;
+        }
     }
+    if (!swept) final_sweep(!gmres_iterate(c, R, nrhs, s), nullptr);
 
     // 5. the root's line next to each child into the child's right-hand side:
     //    x-lines from the solved rows (through Q^T when the bases differ),

@@ -1151,8 +1151,6 @@ void launch_ytransform(const TransformPlan<T>& tp, T* data, long long ld, int nx
 // transform of an nx-row scratch field (one right-hand side) with every
 // workgroup size its engine supports.  Cached per (kind, N, nx, precision)
 // for the process.
-static std::mutex                        g_tt_mu;
-static std::map<std::array<int, 4>, int> g_tt_cache;
 
 template <typename T>
 void tune_transform(TransformPlan<T>& tp, int nx, bool measure)
@@ -1161,12 +1159,12 @@ void tune_transform(TransformPlan<T>& tp, int nx, bool measure)
     for (int t : {128, 256, 512})
         if (tp.engine == 0 ? t == 256 : sb_fits(tp.rad, tp.nrad, tp.L, t, sizeof(T) == 8)) cand.push_back(t);
     if (!measure || cand.size() < 2 || (long long)nx * tp.N < (1LL << 16)) return;
-    const std::array<int, 4> key{tp.kind, tp.N, nx, (int)sizeof(T)};
+    const std::string key = "yt " + std::to_string(tp.kind) + " " + std::to_string(tp.N) + " " +
+                            std::to_string(nx) + " " + std::to_string(sizeof(T));
     {
-        std::lock_guard<std::mutex> lk(g_tt_mu);
-        auto                        it = g_tt_cache.find(key);
-        if (it != g_tt_cache.end()) {
-            tp.nt = it->second;
+        std::vector<int> v;
+        if (tune_lookup(key, v) && v.size() == 1) {
+            tp.nt = v[0];
             return;
         }
     }
@@ -1194,8 +1192,7 @@ void tune_transform(TransformPlan<T>& tp, int nx, bool measure)
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     tp.nt = nt_best;
-    std::lock_guard<std::mutex> lk(g_tt_mu);
-    g_tt_cache[key] = nt_best;
+    tune_store(key, {nt_best});
 }
 // Threads per workgroup for a batch of transforms launched together (the
 // domains of a composite solve): every job of the single-buffer engine takes
@@ -1207,6 +1204,13 @@ template <typename T>
 int tune_batch_nt(std::vector<TransformJob<T>> jobs, hipStream_t s)
 {
     if (jobs.size() < 2) return 0;
+    std::string key = "ytb " + std::to_string(sizeof(T));
+    for (const auto& j : jobs)
+        key += " " + std::to_string(j.plan->kind) + ":" + std::to_string(j.plan->N) + ":" + std::to_string(j.nx);
+    {
+        std::vector<int> v;
+        if (tune_lookup(key, v) && v.size() == 1) return v[0];
+    }
     hipEvent_t e0, e1;
     FP_HIP(hipEventCreate(&e0));
     FP_HIP(hipEventCreate(&e1));
@@ -1234,6 +1238,7 @@ int tune_batch_nt(std::vector<TransformJob<T>> jobs, hipStream_t s)
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    tune_store(key, {nt_best});
     return nt_best;
 }
 template int tune_batch_nt<double>(std::vector<TransformJob<double>>, hipStream_t);

@@ -416,6 +416,7 @@ __global__ __launch_bounds__(tri_threads<CH>()) void k_tridiag_lines(LinesArgs<T
 template <typename T>
 struct SweepProb {
     T*            f;
+    T*            fo;
     long long     ld;
     const T*      lbp;
     const T*      lbm;
@@ -534,9 +535,10 @@ __global__ __launch_bounds__(tri_threads<CH>()) void k_sweep(SweepBatch<T> bt)
         }
     if (act) {
         if (pr.fout) {
+            T* fo = pr.fo + (long long)r * pr.ld;
 #pragma unroll
             for (int k = 0; k < CH; k++)
-                if (i0 + k < nx) f[(long long)(i0 + k) * ny + j] = fv[k];
+                if (i0 + k < nx) fo[(long long)(i0 + k) * ny + j] = fv[k];
         }
         if (pr.xo0 && i0 == 0) pr.xo0[(long long)r * pr.ldxo + j] = fv[0];
         if (pr.xo1) {
@@ -691,6 +693,7 @@ static void launch_sweep_group(const SweepJob<T>* const* jobs, int n, hipStream_
         const int MB = tri_mb(P, CH, sp.mb_field);
         SweepProb<T>& pr = bt.p[q];
         pr.f    = jb.f;
+        pr.fo   = jb.fo ? jb.fo : jb.f;
         pr.ld   = jb.ld;
         pr.lbp  = jb.lbp;
         pr.lbm  = jb.lbm;
@@ -792,12 +795,10 @@ static int default_chunk(int nx)
 struct TriTune {
     int ch, mb;
 };
-static std::mutex                               g_tune_mu;
-static std::map<std::pair<int, int>, TriTune>   g_tri_tuned;
 
 // Chooses (chunk length, modes per workgroup) for the field sweep of this
 // plan by timing every candidate on a scratch field of the plan's size (one
-// right-hand side).  Results are cached per (nx, ny) for the process.
+// right-hand side).  Choices are recorded per (nx, ny) (tune_store).
 void SubPlan::tune_tridiag(bool measure)
 {
     if (!measure || (long long)nx * ny < (1LL << 16)) {
@@ -805,13 +806,12 @@ void SubPlan::tune_tridiag(bool measure)
         mb_field = 4;
         return;
     }
-    const std::pair<int, int> key(nx, ny);
+    const std::string key = "tri " + std::to_string(nx) + " " + std::to_string(ny);
     {
-        std::lock_guard<std::mutex> lk(g_tune_mu);
-        auto                        it = g_tri_tuned.find(key);
-        if (it != g_tri_tuned.end()) {
-            set_chunks(it->second.ch);
-            mb_field = it->second.mb;
+        std::vector<int> v;
+        if (tune_lookup(key, v) && v.size() == 2) {
+            set_chunks(v[0]);
+            mb_field = v[1];
             return;
         }
     }
@@ -844,8 +844,7 @@ void SubPlan::tune_tridiag(bool measure)
     (void)hipEventDestroy(e1);
     set_chunks(best.ch);
     mb_field = best.mb;
-    std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tri_tuned[key] = best;
+    tune_store(key, {best.ch, best.mb});
 }
 
 template void launch_tridiag<double>(const SubPlan&, double*, long long, int, hipStream_t);

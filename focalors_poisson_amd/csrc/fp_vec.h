@@ -81,11 +81,13 @@ using GemvBatch = GemvBatchT<double>;
 template <typename T>
 void launch_gemv_batch(const GemvBatchT<T>& b, int nrhs, hipStream_t s);
 // mode 0: out = v - t (v == nullptr: the unit vector e_{k0 + r} of right-hand
-// side r); mode 1: out = t; t from the x rows xu and the column sums of the
-// partials cp (nblk blocks of 2 nx per rhs; unused without y-lines)
+// side r); mode 1: out = t; mode 2: out = v + w - t; t from the x rows xu and
+// the column sums of the partials cp (nblk blocks of 2 nx per rhs; unused
+// without y-lines)
 template <typename TL>
 void launch_lines_assemble(const double* v, double* out, int K, const int* kface, const int* kpos, const TL* xu,
-                           int ny, const TL* cp, int nblk, int nx, int nrhs, hipStream_t s, int k0 = 0, int mode = 0);
+                           int ny, const TL* cp, int nblk, int nx, int nrhs, hipStream_t s, int k0 = 0, int mode = 0,
+                           const double* w = nullptr);
 template <typename T>
 struct LineCopyT {
     const T*  src;

@@ -349,11 +349,11 @@ template void launch_gemv_batch<float>(const GemvBatchT<float>&, int, hipStream_
 // of unknown k, the latter the total of the sweep's per-workgroup partial
 // column sums (cp, nblk blocks of 2 nx per right-hand side), reduced here by
 // one wavefront per unknown.  mode 0: out = v - t (v == nullptr: the unit
-// vector e_{k0 + r}); mode 1: out = t.
+// vector e_{k0 + r}); mode 1: out = t; mode 2: out = v + w - t.
 template <typename TL>
 __global__ __launch_bounds__(256) void k_lines_assemble(const double* v, double* out, int K, const int* kface,
                                                         const int* kpos, const TL* xu, int ny, const TL* cp, int nblk,
-                                                        int nx, int k0, int mode)
+                                                        int nx, int k0, int mode, const double* w)
 {
     const int k = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (k >= K) return;
@@ -373,21 +373,27 @@ __global__ __launch_bounds__(256) void k_lines_assemble(const double* v, double*
         out[r * K + k] = t;
         return;
     }
+    if (mode == 2) {
+        out[r * K + k] = v[r * K + k] + w[r * K + k] - t;
+        return;
+    }
     const double vv = v ? v[r * K + k] : (k == k0 + (int)r ? 1.0 : 0.0);
     out[r * K + k]  = vv - t;
 }
 template <typename TL>
 void launch_lines_assemble(const double* v, double* out, int K, const int* kface, const int* kpos, const TL* xu,
-                           int ny, const TL* cp, int nblk, int nx, int nrhs, hipStream_t s, int k0, int mode)
+                           int ny, const TL* cp, int nblk, int nx, int nrhs, hipStream_t s, int k0, int mode,
+                           const double* w)
 {
+    if (mode == 2 && (!v || !w)) throw Error(FP_E_INVALID, "lines_assemble mode 2 needs v and w");
     hipLaunchKernelGGL((k_lines_assemble<TL>), dim3((K + 3) / 4, nrhs), dim3(256), 0, s, v, out, K, kface, kpos, xu,
-                       ny, cp, nblk, nx, k0, mode);
+                       ny, cp, nblk, nx, k0, mode, w);
     FP_HIP(hipGetLastError());
 }
 template void launch_lines_assemble<double>(const double*, double*, int, const int*, const int*, const double*, int,
-                                            const double*, int, int, int, hipStream_t, int, int);
+                                            const double*, int, int, int, hipStream_t, int, int, const double*);
 template void launch_lines_assemble<float>(const double*, double*, int, const int*, const int*, const float*, int,
-                                           const float*, int, int, int, hipStream_t, int, int);
+                                           const float*, int, int, int, hipStream_t, int, int, const double*);
 
 // Batched strided line copies: dst[r*ldd + t] = coef * src[r*lds + t*stride],
 // t < len, for up to 8 lines (interface lines handed between subdomains).
