@@ -187,6 +187,11 @@ __device__ __forceinline__ void bfly(const C2<T> (&v)[R], C2<T> (&o)[R])
 #define FP_ZPAD 4
 #endif
 constexpr int kZpad = FP_ZPAD;
+#ifndef FP_ODD_MFMA
+#define FP_ODD_MFMA 1
+#endif
+// FP64 odd-prime passes on the matrix cores (else the VALU pair form)
+constexpr bool kOddMfma64 = FP_ODD_MFMA;
 template <int PAD>
 __device__ __forceinline__ int zp(int i)
 {
@@ -461,7 +466,7 @@ __device__ __forceinline__ void run_fft_sb(const YArgs<T>& a, const Tw<T> W, C2<
         else if (R == 4) pass_reg_sb<4, FWD>(Z, W, a.L, Ns);
         else if (R == 5) pass_reg_sb<5, FWD>(Z, W, a.L, Ns);
         else if (R == 8) pass_reg_sb<8, FWD>(Z, W, a.L, Ns);
-        else if constexpr (sizeof(T) == 8) {
+        else if constexpr (sizeof(T) == 8 && kOddMfma64) {
             pass_odd_mfma<FWD>(Z, W, a.L, Ns, R, fragL + fo);
             fo += 2 * 2 * mfma_ksteps(R) * 64;
         } else {
@@ -972,7 +977,7 @@ void TransformPlan<T>::build(int k, int n)
     }
     // FP64: matrix-core fragments [cos/sin][M tile][K step][lane] of every odd-prime pass
     std::vector<double> fr;
-    if (sizeof(T) == 8) {
+    if (sizeof(T) == 8 && kOddMfma64) {
         ncs = 0;
         cs.clear();
         for (int s = 0; s < nrad; s++) {
@@ -1028,7 +1033,7 @@ void TransformPlan<T>::build(int k, int n)
     engine = 0;
     nt     = 256;
     for (int t : {256, 512, 128})
-        if (sb_fits(rad, nrad, L, t, sizeof(T) == 8)) {
+        if (sb_fits(rad, nrad, L, t, sizeof(T) == 8 && kOddMfma64)) {
             engine = 1;
             nt     = t;
             break;
@@ -1148,16 +1153,18 @@ void launch_ytransform(const TransformPlan<T>& tp, T* data, long long ld, int nx
 }
 
 // Threads per workgroup of a plan's transform, chosen by timing the forward
-// transform of an nx-row scratch field (one right-hand side) with every
-// workgroup size its engine supports.  Cached per (kind, N, nx, precision)
-// for the process.
+// transform of an nx-row scratch field with every workgroup size its engine
+// supports.  The field holds enough right-hand sides for about 32 MB, so
+// that the candidates differ by throughput rather than by launch latency
+// (where they tie within noise), and each candidate keeps its best of three
+// rounds.  Recorded per (kind, N, nx, precision) (tune_store).
 
 template <typename T>
 void tune_transform(TransformPlan<T>& tp, int nx, bool measure)
 {
     std::vector<int> cand;
     for (int t : {128, 256, 512})
-        if (tp.engine == 0 ? t == 256 : sb_fits(tp.rad, tp.nrad, tp.L, t, sizeof(T) == 8)) cand.push_back(t);
+        if (tp.engine == 0 ? t == 256 : sb_fits(tp.rad, tp.nrad, tp.L, t, sizeof(T) == 8 && kOddMfma64)) cand.push_back(t);
     if (!measure || cand.size() < 2 || (long long)nx * tp.N < (1LL << 16)) return;
     const std::string key = "yt " + std::to_string(tp.kind) + " " + std::to_string(tp.N) + " " +
                             std::to_string(nx) + " " + std::to_string(sizeof(T));
@@ -1168,27 +1175,30 @@ void tune_transform(TransformPlan<T>& tp, int nx, bool measure)
             return;
         }
     }
-    DevBuf<T> buf((size_t)nx * tp.N);
-    FP_HIP(hipMemset(buf.p, 0, (size_t)nx * tp.N * sizeof(T)));
+    const long long n1   = (long long)nx * tp.N;
+    const int       nrhs = (int)std::max<long long>(1, std::min<long long>(64, (32LL << 20) / (n1 * sizeof(T))));
+    DevBuf<T>       buf((size_t)(n1 * nrhs));
+    FP_HIP(hipMemset(buf.p, 0, (size_t)(n1 * nrhs) * sizeof(T)));
     hipEvent_t e0, e1;
     FP_HIP(hipEventCreate(&e0));
     FP_HIP(hipEventCreate(&e1));
-    float best    = 1e30f;
-    int   nt_best = tp.nt;
-    for (int nt : cand) {
-        tp.nt = nt;
-        launch_ytransform<T>(tp, buf.p, (long long)nx * tp.N, nx, 1, 0, nullptr);
-        FP_HIP(hipEventRecord(e0, nullptr));
-        for (int r = 0; r < 5; r++) launch_ytransform<T>(tp, buf.p, (long long)nx * tp.N, nx, 1, 0, nullptr);
-        FP_HIP(hipEventRecord(e1, nullptr));
-        FP_HIP(hipEventSynchronize(e1));
-        float ms = 0.f;
-        FP_HIP(hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best) {
-            best    = ms;
-            nt_best = nt;
+    std::vector<float> best_of(cand.size(), 1e30f);
+    for (int round = 0; round < 3; round++)
+        for (size_t ci = 0; ci < cand.size(); ci++) {
+            tp.nt = cand[ci];
+            launch_ytransform<T>(tp, buf.p, n1, nx, nrhs, 0, nullptr);
+            FP_HIP(hipEventRecord(e0, nullptr));
+            for (int r = 0; r < 3; r++) launch_ytransform<T>(tp, buf.p, n1, nx, nrhs, 0, nullptr);
+            FP_HIP(hipEventRecord(e1, nullptr));
+            FP_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            FP_HIP(hipEventElapsedTime(&ms, e0, e1));
+            best_of[ci] = std::min(best_of[ci], ms);
         }
-    }
+    size_t ib = 0;
+    for (size_t ci = 1; ci < cand.size(); ci++)
+        if (best_of[ci] < best_of[ib]) ib = ci;
+    const int nt_best = cand[ib];
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     tp.nt = nt_best;
@@ -1214,28 +1224,33 @@ int tune_batch_nt(std::vector<TransformJob<T>> jobs, hipStream_t s)
     hipEvent_t e0, e1;
     FP_HIP(hipEventCreate(&e0));
     FP_HIP(hipEventCreate(&e1));
-    float best    = 1e30f;
-    int   nt_best = 0;
-    for (int cand : {0, 128, 256, 512}) {
-        bool any = false;
-        for (auto& j : jobs) {
-            const TransformPlan<T>& tp = *j.plan;
-            j.nt = (cand && tp.engine == 1 && sb_fits(tp.rad, tp.nrad, tp.L, cand, sizeof(T) == 8)) ? cand : 0;
-            any |= j.nt != 0;
+    const int          cands[4] = {0, 128, 256, 512};
+    std::vector<float> best_of(4, 1e30f);
+    for (int round = 0; round < 3; round++)
+        for (int ci = 0; ci < 4; ci++) {
+            const int cand = cands[ci];
+            bool      any  = false;
+            for (auto& j : jobs) {
+                const TransformPlan<T>& tp = *j.plan;
+                j.nt = (cand && tp.engine == 1 && sb_fits(tp.rad, tp.nrad, tp.L, cand, sizeof(T) == 8 && kOddMfma64))
+                           ? cand
+                           : 0;
+                any |= j.nt != 0;
+            }
+            if (cand && !any) continue;
+            launch_ytransform_batch<T>(jobs.data(), (int)jobs.size(), 0, s);
+            FP_HIP(hipEventRecord(e0, s));
+            for (int r = 0; r < 5; r++) launch_ytransform_batch<T>(jobs.data(), (int)jobs.size(), 0, s);
+            FP_HIP(hipEventRecord(e1, s));
+            FP_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            FP_HIP(hipEventElapsedTime(&ms, e0, e1));
+            best_of[ci] = std::min(best_of[ci], ms);
         }
-        if (cand && !any) continue;
-        launch_ytransform_batch<T>(jobs.data(), (int)jobs.size(), 0, s);
-        FP_HIP(hipEventRecord(e0, s));
-        for (int r = 0; r < 5; r++) launch_ytransform_batch<T>(jobs.data(), (int)jobs.size(), 0, s);
-        FP_HIP(hipEventRecord(e1, s));
-        FP_HIP(hipEventSynchronize(e1));
-        float ms = 0.f;
-        FP_HIP(hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best) {
-            best    = ms;
-            nt_best = cand;
-        }
-    }
+    int ib = 0;
+    for (int ci = 1; ci < 4; ci++)
+        if (best_of[ci] < best_of[ib]) ib = ci;
+    const int nt_best = cands[ib];
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     tune_store(key, {nt_best});
