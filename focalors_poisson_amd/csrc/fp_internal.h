@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -85,13 +86,14 @@ struct TransformPlan {
     int        nrad = 0;
     int        rad[kMaxRadices] = {};
     int        engine = 1;  // 1: single-buffer FFT, 0: two-buffer fallback (prime factors above 47)
-    int        ncs  = 0;  // cos/sin entries of the odd-prime passes (FP32)
-    DevBuf<double> frag;  // FP64 odd-prime passes: matrix-core operand fragments
+    int        ncs  = 0;  // cos/sin entries of the odd-prime passes (VALU form)
+    DevBuf<double> frag;  // FP64 odd-prime passes: matrix-core operand fragments (matrix-core form)
     DevBuf<T>  W;         // twiddles: 64 of W^m, then L/64+1 of W^(64m), W = exp(-2 pi i / L), then
                           // (cos, sin)(2 pi m / R) of every odd-prime pass R
     DevBuf<T>  aux;       // 2N: NN phase exp(-i pi m / 2N) or ND (sin, cos) of 2N pi (j+1)/(2N+1)
     T          c0 = 0, c1 = 0;
     int        nt = 256;  // threads per workgroup (tuned)
+    int        odd = 0;   // odd-prime pass form: 0 none, 1 matrix cores (FP64), 2 VALU pair sums (tuned)
     void build(int kind, int N);
 };
 // Times the plan's transform on an nx-row field and keeps the fastest
@@ -156,14 +158,16 @@ struct TransformJob {
     double                  hx2   = 1.0;
     const double*           g[4]  = {nullptr, nullptr, nullptr, nullptr};
     double                  ga[4] = {0.0, 0.0, 0.0, 0.0};
-    int                     nt    = 0;  // threads per workgroup (0: the plan's)
+    int                     nt    = 0;   // threads per workgroup (0: the plan's)
+    int                     odd   = -1;  // odd-prime pass form (-1: the plan's)
 };
 template <typename T>
 void launch_ytransform_batch(const TransformJob<T>* jobs, int njobs, int inverse, hipStream_t s);
-// One thread count for a batch launched together (0: every job its own),
-// chosen by timing the forward batch; the jobs' fields are overwritten.
+// One launch shape {threads per workgroup, odd-prime form} for a batch
+// launched together ({0, -1}: every job its own), chosen by timing the
+// forward batch; the jobs' fields are overwritten.
 template <typename T>
-int tune_batch_nt(std::vector<TransformJob<T>> jobs, hipStream_t s);
+std::array<int, 2> tune_batch_shape(std::vector<TransformJob<T>> jobs, hipStream_t s);
 template <typename T>
 struct TridiagJob {
     const SubPlan* sp;

@@ -269,9 +269,10 @@ struct fp_cat_s {
         if (gexec) (void)hipGraphExecDestroy(gexec);
     }
     int                                    precond_kind = 1;  // 0: none, 1: dense inverse of the interface operator
-    // threads per workgroup shared by the batched transforms of all domains
-    // (0: each plan's own; -1: not chosen yet)
-    int                                    yt_nt64 = -1, yt_nt32 = -1;
+    // launch shape {threads per workgroup, odd-prime form} shared by the
+    // batched transforms of all domains ({0, -1}: each plan's own; threads
+    // -1: not chosen yet)
+    std::array<int, 2>                     yt_shape64{-1, -1}, yt_shape32{-1, -1};
     fp::FastLines<double>                  fast;
     fp::FastLines<float>                   fast32;
     // captured solve (fp_cat_graph_capture): replays the two-level sweep with
@@ -1241,15 +1242,15 @@ static FastView<float> fast_view(fp_cat_s* c, double* const*, float*)
     return v;
 }
 
-// The thread count shared by the batched transforms of every domain, so that
+// The launch shape shared by the batched transforms of every domain, so that
 // each direction is one launch whose domains run side by side: timed once per
 // solver and precision on zeroed scratch fields.
 template <typename T>
-static int batch_nt(fp_cat_s* c)
+static std::array<int, 2> batch_shape(fp_cat_s* c)
 {
-    int& v = sizeof(T) == 8 ? c->yt_nt64 : c->yt_nt32;
-    if (v >= 0) return v;
-    if (c->nd < 2) return v = 0;
+    std::array<int, 2>& v = sizeof(T) == 8 ? c->yt_shape64 : c->yt_shape32;
+    if (v[0] >= 0) return v;
+    if (c->nd < 2) return v = {0, -1};
     std::vector<DevBuf<T>>       buf(c->nd);
     std::vector<TransformJob<T>> jobs(c->nd);
     for (int d = 0; d < c->nd; d++) {
@@ -1266,7 +1267,7 @@ static int batch_nt(fp_cat_s* c)
         jobs[d].nrhs  = 1;
         jobs[d].dst64 = sizeof(T) == 8;
     }
-    v = tune_batch_nt<T>(jobs, nullptr);
+    v = tune_batch_shape<T>(jobs, nullptr);
     FP_HIP(hipDeviceSynchronize());
     return v;
 }
@@ -1285,7 +1286,7 @@ static void cat_sweep_fast_t(fp_cat_s* c, double* const* f, const double* const*
     } else if constexpr (!f64)
         R.plan.ensure_f32();
     const FastView<T> v  = fast_view(c, f, (T*)nullptr);
-    const int         nt = batch_nt<T>(c);
+    const auto        shape = batch_shape<T>(c);
     // 1. forward transforms of every domain (FP64: in place, with the
     //    boundary assembly and scaling fused into the load; FP32: from the
     //    assembled FP64 field into the FP32 scratch)
@@ -1299,7 +1300,8 @@ static void cat_sweep_fast_t(fp_cat_s* c, double* const* f, const double* const*
         a.nx    = c->nx[d];
         a.nrhs  = nrhs;
         a.dst64 = f64;
-        a.nt    = nt;
+        a.nt    = shape[0];
+        a.odd   = shape[1];
         b       = a;
         a.data  = v.fld[d];
         a.ld    = n;
@@ -1717,8 +1719,8 @@ int fp_cat_create(fp_ctx ctx, const fp_geometry_desc* g, int m, double tol, int 
         for (int l = c->nlev - 1; l >= 0; l--)
             for (int d : c->levels[l]) fp::build_node(c.get(), d, s);
         if (nd > 1 && fp::fast_shape_ok(c.get())) {
-            if (precision == FP_F64) fp::batch_nt<double>(c.get());
-            else fp::batch_nt<float>(c.get());
+            if (precision == FP_F64) fp::batch_shape<double>(c.get());
+            else fp::batch_shape<float>(c.get());
         }
         FP_HIP(hipDeviceSynchronize());
         *out = c.release();

@@ -187,11 +187,9 @@ __device__ __forceinline__ void bfly(const C2<T> (&v)[R], C2<T> (&o)[R])
 #define FP_ZPAD 4
 #endif
 constexpr int kZpad = FP_ZPAD;
-#ifndef FP_ODD_MFMA
-#define FP_ODD_MFMA 1
-#endif
-// FP64 odd-prime passes on the matrix cores (else the VALU pair form)
-constexpr bool kOddMfma64 = FP_ODD_MFMA;
+// Form of the odd-prime (7..47) passes, a kernel template parameter chosen
+// per plan: none in the radix schedule, matrix cores (FP64), VALU pair sums.
+enum OddForm : int { ODD_NONE = 0, ODD_MFMA = 1, ODD_VALU = 2 };
 template <int PAD>
 __device__ __forceinline__ int zp(int i)
 {
@@ -454,7 +452,7 @@ __device__ __forceinline__ void pass_odd_mfma(C2<double>* __restrict__ Z, const 
     __syncthreads();
 }
 
-template <bool FWD, typename T>
+template <bool FWD, int OM, typename T>
 __device__ __forceinline__ void run_fft_sb(const YArgs<T>& a, const Tw<T> W, C2<T>* Z, const C2<T>* cs,
                                            const double* fragL)
 {
@@ -466,10 +464,10 @@ __device__ __forceinline__ void run_fft_sb(const YArgs<T>& a, const Tw<T> W, C2<
         else if (R == 4) pass_reg_sb<4, FWD>(Z, W, a.L, Ns);
         else if (R == 5) pass_reg_sb<5, FWD>(Z, W, a.L, Ns);
         else if (R == 8) pass_reg_sb<8, FWD>(Z, W, a.L, Ns);
-        else if constexpr (sizeof(T) == 8 && kOddMfma64) {
+        else if constexpr (OM == ODD_MFMA && sizeof(T) == 8) {
             pass_odd_mfma<FWD>(Z, W, a.L, Ns, R, fragL + fo);
             fo += 2 * 2 * mfma_ksteps(R) * 64;
-        } else {
+        } else if constexpr (OM == ODD_VALU) {
             pass_odd_sb<FWD>(Z, W, a.L, Ns, R, cs + co);
             co += ((R - 1) / 2) * (((R - 1) / 2 + kNQ - 1) / kNQ * kNQ);
         }
@@ -816,7 +814,7 @@ __device__ __forceinline__ const YArgs<T>& find_problem(const YBatch<T>& bt, int
 // threads per workgroup: chosen per plan (128, 256 or 512)
 constexpr int kMaxNT = 512;
 
-template <typename T>
+template <typename T, int OM>
 __global__ __launch_bounds__(kMaxNT) void k_ytransform_sb(YBatch<T> bt)
 {
     int             local;
@@ -839,8 +837,8 @@ __global__ __launch_bounds__(kMaxNT) void k_ytransform_sb(YBatch<T> bt)
 
     load_rows<kZpad>(a, rp, Z, y0);
     __syncthreads();
-    if (!(a.kind == T_NN && a.inverse)) run_fft_sb<true>(a, W, Z, cs, fr);
-    else run_fft_sb<false>(a, W, Z, cs, fr);
+    if (!(a.kind == T_NN && a.inverse)) run_fft_sb<true, OM>(a, W, Z, cs, fr);
+    else run_fft_sb<false, OM>(a, W, Z, cs, fr);
     store_rows<kZpad>(a, rp, Z, y0, S, S + a.N / 2 + 1);
 }
 
@@ -919,8 +917,9 @@ static int sb_bpt_rt(int R)
         default: return 0;
     }
 }
-static bool sb_fits(const int* rad, int nrad, int L, int NT, bool mfma)
+static bool sb_fits(const int* rad, int nrad, int L, int NT, int odd)
 {
+    const bool mfma = odd == ODD_MFMA;
     for (int s = 0; s < nrad; s++) {
         const int R = rad[s], nb = L / R;
         if (R <= 8 && R != 7) {
@@ -937,14 +936,25 @@ static bool sb_fits(const int* rad, int nrad, int L, int NT, bool mfma)
     return true;
 }
 
+// odd-prime forms a plan may use, preferred first
 template <typename T>
-static size_t lds_bytes(const TransformPlan<T>& tp)
+static std::vector<int> odd_forms(bool has_odd)
 {
+    if (!has_odd) return {ODD_NONE};
+    if (sizeof(T) == 8) return {ODD_MFMA, ODD_VALU};
+    return {ODD_VALU};
+}
+
+template <typename T>
+static size_t lds_bytes(const TransformPlan<T>& tp, int odd)
+{
+    const size_t ncs   = odd == ODD_VALU ? (size_t)tp.ncs : 0;
+    const size_t nfrag = odd == ODD_MFMA ? tp.frag.n : 0;
     const size_t nhi = tp.L / 64 + 1;
     const size_t zl  = (size_t)tp.L + (kZpad ? ((size_t)tp.L >> kZpad) : 0) + 1;
     const size_t sc  = tp.kind == T_DD ? (size_t)tp.N / 2 + 1 + 8 : 0;  // prefix sums + wavefront totals
     if (tp.engine == 1)
-        return (zl + 64 + nhi + tp.ncs + sc) * sizeof(C2<T>) + tp.frag.n * sizeof(double) + 4 * sizeof(T);
+        return (zl + 64 + nhi + ncs + sc) * sizeof(C2<T>) + nfrag * sizeof(double) + 4 * sizeof(T);
     const size_t scg = gen_scan_in_buffer(tp.L, tp.N) ? 0 : sc;
     return (2 * (size_t)tp.L + 64 + nhi + scg) * sizeof(C2<T>) + 4 * sizeof(T);
 }
@@ -977,9 +987,7 @@ void TransformPlan<T>::build(int k, int n)
     }
     // FP64: matrix-core fragments [cos/sin][M tile][K step][lane] of every odd-prime pass
     std::vector<double> fr;
-    if (sizeof(T) == 8 && kOddMfma64) {
-        ncs = 0;
-        cs.clear();
+    if (sizeof(T) == 8) {
         for (int s = 0; s < nrad; s++) {
             const int R = rad[s];
             if (R <= 8 && R != 7) continue;
@@ -1028,36 +1036,48 @@ void TransformPlan<T>::build(int k, int n)
         for (int j = 0; j < L; j++) x[2 * j] = (T)sinl(PI * (long double)j / (long double)L);
         c0 = (T)std::sqrt(2.0 / (N + 1.0));
     }
-    // engine: single buffer at 256 threads when it fits (the tuner may change
-    // the thread count among the sizes that fit), else the two-buffer fallback
-    engine = 0;
-    nt     = 256;
-    for (int t : {256, 512, 128})
-        if (sb_fits(rad, nrad, L, t, sizeof(T) == 8 && kOddMfma64)) {
-            engine = 1;
-            nt     = t;
-            break;
-        }
-    if (lds_bytes(*this) > 160 * 1024) {
-        // the fallback needs twice the LDS: keep the single buffer if it fits
-        if (engine == 0) throw Error(FP_E_INVALID, "transform length exceeds the LDS of one workgroup");
-    }
-    if (engine == 1 && lds_bytes(*this) > 160 * 1024)
-        throw Error(FP_E_INVALID, "transform length exceeds the LDS of one workgroup");
     W.alloc(w.size());
     W.upload(w.data(), w.size());
     frag.alloc(fr.size());
     if (!fr.empty()) frag.upload(fr.data(), fr.size());
     aux.alloc(x.size());
     aux.upload(x.data(), x.size());
+    // engine: single buffer at 256 threads when it fits (the tuner may change
+    // the thread count and the odd-prime form among those that fit), else
+    // the two-buffer fallback
+    bool has_odd = false;
+    for (int s = 0; s < nrad; s++) has_odd |= rad[s] == 7 || rad[s] > 8;
+    engine = 0;
+    nt     = 256;
+    odd    = ODD_NONE;
+    for (int form : odd_forms<T>(has_odd)) {
+        for (int t : {256, 512, 128})
+            if (sb_fits(rad, nrad, L, t, form) && lds_bytes(*this, form) <= 160 * 1024) {
+                engine = 1;
+                nt     = t;
+                odd    = form;
+                break;
+            }
+        if (engine == 1) break;
+    }
 }
 
 template struct TransformPlan<double>;
 template struct TransformPlan<float>;
 
+// odd-prime form a job runs with: the job's override, else its plan's
+template <typename T>
+static int job_form(const TransformJob<T>& j)
+{
+    const TransformPlan<T>& tp = *j.plan;
+    if (tp.engine != 1) return 0;
+    return j.odd >= 0 ? j.odd : tp.odd;
+}
+
 template <typename T>
 static YArgs<T> yargs(const TransformJob<T>& j, int inverse)
 {
+    const int form = job_form(j);
     const TransformPlan<T>& tp = *j.plan;
     YArgs<T>                a{};
     a.src     = j.src ? j.src : j.data;
@@ -1074,9 +1094,9 @@ static YArgs<T> yargs(const TransformJob<T>& j, int inverse)
     a.nrad    = tp.nrad;
     for (int i = 0; i < tp.nrad; i++) a.rad[i] = tp.rad[i];
     a.W   = reinterpret_cast<const C2<T>*>(tp.W.p);
-    a.ncs   = tp.engine == 1 ? tp.ncs : 0;
+    a.ncs   = (tp.engine == 1 && form == ODD_VALU) ? tp.ncs : 0;
     a.frag  = tp.frag.p;
-    a.nfrag = tp.engine == 1 ? (int)tp.frag.n : 0;
+    a.nfrag = (tp.engine == 1 && form == ODD_MFMA) ? (int)tp.frag.n : 0;
     a.aux = reinterpret_cast<const C2<T>*>(tp.aux.p);
     a.c0  = tp.c0;
     a.c1  = tp.c1;
@@ -1089,9 +1109,11 @@ static YArgs<T> yargs(const TransformJob<T>& j, int inverse)
     return a;
 }
 
-// one launch of up to kMaxBatch jobs of one engine and thread count
+// one launch of up to kMaxBatch jobs of one engine, thread count and
+// odd-prime form
 template <typename T>
-static void launch_group(const TransformJob<T>* const* jobs, int n, int engine, int NT, int inverse, hipStream_t s)
+static void launch_group(const TransformJob<T>* const* jobs, int n, int engine, int NT, int odd, int inverse,
+                         hipStream_t s)
 {
     YBatch<T> bt{};
     size_t    lds    = 0;
@@ -1104,37 +1126,45 @@ static void launch_group(const TransformJob<T>* const* jobs, int n, int engine, 
         const long long nb       = (long long)((j.nx + 1) / 2) * j.nrhs;
         if (blocks + nb > (1LL << 30)) throw Error(FP_E_INVALID, "too many transform workgroups in one launch");
         blocks += (int)nb;
-        lds = std::max(lds, lds_bytes(*j.plan));
+        lds = std::max(lds, lds_bytes(*j.plan, odd));
     }
     bt.start[n] = blocks;
     if (lds > 160 * 1024) throw Error(FP_E_INVALID, "transform length exceeds the LDS of one workgroup");
     if (blocks == 0) return;
-    const void* fn = engine == 1 ? (const void*)k_ytransform_sb<T> : (const void*)k_ytransform_gen<T>;
-    static bool attr_set[2] = {false, false};
-    if (lds > 64 * 1024 && !attr_set[engine]) {
+    if (engine == 1 && odd == ODD_MFMA && sizeof(T) != 8) throw Error(FP_E_INVALID, "matrix-core passes are FP64");
+    const void* fn = engine == 0         ? (const void*)k_ytransform_gen<T>
+                     : odd == ODD_NONE   ? (const void*)k_ytransform_sb<T, ODD_NONE>
+                     : odd == ODD_MFMA   ? (const void*)k_ytransform_sb<T, ODD_MFMA>
+                                         : (const void*)k_ytransform_sb<T, ODD_VALU>;
+    const int   ki = engine == 0 ? 3 : odd;
+    static bool attr_set[4] = {false, false, false, false};
+    if (lds > 64 * 1024 && !attr_set[ki]) {
         FP_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-        attr_set[engine] = true;
+        attr_set[ki] = true;
     }
-    if (engine == 1) hipLaunchKernelGGL(k_ytransform_sb<T>, dim3(blocks), dim3(NT), lds, s, bt);
-    else hipLaunchKernelGGL(k_ytransform_gen<T>, dim3(blocks), dim3(NT), lds, s, bt);
+    if (engine == 0) hipLaunchKernelGGL(k_ytransform_gen<T>, dim3(blocks), dim3(NT), lds, s, bt);
+    else if (odd == ODD_NONE) hipLaunchKernelGGL((k_ytransform_sb<T, ODD_NONE>), dim3(blocks), dim3(NT), lds, s, bt);
+    else if (odd == ODD_MFMA) hipLaunchKernelGGL((k_ytransform_sb<T, ODD_MFMA>), dim3(blocks), dim3(NT), lds, s, bt);
+    else hipLaunchKernelGGL((k_ytransform_sb<T, ODD_VALU>), dim3(blocks), dim3(NT), lds, s, bt);
     FP_HIP(hipGetLastError());
 }
 
-// Jobs are grouped by (engine, threads per workgroup) and launched kMaxBatch
-// at a time.
+// Jobs are grouped by (engine, threads per workgroup, odd-prime form) and
+// launched kMaxBatch at a time.
 template <typename T>
 void launch_ytransform_batch(const TransformJob<T>* jobs, int njobs, int inverse, hipStream_t s)
 {
-    std::map<std::pair<int, int>, std::vector<const TransformJob<T>*>> groups;
+    std::map<std::array<int, 3>, std::vector<const TransformJob<T>*>> groups;
     for (int k = 0; k < njobs; k++) {
         if (jobs[k].nrhs <= 0 || jobs[k].nx <= 0) continue;
-        groups[{jobs[k].plan->engine, jobs[k].nt ? jobs[k].nt : jobs[k].plan->nt}].push_back(&jobs[k]);
+        const TransformPlan<T>& tp = *jobs[k].plan;
+        groups[{tp.engine, jobs[k].nt ? jobs[k].nt : tp.nt, job_form(jobs[k])}].push_back(&jobs[k]);
     }
     for (auto& kv : groups) {
         auto& g = kv.second;
         for (size_t q0 = 0; q0 < g.size(); q0 += kMaxBatch)
-            launch_group<T>(g.data() + q0, (int)std::min<size_t>(kMaxBatch, g.size() - q0), kv.first.first,
-                            kv.first.second, inverse, s);
+            launch_group<T>(g.data() + q0, (int)std::min<size_t>(kMaxBatch, g.size() - q0), kv.first[0],
+                            kv.first[1], kv.first[2], inverse, s);
     }
 }
 
@@ -1152,9 +1182,9 @@ void launch_ytransform(const TransformPlan<T>& tp, T* data, long long ld, int nx
     launch_ytransform_batch<T>(&j, 1, inverse, s);
 }
 
-// Threads per workgroup of a plan's transform, chosen by timing the forward
-// transform of an nx-row scratch field with every workgroup size its engine
-// supports.  The field holds enough right-hand sides for about 32 MB, so
+// Threads per workgroup and odd-prime pass form of a plan's transform,
+// chosen by timing the forward transform of an nx-row scratch field with
+// every combination that fits.  The field holds enough right-hand sides for about 32 MB, so
 // that the candidates differ by throughput rather than by launch latency
 // (where they tie within noise), and each candidate keeps its best of three
 // rounds.  Recorded per (kind, N, nx, precision) (tune_store).
@@ -1162,17 +1192,25 @@ void launch_ytransform(const TransformPlan<T>& tp, T* data, long long ld, int nx
 template <typename T>
 void tune_transform(TransformPlan<T>& tp, int nx, bool measure)
 {
-    std::vector<int> cand;
-    for (int t : {128, 256, 512})
-        if (tp.engine == 0 ? t == 256 : sb_fits(tp.rad, tp.nrad, tp.L, t, sizeof(T) == 8 && kOddMfma64)) cand.push_back(t);
+    if (tp.engine == 0) return;
+    bool has_odd = false;
+    for (int s = 0; s < tp.nrad; s++) has_odd |= tp.rad[s] == 7 || tp.rad[s] > 8;
+    std::vector<std::array<int, 2>> cand;  // (threads, odd-prime form)
+    for (int form : odd_forms<T>(has_odd))
+        for (int t : {128, 256, 512})
+            if (sb_fits(tp.rad, tp.nrad, tp.L, t, form) && lds_bytes(tp, form) <= 160 * 1024) cand.push_back({t, form});
     if (!measure || cand.size() < 2 || (long long)nx * tp.N < (1LL << 16)) return;
     const std::string key = "yt " + std::to_string(tp.kind) + " " + std::to_string(tp.N) + " " +
                             std::to_string(nx) + " " + std::to_string(sizeof(T));
     {
         std::vector<int> v;
-        if (tune_lookup(key, v) && v.size() == 1) {
-            tp.nt = v[0];
-            return;
+        if (tune_lookup(key, v) && v.size() == 2) {
+            for (const auto& c : cand)
+                if (c[0] == v[0] && c[1] == v[1]) {
+                    tp.nt  = v[0];
+                    tp.odd = v[1];
+                    return;
+                }
         }
     }
     const long long n1   = (long long)nx * tp.N;
@@ -1185,7 +1223,8 @@ void tune_transform(TransformPlan<T>& tp, int nx, bool measure)
     std::vector<float> best_of(cand.size(), 1e30f);
     for (int round = 0; round < 3; round++)
         for (size_t ci = 0; ci < cand.size(); ci++) {
-            tp.nt = cand[ci];
+            tp.nt  = cand[ci][0];
+            tp.odd = cand[ci][1];
             launch_ytransform<T>(tp, buf.p, n1, nx, nrhs, 0, nullptr);
             FP_HIP(hipEventRecord(e0, nullptr));
             for (int r = 0; r < 3; r++) launch_ytransform<T>(tp, buf.p, n1, nx, nrhs, 0, nullptr);
@@ -1198,46 +1237,58 @@ void tune_transform(TransformPlan<T>& tp, int nx, bool measure)
     size_t ib = 0;
     for (size_t ci = 1; ci < cand.size(); ci++)
         if (best_of[ci] < best_of[ib]) ib = ci;
-    const int nt_best = cand[ib];
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    tp.nt = nt_best;
-    tune_store(key, {nt_best});
+    tp.nt  = cand[ib][0];
+    tp.odd = cand[ib][1];
+    tune_store(key, {tp.nt, tp.odd});
 }
-// Threads per workgroup for a batch of transforms launched together (the
-// domains of a composite solve): every job of the single-buffer engine takes
-// the same count where it fits, so that jobs whose own tuned counts differ
-// share one launch and run side by side.  The forward batch is timed with
-// each candidate; 0 (every job its own count) is one of them.  The jobs must
-// point at valid fields.
+// Launch shape for a batch of transforms launched together (the domains of
+// a composite solve): every job of the single-buffer engine takes the same
+// thread count and odd-prime pass form where they fit (a plan without odd
+// primes fits any form), so that jobs whose own tuned shapes differ share
+// one launch and run side by side.  The forward batch is timed with each
+// candidate, best of three interleaved rounds; {0, -1} (every job its own
+// shape) is one of them.  The jobs must point at valid fields.
 template <typename T>
-int tune_batch_nt(std::vector<TransformJob<T>> jobs, hipStream_t s)
+std::array<int, 2> tune_batch_shape(std::vector<TransformJob<T>> jobs, hipStream_t s)
 {
-    if (jobs.size() < 2) return 0;
+    if (jobs.size() < 2) return {0, -1};
     std::string key = "ytb " + std::to_string(sizeof(T));
     for (const auto& j : jobs)
         key += " " + std::to_string(j.plan->kind) + ":" + std::to_string(j.plan->N) + ":" + std::to_string(j.nx);
     {
         std::vector<int> v;
-        if (tune_lookup(key, v) && v.size() == 1) return v[0];
+        if (tune_lookup(key, v) && v.size() == 2) return {v[0], v[1]};
     }
+    std::vector<std::array<int, 2>> cands;
+    for (int nt : {0, 128, 256, 512})
+        for (int form : {-1, (int)ODD_MFMA, (int)ODD_VALU})
+            if (!(form == ODD_MFMA && sizeof(T) != 8)) cands.push_back({nt, form});
     hipEvent_t e0, e1;
     FP_HIP(hipEventCreate(&e0));
     FP_HIP(hipEventCreate(&e1));
-    const int          cands[4] = {0, 128, 256, 512};
-    std::vector<float> best_of(4, 1e30f);
+    std::vector<float> best_of(cands.size(), 1e30f);
     for (int round = 0; round < 3; round++)
-        for (int ci = 0; ci < 4; ci++) {
-            const int cand = cands[ci];
-            bool      any  = false;
+        for (size_t ci = 0; ci < cands.size(); ci++) {
+            const int nt = cands[ci][0], form = cands[ci][1];
+            bool      any = nt == 0 && form < 0;
             for (auto& j : jobs) {
                 const TransformPlan<T>& tp = *j.plan;
-                j.nt = (cand && tp.engine == 1 && sb_fits(tp.rad, tp.nrad, tp.L, cand, sizeof(T) == 8 && kOddMfma64))
-                           ? cand
-                           : 0;
-                any |= j.nt != 0;
+                j.nt  = 0;
+                j.odd = -1;
+                if (tp.engine != 1) continue;
+                const int f  = form < 0 ? tp.odd : form;
+                const int t  = nt ? nt : tp.nt;
+                bool      ok = sb_fits(tp.rad, tp.nrad, tp.L, t, f) && lds_bytes(tp, f) <= 160 * 1024;
+                ok           = ok && (f != ODD_NONE || tp.odd == ODD_NONE);
+                if (ok) {
+                    j.nt  = nt;
+                    j.odd = form;
+                    any   = true;
+                }
             }
-            if (cand && !any) continue;
+            if (!any) continue;
             launch_ytransform_batch<T>(jobs.data(), (int)jobs.size(), 0, s);
             FP_HIP(hipEventRecord(e0, s));
             for (int r = 0; r < 5; r++) launch_ytransform_batch<T>(jobs.data(), (int)jobs.size(), 0, s);
@@ -1247,17 +1298,16 @@ int tune_batch_nt(std::vector<TransformJob<T>> jobs, hipStream_t s)
             FP_HIP(hipEventElapsedTime(&ms, e0, e1));
             best_of[ci] = std::min(best_of[ci], ms);
         }
-    int ib = 0;
-    for (int ci = 1; ci < 4; ci++)
+    size_t ib = 0;
+    for (size_t ci = 1; ci < cands.size(); ci++)
         if (best_of[ci] < best_of[ib]) ib = ci;
-    const int nt_best = cands[ib];
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    tune_store(key, {nt_best});
-    return nt_best;
+    tune_store(key, {cands[ib][0], cands[ib][1]});
+    return cands[ib];
 }
-template int tune_batch_nt<double>(std::vector<TransformJob<double>>, hipStream_t);
-template int tune_batch_nt<float>(std::vector<TransformJob<float>>, hipStream_t);
+template std::array<int, 2> tune_batch_shape<double>(std::vector<TransformJob<double>>, hipStream_t);
+template std::array<int, 2> tune_batch_shape<float>(std::vector<TransformJob<float>>, hipStream_t);
 template void tune_transform<double>(TransformPlan<double>&, int, bool);
 template void tune_transform<float>(TransformPlan<float>&, int, bool);
 
