@@ -82,7 +82,8 @@ template <typename T>
 struct TransformPlan {
     int        kind = T_DD;
     int        N    = 0;  // transform length = ny
-    int        L    = 0;  // complex FFT length
+    int        L    = 0;  // complex DFT length
+    int        F    = 0;  // FFT length: L, or the Bluestein length (prime factors of L above 47)
     int        nrad = 0;
     int        rad[kMaxRadices] = {};
     int        engine = 1;  // 1: single-buffer FFT, 0: two-buffer fallback (prime factors above 47)
@@ -91,6 +92,7 @@ struct TransformPlan {
     DevBuf<T>  W;         // twiddles: 64 of W^m, then L/64+1 of W^(64m), W = exp(-2 pi i / L), then
                           // (cos, sin)(2 pi m / R) of every odd-prime pass R
     DevBuf<T>  aux;       // 2N: NN phase exp(-i pi m / 2N) or ND (sin, cos) of 2N pi (j+1)/(2N+1)
+    DevBuf<T>  blu;       // Bluestein: chirp (L), then the DFT of its conjugate / F (F)
     T          c0 = 0, c1 = 0;
     int        nt = 256;  // threads per workgroup (tuned)
     int        odd = 0;   // odd-prime pass form: 0 none, 1 matrix cores (FP64), 2 VALU pair sums (tuned)

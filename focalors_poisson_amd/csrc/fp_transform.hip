@@ -38,6 +38,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <complex>
 #include <map>
 #include <mutex>
 
@@ -73,7 +74,10 @@ struct YArgs {
     long long    lds, ldd;
     int          src64, dst64;
     int          nx, N, L, kind, inverse, nrad;
-    int          rad[kMaxRadices];
+    int          F;  // FFT length: L, or the smooth Bluestein length >= 2L-1
+    int          rad[kMaxRadices];  // radices of F
+    const C2<T>* chirp;  // Bluestein: exp(-i pi n^2 / L), n < L
+    const C2<T>* bh;     // Bluestein: DFT_F of the conjugate chirp, over F
     const C2<T>* W;    // 64 + L/64 + 1 twiddles, then cos/sin tables of the odd-prime passes
     int          ncs;  // entries of those tables (staged in LDS; FP32 odd-prime passes)
     const double* frag;  // FP64 odd-prime passes: matrix-core operand fragments of the DFT blocks
@@ -459,20 +463,53 @@ __device__ __forceinline__ void run_fft_sb(const YArgs<T>& a, const Tw<T> W, C2<
     int Ns = 1, co = 0, fo = 0;
     for (int s = 0; s < a.nrad; s++) {
         const int R = a.rad[s];
-        if (R == 2) pass_reg_sb<2, FWD>(Z, W, a.L, Ns);
-        else if (R == 3) pass_reg_sb<3, FWD>(Z, W, a.L, Ns);
-        else if (R == 4) pass_reg_sb<4, FWD>(Z, W, a.L, Ns);
-        else if (R == 5) pass_reg_sb<5, FWD>(Z, W, a.L, Ns);
-        else if (R == 8) pass_reg_sb<8, FWD>(Z, W, a.L, Ns);
+        if (R == 2) pass_reg_sb<2, FWD>(Z, W, a.F, Ns);
+        else if (R == 3) pass_reg_sb<3, FWD>(Z, W, a.F, Ns);
+        else if (R == 4) pass_reg_sb<4, FWD>(Z, W, a.F, Ns);
+        else if (R == 5) pass_reg_sb<5, FWD>(Z, W, a.F, Ns);
+        else if (R == 8) pass_reg_sb<8, FWD>(Z, W, a.F, Ns);
         else if constexpr (OM == ODD_MFMA && sizeof(T) == 8) {
-            pass_odd_mfma<FWD>(Z, W, a.L, Ns, R, fragL + fo);
+            pass_odd_mfma<FWD>(Z, W, a.F, Ns, R, fragL + fo);
             fo += 2 * 2 * mfma_ksteps(R) * 64;
         } else if constexpr (OM == ODD_VALU) {
-            pass_odd_sb<FWD>(Z, W, a.L, Ns, R, cs + co);
+            pass_odd_sb<FWD>(Z, W, a.F, Ns, R, cs + co);
             co += ((R - 1) / 2) * (((R - 1) / 2 + kNQ - 1) / kNQ * kNQ);
         }
         Ns *= R;
     }
+}
+
+// DFT of length L (sign by FWD) of Z[0..L) through two FFTs of the smooth
+// length F >= 2L - 1 (Bluestein): with c_n = exp(-i pi n^2 / L),
+//   X_k = c_k sum_n (z_n c_n) conj(c_{k-n}),
+// a circular convolution of length F with the conjugate chirp, whose DFT bh
+// (scaled by 1/F) is precomputed.  The + sign conjugates input and output.
+// Every thread reads and writes only its own elements between barriers.
+template <bool FWD, int OM, typename T>
+__device__ __forceinline__ void run_bluestein(const YArgs<T>& a, const Tw<T> W, C2<T>* Z, const C2<T>* cs,
+                                              const double* fragL)
+{
+    const int L = a.L, F = a.F, NT = blockDim.x;
+    for (int n = threadIdx.x; n < F; n += NT) {
+        C2<T> v{T(0), T(0)};
+        if (n < L) {
+            v = Z[zp<kZpad>(n)];
+            if (!FWD) v.y = -v.y;
+            v = cmul(v, a.chirp[n]);
+        }
+        Z[zp<kZpad>(n)] = v;
+    }
+    __syncthreads();
+    run_fft_sb<true, OM>(a, W, Z, cs, fragL);
+    for (int k = threadIdx.x; k < F; k += NT) Z[zp<kZpad>(k)] = cmul(Z[zp<kZpad>(k)], a.bh[k]);
+    __syncthreads();
+    run_fft_sb<false, OM>(a, W, Z, cs, fragL);
+    for (int k = threadIdx.x; k < L; k += NT) {
+        C2<T> v = cmul(Z[zp<kZpad>(k)], a.chirp[k]);
+        if (!FWD) v.y = -v.y;
+        Z[zp<kZpad>(k)] = v;
+    }
+    __syncthreads();
 }
 
 // ---------------------------------------------------------------------------
@@ -824,20 +861,26 @@ __global__ __launch_bounds__(kMaxNT) void k_ytransform_sb(YBatch<T> bt)
 
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C2<T>*    Z   = reinterpret_cast<C2<T>*>(smem_raw);
-    C2<T>*    tlo = Z + zp<kZpad>(a.L) + 1;  // 64 twiddles W^m
-    const int nhi = a.L / 64 + 1;
+    C2<T>*    tlo = Z + zp<kZpad>(a.F) + 1;  // 64 twiddles W^m
+    const int nhi = a.F / 64 + 1;
     C2<T>*    thi = tlo + 64;           // L/64 + 1 twiddles W^(64m)
     C2<T>*    cs  = thi + nhi;          // cos/sin tables of the odd-prime passes (FP32)
     double*   fr  = reinterpret_cast<double*>(cs + a.ncs);  // matrix-core fragments (FP64)
     T*        y0  = reinterpret_cast<T*>(fr + a.nfrag);     // NN inverse: y_0 of rows a, b
-    C2<T>*    S   = reinterpret_cast<C2<T>*>(y0 + 4);  // DD: prefix sums (N/2 + 1), then 8 wavefront totals
+    // DD: prefix sums (N/2 + 1), then 8 wavefront totals; with Bluestein in
+    // the part of Z past the result
+    C2<T>* S = a.F != a.L ? Z + zp<kZpad>(a.L) + 1 : reinterpret_cast<C2<T>*>(y0 + 4);
     for (int q = threadIdx.x; q < 64 + nhi + a.ncs; q += NT) tlo[q] = a.W[q];
     for (int q = threadIdx.x; q < a.nfrag; q += NT) fr[q] = a.frag[q];
     const Tw<T> W{tlo, thi};
 
     load_rows<kZpad>(a, rp, Z, y0);
     __syncthreads();
-    if (!(a.kind == T_NN && a.inverse)) run_fft_sb<true, OM>(a, W, Z, cs, fr);
+    const bool fwd = !(a.kind == T_NN && a.inverse);
+    if (a.F != a.L) {
+        if (fwd) run_bluestein<true, OM>(a, W, Z, cs, fr);
+        else run_bluestein<false, OM>(a, W, Z, cs, fr);
+    } else if (fwd) run_fft_sb<true, OM>(a, W, Z, cs, fr);
     else run_fft_sb<false, OM>(a, W, Z, cs, fr);
     store_rows<kZpad>(a, rp, Z, y0, S, S + a.N / 2 + 1);
 }
@@ -877,6 +920,41 @@ __global__ __launch_bounds__(kMaxNT) void k_ytransform_gen(YBatch<T> bt)
 
 // Radix schedule: the odd primes above 5 first, largest first (their passes
 // then run with Ns = 1, i.e. without twiddle factors), then 8, 4, 2, 3, 5.
+// smallest 2^a 3^b 5^c >= n
+static int smooth235(int n)
+{
+    for (int m = n;; m++) {
+        int r = m;
+        for (int p : {2, 3, 5})
+            while (r % p == 0) r /= p;
+        if (r == 1) return m;
+    }
+}
+
+// Forward DFT (sign -) of a 2,3,5-smooth length in long double, recursive
+// decimation in time: X_k = sum_r W^(rk) S_r[k mod m], S_r the DFT of the
+// r-th decimated subsequence.  Host-side table construction only.
+static void fft_ld(std::vector<std::complex<long double>>& x)
+{
+    const size_t n = x.size();
+    if (n <= 1) return;
+    const size_t p = n % 2 == 0 ? 2 : n % 3 == 0 ? 3 : 5;
+    const size_t m = n / p;
+    std::vector<std::vector<std::complex<long double>>> sub(p, std::vector<std::complex<long double>>(m));
+    for (size_t r = 0; r < p; r++)
+        for (size_t j = 0; j < m; j++) sub[r][j] = x[j * p + r];
+    for (auto& v : sub) fft_ld(v);
+    const long double PI = 3.141592653589793238462643383279502884L;
+    for (size_t k = 0; k < n; k++) {
+        std::complex<long double> acc = sub[0][k % m];
+        for (size_t r = 1; r < p; r++) {
+            const long double ang = -2.0L * PI * (long double)((r * k) % n) / (long double)n;
+            acc += sub[r][k % m] * std::complex<long double>(cosl(ang), sinl(ang));
+        }
+        x[k] = acc;
+    }
+}
+
 static void factor_radices(int L, int* rad, int& nrad)
 {
     nrad  = 0;
@@ -950,9 +1028,10 @@ static size_t lds_bytes(const TransformPlan<T>& tp, int odd)
 {
     const size_t ncs   = odd == ODD_VALU ? (size_t)tp.ncs : 0;
     const size_t nfrag = odd == ODD_MFMA ? tp.frag.n : 0;
-    const size_t nhi = tp.L / 64 + 1;
-    const size_t zl  = (size_t)tp.L + (kZpad ? ((size_t)tp.L >> kZpad) : 0) + 1;
-    const size_t sc  = tp.kind == T_DD ? (size_t)tp.N / 2 + 1 + 8 : 0;  // prefix sums + wavefront totals
+    const size_t nhi = tp.F / 64 + 1;
+    const size_t zl  = (size_t)tp.F + (kZpad ? ((size_t)tp.F >> kZpad) : 0) + 1;
+    // DD prefix sums + wavefront totals (with Bluestein inside Z, past the result)
+    const size_t sc = (tp.kind == T_DD && tp.F == tp.L) ? (size_t)tp.N / 2 + 1 + 8 : 0;
     if (tp.engine == 1)
         return (zl + 64 + nhi + ncs + sc) * sizeof(C2<T>) + nfrag * sizeof(double) + 4 * sizeof(T);
     const size_t scg = gen_scan_in_buffer(tp.L, tp.N) ? 0 : sc;
@@ -967,9 +1046,33 @@ void TransformPlan<T>::build(int k, int n)
     if (kind == T_DD) L = N + 1;
     else if (kind == T_NN) L = 2 * N;
     else L = 2 * N + 1;
-    factor_radices(L, rad, nrad);
+    // prime factors above 47: Bluestein over the smallest 2,3,5-smooth F >=
+    // 2L - 1 when its single buffer fits the LDS, else the fallback engine
+    F        = L;
+    bool big = false;
+    try {
+        factor_radices(L, rad, nrad);
+        for (int s = 0; s < nrad; s++) big |= rad[s] > kMaxOddPrime;
+    } catch (const Error&) {
+        big = true;
+    }
+    if (big) {
+        const int    Fb   = smooth235(2 * L - 1);
+        const size_t need = ((size_t)Fb + ((size_t)Fb >> kZpad) + 1 + 64 + Fb / 64 + 1) * sizeof(C2<T>) + 4 * sizeof(T);
+        int          rb[kMaxRadices], nb = 0;
+        factor_radices(Fb, rb, nb);
+        bool fits = false;
+        for (int t : {128, 256, 512}) fits |= sb_fits(rb, nb, Fb, t, ODD_NONE);
+        if (fits && need <= 160 * 1024) {
+            F    = Fb;
+            nrad = nb;
+            for (int s = 0; s < nb; s++) rad[s] = rb[s];
+        } else {
+            factor_radices(L, rad, nrad);
+        }
+    }
     const long double PI  = 3.141592653589793238462643383279502884L;
-    const int         nhi = L / 64 + 1;
+    const int         nhi = F / 64 + 1;
     // odd-prime cos/sin tables, one per odd-prime pass in schedule order
     std::vector<long double> cs;
     ncs = 0;
@@ -1007,7 +1110,7 @@ void TransformPlan<T>::build(int k, int n)
     std::vector<T> w(2 * (size_t)(64 + nhi + ncs)), x(2 * (size_t)(N + 1), T(0));
     for (int q = 0; q < 64 + nhi; q++) {
         const long long   t   = q < 64 ? q : 64LL * (q - 64);
-        const long double ang = -2.0L * PI * (long double)(t % L) / (long double)L;
+        const long double ang = -2.0L * PI * (long double)(t % F) / (long double)F;
         w[2 * q]              = (T)cosl(ang);
         w[2 * q + 1]          = (T)sinl(ang);
     }
@@ -1036,6 +1139,32 @@ void TransformPlan<T>::build(int k, int n)
         for (int j = 0; j < L; j++) x[2 * j] = (T)sinl(PI * (long double)j / (long double)L);
         c0 = (T)std::sqrt(2.0 / (N + 1.0));
     }
+    if (F != L) {
+        // chirp c_n = exp(-i pi n^2 / L) (n^2 reduced mod 2L), then the DFT of
+        // length F of b (b_m = conj c_m for |m| < L, wrapped), scaled by 1/F
+        std::vector<std::complex<long double>> c(L), b(F, 0.0L);
+        for (int n = 0; n < L; n++) {
+            const long long   q   = ((long long)n * n) % (2LL * L);
+            const long double ang = -PI * (long double)q / (long double)L;
+            c[n]                  = {cosl(ang), sinl(ang)};
+        }
+        for (int m = 0; m < L; m++) {
+            b[m] = std::conj(c[m]);
+            if (m) b[F - m] = std::conj(c[m]);
+        }
+        fft_ld(b);
+        std::vector<T> t(2 * (size_t)(L + F));
+        for (int n = 0; n < L; n++) {
+            t[2 * n]     = (T)c[n].real();
+            t[2 * n + 1] = (T)c[n].imag();
+        }
+        for (int k = 0; k < F; k++) {
+            t[2 * (size_t)(L + k)]     = (T)(b[k].real() / F);
+            t[2 * (size_t)(L + k) + 1] = (T)(b[k].imag() / F);
+        }
+        blu.alloc(t.size());
+        blu.upload(t.data(), t.size());
+    }
     W.alloc(w.size());
     W.upload(w.data(), w.size());
     frag.alloc(fr.size());
@@ -1052,7 +1181,7 @@ void TransformPlan<T>::build(int k, int n)
     odd    = ODD_NONE;
     for (int form : odd_forms<T>(has_odd)) {
         for (int t : {256, 512, 128})
-            if (sb_fits(rad, nrad, L, t, form) && lds_bytes(*this, form) <= 160 * 1024) {
+            if (sb_fits(rad, nrad, F, t, form) && lds_bytes(*this, form) <= 160 * 1024) {
                 engine = 1;
                 nt     = t;
                 odd    = form;
@@ -1089,6 +1218,9 @@ static YArgs<T> yargs(const TransformJob<T>& j, int inverse)
     a.nx      = j.nx;
     a.N       = tp.N;
     a.L       = tp.L;
+    a.F       = tp.F;
+    a.chirp   = tp.F != tp.L ? reinterpret_cast<const C2<T>*>(tp.blu.p) : nullptr;
+    a.bh      = tp.F != tp.L ? reinterpret_cast<const C2<T>*>(tp.blu.p) + tp.L : nullptr;
     a.kind    = tp.kind;
     a.inverse = inverse;
     a.nrad    = tp.nrad;
@@ -1198,7 +1330,7 @@ void tune_transform(TransformPlan<T>& tp, int nx, bool measure)
     std::vector<std::array<int, 2>> cand;  // (threads, odd-prime form)
     for (int form : odd_forms<T>(has_odd))
         for (int t : {128, 256, 512})
-            if (sb_fits(tp.rad, tp.nrad, tp.L, t, form) && lds_bytes(tp, form) <= 160 * 1024) cand.push_back({t, form});
+            if (sb_fits(tp.rad, tp.nrad, tp.F, t, form) && lds_bytes(tp, form) <= 160 * 1024) cand.push_back({t, form});
     if (!measure || cand.size() < 2 || (long long)nx * tp.N < (1LL << 16)) return;
     const std::string key = "yt " + std::to_string(tp.kind) + " " + std::to_string(tp.N) + " " +
                             std::to_string(nx) + " " + std::to_string(sizeof(T));
@@ -1280,7 +1412,7 @@ std::array<int, 2> tune_batch_shape(std::vector<TransformJob<T>> jobs, hipStream
                 if (tp.engine != 1) continue;
                 const int f  = form < 0 ? tp.odd : form;
                 const int t  = nt ? nt : tp.nt;
-                bool      ok = sb_fits(tp.rad, tp.nrad, tp.L, t, f) && lds_bytes(tp, f) <= 160 * 1024;
+                bool      ok = sb_fits(tp.rad, tp.nrad, tp.F, t, f) && lds_bytes(tp, f) <= 160 * 1024;
                 ok           = ok && (f != ODD_NONE || tp.odd == ODD_NONE);
                 if (ok) {
                     j.nt  = nt;

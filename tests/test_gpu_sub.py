@@ -172,7 +172,7 @@ def test_errors_are_status_codes(ctx):
 
 
 @pytest.mark.parametrize("bcp", [(D, D), (N, N), (D, N), (N, D)])
-@pytest.mark.parametrize("ny", [5, 64, 100, 1024])
+@pytest.mark.parametrize("ny", [5, 64, 100, 256, 1024, 2048])
 def test_transforms_match_reference_coefficients(ctx, bcp, ny):
     """PoissonFFT2D_*::transform / transform_transpose coefficient for
     coefficient (same signs and mode order as the reference)."""
