@@ -198,6 +198,57 @@ def extras(fpa, ctx, torch, np, n):
                              "rel_l2_f32_refine_vs_f64": float((u32 - u64).norm() / u64.norm())}
     del sub, src, u64, u32
 
+    # L-shape of three 2048^2 Dirichlet domains: FP64 vs FP32 + FP64 refinement
+    doms, conn, fields, _ = cases.lshape(2048, seed=4)
+    src = [torch.from_numpy(f).to(dev) for f in fields]
+    res = {}
+    for prec, key in ((fpa.FP_F64, "f64"), (fpa.FP_F32_REFINE, "f32_refine")):
+        cat = fpa.ConcatPoissonSolver2D(ctx, doms, conn, m=200, tol=1e-12, tol_is_relative=True, max_iter=100,
+                                        precision=prec)
+        if prec == fpa.FP_F32_REFINE:
+            cat.set_refinement(tol=1e-12, max_steps=10)
+        wk = [torch.empty_like(x) for x in src]
+        info = {}
+
+        def solve_l():
+            for w, x in zip(wk, src):
+                w.copy_(x)
+            info.update(cat.solve(wk))
+
+        res[key] = (timed(solve_l, torch), dict(info), [w.clone() for w in wk])
+        del cat
+    d = max(float((a - b).norm() / b.norm()) for a, b in zip(res["f32_refine"][2], res["f64"][2]))
+    out["lshape_3x2048"] = {"f64_ms": round(res["f64"][0] * 1e3, 3),
+                            "f32_refine_ms": round(res["f32_refine"][0] * 1e3, 3),
+                            "refine_steps": res["f32_refine"][1].get("refine_steps"),
+                            "rel_l2_f32_refine_vs_f64": d,
+                            "gmres_rel_residual_f64": res["f64"][1].get("rel_residual")}
+    del res, src
+
+    # single 256^2 Dirichlet domain (the reference's own CPU case): solves/s,
+    # error vs the oracle, and the oracle's own time on the host
+    doms, conn, fields, _ = cases.single(256)
+    cat = fpa.ConcatPoissonSolver2D(ctx, doms, conn, m=20, tol=1e-12, tol_is_relative=True)
+    src = [torch.from_numpy(f).to(dev) for f in fields]
+    wk = [torch.empty_like(x) for x in src]
+
+    def solve_s():
+        for w, x in zip(wk, src):
+            w.copy_(x)
+        cat.solve(wk)
+
+    ts = timed(solve_s, torch, reps=50)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    t0 = time.perf_counter()
+    ref, _, _ = O.Composite(doms, conn).solve(fields)
+    t_cpu = time.perf_counter() - t0
+    u = wk[0].cpu().numpy()
+    out["single_256"] = {"solves_per_s": round(1.0 / ts, 1), "ms_per_solve": round(ts * 1e3, 4),
+                         "rel_l2_vs_oracle": float(np.linalg.norm(u - ref[0]) / np.linalg.norm(ref[0])),
+                         "cpu_oracle_solves_per_s": round(1.0 / t_cpu, 1), "cpu_threads": O.num_threads()}
+    del cat, src, wk
+
     # adjoint on the cross, batch of 32: (forward + backward) / forward time
     doms, conn, fields, _ = cases.cross(n, seed=1, nrhs=32)
     cat = fpa.ConcatPoissonSolver2D(ctx, doms, conn, m=200, tol=1e-12, tol_is_relative=True, max_iter=100)

@@ -71,6 +71,18 @@ def cross(n=1024, seed=1, arm=None, nrhs=1):
     return doms, conn, fields, {}
 
 
+def lshape(n=2048, seed=4):
+    """L-shaped geometry of three n x n Dirichlet domains, h = 1/n: domain 0
+    on [0,1]^2, domain 1 to its right, domain 2 above it; right-hand side the
+    multiscale sum of 32 seeded Fourier modes on the whole L."""
+    h = 1.0 / n
+    doms = [(n, n, h, h, (D, A, D, A)), (n, n, h, h, (A, D, D, D)), (n, n, h, h, (D, D, A, D))]
+    conn = [(0, XPOS, 1), (0, YPOS, 2)]
+    offs = [(0.0, 0.0), (1.0, 0.0), (0.0, 1.0)]
+    fields = [multiscale_rhs(n, n, h, h, ox, oy, seed=seed) for ox, oy in offs]
+    return doms, conn, fields, {}
+
+
 def batched_sizes(count=512, seed=2, lo=64, hi=1024):
     """Side lengths log-uniform in [lo, hi]."""
     rng = np.random.default_rng(seed)

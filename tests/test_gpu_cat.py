@@ -168,3 +168,16 @@ def test_restart_after_start_check(ctx):
     ref, _, _ = O.Composite(doms, conn, tol=1e-14, tol_is_relative=True, maxiter=400,
                             schur_mode=O.SCHUR_SPECTRAL).solve(fs)
     assert rel(us, ref) < 1e-10, info
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_lshape_vs_oracle(ctx, precision):
+    """L of three Dirichlet squares (children on the root's x+ and y+ faces)
+    with a multiscale right-hand side, FP64 and FP32+refine."""
+    from focalors_poisson_amd.cases import lshape
+    doms, conn, fields, _ = lshape(96, seed=4)
+    banner("lshape 3x96 precision=%d" % precision)
+    us, info, _ = run(ctx, doms, conn, fields, precision=precision)
+    ref, _, _ = O.Composite(doms, conn, tol=1e-14, tol_is_relative=True, maxiter=400,
+                            schur_mode=O.SCHUR_SPECTRAL).solve(fields)
+    assert rel(us, ref) < 1e-10, info
