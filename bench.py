@@ -301,7 +301,10 @@ def main():
     cells = sum(d[0] * d[1] for d in doms)
     pristine = [torch.from_numpy(f[0]).to(dev) for f in fields]
 
-    def run_variant(precision):
+    def run_variant(precision, into=False):
+        """Eager solves.  into=False: in place (fp_cat_solve), the right-hand
+        sides refilled by one device copy per step; into=True: out of place
+        (fp_cat_solve_into), the resident right-hand sides read directly."""
         t0 = time.perf_counter()
         cat = fpa.ConcatPoissonSolver2D(ctx, doms, conn, m=a.m, tol=a.tol, tol_is_relative=True, max_iter=100,
                                         precision=precision)
@@ -310,11 +313,15 @@ def main():
         torch.cuda.synchronize()
         t_build = time.perf_counter() - t0
         src, dst, work = flat_fields(torch, pristine)
+        rhs = pristine if into else None  # resident device tensors, read only
         info = {}
 
         def step():
-            dst.copy_(src)  # refill the in-place right-hand sides
-            info.update(cat.solve(work))
+            if into:
+                info.update(cat.solve_into(rhs, work))
+            else:
+                dst.copy_(src)  # refill the in-place right-hand sides
+                info.update(cat.solve(work))
 
         for _ in range(a.warmup):
             step()
@@ -326,17 +333,20 @@ def main():
         ms = (time.perf_counter() - t0) / a.steps * 1e3
         return {"ms": ms, "build_s": t_build, "info": dict(info), "u": [w.cpu().numpy() for w in work]}
 
-    def run_graph():
-        """FP64 solve captured once in a HIP graph and replayed per step (the
-        right-hand sides refilled by eager copies on the same stream)."""
+    def run_graph(into=False):
+        """FP64 solve captured once in a HIP graph and replayed per step: in
+        place with the right-hand sides refilled by an eager copy on the same
+        stream, or (into=True) out of place from the resident right-hand sides."""
         cat = fpa.ConcatPoissonSolver2D(ctx, doms, conn, m=a.m, tol=a.tol, tol_is_relative=True, max_iter=100)
         stream = torch.cuda.Stream()
         src, dst, work = flat_fields(torch, pristine)
+        rhs = pristine if into else None  # resident device tensors, read only
         with torch.cuda.stream(stream):
-            cat.graph_capture(work, stream=stream)
+            cat.graph_capture(work, stream=stream, rhs=rhs)
 
             def step(wait=False):
-                dst.copy_(src)
+                if not into:
+                    dst.copy_(src)
                 return cat.graph_launch(stream=stream, wait=wait)
 
             for _ in range(a.warmup):
@@ -353,6 +363,8 @@ def main():
 
     v64 = run_variant(fpa.FP_F64)
     vg = run_graph()
+    vi = run_variant(fpa.FP_F64, into=True)
+    vgi = run_graph(into=True)
     v32 = run_variant(fpa.FP_F32_REFINE)
 
     # ---- per-kernel timing on the centre subdomain ----
@@ -365,7 +377,8 @@ def main():
 
     # ---- CPU baseline + accuracy: the oracle on the host cores ----
     cpu = None
-    err = {"f64": None, "f64_graph": None, "f32_refine": None}
+    runs = {"f64": v64, "f64_graph": vg, "f64_into": vi, "f64_graph_into": vgi, "f32_refine": v32}
+    err = {k: None for k in runs}
     if not a.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
@@ -377,7 +390,7 @@ def main():
         ref, it, _ = oc.solve([f[0] for f in fields])
         t_cpu = time.perf_counter() - t0
         y = np.concatenate([r.ravel() for r in ref])
-        for key, v in (("f64", v64), ("f64_graph", vg), ("f32_refine", v32)):
+        for key, v in runs.items():
             x = np.concatenate([u.ravel() for u in v["u"]])
             err[key] = float(np.linalg.norm(x - y) / np.linalg.norm(y))
         cpu = {"value": round(1.0 / t_cpu, 4), "unit": "cross solves/s", "cores": O.num_threads(),
@@ -393,12 +406,15 @@ def main():
                 "gmres_iterations": i.get("iterations"), "rel_residual": i.get("rel_residual"),
                 "refine_sweeps": i.get("refine_steps"), "refine_error_estimate": i.get("refine_residual")}
 
-    variants = {"f64": variant(v64, "f64"), "f64_graph": variant(vg, "f64_graph"),
-                "f32_refine": variant(v32, "f32_refine")}
-    # the headline is the faster FP64 form whose residual check passed
-    graph_ok = vg["info"].get("status") == 0
-    main_key = "f64_graph" if graph_ok and vg["ms"] < v64["ms"] else "f64"
-    main_v = vg if main_key == "f64_graph" else v64
+    variants = {k: variant(v, k) for k, v in runs.items()}
+    # the headline is the fastest FP64 form whose residual check passed and
+    # (when the oracle ran) whose error meets the 1e-10 bar
+    def passed(k):
+        ok = runs[k]["info"].get("status", 0) == 0
+        return ok and (err[k] is None or err[k] <= 1e-10)
+    cands = [k for k in ("f64", "f64_graph", "f64_into", "f64_graph_into") if passed(k)] or ["f64"]
+    main_key = min(cands, key=lambda k: runs[k]["ms"])
+    main_v = runs[main_key]
     out = {
         "metric": "cross-geometry solves/s at 1e-10 rel L2: FP64 vs FP32+refine; HBM GB/s",
         "value": round(1000.0 / main_v["ms"], 3),

@@ -196,15 +196,39 @@ class ConcatPoissonSolver2D:
         d["status"] = st
         return d
 
-    def graph_capture(self, fields, ghost=None, stream=None):
+    def solve_into(self, rhs, out, ghost=None, stream=None, allow_not_converged=False):
+        """Out-of-place composite solve (fp_cat_solve_into): reads the raw
+        right-hand sides `rhs` (unchanged), writes the solution to `out`."""
+        rp, gp, nrhs = self._ptrs(rhs, ghost)
+        op, _, nout = self._ptrs(out, None)
+        if nout != nrhs:
+            raise ValueError("rhs and out need the same number of right-hand sides")
+        info = _capi.SolveInfo()
+        allow = (_capi.FP_E_NOT_CONVERGED,) if allow_not_converged else ()
+        st = _capi.check(_capi.lib().fp_cat_solve_into(self._h, rp, op, gp, nrhs, _stream_ptr(stream),
+                                                       ctypes.byref(info)), allow)
+        d = info.as_dict()
+        d["status"] = st
+        return d
+
+    def graph_capture(self, fields, ghost=None, stream=None, rhs=None):
         """Records the FP64 solve of these fields (and ghost arrays) into a HIP
         graph on `stream` (a non-default torch stream; fp_cat_graph_capture).
-        The tensors stay bound to the graph: refill them, then graph_launch()."""
+        The tensors stay bound to the graph: refill them, then graph_launch().
+        rhs given: the replays read the right-hand sides there and write the
+        solution to `fields` (fp_cat_graph_capture_into)."""
         fp, gp, nrhs = self._ptrs(fields, ghost)
         if stream is None:
             raise ValueError("graph capture needs an explicit, non-default stream")
-        self._graph_keep = (list(fields), dict(ghost or {}), fp, gp)
-        _capi.check(_capi.lib().fp_cat_graph_capture(self._h, fp, gp, nrhs, _stream_ptr(stream)))
+        if rhs is None:
+            self._graph_keep = (list(fields), dict(ghost or {}), fp, gp)
+            _capi.check(_capi.lib().fp_cat_graph_capture(self._h, fp, gp, nrhs, _stream_ptr(stream)))
+            return
+        rp, _, nr = self._ptrs(rhs, None)
+        if nr != nrhs:
+            raise ValueError("rhs and fields need the same number of right-hand sides")
+        self._graph_keep = (list(fields), list(rhs), dict(ghost or {}), fp, gp, rp)
+        _capi.check(_capi.lib().fp_cat_graph_capture_into(self._h, rp, fp, gp, nrhs, _stream_ptr(stream)))
 
     def graph_launch(self, stream=None, wait=True, allow_not_converged=False):
         """Replays the captured solve.  wait=True: synchronises and returns the

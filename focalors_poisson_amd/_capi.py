@@ -92,6 +92,10 @@ def lib():
     L.fp_cat_vjp.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.c_int, _vp,
                              ctypes.POINTER(SolveInfo)]
     L.fp_cat_graph_capture.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.c_int, _vp]
+    L.fp_cat_solve_into.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.c_int,
+                                    _vp, ctypes.POINTER(SolveInfo)]
+    L.fp_cat_graph_capture_into.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                            ctypes.c_int, _vp]
     L.fp_cat_graph_launch.argtypes = [_vp, _vp, ctypes.POINTER(SolveInfo)]
     _lib = L
     return L

@@ -998,7 +998,8 @@ static void cat_residual(fp_cat_s* c, double* const* u, int nrhs, hipStream_t s)
 // every right-hand side, or after refine_max sweeps.
 static bool fast_shape_ok(const fp_cat_s* c);
 template <typename T>
-static void cat_sweep_fast_t(fp_cat_s* c, double* const* f, const double* const* ghost, int nrhs, hipStream_t s);
+static void cat_sweep_fast_t(fp_cat_s* c, double* const* f, const double* const* ghost, int nrhs, hipStream_t s,
+                             const double* const* rhs = nullptr);
 
 static void cat_refine(fp_cat_s* c, double* const* f, int nrhs, hipStream_t s)
 {
@@ -1272,8 +1273,11 @@ static std::array<int, 2> batch_shape(fp_cat_s* c)
     return v;
 }
 
+// rhs (FP64 only; null: f holds them): raw right-hand sides read by the
+// forward transforms instead of f, which then only receives the solution.
 template <typename T>
-static void cat_sweep_fast_t(fp_cat_s* c, double* const* f, const double* const* ghost, int nrhs, hipStream_t s)
+static void cat_sweep_fast_t(fp_cat_s* c, double* const* f, const double* const* ghost, int nrhs, hipStream_t s,
+                             const double* const* rhs)
 {
     constexpr bool f64 = sizeof(T) == 8;
     const int      nd = c->nd, root = c->root;
@@ -1312,6 +1316,10 @@ static void cat_sweep_fast_t(fp_cat_s* c, double* const* f, const double* const*
             a.asmb = 1;
             a.hx2  = c->hx[d] * c->hx[d];
             face_terms(c, d, ghost, a.g, a.ga);
+            if (rhs && rhs[d] != f[d]) {
+                a.src = rhs[d];
+                a.lds = n;
+            }
         } else {
             a.src   = f[d];
             a.lds   = n;
@@ -1488,16 +1496,29 @@ static void cat_sweep_fast_t(fp_cat_s* c, double* const* f, const double* const*
     launch_sweep_batch<T>(dn.data(), (int)nc, s);
     launch_ytransform_batch<T>(iv.data(), nd, 1, s);
 }
-static void cat_sweep_fast(fp_cat_s* c, double* const* f, const double* const* ghost, int nrhs, hipStream_t s)
+static void cat_sweep_fast(fp_cat_s* c, double* const* f, const double* const* ghost, int nrhs, hipStream_t s,
+                           const double* const* rhs = nullptr)
 {
-    cat_sweep_fast_t<double>(c, f, ghost, nrhs, s);
+    cat_sweep_fast_t<double>(c, f, ghost, nrhs, s, rhs);
 }
 
-// ConcatPoissonSolver2D::solve (pe/concat/concat_solver2d.cpp:101-157)
-static void cat_solve(fp_cat_s* c, double* const* f, const double* const* ghost, int nrhs, hipStream_t s)
+// ConcatPoissonSolver2D::solve (pe/concat/concat_solver2d.cpp:101-157).
+// rhs != null: the right-hand sides are read from there (unchanged) and the
+// solution written to f; the two-level FP64 path reads them in its forward
+// transforms, the other paths start from a copy.
+static void cat_solve(fp_cat_s* c, double* const* f, const double* const* ghost, int nrhs, hipStream_t s,
+                      const double* const* rhs = nullptr)
 {
     c->last_refine_steps = 0;
     c->last_refine_est   = 0.0;
+    if (rhs && c->precision != FP_F32_REFINE && fast_ok(c)) {
+        cat_sweep_fast(c, f, ghost, nrhs, s, rhs);
+        return;
+    }
+    if (rhs)
+        for (int d = 0; d < c->nd; d++)
+            if (rhs[d] != f[d])
+                FP_HIP(hipMemcpyAsync(f[d], rhs[d], sizeof(double) * dom_n(c, d) * nrhs, hipMemcpyDeviceToDevice, s));
     if (c->precision == FP_F32_REFINE) {
         cat_assemble(c, f, ghost, nrhs, s);
         cat_refine(c, f, nrhs, s);
@@ -1731,14 +1752,14 @@ void fp_cat_destroy(fp_cat cat) { delete cat; }
 
 int fp_cat_root(fp_cat cat) { return cat ? cat->root : -1; }
 
-int fp_cat_solve(fp_cat cat, double* const* d_fields, const double* const* d_ghost, int nrhs, void* stream,
-                 fp_solve_info* info)
+static int cat_solve_entry(fp_cat cat, const double* const* d_rhs, double* const* d_fields,
+                           const double* const* d_ghost, int nrhs, void* stream, fp_solve_info* info)
 {
     int st = guarded([&] {
         FP_REQUIRE(cat && d_fields && nrhs >= 1, "null argument");
-        for (int d = 0; d < cat->nd; d++) FP_REQUIRE(d_fields[d], "null field");
+        for (int d = 0; d < cat->nd; d++) FP_REQUIRE(d_fields[d] && (!d_rhs || d_rhs[d]), "null field");
         FP_HIP(hipSetDevice(cat->ctx->device));
-        fp::cat_solve(cat, d_fields, d_ghost, nrhs, (hipStream_t)stream);
+        fp::cat_solve(cat, d_fields, d_ghost, nrhs, (hipStream_t)stream, d_rhs);
         if (info) {
             info->iterations      = cat->last_steps;
             info->restarts        = cat->last_restarts;
@@ -1752,12 +1773,26 @@ int fp_cat_solve(fp_cat cat, double* const* d_fields, const double* const* d_gho
     return st;
 }
 
-int fp_cat_graph_capture(fp_cat cat, double* const* d_fields, const double* const* d_ghost, int nrhs, void* stream)
+int fp_cat_solve(fp_cat cat, double* const* d_fields, const double* const* d_ghost, int nrhs, void* stream,
+                 fp_solve_info* info)
+{
+    return cat_solve_entry(cat, nullptr, d_fields, d_ghost, nrhs, stream, info);
+}
+
+int fp_cat_solve_into(fp_cat cat, const double* const* d_rhs, double* const* d_out, const double* const* d_ghost,
+                      int nrhs, void* stream, fp_solve_info* info)
+{
+    if (!d_rhs) return guarded([] { FP_REQUIRE(false, "null argument"); });
+    return cat_solve_entry(cat, d_rhs, d_out, d_ghost, nrhs, stream, info);
+}
+
+static int graph_capture(fp_cat cat, const double* const* d_rhs, double* const* d_fields,
+                         const double* const* d_ghost, int nrhs, void* stream)
 {
     return guarded([&] {
         FP_REQUIRE(cat && d_fields && nrhs >= 1, "null argument");
         FP_REQUIRE(stream, "graph capture needs a stream other than the null stream");
-        for (int d = 0; d < cat->nd; d++) FP_REQUIRE(d_fields[d], "null field");
+        for (int d = 0; d < cat->nd; d++) FP_REQUIRE(d_fields[d] && (!d_rhs || d_rhs[d]), "null field");
         FP_REQUIRE(cat->precision == FP_F64 && fp::fast_ok(cat),
                    "graph capture covers the FP64 solve of a root with leaf children");
         fp::Node& R = *cat->nodes[cat->root];
@@ -1778,7 +1813,7 @@ int fp_cat_graph_capture(fp_cat cat, double* const* d_fields, const double* cons
         FP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         cat->graph_mode = true;
         try {
-            fp::cat_sweep_fast(cat, d_fields, d_ghost, nrhs, s);
+            fp::cat_sweep_fast(cat, d_fields, d_ghost, nrhs, s, d_rhs);
         } catch (...) {
             cat->graph_mode = false;
             (void)hipStreamEndCapture(s, &graph);
@@ -1792,6 +1827,18 @@ int fp_cat_graph_capture(fp_cat cat, double* const* d_fields, const double* cons
         FP_HIP(e);
         cat->gnrhs = nrhs;
     });
+}
+
+int fp_cat_graph_capture(fp_cat cat, double* const* d_fields, const double* const* d_ghost, int nrhs, void* stream)
+{
+    return graph_capture(cat, nullptr, d_fields, d_ghost, nrhs, stream);
+}
+
+int fp_cat_graph_capture_into(fp_cat cat, const double* const* d_rhs, double* const* d_out,
+                              const double* const* d_ghost, int nrhs, void* stream)
+{
+    if (!d_rhs) return guarded([] { FP_REQUIRE(false, "null argument"); });
+    return graph_capture(cat, d_rhs, d_out, d_ghost, nrhs, stream);
 }
 
 int fp_cat_graph_launch(fp_cat cat, void* stream, fp_solve_info* info)

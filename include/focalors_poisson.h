@@ -151,6 +151,13 @@ int  fp_cat_root(fp_cat cat);
  * Variable2D::buffer_map values, variable2d.cpp:345-416), or NULL for none. */
 int fp_cat_solve(fp_cat cat, double* const* d_fields, const double* const* d_ghost, int nrhs, void* stream,
                  fp_solve_info* info);
+/* Out-of-place form of fp_cat_solve (no reference counterpart: the
+ * reference solves in place): reads the raw right-hand sides d_rhs[d], which
+ * stay unchanged, and writes the solution to d_out[d] (d_rhs[d] == d_out[d]
+ * is fp_cat_solve).  The FP64 two-level path reads them directly in its
+ * forward transforms; other paths start from a copy. */
+int fp_cat_solve_into(fp_cat cat, const double* const* d_rhs, double* const* d_out, const double* const* d_ghost,
+                      int nrhs, void* stream, fp_solve_info* info);
 /* Captured composite solve (no reference counterpart; the reference has no
  * device): records the whole FP64 solve of fp_cat_solve on these fields and
  * ghost arrays into a HIP graph.  Covered: a root with leaf children (depth
@@ -159,6 +166,10 @@ int fp_cat_solve(fp_cat cat, double* const* d_fields, const double* const* d_gho
  * product and takes no further Krylov steps; its residual check is recorded
  * too.  stream must not be the null stream.  Nothing is computed at capture. */
 int fp_cat_graph_capture(fp_cat cat, double* const* d_fields, const double* const* d_ghost, int nrhs, void* stream);
+/* fp_cat_graph_capture of fp_cat_solve_into: replays read d_rhs and write
+ * d_out (no reference counterpart). */
+int fp_cat_graph_capture_into(fp_cat cat, const double* const* d_rhs, double* const* d_out,
+                              const double* const* d_ghost, int nrhs, void* stream);
 /* Replays the captured solve on the captured buffers (the caller refills
  * them in between).  info != NULL: waits for it and reports the residual
  * check; FP_E_NOT_CONVERGED if the preconditioned start missed the tolerance

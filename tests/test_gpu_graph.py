@@ -79,3 +79,62 @@ def test_graph_requires_stream(ctx):
         cat.graph_capture([to_dev(f[0]) for f in fields])
     with pytest.raises(fpa.FPError):
         cat.graph_launch()
+
+
+@pytest.mark.parametrize("geom,precision", [("cross", 0), ("cross", 1), ("six", 0)])
+def test_solve_into_matches_in_place(ctx, geom, precision):
+    """fp_cat_solve_into: the right-hand sides stay unchanged and the output
+    equals the in-place solve (two-level path reading them in its transforms;
+    the refinement and general-tree paths from a copy)."""
+    import os
+    import torch
+    import focalors_poisson_amd as fpa
+    from focalors_poisson_amd.cases import cross
+    banner("solve_into %s precision=%d" % (geom, precision))
+    if geom == "cross":
+        doms, conn, fields, _ = cross(64, seed=7)
+        fs = [f[0] for f in fields]
+    else:
+        g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "concat_six.npz"))
+        doms = [(int(a), int(b), 1.0, 1.0, (0, 0, 0, 0)) for a, b in g["sizes"]]
+        conn = [tuple(int(v) for v in c) for c in g["conn"]]
+        fs = [g["f%d" % k] for k in range(len(doms))]
+    cat = fpa.ConcatPoissonSolver2D(ctx, doms, conn, m=40, tol=1e-13, tol_is_relative=True, precision=precision)
+    rhs = [to_dev(f) for f in fs]
+    out = [torch.full_like(r, float("nan")) for r in rhs]
+    cat.solve_into(rhs, out)
+    inplace = [to_dev(f) for f in fs]
+    cat.solve(inplace)
+    torch.cuda.synchronize()
+    assert all(np.array_equal(r.cpu().numpy(), f) for r, f in zip(rhs, fs))
+    assert rel([o.cpu().numpy() for o in out], [i.cpu().numpy() for i in inplace]) < 1e-14
+
+
+def test_graph_capture_into_replays(ctx):
+    """Captured out-of-place solve: replays read the bound right-hand sides
+    (refilled between replays) and match the eager out-of-place solve."""
+    import torch
+    import focalors_poisson_amd as fpa
+    from focalors_poisson_amd.cases import cross
+    banner("graph capture into")
+    cases = [cross(64, seed=sd) for sd in (11, 12, 13)]
+    doms, conn = cases[0][0], cases[0][1]
+    cat = fpa.ConcatPoissonSolver2D(ctx, doms, conn, m=40, tol=1e-12, tol_is_relative=True, max_iter=100)
+    stream = torch.cuda.Stream()
+    rhs = [torch.empty(f[0].shape, dtype=torch.float64, device=device()) for f in cases[0][2]]
+    out = [torch.empty_like(r) for r in rhs]
+    with torch.cuda.stream(stream):
+        cat.graph_capture(out, stream=stream, rhs=rhs)
+    for k, (_, _, fields, _) in enumerate(cases):
+        fs = [f[0] for f in fields]
+        eager_out = [torch.empty_like(r) for r in rhs]
+        cat.solve_into([to_dev(f) for f in fs], eager_out)
+        torch.cuda.synchronize()
+        with torch.cuda.stream(stream):
+            for r, f in zip(rhs, fs):
+                r.copy_(torch.from_numpy(f).to(device()))
+            info = cat.graph_launch(stream=stream)
+        torch.cuda.synchronize()
+        assert info["status"] == 0, info
+        assert rel([o.cpu().numpy() for o in out], [e.cpu().numpy() for e in eager_out]) < 1e-13, k
+        assert all(np.array_equal(r.cpu().numpy(), f) for r, f in zip(rhs, fs)), k
