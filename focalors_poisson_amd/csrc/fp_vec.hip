@@ -30,24 +30,81 @@ static rocblas_handle blas_handle()
 
 // Y[r] = alpha A X[r] + beta Y[r] for r < nrhs: A is m x k row-major (lda),
 // X[r] has k values (ldx apart), Y[r] m values (ldy apart).  In rocBLAS's
-// column-major terms: Y = alpha op(A^T) X + beta Y.
-static void dense_apply(const double* A, int m, int k, int lda, const double* X, long long ldx, double* Y,
-                        long long ldy, int nrhs, double alpha, double beta, hipStream_t s)
+// column-major terms: Y = alpha op(A^T) X + beta Y.  The outputs are few
+// (m x nrhs, e.g. 4096 x 64) against a long k, so a plain GEMM would run on a
+// few dozen workgroups: k is split into ks equal chunks, one strided-batched
+// GEMM writes ks partial products into stream-ordered scratch
+// (hipMallocAsync, returned to the pool after use) and k_splitk_sum adds them
+// in fixed order (deterministic) with alpha and beta.
+static rocblas_status gemm(rocblas_handle h, int m, int n, int k, const double* A, int lda, const double* X,
+                           int ldx, double* Y, int ldy, double alpha, double beta)
 {
-    rocblas_handle h = blas_handle();
-    if (rocblas_set_stream(h, s) != rocblas_status_success) throw Error(FP_E_HIP, "rocblas_set_stream failed");
-    const rocblas_status st = rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, m, nrhs, k,
-                                            &alpha, A, lda, X, (rocblas_int)ldx, &beta, Y, (rocblas_int)ldy);
-    if (st != rocblas_status_success) throw Error(FP_E_HIP, std::string("rocblas_dgemm: ") + rocblas_status_to_string(st));
+    return rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, m, n, k, &alpha, A, lda, X, ldx,
+                         &beta, Y, ldy);
 }
-static void dense_apply(const float* A, int m, int k, int lda, const float* X, long long ldx, float* Y, long long ldy,
-                        int nrhs, float alpha, float beta, hipStream_t s)
+static rocblas_status gemm(rocblas_handle h, int m, int n, int k, const float* A, int lda, const float* X, int ldx,
+                           float* Y, int ldy, float alpha, float beta)
+{
+    return rocblas_sgemm(h, rocblas_operation_transpose, rocblas_operation_none, m, n, k, &alpha, A, lda, X, ldx,
+                         &beta, Y, ldy);
+}
+static rocblas_status gemm_batched(rocblas_handle h, int m, int n, int k, const double* A, int lda, long long sa,
+                                   const double* X, int ldx, long long sx, double* W, int ldw, long long sw, int cnt)
+{
+    const double one = 1.0, zero = 0.0;
+    return rocblas_dgemm_strided_batched(h, rocblas_operation_transpose, rocblas_operation_none, m, n, k, &one, A,
+                                         lda, sa, X, ldx, sx, &zero, W, ldw, sw, cnt);
+}
+static rocblas_status gemm_batched(rocblas_handle h, int m, int n, int k, const float* A, int lda, long long sa,
+                                   const float* X, int ldx, long long sx, float* W, int ldw, long long sw, int cnt)
+{
+    const float one = 1.0f, zero = 0.0f;
+    return rocblas_sgemm_strided_batched(h, rocblas_operation_transpose, rocblas_operation_none, m, n, k, &one, A,
+                                         lda, sa, X, ldx, sx, &zero, W, ldw, sw, cnt);
+}
+
+template <typename T>
+__global__ void k_splitk_sum(const T* W, int ks, int m, int nrhs, T* Y, long long ldy, T alpha, T beta)
+{
+    const long long tot = (long long)m * nrhs;
+    for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < tot;
+         q += (long long)gridDim.x * blockDim.x) {
+        T acc = T(0);
+        for (int c = 0; c < ks; c++) acc += W[c * tot + q];
+        const long long r = q / m, i = q - r * m;
+        T&              y = Y[r * ldy + i];
+        y                 = alpha * acc + (beta != T(0) ? beta * y : T(0));
+    }
+}
+
+template <typename T>
+static void dense_apply(const T* A, int m, int k, int lda, const T* X, long long ldx, T* Y, long long ldy, int nrhs,
+                        T alpha, T beta, hipStream_t s)
 {
     rocblas_handle h = blas_handle();
     if (rocblas_set_stream(h, s) != rocblas_status_success) throw Error(FP_E_HIP, "rocblas_set_stream failed");
-    const rocblas_status st = rocblas_sgemm(h, rocblas_operation_transpose, rocblas_operation_none, m, nrhs, k,
-                                            &alpha, A, lda, X, (rocblas_int)ldx, &beta, Y, (rocblas_int)ldy);
-    if (st != rocblas_status_success) throw Error(FP_E_HIP, std::string("rocblas_sgemm: ") + rocblas_status_to_string(st));
+    // chunks of at least 128 of k, until about 512 tiles of 64 x 64 outputs
+    int ks = 1;
+    const long long tiles = (long long)((m + 63) / 64) * ((nrhs + 63) / 64);
+    while (ks < 16 && tiles * ks * 2 <= 512 && k % (2 * ks) == 0 && k / (2 * ks) >= 128) ks *= 2;
+    rocblas_status st;
+    if (ks == 1) {
+        st = gemm(h, m, nrhs, k, A, lda, X, (int)ldx, Y, (int)ldy, alpha, beta);
+    } else {
+        const int       kc  = k / ks;
+        const long long tot = (long long)m * nrhs;
+        T*              W   = nullptr;
+        FP_HIP(hipMallocAsync((void**)&W, sizeof(T) * (size_t)(tot * ks), s));
+        st = gemm_batched(h, m, nrhs, kc, A, lda, kc, X, (int)ldx, kc, W, m, tot, ks);
+        if (st == rocblas_status_success) {
+            const int nb = (int)std::min<long long>((tot + 255) / 256, 2048);
+            hipLaunchKernelGGL(k_splitk_sum<T>, dim3(nb), dim3(256), 0, s, W, ks, m, nrhs, Y, ldy, alpha, beta);
+        }
+        const hipError_t le = hipGetLastError();
+        FP_HIP(hipFreeAsync(W, s));
+        FP_HIP(le);
+    }
+    if (st != rocblas_status_success) throw Error(FP_E_HIP, std::string("rocblas gemm: ") + rocblas_status_to_string(st));
 }
 
 constexpr int VB = 256;  // threads per block of the vector kernels
