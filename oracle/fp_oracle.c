@@ -385,6 +385,62 @@ static void row_transform(int kind, int inverse, int ny, const double* in, doubl
             for (int j = 0; j < ny; j++) out[j] -= X[2 * (j + 1) + 1];
             for (int j = 0; j < ny; j++) out[j] = (j % 2 == 0 ? -1.0 : 1.0) * out[j];
         }
+    } else if (kind == T_PP) {
+        /* even ny; r2c of length ny.  ex is 4 ny zeroed values, X 4 bin sets */
+        const int    L  = ny, H = ny / 2;
+        const double sq = sqrt(2.0 / ny);
+        double*      e0 = ex;
+        double*      e1 = ex + ny;
+        double*      e2 = ex + 2 * ny;
+        double*      e3 = ex + 3 * ny;
+        double*      X0 = X;
+        double*      X1 = X + 2 * (H + 1);
+        double*      X2 = X + 4 * (H + 1);
+        double*      X3 = X + 6 * (H + 1);
+        memset(ex, 0, sizeof(double) * 4 * (size_t)ny);
+        if (!inverse) {
+            /* poisson_fft2d.cpp:367-403: r2c of [0, f_0 .. f_{N-2}] and of the
+             * same with alternating signs; ts / tc from their bins */
+            for (int j = 0; j < ny - 1; j++) e0[j + 1] = in[j];
+            for (int j = 0; j < ny - 1; j++) e1[j + 1] = (j % 2 == 0 ? 1.0 : -1.0) * in[j];
+            r2c_exec(L, e0, X0, s1, s2, s3);
+            r2c_exec(L, e1, X1, s1, s2, s3);
+            for (int m = 0; m < ny; m++) {
+                double ts = 0.0, tc;
+                if (m > 0) ts = -X0[2 * (H - (m + 1) / 2) + 1];  /* m = 2j+1 or 2j+2: bin N/2-j-1 */
+                tc = in[ny - 1] - X1[2 * ((m + 1) / 2)];         /* m = 2j: bin j; m = 2j+1: bin j+1 */
+                out[m] = sq * (cos(OR_PI * (2.0 * m + 1.0) / 4.0) * ts + sin(OR_PI * (2.0 * m + 1.0) / 4.0) * tc);
+            }
+        } else {
+            /* poisson_fft2d.cpp:406-465: four r2c of even/odd, cosine/sine
+             * weighted coefficients */
+            for (int j = 0; j < H; j++) {
+                e0[j]         = cos(OR_PI * (4.0 * j + 1.0) / 4.0) * in[2 * j];
+                e1[j + 1]     = cos(OR_PI * (4.0 * j + 3.0) / 4.0) * in[2 * j + 1];
+                e2[H - j]     = sin(OR_PI * (4.0 * j + 1.0) / 4.0) * in[2 * j];
+                e3[H - j - 1] = sin(OR_PI * (4.0 * j + 3.0) / 4.0) * in[2 * j + 1];
+            }
+            r2c_exec(L, e0, X0, s1, s2, s3);
+            r2c_exec(L, e1, X1, s1, s2, s3);
+            r2c_exec(L, e2, X2, s1, s2, s3);
+            r2c_exec(L, e3, X3, s1, s2, s3);
+            double* fc = (double*)calloc((size_t)ny, sizeof(double));
+            double* fs = (double*)calloc((size_t)ny, sizeof(double));
+            double  sg = -1.0;
+            for (int j = 0; j < H; j++) {
+                fc[j] = sg * (X0[2 * (j + 1) + 1] + X1[2 * (j + 1) + 1]);
+                fs[j] = X2[2 * (j + 1)] + X3[2 * (j + 1)];
+                sg    = -sg;
+            }
+            for (int j = 0; j < H - 1; j++) {
+                fc[H + j] = -fc[H - 2 - j];
+                fs[H + j] = fs[H - 2 - j];
+            }
+            fs[ny - 1] = X2[0] + X3[0];
+            for (int j = 0; j < ny; j++) out[j] = sq * (fc[j] + fs[j]);
+            free(fc);
+            free(fs);
+        }
     } else if (kind == T_DN) {
         /* poisson_fft2d.cpp:242-262: the same map both ways */
         const int    L   = 2 * ny + 1;
@@ -399,12 +455,13 @@ static void row_transform(int kind, int inverse, int ny, const double* in, doubl
 int or_transform(int bc_neg, int bc_pos, int inverse, int nx, int ny, const double* in, double* out)
 {
     const int kind = transform_kind(bc_neg, bc_pos);
-    if (kind == T_BAD || kind == T_PP || nx <= 0 || ny <= 0) return 1;
+    if (kind == T_BAD || nx <= 0 || ny <= 0) return 1;
+    if (kind == T_PP && (ny % 2 != 0 || ny < 2)) return 1;
     int ok = 0;
 #pragma omp parallel
     {
-        double* ex = (double*)malloc(sizeof(double) * (2 * (size_t)ny + 4));
-        double* X  = (double*)malloc(sizeof(double) * 2 * (2 * (size_t)ny + 4));
+        double* ex = (double*)malloc(sizeof(double) * (4 * (size_t)ny + 4));
+        double* X  = (double*)malloc(sizeof(double) * 2 * (4 * (size_t)ny + 8));
         double* s  = (double*)malloc(sizeof(double) * 6 * (2 * (size_t)ny + 4));
         double* o  = (double*)malloc(sizeof(double) * (size_t)ny);
 #pragma omp for schedule(dynamic)
@@ -464,6 +521,38 @@ void or_chasing_singular(int n, const double* f, double* x)
     for (int i = 2; i < n; i++) x[i] = f[i - 1] - x[i - 2] + 2.0 * x[i - 1];
 }
 
+/* chasing_periodic_precompute + chasing_periodic_with_precompute
+ * (chasing_method_base.cpp:178-258): Dirichlet-end Thomas for f and for the
+ * unit vectors e_0, e_{n-1} (z_1, z_2), then the Sherman-Morrison
+ * correction with the reference's gamma formulas. */
+void or_chasing_periodic(double d, int n, const double* f, double* x)
+{
+    double* y  = (double*)malloc(sizeof(double) * (size_t)n);
+    double* z1 = (double*)malloc(sizeof(double) * (size_t)n);
+    double* z2 = (double*)malloc(sizeof(double) * (size_t)n);
+    double* u  = (double*)calloc((size_t)n, sizeof(double));
+    or_chasing(d, n, OR_DIRICHLET, OR_DIRICHLET, f, y);
+    u[0] = 1.0;
+    or_chasing(d, n, OR_DIRICHLET, OR_DIRICHLET, u, z1);
+    u[0]     = 0.0;
+    u[n - 1] = 1.0;
+    or_chasing(d, n, OR_DIRICHLET, OR_DIRICHLET, u, z2);
+    const double g2 = (y[n - 1] / (1.0 + z1[n - 1]) - y[0] / z1[0]) /
+                      (z2[n - 1] / (1.0 + z1[n - 1]) - (1.0 + z2[0]) / z1[0]);
+    const double g1 = y[0] / z1[0] - g2 * ((1.0 + z2[0]) / z1[0]);
+    for (int i = 0; i < n; i++) x[i] = y[i] - g1 * z1[i] - g2 * z2[i];
+    free(y); free(z1); free(z2); free(u);
+}
+
+/* chasing_periodic_singular (chasing_method_base.cpp:274-285), p_const = 0 */
+void or_chasing_periodic_singular(int n, const double* f, double* x)
+{
+    x[0] = 0.0;
+    x[1] = f[0] / n;
+    for (int i = 1; i < n - 1; i++) x[1] = x[1] - (n - 1 - i) * f[i] / n;
+    for (int i = 2; i < n; i++) x[i] = f[i - 1] - x[i - 2] + 2.0 * x[i - 1];
+}
+
 /* ------------------------------------------------------------------------ */
 /* Subdomain solver (pe/poisson/poisson_solver2d.cpp:37-90)                   */
 /* ------------------------------------------------------------------------ */
@@ -480,7 +569,10 @@ typedef struct sub_solver {
 
 static sub_solver* sub_create(int nx, int ny, double hx, double hy, const int* bc)
 {
-    if (transform_kind(bc[2], bc[3]) == T_BAD || transform_kind(bc[2], bc[3]) == T_PP) return NULL;
+    if (transform_kind(bc[2], bc[3]) == T_BAD) return NULL;
+    if (transform_kind(bc[2], bc[3]) == T_PP && ny % 2 != 0) return NULL;
+    /* a periodic x face needs its partner (ChasingMethod2D::init, chasing_method2d.cpp:21) */
+    if ((bc[0] == OR_PERIODIC) != (bc[1] == OR_PERIODIC)) return NULL;
     if (nx < 2 || ny < 1) return NULL;
     sub_solver* s = (sub_solver*)calloc(1, sizeof(sub_solver));
     s->nx = nx; s->ny = ny; s->hx = hx; s->hy = hy;
@@ -516,8 +608,15 @@ static void sub_solve(sub_solver* s, double* f)
         double* o = (double*)malloc(sizeof(double) * nx);
 #pragma omp for schedule(dynamic)
         for (int j = 0; j < ny; j++) {
-            if (s->no_dirichlet && j == ny - 1) or_chasing_singular(nx, tt + (size_t)j * nx, o);
-            else or_chasing(s->x_diag[j], nx, s->bc[0], s->bc[1], tt + (size_t)j * nx, o);
+            const int per = s->bc[0] == OR_PERIODIC;
+            if (s->no_dirichlet && j == ny - 1) {
+                if (per) or_chasing_periodic_singular(nx, tt + (size_t)j * nx, o);
+                else or_chasing_singular(nx, tt + (size_t)j * nx, o);
+            } else if (per) {
+                or_chasing_periodic(s->x_diag[j], nx, tt + (size_t)j * nx, o);
+            } else {
+                or_chasing(s->x_diag[j], nx, s->bc[0], s->bc[1], tt + (size_t)j * nx, o);
+            }
             memcpy(t + (size_t)j * nx, o, sizeof(double) * nx);
         }
         free(o);

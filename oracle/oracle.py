@@ -41,6 +41,10 @@ def lib():
         L.or_chasing.restype = None
         L.or_chasing_singular.argtypes = [ctypes.c_int, _dp, _dp]
         L.or_chasing_singular.restype = None
+        L.or_chasing_periodic.argtypes = [ctypes.c_double, ctypes.c_int, _dp, _dp]
+        L.or_chasing_periodic.restype = None
+        L.or_chasing_periodic_singular.argtypes = [ctypes.c_int, _dp, _dp]
+        L.or_chasing_periodic_singular.restype = None
         L.or_sub_solve.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _ip, _dp]
         L.or_cat_create.restype = ctypes.c_void_p
         L.or_cat_create.argtypes = [ctypes.c_int, _ip, _ip, _dp, _dp, _ip, ctypes.c_int, _ip, ctypes.c_int,
@@ -106,6 +110,20 @@ def chasing_singular(f):
     f = _f64(f)
     x = np.empty_like(f)
     lib().or_chasing_singular(len(f), _p(f), _p(x))
+    return x
+
+
+def chasing_periodic(diag, f):
+    f = _f64(f)
+    x = np.empty_like(f)
+    lib().or_chasing_periodic(float(diag), len(f), _p(f), _p(x))
+    return x
+
+
+def chasing_periodic_singular(f):
+    f = _f64(f)
+    x = np.empty_like(f)
+    lib().or_chasing_periodic_singular(len(f), _p(f), _p(x))
     return x
 
 

@@ -11,6 +11,7 @@ concat_solver2d.cpp:110-223):
   Neumann face: ghost = adjacent cell; data g adds +g/h on XNeg/YNeg, -g/h on
     XPos/YPos.
   Adjacented face: the neighbour's adjacent cell enters with coefficient exactly 1.
+  Periodic pair of faces: the opposite line's cell is the neighbour.
 """
 import numpy as np
 import scipy.sparse as sp
@@ -77,6 +78,13 @@ def assemble(domains, conn, fields, buffers=None):
         rows.append(idx[:-1, :].ravel()); cols.append(idx[1:, :].ravel()); vals.append(np.ones((nx - 1) * ny))
         rows.append(idx[:, 1:].ravel()); cols.append(idx[:, :-1].ravel()); vals.append(np.full(nx * (ny - 1), r))
         rows.append(idx[:, :-1].ravel()); cols.append(idx[:, 1:].ravel()); vals.append(np.full(nx * (ny - 1), r))
+        # periodic axes: the first and last lines are neighbours
+        if bc[XNEG] == P and bc[XPOS] == P:
+            rows.append(idx[0, :]); cols.append(idx[nx - 1, :]); vals.append(np.ones(ny))
+            rows.append(idx[nx - 1, :]); cols.append(idx[0, :]); vals.append(np.ones(ny))
+        if bc[YNEG] == P and bc[YPOS] == P:
+            rows.append(idx[:, 0]); cols.append(idx[:, ny - 1]); vals.append(np.full(nx, r))
+            rows.append(idx[:, ny - 1]); cols.append(idx[:, 0]); vals.append(np.full(nx, r))
     for a, dr, b in conn:
         nxa, nya = domains[a][0], domains[a][1]
         nxb, nyb = domains[b][0], domains[b][1]

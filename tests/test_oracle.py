@@ -43,11 +43,14 @@ def y_operator(bcp, ny):
         M[0, 0] += 1
     if bcp[1] == N:
         M[-1, -1] += 1
+    if bcp == (P, P):
+        M[0, -1] += 1
+        M[-1, 0] += 1
     return M
 
 
-@pytest.mark.parametrize("bcp", [(D, D), (N, N), (D, N), (N, D), (A, D), (D, A), (A, N), (N, A)])
-@pytest.mark.parametrize("ny", [2, 5, 16, 33, 64])
+@pytest.mark.parametrize("bcp,ny", [(b, n) for b in [(D, D), (N, N), (D, N), (N, D), (A, D), (D, A), (A, N), (N, A)]
+                                     for n in [2, 5, 16, 33, 64]] + [((P, P), n) for n in [4, 6, 16, 34, 64]])
 def test_transform_is_orthonormal_eigenbasis(bcp, ny):
     Phi = basis(bcp, ny)
     PhiT = basis(bcp, ny, inverse=True)
@@ -70,6 +73,21 @@ def test_dd_is_scipy_dst1_and_nn_is_dct2():
     got = O.transform(N, N, x)
     # mode j <-> DCT frequency N-1-j (mean mode last); signs are the reference's own
     assert np.allclose(np.abs(got[:, ::-1]), np.abs(c), atol=1e-13)
+
+
+@pytest.mark.parametrize("ny", [4, 10, 64])
+def test_pp_transform_closed_form(ny):
+    """The periodic transform's matrix (poisson_fft2d.cpp:367-403) is
+    Phi[m, j] = sqrt(2/N) sin(2 pi q_m (j+1)/N + pi (2m+1)/4) with
+    q_m = N/2 - ceil(m/2): mode 0 the Nyquist mode, the pairs (2a-1, 2a)
+    one frequency each, mode N-1 the constant."""
+    Phi = basis((P, P), ny)
+    m = np.arange(ny)[:, None]
+    j = np.arange(ny)[None, :]
+    q = ny // 2 - (m + 1) // 2
+    ref = np.sqrt(2.0 / ny) * np.sin(2 * np.pi * q * (j + 1) / ny + np.pi * (2 * m + 1) / 4)
+    assert np.abs(Phi - ref).max() < 1e-13
+    assert np.allclose(np.abs(Phi[-1]), 1 / np.sqrt(ny))
 
 
 def test_x_diag_scaling():
@@ -102,6 +120,45 @@ def test_chasing_singular_gauge():
     x = O.chasing_singular(f)
     assert x[0] == 0.0
     assert np.abs(T @ x - f).max() < 1e-12
+
+
+def test_chasing_periodic_matches_dense():
+    rng = np.random.default_rng(15)
+    for n, d in ((17, -2.3), (8, -2.05), (3, -3.0)):
+        T = d * np.eye(n) + np.eye(n, k=1) + np.eye(n, k=-1)
+        T[0, -1] += 1
+        T[-1, 0] += 1
+        f = rng.standard_normal(n)
+        assert np.allclose(O.chasing_periodic(d, f), np.linalg.solve(T, f), rtol=1e-12, atol=1e-12)
+
+
+def test_chasing_periodic_singular_gauge():
+    rng = np.random.default_rng(16)
+    n = 12
+    T = -2 * np.eye(n) + np.eye(n, k=1) + np.eye(n, k=-1)
+    T[0, -1] += 1
+    T[-1, 0] += 1
+    f = rng.standard_normal(n)
+    f -= f.mean()
+    x = O.chasing_periodic_singular(f)
+    assert x[0] == 0.0
+    assert np.abs(T @ x - f).max() < 1e-12
+
+
+@pytest.mark.parametrize("bc", [(P, P, D, D), (D, D, P, P), (P, P, N, D), (N, A, P, P), (P, P, D, N), (D, N, P, P),
+                                (P, P, P, P), (P, P, N, N), (N, N, P, P)])
+def test_sub_solve_periodic_matches_sparse(bc):
+    rng = np.random.default_rng(20 + sum(bc))
+    nx, ny, hx, hy = 9, 8, 0.3, 0.5
+    f = rng.standard_normal((nx, ny))
+    singular = not any(b in (D, A) for b in bc)
+    if singular:
+        f -= f.mean()
+    u = O.sub_solve(hx * hx * f, hx, hy, bc)
+    M, rhs, _ = discrete.assemble([(nx, ny, hx, hy, bc)], [], [f])
+    assert np.abs(M @ u.ravel() - rhs).max() < 1e-11 * max(1.0, np.abs(rhs).max())
+    if not singular:
+        assert rel(u, discrete.solve([(nx, ny, hx, hy, bc)], [], [f])[0]) < 1e-12
 
 
 def test_poisson_3x4_golden():
