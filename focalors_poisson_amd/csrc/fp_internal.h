@@ -21,7 +21,7 @@ enum Face : int { XNEG = 0, XPOS = 1, YNEG = 2, YPOS = 3 };
 
 // y-transform kinds, chosen by the (yneg, ypos) pair exactly as
 // PoissonSolver2DBase::create_fft (pe/poisson/poisson_solver2d_base.cpp:64-91).
-enum TKind : int { T_DD = 0, T_NN = 1, T_DN = 2, T_ND = 3 };
+enum TKind : int { T_DD = 0, T_NN = 1, T_DN = 2, T_ND = 3, T_PP = 4 };
 
 inline bool dir_like(int b) { return b == DIRICHLET || b == ADJACENTED; }
 

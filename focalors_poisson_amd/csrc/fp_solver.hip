@@ -23,7 +23,8 @@ namespace fp {
 
 static int transform_kind(int neg, int pos)
 {
-    if (neg == PERIODIC || pos == PERIODIC) throw Error(FP_E_UNSUPPORTED, "periodic faces are not supported");
+    if (neg == PERIODIC && pos == PERIODIC) return T_PP;
+    if (neg == PERIODIC || pos == PERIODIC) throw Error(FP_E_INVALID, "a periodic face needs a periodic partner");
     if (neg == BC_NULL || pos == BC_NULL || neg < 0 || neg > 4 || pos < 0 || pos > 4)
         throw Error(FP_E_INVALID, "invalid boundary type");
     if (neg == NEUMANN && pos == NEUMANN) return T_NN;
@@ -36,10 +37,12 @@ void SubPlan::build(int nx_, int ny_, double hx_, double hy_, const int bc_[4])
 {
     FP_REQUIRE(nx_ >= 2 && ny_ >= 1, "subdomain needs nx >= 2 and ny >= 1");
     FP_REQUIRE(hx_ > 0 && hy_ > 0 && std::isfinite(hx_) && std::isfinite(hy_), "spacings must be positive");
-    for (int f = 0; f < 4; f++) {
-        if (bc_[f] == PERIODIC) throw Error(FP_E_UNSUPPORTED, "periodic faces are not supported");
-        FP_REQUIRE(bc_[f] == DIRICHLET || bc_[f] == NEUMANN || bc_[f] == ADJACENTED, "invalid boundary type");
-    }
+    for (int f = 0; f < 4; f++)
+        FP_REQUIRE(bc_[f] == DIRICHLET || bc_[f] == NEUMANN || bc_[f] == ADJACENTED || bc_[f] == PERIODIC,
+                   "invalid boundary type");
+    FP_REQUIRE((bc_[XNEG] == PERIODIC) == (bc_[XPOS] == PERIODIC), "a periodic face needs a periodic partner");
+    FP_REQUIRE(bc_[YNEG] != PERIODIC || ny_ % 2 == 0, "a periodic y axis needs an even ny");
+    if (bc_[XNEG] == PERIODIC) throw Error(FP_E_UNSUPPORTED, "periodic x faces are not supported");
     nx = nx_; ny = ny_; hx = hx_; hy = hy_;
     for (int f = 0; f < 4; f++) bc[f] = bc_[f];
     kind         = transform_kind(bc[YNEG], bc[YPOS]);
@@ -52,7 +55,8 @@ void SubPlan::build(int nx_, int ny_, double hx_, double hy_, const int bc_[4])
     for (int j = 0; j < ny; j++) {
         const int i = j + 1;
         double    lam;
-        if (kind == T_NN) lam = -2.0 * std::cos(PI / ny * i);
+        if (kind == T_PP) lam = -2.0 * std::cos(2.0 * PI / ny * std::floor(i / 2.0));
+        else if (kind == T_NN) lam = -2.0 * std::cos(PI / ny * i);
         else if (kind == T_DD) lam = 2.0 * std::cos(PI / (ny + 1) * i);
         else lam = -2.0 * std::cos(2.0 * PI * i / (2 * ny + 1));
         x_diag[j] = -2.0 + hx * hx / hy / hy * (lam - 2.0);

@@ -645,8 +645,38 @@ __device__ __forceinline__ void load_rows(const YArgs<T>& a, const RowPair<T>& r
     // load of a chunk issued before the first LDS write, so the HBM latency
     // is paid once per chunk instead of once per element.
     const int kind = a.kind, inv = a.inverse;
+    if (kind == T_PP && inv) {
+        // periodic inverse: for each frequency q <= N/2 the coefficients of its
+        // modes (m = 0 for q = N/2; 2a-1, 2a for q = N/2 - a; N-1 for q = 0)
+        // give G_q = sum p_m exp(i pi (2m+1)/4); the Hermitian spectrum
+        // X_q = G_q / 2i, X_{N-q} = conj X_q (X_0 = Im G_0, X_{N/2} = Im G_{N/2})
+        // of both rows packed as X_a + i X_b
+        const T r2 = T(0.70710678118654752440);
+        for (int q = threadIdx.x; 2 * q <= N; q += NT) {
+            const int a1 = N / 2 - q;  // modes 2 a1 - 1 and 2 a1 (when in range)
+            T         gar = 0, gai = 0, gbr = 0, gbi = 0;
+            for (int m = 2 * a1 - 1; m <= 2 * a1; m++) {
+                if (m < 0 || m >= N) continue;
+                const T  pa = ld_src(rp.ia + m), pb = rp.hb ? ld_src(rp.ia + N + m) : T(0);
+                const int r = m & 3;
+                const T   c = (r == 0 || r == 3) ? r2 : -r2, sn = (r < 2) ? r2 : -r2;
+                gar += c * pa;
+                gai += sn * pa;
+                gbr += c * pb;
+                gbi += sn * pb;
+            }
+            if (q == 0 || 2 * q == N) {
+                A[zp<PAD>(q)] = {gai, gbi};
+            } else {
+                A[zp<PAD>(q)]     = {T(0.5) * (gai + gbr), T(0.5) * (gbi - gar)};
+                A[zp<PAD>(N - q)] = {T(0.5) * (gai - gbr), T(0.5) * (gar + gbi)};
+            }
+        }
+        return;
+    }
     if (kind == T_DD) {
         if (threadIdx.x == 0) A[zp<PAD>(0)] = {T(0), T(0)};
+    } else if (kind == T_PP) {
     } else if (kind == T_NN && inv) {
         if (threadIdx.x == 0) A[zp<PAD>(N)] = {T(0), T(0)};
     } else if (kind != T_NN) {
@@ -672,6 +702,8 @@ __device__ __forceinline__ void load_rows(const YArgs<T>& a, const RowPair<T>& r
             }
             if (kind == T_DD) {
                 A[zp<PAD>(n + 1)] = {x, y};
+            } else if (kind == T_PP) {
+                A[zp<PAD>(n)] = {x, y};
             } else if (kind == T_NN && !inv) {
                 A[zp<PAD>(n)]             = {x, y};
                 A[zp<PAD>(2 * N - 1 - n)] = {x, y};
@@ -793,7 +825,7 @@ __device__ __forceinline__ void store_rows(const YArgs<T>& a, const RowPair<T>& 
                 const int j = base + k * NT + threadIdx.x;
                 sc[k]       = j < N ? a.aux[N - 1 - j] : C2<T>{T(0), T(0)};
             }
-        } else if (kind == T_ND && !inv) {
+        } else if ((kind == T_ND || kind == T_PP) && !inv) {
 #pragma unroll
             for (int k = 0; k < VPT; k++) {
                 const int j = base + k * NT + threadIdx.x;
@@ -815,6 +847,20 @@ __device__ __forceinline__ void store_rows(const YArgs<T>& a, const RowPair<T>& 
                 const C2<T> z = Z[zp<PAD>(j)];
                 oa            = T(0.5) * (z.x + y0[0]);
                 ob            = T(0.5) * (z.y + y0[1]);
+            } else if (kind == T_PP && !inv) {
+                // mode j of frequency q: sqrt(2/N) Im(w_j conj F_q), F_a / F_b
+                // of the rows split from Z_q and Z_{N-q}
+                const int   q  = N / 2 - (j + 1) / 2;
+                const C2<T> zq = Z[zp<PAD>(q)], zm = Z[zp<PAD>(q == 0 ? 0 : N - q)];
+                const T     far = T(0.5) * (zq.x + zm.x), fai = T(0.5) * (zq.y - zm.y);
+                const T     fbr = T(0.5) * (zq.y + zm.y), fbi = T(0.5) * (zm.x - zq.x);
+                oa              = a.c0 * (sc[k].y * far - sc[k].x * fai);
+                ob              = a.c0 * (sc[k].y * fbr - sc[k].x * fbi);
+            } else if (kind == T_PP) {
+                // the inverse ran the forward FFT: r_n = Z_{(N - n) mod N}, n = j + 1
+                const C2<T> z = Z[zp<PAD>(N - 1 - j)];
+                oa            = a.c0 * z.x;
+                ob            = a.c0 * z.y;
             } else if (kind == T_DN) {
                 const C2<T> zm = Z[zp<PAD>(j + 1)], zc = Z[zp<PAD>(L - 1 - j)];
                 const T     s  = sgn_pow<T>(j) * T(0.5);
@@ -1045,7 +1091,9 @@ void TransformPlan<T>::build(int k, int n)
     N    = n;
     if (kind == T_DD) L = N + 1;
     else if (kind == T_NN) L = 2 * N;
+    else if (kind == T_PP) L = N;
     else L = 2 * N + 1;
+    if (kind == T_PP && N % 2 != 0) throw Error(FP_E_INVALID, "the periodic transform needs an even length");
     // prime factors above 47: Bluestein over the smallest 2,3,5-smooth F >=
     // 2L - 1 when its single buffer fits the LDS, else the fallback engine
     F        = L;
@@ -1134,6 +1182,16 @@ void TransformPlan<T>::build(int k, int n)
         c0 = (T)std::sqrt(4.0 / (2.0 * N + 1.0));
     } else if (kind == T_DN) {
         c0 = (T)std::sqrt(4.0 / (2.0 * N + 1.0));
+    } else if (kind == T_PP) {
+        // w_m = exp(i (pi (2m+1)/4 + 2 pi q_m / N)), q_m = N/2 - ceil(m/2): the
+        // angle in eighths of a turn plus q_m / N turns, reduced exactly
+        for (int m = 0; m < N; m++) {
+            const long long   q   = N / 2 - (m + 1) / 2;
+            const long double ang = PI * (long double)((2LL * m + 1) % 8) / 4.0L + 2.0L * PI * (long double)q / N;
+            x[2 * m]              = (T)cosl(ang);
+            x[2 * m + 1]          = (T)sinl(ang);
+        }
+        c0 = (T)std::sqrt(2.0 / N);
     } else {
         // DD: sin(pi j / L), j < L, and the orthonormal scale sqrt(2 / (N+1))
         for (int j = 0; j < L; j++) x[2 * j] = (T)sinl(PI * (long double)j / (long double)L);

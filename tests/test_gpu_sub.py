@@ -11,7 +11,7 @@ import oracle as O
 from gpu_util import banner, device, rel, to_dev
 
 pytestmark = pytest.mark.gpu
-D, N, A = 0, 1, 3
+D, N, P, A = 0, 1, 2, 3
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
@@ -219,3 +219,49 @@ def test_fp32_refine_reaches_fp64_oracle(ctx, nx, ny, bc):
     err = rel(t.cpu().numpy(), ref)
     banner("refine nx=%d ny=%d converged in %d steps, rel err %.3e" % (nx, ny, info["refine_steps"], err))
     assert err < 1e-10 and info["refine_residual"] <= 1e-13
+
+
+# ---- periodic y (PoissonFFT2D_PP, poisson_fft2d.cpp:264-465) ----
+
+@pytest.mark.parametrize("ny", [2, 4, 64, 100, 256, 514, 1024, 2048])
+def test_periodic_transform_coefficients(ctx, ny):
+    """PoissonFFT2D_PP::transform / transform_transpose coefficient for
+    coefficient (mode 0 the Nyquist mode, mode ny-1 the constant); 514 =
+    2 x 257 runs by Bluestein."""
+    import torch
+    import focalors_poisson_amd as fpa
+    rng = np.random.default_rng(ny + 5)
+    nx = 7
+    f = rng.standard_normal((nx, ny))
+    s = fpa.PoissonSolver2D(ctx, nx, ny, 1.0, 1.0, (D, D, P, P))
+    for inv in (False, True):
+        t = to_dev(f)
+        banner("periodic transform ny=%d inverse=%d" % (ny, inv))
+        s.transform(t, inverse=inv)
+        torch.cuda.synchronize()
+        ref = O.transform(P, P, f, inverse=inv)
+        assert np.abs(t.cpu().numpy() - ref).max() < 1e-12 * max(1.0, np.abs(ref).max())
+
+
+@pytest.mark.parametrize("nx,ny,bc", [(9, 8, (D, D, P, P)), (9, 8, (N, D, P, P)), (9, 8, (A, N, P, P)),
+                                      (33, 100, (D, N, P, P)), (64, 514, (D, D, P, P)),
+                                      (512, 1024, (N, D, P, P)), (256, 256, (D, D, P, P))])
+def test_periodic_y_solve(ctx, nx, ny, bc):
+    rng = np.random.default_rng(nx + ny)
+    h = 1.0 / max(nx, ny)
+    f = rng.standard_normal((nx, ny)) * h * h
+    banner("periodic y nx=%d ny=%d bc=%s" % (nx, ny, bc))
+    assert rel(gpu_solve(ctx, f, h, h, bc), O.sub_solve(f, h, h, bc)) < 1e-12
+
+
+def test_periodic_y_singular_gauge(ctx):
+    """Neumann x ends with periodic y: the constant mode (last) takes the
+    reference's x0 = 0 gauge; compatible right-hand side."""
+    rng = np.random.default_rng(41)
+    nx, ny, h = 12, 16, 1.0 / 16
+    f = rng.standard_normal((nx, ny))
+    f -= f.mean()
+    f *= h * h
+    bc = (N, N, P, P)
+    banner("periodic y singular")
+    assert rel(gpu_solve(ctx, f, h, h, bc), O.sub_solve(f, h, h, bc)) < 1e-11
