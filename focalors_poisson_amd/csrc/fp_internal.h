@@ -117,6 +117,9 @@ struct SubPlan {
     DevBuf<double>        pst;      // ny*tri_p: Thomas pivot preceding each row chunk (mode-major)
     int                   tri_ch = 16, tri_p = 0;        // rows per chunk of the x sweep, chunks per mode
     int                   mb_field = 4, mb_lines = 4;    // modes per workgroup of the two sweeps
+    // periodic x: per mode A_D^-1 e_0 then A_D^-1 e_{nx-1} (2 nx ny, field layout)
+    bool                  cyc = false;
+    DevBuf<double>        zc;
     int                   dev = 0;
     void build(int nx, int ny, double hx, double hy, const int bc[4]);
     void ensure_f32();

@@ -13,6 +13,7 @@ struct RefineArgs {
     int           nx, ny;
     double        ry;                           // hx^2 / hy^2
     int           nxneg, nxpos, nyneg, nypos;   // Neumann faces
+    int           perx, pery;                   // periodic axes (first and last lines neighbours)
     const double* u_old;                        // null: zero
     double*       u_new;
     const float*  d;                            // FP32 correction
@@ -48,6 +49,7 @@ struct CatResidualArgs {
     int           nx, ny;
     double        ry;           // hx^2 / hy^2
     int           neu[4];       // Neumann faces
+    int           perx, pery;   // periodic axes
     const double* nb[4];        // neighbour field on Adjacented faces, else null
     int           nbx[4], nby[4];
     long long     ldn[4];

@@ -48,12 +48,16 @@ __global__ __launch_bounds__(RB) void k_refine_step(RefineArgs a)
         double       ax = -2.0 * c, ay = -2.0 * c;
         if (i > 0) ax += U(q - ny);
         else if (a.nxneg) ax += c;
+        else if (a.perx) ax += U(q + (long long)(nx - 1) * ny);
         if (i < nx - 1) ax += U(q + ny);
         else if (a.nxpos) ax += c;
+        else if (a.perx) ax += U(q - (long long)(nx - 1) * ny);
         if (j > 0) ay += U(q - 1);
         else if (a.nyneg) ay += c;
+        else if (a.pery) ay += U(q + ny - 1);
         if (j < ny - 1) ay += U(q + 1);
         else if (a.nypos) ay += c;
+        else if (a.pery) ay += U(q - (ny - 1));
         const double Au = ax + a.ry * ay;
         un[q]           = c;
         rr[q]           = (float)(b[q] - Au);
@@ -121,6 +125,8 @@ RefineStats refine_solve(SubPlan& sp, double* f, long long ld, int nrhs, double 
     a.nxpos = sp.bc[XPOS] == NEUMANN;
     a.nyneg = sp.bc[YNEG] == NEUMANN;
     a.nypos = sp.bc[YPOS] == NEUMANN;
+    a.perx  = sp.bc[XNEG] == PERIODIC;
+    a.pery  = sp.bc[YNEG] == PERIODIC;
     a.b = b.p;
     a.ldb = n;
     a.d = d.p;
@@ -183,15 +189,19 @@ __global__ __launch_bounds__(RB) void k_cat_residual(CatResidualArgs a)
         double       ax = -2.0 * c, ay = -2.0 * c, cpl = 0.0;
         if (i > 0) ax += u[q - ny];
         else if (a.neu[XNEG]) ax += c;
+        else if (a.perx) ax += u[q + (long long)(nx - 1) * ny];
         else if (a.nb[XNEG]) cpl += a.nb[XNEG][r * a.ldn[XNEG] + (long long)(a.nbx[XNEG] - 1) * a.nby[XNEG] + j];
         if (i < nx - 1) ax += u[q + ny];
         else if (a.neu[XPOS]) ax += c;
+        else if (a.perx) ax += u[q - (long long)(nx - 1) * ny];
         else if (a.nb[XPOS]) cpl += a.nb[XPOS][r * a.ldn[XPOS] + j];
         if (j > 0) ay += u[q - 1];
         else if (a.neu[YNEG]) ay += c;
+        else if (a.pery) ay += u[q + ny - 1];
         else if (a.nb[YNEG]) cpl += a.nb[YNEG][r * a.ldn[YNEG] + (long long)i * a.nby[YNEG] + a.nby[YNEG] - 1];
         if (j < ny - 1) ay += u[q + 1];
         else if (a.neu[YPOS]) ay += c;
+        else if (a.pery) ay += u[q - (ny - 1)];
         else if (a.nb[YPOS]) cpl += a.nb[YPOS][r * a.ldn[YPOS] + (long long)i * a.nby[YPOS]];
         rr[q] = b[q] - cpl - (ax + a.ry * ay);
     }

@@ -42,7 +42,6 @@ void SubPlan::build(int nx_, int ny_, double hx_, double hy_, const int bc_[4])
                    "invalid boundary type");
     FP_REQUIRE((bc_[XNEG] == PERIODIC) == (bc_[XPOS] == PERIODIC), "a periodic face needs a periodic partner");
     FP_REQUIRE(bc_[YNEG] != PERIODIC || ny_ % 2 == 0, "a periodic y axis needs an even ny");
-    if (bc_[XNEG] == PERIODIC) throw Error(FP_E_UNSUPPORTED, "periodic x faces are not supported");
     nx = nx_; ny = ny_; hx = hx_; hy = hy_;
     for (int f = 0; f < 4; f++) bc[f] = bc_[f];
     kind         = transform_kind(bc[YNEG], bc[YPOS]);
@@ -63,6 +62,31 @@ void SubPlan::build(int nx_, int ny_, double hx_, double hy_, const int bc_[4])
     }
     xd.alloc(ny);
     xd.upload(x_diag.data(), ny);
+    // periodic x (ChasingMethod2D::init, chasing_method2d.cpp:21-33 with
+    // chasing_periodic_precompute, chasing_method_base.cpp:232-258): the two
+    // correction vectors of every mode, Dirichlet-end Thomas in long double
+    cyc = bc[XNEG] == PERIODIC;
+    if (cyc) {
+        std::vector<double>      z(2 * (size_t)nx * ny);
+        std::vector<long double> cp(nx), y(nx);
+        for (int j = 0; j < ny; j++) {
+            const long double d = x_diag[j];
+            for (int e = 0; e < 2; e++) {
+                // forward elimination of tridiag(1, d, 1) with the unit right-hand side e_0 / e_{nx-1}
+                cp[0] = 1.0L / d;
+                y[0]  = (e == 0 ? 1.0L : 0.0L) / d;
+                for (int i = 1; i < nx; i++) {
+                    const long double den = d - cp[i - 1];
+                    cp[i]                 = 1.0L / den;
+                    y[i]                  = ((e == 1 && i == nx - 1 ? 1.0L : 0.0L) - y[i - 1]) / den;
+                }
+                for (int i = nx - 2; i >= 0; i--) y[i] -= cp[i] * y[i + 1];
+                for (int i = 0; i < nx; i++) z[(size_t)e * nx * ny + (size_t)i * ny + j] = (double)y[i];
+            }
+        }
+        zc.alloc(z.size());
+        zc.upload(z.data(), z.size());
+    }
     // reciprocal Thomas pivots (the reference's c vector,
     // chasing_method_base.cpp:33-46) are regenerated inside the sweep from
     // one stored pivot per row chunk; the chunk schedule is tuned here
@@ -457,9 +481,13 @@ static bool full_field_matvec()
     return v;
 }
 
+// The full-field matvec for this node: when asked for, and always for a
+// periodic x axis (the line-sourced operator runs Dirichlet-end sweeps only).
+static bool node_full_matvec(const Node& nd) { return full_field_matvec() || nd.plan.cyc; }
+
 static void matvec(fp_cat_s* c, Node& nd, const double* v, double* out, int nrhs, hipStream_t s)
 {
-    if (!full_field_matvec()) {
+    if (!node_full_matvec(nd)) {
         matvec_lines(c, nd, v, out, nrhs, s);
         return;
     }
@@ -549,7 +577,7 @@ static bool gmres_loop(fp_cat_s* c, Node& nd, int nrhs, hipStream_t s, const Gmr
                        bool first_pending)
 {
     const int  K = nd.K, m = c->m;
-    const bool lines = !full_field_matvec() && (!nd.has_x || nd.spectral_x);
+    const bool lines = !node_full_matvec(nd) && (!nd.has_x || nd.spectral_x);
     int        restarts = 0;
     bool       fresh    = false;  // host copy of the state is current
     bool       lb_x     = false;
@@ -607,7 +635,7 @@ static bool gmres_iterate(fp_cat_s* c, Node& nd, int nrhs, hipStream_t s)
         // memory and read after the replay; no host synchronisation here
         matvec(c, nd, nd.X.p, nd.Wv.p, nrhs, s);
         FP_REQUIRE(fused && gm_first(g, c->tol, c->tol_rel, s), "graph capture needs the one-workgroup GMRES start");
-        return !full_field_matvec() && (!nd.has_x || nd.spectral_x);
+        return !node_full_matvec(nd) && (!nd.has_x || nd.spectral_x);
     }
     return gmres_loop(c, nd, nrhs, s, g, fused, true);
 }
@@ -769,7 +797,7 @@ static void build_node(fp_cat_s* c, int d, hipStream_t s)
     c->nodes[d] = std::move(nd);
     Node& N = *c->nodes[d];
     for (auto& L : N.ch) build_schur(c, N, L, s);
-    N.spectral_x = N.has_x && !full_field_matvec();
+    N.spectral_x = N.has_x && !node_full_matvec(N);
     for (auto& L : N.ch) {
         if (L.loc != XNEG && L.loc != XPOS) continue;
         N.nxl++;
@@ -802,7 +830,7 @@ static void build_node(fp_cat_s* c, int d, hipStream_t s)
         launch_transpose(L.Qt.p, L.Q.p, n, s);
         FP_HIP(hipStreamSynchronize(s));
     }
-    if (c->precond_kind == 1 && N.K > 0 && N.K <= 16384) build_interface_inverse(c, N, s);
+    if (c->precond_kind == 1 && N.K > 0 && N.K <= 16384 && !N.plan.cyc) build_interface_inverse(c, N, s);
 }
 
 // TreeUtils::findOptimalRoot / buildTreeMapFromRoot / BFS
@@ -980,6 +1008,8 @@ static void cat_residual(fp_cat_s* c, double* const* u, int nrhs, hipStream_t s)
         a.nx  = c->nx[d];
         a.ny  = c->ny[d];
         a.ry  = c->hx[d] * c->hx[d] / (c->hy[d] * c->hy[d]);
+        a.perx = c->bc[4 * d + XNEG] == PERIODIC;
+        a.pery = c->bc[4 * d + YNEG] == PERIODIC;
         for (int face = 0; face < 4; face++) {
             a.neu[face] = c->bc[4 * d + face] == NEUMANN;
             const int nb = c->adj[4 * d + face];
@@ -1113,6 +1143,8 @@ static void cat_refine(fp_cat_s* c, double* const* f, int nrhs, hipStream_t s)
 static bool fast_shape_ok(const fp_cat_s* c)
 {
     if (c->nlev > 2) return false;
+    for (int d = 0; d < c->nd; d++)
+        if (c->nodes[d]->plan.cyc) return false;  // the line sweeps are Dirichlet-end only
     const Node& R = *c->nodes[c->root];
     return R.ch.empty() || !R.has_x || R.spectral_x;
 }

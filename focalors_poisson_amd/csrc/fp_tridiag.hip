@@ -139,6 +139,11 @@ struct TriProb {
     const double* xd;   // x_diag (ny)
     const double* pst;  // chunk-start pivots, P per mode
     int           nx, ny, P, MB, singular_last, nblk, ns, ne;  // nblk: mode blocks per right-hand side
+    // periodic x (cyc): z1 = A_D^-1 e_0 and z2 = A_D^-1 e_{nx-1} per mode, laid
+    // out like the field (A_D: the Dirichlet-end tridiagonal of the mode)
+    int           cyc;
+    const double* z1;
+    const double* z2;
 };
 template <typename T>
 struct TriBatch {
@@ -269,6 +274,42 @@ __global__ __launch_bounds__(tri_threads<CH>()) void k_tridiag(TriBatch<T> bt)
                 fv[k]     = x;  // x_i = sum_{q<i} d_q
                 x += d;
             }
+    }
+    if (pr.cyc) {
+        // periodic x (ChasingMethodBase::chasing_periodic_with_precompute /
+        // chasing_periodic_singular, chasing_method_base.cpp:178-285): the
+        // sweep above solved the Dirichlet-end system; the mode's first and
+        // last values meet through the workgroup's LDS (every chain read of
+        // SA / SB ended at the barrier above) and the Sherman-Morrison
+        // correction x = y - g1 z1 - g2 z2 follows with the reference's gamma
+        // formulas.  The singular mode ran the Neumann-singular recurrence
+        // x^N; the periodic one with the same gauge is x^N_i - (i / nx) x^N_{nx-1}.
+        if (act && c == 0) SA[jl] = fv[0];
+#pragma unroll
+        for (int k = 0; k < CH; k++)
+            if (act && i0 + k == nx - 1) SB[jl] = fv[k];
+        __syncthreads();
+        if (act) {
+            const T y0 = SA[jl], y1 = SB[jl];
+            if (sing) {
+#pragma unroll
+                for (int k = 0; k < CH; k++)
+                    if (i0 + k < nx) fv[k] -= (T)(i0 + k) / (T)nx * y1;
+            } else {
+                // the 2 x 2 system nearly cancels for the lowest modes: FP64
+                const double a0 = pr.z1[j], a1 = pr.z1[(long long)(nx - 1) * ny + j];
+                const double b0 = pr.z2[j], b1 = pr.z2[(long long)(nx - 1) * ny + j];
+                const double w0 = (double)y0, w1 = (double)y1;
+                const double h2 = (w1 / (1.0 + a1) - w0 / a0) / (b1 / (1.0 + a1) - (1.0 + b0) / a0);
+                const T      g2 = (T)h2;
+                const T      g1 = (T)(w0 / a0 - h2 * ((1.0 + b0) / a0));
+#pragma unroll
+                for (int k = 0; k < CH; k++) {
+                    const long long q = (long long)(i0 + k) * ny + j;
+                    if (i0 + k < nx) fv[k] -= g1 * (T)pr.z1[q] + g2 * (T)pr.z2[q];
+                }
+            }
+        }
     }
 #pragma unroll
     for (int k = 0; k < CH; k++) {
@@ -605,6 +646,9 @@ static void launch_tridiag_group(const TridiagJob<T>* const* jobs, int n, hipStr
         pr.nblk          = (sp.ny + MB - 1) / MB;
         pr.ns            = sp.bc[XNEG] == NEUMANN;
         pr.ne            = sp.bc[XPOS] == NEUMANN;
+        pr.cyc           = sp.cyc ? 1 : 0;
+        pr.z1            = sp.cyc ? sp.zc.p : nullptr;
+        pr.z2            = sp.cyc ? sp.zc.p + (size_t)sp.nx * sp.ny : nullptr;
         bt.start[q]      = blocks;
         const long long nb = (long long)pr.nblk * jb.nrhs;
         if (blocks + nb > (1LL << 30)) throw Error(FP_E_INVALID, "too many tridiagonal workgroups in one launch");
@@ -667,6 +711,7 @@ template <typename T>
 int launch_tridiag_lines(const SubPlan& sp, const LinesArgs<T>& a, int nrhs, hipStream_t s)
 {
     FP_REQUIRE(!sp.no_dirichlet, "line operator needs a Dirichlet-like face");
+    FP_REQUIRE(!sp.cyc, "line operator of a periodic x axis");
     LinesArgs<T> b = a;
     b.xd           = sp.xd.p;
     b.pst          = sp.pst.p;
@@ -689,6 +734,7 @@ static void launch_sweep_group(const SweepJob<T>* const* jobs, int n, hipStream_
         const SweepJob<T>& jb = *jobs[q];
         const SubPlan&     sp = *jb.sp;
         FP_REQUIRE(!sp.no_dirichlet, "the composite sweep needs a Dirichlet-like face");
+        FP_REQUIRE(!sp.cyc, "the composite sweep of a periodic x axis");
         const int P  = sp.tri_p;
         const int MB = tri_mb(P, CH, sp.mb_field);
         SweepProb<T>& pr = bt.p[q];

@@ -52,7 +52,7 @@ typedef enum fp_status {
     FP_E_HIP            = 3, /* HIP runtime error */
     FP_E_NOT_CONVERGED  = 4, /* GMRES stopped at max_iter above tolerance (results still written) */
     FP_E_NUMERIC        = 5, /* NaN/Inf met in the iteration */
-    FP_E_UNSUPPORTED    = 6  /* valid in the reference but not provided here (periodic faces) */
+    FP_E_UNSUPPORTED    = 6  /* valid in the reference but not provided here */
 } fp_status;
 
 typedef enum fp_precision {

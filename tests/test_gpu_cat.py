@@ -199,3 +199,39 @@ def test_cross_multi_rhs_split_gemm(ctx):
     ref, _, _ = O.Composite(doms, conn, tol=1e-14, tol_is_relative=True, maxiter=400,
                             schur_mode=O.SCHUR_SPECTRAL).solve([f[0] for f in fields])
     assert rel([u[0] for u in us], ref) < 1e-10, info
+
+
+P = 2
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_periodic_y_channel_two_level(ctx, precision):
+    """Two domains side by side, both periodic in y (PP transforms): the
+    two-level transform-space path with x-lines in the periodic basis."""
+    rng = np.random.default_rng(51)
+    h = 1.0 / 64
+    doms = [(48, 64, h, h, (D, A, P, P)), (40, 64, h, h, (A, N, P, P))]
+    conn = [(0, 1, 1)]
+    fs = [rng.standard_normal((d[0], d[1])) for d in doms]
+    banner("periodic y channel precision=%d" % precision)
+    us, info, _ = run(ctx, doms, conn, fs, precision=precision)
+    ref, _, _ = O.Composite(doms, conn, tol=1e-14, tol_is_relative=True, maxiter=400).solve(fs)
+    assert rel(us, ref) < 1e-10, info
+    import discrete
+    assert rel(us, discrete.solve(doms, conn, fs)) < 1e-10
+
+
+def test_periodic_x_root_with_child(ctx):
+    """A root periodic in x with a child on its y+ face (also periodic in x):
+    the general sweep with periodic Thomas, full-field interface matvec."""
+    rng = np.random.default_rng(52)
+    h = 1.0 / 64
+    doms = [(64, 32, h, h, (P, P, D, A)), (64, 24, h, h, (P, P, A, D))]
+    conn = [(0, 3, 1)]
+    fs = [rng.standard_normal((d[0], d[1])) for d in doms]
+    banner("periodic x root")
+    us, info, _ = run(ctx, doms, conn, fs, m=80)
+    import discrete
+    assert rel(us, discrete.solve(doms, conn, fs)) < 1e-10, info
+    ref, _, _ = O.Composite(doms, conn, tol=1e-14, tol_is_relative=True, maxiter=400).solve(fs)
+    assert rel(us, ref) < 1e-10, info

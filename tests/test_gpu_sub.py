@@ -166,9 +166,12 @@ def test_errors_are_status_codes(ctx):
     with pytest.raises(fpa.FPError) as e:
         fpa.PoissonSolver2D(ctx, 1, 8, 1.0, 1.0, (D, D, D, D))
     assert e.value.status == 1
-    with pytest.raises(fpa.FPError) as e:
-        fpa.PoissonSolver2D(ctx, 8, 8, 1.0, 1.0, (2, 2, D, D))
-    assert e.value.status == 6
+    with pytest.raises(fpa.FPError) as e:  # a periodic face without its partner
+        fpa.PoissonSolver2D(ctx, 8, 8, 1.0, 1.0, (P, D, D, D))
+    assert e.value.status == 1
+    with pytest.raises(fpa.FPError) as e:  # periodic y needs an even ny
+        fpa.PoissonSolver2D(ctx, 8, 7, 1.0, 1.0, (D, D, P, P))
+    assert e.value.status == 1
 
 
 @pytest.mark.parametrize("bcp", [(D, D), (N, N), (D, N), (N, D)])
@@ -265,3 +268,47 @@ def test_periodic_y_singular_gauge(ctx):
     bc = (N, N, P, P)
     banner("periodic y singular")
     assert rel(gpu_solve(ctx, f, h, h, bc), O.sub_solve(f, h, h, bc)) < 1e-11
+
+
+# ---- periodic x (ChasingMethodBase::chasing_periodic_*, chasing_method_base.cpp:178-285) ----
+
+@pytest.mark.parametrize("nx,ny,bc", [(2, 5, (P, P, D, D)), (3, 4, (P, P, D, N)), (9, 7, (P, P, D, D)),
+                                      (9, 7, (P, P, N, D)), (9, 8, (P, P, A, N)), (64, 257, (P, P, D, D)),
+                                      (1024, 37, (P, P, D, N)), (512, 1024, (P, P, D, D)), (100, 64, (P, P, N, D)),
+                                      (48, 64, (P, P, D, D))])
+def test_periodic_x_solve(ctx, nx, ny, bc):
+    rng = np.random.default_rng(nx * 7 + ny)
+    h = 1.0 / max(nx, ny)
+    f = rng.standard_normal((nx, ny)) * h * h
+    banner("periodic x nx=%d ny=%d bc=%s" % (nx, ny, bc))
+    assert rel(gpu_solve(ctx, f, h, h, bc), O.sub_solve(f, h, h, bc)) < 1e-11
+
+
+@pytest.mark.parametrize("nx,ny,bc", [(12, 16, (P, P, P, P)), (12, 9, (P, P, N, N)), (256, 256, (P, P, P, P))])
+def test_periodic_x_singular_gauge(ctx, nx, ny, bc):
+    """No Dirichlet face: the constant mode of the periodic x axis takes the
+    reference's x0 = 0 gauge (compatible right-hand side)."""
+    rng = np.random.default_rng(nx + 3 * ny)
+    h = 1.0 / max(nx, ny)
+    f = rng.standard_normal((nx, ny))
+    f -= f.mean()
+    f *= h * h
+    banner("periodic x singular nx=%d ny=%d bc=%s" % (nx, ny, bc))
+    assert rel(gpu_solve(ctx, f, h, h, bc), O.sub_solve(f, h, h, bc)) < 1e-10
+
+
+@pytest.mark.parametrize("bc", [(P, P, D, N), (D, N, P, P)])
+def test_periodic_fp32_refine(ctx, bc):
+    """FP32 inner solves with the FP64 residual of the periodic operator."""
+    import torch
+    import focalors_poisson_amd as fpa
+    rng = np.random.default_rng(77)
+    nx, ny = 256, 128
+    h = 1.0 / 256
+    f = rng.standard_normal((nx, ny)) * h * h
+    banner("periodic fp32 refine bc=%s" % (bc,))
+    s = fpa.PoissonSolver2D(ctx, nx, ny, h, h, bc)
+    t = to_dev(f)
+    info = s.solve_refine(t, tol=1e-12, max_steps=12)
+    torch.cuda.synchronize()
+    assert rel(t.cpu().numpy(), O.sub_solve(f, h, h, bc)) < 1e-10, info
