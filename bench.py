@@ -274,8 +274,8 @@ def extras(fpa, ctx, torch, np, n):
 def main():
     a = parse()
     if a.gpus != 1:
-        print("bench.py runs on one GPU: this build has no multi-GPU path (the composite solve would shard by "
-              "subdomain with interface all-gathers; see DESIGN.md). Use --gpus 1.", file=sys.stderr)
+        print("bench.py runs on one GPU: the solve path has no multi-GPU form (the composite solve would shard "
+              "by subdomain, exchanging interface lines between ranks). Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch
