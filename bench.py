@@ -158,12 +158,10 @@ def extras(fpa, ctx, torch, np, n):
     rhs = [torch.from_numpy(rng.uniform(-1, 1, (nx, ny))).to(dev) for nx, ny in sizes]
     cells = sum(nx * ny for nx, ny in sizes)
     for prec, dt in (("f64", torch.float64), ("f32", torch.float32)):
-        src = [r.to(dt) for r in rhs]
-        wk = [torch.empty_like(r) for r in src]
+        src, dst, wk = flat_fields(torch, [r.to(dt) for r in rhs])
 
         def batch():
-            for w, r in zip(wk, src):
-                w.copy_(r)
+            dst.copy_(src)  # one copy refills all 512 right-hand sides
             fpa.solve_batched(ctx, subs, wk)
 
         t = timed(batch, torch)

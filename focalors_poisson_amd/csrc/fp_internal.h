@@ -6,6 +6,7 @@
 
 #include <array>
 #include <cstdint>
+#include <cstring>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -72,8 +73,57 @@ struct DevBuf {
 };
 
 constexpr int kMaxRadices = 24;
-// subdomain problems per batched launch (descriptors travel as kernel arguments)
+// subdomain problems whose descriptors travel as kernel arguments; larger
+// batches put them in device memory (JobTable) and still take one launch
 constexpr int kMaxBatch = 8;
+
+// Descriptors of a batch too large for kernel arguments, in device memory:
+// problem q owns workgroups start[q] .. start[q+1]-1.
+template <typename P>
+struct JobTable {
+    int        nprob;
+    const int* start;
+    const P*   p;
+};
+// the problem of workgroup blk in a table (binary search over start)
+template <typename P>
+__device__ __forceinline__ const P& table_find(const JobTable<P>& t, int blk, int& local)
+{
+    int lo = 0, hi = t.nprob - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (blk >= t.start[mid]) lo = mid;
+        else hi = mid - 1;
+    }
+    local = blk - t.start[lo];
+    return t.p[lo];
+}
+// Copies n descriptors and their workgroup offsets into stream-ordered device
+// memory (released with table_release after the launch on the same stream).
+// Not for a stream under graph capture: the copy reads host memory.
+template <typename P>
+JobTable<P> table_upload(const std::vector<P>& p, const std::vector<int>& start, hipStream_t s, void*& mem)
+{
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    FP_HIP(hipStreamIsCapturing(s, &cs));
+    if (cs != hipStreamCaptureStatusNone)
+        throw Error(FP_E_INVALID, "batches of more than 8 subdomains cannot be captured");
+    const size_t pb = (p.size() * sizeof(P) + 255) & ~(size_t)255;
+    std::vector<unsigned char> h(pb + start.size() * sizeof(int));
+    std::memcpy(h.data(), p.data(), p.size() * sizeof(P));
+    std::memcpy(h.data() + pb, start.data(), start.size() * sizeof(int));
+    FP_HIP(hipMallocAsync(&mem, h.size(), s));
+    FP_HIP(hipMemcpyAsync(mem, h.data(), h.size(), hipMemcpyHostToDevice, s));
+    JobTable<P> t;
+    t.nprob = (int)p.size();
+    t.p     = static_cast<const P*>(mem);
+    t.start = reinterpret_cast<const int*>(static_cast<unsigned char*>(mem) + pb);
+    return t;
+}
+inline void table_release(void* mem, hipStream_t s)
+{
+    if (mem) FP_HIP(hipFreeAsync(mem, s));
+}
 
 // A batched y transform of one subdomain size and boundary pair: the packed
 // complex FFT of length L (two field rows per FFT) with its radix schedule

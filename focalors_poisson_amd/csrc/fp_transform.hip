@@ -893,12 +893,17 @@ __device__ __forceinline__ const YArgs<T>& find_problem(const YBatch<T>& bt, int
     local = (int)blockIdx.x - bt.start[q];
     return bt.p[q];
 }
+template <typename T>
+__device__ __forceinline__ const YArgs<T>& find_problem(const JobTable<YArgs<T>>& bt, int& local)
+{
+    return table_find(bt, (int)blockIdx.x, local);
+}
 
 // threads per workgroup: chosen per plan (128, 256 or 512)
 constexpr int kMaxNT = 512;
 
-template <typename T, int OM>
-__global__ __launch_bounds__(kMaxNT) void k_ytransform_sb(YBatch<T> bt)
+template <typename T, int OM, typename BT>
+__global__ __launch_bounds__(kMaxNT) void k_ytransform_sb(BT bt)
 {
     int             local;
     const YArgs<T>& a  = find_problem(bt, local);
@@ -933,8 +938,8 @@ __global__ __launch_bounds__(kMaxNT) void k_ytransform_sb(YBatch<T> bt)
 
 __host__ __device__ __forceinline__ bool gen_scan_in_buffer(int L, int N) { return L >= N / 2 + 1 + 8; }
 
-template <typename T>
-__global__ __launch_bounds__(kMaxNT) void k_ytransform_gen(YBatch<T> bt)
+template <typename T, typename BT>
+__global__ __launch_bounds__(kMaxNT) void k_ytransform_gen(BT bt)
 {
     int             local;
     const YArgs<T>& a  = find_problem(bt, local);
@@ -1299,48 +1304,78 @@ static YArgs<T> yargs(const TransformJob<T>& j, int inverse)
     return a;
 }
 
-// one launch of up to kMaxBatch jobs of one engine, thread count and
-// odd-prime form
+// The kernel of an engine and odd-prime form for the batch type B (YBatch
+// by value, or a JobTable in device memory).
+template <typename T, typename B>
+static void launch_kernel(int engine, int odd, int blocks, int NT, size_t lds, hipStream_t s, const B& bt)
+{
+    const void* fn = engine == 0         ? (const void*)k_ytransform_gen<T, B>
+                     : odd == ODD_NONE   ? (const void*)k_ytransform_sb<T, ODD_NONE, B>
+                     : odd == ODD_MFMA   ? (const void*)k_ytransform_sb<T, ODD_MFMA, B>
+                                         : (const void*)k_ytransform_sb<T, ODD_VALU, B>;
+    static bool attr_set[4] = {false, false, false, false};
+    const int   ki          = engine == 0 ? 3 : odd;
+    if (lds > 64 * 1024 && !attr_set[ki]) {
+        FP_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        attr_set[ki] = true;
+    }
+    if (engine == 0) hipLaunchKernelGGL((k_ytransform_gen<T, B>), dim3(blocks), dim3(NT), lds, s, bt);
+    else if (odd == ODD_NONE)
+        hipLaunchKernelGGL((k_ytransform_sb<T, ODD_NONE, B>), dim3(blocks), dim3(NT), lds, s, bt);
+    else if (odd == ODD_MFMA)
+        hipLaunchKernelGGL((k_ytransform_sb<T, ODD_MFMA, B>), dim3(blocks), dim3(NT), lds, s, bt);
+    else hipLaunchKernelGGL((k_ytransform_sb<T, ODD_VALU, B>), dim3(blocks), dim3(NT), lds, s, bt);
+    FP_HIP(hipGetLastError());
+}
+
+// One launch for the jobs of one engine, thread count and odd-prime form:
+// their descriptors as kernel arguments up to kMaxBatch jobs, else in a
+// device-memory table.
 template <typename T>
 static void launch_group(const TransformJob<T>* const* jobs, int n, int engine, int NT, int odd, int inverse,
                          hipStream_t s)
 {
-    YBatch<T> bt{};
-    size_t    lds    = 0;
-    int       blocks = 0;
-    bt.nprob         = n;
+    std::vector<YArgs<T>> args(n);
+    std::vector<int>      start(n + 1);
+    size_t                lds    = 0;
+    int                   blocks = 0;
     for (int q = 0; q < n; q++) {
         const TransformJob<T>& j = *jobs[q];
-        bt.p[q]                  = yargs(j, inverse);
-        bt.start[q]              = blocks;
+        args[q]                  = yargs(j, inverse);
+        start[q]                 = blocks;
         const long long nb       = (long long)((j.nx + 1) / 2) * j.nrhs;
         if (blocks + nb > (1LL << 30)) throw Error(FP_E_INVALID, "too many transform workgroups in one launch");
         blocks += (int)nb;
         lds = std::max(lds, lds_bytes(*j.plan, odd));
     }
-    bt.start[n] = blocks;
+    start[n] = blocks;
     if (lds > 160 * 1024) throw Error(FP_E_INVALID, "transform length exceeds the LDS of one workgroup");
     if (blocks == 0) return;
     if (engine == 1 && odd == ODD_MFMA && sizeof(T) != 8) throw Error(FP_E_INVALID, "matrix-core passes are FP64");
-    const void* fn = engine == 0         ? (const void*)k_ytransform_gen<T>
-                     : odd == ODD_NONE   ? (const void*)k_ytransform_sb<T, ODD_NONE>
-                     : odd == ODD_MFMA   ? (const void*)k_ytransform_sb<T, ODD_MFMA>
-                                         : (const void*)k_ytransform_sb<T, ODD_VALU>;
-    const int   ki = engine == 0 ? 3 : odd;
-    static bool attr_set[4] = {false, false, false, false};
-    if (lds > 64 * 1024 && !attr_set[ki]) {
-        FP_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-        attr_set[ki] = true;
+    if (n <= kMaxBatch) {
+        YBatch<T> bt{};
+        bt.nprob = n;
+        for (int q = 0; q < n; q++) {
+            bt.p[q]     = args[q];
+            bt.start[q] = start[q];
+        }
+        bt.start[n] = blocks;
+        launch_kernel<T>(engine, odd, blocks, NT, lds, s, bt);
+        return;
     }
-    if (engine == 0) hipLaunchKernelGGL(k_ytransform_gen<T>, dim3(blocks), dim3(NT), lds, s, bt);
-    else if (odd == ODD_NONE) hipLaunchKernelGGL((k_ytransform_sb<T, ODD_NONE>), dim3(blocks), dim3(NT), lds, s, bt);
-    else if (odd == ODD_MFMA) hipLaunchKernelGGL((k_ytransform_sb<T, ODD_MFMA>), dim3(blocks), dim3(NT), lds, s, bt);
-    else hipLaunchKernelGGL((k_ytransform_sb<T, ODD_VALU>), dim3(blocks), dim3(NT), lds, s, bt);
-    FP_HIP(hipGetLastError());
+    void*                  mem = nullptr;
+    const JobTable<YArgs<T>> t = table_upload(args, start, s, mem);
+    try {
+        launch_kernel<T>(engine, odd, blocks, NT, lds, s, t);
+    } catch (...) {
+        table_release(mem, s);
+        throw;
+    }
+    table_release(mem, s);
 }
 
-// Jobs are grouped by (engine, threads per workgroup, odd-prime form) and
-// launched kMaxBatch at a time.
+// Jobs are grouped by (engine, threads per workgroup, odd-prime form), one
+// launch per group.
 template <typename T>
 void launch_ytransform_batch(const TransformJob<T>* jobs, int njobs, int inverse, hipStream_t s)
 {
@@ -1350,12 +1385,8 @@ void launch_ytransform_batch(const TransformJob<T>* jobs, int njobs, int inverse
         const TransformPlan<T>& tp = *jobs[k].plan;
         groups[{tp.engine, jobs[k].nt ? jobs[k].nt : tp.nt, job_form(jobs[k])}].push_back(&jobs[k]);
     }
-    for (auto& kv : groups) {
-        auto& g = kv.second;
-        for (size_t q0 = 0; q0 < g.size(); q0 += kMaxBatch)
-            launch_group<T>(g.data() + q0, (int)std::min<size_t>(kMaxBatch, g.size() - q0), kv.first[0],
-                            kv.first[1], kv.first[2], inverse, s);
-    }
+    for (auto& kv : groups)
+        launch_group<T>(kv.second.data(), (int)kv.second.size(), kv.first[0], kv.first[1], kv.first[2], inverse, s);
 }
 
 template <typename T>

@@ -158,15 +158,27 @@ constexpr int tri_threads()
     return CH >= 32 ? 512 : 1024;
 }
 
-template <typename T, int CH>
-__global__ __launch_bounds__(tri_threads<CH>()) void k_tridiag(TriBatch<T> bt)
+template <typename T>
+__device__ __forceinline__ const TriProb<T>& tri_find(const TriBatch<T>& bt, int blk, int& local)
 {
-    const int blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
-    int       q   = 0;
+    int q = 0;
     while (q + 1 < bt.nprob && blk >= bt.start[q + 1]) q++;
-    const TriProb<T>& pr = bt.p[q];
+    local = blk - bt.start[q];
+    return bt.p[q];
+}
+template <typename T>
+__device__ __forceinline__ const TriProb<T>& tri_find(const JobTable<TriProb<T>>& bt, int blk, int& local)
+{
+    return table_find(bt, blk, local);
+}
+
+template <typename T, int CH, typename BT>
+__global__ __launch_bounds__(tri_threads<CH>()) void k_tridiag(BT bt)
+{
+    const int         blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
+    int               local;
+    const TriProb<T>& pr = tri_find<T>(bt, blk, local);
     const int nx = pr.nx, ny = pr.ny, P = pr.P, MB = pr.MB, singular_last = pr.singular_last;
-    const int local = blk - bt.start[q];
     const int rhs = local / pr.nblk, mblk = local - rhs * pr.nblk;
 
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -623,17 +635,17 @@ static int tri_mb(int P, int ch, int mb_cap)
 template <typename T, int CH>
 static void launch_tridiag_group(const TridiagJob<T>* const* jobs, int n, hipStream_t s)
 {
-    TriBatch<T> bt{};
-    int         blocks = 0, threads = 64;
-    size_t      lds = 0;
-    bt.nprob        = n;
+    std::vector<TriProb<T>> probs(n);
+    std::vector<int>        start(n + 1);
+    int                     blocks = 0, threads = 64;
+    size_t                  lds = 0;
     for (int q = 0; q < n; q++) {
         const TridiagJob<T>& jb = *jobs[q];
         const SubPlan&       sp = *jb.sp;
         const int            P  = sp.tri_p;
         const int            MB = tri_mb(P, CH, sp.mb_field);
         if (MB * P > tri_max_threads(CH)) throw Error(FP_E_INVALID, "nx too large for the tridiagonal kernel");
-        TriProb<T>& pr   = bt.p[q];
+        TriProb<T>& pr   = probs[q];
         pr.data          = jb.data;
         pr.ld            = jb.ld;
         pr.xd            = sp.xd.p;
@@ -649,20 +661,33 @@ static void launch_tridiag_group(const TridiagJob<T>* const* jobs, int n, hipStr
         pr.cyc           = sp.cyc ? 1 : 0;
         pr.z1            = sp.cyc ? sp.zc.p : nullptr;
         pr.z2            = sp.cyc ? sp.zc.p + (size_t)sp.nx * sp.ny : nullptr;
-        bt.start[q]      = blocks;
+        start[q]         = blocks;
         const long long nb = (long long)pr.nblk * jb.nrhs;
         if (blocks + nb > (1LL << 30)) throw Error(FP_E_INVALID, "too many tridiagonal workgroups in one launch");
         blocks += (int)nb;
         threads = std::max(threads, (MB * P + 63) / 64 * 64);
         lds     = std::max(lds, 3 * (size_t)P * MB * sizeof(T));
     }
-    bt.start[n] = blocks;
+    start[n] = blocks;
     if (blocks == 0) return;
-    hipLaunchKernelGGL((k_tridiag<T, CH>), dim3(blocks), dim3(threads), lds, s, bt);
-    FP_HIP(hipGetLastError());
+    if (n <= kMaxBatch) {
+        TriBatch<T> bt{};
+        bt.nprob = n;
+        for (int q = 0; q <= n; q++) bt.start[q] = start[q];
+        for (int q = 0; q < n; q++) bt.p[q] = probs[q];
+        hipLaunchKernelGGL((k_tridiag<T, CH, TriBatch<T>>), dim3(blocks), dim3(threads), lds, s, bt);
+        FP_HIP(hipGetLastError());
+        return;
+    }
+    void*                          mem = nullptr;
+    const JobTable<TriProb<T>>     t   = table_upload(probs, start, s, mem);
+    hipLaunchKernelGGL((k_tridiag<T, CH, JobTable<TriProb<T>>>), dim3(blocks), dim3(threads), lds, s, t);
+    const hipError_t e = hipGetLastError();
+    table_release(mem, s);
+    FP_HIP(e);
 }
 
-// jobs are grouped by their plan's chunk length and launched kMaxBatch at a time
+// jobs are grouped by their plan's chunk length, one launch per group
 template <typename T>
 void launch_tridiag_batch(const TridiagJob<T>* jobs, int njobs, hipStream_t s)
 {
@@ -672,12 +697,9 @@ void launch_tridiag_batch(const TridiagJob<T>* jobs, int njobs, hipStream_t s)
         const int ch = jobs[k].sp->tri_ch;
         (ch == 8 ? g8 : ch == 16 ? g16 : g32).push_back(&jobs[k]);
     }
-    for (size_t q0 = 0; q0 < g8.size(); q0 += kMaxBatch)
-        launch_tridiag_group<T, 8>(g8.data() + q0, (int)std::min<size_t>(kMaxBatch, g8.size() - q0), s);
-    for (size_t q0 = 0; q0 < g16.size(); q0 += kMaxBatch)
-        launch_tridiag_group<T, 16>(g16.data() + q0, (int)std::min<size_t>(kMaxBatch, g16.size() - q0), s);
-    for (size_t q0 = 0; q0 < g32.size(); q0 += kMaxBatch)
-        launch_tridiag_group<T, 32>(g32.data() + q0, (int)std::min<size_t>(kMaxBatch, g32.size() - q0), s);
+    if (!g8.empty()) launch_tridiag_group<T, 8>(g8.data(), (int)g8.size(), s);
+    if (!g16.empty()) launch_tridiag_group<T, 16>(g16.data(), (int)g16.size(), s);
+    if (!g32.empty()) launch_tridiag_group<T, 32>(g32.data(), (int)g32.size(), s);
 }
 
 template <typename T>

@@ -312,3 +312,28 @@ def test_periodic_fp32_refine(ctx, bc):
     info = s.solve_refine(t, tol=1e-12, max_steps=12)
     torch.cuda.synchronize()
     assert rel(t.cpu().numpy(), O.sub_solve(f, h, h, bc)) < 1e-10, info
+
+
+def test_batched_sixty_with_periodic(ctx):
+    """Sixty problems in one call (device-resident descriptor tables: one
+    launch per kernel group), every boundary family including periodic axes."""
+    import torch
+    import focalors_poisson_amd as fpa
+    rng = np.random.default_rng(61)
+    fams = [(D, D, D, D), (N, D, N, N), (D, N, P, P), (P, P, D, D), (P, P, N, D), (A, N, D, N), (N, A, P, P)]
+    solvers, fields, hosts, specs = [], [], [], []
+    for k in range(60):
+        bc = fams[k % len(fams)]
+        nx = int(rng.integers(8, 300))
+        ny = int(rng.integers(4, 300)) // 2 * 2
+        h = 1.0 / max(nx, ny)
+        f = rng.standard_normal((nx, ny)) * h * h
+        solvers.append(fpa.PoissonSolver2D(ctx, nx, ny, h, h, bc))
+        fields.append(to_dev(f))
+        hosts.append(f)
+        specs.append((h, bc))
+    banner("batched sixty with periodic")
+    fpa.solve_batched(ctx, solvers, fields)
+    torch.cuda.synchronize()
+    for f, t, (h, bc) in zip(hosts, fields, specs):
+        assert rel(t.cpu().numpy(), O.sub_solve(f, h, h, bc)) < 1e-11, (f.shape, bc)
