@@ -146,6 +146,7 @@ struct TransformPlan {
     T          c0 = 0, c1 = 0;
     int        nt = 256;  // threads per workgroup (tuned)
     int        odd = 0;   // odd-prime pass form: 0 none, 1 matrix cores (FP64), 2 VALU pair sums (tuned)
+    int        ept = 16;  // values per thread of the register butterflies (24 for long Bluestein lengths)
     void build(int kind, int N);
 };
 // Times the plan's transform on an nx-row field and keeps the fastest

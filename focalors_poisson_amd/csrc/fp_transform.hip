@@ -41,6 +41,7 @@
 #include <complex>
 #include <map>
 #include <mutex>
+#include <set>
 
 namespace fp {
 
@@ -204,10 +205,12 @@ __device__ __forceinline__ int zp(int i)
 // butterflies of radix R one thread holds in registers in one pass
 // (16 complex values at most)
 constexpr int kEPT = 16;
-template <int R>
+// long FFTs (Bluestein lengths above 16 x 512) hold 24 values per thread
+constexpr int kEPTLong = 24;
+template <int R, int EPT = kEPT>
 constexpr int sb_bpt()
 {
-    return (kEPT + R - 1) / R;
+    return (EPT + R - 1) / R;
 }
 // odd-prime butterflies: output pairs per thread group, groups per thread
 constexpr int kNQ  = 5;
@@ -218,10 +221,10 @@ constexpr int kMaxOddPrime = 47;
 // synchronise, twiddle + butterfly, write, synchronise.  Twiddles of a
 // butterfly are w^r, r = 1..R-1, with w = W^(k * stride) from the table and
 // its powers by repeated products (depth <= 3 multiplications).
-template <int R, bool FWD, typename T>
+template <int R, bool FWD, int EPT, typename T>
 __device__ __forceinline__ void pass_reg_sb(C2<T>* __restrict__ Z, const Tw<T> W, int L, int Ns)
 {
-    constexpr int BPT = sb_bpt<R>();
+    constexpr int BPT = sb_bpt<R, EPT>();
     const int     NT  = blockDim.x;
     const int     nb = L / R, stride = L / (Ns * R);
     C2<T>         v[BPT][R];
@@ -456,18 +459,18 @@ __device__ __forceinline__ void pass_odd_mfma(C2<double>* __restrict__ Z, const 
     __syncthreads();
 }
 
-template <bool FWD, int OM, typename T>
+template <bool FWD, int OM, int EPT, typename T>
 __device__ __forceinline__ void run_fft_sb(const YArgs<T>& a, const Tw<T> W, C2<T>* Z, const C2<T>* cs,
                                            const double* fragL)
 {
     int Ns = 1, co = 0, fo = 0;
     for (int s = 0; s < a.nrad; s++) {
         const int R = a.rad[s];
-        if (R == 2) pass_reg_sb<2, FWD>(Z, W, a.F, Ns);
-        else if (R == 3) pass_reg_sb<3, FWD>(Z, W, a.F, Ns);
-        else if (R == 4) pass_reg_sb<4, FWD>(Z, W, a.F, Ns);
-        else if (R == 5) pass_reg_sb<5, FWD>(Z, W, a.F, Ns);
-        else if (R == 8) pass_reg_sb<8, FWD>(Z, W, a.F, Ns);
+        if (R == 2) pass_reg_sb<2, FWD, EPT>(Z, W, a.F, Ns);
+        else if (R == 3) pass_reg_sb<3, FWD, EPT>(Z, W, a.F, Ns);
+        else if (R == 4) pass_reg_sb<4, FWD, EPT>(Z, W, a.F, Ns);
+        else if (R == 5) pass_reg_sb<5, FWD, EPT>(Z, W, a.F, Ns);
+        else if (R == 8) pass_reg_sb<8, FWD, EPT>(Z, W, a.F, Ns);
         else if constexpr (OM == ODD_MFMA && sizeof(T) == 8) {
             pass_odd_mfma<FWD>(Z, W, a.F, Ns, R, fragL + fo);
             fo += 2 * 2 * mfma_ksteps(R) * 64;
@@ -485,7 +488,7 @@ __device__ __forceinline__ void run_fft_sb(const YArgs<T>& a, const Tw<T> W, C2<
 // a circular convolution of length F with the conjugate chirp, whose DFT bh
 // (scaled by 1/F) is precomputed.  The + sign conjugates input and output.
 // Every thread reads and writes only its own elements between barriers.
-template <bool FWD, int OM, typename T>
+template <bool FWD, int OM, int EPT, typename T>
 __device__ __forceinline__ void run_bluestein(const YArgs<T>& a, const Tw<T> W, C2<T>* Z, const C2<T>* cs,
                                               const double* fragL)
 {
@@ -500,10 +503,10 @@ __device__ __forceinline__ void run_bluestein(const YArgs<T>& a, const Tw<T> W, 
         Z[zp<kZpad>(n)] = v;
     }
     __syncthreads();
-    run_fft_sb<true, OM>(a, W, Z, cs, fragL);
+    run_fft_sb<true, OM, EPT>(a, W, Z, cs, fragL);
     for (int k = threadIdx.x; k < F; k += NT) Z[zp<kZpad>(k)] = cmul(Z[zp<kZpad>(k)], a.bh[k]);
     __syncthreads();
-    run_fft_sb<false, OM>(a, W, Z, cs, fragL);
+    run_fft_sb<false, OM, EPT>(a, W, Z, cs, fragL);
     for (int k = threadIdx.x; k < L; k += NT) {
         C2<T> v = cmul(Z[zp<kZpad>(k)], a.chirp[k]);
         if (!FWD) v.y = -v.y;
@@ -902,7 +905,7 @@ __device__ __forceinline__ const YArgs<T>& find_problem(const JobTable<YArgs<T>>
 // threads per workgroup: chosen per plan (128, 256 or 512)
 constexpr int kMaxNT = 512;
 
-template <typename T, int OM, typename BT>
+template <typename T, int OM, typename BT, int EPT>
 __global__ __launch_bounds__(kMaxNT) void k_ytransform_sb(BT bt)
 {
     int             local;
@@ -929,10 +932,10 @@ __global__ __launch_bounds__(kMaxNT) void k_ytransform_sb(BT bt)
     __syncthreads();
     const bool fwd = !(a.kind == T_NN && a.inverse);
     if (a.F != a.L) {
-        if (fwd) run_bluestein<true, OM>(a, W, Z, cs, fr);
-        else run_bluestein<false, OM>(a, W, Z, cs, fr);
-    } else if (fwd) run_fft_sb<true, OM>(a, W, Z, cs, fr);
-    else run_fft_sb<false, OM>(a, W, Z, cs, fr);
+        if (fwd) run_bluestein<true, OM, EPT>(a, W, Z, cs, fr);
+        else run_bluestein<false, OM, EPT>(a, W, Z, cs, fr);
+    } else if (fwd) run_fft_sb<true, OM, EPT>(a, W, Z, cs, fr);
+    else run_fft_sb<false, OM, EPT>(a, W, Z, cs, fr);
     store_rows<kZpad>(a, rp, Z, y0, S, S + a.N / 2 + 1);
 }
 
@@ -1035,24 +1038,17 @@ static void factor_radices(int L, int* rad, int& nrad)
 
 // Whether the single-buffer engine runs this schedule with NT threads: every
 // pass's butterflies (or odd-prime work items) fit the per-thread registers.
-static int sb_bpt_rt(int R)
+static int sb_bpt_rt(int R, int ept)
 {
-    switch (R) {
-        case 2: return sb_bpt<2>();
-        case 3: return sb_bpt<3>();
-        case 4: return sb_bpt<4>();
-        case 5: return sb_bpt<5>();
-        case 8: return sb_bpt<8>();
-        default: return 0;
-    }
+    return (R == 2 || R == 3 || R == 4 || R == 5 || R == 8) ? (ept + R - 1) / R : 0;
 }
-static bool sb_fits(const int* rad, int nrad, int L, int NT, int odd)
+static bool sb_fits(const int* rad, int nrad, int L, int NT, int odd, int ept = kEPT)
 {
     const bool mfma = odd == ODD_MFMA;
     for (int s = 0; s < nrad; s++) {
         const int R = rad[s], nb = L / R;
         if (R <= 8 && R != 7) {
-            if ((nb + NT - 1) / NT > sb_bpt_rt(R)) return false;
+            if ((nb + NT - 1) / NT > sb_bpt_rt(R, ept)) return false;
         } else if (R > kMaxOddPrime) {
             return false;
         } else if (mfma) {
@@ -1102,6 +1098,7 @@ void TransformPlan<T>::build(int k, int n)
     // prime factors above 47: Bluestein over the smallest 2,3,5-smooth F >=
     // 2L - 1 when its single buffer fits the LDS, else the fallback engine
     F        = L;
+    ept      = kEPT;
     bool big = false;
     try {
         factor_radices(L, rad, nrad);
@@ -1114,10 +1111,13 @@ void TransformPlan<T>::build(int k, int n)
         const size_t need = ((size_t)Fb + ((size_t)Fb >> kZpad) + 1 + 64 + Fb / 64 + 1) * sizeof(C2<T>) + 4 * sizeof(T);
         int          rb[kMaxRadices], nb = 0;
         factor_radices(Fb, rb, nb);
-        bool fits = false;
-        for (int t : {128, 256, 512}) fits |= sb_fits(rb, nb, Fb, t, ODD_NONE);
+        int fits = 0;  // values per thread that fit
+        for (int e : {kEPT, kEPTLong})
+            for (int t : {128, 256, 512})
+                if (!fits && sb_fits(rb, nb, Fb, t, ODD_NONE, e)) fits = e;
         if (fits && need <= 160 * 1024) {
             F    = Fb;
+            ept  = fits;
             nrad = nb;
             for (int s = 0; s < nb; s++) rad[s] = rb[s];
         } else {
@@ -1239,19 +1239,21 @@ void TransformPlan<T>::build(int k, int n)
     // the two-buffer fallback
     bool has_odd = false;
     for (int s = 0; s < nrad; s++) has_odd |= rad[s] == 7 || rad[s] > 8;
-    engine = 0;
-    nt     = 256;
-    odd    = ODD_NONE;
+    nt          = 256;
+    odd         = ODD_NONE;
+    bool chosen = false;
+    engine      = 1;  // lds_bytes below sizes the single buffer
     for (int form : odd_forms<T>(has_odd)) {
         for (int t : {256, 512, 128})
-            if (sb_fits(rad, nrad, F, t, form) && lds_bytes(*this, form) <= 160 * 1024) {
-                engine = 1;
+            if (sb_fits(rad, nrad, F, t, form, ept) && lds_bytes(*this, form) <= 160 * 1024) {
+                chosen = true;
                 nt     = t;
                 odd    = form;
                 break;
             }
-        if (engine == 1) break;
+        if (chosen) break;
     }
+    if (!chosen) engine = 0;
 }
 
 template struct TransformPlan<double>;
@@ -1307,24 +1309,32 @@ static YArgs<T> yargs(const TransformJob<T>& j, int inverse)
 // The kernel of an engine and odd-prime form for the batch type B (YBatch
 // by value, or a JobTable in device memory).
 template <typename T, typename B>
-static void launch_kernel(int engine, int odd, int blocks, int NT, size_t lds, hipStream_t s, const B& bt)
+static void launch_kernel(int engine, int odd, int ept, int blocks, int NT, size_t lds, hipStream_t s, const B& bt)
 {
+    if (engine == 1 && ept != kEPT && odd != ODD_NONE) throw Error(FP_E_INVALID, "long transforms are 2,3,5-smooth");
     const void* fn = engine == 0         ? (const void*)k_ytransform_gen<T, B>
-                     : odd == ODD_NONE   ? (const void*)k_ytransform_sb<T, ODD_NONE, B>
-                     : odd == ODD_MFMA   ? (const void*)k_ytransform_sb<T, ODD_MFMA, B>
-                                         : (const void*)k_ytransform_sb<T, ODD_VALU, B>;
-    static bool attr_set[4] = {false, false, false, false};
-    const int   ki          = engine == 0 ? 3 : odd;
-    if (lds > 64 * 1024 && !attr_set[ki]) {
-        FP_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-        attr_set[ki] = true;
+                     : ept != kEPT       ? (const void*)k_ytransform_sb<T, ODD_NONE, B, kEPTLong>
+                     : odd == ODD_NONE   ? (const void*)k_ytransform_sb<T, ODD_NONE, B, kEPT>
+                     : odd == ODD_MFMA   ? (const void*)k_ytransform_sb<T, ODD_MFMA, B, kEPT>
+                                         : (const void*)k_ytransform_sb<T, ODD_VALU, B, kEPT>;
+    // one attribute flag per kernel instantiation
+    static std::mutex            mu;
+    static std::set<const void*> attr_set;
+    if (lds > 64 * 1024) {
+        std::lock_guard<std::mutex> lk(mu);
+        if (!attr_set.count(fn)) {
+            FP_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+            attr_set.insert(fn);
+        }
     }
     if (engine == 0) hipLaunchKernelGGL((k_ytransform_gen<T, B>), dim3(blocks), dim3(NT), lds, s, bt);
+    else if (ept != kEPT)
+        hipLaunchKernelGGL((k_ytransform_sb<T, ODD_NONE, B, kEPTLong>), dim3(blocks), dim3(NT), lds, s, bt);
     else if (odd == ODD_NONE)
-        hipLaunchKernelGGL((k_ytransform_sb<T, ODD_NONE, B>), dim3(blocks), dim3(NT), lds, s, bt);
+        hipLaunchKernelGGL((k_ytransform_sb<T, ODD_NONE, B, kEPT>), dim3(blocks), dim3(NT), lds, s, bt);
     else if (odd == ODD_MFMA)
-        hipLaunchKernelGGL((k_ytransform_sb<T, ODD_MFMA, B>), dim3(blocks), dim3(NT), lds, s, bt);
-    else hipLaunchKernelGGL((k_ytransform_sb<T, ODD_VALU, B>), dim3(blocks), dim3(NT), lds, s, bt);
+        hipLaunchKernelGGL((k_ytransform_sb<T, ODD_MFMA, B, kEPT>), dim3(blocks), dim3(NT), lds, s, bt);
+    else hipLaunchKernelGGL((k_ytransform_sb<T, ODD_VALU, B, kEPT>), dim3(blocks), dim3(NT), lds, s, bt);
     FP_HIP(hipGetLastError());
 }
 
@@ -1332,8 +1342,8 @@ static void launch_kernel(int engine, int odd, int blocks, int NT, size_t lds, h
 // their descriptors as kernel arguments up to kMaxBatch jobs, else in a
 // device-memory table.
 template <typename T>
-static void launch_group(const TransformJob<T>* const* jobs, int n, int engine, int NT, int odd, int inverse,
-                         hipStream_t s)
+static void launch_group(const TransformJob<T>* const* jobs, int n, int engine, int NT, int odd, int ept,
+                         int inverse, hipStream_t s)
 {
     std::vector<YArgs<T>> args(n);
     std::vector<int>      start(n + 1);
@@ -1360,13 +1370,13 @@ static void launch_group(const TransformJob<T>* const* jobs, int n, int engine, 
             bt.start[q] = start[q];
         }
         bt.start[n] = blocks;
-        launch_kernel<T>(engine, odd, blocks, NT, lds, s, bt);
+        launch_kernel<T>(engine, odd, ept, blocks, NT, lds, s, bt);
         return;
     }
     void*                  mem = nullptr;
     const JobTable<YArgs<T>> t = table_upload(args, start, s, mem);
     try {
-        launch_kernel<T>(engine, odd, blocks, NT, lds, s, t);
+        launch_kernel<T>(engine, odd, ept, blocks, NT, lds, s, t);
     } catch (...) {
         table_release(mem, s);
         throw;
@@ -1379,14 +1389,15 @@ static void launch_group(const TransformJob<T>* const* jobs, int n, int engine, 
 template <typename T>
 void launch_ytransform_batch(const TransformJob<T>* jobs, int njobs, int inverse, hipStream_t s)
 {
-    std::map<std::array<int, 3>, std::vector<const TransformJob<T>*>> groups;
+    std::map<std::array<int, 4>, std::vector<const TransformJob<T>*>> groups;
     for (int k = 0; k < njobs; k++) {
         if (jobs[k].nrhs <= 0 || jobs[k].nx <= 0) continue;
         const TransformPlan<T>& tp = *jobs[k].plan;
-        groups[{tp.engine, jobs[k].nt ? jobs[k].nt : tp.nt, job_form(jobs[k])}].push_back(&jobs[k]);
+        groups[{tp.engine, jobs[k].nt ? jobs[k].nt : tp.nt, job_form(jobs[k]), tp.ept}].push_back(&jobs[k]);
     }
     for (auto& kv : groups)
-        launch_group<T>(kv.second.data(), (int)kv.second.size(), kv.first[0], kv.first[1], kv.first[2], inverse, s);
+        launch_group<T>(kv.second.data(), (int)kv.second.size(), kv.first[0], kv.first[1], kv.first[2], kv.first[3],
+                        inverse, s);
 }
 
 template <typename T>
@@ -1419,7 +1430,7 @@ void tune_transform(TransformPlan<T>& tp, int nx, bool measure)
     std::vector<std::array<int, 2>> cand;  // (threads, odd-prime form)
     for (int form : odd_forms<T>(has_odd))
         for (int t : {128, 256, 512})
-            if (sb_fits(tp.rad, tp.nrad, tp.F, t, form) && lds_bytes(tp, form) <= 160 * 1024) cand.push_back({t, form});
+            if (sb_fits(tp.rad, tp.nrad, tp.F, t, form, tp.ept) && lds_bytes(tp, form) <= 160 * 1024) cand.push_back({t, form});
     if (!measure || cand.size() < 2 || (long long)nx * tp.N < (1LL << 16)) return;
     const std::string key = "yt " + std::to_string(tp.kind) + " " + std::to_string(tp.N) + " " +
                             std::to_string(nx) + " " + std::to_string(sizeof(T));
@@ -1501,8 +1512,8 @@ std::array<int, 2> tune_batch_shape(std::vector<TransformJob<T>> jobs, hipStream
                 if (tp.engine != 1) continue;
                 const int f  = form < 0 ? tp.odd : form;
                 const int t  = nt ? nt : tp.nt;
-                bool      ok = sb_fits(tp.rad, tp.nrad, tp.F, t, f) && lds_bytes(tp, f) <= 160 * 1024;
-                ok           = ok && (f != ODD_NONE || tp.odd == ODD_NONE);
+                bool      ok = sb_fits(tp.rad, tp.nrad, tp.F, t, f, tp.ept) && lds_bytes(tp, f) <= 160 * 1024;
+                ok           = ok && (f != ODD_NONE || tp.odd == ODD_NONE) && (tp.ept == kEPT || f == ODD_NONE);
                 if (ok) {
                     j.nt  = nt;
                     j.odd = form;

@@ -174,8 +174,8 @@ def test_errors_are_status_codes(ctx):
     assert e.value.status == 1
 
 
-@pytest.mark.parametrize("bcp", [(D, D), (N, N), (D, N), (N, D)])
-@pytest.mark.parametrize("ny", [5, 64, 100, 256, 1024, 2048])
+@pytest.mark.parametrize("bcp,ny", [(b, n) for b in [(D, D), (N, N), (D, N), (N, D)] for n in [5, 64, 100, 256, 1024, 2048]]
+                         + [((D, D), 4096), ((N, N), 4096)])
 def test_transforms_match_reference_coefficients(ctx, bcp, ny):
     """PoissonFFT2D_*::transform / transform_transpose coefficient for
     coefficient (same signs and mode order as the reference)."""
@@ -337,3 +337,17 @@ def test_batched_sixty_with_periodic(ctx):
     torch.cuda.synchronize()
     for f, t, (h, bc) in zip(hosts, fields, specs):
         assert rel(t.cpu().numpy(), O.sub_solve(f, h, h, bc)) < 1e-11, (f.shape, bc)
+
+
+@pytest.mark.parametrize("f32", [False, True])
+def test_dirichlet_4096_long_bluestein(ctx, f32):
+    """4096 DD rows: L = 4097 = 17 x 241, Bluestein over F = 8640 with 24
+    values per thread; the whole 4096^2 solve against the oracle."""
+    import torch
+    rng = np.random.default_rng(4096)
+    n = 4096
+    h = 1.0 / n
+    f = rng.standard_normal((n, n)) * h * h
+    banner("dirichlet 4096 f32=%d" % f32)
+    got = gpu_solve(ctx, f, h, h, (D, D, D, D), torch.float32 if f32 else None)
+    assert rel(got, O.sub_solve(f, h, h, (D, D, D, D))) < (1e-4 if f32 else 1e-11)
