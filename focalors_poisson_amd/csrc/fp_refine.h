@@ -55,10 +55,29 @@ struct CatResidualArgs {
     long long     ldn[4];
 };
 void launch_cat_residual(const CatResidualArgs& a, int nrhs, hipStream_t s);
+// the residuals of up to 8 domains in one launch
+struct CatResidualBatch {
+    int             n = 0;
+    CatResidualArgs a[8];
+};
+void launch_cat_residual_batch(const CatResidualBatch& b, int nrhs, hipStream_t s);
 // u += d (d may be null: u unchanged) over nx*ny values per right-hand side;
 // writes per-block partial sums (|d|^2, |u|^2) to part[(r * nblk + blk) * 2]
 constexpr int kNormBlocks = 256;
 void launch_update_norms(double* u, long long ldu, const double* d, long long ldd, long long n, int nrhs,
                          double* part, hipStream_t s);
+// launch_update_norms of up to 8 domains in one launch
+struct NormsJob {
+    double*       u;
+    long long     ldu;
+    const double* d;
+    long long     ldd, n;
+    double*       part;
+};
+struct NormsBatch {
+    int      n = 0;
+    NormsJob j[8];
+};
+void launch_update_norms_batch(const NormsBatch& b, int nrhs, hipStream_t s);
 
 }  // namespace fp

@@ -175,8 +175,9 @@ RefineStats refine_solve(SubPlan& sp, double* f, long long ld, int nrhs, double 
 
 // ---- composite refinement ------------------------------------------------------
 
-__global__ __launch_bounds__(RB) void k_cat_residual(CatResidualArgs a)
+__global__ __launch_bounds__(RB) void k_cat_residual(CatResidualBatch bt)
 {
+    const CatResidualArgs& a = bt.a[blockIdx.z];
     const int       nx = a.nx, ny = a.ny;
     const long long n  = (long long)nx * ny;
     const long long r  = blockIdx.y;
@@ -207,17 +208,30 @@ __global__ __launch_bounds__(RB) void k_cat_residual(CatResidualArgs a)
     }
 }
 
-void launch_cat_residual(const CatResidualArgs& a, int nrhs, hipStream_t s)
+void launch_cat_residual_batch(const CatResidualBatch& b, int nrhs, hipStream_t s)
 {
-    const long long n = (long long)a.nx * a.ny;
-    hipLaunchKernelGGL(k_cat_residual, dim3(refine_blocks(n), nrhs), dim3(RB), 0, s, a);
+    if (b.n <= 0) return;
+    long long nmax = 0;
+    for (int k = 0; k < b.n; k++) nmax = std::max(nmax, (long long)b.a[k].nx * b.a[k].ny);
+    hipLaunchKernelGGL(k_cat_residual, dim3(refine_blocks(nmax), nrhs, b.n), dim3(RB), 0, s, b);
     FP_HIP(hipGetLastError());
 }
+void launch_cat_residual(const CatResidualArgs& a, int nrhs, hipStream_t s)
+{
+    CatResidualBatch b{};
+    b.n    = 1;
+    b.a[0] = a;
+    launch_cat_residual_batch(b, nrhs, s);
+}
 
-__global__ __launch_bounds__(RB) void k_update_norms(double* u, long long ldu, const double* d, long long ldd,
-                                                     long long n, double* part)
+__global__ __launch_bounds__(RB) void k_update_norms(NormsBatch bt)
 {
     __shared__ double sd[RB / 64], su[RB / 64];
+    const NormsJob&  J    = bt.j[blockIdx.z];
+    double*          u    = J.u;
+    const double*    d    = J.d;
+    const long long  ldu  = J.ldu, ldd = J.ldd, n = J.n;
+    double*          part = J.part;
     const long long r  = blockIdx.y;
     double*         ur = u + r * ldu;
     const double*   dr = d ? d + r * ldd : nullptr;
@@ -250,11 +264,19 @@ __global__ __launch_bounds__(RB) void k_update_norms(double* u, long long ldu, c
     }
 }
 
+void launch_update_norms_batch(const NormsBatch& b, int nrhs, hipStream_t s)
+{
+    if (b.n <= 0) return;
+    hipLaunchKernelGGL(k_update_norms, dim3(kNormBlocks, nrhs, b.n), dim3(RB), 0, s, b);
+    FP_HIP(hipGetLastError());
+}
 void launch_update_norms(double* u, long long ldu, const double* d, long long ldd, long long n, int nrhs,
                          double* part, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_update_norms, dim3(kNormBlocks, nrhs), dim3(RB), 0, s, u, ldu, d, ldd, n, part);
-    FP_HIP(hipGetLastError());
+    NormsBatch b{};
+    b.n    = 1;
+    b.j[0] = {u, ldu, d, ldd, n, part};
+    launch_update_norms_batch(b, nrhs, s);
 }
 
 }  // namespace fp

@@ -306,6 +306,10 @@ struct fp_cat_s {
     // captured solve (fp_cat_graph_capture): replays the two-level sweep with
     // the GMRES start from the explicit inverse and no further Krylov steps
     bool                                   graph_mode = false;
+    // inner sweeps of a refinement: the interface solve is the explicit
+    // inverse's product alone (no residual check, no host wait); the outer
+    // FP64 residual decides convergence
+    bool                                   no_wait    = false;
     hipGraphExec_t                         gexec      = nullptr;
     int                                    gnrhs      = 0;
     // statistics of the last solve
@@ -630,6 +634,7 @@ static bool gmres_iterate(fp_cat_s* c, Node& nd, int nrhs, hipStream_t s)
 {
     bool           fused;
     const GmresDev g = gmres_begin(c, nd, nrhs, s, fused);
+    if (c->no_wait && nd.precond) return false;
     if (c->graph_mode) {
         // captured: the residual check at the start is published to host
         // memory and read after the replay; no host synchronisation here
@@ -998,6 +1003,7 @@ static void ensure_refine_ws(fp_cat_s* c, int nrhs)
 // r_d = b_d - A_d u_d - (neighbour lines) for every domain
 static void cat_residual(fp_cat_s* c, double* const* u, int nrhs, hipStream_t s)
 {
+    CatResidualBatch bt;
     for (int d = 0; d < c->nd; d++) {
         CatResidualArgs a{};
         const long long n = dom_n(c, d);
@@ -1018,7 +1024,11 @@ static void cat_residual(fp_cat_s* c, double* const* u, int nrhs, hipStream_t s)
             a.nby[face]  = nb >= 0 ? c->ny[nb] : 0;
             a.ldn[face]  = nb >= 0 ? dom_n(c, nb) : 0;
         }
-        launch_cat_residual(a, nrhs, s);
+        bt.a[bt.n++] = a;
+        if (bt.n == 8 || d == c->nd - 1) {
+            launch_cat_residual_batch(bt, nrhs, s);
+            bt.n = 0;
+        }
     }
 }
 
@@ -1051,9 +1061,15 @@ static void cat_refine(fp_cat_s* c, double* const* f, int nrhs, hipStream_t s)
         FP_HIP(hipMemcpyAsync(c->rb[d].p, f[d], sizeof(double) * dom_n(c, d) * nrhs, hipMemcpyDeviceToDevice, s));
     const size_t half = (size_t)2 * nd * nrhs * kNormBlocks;  // doubles per norm set
     auto         norms = [&](double* const* u, const double* const* dlt, size_t slot) {
-        for (int d = 0; d < nd; d++)
-            launch_update_norms(u[d], dom_n(c, d), dlt ? dlt[d] : nullptr, dom_n(c, d), dom_n(c, d), nrhs,
-                                c->rpart.p + slot * half + (size_t)2 * d * nrhs * kNormBlocks, s);
+        NormsBatch nb;
+        for (int d = 0; d < nd; d++) {
+            nb.j[nb.n++] = {u[d], dom_n(c, d), dlt ? dlt[d] : nullptr, dom_n(c, d), dom_n(c, d),
+                            c->rpart.p + slot * half + (size_t)2 * d * nrhs * kNormBlocks};
+            if (nb.n == 8 || d == nd - 1) {
+                launch_update_norms_batch(nb, nrhs, s);
+                nb.n = 0;
+            }
+        }
     };
     auto sums = [&](size_t slot, int r, double& dd, double& uu) {
         dd = uu = 0.0;
@@ -1068,7 +1084,8 @@ static void cat_refine(fp_cat_s* c, double* const* f, int nrhs, hipStream_t s)
     std::vector<double*>       w(nd);
     std::vector<const double*> wc(nd);
     for (int d = 0; d < nd; d++) wc[d] = w[d] = c->rw[d].p;
-    c->mixed = true;
+    c->mixed   = true;
+    c->no_wait = fast;
     std::vector<double> prev(nrhs, 0.0);
     int                 steps = 0;
     double              est   = INFINITY;
@@ -1107,10 +1124,12 @@ static void cat_refine(fp_cat_s* c, double* const* f, int nrhs, hipStream_t s)
             if (bad || est <= c->refine_tol) break;
         }
     } catch (...) {
-        c->mixed = false;
+        c->mixed   = false;
+        c->no_wait = false;
         throw;
     }
     c->mixed             = false;
+    c->no_wait           = false;
     c->last_refine_steps = steps;
     c->last_refine_est   = est;
     if (!std::isfinite(est)) c->last_status = FP_E_NUMERIC;
