@@ -119,7 +119,10 @@ int fp_sub_solve_refine(fp_sub sub, double* d_f, int nrhs, long long ld, double 
                         fp_solve_info* info);
 /* Many independent subdomain solves of different sizes in one call: the
  * reference's loop over independent PoissonSolver2D objects.  d_f[k] holds
- * nrhs[k] fields of subs[k] (ld = nx*ny). */
+ * nrhs[k] fields of subs[k] (ld = nx*ny).  Each phase (forward transform,
+ * x sweep, inverse transform) is one launch per group of problems sharing a
+ * launch shape, whatever their number; more than 8 problems cannot be
+ * recorded into a HIP graph (their descriptors are uploaded per call). */
 int fp_sub_solve_batched(fp_ctx ctx, int nprob, const fp_sub* subs, double* const* d_f, const int* nrhs,
                          void* stream);
 int fp_sub_solve_batched_f32(fp_ctx ctx, int nprob, const fp_sub* subs, float* const* d_f, const int* nrhs,
