@@ -677,12 +677,27 @@ __device__ __forceinline__ void load_rows(const YArgs<T>& a, const RowPair<T>& r
         }
         return;
     }
+    if (kind == T_NN && inv) {
+        // DCT pair inverse by Makhoul's N-point algorithm: x = P^T Re FFT(w),
+        // w_m = a_m exp(-i pi m / 2N), a_m = s_m p_{N-1-m}; the real part is
+        // the FFT of the Hermitian part h_m = (w_m + conj w_{(N-m) mod N}) / 2,
+        // so both rows pack as h_a + i h_b
+        for (int m = threadIdx.x; m < N; m += NT) {
+            const int mr = m == 0 ? 0 : N - m;
+            const T   sm = m == 0 ? a.c1 : a.c0, sr = mr == 0 ? a.c1 : a.c0;  // sqrt(1/N) or -sqrt(2/N)
+            const C2<T> em = a.aux[m], er = a.aux[mr];                        // exp(-i pi m / 2N)
+            const T pam = sm * ld_src(rp.ia + (N - 1 - m)), pbm = rp.hb ? sm * ld_src(rp.ia + N + (N - 1 - m)) : T(0);
+            const T par = sr * ld_src(rp.ia + (N - 1 - mr)), pbr = rp.hb ? sr * ld_src(rp.ia + N + (N - 1 - mr)) : T(0);
+            const T har = T(0.5) * (pam * em.x + par * er.x), hai = T(0.5) * (pam * em.y - par * er.y);
+            const T hbr = T(0.5) * (pbm * em.x + pbr * er.x), hbi = T(0.5) * (pbm * em.y - pbr * er.y);
+            A[zp<PAD>(m)] = {har - hbi, hai + hbr};
+        }
+        return;
+    }
     if (kind == T_DD) {
         if (threadIdx.x == 0) A[zp<PAD>(0)] = {T(0), T(0)};
-    } else if (kind == T_PP) {
-    } else if (kind == T_NN && inv) {
-        if (threadIdx.x == 0) A[zp<PAD>(N)] = {T(0), T(0)};
-    } else if (kind != T_NN) {
+    } else if (kind == T_PP || kind == T_NN) {
+    } else {
         if (threadIdx.x == 0) A[zp<PAD>(0)] = {T(0), T(0)};
         for (int p = N + 1 + threadIdx.x; p < L; p += NT) A[zp<PAD>(p)] = {T(0), T(0)};
     }
@@ -707,20 +722,9 @@ __device__ __forceinline__ void load_rows(const YArgs<T>& a, const RowPair<T>& r
                 A[zp<PAD>(n + 1)] = {x, y};
             } else if (kind == T_PP) {
                 A[zp<PAD>(n)] = {x, y};
-            } else if (kind == T_NN && !inv) {
-                A[zp<PAD>(n)]             = {x, y};
-                A[zp<PAD>(2 * N - 1 - n)] = {x, y};
             } else if (kind == T_NN) {
-                const int   m  = N - 1 - n;
-                const T     s  = m == 0 ? a.c1 : a.c0;  // sqrt(1/N) or -sqrt(2/N)
-                const T     ya = s * x, yb = s * y;
-                const C2<T> e  = a.aux[m];              // exp(-i pi m / 2N); Y_m uses conj(e)
-                A[zp<PAD>(m)]           = {ya * e.x + yb * e.y, -ya * e.y + yb * e.x};
-                if (m > 0) A[zp<PAD>(2 * N - m)] = {ya * e.x - yb * e.y, ya * e.y + yb * e.x};
-                else {
-                    y0[0] = ya;
-                    y0[1] = yb;
-                }
+                // Makhoul's reordering: even entries ascending, odd ones descending
+                A[zp<PAD>((n & 1) ? N - 1 - (n >> 1) : (n >> 1))] = {x, y};
             } else if (kind == T_ND && inv) {
                 A[zp<PAD>(n + 1)] = {x, y};
             } else {  // DN both ways, ND forward
@@ -841,15 +845,19 @@ __device__ __forceinline__ void store_rows(const YArgs<T>& a, const RowPair<T>& 
             if (j >= N) break;
             T oa, ob;
             if (kind == T_NN && !inv) {
-                const int   m = N - 1 - j;
-                const C2<T> w = cmul(Z[zp<PAD>(m)], sc[k]);  // 2 C_a(m) + 2i C_b(m)
-                const T     s = T(0.5) * ((j == N - 1) ? a.c1 : a.c0);
-                oa            = s * w.x;
-                ob            = s * w.y;
+                // C(m) = Re(exp(-i pi m / 2N) V_m), V_a / V_b of the rows split
+                // from Z_m and Z_{(N-m) mod N}; mode j holds m = N-1-j
+                const int   m  = N - 1 - j;
+                const C2<T> zm = Z[zp<PAD>(m)], zr = Z[zp<PAD>(m == 0 ? 0 : N - m)];
+                const T     var = T(0.5) * (zm.x + zr.x), vai = T(0.5) * (zm.y - zr.y);
+                const T     vbr = T(0.5) * (zm.y + zr.y), vbi = T(0.5) * (zr.x - zm.x);
+                const T     s   = (j == N - 1) ? a.c1 : a.c0;
+                oa              = s * (sc[k].x * var - sc[k].y * vai);
+                ob              = s * (sc[k].x * vbr - sc[k].y * vbi);
             } else if (kind == T_NN) {
-                const C2<T> z = Z[zp<PAD>(j)];
-                oa            = T(0.5) * (z.x + y0[0]);
-                ob            = T(0.5) * (z.y + y0[1]);
+                const C2<T> z = Z[zp<PAD>((j & 1) ? N - 1 - (j >> 1) : (j >> 1))];
+                oa            = z.x;
+                ob            = z.y;
             } else if (kind == T_PP && !inv) {
                 // mode j of frequency q: sqrt(2/N) Im(w_j conj F_q), F_a / F_b
                 // of the rows split from Z_q and Z_{N-q}
@@ -930,7 +938,9 @@ __global__ __launch_bounds__(kMaxNT) void k_ytransform_sb(BT bt)
 
     load_rows<kZpad>(a, rp, Z, y0);
     __syncthreads();
-    const bool fwd = !(a.kind == T_NN && a.inverse);
+    // every kind runs the forward FFT (the inverses by conjugate symmetry or
+    // index reversal)
+    const bool fwd = true;
     if (a.F != a.L) {
         if (fwd) run_bluestein<true, OM, EPT>(a, W, Z, cs, fr);
         else run_bluestein<false, OM, EPT>(a, W, Z, cs, fr);
@@ -962,8 +972,7 @@ __global__ __launch_bounds__(kMaxNT) void k_ytransform_gen(BT bt)
 
     load_rows<0>(a, rp, A, y0);
     __syncthreads();
-    const bool   fwd = !(a.kind == T_NN && a.inverse);
-    const C2<T>* Z   = fwd ? run_fft_gen<true>(a, W, A, B) : run_fft_gen<false>(a, W, A, B);
+    const C2<T>* Z = run_fft_gen<true>(a, W, A, B);
     // DD prefix sums: in the buffer the FFT did not end in, when it is long enough
     C2<T>* Sd = gen_scan_in_buffer(a.L, a.N) ? (Z == A ? B : A) : S;
     store_rows<0>(a, rp, Z, y0, Sd, Sd + a.N / 2 + 1);
@@ -1091,7 +1100,7 @@ void TransformPlan<T>::build(int k, int n)
     kind = k;
     N    = n;
     if (kind == T_DD) L = N + 1;
-    else if (kind == T_NN) L = 2 * N;
+    else if (kind == T_NN) L = N;  // Makhoul: one N-point FFT per row pair
     else if (kind == T_PP) L = N;
     else L = 2 * N + 1;
     if (kind == T_PP && N % 2 != 0) throw Error(FP_E_INVALID, "the periodic transform needs an even length");
