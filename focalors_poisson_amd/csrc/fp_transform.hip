@@ -214,8 +214,26 @@ constexpr int sb_bpt()
 }
 // odd-prime butterflies: output pairs per thread group, groups per thread
 constexpr int kNQ  = 5;
-constexpr int kIPT = 2;
+constexpr int kIPT = 1;
 constexpr int kMaxOddPrime = 47;
+
+// The pass functions below are inlined into one loop over the passes; their
+// index arithmetic depends only on (L, Ns, R), which do not change inside
+// the loop, so the compiler would hoist every pass's addresses out of it and
+// keep them all live (about 130 extra VGPRs).  An empty asm that "redefines"
+// the lengths at the top of each pass keeps that arithmetic inside the pass.
+__device__ __forceinline__ void pass_lengths(int& L, int& Ns)
+{
+    L  = __builtin_amdgcn_readfirstlane(L);
+    Ns = __builtin_amdgcn_readfirstlane(Ns);
+    asm volatile("" : "+s"(L), "+s"(Ns));
+}
+__device__ __forceinline__ void pass_lengths(int& L, int& Ns, int& R)
+{
+    R = __builtin_amdgcn_readfirstlane(R);
+    asm volatile("" : "+s"(R));
+    pass_lengths(L, Ns);
+}
 
 // One Stockham pass of radix R on the single buffer Z: read every input,
 // synchronise, twiddle + butterfly, write, synchronise.  Twiddles of a
@@ -224,6 +242,7 @@ constexpr int kMaxOddPrime = 47;
 template <int R, bool FWD, int EPT, typename T>
 __device__ __forceinline__ void pass_reg_sb(C2<T>* __restrict__ Z, const Tw<T> W, int L, int Ns)
 {
+    pass_lengths(L, Ns);
     constexpr int BPT = sb_bpt<R, EPT>();
     const int     NT  = blockDim.x;
     const int     nb = L / R, stride = L / (Ns * R);
@@ -285,6 +304,7 @@ template <bool FWD, typename T>
 __device__ __forceinline__ void pass_odd_sb(C2<T>* __restrict__ Z, const Tw<T> W, int L, int Ns, int R,
                                             const C2<T>* __restrict__ csg)
 {
+    pass_lengths(L, Ns, R);
     const int NT = blockDim.x;
     const int H = (R - 1) >> 1, G = (H + kNQ - 1) / kNQ;
     const int nb = L / R, stride = L / (Ns * R), nbp = odd_slots(nb);
@@ -310,7 +330,6 @@ __device__ __forceinline__ void pass_odd_sb(C2<T>* __restrict__ Z, const Tw<T> W
             // row r of the table holds (cos, sin)(2 pi q r / R) for q = 1..G*kNQ
             const C2<T>* crow = csg + (QB - 1);
             const int    ldc  = G * kNQ;
-#pragma unroll 4
             for (int r = 1; r <= H; r++) {
                 C2<T> x1 = Z[zp<kZpad>(jb + r * nb)], x2 = Z[zp<kZpad>(jb + (R - r) * nb)];
                 if (Ns > 1) {
@@ -380,6 +399,7 @@ template <bool FWD>
 __device__ __forceinline__ void pass_odd_mfma(C2<double>* __restrict__ Z, const Tw<double> W, int L, int Ns, int R,
                                               const double* __restrict__ frag)
 {
+    pass_lengths(L, Ns, R);
     const int H = (R - 1) >> 1, MT = (H + 16) >> 4, KT = mfma_ksteps(R);
     const int nb = L / R, stride = L / (Ns * R);
     const int NTL  = (2 * nb + 15) >> 4;
