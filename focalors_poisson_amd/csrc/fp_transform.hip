@@ -656,7 +656,7 @@ __device__ __forceinline__ T assemble_at(const YArgs<T>& a, int rhs, int i, int 
 
 // Builds the packed extended sequence of the row pair in A (length L).
 template <int PAD, typename T>
-__device__ __forceinline__ void load_rows(const YArgs<T>& a, const RowPair<T>& rp, C2<T>* A, T* y0)
+__device__ __forceinline__ void load_rows(const YArgs<T>& a, const RowPair<T>& rp, C2<T>* A)
 {
     const int  NT = blockDim.x;
     const int  N = a.N, L = a.L;
@@ -775,16 +775,17 @@ __device__ __forceinline__ void load_rows(const YArgs<T>& a, const RowPair<T>& r
 // thread chunk, a wavefront scan of the chunk totals, the wavefronts'
 // totals (wt, one per wavefront) added in order.  Summation depth is
 // logarithmic in n apart from the chunk (ceil(n / blockDim) terms).
-template <typename T>
+template <int PAD, typename T>
 __device__ void block_scan(C2<T>* S, int n, C2<T>* wt)
 {
+    // S[i] lives at S[zp<PAD>(i)] (the transform's own buffer)
     const int NT = blockDim.x, t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int cpt = (n + NT - 1) / NT;
     const int b0 = min(n, t * cpt), b1 = min(n, b0 + cpt);
     C2<T>     run = {T(0), T(0)};
     for (int q = b0; q < b1; q++) {
-        run  = cadd(run, S[q]);
-        S[q] = run;
+        run              = cadd(run, S[zp<PAD>(q)]);
+        S[zp<PAD>(q)] = run;
     }
     C2<T> v = run;
     for (int d = 1; d < 64; d <<= 1) {
@@ -796,14 +797,13 @@ __device__ void block_scan(C2<T>* S, int n, C2<T>* wt)
     if (lane == 63) wt[wave] = v;
     __syncthreads();
     for (int w = 0; w < wave; w++) ex = cadd(wt[w], ex);
-    for (int q = b0; q < b1; q++) S[q] = cadd(S[q], ex);
+    for (int q = b0; q < b1; q++) S[zp<PAD>(q)] = cadd(S[zp<PAD>(q)], ex);
     __syncthreads();
 }
 
 // Splits the two rows' coefficients out of the spectrum Z and stores them.
 template <int PAD, typename T>
-__device__ __forceinline__ void store_rows(const YArgs<T>& a, const RowPair<T>& rp, const C2<T>* Z, const T* y0,
-                                           C2<T>* S, C2<T>* wt)
+__device__ __forceinline__ void store_rows(const YArgs<T>& a, const RowPair<T>& rp, C2<T>* Z, C2<T>* wt)
 {
     const int  NT = blockDim.x;
     const int  N = a.N, L = a.L;
@@ -815,29 +815,27 @@ __device__ __forceinline__ void store_rows(const YArgs<T>& a, const RowPair<T>& 
     };
     if (kind == T_DD) {
         // rows a, b: F_a(q) = (Z_q + conj Z_{L-q}) / 2, F_b(q) = (Z_q - conj Z_{L-q}) / 2i
-        const int Ko = (N + 1) / 2;  // odd outputs X_1, X_3, .. X_{2Ko-1}
-        for (int q = threadIdx.x; q < Ko; q += NT) {
+        // odd outputs X_1, X_3, .. X_{2Ko-1}: prefix sums of Re F(q), q < Ko;
+        // even outputs X_2 .. X_{2Ke}: -Im F(k), k = 1..Ke.  Each thread
+        // reads the pair Z_q, Z_{L-q} and writes its results back into the
+        // same two slots (Re F into slot q, X_{2q} into slot L-q), so the
+        // scan runs in place in the transform's buffer
+        const int Ko = (N + 1) / 2, Ke = N / 2;
+        for (int q = threadIdx.x; q <= Ke || q < Ko; q += NT) {
             const C2<T> zq = Z[zp<PAD>(q)];
-            if (q == 0) S[0] = {T(0.5) * zq.x, T(0.5) * zq.y};
-            else {
-                const C2<T> zm = Z[zp<PAD>(L - q)];
-                S[q]           = {T(0.5) * (zq.x + zm.x), T(0.5) * (zq.y + zm.y)};  // (Re F_a, Re F_b)
-            }
+            const C2<T> zm = q > 0 ? Z[zp<PAD>(L - q)] : C2<T>{T(0), T(0)};  // X_1 starts at Re F(0) / 2
+            if (q >= 1) Z[zp<PAD>(L - q)] = {T(0.5) * (zm.y - zq.y), T(0.5) * (zq.x - zm.x)};
+            if (q < Ko) Z[zp<PAD>(q)] = {T(0.5) * (zq.x + zm.x), T(0.5) * (zq.y + zm.y)};  // (Re F_a, Re F_b)
         }
         __syncthreads();
-        block_scan(S, Ko, wt);
+        block_scan<PAD>(Z, Ko, wt);
         for (int base = 0; base < N; base += VPT * NT) {
 #pragma unroll
             for (int k = 0; k < VPT; k++) {
                 const int j = base + k * NT + threadIdx.x;
                 if (j >= N) break;
-                C2<T> X;
-                if (j & 1) {  // X_{j+1}, even: -Im F(k), k = (j+1)/2
-                    const int   kk = (j + 1) >> 1;
-                    const C2<T> zk = Z[zp<PAD>(kk)], zm = Z[zp<PAD>(L - kk)];
-                    X              = {T(0.5) * (zm.y - zk.y), T(0.5) * (zk.x - zm.x)};
-                } else
-                    X = S[j >> 1];
+                // X_{j+1}: even ones in slot L - (j+1)/2, odd ones in slot j/2
+                const C2<T> X = Z[zp<PAD>((j & 1) ? L - ((j + 1) >> 1) : (j >> 1))];
                 st_dst(rp.oa + j, a.c0 * X.x);
                 if (rp.hb) st_dst(rp.oa + N + j, a.c0 * X.y);
             }
@@ -948,15 +946,12 @@ __global__ __launch_bounds__(kMaxNT) void k_ytransform_sb(BT bt)
     C2<T>*    thi = tlo + 64;           // L/64 + 1 twiddles W^(64m)
     C2<T>*    cs  = thi + nhi;          // cos/sin tables of the odd-prime passes (FP32)
     double*   fr  = reinterpret_cast<double*>(cs + a.ncs);  // matrix-core fragments (FP64)
-    T*        y0  = reinterpret_cast<T*>(fr + a.nfrag);     // NN inverse: y_0 of rows a, b
-    // DD: prefix sums (N/2 + 1), then 8 wavefront totals; with Bluestein in
-    // the part of Z past the result
-    C2<T>* S = a.F != a.L ? Z + zp<kZpad>(a.L) + 1 : reinterpret_cast<C2<T>*>(y0 + 4);
+    C2<T>*    wt  = reinterpret_cast<C2<T>*>(fr + a.nfrag); // DD: wavefront totals of the prefix scan
     for (int q = threadIdx.x; q < 64 + nhi + a.ncs; q += NT) tlo[q] = a.W[q];
     for (int q = threadIdx.x; q < a.nfrag; q += NT) fr[q] = a.frag[q];
     const Tw<T> W{tlo, thi};
 
-    load_rows<kZpad>(a, rp, Z, y0);
+    load_rows<kZpad>(a, rp, Z);
     __syncthreads();
     // every kind runs the forward FFT (the inverses by conjugate symmetry or
     // index reversal)
@@ -966,10 +961,8 @@ __global__ __launch_bounds__(kMaxNT) void k_ytransform_sb(BT bt)
         else run_bluestein<false, OM, EPT>(a, W, Z, cs, fr);
     } else if (fwd) run_fft_sb<true, OM, EPT>(a, W, Z, cs, fr);
     else run_fft_sb<false, OM, EPT>(a, W, Z, cs, fr);
-    store_rows<kZpad>(a, rp, Z, y0, S, S + a.N / 2 + 1);
+    store_rows<kZpad>(a, rp, Z, wt);
 }
-
-__host__ __device__ __forceinline__ bool gen_scan_in_buffer(int L, int N) { return L >= N / 2 + 1 + 8; }
 
 template <typename T, typename BT>
 __global__ __launch_bounds__(kMaxNT) void k_ytransform_gen(BT bt)
@@ -985,17 +978,14 @@ __global__ __launch_bounds__(kMaxNT) void k_ytransform_gen(BT bt)
     C2<T>*    tlo = B + a.L;
     C2<T>*    thi = tlo + 64;
     const int nhi = a.L / 64 + 1;
-    T*        y0  = reinterpret_cast<T*>(thi + nhi);
-    C2<T>*    S   = reinterpret_cast<C2<T>*>(y0 + 4);
+    C2<T>*    wt  = thi + nhi;  // DD: wavefront totals of the prefix scan
     for (int q = threadIdx.x; q < 64 + nhi; q += NT) tlo[q] = a.W[q];
     const Tw<T> W{tlo, thi};
 
-    load_rows<0>(a, rp, A, y0);
+    load_rows<0>(a, rp, A);
     __syncthreads();
-    const C2<T>* Z = run_fft_gen<true>(a, W, A, B);
-    // DD prefix sums: in the buffer the FFT did not end in, when it is long enough
-    C2<T>* Sd = gen_scan_in_buffer(a.L, a.N) ? (Z == A ? B : A) : S;
-    store_rows<0>(a, rp, Z, y0, Sd, Sd + a.N / 2 + 1);
+    C2<T>* Z = const_cast<C2<T>*>(run_fft_gen<true>(a, W, A, B));
+    store_rows<0>(a, rp, Z, wt);
 }
 
 // ---------------------------------------------------------------------------
@@ -1106,12 +1096,10 @@ static size_t lds_bytes(const TransformPlan<T>& tp, int odd)
     const size_t nfrag = odd == ODD_MFMA ? tp.frag.n : 0;
     const size_t nhi = tp.F / 64 + 1;
     const size_t zl  = (size_t)tp.F + (kZpad ? ((size_t)tp.F >> kZpad) : 0) + 1;
-    // DD prefix sums + wavefront totals (with Bluestein inside Z, past the result)
-    const size_t sc = (tp.kind == T_DD && tp.F == tp.L) ? (size_t)tp.N / 2 + 1 + 8 : 0;
-    if (tp.engine == 1)
-        return (zl + 64 + nhi + ncs + sc) * sizeof(C2<T>) + nfrag * sizeof(double) + 4 * sizeof(T);
-    const size_t scg = gen_scan_in_buffer(tp.L, tp.N) ? 0 : sc;
-    return (2 * (size_t)tp.L + 64 + nhi + scg) * sizeof(C2<T>) + 4 * sizeof(T);
+    // wavefront totals of the DD prefix scan (which runs inside Z), one per wavefront
+    const size_t sc = tp.kind == T_DD ? (size_t)kMaxNT / 64 : 0;
+    if (tp.engine == 1) return (zl + 64 + nhi + ncs + sc) * sizeof(C2<T>) + nfrag * sizeof(double);
+    return (2 * (size_t)tp.L + 64 + nhi + sc) * sizeof(C2<T>);
 }
 
 template <typename T>
