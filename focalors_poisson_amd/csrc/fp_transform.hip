@@ -295,10 +295,10 @@ __host__ __device__ __forceinline__ int odd_slots(int nb)
 // The H = (R-1)/2 output pairs of a butterfly are split over G groups of kNQ
 // pairs; work item t = (group, butterfly) with the butterflies padded to a
 // multiple of 64 (or to 16 / 32 lanes for few butterflies, odd_slots), so a
-// wavefront holds one to four groups and reads the cosines of its groups,
-// (cos, sin)(2 pi q r / R) in a table of r-major rows of G*kNQ values staged
-// in LDS with the twiddles, at one to four addresses per read.  Items a
-// thread owns keep their accumulators in registers across the
+// wavefront holds one to four groups.  The cosines come from the R values
+// (cos, sin)(2 pi m / R) staged in LDS with the twiddles, at m = q r mod R
+// kept by one addition per read (one to four addresses per wavefront).  The
+// item a thread owns keeps its accumulators in registers across the
 // synchronisation.
 template <bool FWD, typename T>
 __device__ __forceinline__ void pass_odd_sb(C2<T>* __restrict__ Z, const Tw<T> W, int L, int Ns, int R,
@@ -327,9 +327,9 @@ __device__ __forceinline__ void pass_odd_sb(C2<T>* __restrict__ Z, const Tw<T> W
                 bx[it][q] = T(0);
                 by[it][q] = T(0);
             }
-            // row r of the table holds (cos, sin)(2 pi q r / R) for q = 1..G*kNQ
-            const C2<T>* crow = csg + (QB - 1);
-            const int    ldc  = G * kNQ;
+            int m[kNQ];  // (QB + q) r mod R
+#pragma unroll
+            for (int q = 0; q < kNQ; q++) m[q] = 0;
             for (int r = 1; r <= H; r++) {
                 C2<T> x1 = Z[zp<kZpad>(jb + r * nb)], x2 = Z[zp<kZpad>(jb + (R - r) * nb)];
                 if (Ns > 1) {
@@ -338,10 +338,11 @@ __device__ __forceinline__ void pass_odd_sb(C2<T>* __restrict__ Z, const Tw<T> W
                 }
                 const C2<T> a = cadd(x1, x2), b = csub(x1, x2);
                 x0[it] = cadd(x0[it], a);
-                const C2<T>* cr = crow + (r - 1) * ldc;
 #pragma unroll
                 for (int q = 0; q < kNQ; q++) {
-                    const C2<T> w = cr[q];
+                    m[q] += QB + q;  // QB + q < R
+                    if (m[q] >= R) m[q] -= R;
+                    const C2<T> w = csg[m[q]];
                     ax[it][q] += a.x * w.x;
                     ay[it][q] += a.y * w.x;
                     bx[it][q] += b.x * w.y;
@@ -496,7 +497,7 @@ __device__ __forceinline__ void run_fft_sb(const YArgs<T>& a, const Tw<T> W, C2<
             fo += 2 * 2 * mfma_ksteps(R) * 64;
         } else if constexpr (OM == ODD_VALU) {
             pass_odd_sb<FWD>(Z, W, a.F, Ns, R, cs + co);
-            co += ((R - 1) / 2) * (((R - 1) / 2 + kNQ - 1) / kNQ * kNQ);
+            co += R;
         }
         Ns *= R;
     }
@@ -1143,20 +1144,18 @@ void TransformPlan<T>::build(int k, int n)
     }
     const long double PI  = 3.141592653589793238462643383279502884L;
     const int         nhi = F / 64 + 1;
-    // odd-prime cos/sin tables, one per odd-prime pass in schedule order
+    // odd-prime passes: (cos, sin)(2 pi m / R), m = 0..R-1, one table per
+    // pass in schedule order
     std::vector<long double> cs;
     ncs = 0;
     for (int s = 0; s < nrad; s++) {
         const int R = rad[s];
         if (R <= 8 && R != 7) continue;
-        const int H = (R - 1) / 2, QW = (H + kNQ - 1) / kNQ * kNQ;
-        for (int r = 1; r <= H; r++)
-            for (int q = 1; q <= QW; q++) {
-                const int m = (int)(((long long)q * r) % R);
-                cs.push_back(q <= H ? cosl(2.0L * PI * m / R) : 0.0L);
-                cs.push_back(q <= H ? sinl(2.0L * PI * m / R) : 0.0L);
-            }
-        ncs += H * QW;
+        for (int m = 0; m < R; m++) {
+            cs.push_back(cosl(2.0L * PI * m / R));
+            cs.push_back(sinl(2.0L * PI * m / R));
+        }
+        ncs += R;
     }
     // FP64: matrix-core fragments [cos/sin][M tile][K step][lane] of every odd-prime pass
     std::vector<double> fr;
