@@ -49,11 +49,15 @@ def parse():
 
 def time_kernel(fn, reps, torch):
     """Average duration (ms) of fn() over reps launches, HIP events on the
-    current stream (the stream the library launches on)."""
+    current stream (the stream the library launches on).  A spin kernel of a
+    few milliseconds holds the stream while the host queues the launches, so
+    that they run back to back and the average is the kernel's duration, not
+    the host's launch interval."""
     fn()
     torch.cuda.synchronize()
     s = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda._sleep(5_000_000)
     e0.record(s)
     for _ in range(reps):
         fn()

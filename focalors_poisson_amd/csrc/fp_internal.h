@@ -98,8 +98,15 @@ __device__ __forceinline__ const P& table_find(const JobTable<P>& t, int blk, in
     local = blk - t.start[lo];
     return t.p[lo];
 }
+// Pinned host staging (fp_staging.cpp): a block of at least n bytes, free
+// for the caller to fill; staging_put queues its reuse after the work queued
+// on s so far (call it right after queuing the copy that reads the block).
+void* staging_get(size_t n);
+void  staging_put(void* p, hipStream_t s);
+
 // Copies n descriptors and their workgroup offsets into stream-ordered device
-// memory (released with table_release after the launch on the same stream).
+// memory (released with table_release after the launch on the same stream),
+// through pinned staging so that the host does not wait for the device.
 // Not for a stream under graph capture: the copy reads host memory.
 template <typename P>
 JobTable<P> table_upload(const std::vector<P>& p, const std::vector<int>& start, hipStream_t s, void*& mem)
@@ -109,11 +116,14 @@ JobTable<P> table_upload(const std::vector<P>& p, const std::vector<int>& start,
     if (cs != hipStreamCaptureStatusNone)
         throw Error(FP_E_INVALID, "batches of more than 8 subdomains cannot be captured");
     const size_t pb = (p.size() * sizeof(P) + 255) & ~(size_t)255;
-    std::vector<unsigned char> h(pb + start.size() * sizeof(int));
-    std::memcpy(h.data(), p.data(), p.size() * sizeof(P));
-    std::memcpy(h.data() + pb, start.data(), start.size() * sizeof(int));
-    FP_HIP(hipMallocAsync(&mem, h.size(), s));
-    FP_HIP(hipMemcpyAsync(mem, h.data(), h.size(), hipMemcpyHostToDevice, s));
+    const size_t nb = pb + start.size() * sizeof(int);
+    FP_HIP(hipMallocAsync(&mem, nb, s));
+    unsigned char* h = static_cast<unsigned char*>(staging_get(nb));
+    std::memcpy(h, p.data(), p.size() * sizeof(P));
+    std::memcpy(h + pb, start.data(), start.size() * sizeof(int));
+    const hipError_t e = hipMemcpyAsync(mem, h, nb, hipMemcpyHostToDevice, s);
+    staging_put(h, s);
+    FP_HIP(e);
     JobTable<P> t;
     t.nprob = (int)p.size();
     t.p     = static_cast<const P*>(mem);
@@ -146,6 +156,7 @@ struct TransformPlan {
     T          c0 = 0, c1 = 0;
     int        nt = 256;  // threads per workgroup (tuned)
     int        odd = 0;   // odd-prime pass form: 0 none, 1 matrix cores (FP64), 2 VALU pair sums (tuned)
+    int        nt1 = 0, odd1 = -1;  // the same for launches of one field (tuned; 0 / -1: nt, odd)
     int        ept = 16;  // values per thread of the register butterflies (24 for long Bluestein lengths)
     void build(int kind, int N);
 };

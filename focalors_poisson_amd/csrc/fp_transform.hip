@@ -1427,15 +1427,20 @@ void launch_ytransform(const TransformPlan<T>& tp, T* data, long long ld, int nx
     j.nx    = nx;
     j.nrhs  = nrhs;
     j.dst64 = sizeof(T) == 8;
+    if (nrhs == 1 && tp.nt1) {
+        j.nt  = tp.nt1;
+        j.odd = tp.odd1;
+    }
     launch_ytransform_batch<T>(&j, 1, inverse, s);
 }
 
 // Threads per workgroup and odd-prime pass form of a plan's transform,
 // chosen by timing the forward transform of an nx-row scratch field with
-// every combination that fits.  The field holds enough right-hand sides for about 32 MB, so
-// that the candidates differ by throughput rather than by launch latency
-// (where they tie within noise), and each candidate keeps its best of three
-// rounds.  Recorded per (kind, N, nx, precision) (tune_store).
+// every combination that fits, twice: with enough right-hand sides for about
+// 32 MB (the throughput regime; nt, odd) and with one (a launch of a few
+// workgroups per CU, bound by each workgroup's latency; nt1, odd1, used for
+// launches of one field).  Each candidate keeps its best of three rounds.
+// Recorded per (kind, N, nx, precision) (tune_store).
 
 template <typename T>
 void tune_transform(TransformPlan<T>& tp, int nx, bool measure)
@@ -1452,44 +1457,56 @@ void tune_transform(TransformPlan<T>& tp, int nx, bool measure)
                             std::to_string(nx) + " " + std::to_string(sizeof(T));
     {
         std::vector<int> v;
-        if (tune_lookup(key, v) && v.size() == 2) {
+        if (tune_lookup(key, v) && v.size() == 4) {
             for (const auto& c : cand)
-                if (c[0] == v[0] && c[1] == v[1]) {
-                    tp.nt  = v[0];
-                    tp.odd = v[1];
-                    return;
-                }
+                if (c[0] == v[0] && c[1] == v[1])
+                    for (const auto& c1 : cand)
+                        if (c1[0] == v[2] && c1[1] == v[3]) {
+                            tp.nt   = v[0];
+                            tp.odd  = v[1];
+                            tp.nt1  = v[2];
+                            tp.odd1 = v[3];
+                            return;
+                        }
         }
     }
     const long long n1   = (long long)nx * tp.N;
-    const int       nrhs = (int)std::max<long long>(1, std::min<long long>(64, (32LL << 20) / (n1 * sizeof(T))));
-    DevBuf<T>       buf((size_t)(n1 * nrhs));
-    FP_HIP(hipMemset(buf.p, 0, (size_t)(n1 * nrhs) * sizeof(T)));
+    const int       nmax = (int)std::max<long long>(1, std::min<long long>(64, (32LL << 20) / (n1 * sizeof(T))));
+    DevBuf<T>       buf((size_t)(n1 * nmax));
+    FP_HIP(hipMemset(buf.p, 0, (size_t)(n1 * nmax) * sizeof(T)));
     hipEvent_t e0, e1;
     FP_HIP(hipEventCreate(&e0));
     FP_HIP(hipEventCreate(&e1));
-    std::vector<float> best_of(cand.size(), 1e30f);
-    for (int round = 0; round < 3; round++)
-        for (size_t ci = 0; ci < cand.size(); ci++) {
-            tp.nt  = cand[ci][0];
-            tp.odd = cand[ci][1];
-            launch_ytransform<T>(tp, buf.p, n1, nx, nrhs, 0, nullptr);
-            FP_HIP(hipEventRecord(e0, nullptr));
-            for (int r = 0; r < 3; r++) launch_ytransform<T>(tp, buf.p, n1, nx, nrhs, 0, nullptr);
-            FP_HIP(hipEventRecord(e1, nullptr));
-            FP_HIP(hipEventSynchronize(e1));
-            float ms = 0.f;
-            FP_HIP(hipEventElapsedTime(&ms, e0, e1));
-            best_of[ci] = std::min(best_of[ci], ms);
-        }
-    size_t ib = 0;
-    for (size_t ci = 1; ci < cand.size(); ci++)
-        if (best_of[ci] < best_of[ib]) ib = ci;
+    tp.nt1 = 0;
+    auto pick = [&](int nrhs) {
+        std::vector<float> best_of(cand.size(), 1e30f);
+        for (int round = 0; round < 3; round++)
+            for (size_t ci = 0; ci < cand.size(); ci++) {
+                tp.nt  = cand[ci][0];
+                tp.odd = cand[ci][1];
+                launch_ytransform<T>(tp, buf.p, n1, nx, nrhs, 0, nullptr);
+                FP_HIP(hipEventRecord(e0, nullptr));
+                for (int r = 0; r < 3; r++) launch_ytransform<T>(tp, buf.p, n1, nx, nrhs, 0, nullptr);
+                FP_HIP(hipEventRecord(e1, nullptr));
+                FP_HIP(hipEventSynchronize(e1));
+                float ms = 0.f;
+                FP_HIP(hipEventElapsedTime(&ms, e0, e1));
+                best_of[ci] = std::min(best_of[ci], ms);
+            }
+        size_t ib = 0;
+        for (size_t ci = 1; ci < cand.size(); ci++)
+            if (best_of[ci] < best_of[ib]) ib = ci;
+        return cand[ib];
+    };
+    const std::array<int, 2> one = pick(1);
+    const std::array<int, 2> many = nmax > 1 ? pick(nmax) : one;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    tp.nt  = cand[ib][0];
-    tp.odd = cand[ib][1];
-    tune_store(key, {tp.nt, tp.odd});
+    tp.nt   = many[0];
+    tp.odd  = many[1];
+    tp.nt1  = one[0];
+    tp.odd1 = one[1];
+    tune_store(key, {tp.nt, tp.odd, tp.nt1, tp.odd1});
 }
 // Launch shape for a batch of transforms launched together (the domains of
 // a composite solve): every job of the single-buffer engine takes the same
