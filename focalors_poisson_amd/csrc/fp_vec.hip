@@ -327,24 +327,30 @@ void launch_schur_lines(const SchurLinesArgs& a, int nrhs, hipStream_t s)
 }
 
 // Column sums of the line-sourced sweeps: yl[r][w*nx + i] (w = 0, 1) =
-// sum over the mode blocks b of cp[(r*nblk + b)*2nx + w*nx + i], one
-// wavefront per output in a fixed order (reproducible).  yl entries are ldy
-// apart per right-hand side.
+// sum over the mode blocks b of cp[(r*nblk + b)*2nx + w*nx + i]: 64
+// consecutive outputs per workgroup (coalesced reads), the four wavefronts
+// take every fourth block, in a fixed order (reproducible).  yl entries are
+// ldy apart per right-hand side.
 // half = 0 or 1: only that column's sums, written at yl[r * ldy + i].
 // Up to 8 independent reductions per launch (blockIdx.z).
 template <typename T>
 __global__ __launch_bounds__(256) void k_colsum(ColsumBatchT<T> b)
 {
+    __shared__ T         red[4][64];
     const ColsumJobT<T>& J    = b.j[blockIdx.z];
     const int            wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int            o    = blockIdx.x * 4 + wave;  // output index
-    if (o >= (J.half < 0 ? 2 : 1) * J.nx) return;
-    const long long r   = blockIdx.y;
-    const T*        src = J.cp + r * (long long)J.nblk * 2 * J.nx + o + (J.half > 0 ? J.nx : 0);
-    T               acc = T(0);
-    for (int q = lane; q < J.nblk; q += 64) acc += src[(long long)q * 2 * J.nx];
-    acc = wave_reduce_sum(acc);
-    if (lane == 0) J.yl[r * J.ldy + o] = acc;
+    const int            o    = blockIdx.x * 64 + lane;  // output index
+    const bool           ok   = o < (J.half < 0 ? 2 : 1) * J.nx;
+    const long long      r    = blockIdx.y;
+    T                    acc  = T(0);
+    if (ok) {
+        const T* src = J.cp + r * (long long)J.nblk * 2 * J.nx + o + (J.half > 0 ? J.nx : 0);
+#pragma unroll 8
+        for (int q = wave; q < J.nblk; q += 4) acc += src[(long long)q * 2 * J.nx];
+    }
+    red[wave][lane] = acc;
+    __syncthreads();
+    if (wave == 0 && ok) J.yl[r * J.ldy + o] = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
 }
 template <typename T>
 void launch_colsum_batch(const ColsumBatchT<T>& b, int nrhs, hipStream_t s)
@@ -352,7 +358,7 @@ void launch_colsum_batch(const ColsumBatchT<T>& b, int nrhs, hipStream_t s)
     if (b.n <= 0) return;
     int nout = 0;
     for (int q = 0; q < b.n; q++) nout = std::max(nout, (b.j[q].half < 0 ? 2 : 1) * b.j[q].nx);
-    hipLaunchKernelGGL((k_colsum<T>), dim3((nout + 3) / 4, nrhs, b.n), dim3(256), 0, s, b);
+    hipLaunchKernelGGL((k_colsum<T>), dim3((nout + 63) / 64, nrhs, b.n), dim3(256), 0, s, b);
     FP_HIP(hipGetLastError());
 }
 template void launch_colsum_batch<double>(const ColsumBatchT<double>&, int, hipStream_t);
@@ -404,28 +410,37 @@ template void launch_gemv_batch<float>(const GemvBatchT<float>&, int, hipStream_
 // Restriction of a line-sourced solution to the interface unknowns: t[k] is
 // the x-line coefficient (rows 0 / nx-1 in xu) or the physical y-line value
 // of unknown k, the latter the total of the sweep's per-workgroup partial
-// column sums (cp, nblk blocks of 2 nx per right-hand side), reduced here by
-// one wavefront per unknown.  mode 0: out = v - t (v == nullptr: the unit
+// column sums (cp, nblk blocks of 2 nx per right-hand side), reduced here
+// (64 unknowns per workgroup).  mode 0: out = v - t (v == nullptr: the unit
 // vector e_{k0 + r}); mode 1: out = t; mode 2: out = v + w - t.
 template <typename TL>
 __global__ __launch_bounds__(256) void k_lines_assemble(const double* v, double* out, int K, const int* kface,
                                                         const int* kpos, const TL* xu, int ny, const TL* cp, int nblk,
                                                         int nx, int k0, int mode, const double* w)
 {
-    const int k = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-    if (k >= K) return;
-    const long long r = blockIdx.y;
-    const int       f = kface[k], p = kpos[k];
-    double          t;
-    if (f == XNEG || f == XPOS) {
-        t = (double)xu[r * 2 * ny + (f == XPOS ? ny : 0) + p];
-    } else {
+    // 64 consecutive unknowns per workgroup; on y faces the four wavefronts
+    // take every fourth mode block of the column partials (coalesced reads,
+    // fixed order), one LDS step adds their sums
+    __shared__ double red[4][64];
+    const int       wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int       k    = blockIdx.x * 64 + lane;
+    const bool      ok   = k < K;
+    const long long r    = blockIdx.y;
+    const int       f = ok ? kface[k] : XNEG, p = ok ? kpos[k] : 0;
+    double          t = 0.0;
+    if (ok && (f == XNEG || f == XPOS)) {
+        if (wave == 0) t = (double)xu[r * 2 * ny + (f == XPOS ? ny : 0) + p];
+    } else if (ok) {
         const TL* src = cp + r * (long long)nblk * 2 * nx + (f == YNEG ? 0 : nx) + p;
         TL        acc = TL(0);
-        for (int q = lane; q < nblk; q += 64) acc += src[(long long)q * 2 * nx];
-        t = (double)wave_reduce_sum(acc);
+#pragma unroll 8
+        for (int q = wave; q < nblk; q += 4) acc += src[(long long)q * 2 * nx];
+        t = (double)acc;
     }
-    if (lane) return;
+    red[wave][lane] = t;
+    __syncthreads();
+    if (wave != 0 || !ok) return;
+    t = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
     if (mode == 1) {
         out[r * K + k] = t;
         return;
@@ -443,7 +458,7 @@ void launch_lines_assemble(const double* v, double* out, int K, const int* kface
                            const double* w)
 {
     if (mode == 2 && (!v || !w)) throw Error(FP_E_INVALID, "lines_assemble mode 2 needs v and w");
-    hipLaunchKernelGGL((k_lines_assemble<TL>), dim3((K + 3) / 4, nrhs), dim3(256), 0, s, v, out, K, kface, kpos, xu,
+    hipLaunchKernelGGL((k_lines_assemble<TL>), dim3((K + 63) / 64, nrhs), dim3(256), 0, s, v, out, K, kface, kpos, xu,
                        ny, cp, nblk, nx, k0, mode, w);
     FP_HIP(hipGetLastError());
 }
@@ -491,14 +506,22 @@ __global__ __launch_bounds__(256) void k_exchange(ExchangeBatch<T> b)
     const int       wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const long long r    = blockIdx.y;
     if (q < b.cs.n) {
+        // 64 consecutive outputs per workgroup (lanes read consecutive
+        // partials of one mode block), the four wavefronts take every fourth
+        // mode block, one LDS step adds their sums
+        __shared__ T red[4][64];
         const ColsumJobT<T>& J = b.cs.j[q];
-        const int            o = blk * 4 + wave;
-        if (o >= (J.half < 0 ? 2 : 1) * J.nx) return;
-        const T* src = J.cp + r * (long long)J.nblk * 2 * J.nx + o + (J.half > 0 ? J.nx : 0);
-        T        acc = T(0);
-        for (int k = lane; k < J.nblk; k += 64) acc += src[(long long)k * 2 * J.nx];
-        acc = wave_reduce_sum(acc);
-        if (lane == 0) J.yl[r * J.ldy + o] = acc;
+        const int            o = blk * 64 + lane;
+        T                    acc = T(0);
+        if (o < (J.half < 0 ? 2 : 1) * J.nx) {
+            const T* src = J.cp + r * (long long)J.nblk * 2 * J.nx + o + (J.half > 0 ? J.nx : 0);
+#pragma unroll 8
+            for (int k = wave; k < J.nblk; k += 4) acc += src[(long long)k * 2 * J.nx];
+        }
+        red[wave][lane] = acc;
+        __syncthreads();
+        if (wave == 0 && o < (J.half < 0 ? 2 : 1) * J.nx)
+            J.yl[r * J.ldy + o] = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
         return;
     }
     q -= b.cs.n;
@@ -530,7 +553,7 @@ void launch_exchange(const ColsumBatchT<T>& cs, const GemvBatchT<T>& gv, const L
     int blocks = 0, q = 0;
     for (int k = 0; k < cs.n; k++, q++) {
         b.start[q] = blocks;
-        blocks += ((cs.j[k].half < 0 ? 2 : 1) * cs.j[k].nx + 3) / 4;
+        blocks += ((cs.j[k].half < 0 ? 2 : 1) * cs.j[k].nx + 63) / 64;
     }
     if (nrhs >= kGemmMinRhs) {
         launch_gemv_batch(gv, nrhs, s);
