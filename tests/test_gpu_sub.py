@@ -339,6 +339,39 @@ def test_batched_sixty_with_periodic(ctx):
         assert rel(t.cpu().numpy(), O.sub_solve(f, h, h, bc)) < 1e-11, (f.shape, bc)
 
 
+def test_batched_tables_queued_back_to_back(ctx):
+    """Two different batches of twenty problems (descriptor tables in device
+    memory, staged through pinned host blocks that are reused once their copy
+    has run), called alternately ten times without synchronising, so that
+    uploads are queued while earlier ones are still in flight: every field
+    of the last call of each batch matches its own oracle solve."""
+    import torch
+    import focalors_poisson_amd as fpa
+    rng = np.random.default_rng(62)
+    batches = []
+    for b in range(2):
+        solvers, hosts = [], []
+        for k in range(20):
+            nx = int(rng.integers(16, 200))
+            ny = int(rng.integers(16, 200))
+            h = 1.0 / max(nx, ny)
+            bc = (D, D, D, D) if b == 0 else (N, D, D, N)
+            solvers.append(fpa.PoissonSolver2D(ctx, nx, ny, h, h, bc))
+            hosts.append((rng.standard_normal((nx, ny)) * h * h, h, bc))
+        batches.append((solvers, hosts))
+    banner("batched tables queued back to back")
+    # every call's fields on the device first, so that the calls themselves
+    # queue without a host-side copy in between
+    sets = [[to_dev(f) for f, _, _ in batches[it % 2][1]] for it in range(10)]
+    torch.cuda.synchronize()
+    for it in range(10):
+        fpa.solve_batched(ctx, batches[it % 2][0], sets[it])
+    torch.cuda.synchronize()
+    for it in range(10):
+        for t, (f, h, bc) in zip(sets[it], batches[it % 2][1]):
+            assert rel(t.cpu().numpy(), O.sub_solve(f, h, h, bc)) < 1e-11, (it, f.shape)
+
+
 @pytest.mark.parametrize("f32", [False, True])
 def test_dirichlet_4096_long_bluestein(ctx, f32):
     """4096 DD rows: L = 4097 = 17 x 241, Bluestein over F = 8640 with 24
