@@ -716,9 +716,51 @@ __device__ __forceinline__ void load_rows(const YArgs<T>& a, const RowPair<T>& r
         return;
     }
     if (kind == T_DD) {
+        // x_0 = 0, x_j = row[j-1] (j = 1..N): each thread reads the pair x_j,
+        // x_{L-j} of both rows straight from global memory and writes the
+        // half-length algorithm's inputs y_j = s_j (x_j + x_{L-j}) + (x_j -
+        // x_{L-j}) / 2, y_{L-j} = s_j (x_j + x_{L-j}) - (x_j - x_{L-j}) / 2,
+        // s_j = sin(pi j / L) (y_{L/2} = 2 s x_{L/2} for even L)
         if (threadIdx.x == 0) A[zp<PAD>(0)] = {T(0), T(0)};
-    } else if (kind == T_PP || kind == T_NN) {
-    } else {
+        constexpr int PPT = VPT / 2;
+        for (int base = 1; 2 * base <= L; base += PPT * NT) {
+            T xa[PPT], xb[PPT], ua[PPT], ub[PPT];
+#pragma unroll
+            for (int k = 0; k < PPT; k++) {
+                const int  j  = base + k * NT + threadIdx.x;
+                const bool ok = 2 * j <= L;
+                xa[k]         = ok ? ld_src(rp.ia + (j - 1)) : T(0);
+                ua[k]         = ok ? ld_src(rp.ia + (N - j)) : T(0);
+                xb[k]         = (ok && rp.hb) ? ld_src(rp.ia + N + (j - 1)) : T(0);
+                ub[k]         = (ok && rp.hb) ? ld_src(rp.ia + N + (N - j)) : T(0);
+            }
+#pragma unroll
+            for (int k = 0; k < PPT; k++) {
+                const int j = base + k * NT + threadIdx.x;
+                if (2 * j > L) break;
+                T x = xa[k], y = xb[k], u = ua[k], w = ub[k];
+                if (a.asmb) {
+                    x = assemble_at(a, rp.rhs, rp.ra, j - 1, x);
+                    u = assemble_at(a, rp.rhs, rp.ra, N - j, u);
+                    if (rp.hb) {
+                        y = assemble_at(a, rp.rhs, rp.ra + 1, j - 1, y);
+                        w = assemble_at(a, rp.rhs, rp.ra + 1, N - j, w);
+                    }
+                }
+                const T   sj = a.aux[j].x;
+                const int jm = L - j;
+                if (j == jm) {
+                    A[zp<PAD>(j)] = {T(2) * sj * x, T(2) * sj * y};
+                } else {
+                    const T sa = x + u, sb = y + w, da = x - u, db = y - w;
+                    A[zp<PAD>(j)]  = {sj * sa + T(0.5) * da, sj * sb + T(0.5) * db};
+                    A[zp<PAD>(jm)] = {sj * sa - T(0.5) * da, sj * sb - T(0.5) * db};
+                }
+            }
+        }
+        return;
+    }
+    if (kind != T_PP && kind != T_NN) {
         if (threadIdx.x == 0) A[zp<PAD>(0)] = {T(0), T(0)};
         for (int p = N + 1 + threadIdx.x; p < L; p += NT) A[zp<PAD>(p)] = {T(0), T(0)};
     }
@@ -739,9 +781,7 @@ __device__ __forceinline__ void load_rows(const YArgs<T>& a, const RowPair<T>& r
                 x = assemble_at(a, rp.rhs, rp.ra, n, x);
                 if (rp.hb) y = assemble_at(a, rp.rhs, rp.ra + 1, n, y);
             }
-            if (kind == T_DD) {
-                A[zp<PAD>(n + 1)] = {x, y};
-            } else if (kind == T_PP) {
+            if (kind == T_PP) {
                 A[zp<PAD>(n)] = {x, y};
             } else if (kind == T_NN) {
                 // Makhoul's reordering: even entries ascending, odd ones descending
@@ -751,22 +791,6 @@ __device__ __forceinline__ void load_rows(const YArgs<T>& a, const RowPair<T>& r
             } else {  // DN both ways, ND forward
                 const T s = sgn_pow<T>(n + 1) * a.c0;
                 A[zp<PAD>(n + 1)]  = {s * x, s * y};
-            }
-        }
-    }
-    if (kind == T_DD) {
-        // y_j, y_{L-j} from x_j, x_{L-j}; each pair is one thread's
-        __syncthreads();
-        for (int j = 1 + threadIdx.x; 2 * j <= L; j += NT) {
-            const int   jm = L - j;
-            const T     s  = a.aux[j].x;  // sin(pi j / L)
-            const C2<T> xj = A[zp<PAD>(j)], xm = A[zp<PAD>(jm)];
-            if (j == jm) {
-                A[zp<PAD>(j)] = {T(2) * s * xj.x, T(2) * s * xj.y};
-            } else {
-                const C2<T> su = cadd(xj, xm), df = csub(xj, xm);
-                A[zp<PAD>(j)]  = {s * su.x + T(0.5) * df.x, s * su.y + T(0.5) * df.y};
-                A[zp<PAD>(jm)] = {s * su.x - T(0.5) * df.x, s * su.y - T(0.5) * df.y};
             }
         }
     }
@@ -948,11 +972,17 @@ __global__ __launch_bounds__(kMaxNT) void k_ytransform_sb(BT bt)
     C2<T>*    cs  = thi + nhi;          // cos/sin tables of the odd-prime passes (FP32)
     double*   fr  = reinterpret_cast<double*>(cs + a.ncs);  // matrix-core fragments (FP64)
     C2<T>*    wt  = reinterpret_cast<C2<T>*>(fr + a.nfrag); // DD: wavefront totals of the prefix scan
-    for (int q = threadIdx.x; q < 64 + nhi + a.ncs; q += NT) tlo[q] = a.W[q];
+    // the first table entry of each thread is loaded before the rows (the
+    // two latencies overlap; at 128 threads and up that is the whole
+    // twiddle and cosine table of a 1025-point plan) and stored once the
+    // rows are in
+    const int   ntab = 64 + nhi + a.ncs;
+    const C2<T> tv   = (int)threadIdx.x < ntab ? a.W[threadIdx.x] : C2<T>{T(0), T(0)};
+    load_rows<kZpad>(a, rp, Z);
+    if ((int)threadIdx.x < ntab) tlo[threadIdx.x] = tv;
+    for (int q = threadIdx.x + NT; q < ntab; q += NT) tlo[q] = a.W[q];
     for (int q = threadIdx.x; q < a.nfrag; q += NT) fr[q] = a.frag[q];
     const Tw<T> W{tlo, thi};
-
-    load_rows<kZpad>(a, rp, Z);
     __syncthreads();
     // every kind runs the forward FFT (the inverses by conjugate symmetry or
     // index reversal)
