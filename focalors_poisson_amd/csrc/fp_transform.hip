@@ -297,7 +297,8 @@ __host__ __device__ __forceinline__ int odd_slots(int nb)
 // multiple of 64 (or to 16 / 32 lanes for few butterflies, odd_slots), so a
 // wavefront holds one to four groups.  The cosines come from the R values
 // (cos, sin)(2 pi m / R) staged in LDS with the twiddles, at m = q r mod R
-// kept by one addition per read (one to four addresses per wavefront).  The
+// kept by an addition and an unsigned minimum per read (one to four
+// addresses per wavefront).  The
 // item a thread owns keeps its accumulators in registers across the
 // synchronisation.
 template <bool FWD, typename T>
@@ -327,7 +328,7 @@ __device__ __forceinline__ void pass_odd_sb(C2<T>* __restrict__ Z, const Tw<T> W
                 bx[it][q] = T(0);
                 by[it][q] = T(0);
             }
-            int m[kNQ];  // (QB + q) r mod R
+            unsigned m[kNQ];  // (QB + q) r mod R
 #pragma unroll
             for (int q = 0; q < kNQ; q++) m[q] = 0;
             for (int r = 1; r <= H; r++) {
@@ -340,9 +341,11 @@ __device__ __forceinline__ void pass_odd_sb(C2<T>* __restrict__ Z, const Tw<T> W
                 x0[it] = cadd(x0[it], a);
 #pragma unroll
                 for (int q = 0; q < kNQ; q++) {
-                    m[q] += QB + q;  // QB + q < R
-                    if (m[q] >= R) m[q] -= R;
-                    const C2<T> w = csg[m[q]];
+                    // m + QB + q < 2R: the unsigned difference wraps to a
+                    // large value unless m + QB + q >= R
+                    const unsigned t = m[q] + (unsigned)(QB + q);
+                    m[q]             = min(t, t - (unsigned)R);
+                    const C2<T> w    = csg[m[q]];
                     ax[it][q] += a.x * w.x;
                     ay[it][q] += a.y * w.x;
                     bx[it][q] += b.x * w.y;
