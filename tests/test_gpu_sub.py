@@ -194,6 +194,28 @@ def test_transforms_match_reference_coefficients(ctx, bcp, ny):
         assert np.abs(t.cpu().numpy() - ref).max() < 1e-12 * max(1.0, np.abs(ref).max())
 
 
+@pytest.mark.parametrize("bcp,ny", [(b, n) for b in [(D, D), (N, N), (D, N), (N, D)]
+                                     for n in [1, 2, 3, 7, 33, 127, 255, 257, 511, 1023, 1025]])
+def test_transform_coefficients_edge_lengths(ctx, bcp, ny):
+    """The same coefficient check at lengths 1-3 and odd lengths around powers
+    of two: the DCT pair's even/odd reordering with odd N, the DST-I pair
+    loads with even and odd L = N + 1 (the middle entry of even L), and the
+    lengths whose FFT goes to Bluestein (L = 257, 2 x 257 + 1, ...)."""
+    import torch
+    import focalors_poisson_amd as fpa
+    rng = np.random.default_rng(1000 + ny)
+    nx = 5
+    f = rng.standard_normal((nx, ny))
+    s = fpa.PoissonSolver2D(ctx, nx, ny, 1.0, 1.0, (D, D) + bcp)
+    for inv in (False, True):
+        t = to_dev(f)
+        banner("transform edge bc=%s ny=%d inverse=%d" % (bcp, ny, inv))
+        s.transform(t, inverse=inv)
+        torch.cuda.synchronize()
+        ref = O.transform(bcp[0], bcp[1], f, inverse=inv)
+        assert np.abs(t.cpu().numpy() - ref).max() < 1e-12 * max(1.0, np.abs(ref).max()), (bcp, ny, inv)
+
+
 @pytest.mark.parametrize("nx,ny,bc", [(64, 64, (D, D, D, D)), (200, 131, (N, D, A, N)), (512, 1024, (D, A, N, N)),
                                       (1024, 1024, (A, A, A, A))])
 def test_fp32_refine_reaches_fp64_oracle(ctx, nx, ny, bc):
