@@ -155,7 +155,7 @@ struct TransformPlan {
     DevBuf<T>  blu;       // Bluestein: chirp (L), then the DFT of its conjugate / F (F)
     T          c0 = 0, c1 = 0;
     int        nt = 256;  // threads per workgroup (tuned)
-    int        odd = 0;   // odd-prime pass form: 0 none, 1 matrix cores (FP64), 2 VALU pair sums (tuned)
+    int        odd = 0;   // odd-prime pass form: 0 none, 1 matrix cores (FP64), 2 / 3 VALU pair sums with 5 / 3 pairs per item (tuned)
     int        nt1 = 0, odd1 = -1;  // the same for launches of one field (tuned; 0 / -1: nt, odd)
     int        ept = 16;  // values per thread of the register butterflies (24 for long Bluestein lengths)
     void build(int kind, int N);

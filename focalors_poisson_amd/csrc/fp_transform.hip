@@ -38,6 +38,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdlib>
 #include <complex>
 #include <map>
 #include <mutex>
@@ -194,7 +195,8 @@ __device__ __forceinline__ void bfly(const C2<T> (&v)[R], C2<T> (&o)[R])
 constexpr int kZpad = FP_ZPAD;
 // Form of the odd-prime (7..47) passes, a kernel template parameter chosen
 // per plan: none in the radix schedule, matrix cores (FP64), VALU pair sums.
-enum OddForm : int { ODD_NONE = 0, ODD_MFMA = 1, ODD_VALU = 2 };
+// ODD_VALU / ODD_VALU3: VALU pair sums, kNQ / 3 output pairs per work item
+enum OddForm : int { ODD_NONE = 0, ODD_MFMA = 1, ODD_VALU = 2, ODD_VALU3 = 3 };
 template <int PAD>
 __device__ __forceinline__ int zp(int i)
 {
@@ -212,7 +214,8 @@ constexpr int sb_bpt()
 {
     return (EPT + R - 1) / R;
 }
-// odd-prime butterflies: output pairs per thread group, groups per thread
+// odd-prime butterflies: output pairs per work item (VALU form; the VALU3
+// form takes 3, more items of shorter latency), items per thread
 constexpr int kNQ  = 5;
 constexpr int kIPT = 1;
 constexpr int kMaxOddPrime = 47;
@@ -301,15 +304,15 @@ __host__ __device__ __forceinline__ int odd_slots(int nb)
 // addresses per wavefront).  The
 // item a thread owns keeps its accumulators in registers across the
 // synchronisation.
-template <bool FWD, typename T>
+template <bool FWD, int NQ, typename T>
 __device__ __forceinline__ void pass_odd_sb(C2<T>* __restrict__ Z, const Tw<T> W, int L, int Ns, int R,
                                             const C2<T>* __restrict__ csg)
 {
     pass_lengths(L, Ns, R);
     const int NT = blockDim.x;
-    const int H = (R - 1) >> 1, G = (H + kNQ - 1) / kNQ;
+    const int H = (R - 1) >> 1, G = (H + NQ - 1) / NQ;
     const int nb = L / R, stride = L / (Ns * R), nbp = odd_slots(nb);
-    T         ax[kIPT][kNQ], ay[kIPT][kNQ], bx[kIPT][kNQ], by[kIPT][kNQ];
+    T         ax[kIPT][NQ], ay[kIPT][NQ], bx[kIPT][NQ], by[kIPT][NQ];
     C2<T>     x0[kIPT];
 #pragma unroll
     for (int it = 0; it < kIPT; it++) {
@@ -317,20 +320,20 @@ __device__ __forceinline__ void pass_odd_sb(C2<T>* __restrict__ Z, const Tw<T> W
         const int grp = nbp >= 64 ? __builtin_amdgcn_readfirstlane(t / nbp) : t / nbp;
         const int jb  = t - grp * nbp;
         if (grp < G && jb < nb) {
-            const int   QB = 1 + grp * kNQ;
+            const int   QB = 1 + grp * NQ;
             const int   k  = jb % Ns;
             const C2<T> v0 = Z[zp<kZpad>(jb)];
             x0[it]         = v0;
 #pragma unroll
-            for (int q = 0; q < kNQ; q++) {
+            for (int q = 0; q < NQ; q++) {
                 ax[it][q] = v0.x;
                 ay[it][q] = v0.y;
                 bx[it][q] = T(0);
                 by[it][q] = T(0);
             }
-            unsigned m[kNQ];  // (QB + q) r mod R
+            unsigned m[NQ];  // (QB + q) r mod R
 #pragma unroll
-            for (int q = 0; q < kNQ; q++) m[q] = 0;
+            for (int q = 0; q < NQ; q++) m[q] = 0;
             for (int r = 1; r <= H; r++) {
                 C2<T> x1 = Z[zp<kZpad>(jb + r * nb)], x2 = Z[zp<kZpad>(jb + (R - r) * nb)];
                 if (Ns > 1) {
@@ -340,7 +343,7 @@ __device__ __forceinline__ void pass_odd_sb(C2<T>* __restrict__ Z, const Tw<T> W
                 const C2<T> a = cadd(x1, x2), b = csub(x1, x2);
                 x0[it] = cadd(x0[it], a);
 #pragma unroll
-                for (int q = 0; q < kNQ; q++) {
+                for (int q = 0; q < NQ; q++) {
                     // m + QB + q < 2R: the unsigned difference wraps to a
                     // large value unless m + QB + q >= R
                     const unsigned t = m[q] + (unsigned)(QB + q);
@@ -360,12 +363,12 @@ __device__ __forceinline__ void pass_odd_sb(C2<T>* __restrict__ Z, const Tw<T> W
         const int t = threadIdx.x + it * NT;
         const int grp = t / nbp, jb = t - grp * nbp;
         if (grp < G && jb < nb) {
-            const int QB   = 1 + grp * kNQ;
+            const int QB   = 1 + grp * NQ;
             const int k    = jb % Ns;
             const int base = (jb / Ns) * Ns * R + k;
             if (grp == 0) Z[zp<kZpad>(base)] = x0[it];
 #pragma unroll
-            for (int q = 0; q < kNQ; q++) {
+            for (int q = 0; q < NQ; q++) {
                 if (QB + q > H) break;
                 const C2<T> xq  = FWD ? C2<T>{ax[it][q] + by[it][q], ay[it][q] - bx[it][q]}
                                       : C2<T>{ax[it][q] - by[it][q], ay[it][q] + bx[it][q]};
@@ -498,8 +501,8 @@ __device__ __forceinline__ void run_fft_sb(const YArgs<T>& a, const Tw<T> W, C2<
         else if constexpr (OM == ODD_MFMA && sizeof(T) == 8) {
             pass_odd_mfma<FWD>(Z, W, a.F, Ns, R, fragL + fo);
             fo += 2 * 2 * mfma_ksteps(R) * 64;
-        } else if constexpr (OM == ODD_VALU) {
-            pass_odd_sb<FWD>(Z, W, a.F, Ns, R, cs + co);
+        } else if constexpr (OM == ODD_VALU || OM == ODD_VALU3) {
+            pass_odd_sb<FWD, OM == ODD_VALU ? kNQ : 3>(Z, W, a.F, Ns, R, cs + co);
             co += R;
         }
         Ns *= R;
@@ -1107,7 +1110,8 @@ static bool sb_fits(const int* rad, int nrad, int L, int NT, int odd, int ept = 
         } else if (mfma) {
             if ((2 * nb + 15) / 16 > kMaxNTw * (NT / 64)) return false;
         } else {
-            const int H = (R - 1) / 2, G = (H + kNQ - 1) / kNQ, nbp = odd_slots(nb);
+            const int nq = odd == ODD_VALU3 ? 3 : kNQ;
+            const int H = (R - 1) / 2, G = (H + nq - 1) / nq, nbp = odd_slots(nb);
             if ((nbp * G + NT - 1) / NT > kIPT) return false;
         }
     }
@@ -1119,14 +1123,14 @@ template <typename T>
 static std::vector<int> odd_forms(bool has_odd)
 {
     if (!has_odd) return {ODD_NONE};
-    if (sizeof(T) == 8) return {ODD_MFMA, ODD_VALU};
-    return {ODD_VALU};
+    if (sizeof(T) == 8) return {ODD_MFMA, ODD_VALU, ODD_VALU3};
+    return {ODD_VALU, ODD_VALU3};
 }
 
 template <typename T>
 static size_t lds_bytes(const TransformPlan<T>& tp, int odd)
 {
-    const size_t ncs   = odd == ODD_VALU ? (size_t)tp.ncs : 0;
+    const size_t ncs   = (odd == ODD_VALU || odd == ODD_VALU3) ? (size_t)tp.ncs : 0;
     const size_t nfrag = odd == ODD_MFMA ? tp.frag.n : 0;
     const size_t nhi = tp.F / 64 + 1;
     const size_t zl  = (size_t)tp.F + (kZpad ? ((size_t)tp.F >> kZpad) : 0) + 1;
@@ -1340,7 +1344,7 @@ static YArgs<T> yargs(const TransformJob<T>& j, int inverse)
     a.nrad    = tp.nrad;
     for (int i = 0; i < tp.nrad; i++) a.rad[i] = tp.rad[i];
     a.W   = reinterpret_cast<const C2<T>*>(tp.W.p);
-    a.ncs   = (tp.engine == 1 && form == ODD_VALU) ? tp.ncs : 0;
+    a.ncs   = (tp.engine == 1 && (form == ODD_VALU || form == ODD_VALU3)) ? tp.ncs : 0;
     a.frag  = tp.frag.p;
     a.nfrag = (tp.engine == 1 && form == ODD_MFMA) ? (int)tp.frag.n : 0;
     a.aux = reinterpret_cast<const C2<T>*>(tp.aux.p);
@@ -1365,6 +1369,7 @@ static void launch_kernel(int engine, int odd, int ept, int blocks, int NT, size
                      : ept != kEPT       ? (const void*)k_ytransform_sb<T, ODD_NONE, B, kEPTLong>
                      : odd == ODD_NONE   ? (const void*)k_ytransform_sb<T, ODD_NONE, B, kEPT>
                      : odd == ODD_MFMA   ? (const void*)k_ytransform_sb<T, ODD_MFMA, B, kEPT>
+                     : odd == ODD_VALU3  ? (const void*)k_ytransform_sb<T, ODD_VALU3, B, kEPT>
                                          : (const void*)k_ytransform_sb<T, ODD_VALU, B, kEPT>;
     // one attribute flag per kernel instantiation
     static std::mutex            mu;
@@ -1383,6 +1388,8 @@ static void launch_kernel(int engine, int odd, int ept, int blocks, int NT, size
         hipLaunchKernelGGL((k_ytransform_sb<T, ODD_NONE, B, kEPT>), dim3(blocks), dim3(NT), lds, s, bt);
     else if (odd == ODD_MFMA)
         hipLaunchKernelGGL((k_ytransform_sb<T, ODD_MFMA, B, kEPT>), dim3(blocks), dim3(NT), lds, s, bt);
+    else if (odd == ODD_VALU3)
+        hipLaunchKernelGGL((k_ytransform_sb<T, ODD_VALU3, B, kEPT>), dim3(blocks), dim3(NT), lds, s, bt);
     else hipLaunchKernelGGL((k_ytransform_sb<T, ODD_VALU, B, kEPT>), dim3(blocks), dim3(NT), lds, s, bt);
     FP_HIP(hipGetLastError());
 }
@@ -1485,6 +1492,17 @@ void tune_transform(TransformPlan<T>& tp, int nx, bool measure)
     for (int form : odd_forms<T>(has_odd))
         for (int t : {128, 256, 512})
             if (sb_fits(tp.rad, tp.nrad, tp.F, t, form, tp.ept) && lds_bytes(tp, form) <= 160 * 1024) cand.push_back({t, form});
+    // FP_ODD_FORM=<form> (tests): that odd-prime form, at the smallest
+    // thread count it fits, for every launch, without timing
+    if (const char* env = std::getenv("FP_ODD_FORM")) {
+        const int want = std::atoi(env);
+        for (const auto& c : cand)
+            if (c[1] == want) {
+                tp.nt = tp.nt1 = c[0];
+                tp.odd = tp.odd1 = want;
+                return;
+            }
+    }
     if (!measure || cand.size() < 2 || (long long)nx * tp.N < (1LL << 16)) return;
     const std::string key = "yt " + std::to_string(tp.kind) + " " + std::to_string(tp.N) + " " +
                             std::to_string(nx) + " " + std::to_string(sizeof(T));
@@ -1561,7 +1579,7 @@ std::array<int, 2> tune_batch_shape(std::vector<TransformJob<T>> jobs, hipStream
     }
     std::vector<std::array<int, 2>> cands;
     for (int nt : {0, 128, 256, 512})
-        for (int form : {-1, (int)ODD_MFMA, (int)ODD_VALU})
+        for (int form : {-1, (int)ODD_MFMA, (int)ODD_VALU, (int)ODD_VALU3})
             if (!(form == ODD_MFMA && sizeof(T) != 8)) cands.push_back({nt, form});
     hipEvent_t e0, e1;
     FP_HIP(hipEventCreate(&e0));

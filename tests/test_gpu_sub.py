@@ -216,6 +216,53 @@ def test_transform_coefficients_edge_lengths(ctx, bcp, ny):
         assert np.abs(t.cpu().numpy() - ref).max() < 1e-12 * max(1.0, np.abs(ref).max()), (bcp, ny, inv)
 
 
+@pytest.mark.parametrize("form", [1, 2, 3])
+@pytest.mark.parametrize("bcp,ny", [((D, D), 1024), ((D, N), 512), ((N, D), 512), ((D, D), 1022), ((N, N), 1025),
+                                    ((D, D), 40)])
+def test_odd_prime_pass_forms(ctx, monkeypatch, form, bcp, ny):
+    """Each odd-prime pass form, forced through FP_ODD_FORM (1: matrix
+    cores, 2 / 3: VALU pair sums with 5 / 3 output pairs per work item), on
+    lengths with prime factors 41 (L = 1025, 41), 31 and 11 (L = 1023, the
+    second pass twiddled): coefficients against the oracle both ways, and a
+    solve."""
+    import torch
+    import focalors_poisson_amd as fpa
+    monkeypatch.setenv("FP_ODD_FORM", str(form))
+    rng = np.random.default_rng(7 * ny + form)
+    nx = 6
+    f = rng.standard_normal((nx, ny))
+    s = fpa.PoissonSolver2D(ctx, nx, ny, 1.0, 1.0, (D, D) + bcp)
+    for inv in (False, True):
+        t = to_dev(f)
+        banner("odd form %d bc=%s ny=%d inverse=%d" % (form, bcp, ny, inv))
+        s.transform(t, inverse=inv)
+        torch.cuda.synchronize()
+        ref = O.transform(bcp[0], bcp[1], f, inverse=inv)
+        assert np.abs(t.cpu().numpy() - ref).max() < 1e-12 * max(1.0, np.abs(ref).max()), (form, bcp, ny, inv)
+    h = 1.0 / ny
+    g = f * h * h
+    assert rel(gpu_solve(ctx, g, h, h, (D, D) + bcp), O.sub_solve(g, h, h, (D, D) + bcp)) < 1e-12
+
+
+@pytest.mark.parametrize("form", [2, 3])
+def test_odd_prime_forms_fp32_refine(ctx, monkeypatch, form):
+    """The FP32 transforms of the refinement path in both VALU forms (the
+    matrix-core form is FP64 only): 1024 DD rows (L = 1025)."""
+    import torch
+    import focalors_poisson_amd as fpa
+    monkeypatch.setenv("FP_ODD_FORM", str(form))
+    rng = np.random.default_rng(90 + form)
+    nx, ny = 48, 1024
+    h = 1.0 / ny
+    f = rng.standard_normal((nx, ny)) * h * h
+    s = fpa.PoissonSolver2D(ctx, nx, ny, h, h, (D, D, D, D))
+    t = to_dev(f)
+    banner("odd form %d fp32 refine" % form)
+    s.solve_refine(t, tol=1e-12, max_steps=10)
+    torch.cuda.synchronize()
+    assert rel(t.cpu().numpy(), O.sub_solve(f, h, h, (D, D, D, D))) < 1e-10
+
+
 @pytest.mark.parametrize("nx,ny,bc", [(64, 64, (D, D, D, D)), (200, 131, (N, D, A, N)), (512, 1024, (D, A, N, N)),
                                       (1024, 1024, (A, A, A, A))])
 def test_fp32_refine_reaches_fp64_oracle(ctx, nx, ny, bc):
