@@ -23,6 +23,8 @@
 // kernel therefore moves one read and one write of the field and nothing
 // else of its size.  The singular zero mode (x_0 = 0,
 // x_{i+1} = f_i - x_{i-1} + 2 x_i) is two prefix sums and uses the same chain.
+#include <cstdlib>
+#include <cstdio>
 #include "fp_internal.h"
 
 #include <algorithm>
@@ -869,6 +871,18 @@ struct TriTune {
 // right-hand side).  Choices are recorded per (nx, ny) (tune_store).
 void SubPlan::tune_tridiag(bool measure)
 {
+    // FP_TRI_SHAPE=<chunk>,<modes per workgroup> (tests, measurements): that
+    // shape wherever it fits, without timing
+    if (const char* env = std::getenv("FP_TRI_SHAPE")) {
+        int ch = 0, mb = 0;
+        if (std::sscanf(env, "%d,%d", &ch, &mb) == 2)
+            for (int c : chunk_candidates(nx))
+                if (c == ch && mb > 0 && tri_mb((nx + ch - 1) / ch, ch, mb) == mb) {
+                    set_chunks(ch);
+                    mb_field = mb;
+                    return;
+                }
+    }
     if (!measure || (long long)nx * ny < (1LL << 16)) {
         set_chunks(default_chunk(nx));
         mb_field = 4;

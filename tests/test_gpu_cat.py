@@ -235,3 +235,18 @@ def test_periodic_x_root_with_child(ctx):
     assert rel(us, discrete.solve(doms, conn, fs)) < 1e-10, info
     ref, _, _ = O.Composite(doms, conn, tol=1e-14, tol_is_relative=True, maxiter=400).solve(fs)
     assert rel(us, ref) < 1e-10, info
+
+
+@pytest.mark.parametrize("shape", ["8,1", "8,4", "16,16", "32,8"])
+def test_cross_sweep_shapes(ctx, monkeypatch, shape):
+    """The composite sweep at forced chunk lengths and modes per workgroup
+    (FP_TRI_SHAPE): the same solution as the oracle at each shape."""
+    from focalors_poisson_amd.cases import cross
+    monkeypatch.setenv("FP_TRI_SHAPE", shape)
+    doms, conn, fields, _ = cross(128, seed=5)
+    fs = [f[0] for f in fields]
+    banner("cross sweep shape %s" % shape)
+    us, info, _ = run(ctx, doms, conn, fs)
+    ref, _, _ = O.Composite(doms, conn, tol=1e-14, tol_is_relative=True, maxiter=400,
+                            schur_mode=O.SCHUR_SPECTRAL).solve(fs)
+    assert rel(us, ref) < 1e-10, (shape, info)
