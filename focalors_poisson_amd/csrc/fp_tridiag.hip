@@ -492,14 +492,6 @@ struct SweepBatch {
     SweepProb<T> p[kMaxBatch];
 };
 
-template <typename T>
-__device__ __forceinline__ T line_rhs(const T* lb, int nx, int ny, int i, int j, T p0, T p1)
-{
-    T v = p0 * lb[2 * ny + i] + p1 * lb[2 * ny + nx + i];
-    if (i == 0) v += lb[j];
-    if (i == nx - 1) v += lb[ny + j];
-    return v;
-}
 
 template <typename T, int CH>
 __global__ __launch_bounds__(tri_threads<CH>()) void k_sweep(SweepBatch<T> bt)
@@ -534,11 +526,37 @@ __global__ __launch_bounds__(tri_threads<CH>()) void k_sweep(SweepBatch<T> bt)
     for (int k = 0; k < CH; k++) {
         const int  i  = i0 + k;
         const bool ok = act && i < nx;
-        T          v  = (ok && pr.fin) ? f[(long long)i * ny + j] : T(0);
-        if (ok && lp) v += line_rhs(lp, nx, ny, i, j, p0, p1);
-        if (ok && lm) v -= line_rhs(lm, nx, ny, i, j, p0, p1);
-        fv[k] = v;
+        fv[k]         = (ok && pr.fin) ? f[(long long)i * ny + j] : T(0);
     }
+    // line sources lb = [x0 (ny) | x1 (ny) | y0 (nx) | y1 (nx)]: the source of
+    // mode j at row i is p0 y0[i] + p1 y1[i] (p = phi of the mode on the y
+    // faces), plus x0[j] at row 0 and x1[j] at row nx-1.  The y-line terms
+    // depend on the row only, so a thread reads its CH consecutive rows of
+    // each y line in one run
+    auto add_lines = [&](const T* lb, T sg) {
+        const T* y0 = lb + 2 * ny + i0;
+        const T* y1 = y0 + nx;
+        if (act && i0 + CH <= nx) {
+            T a0[CH], a1[CH];
+#pragma unroll
+            for (int k = 0; k < CH; k++) {
+                a0[k] = y0[k];
+                a1[k] = y1[k];
+            }
+#pragma unroll
+            for (int k = 0; k < CH; k++) fv[k] += sg * (p0 * a0[k] + p1 * a1[k]);
+        } else if (act) {
+#pragma unroll
+            for (int k = 0; k < CH; k++)
+                if (i0 + k < nx) fv[k] += sg * (p0 * y0[k] + p1 * y1[k]);
+        }
+        if (act && i0 == 0) fv[0] += sg * lb[j];
+#pragma unroll
+        for (int k = 0; k < CH; k++)
+            if (act && i0 + k == nx - 1) fv[k] += sg * lb[ny + j];
+    };
+    if (lp) add_lines(lp, T(1));
+    if (lm) add_lines(lm, T(-1));
     {
         const double d   = act ? pr.xd[j] : 0.0;
         const double pp0 = act ? pr.pst[(long long)j * P + c] : 0.0;
