@@ -306,10 +306,17 @@ __global__ __launch_bounds__(256) void k_schur_lines(SchurLinesArgs a)
     const long long   r  = blockIdx.y;
     const double*     Si = C.S + (long long)i * C.cn;
     const double*     xr = a.v + r * a.ldv + C.voff;
-    double            acc = 0.0;
-#pragma unroll 4
-    for (int j = lane; j < C.cn; j += 64) acc += Si[j] * xr[j];
-    acc = wave_reduce_sum(acc);
+    // four independent sums: four row loads in flight per wavefront
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int    j  = lane;
+    for (; j + 192 < C.cn; j += 256) {
+        a0 += Si[j] * xr[j];
+        a1 += Si[j + 64] * xr[j + 64];
+        a2 += Si[j + 128] * xr[j + 128];
+        a3 += Si[j + 192] * xr[j + 192];
+    }
+    for (; j < C.cn; j += 64) a0 += Si[j] * xr[j];
+    const double acc = wave_reduce_sum((a0 + a1) + (a2 + a3));
     if (lane == 0) a.lb[r * a.ldl + C.lboff + i] = acc;
 }
 void launch_schur_lines(const SchurLinesArgs& a, int nrhs, hipStream_t s)
@@ -529,11 +536,19 @@ __global__ __launch_bounds__(256) void k_exchange(ExchangeBatch<T> b)
         const GemvJobT<T>& J = b.gv.j[q];
         const int          i = blk * 4 + wave;
         if (i >= J.n) return;
-        const T* Ai  = J.A + (long long)i * J.n;
-        const T* xr  = J.x + r * J.ldx;
-        T        acc = T(0);
-        for (int k = lane; k < J.n; k += 64) acc += Ai[k] * xr[k];
-        acc = wave_reduce_sum(acc);
+        const T* Ai = J.A + (long long)i * J.n;
+        const T* xr = J.x + r * J.ldx;
+        // four independent sums: four row loads in flight per wavefront
+        T   a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
+        int k  = lane;
+        for (; k + 192 < J.n; k += 256) {
+            a0 += Ai[k] * xr[k];
+            a1 += Ai[k + 64] * xr[k + 64];
+            a2 += Ai[k + 128] * xr[k + 128];
+            a3 += Ai[k + 192] * xr[k + 192];
+        }
+        for (; k < J.n; k += 64) a0 += Ai[k] * xr[k];
+        const T acc = wave_reduce_sum((a0 + a1) + (a2 + a3));
         if (lane == 0) J.y[r * J.ldy + i] = acc;
         return;
     }
