@@ -333,31 +333,48 @@ void launch_schur_lines(const SchurLinesArgs& a, int nrhs, hipStream_t s)
     FP_HIP(hipGetLastError());
 }
 
+// Reductions over the column partials of the line-sourced sweeps: kRedU
+// consecutive outputs per 256-thread workgroup (one 128-byte line per
+// output run), kRedS slices of the mode blocks, summed in a fixed order.
+constexpr int kRedU = 16, kRedS = 256 / kRedU;
+template <typename T>
+__device__ __forceinline__ T red_slices(const T (*red)[kRedU], int u)
+{
+    T v[kRedS];
+#pragma unroll
+    for (int q = 0; q < kRedS; q++) v[q] = red[q][u];
+#pragma unroll
+    for (int w = 1; w < kRedS; w *= 2)
+#pragma unroll
+        for (int q = 0; q + w < kRedS; q += 2 * w) v[q] += v[q + w];
+    return v[0];
+}
+
 // Column sums of the line-sourced sweeps: yl[r][w*nx + i] (w = 0, 1) =
-// sum over the mode blocks b of cp[(r*nblk + b)*2nx + w*nx + i]: 64
-// consecutive outputs per workgroup (coalesced reads), the four wavefronts
-// take every fourth block, in a fixed order (reproducible).  yl entries are
-// ldy apart per right-hand side.
+// sum over the mode blocks b of cp[(r*nblk + b)*2nx + w*nx + i]: kRedU
+// consecutive outputs per workgroup (coalesced reads), kRedS slices of the
+// blocks, in a fixed order (reproducible).  yl entries are ldy apart per
+// right-hand side.
 // half = 0 or 1: only that column's sums, written at yl[r * ldy + i].
 // Up to 8 independent reductions per launch (blockIdx.z).
 template <typename T>
 __global__ __launch_bounds__(256) void k_colsum(ColsumBatchT<T> b)
 {
-    __shared__ T         red[4][64];
-    const ColsumJobT<T>& J    = b.j[blockIdx.z];
-    const int            wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int            o    = blockIdx.x * 64 + lane;  // output index
-    const bool           ok   = o < (J.half < 0 ? 2 : 1) * J.nx;
-    const long long      r    = blockIdx.y;
-    T                    acc  = T(0);
+    __shared__ T         red[kRedS][kRedU];
+    const ColsumJobT<T>& J  = b.j[blockIdx.z];
+    const int            u  = threadIdx.x % kRedU, sl = threadIdx.x / kRedU;
+    const int            o  = blockIdx.x * kRedU + u;  // output index
+    const bool           ok = o < (J.half < 0 ? 2 : 1) * J.nx;
+    const long long      r  = blockIdx.y;
+    T                    acc = T(0);
     if (ok) {
         const T* src = J.cp + r * (long long)J.nblk * 2 * J.nx + o + (J.half > 0 ? J.nx : 0);
-#pragma unroll 8
-        for (int q = wave; q < J.nblk; q += 4) acc += src[(long long)q * 2 * J.nx];
+#pragma unroll 4
+        for (int q = sl; q < J.nblk; q += kRedS) acc += src[(long long)q * 2 * J.nx];
     }
-    red[wave][lane] = acc;
+    red[sl][u] = acc;
     __syncthreads();
-    if (wave == 0 && ok) J.yl[r * J.ldy + o] = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+    if (sl == 0 && ok) J.yl[r * J.ldy + o] = red_slices<T>(red, u);
 }
 template <typename T>
 void launch_colsum_batch(const ColsumBatchT<T>& b, int nrhs, hipStream_t s)
@@ -365,7 +382,7 @@ void launch_colsum_batch(const ColsumBatchT<T>& b, int nrhs, hipStream_t s)
     if (b.n <= 0) return;
     int nout = 0;
     for (int q = 0; q < b.n; q++) nout = std::max(nout, (b.j[q].half < 0 ? 2 : 1) * b.j[q].nx);
-    hipLaunchKernelGGL((k_colsum<T>), dim3((nout + 63) / 64, nrhs, b.n), dim3(256), 0, s, b);
+    hipLaunchKernelGGL((k_colsum<T>), dim3((nout + kRedU - 1) / kRedU, nrhs, b.n), dim3(256), 0, s, b);
     FP_HIP(hipGetLastError());
 }
 template void launch_colsum_batch<double>(const ColsumBatchT<double>&, int, hipStream_t);
@@ -425,29 +442,28 @@ __global__ __launch_bounds__(256) void k_lines_assemble(const double* v, double*
                                                         const int* kpos, const TL* xu, int ny, const TL* cp, int nblk,
                                                         int nx, int k0, int mode, const double* w)
 {
-    // 64 consecutive unknowns per workgroup; on y faces the four wavefronts
-    // take every fourth mode block of the column partials (coalesced reads,
-    // fixed order), one LDS step adds their sums
-    __shared__ double red[4][64];
-    const int       wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int       k    = blockIdx.x * 64 + lane;
-    const bool      ok   = k < K;
-    const long long r    = blockIdx.y;
+    // kRedU consecutive unknowns per workgroup; on y faces kRedS slices of
+    // the mode blocks of the column partials (coalesced reads, fixed order)
+    __shared__ double red[kRedS][kRedU];
+    const int       u  = threadIdx.x % kRedU, sl = threadIdx.x / kRedU;
+    const int       k  = blockIdx.x * kRedU + u;
+    const bool      ok = k < K;
+    const long long r  = blockIdx.y;
     const int       f = ok ? kface[k] : XNEG, p = ok ? kpos[k] : 0;
     double          t = 0.0;
     if (ok && (f == XNEG || f == XPOS)) {
-        if (wave == 0) t = (double)xu[r * 2 * ny + (f == XPOS ? ny : 0) + p];
+        if (sl == 0) t = (double)xu[r * 2 * ny + (f == XPOS ? ny : 0) + p];
     } else if (ok) {
         const TL* src = cp + r * (long long)nblk * 2 * nx + (f == YNEG ? 0 : nx) + p;
         TL        acc = TL(0);
-#pragma unroll 8
-        for (int q = wave; q < nblk; q += 4) acc += src[(long long)q * 2 * nx];
+#pragma unroll 4
+        for (int q = sl; q < nblk; q += kRedS) acc += src[(long long)q * 2 * nx];
         t = (double)acc;
     }
-    red[wave][lane] = t;
+    red[sl][u] = t;
     __syncthreads();
-    if (wave != 0 || !ok) return;
-    t = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+    if (sl != 0 || !ok) return;
+    t = red_slices<double>(red, u);
     if (mode == 1) {
         out[r * K + k] = t;
         return;
@@ -465,7 +481,8 @@ void launch_lines_assemble(const double* v, double* out, int K, const int* kface
                            const double* w)
 {
     if (mode == 2 && (!v || !w)) throw Error(FP_E_INVALID, "lines_assemble mode 2 needs v and w");
-    hipLaunchKernelGGL((k_lines_assemble<TL>), dim3((K + 63) / 64, nrhs), dim3(256), 0, s, v, out, K, kface, kpos, xu,
+    hipLaunchKernelGGL((k_lines_assemble<TL>), dim3((K + kRedU - 1) / kRedU, nrhs), dim3(256), 0, s, v, out, K, kface,
+                       kpos, xu,
                        ny, cp, nblk, nx, k0, mode, w);
     FP_HIP(hipGetLastError());
 }
@@ -513,22 +530,22 @@ __global__ __launch_bounds__(256) void k_exchange(ExchangeBatch<T> b)
     const int       wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const long long r    = blockIdx.y;
     if (q < b.cs.n) {
-        // 64 consecutive outputs per workgroup (lanes read consecutive
-        // partials of one mode block), the four wavefronts take every fourth
-        // mode block, one LDS step adds their sums
-        __shared__ T red[4][64];
-        const ColsumJobT<T>& J = b.cs.j[q];
-        const int            o = blk * 64 + lane;
+        // kRedU consecutive outputs per workgroup (consecutive partials of
+        // one mode block per load), kRedS slices of the mode blocks
+        __shared__ T         red[kRedS][kRedU];
+        const ColsumJobT<T>& J  = b.cs.j[q];
+        const int            u  = threadIdx.x % kRedU, sl = threadIdx.x / kRedU;
+        const int            o  = blk * kRedU + u;
+        const bool           ok = o < (J.half < 0 ? 2 : 1) * J.nx;
         T                    acc = T(0);
-        if (o < (J.half < 0 ? 2 : 1) * J.nx) {
+        if (ok) {
             const T* src = J.cp + r * (long long)J.nblk * 2 * J.nx + o + (J.half > 0 ? J.nx : 0);
-#pragma unroll 8
-            for (int k = wave; k < J.nblk; k += 4) acc += src[(long long)k * 2 * J.nx];
+#pragma unroll 4
+            for (int k = sl; k < J.nblk; k += kRedS) acc += src[(long long)k * 2 * J.nx];
         }
-        red[wave][lane] = acc;
+        red[sl][u] = acc;
         __syncthreads();
-        if (wave == 0 && o < (J.half < 0 ? 2 : 1) * J.nx)
-            J.yl[r * J.ldy + o] = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+        if (sl == 0 && ok) J.yl[r * J.ldy + o] = red_slices<T>(red, u);
         return;
     }
     q -= b.cs.n;
@@ -568,7 +585,7 @@ void launch_exchange(const ColsumBatchT<T>& cs, const GemvBatchT<T>& gv, const L
     int blocks = 0, q = 0;
     for (int k = 0; k < cs.n; k++, q++) {
         b.start[q] = blocks;
-        blocks += ((cs.j[k].half < 0 ? 2 : 1) * cs.j[k].nx + 63) / 64;
+        blocks += ((cs.j[k].half < 0 ? 2 : 1) * cs.j[k].nx + kRedU - 1) / kRedU;
     }
     if (nrhs >= kGemmMinRhs) {
         launch_gemv_batch(gv, nrhs, s);
