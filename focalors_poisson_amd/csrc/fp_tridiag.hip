@@ -652,6 +652,14 @@ static int tri_mb(int P, int ch, int mb_cap)
     return MB;
 }
 
+// modes per workgroup of a composite sweep that writes column partials: at
+// least 4 (fewer partials to write and reduce, 32-byte row segments per
+// load), whatever the plain sweep's tuned shape
+static int sweep_mb(const SubPlan& sp)
+{
+    return tri_mb(sp.tri_p, sp.tri_ch, std::max(sp.mb_field, 4));
+}
+
 template <typename T, int CH>
 static void launch_tridiag_group(const TridiagJob<T>* const* jobs, int n, hipStream_t s)
 {
@@ -778,7 +786,7 @@ static void launch_sweep_group(const SweepJob<T>* const* jobs, int n, hipStream_
         FP_REQUIRE(!sp.no_dirichlet, "the composite sweep needs a Dirichlet-like face");
         FP_REQUIRE(!sp.cyc, "the composite sweep of a periodic x axis");
         const int P  = sp.tri_p;
-        const int MB = tri_mb(P, CH, sp.mb_field);
+        const int MB = jb.cp ? sweep_mb(sp) : tri_mb(P, CH, sp.mb_field);
         SweepProb<T>& pr = bt.p[q];
         pr.f    = jb.f;
         pr.fo   = jb.fo ? jb.fo : jb.f;
@@ -834,7 +842,8 @@ template void launch_sweep_batch<float>(const SweepJob<float>*, int, hipStream_t
 
 int sweep_blocks(const SubPlan& sp)
 {
-    return (sp.ny + tri_mb(sp.tri_p, sp.tri_ch, sp.mb_field) - 1) / tri_mb(sp.tri_p, sp.tri_ch, sp.mb_field);
+    const int mb = sweep_mb(sp);
+    return (sp.ny + mb - 1) / mb;
 }
 
 // ---- chunk schedule and its autotuning ----
