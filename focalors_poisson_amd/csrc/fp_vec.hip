@@ -435,7 +435,7 @@ template void launch_gemv_batch<float>(const GemvBatchT<float>&, int, hipStream_
 // the x-line coefficient (rows 0 / nx-1 in xu) or the physical y-line value
 // of unknown k, the latter the total of the sweep's per-workgroup partial
 // column sums (cp, nblk blocks of 2 nx per right-hand side), reduced here
-// (64 unknowns per workgroup).  mode 0: out = v - t (v == nullptr: the unit
+// (kRedU unknowns per workgroup).  mode 0: out = v - t (v == nullptr: the unit
 // vector e_{k0 + r}); mode 1: out = t; mode 2: out = v + w - t.
 template <typename TL>
 __global__ __launch_bounds__(256) void k_lines_assemble(const double* v, double* out, int K, const int* kface,
