@@ -244,6 +244,34 @@ def test_odd_prime_pass_forms(ctx, monkeypatch, form, bcp, ny):
     assert rel(gpu_solve(ctx, g, h, h, (D, D) + bcp), O.sub_solve(g, h, h, (D, D) + bcp)) < 1e-12
 
 
+def test_long_prime_47_length(ctx):
+    """L = N + 1 = 3055 = 5 x 13 x 47 with 65 radix-47 butterflies: more work
+    items than the VALU odd-prime forms fit in one workgroup, so the FP64
+    transform takes the matrix-core form and the FP32 one the two-buffer
+    engine; coefficients against the oracle, and an FP32+refinement solve."""
+    import torch
+    import focalors_poisson_amd as fpa
+    rng = np.random.default_rng(3054)
+    ny = 3054
+    f = rng.standard_normal((5, ny))
+    s = fpa.PoissonSolver2D(ctx, 5, ny, 1.0, 1.0, (D, D, D, D))
+    for inv in (False, True):
+        t = to_dev(f)
+        banner("transform ny=3054 inverse=%d" % inv)
+        s.transform(t, inverse=inv)
+        torch.cuda.synchronize()
+        ref = O.transform(D, D, f, inverse=inv)
+        assert np.abs(t.cpu().numpy() - ref).max() < 1e-12 * max(1.0, np.abs(ref).max()), inv
+    h = 1.0 / ny
+    g = rng.standard_normal((8, ny)) * h * h
+    s2 = fpa.PoissonSolver2D(ctx, 8, ny, h, h, (D, D, D, D))
+    t = to_dev(g)
+    banner("fp32 refine ny=3054")
+    s2.solve_refine(t, tol=1e-12, max_steps=10)
+    torch.cuda.synchronize()
+    assert rel(t.cpu().numpy(), O.sub_solve(g, h, h, (D, D, D, D))) < 1e-10
+
+
 @pytest.mark.parametrize("form", [2, 3])
 def test_odd_prime_forms_fp32_refine(ctx, monkeypatch, form):
     """The FP32 transforms of the refinement path in both VALU forms (the
