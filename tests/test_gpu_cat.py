@@ -250,3 +250,24 @@ def test_cross_sweep_shapes(ctx, monkeypatch, shape):
     ref, _, _ = O.Composite(doms, conn, tol=1e-14, tol_is_relative=True, maxiter=400,
                             schur_mode=O.SCHUR_SPECTRAL).solve(fs)
     assert rel(us, ref) < 1e-10, (shape, info)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_tshape_unequal_children(ctx, precision):
+    """A non-square root (96 x 64) with children of different depths on both
+    x faces (40 and 56 rows) and a 48-row child on its y+ face, mixed
+    Dirichlet/Neumann outer faces: the two-level path against the oracle and
+    the assembled sparse system, FP64 and FP32+refine."""
+    rng = np.random.default_rng(77)
+    h = 1.0 / 64
+    doms = [(96, 64, h, h, (A, A, D, A)), (40, 64, h, h, (N, A, D, N)), (56, 64, h, h, (A, D, N, D)),
+            (96, 48, h, h, (D, N, A, N))]
+    conn = [(0, 0, 1), (0, 1, 2), (0, 3, 3)]
+    fs = [rng.standard_normal((d[0], d[1])) for d in doms]
+    banner("tshape precision=%d" % precision)
+    us, info, _ = run(ctx, doms, conn, fs, precision=precision)
+    ref, _, _ = O.Composite(doms, conn, tol=1e-14, tol_is_relative=True, maxiter=400,
+                            schur_mode=O.SCHUR_SPECTRAL).solve(fs)
+    assert rel(us, ref) < 1e-10, info
+    import discrete
+    assert rel(us, discrete.solve(doms, conn, fs)) < 1e-10
