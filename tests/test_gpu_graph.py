@@ -138,3 +138,49 @@ def test_graph_capture_into_replays(ctx):
         assert info["status"] == 0, info
         assert rel([o.cpu().numpy() for o in out], [e.cpu().numpy() for e in eager_out]) < 1e-13, k
         assert all(np.array_equal(r.cpu().numpy(), f) for r, f in zip(rhs, fs)), k
+
+
+def test_tshape_graph_and_multi_rhs(ctx):
+    """The T of unequal domains (non-square root, children of 40 and 56 rows
+    on the x faces, 48 on y+): graph replays with eager work in between
+    match the eager solve and the oracle, and a three-right-hand-side solve
+    matches the single ones."""
+    import torch
+    import focalors_poisson_amd as fpa
+    D, N, A = 0, 1, 3
+    h = 1.0 / 64
+    doms = [(96, 64, h, h, (A, A, D, A)), (40, 64, h, h, (N, A, D, N)), (56, 64, h, h, (A, D, N, D)),
+            (96, 48, h, h, (D, N, A, N))]
+    conn = [(0, 0, 1), (0, 1, 2), (0, 3, 3)]
+    rng = np.random.default_rng(78)
+    rhs = [[rng.standard_normal((d[0], d[1])) for d in doms] for _ in range(3)]
+    banner("tshape graph and multi rhs")
+    cat = fpa.ConcatPoissonSolver2D(ctx, doms, conn, m=40, tol=1e-12, tol_is_relative=True, max_iter=100)
+    stream = torch.cuda.Stream()
+    bound = [torch.empty((d[0], d[1]), dtype=torch.float64, device=device()) for d in doms]
+    with torch.cuda.stream(stream):
+        cat.graph_capture(bound, stream=stream)
+    singles = []
+    for k, fs in enumerate(rhs):
+        eager = [to_dev(f) for f in fs]
+        cat.solve(eager)
+        torch.cuda.synchronize()
+        with torch.cuda.stream(stream):
+            for b, f in zip(bound, fs):
+                b.copy_(torch.from_numpy(f).to(device()))
+            info = cat.graph_launch(stream=stream)
+        torch.cuda.synchronize()
+        got = [b.cpu().numpy() for b in bound]
+        want = [e.cpu().numpy() for e in eager]
+        singles.append(want)
+        assert info["status"] == 0 and info["rel_residual"] <= 1e-12, info
+        assert rel(got, want) < 1e-13, k
+        ref, _, _ = O.Composite(doms, conn, tol=1e-14, tol_is_relative=True, maxiter=400,
+                                schur_mode=O.SCHUR_SPECTRAL).solve(fs)
+        assert rel(got, ref) < 1e-10, k
+    multi = fpa.ConcatPoissonSolver2D(ctx, doms, conn, m=40, tol=1e-12, tol_is_relative=True, max_iter=100)
+    ts = [to_dev(np.stack([rhs[r][d] for r in range(3)])) for d in range(len(doms))]
+    multi.solve(ts)
+    torch.cuda.synchronize()
+    for r in range(3):
+        assert rel([t[r].cpu().numpy() for t in ts], singles[r]) < 1e-12, r
