@@ -39,7 +39,8 @@ def parse():
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--extras", action="store_true",
                     help="measure the other workloads instead (cross x 64 right-hand sides, 512 batched uneven "
-                         "problems in FP64 and FP32, 4096^2 FP64 vs FP32+refine, adjoint fwd+bwd ratio); "
+                         "problems in FP64 and FP32, 4096^2 FP64 vs FP32+refine, L-shape, periodic channel, "
+                         "adjoint fwd+bwd ratio); "
                          "prints one JSON line")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the per-kernel timing on the centre subdomain plus an 8 MiB device copy (the "
@@ -226,6 +227,39 @@ def extras(fpa, ctx, torch, np, n):
                             "refine_steps": res["f32_refine"][1].get("refine_steps"),
                             "rel_l2_f32_refine_vs_f64": d,
                             "gmres_rel_residual_f64": res["f64"][1].get("rel_residual")}
+    del res, src
+
+    # periodic channel: two 1024^2 domains side by side, periodic in y,
+    # Dirichlet / Neumann x ends, coupled across x (the periodic transform on
+    # the two-level path), RHS N(0,1) (seed 9); FP64 and FP32 + refinement
+    hp = 1.0 / 1024
+    pdoms = [(1024, 1024, hp, hp, (0, 3, 2, 2)), (1024, 1024, hp, hp, (3, 1, 2, 2))]
+    pconn = [(0, 1, 1)]
+    prng = np.random.default_rng(9)
+    src = [torch.from_numpy(prng.standard_normal((d[0], d[1]))).to(dev) for d in pdoms]
+    res = {}
+    for prec, key in ((fpa.FP_F64, "f64"), (fpa.FP_F32_REFINE, "f32_refine")):
+        cat = fpa.ConcatPoissonSolver2D(ctx, pdoms, pconn, m=200, tol=1e-12, tol_is_relative=True, max_iter=100,
+                                        precision=prec)
+        if prec == fpa.FP_F32_REFINE:
+            cat.set_refinement(tol=1e-12, max_steps=10)
+        wk = [torch.empty_like(x) for x in src]
+        info = {}
+
+        def solve_p():
+            for w, x in zip(wk, src):
+                w.copy_(x)
+            info.update(cat.solve(wk))
+
+        res[key] = (timed(solve_p, torch), dict(info), [w.clone() for w in wk])
+        del cat
+    d = max(float((a - b).norm() / b.norm()) for a, b in zip(res["f32_refine"][2], res["f64"][2]))
+    out["periodic_channel_2x1024"] = {"f64_ms": round(res["f64"][0] * 1e3, 3),
+                                      "f32_refine_ms": round(res["f32_refine"][0] * 1e3, 3),
+                                      "refine_steps": res["f32_refine"][1].get("refine_steps"),
+                                      "rel_l2_f32_refine_vs_f64": d,
+                                      "gmres_iterations_f64": res["f64"][1].get("iterations"),
+                                      "gmres_rel_residual_f64": res["f64"][1].get("rel_residual")}
     del res, src
 
     # single 256^2 Dirichlet domain (the reference's own CPU case): solves/s,
