@@ -15,7 +15,7 @@ stream provider only); every computation runs in libfocalors_poisson.so.
 import ctypes
 
 from . import _capi
-from ._capi import FP_F32_REFINE, FP_F64, FPError, SolveInfo  # noqa: F401
+from ._capi import FP_F32_REFINE, FP_F64, FP_F64_REFINE, FPError, SolveInfo  # noqa: F401
 
 DIRICHLET, NEUMANN, PERIODIC, ADJACENTED, NULL = 0, 1, 2, 3, 4
 XNEG, XPOS, YNEG, YPOS = 0, 1, 2, 3
@@ -158,8 +158,8 @@ class ConcatPoissonSolver2D:
         self.root = _capi.lib().fp_cat_root(self._h)
 
     def set_refinement(self, tol=1e-12, max_steps=10):
-        """FP_F32_REFINE handles: target estimated relative error and sweep cap
-        of the iterative refinement (fp_cat_set_refinement)."""
+        """FP_F32_REFINE / FP_F64_REFINE handles: target estimated relative
+        error and sweep cap of the iterative refinement (fp_cat_set_refinement)."""
         _capi.check(_capi.lib().fp_cat_set_refinement(self._h, float(tol), int(max_steps)))
 
     def _ptrs(self, fields, ghost):

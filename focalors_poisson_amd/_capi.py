@@ -12,7 +12,7 @@ LIB_PATH = os.path.join(PKG, "libfocalors_poisson.so")
 HEADER = os.path.join(os.path.dirname(PKG), "include", "focalors_poisson.h")
 
 FP_OK, FP_E_INVALID, FP_E_NOMEM, FP_E_HIP, FP_E_NOT_CONVERGED, FP_E_NUMERIC, FP_E_UNSUPPORTED = range(7)
-FP_F64, FP_F32_REFINE = 0, 1
+FP_F64, FP_F32_REFINE, FP_F64_REFINE = 0, 1, 2
 
 _lib = None
 _vp = ctypes.c_void_p
@@ -24,7 +24,7 @@ _ip = ctypes.POINTER(ctypes.c_int)
 class SolveInfo(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("restarts", ctypes.c_int), ("residual", ctypes.c_double),
                 ("rel_residual", ctypes.c_double), ("refine_steps", ctypes.c_int),
-                ("refine_residual", ctypes.c_double)]
+                ("refine_residual", ctypes.c_double), ("refine_correction", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

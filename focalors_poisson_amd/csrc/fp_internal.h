@@ -235,6 +235,13 @@ void launch_ytransform_batch(const TransformJob<T>* jobs, int njobs, int inverse
 // forward batch; the jobs' fields are overwritten.
 template <typename T>
 std::array<int, 2> tune_batch_shape(std::vector<TransformJob<T>> jobs, hipStream_t s);
+// Whether a batch launch shape {threads per workgroup (0: the plan's own),
+// odd-prime form (-1: the plan's own)} runs plan tp correctly; and the job's
+// shape set to it where it does, else to the plan's own.
+template <typename T>
+bool batch_shape_fits(const TransformPlan<T>& tp, int nt, int form);
+template <typename T>
+void apply_batch_shape(TransformJob<T>& j, const std::array<int, 2>& shape);
 template <typename T>
 struct TridiagJob {
     const SubPlan* sp;

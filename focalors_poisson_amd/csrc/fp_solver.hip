@@ -10,6 +10,7 @@
 #include "fp_vec.h"
 
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -291,6 +292,7 @@ struct fp_cat_s {
     double*                                h_rpart = nullptr;
     int                                    last_refine_steps = 0;
     double                                 last_refine_est   = 0.0;
+    double                                 last_refine_corr  = 0.0;
     ~fp_cat_s()
     {
         if (h_rpart) (void)hipHostFree(h_rpart);
@@ -312,6 +314,7 @@ struct fp_cat_s {
     bool                                   no_wait    = false;
     hipGraphExec_t                         gexec      = nullptr;
     int                                    gnrhs      = 0;
+    int                                    gsteps     = 0;  // refinement sweeps recorded in the graph
     // statistics of the last solve
     int    last_steps = 0, last_restarts = 0, last_status = 0;
     double last_res = 0.0, last_rel = 0.0;
@@ -372,9 +375,23 @@ static bool same_basis(const SubPlan& a, const SubPlan& b)
     return a.kind == b.kind && a.ny == b.ny;
 }
 
+// A captured solve refers to the workspace it was recorded on: every
+// reallocation of that workspace (a later solve with more right-hand sides)
+// destroys the graph first, so that fp_cat_graph_launch reports that no
+// solve is captured instead of replaying into freed memory.
+static void drop_graph(fp_cat_s* c)
+{
+    if (!c->gexec) return;
+    FP_HIP(hipDeviceSynchronize());
+    (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+    c->gnrhs = 0;
+}
+
 static void ensure_ws(fp_cat_s* c, Node& nd, int nrhs)
 {
     if (nd.ws_nrhs >= nrhs && nd.ws_m == c->m) return;
+    drop_graph(c);
     const int       K = nd.K, m = c->m;
     const long long n = dom_n(c, nd.dom);
     nd.nblk           = (K + 63) / 64;
@@ -980,6 +997,7 @@ static void cat_sweep(fp_cat_s* c, double* const* f, int nrhs, hipStream_t s)
 static void ensure_refine_ws(fp_cat_s* c, int nrhs)
 {
     if (c->ws32_nrhs >= nrhs) return;
+    drop_graph(c);
     const int nd = c->nd;
     c->s32.clear();
     c->rb.clear();
@@ -1032,46 +1050,84 @@ static void cat_residual(fp_cat_s* c, double* const* u, int nrhs, hipStream_t s)
     }
 }
 
-// FP32 + refinement of the composite solve (no reference counterpart):
-//   u_1 = S32(b);  u_{k+1} = u_k + S32(b - A u_k)
-// with S32 the tree sweep whose subdomain solves run in FP32 arithmetic (the
-// interface GMRES, Schur products and this residual stay FP64).  With
-// d_k = u_k - u_{k-1} and rho = |d_k| / |d_{k-1}| (k >= 3) the error of u_k
-// is estimated as rho / (1 - rho) |d_k| (contraction of the iteration); the
-// loop stops when that estimate, or |d_k| itself, is at most tol |u_k| for
+// Iterative refinement of the composite solve (no reference counterpart):
+//   u_1 = S(b);  u_{k+1} = u_k + S(b - A u_k)
+// with the FP64 residual of the assembled composite operator.  S is the tree
+// sweep: FP_F32_REFINE runs its subdomain solves in FP32 arithmetic (the
+// interface GMRES, Schur products and the residual stay FP64); FP_F64_REFINE
+// runs the plain FP64 sweep, whose first solve keeps its GMRES residual check
+// and whose correction solves start the interface solve from the explicit
+// inverse without a check.  The FP64 composite solve has a rounding floor
+// that grows with the grid (about 4e-11 relative on the 1024 cross, for the
+// reference's algorithm as for this one); one correction with the FP64
+// residual takes it to about 1e-14.
+//
+// Error estimate of u_k, with d_k = u_k - u_{k-1} and ratio_k = |d_k| / |u_k|:
+// d_2 is about the error of u_1, so ratio_2 estimates the inner solve's
+// relative accuracy, which is also the contraction of the iteration;
+// from k = 3 on the contraction is measured, rho = |d_k| / |d_{k-1}|.  The
+// estimate rho / (1 - rho) ratio_k (rho = ratio_2 at k = 2 for FP64 inner
+// solves; FP32 ones wait for the measured contraction) is never below
+// kRefineFloor, the accuracy an FP64 residual allows, nor used when rho >= 1
+// (then ratio_k itself).  The loop stops when the estimate is at most tol for
 // every right-hand side, or after refine_max sweeps.
+constexpr double kRefineFloor = 64.0 * 2.220446049250313e-16;
+
 static bool fast_shape_ok(const fp_cat_s* c);
 template <typename T>
 static void cat_sweep_fast_t(fp_cat_s* c, double* const* f, const double* const* ghost, int nrhs, hipStream_t s,
-                             const double* const* rhs = nullptr);
+                             const double* const* rhs = nullptr, bool assembled = false);
 
-static void cat_refine(fp_cat_s* c, double* const* f, int nrhs, hipStream_t s)
+// u += d (d null: u unchanged) for every domain, with the per-block norms
+// |d|^2, |u|^2 into norm slot `slot`
+static void refine_norms(fp_cat_s* c, double* const* u, const double* const* dlt, size_t slot, int nrhs,
+                         hipStream_t s)
 {
-    const int nd = c->nd;
+    const int    nd   = c->nd;
+    const size_t half = (size_t)2 * nd * nrhs * kNormBlocks;  // doubles per norm set
+    NormsBatch   nb;
+    for (int d = 0; d < nd; d++) {
+        nb.j[nb.n++] = {u[d], dom_n(c, d), dlt ? dlt[d] : nullptr, dom_n(c, d), dom_n(c, d),
+                        c->rpart.p + slot * half + (size_t)2 * d * nrhs * kNormBlocks};
+        if (nb.n == 8 || d == nd - 1) {
+            launch_update_norms_batch(nb, nrhs, s);
+            nb.n = 0;
+        }
+    }
+}
+
+// The refinement on assembled, hx^2-scaled right-hand sides f (solution in
+// place).  capture: the stream is being recorded into a graph, so the loop
+// takes exactly refine_max - 1 corrections with no host reads.
+static void cat_refine(fp_cat_s* c, double* const* f, int nrhs, hipStream_t s, bool capture = false)
+{
+    const int  nd  = c->nd;
+    const bool f32 = c->precision == FP_F32_REFINE;
     ensure_refine_ws(c, nrhs);
-    // the inner sweep: the two-level sweep in transform space with FP32
-    // transforms and sweeps where the tree allows it, else the general sweep
-    // with FP32 subdomain solves (c->mixed)
-    const bool fast  = fast_shape_ok(c);
-    auto       sweep = [&](double* const* u) {
-        if (fast) cat_sweep_fast_t<float>(c, u, nullptr, nrhs, s);
-        else cat_sweep(c, u, nrhs, s);
+    // the inner sweep: the two-level sweep in transform space where the tree
+    // allows it, else the general sweep (FP32 subdomain solves when c->mixed)
+    const bool fast = fast_shape_ok(c);
+    int        gm_steps = 0, gm_restarts = 0, gm_status = 0;
+    double     gm_res = 0.0, gm_rel = 0.0;
+    auto       sweep = [&](double* const* u, bool first) {
+        if (fast && f32) cat_sweep_fast_t<float>(c, u, nullptr, nrhs, s);
+        else if (fast) {
+            c->no_wait = !first;
+            cat_sweep_fast_t<double>(c, u, nullptr, nrhs, s, nullptr, true);
+            c->no_wait = false;
+        } else cat_sweep(c, u, nrhs, s);
+        if (first) {
+            gm_steps    = c->last_steps;
+            gm_restarts = c->last_restarts;
+            gm_status   = c->last_status;
+            gm_res      = c->last_res;
+            gm_rel      = c->last_rel;
+        }
     };
     for (int d = 0; d < nd; d++)
         FP_HIP(hipMemcpyAsync(c->rb[d].p, f[d], sizeof(double) * dom_n(c, d) * nrhs, hipMemcpyDeviceToDevice, s));
-    const size_t half = (size_t)2 * nd * nrhs * kNormBlocks;  // doubles per norm set
-    auto         norms = [&](double* const* u, const double* const* dlt, size_t slot) {
-        NormsBatch nb;
-        for (int d = 0; d < nd; d++) {
-            nb.j[nb.n++] = {u[d], dom_n(c, d), dlt ? dlt[d] : nullptr, dom_n(c, d), dom_n(c, d),
-                            c->rpart.p + slot * half + (size_t)2 * d * nrhs * kNormBlocks};
-            if (nb.n == 8 || d == nd - 1) {
-                launch_update_norms_batch(nb, nrhs, s);
-                nb.n = 0;
-            }
-        }
-    };
-    auto sums = [&](size_t slot, int r, double& dd, double& uu) {
+    const size_t half = (size_t)2 * nd * nrhs * kNormBlocks;
+    auto         sums = [&](size_t slot, int r, double& dd, double& uu) {
         dd = uu = 0.0;
         for (int d = 0; d < nd; d++) {
             const double* p = c->h_rpart + slot * half + (size_t)2 * d * nrhs * kNormBlocks + (size_t)2 * r * kNormBlocks;
@@ -1084,23 +1140,28 @@ static void cat_refine(fp_cat_s* c, double* const* f, int nrhs, hipStream_t s)
     std::vector<double*>       w(nd);
     std::vector<const double*> wc(nd);
     for (int d = 0; d < nd; d++) wc[d] = w[d] = c->rw[d].p;
-    c->mixed   = true;
-    c->no_wait = fast;
-    std::vector<double> prev(nrhs, 0.0);
+    c->mixed   = f32;
+    c->no_wait = fast && f32;
+    std::vector<double> prev(nrhs, 0.0), first_ratio(nrhs, 0.0);
     int                 steps = 0;
-    double              est   = INFINITY;
+    double              est = INFINITY, corr = INFINITY;
+    auto                restore = [&] {
+        c->mixed   = false;
+        c->no_wait = false;
+    };
     try {
-        sweep(f);  // u_1 = d_1
+        sweep(f, true);  // u_1 = d_1
         steps = 1;
-        norms(f, nullptr, 0);      // |u_1|^2 in slot 0
+        if (!capture) refine_norms(c, f, nullptr, 0, nrhs, s);  // |u_1|^2 in slot 0
         for (int k = 2; k <= c->refine_max; k++) {
             cat_residual(c, f, nrhs, s);
-            sweep(w.data());
-            norms(f, wc.data(), 1);  // u += d, |d|^2, |u|^2 in slot 1
+            sweep(w.data(), false);
+            refine_norms(c, f, wc.data(), 1, nrhs, s);  // u += d, |d|^2, |u|^2 in slot 1
             steps = k;
+            if (capture) continue;
             FP_HIP(hipMemcpyAsync(c->h_rpart, c->rpart.p, sizeof(double) * 2 * half, hipMemcpyDeviceToHost, s));
             FP_HIP(hipStreamSynchronize(s));
-            double worst = 0.0;
+            double worst = 0.0, wcorr = 0.0;
             bool   bad   = false;
             for (int r = 0; r < nrhs; r++) {
                 double dd, uu, d0, u0;
@@ -1111,27 +1172,38 @@ static void cat_refine(fp_cat_s* c, double* const* f, int nrhs, hipStream_t s)
                 sums(1, r, dd, uu);
                 const double dn = std::sqrt(dd), un = std::sqrt(uu);
                 const double ratio = un > 0 ? dn / un : dn;
-                // d_1 = u_1 is the whole solution, so |d_2| / |d_1| is the inner
-                // solve's accuracy, not the contraction: the contraction is
-                // first seen in |d_3| / |d_2|
-                const double rho   = (k > 2 && prev[r] > 0) ? dn / prev[r] : 1.0;
-                double       e     = rho < 1.0 ? std::min(ratio, rho / (1.0 - rho) * ratio) : ratio;
-                if (!std::isfinite(e)) bad = true;
+                if (k == 2) first_ratio[r] = ratio;
+                // d_1 = u_1 is the whole solution: the contraction is first
+                // measured as |d_3| / |d_2|; at k = 2 an FP64 inner solve's own
+                // relative accuracy (ratio_2) stands in for it
+                double rho = 1.0;
+                if (k > 2 && prev[r] > 0) rho = dn / prev[r];
+                else if (k == 2 && !f32) rho = ratio;
+                double e = rho < 1.0 ? rho / (1.0 - rho) * ratio : ratio;
+                e        = std::max(e, kRefineFloor);
+                if (!std::isfinite(e) || !std::isfinite(ratio)) bad = true;
                 worst   = std::max(worst, e);
+                wcorr   = std::max(wcorr, ratio);
                 prev[r] = dn;
             }
-            est = bad ? INFINITY : worst;
+            est  = bad ? INFINITY : worst;
+            corr = bad ? INFINITY : wcorr;
             if (bad || est <= c->refine_tol) break;
         }
     } catch (...) {
-        c->mixed   = false;
-        c->no_wait = false;
+        restore();
         throw;
     }
-    c->mixed             = false;
-    c->no_wait           = false;
+    restore();
+    c->last_steps        = gm_steps;
+    c->last_restarts     = gm_restarts;
+    c->last_res          = gm_res;
+    c->last_rel          = gm_rel;
+    c->last_status       = gm_status;
     c->last_refine_steps = steps;
-    c->last_refine_est   = est;
+    c->last_refine_est   = capture ? 0.0 : est;
+    c->last_refine_corr  = capture ? 0.0 : corr;
+    if (capture) return;
     if (!std::isfinite(est)) c->last_status = FP_E_NUMERIC;
     else if (est > c->refine_tol && c->last_status == 0) c->last_status = FP_E_NOT_CONVERGED;
 }
@@ -1244,6 +1316,7 @@ static void ensure_fast_ws(fp_cat_s* c, int nrhs)
 {
     FastLines<T>& w = fast_lines<T>(c);
     if (w.nrhs >= nrhs) return;
+    drop_graph(c);
     alloc_fast_lines(c, w, nrhs);
     FP_HIP(hipDeviceSynchronize());
     w.nrhs = nrhs;
@@ -1307,6 +1380,12 @@ static std::array<int, 2> batch_shape(fp_cat_s* c)
     std::array<int, 2>& v = sizeof(T) == 8 ? c->yt_shape64 : c->yt_shape32;
     if (v[0] >= 0) return v;
     if (c->nd < 2) return v = {0, -1};
+    // FP_BATCH_SHAPE=<threads>,<form> forces the shared shape (tests): it is
+    // still applied only to the domains it fits
+    if (const char* e = std::getenv("FP_BATCH_SHAPE")) {
+        int nt = 0, form = -1;
+        if (std::sscanf(e, "%d,%d", &nt, &form) == 2) return v = {nt, form};
+    }
     std::vector<DevBuf<T>>       buf(c->nd);
     std::vector<TransformJob<T>> jobs(c->nd);
     for (int d = 0; d < c->nd; d++) {
@@ -1330,9 +1409,12 @@ static std::array<int, 2> batch_shape(fp_cat_s* c)
 
 // rhs (FP64 only; null: f holds them): raw right-hand sides read by the
 // forward transforms instead of f, which then only receives the solution.
+// assembled (FP64 only): f already holds assembled, hx^2-scaled right-hand
+// sides (the corrections of a refinement), so the forward transforms read
+// them as they are.
 template <typename T>
 static void cat_sweep_fast_t(fp_cat_s* c, double* const* f, const double* const* ghost, int nrhs, hipStream_t s,
-                             const double* const* rhs)
+                             const double* const* rhs, bool assembled)
 {
     constexpr bool f64 = sizeof(T) == 8;
     const int      nd = c->nd, root = c->root;
@@ -1359,8 +1441,7 @@ static void cat_sweep_fast_t(fp_cat_s* c, double* const* f, const double* const*
         a.nx    = c->nx[d];
         a.nrhs  = nrhs;
         a.dst64 = f64;
-        a.nt    = shape[0];
-        a.odd   = shape[1];
+        apply_batch_shape(a, shape);
         b       = a;
         a.data  = v.fld[d];
         a.ld    = n;
@@ -1368,9 +1449,11 @@ static void cat_sweep_fast_t(fp_cat_s* c, double* const* f, const double* const*
         b.ld    = n;
         b.dst64 = 1;
         if constexpr (f64) {
-            a.asmb = 1;
-            a.hx2  = c->hx[d] * c->hx[d];
-            face_terms(c, d, ghost, a.g, a.ga);
+            if (!assembled) {
+                a.asmb = 1;
+                a.hx2  = c->hx[d] * c->hx[d];
+                face_terms(c, d, ghost, a.g, a.ga);
+            }
             if (rhs && rhs[d] != f[d]) {
                 a.src = rhs[d];
                 a.lds = n;
@@ -1486,7 +1569,11 @@ static void cat_sweep_fast_t(fp_cat_s* c, double* const* f, const double* const*
             bool           fused;
             const GmresDev g = gmres_begin(c, R, nrhs, s, fused);
             FP_REQUIRE(fused || !c->graph_mode, "graph capture needs the one-workgroup GMRES start");
-            if (fused) {
+            if (c->no_wait) {
+                // a correction solve of an FP64 refinement: x0 = M C with no
+                // residual check (the outer FP64 residual decides)
+                final_sweep(true, nullptr);
+            } else if (fused) {
                 final_sweep(true, R.G.p);
                 launch_lines_assemble<double>(R.X.p, R.Wv.p, R.K, R.kface.p, R.kpos.p, R.XU.p, nyr, R.CP.p,
                                               sweep_blocks(R.plan), nxr, nrhs, s, 0, 2, R.C.p);
@@ -1566,7 +1653,8 @@ static void cat_solve(fp_cat_s* c, double* const* f, const double* const* ghost,
 {
     c->last_refine_steps = 0;
     c->last_refine_est   = 0.0;
-    if (rhs && c->precision != FP_F32_REFINE && fast_ok(c)) {
+    c->last_refine_corr  = 0.0;
+    if (rhs && c->precision == FP_F64 && fast_ok(c)) {
         cat_sweep_fast(c, f, ghost, nrhs, s, rhs);
         return;
     }
@@ -1574,7 +1662,7 @@ static void cat_solve(fp_cat_s* c, double* const* f, const double* const* ghost,
         for (int d = 0; d < c->nd; d++)
             if (rhs[d] != f[d])
                 FP_HIP(hipMemcpyAsync(f[d], rhs[d], sizeof(double) * dom_n(c, d) * nrhs, hipMemcpyDeviceToDevice, s));
-    if (c->precision == FP_F32_REFINE) {
+    if (c->precision != FP_F64) {
         cat_assemble(c, f, ghost, nrhs, s);
         cat_refine(c, f, nrhs, s);
     } else if (fast_ok(c))
@@ -1734,8 +1822,9 @@ int fp_sub_solve_refine(fp_sub sub, double* d_f, int nrhs, long long ld, double 
         const fp::RefineStats r = fp::refine_solve(sub->plan, d_f, ldd, nrhs, tol, max_steps, (hipStream_t)stream);
         if (info) {
             *info                 = fp_solve_info{};
-            info->refine_steps    = r.steps;
-            info->refine_residual = r.correction;
+            info->refine_steps      = r.steps;
+            info->refine_residual   = r.correction;
+            info->refine_correction = r.correction;
         }
         if (r.numeric) extra = FP_E_NUMERIC;
         else if (r.correction > tol) extra = FP_E_NOT_CONVERGED;
@@ -1762,7 +1851,8 @@ int fp_cat_create(fp_ctx ctx, const fp_geometry_desc* g, int m, double tol, int 
         FP_REQUIRE(g->nd >= 1 && g->nx && g->ny && g->hx && g->hy && g->bc, "empty geometry");
         FP_REQUIRE(m >= 1 && m <= 1000, "m must be in [1, 1000]");
         FP_REQUIRE(max_iter >= 1 && tol >= 0.0, "invalid GMRES parameters");
-        FP_REQUIRE(precision == FP_F64 || precision == FP_F32_REFINE, "unknown precision");
+        FP_REQUIRE(precision == FP_F64 || precision == FP_F32_REFINE || precision == FP_F64_REFINE,
+                   "unknown precision");
         FP_HIP(hipSetDevice(ctx->device));
         auto c       = std::make_unique<fp_cat_s>();
         c->ctx       = ctx;
@@ -1772,6 +1862,8 @@ int fp_cat_create(fp_ctx ctx, const fp_geometry_desc* g, int m, double tol, int 
         c->tol_rel   = tol_is_relative ? 1 : 0;
         c->maxiter   = max_iter;
         c->precision = precision;
+        // FP64 refinement: one correction takes the FP64 floor to about 1e-14
+        if (precision == FP_F64_REFINE) c->refine_max = 2;
         const int nd = g->nd;
         c->nx.assign(g->nx, g->nx + nd);
         c->ny.assign(g->ny, g->ny + nd);
@@ -1788,6 +1880,21 @@ int fp_cat_create(fp_ctx ctx, const fp_geometry_desc* g, int m, double tol, int 
             c->bc[4 * b + (dir ^ 1)]  = fp::ADJACENTED;
         }
         if (const char* e = std::getenv("FP_INTERFACE_PRECOND")) c->precond_kind = std::atoi(e);
+        // No Dirichlet face anywhere (Neumann and periodic faces only): the
+        // assembled operator has the constants as its null space, and so has
+        // the interface operator of a composite.  Its explicit inverse does not
+        // exist; the interface GMRES runs unpreconditioned, as the
+        // reference's does (gmres_solver2d.cpp:123-252), and converges for a
+        // compatible right-hand side (zero total).  A lone domain is gauged by
+        // its sweep (x_0 = 0 in the zero mode, chasing_method_base.cpp:146-154).
+        bool any_dirichlet = false;
+        for (int k = 0; k < 4 * nd; k++)
+            if (c->bc[k] == fp::DIRICHLET) any_dirichlet = true;
+        if (nd > 1 && !any_dirichlet) {
+            if (precision != FP_F64)
+                throw fp::Error(FP_E_UNSUPPORTED, "refinement of a composite without a Dirichlet face (singular operator)");
+            c->precond_kind = 0;
+        }
         fp::build_tree(c.get());
         c->nodes.resize(nd);
         // construct_solver_map: deepest level first, root last (concat_solver2d.cpp:88-99)
@@ -1795,8 +1902,8 @@ int fp_cat_create(fp_ctx ctx, const fp_geometry_desc* g, int m, double tol, int 
         for (int l = c->nlev - 1; l >= 0; l--)
             for (int d : c->levels[l]) fp::build_node(c.get(), d, s);
         if (nd > 1 && fp::fast_shape_ok(c.get())) {
-            if (precision == FP_F64) fp::batch_shape<double>(c.get());
-            else fp::batch_shape<float>(c.get());
+            if (precision == FP_F32_REFINE) fp::batch_shape<float>(c.get());
+            else fp::batch_shape<double>(c.get());
         }
         FP_HIP(hipDeviceSynchronize());
         *out = c.release();
@@ -1820,8 +1927,9 @@ static int cat_solve_entry(fp_cat cat, const double* const* d_rhs, double* const
             info->restarts        = cat->last_restarts;
             info->residual        = cat->last_res;
             info->rel_residual    = cat->last_rel;
-            info->refine_steps    = cat->last_refine_steps;
-            info->refine_residual = cat->last_refine_est;
+            info->refine_steps      = cat->last_refine_steps;
+            info->refine_residual   = cat->last_refine_est;
+            info->refine_correction = cat->last_refine_corr;
         }
     });
     if (st == FP_OK && cat && cat->last_status) st = cat->last_status;
@@ -1848,8 +1956,9 @@ static int graph_capture(fp_cat cat, const double* const* d_rhs, double* const* 
         FP_REQUIRE(cat && d_fields && nrhs >= 1, "null argument");
         FP_REQUIRE(stream, "graph capture needs a stream other than the null stream");
         for (int d = 0; d < cat->nd; d++) FP_REQUIRE(d_fields[d] && (!d_rhs || d_rhs[d]), "null field");
-        FP_REQUIRE(cat->precision == FP_F64 && fp::fast_ok(cat),
-                   "graph capture covers the FP64 solve of a root with leaf children");
+        const bool refine = cat->precision == FP_F64_REFINE;
+        FP_REQUIRE((cat->precision == FP_F64 || refine) && fp::fast_ok(cat),
+                   "graph capture covers the FP64 solve (plain or refined) of a root with leaf children");
         fp::Node& R = *cat->nodes[cat->root];
         FP_REQUIRE(R.ch.empty() || R.precond, "graph capture needs the explicit-inverse interface preconditioner");
         FP_HIP(hipSetDevice(cat->ctx->device));
@@ -1859,6 +1968,7 @@ static int graph_capture(fp_cat cat, const double* const* d_rhs, double* const* 
             fp::ensure_ws(cat, R, nrhs);
             fp::ensure_fast_ws<double>(cat, nrhs);
         }
+        if (refine) fp::ensure_refine_ws(cat, nrhs);
         FP_HIP(hipDeviceSynchronize());
         if (cat->gexec) {
             (void)hipGraphExecDestroy(cat->gexec);
@@ -1868,7 +1978,16 @@ static int graph_capture(fp_cat cat, const double* const* d_rhs, double* const* 
         FP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         cat->graph_mode = true;
         try {
-            fp::cat_sweep_fast(cat, d_fields, d_ghost, nrhs, s, d_rhs);
+            if (refine) {
+                if (d_rhs)
+                    for (int d = 0; d < cat->nd; d++)
+                        if (d_rhs[d] != d_fields[d])
+                            FP_HIP(hipMemcpyAsync(d_fields[d], d_rhs[d], sizeof(double) * fp::dom_n(cat, d) * nrhs,
+                                                  hipMemcpyDeviceToDevice, s));
+                fp::cat_assemble(cat, d_fields, d_ghost, nrhs, s);
+                fp::cat_refine(cat, d_fields, nrhs, s, true);
+            } else
+                fp::cat_sweep_fast(cat, d_fields, d_ghost, nrhs, s, d_rhs);
         } catch (...) {
             cat->graph_mode = false;
             (void)hipStreamEndCapture(s, &graph);
@@ -1880,7 +1999,8 @@ static int graph_capture(fp_cat cat, const double* const* d_rhs, double* const* 
         hipError_t e = hipGraphInstantiate(&cat->gexec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
         FP_HIP(e);
-        cat->gnrhs = nrhs;
+        cat->gnrhs  = nrhs;
+        cat->gsteps = refine ? cat->last_refine_steps : 0;
     });
 }
 
@@ -1914,8 +2034,11 @@ int fp_cat_graph_launch(fp_cat cat, void* stream, fp_solve_info* info)
         info->restarts        = cat->last_restarts;
         info->residual        = cat->last_res;
         info->rel_residual    = cat->last_rel;
-        info->refine_steps    = 0;
-        info->refine_residual = 0.0;
+        // a replayed refinement took its recorded number of sweeps; no
+        // estimate is formed (no host reads inside the graph)
+        info->refine_steps      = cat->gsteps;
+        info->refine_residual   = 0.0;
+        info->refine_correction = 0.0;
     });
     if (st == FP_OK && cat && cat->last_status) st = cat->last_status;
     return st;
@@ -1980,8 +2103,9 @@ int fp_cat_vjp(fp_cat cat, const double* const* d_gu, double* const* d_gf, doubl
             info->restarts        = cat->last_restarts;
             info->residual        = cat->last_res;
             info->rel_residual    = cat->last_rel;
-            info->refine_steps    = cat->last_refine_steps;
-            info->refine_residual = cat->last_refine_est;
+            info->refine_steps      = cat->last_refine_steps;
+            info->refine_residual   = cat->last_refine_est;
+            info->refine_correction = cat->last_refine_corr;
         }
     });
     if (st == FP_OK && cat && cat->last_status) st = cat->last_status;

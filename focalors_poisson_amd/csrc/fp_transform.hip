@@ -1413,6 +1413,10 @@ static void launch_group(const TransformJob<T>* const* jobs, int n, int engine, 
         if (blocks + nb > (1LL << 30)) throw Error(FP_E_INVALID, "too many transform workgroups in one launch");
         blocks += (int)nb;
         lds = std::max(lds, lds_bytes(*j.plan, odd));
+        // a shape that leaves butterflies without a thread would give wrong
+        // coefficients without any error: refuse it
+        if (engine == 1 && !sb_fits(j.plan->rad, j.plan->nrad, j.plan->F, NT, odd, j.plan->ept))
+            throw Error(FP_E_INVALID, "transform launch shape does not fit the plan");
     }
     start[n] = blocks;
     if (lds > 160 * 1024) throw Error(FP_E_INVALID, "transform length exceeds the LDS of one workgroup");
@@ -1567,6 +1571,30 @@ void tune_transform(TransformPlan<T>& tp, int nx, bool measure)
 // candidate, best of three interleaved rounds; {0, -1} (every job its own
 // shape) is one of them.  The jobs must point at valid fields.
 template <typename T>
+bool batch_shape_fits(const TransformPlan<T>& tp, int nt, int form)
+{
+    if (tp.engine != 1) return false;
+    const int f  = form < 0 ? tp.odd : form;
+    const int t  = nt ? nt : tp.nt;
+    const bool ok = sb_fits(tp.rad, tp.nrad, tp.F, t, f, tp.ept) && lds_bytes(tp, f) <= 160 * 1024;
+    return ok && (f != ODD_NONE || tp.odd == ODD_NONE) && (tp.ept == kEPT || f == ODD_NONE);
+}
+template <typename T>
+void apply_batch_shape(TransformJob<T>& j, const std::array<int, 2>& shape)
+{
+    j.nt  = 0;
+    j.odd = -1;
+    if (batch_shape_fits(*j.plan, shape[0], shape[1])) {
+        j.nt  = shape[0];
+        j.odd = shape[1];
+    }
+}
+template bool batch_shape_fits<double>(const TransformPlan<double>&, int, int);
+template bool batch_shape_fits<float>(const TransformPlan<float>&, int, int);
+template void apply_batch_shape<double>(TransformJob<double>&, const std::array<int, 2>&);
+template void apply_batch_shape<float>(TransformJob<float>&, const std::array<int, 2>&);
+
+template <typename T>
 std::array<int, 2> tune_batch_shape(std::vector<TransformJob<T>> jobs, hipStream_t s)
 {
     if (jobs.size() < 2) return {0, -1};
@@ -1590,19 +1618,8 @@ std::array<int, 2> tune_batch_shape(std::vector<TransformJob<T>> jobs, hipStream
             const int nt = cands[ci][0], form = cands[ci][1];
             bool      any = nt == 0 && form < 0;
             for (auto& j : jobs) {
-                const TransformPlan<T>& tp = *j.plan;
-                j.nt  = 0;
-                j.odd = -1;
-                if (tp.engine != 1) continue;
-                const int f  = form < 0 ? tp.odd : form;
-                const int t  = nt ? nt : tp.nt;
-                bool      ok = sb_fits(tp.rad, tp.nrad, tp.F, t, f, tp.ept) && lds_bytes(tp, f) <= 160 * 1024;
-                ok           = ok && (f != ODD_NONE || tp.odd == ODD_NONE) && (tp.ept == kEPT || f == ODD_NONE);
-                if (ok) {
-                    j.nt  = nt;
-                    j.odd = form;
-                    any   = true;
-                }
+                apply_batch_shape(j, cands[ci]);
+                if (j.nt != 0 || j.odd != -1) any = true;
             }
             if (!any) continue;
             launch_ytransform_batch<T>(jobs.data(), (int)jobs.size(), 0, s);

@@ -57,7 +57,10 @@ typedef enum fp_status {
 
 typedef enum fp_precision {
     FP_F64        = 0, /* transforms, tridiagonal sweeps and Krylov work in FP64 */
-    FP_F32_REFINE = 1  /* FP32 transforms and sweeps, FP64 residual and iterative refinement */
+    FP_F32_REFINE = 1, /* FP32 transforms and sweeps, FP64 residual and iterative refinement */
+    FP_F64_REFINE = 2  /* the FP64 solve plus iterative refinement with the FP64 residual
+                          (composite handles): removes the FP64 rounding floor of the
+                          composite solve, which grows with the grid */
 } fp_precision;
 
 typedef struct fp_solve_info {
@@ -65,8 +68,11 @@ typedef struct fp_solve_info {
     int    restarts;       /* GMRES restart cycles */
     double residual;       /* final interface residual norm, max over right-hand sides */
     double rel_residual;   /* residual / initial preconditioned right-hand-side norm */
-    int    refine_steps;   /* iterative-refinement steps (FP_F32_REFINE only) */
-    double refine_residual;/* final relative FP64 residual of the refinement */
+    int    refine_steps;   /* sweeps of an iterative refinement (the first solve included) */
+    double refine_residual;/* estimated relative error of the refined solution (never below
+                              64 * DBL_EPSILON, what an FP64 residual resolves); fp_sub_solve_refine:
+                              the last relative correction */
+    double refine_correction; /* last relative correction |u_k - u_(k-1)| / |u_k| */
 } fp_solve_info;
 
 /* Geometry of a composite problem: domains and Geometry2D::connect calls
@@ -137,14 +143,17 @@ int fp_sub_solve_batched_f32(fp_ctx ctx, int nprob, const fp_sub* subs, float* c
  * interface right-hand side.  precision: fp_precision. */
 int fp_cat_create(fp_ctx ctx, const fp_geometry_desc* g, int m, double tol, int tol_is_relative, int max_iter,
                   int precision, fp_cat* out);
-/* FP_F32_REFINE handles only (no reference counterpart): the refinement runs
- * sweeps with FP32 subdomain solves on the FP64 residual of the assembled
- * composite operator until the estimated relative error of the solution
- * (contraction of successive corrections) or the last relative correction is
- * at most tol for every right-hand side, or max_steps sweeps were done
- * (defaults 1e-12, 10).  fp_solve_info reports refine_steps (sweeps) and
- * refine_residual (that estimate); FP_E_NOT_CONVERGED if max_steps ended it
- * above tol. */
+/* FP_F32_REFINE and FP_F64_REFINE handles (no reference counterpart): the
+ * refinement runs sweeps (FP32 subdomain solves, or the plain FP64 solve) on
+ * the FP64 residual of the assembled composite operator until the estimated
+ * relative error of the solution (size of the last correction times the
+ * contraction of the iteration, floored at 64 * DBL_EPSILON) is at most tol
+ * for every right-hand side, or max_steps sweeps were done (defaults: tol
+ * 1e-12; max_steps 10 for FP_F32_REFINE, 2 = one correction for
+ * FP_F64_REFINE).  fp_solve_info reports refine_steps (sweeps),
+ * refine_residual (that estimate) and refine_correction;
+ * FP_E_NOT_CONVERGED if max_steps ended it above tol.  A captured
+ * FP_F64_REFINE solve takes exactly max_steps - 1 corrections. */
 int fp_cat_set_refinement(fp_cat cat, double tol, int max_steps);
 void fp_cat_destroy(fp_cat cat);
 int  fp_cat_root(fp_cat cat);
@@ -163,11 +172,15 @@ int fp_cat_solve_into(fp_cat cat, const double* const* d_rhs, double* const* d_o
                       int nrhs, void* stream, fp_solve_info* info);
 /* Captured composite solve (no reference counterpart; the reference has no
  * device): records the whole FP64 solve of fp_cat_solve on these fields and
- * ghost arrays into a HIP graph.  Covered: a root with leaf children (depth
- * two, e.g. the cross) or one domain, with the explicit-inverse interface
- * preconditioner.  The captured solve starts GMRES from that inverse's
- * product and takes no further Krylov steps; its residual check is recorded
- * too.  stream must not be the null stream.  Nothing is computed at capture. */
+ * ghost arrays into a HIP graph (FP_F64 and FP_F64_REFINE handles).  Covered:
+ * a root with leaf children (depth two, e.g. the cross) or one domain, with
+ * the explicit-inverse interface preconditioner.  The captured solve starts
+ * GMRES from that inverse's product and takes no further Krylov steps; its
+ * residual check is recorded too.  stream must not be the null stream.
+ * Nothing is computed at capture.  The graph is bound to the handle's
+ * workspace: a later solve with more right-hand sides than captured
+ * reallocates it and destroys the graph (fp_cat_graph_launch then returns
+ * FP_E_INVALID, "no captured solve"). */
 int fp_cat_graph_capture(fp_cat cat, double* const* d_fields, const double* const* d_ghost, int nrhs, void* stream);
 /* fp_cat_graph_capture of fp_cat_solve_into: replays read d_rhs and write
  * d_out (no reference counterpart). */

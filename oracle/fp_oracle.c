@@ -1215,6 +1215,112 @@ int or_cat_solve(void* h, double* const* fields, double* const* buf, int* iters,
     return 0;
 }
 
+/* Residual of the assembled composite system, accumulated in long double:
+ *   r_d = hx_d^2 F_d - A u,  F_d = f_d + ghost terms (boundary_assembly),
+ * A the scaled 5-point operator of every domain (chasing_method_base.cpp:33-46
+ * Neumann ends, poisson_solver2d.cpp:50 ratio hx^2/hy^2, periodic pairs of
+ * chasing_method2d.cpp:44-66 / poisson_fft2d.cpp:264-465) with the
+ * neighbour's adjacent line entering each Adjacented face with coefficient 1
+ * (concat_solver2d.cpp:127-128,140,150, field2.cpp:257-302). */
+static void cat_residual_ld(const or_cat* c, double* const* u, double* const* f, double* const* buf,
+                            double* const* r)
+{
+    for (int d = 0; d < c->nd; d++) {
+        const int    nx = c->nx[d], ny = c->ny[d];
+        const double hx = c->hx[d], hy = c->hy[d];
+        const double ry = hx * hx / (hy * hy);
+        const int*   t  = c->bc + 4 * d;
+        const double* ud = u[d];
+        /* assembled right-hand side, as or_cat_solve forms it */
+        memcpy(r[d], f[d], sizeof(double) * (size_t)nx * ny);
+        boundary_assembly((or_cat*)c, d, r[d], buf);
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < nx; i++) {
+            for (int j = 0; j < ny; j++) {
+                const size_t k  = (size_t)i * ny + j;
+                long double  au = -(2.0L + 2.0L * ry) * ud[k];
+                /* x neighbours */
+                if (i > 0) au += ud[k - ny];
+                else if (t[OR_XNEG] == OR_NEUMANN) au += ud[k];
+                else if (t[OR_XNEG] == OR_PERIODIC) au += ud[(size_t)(nx - 1) * ny + j];
+                else if (t[OR_XNEG] == OR_ADJACENTED) {
+                    const int nb = c->adj[4 * d + OR_XNEG];
+                    au += u[nb][(size_t)(c->nx[nb] - 1) * c->ny[nb] + j];
+                }
+                if (i < nx - 1) au += ud[k + ny];
+                else if (t[OR_XPOS] == OR_NEUMANN) au += ud[k];
+                else if (t[OR_XPOS] == OR_PERIODIC) au += ud[j];
+                else if (t[OR_XPOS] == OR_ADJACENTED) au += u[c->adj[4 * d + OR_XPOS]][j];
+                /* y neighbours */
+                if (j > 0) au += (long double)ry * ud[k - 1];
+                else if (t[OR_YNEG] == OR_NEUMANN) au += (long double)ry * ud[k];
+                else if (t[OR_YNEG] == OR_PERIODIC) au += (long double)ry * ud[k + ny - 1];
+                else if (t[OR_YNEG] == OR_ADJACENTED) {
+                    const int nb = c->adj[4 * d + OR_YNEG];
+                    au += u[nb][(size_t)i * c->ny[nb] + c->ny[nb] - 1];
+                }
+                if (j < ny - 1) au += (long double)ry * ud[k + 1];
+                else if (t[OR_YPOS] == OR_NEUMANN) au += (long double)ry * ud[k];
+                else if (t[OR_YPOS] == OR_PERIODIC) au += (long double)ry * ud[k - (ny - 1)];
+                else if (t[OR_YPOS] == OR_ADJACENTED) {
+                    const int nb = c->adj[4 * d + OR_YPOS];
+                    au += u[nb][(size_t)i * c->ny[nb]];
+                }
+                r[d][k] = (double)((long double)hx * hx * r[d][k] - au);
+            }
+        }
+    }
+}
+
+int or_cat_residual(void* h, double* const* u, double* const* fields, double* const* buf, double* const* r)
+{
+    or_cat* c = (or_cat*)h;
+    if (!c) return 1;
+    cat_residual_ld(c, u, fields, buf, r);
+    return 0;
+}
+
+int or_cat_solve_refined(void* h, double* const* fields, double* const* buf, int steps, double tol_refine,
+                         int* iters, double* res)
+{
+    or_cat* c = (or_cat*)h;
+    if (!c || steps < 0) return 1;
+    const int nd = c->nd;
+    double**  f0 = (double**)calloc(nd, sizeof(double*));
+    double**  rr = (double**)calloc(nd, sizeof(double*));
+    for (int d = 0; d < nd; d++) {
+        const size_t n = (size_t)c->nx[d] * c->ny[d];
+        f0[d] = (double*)malloc(sizeof(double) * n);
+        rr[d] = (double*)malloc(sizeof(double) * n);
+        memcpy(f0[d], fields[d], sizeof(double) * n);
+    }
+    int st = or_cat_solve(h, fields, buf, iters, res);
+    const double tol0 = c->tol;
+    const int    rel0 = c->tol_rel;
+    for (int s = 0; s < steps && st == 0; s++) {
+        cat_residual_ld(c, fields, f0, buf, rr);
+        /* A delta = r: r is already scaled by hx^2 and assembled, so it enters
+         * or_cat_solve as a raw right-hand side r / hx^2 with no ghost data */
+        for (int d = 0; d < nd; d++) {
+            const size_t n = (size_t)c->nx[d] * c->ny[d];
+            const double s2 = 1.0 / (c->hx[d] * c->hx[d]);
+            for (size_t i = 0; i < n; i++) rr[d][i] *= s2;
+        }
+        c->tol     = tol_refine;
+        c->tol_rel = 1;
+        st         = or_cat_solve(h, rr, NULL, NULL, NULL);
+        c->tol     = tol0;
+        c->tol_rel = rel0;
+        for (int d = 0; d < nd; d++) {
+            const size_t n = (size_t)c->nx[d] * c->ny[d];
+            for (size_t i = 0; i < n; i++) fields[d][i] += rr[d][i];
+        }
+    }
+    for (int d = 0; d < nd; d++) { free(f0[d]); free(rr[d]); }
+    free(f0); free(rr);
+    return st;
+}
+
 void or_cat_destroy(void* h)
 {
     or_cat* c = (or_cat*)h;

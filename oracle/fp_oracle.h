@@ -59,6 +59,15 @@ int   or_cat_tree(void* h, int* level, int* parent, int* parent_loc);
 /* fields: nd raw RHS arrays (overwritten with the solution); buf: nd*4 ghost-centre
  * data arrays (NULL allowed for faces without data).  iters/res: last GMRES stats. */
 int   or_cat_solve(void* h, double* const* fields, double* const* buf, int* iters, double* res);
+/* Residual r_d = hx_d^2 F_d - A u of the assembled composite system, in long
+ * double (F_d = raw f_d plus the ghost terms of buf). */
+int   or_cat_residual(void* h, double* const* u, double* const* fields, double* const* buf, double* const* r);
+/* or_cat_solve followed by `steps` steps of iterative refinement
+ * u += A^-1 (hx^2 F - A u) with that long-double residual, the correction
+ * solves run with GMRES to relative tol_refine.  Converges to the discrete
+ * solution to about FP64 rounding, independently of the GMRES tolerance. */
+int   or_cat_solve_refined(void* h, double* const* fields, double* const* buf, int steps, double tol_refine,
+                           int* iters, double* res);
 /* Copy the dense Schur block of child `child` (cn x cn, row-major value(j,i)). */
 int   or_cat_schur(void* h, int child, double* out, int* cn);
 void  or_cat_destroy(void* h);

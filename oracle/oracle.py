@@ -53,6 +53,10 @@ def lib():
         L.or_cat_tree.argtypes = [ctypes.c_void_p, _ip, _ip, _ip]
         L.or_cat_solve.argtypes = [ctypes.c_void_p, ctypes.POINTER(_dp), ctypes.POINTER(_dp), _ip, _dp]
         L.or_cat_schur.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _ip]
+        L.or_cat_residual.argtypes = [ctypes.c_void_p, ctypes.POINTER(_dp), ctypes.POINTER(_dp),
+                                      ctypes.POINTER(_dp), ctypes.POINTER(_dp)]
+        L.or_cat_solve_refined.argtypes = [ctypes.c_void_p, ctypes.POINTER(_dp), ctypes.POINTER(_dp), ctypes.c_int,
+                                           ctypes.c_double, _ip, _dp]
         L.or_cat_destroy.argtypes = [ctypes.c_void_p]
         L.or_cat_destroy.restype = None
         L.or_gmres_mock.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_int,
@@ -176,26 +180,56 @@ class Composite:
         lib().or_cat_schur(self.h, child, _p(out), ctypes.byref(cn))
         return out
 
-    def solve(self, fields, buffers=None):
-        """fields: list of raw RHS arrays; buffers: dict (dom, face) -> ghost data.
-        Returns (solutions, gmres_iterations, final_residual)."""
-        fs = [_f64(f).copy() for f in fields]
-        fptr = (_dp * self.nd)(*[_p(f) for f in fs])
+    def _buffers(self, buffers):
         keep = []
         bptr = (_dp * (4 * self.nd))()
         for (d, face), g in (buffers or {}).items():
             g = _f64(g)
             keep.append(g)
             bptr[4 * d + face] = _p(g)
+        return bptr, keep
+
+    def solve(self, fields, buffers=None, refine=0, tol_refine=1e-10):
+        """fields: list of raw RHS arrays; buffers: dict (dom, face) -> ghost data.
+        refine > 0: that many refinement steps with the long-double composite
+        residual after the solve (or_cat_solve_refined; converged to about FP64
+        rounding).  Returns (solutions, gmres_iterations, final_residual)."""
+        fs = [_f64(f).copy() for f in fields]
+        fptr = (_dp * self.nd)(*[_p(f) for f in fs])
+        bptr, keep = self._buffers(buffers)
         it = ctypes.c_int(0)
         res = ctypes.c_double(0)
-        assert lib().or_cat_solve(self.h, fptr, bptr, ctypes.byref(it), ctypes.byref(res)) == 0
+        if refine > 0:
+            rc = lib().or_cat_solve_refined(self.h, fptr, bptr, int(refine), float(tol_refine), ctypes.byref(it),
+                                            ctypes.byref(res))
+        else:
+            rc = lib().or_cat_solve(self.h, fptr, bptr, ctypes.byref(it), ctypes.byref(res))
+        assert rc == 0
         return fs, it.value, res.value
+
+    def residual(self, u, fields, buffers=None):
+        """hx^2 F - A u of the assembled composite system (long-double sums)."""
+        us = [_f64(x) for x in u]
+        fs = [_f64(f) for f in fields]
+        rs = [np.empty_like(f) for f in fs]
+        bptr, keep = self._buffers(buffers)
+        assert lib().or_cat_residual(self.h, (_dp * self.nd)(*[_p(x) for x in us]),
+                                     (_dp * self.nd)(*[_p(f) for f in fs]), bptr,
+                                     (_dp * self.nd)(*[_p(r) for r in rs])) == 0
+        return rs
 
     def __del__(self):
         if getattr(self, "h", None):
             lib().or_cat_destroy(self.h)
             self.h = None
+
+
+def reference_solution(domains, conn, fields, buffers=None, refine=2, tol=1e-13, m=40):
+    """The converged FP64 composite solution the parity tests compare with:
+    interface GMRES(m) to relative tol, then `refine` steps of refinement with
+    the long-double composite residual."""
+    cat = Composite(domains, conn, m=m, tol=tol, tol_is_relative=True, maxiter=2000, schur_mode=SCHUR_SPECTRAL)
+    return cat.solve(fields, buffers, refine=refine)[0]
 
 
 def gmres_mock(ainv, S, b, m, tol, maxiter):

@@ -45,9 +45,8 @@ def test_cross_graph_replays_match_eager(ctx, n):
         want = [e.cpu().numpy() for e in eager]
         assert info["status"] == 0 and info["rel_residual"] <= 1e-12, info
         assert rel(got, want) < 1e-13, k
-        ref, _, _ = O.Composite(doms, conn, tol=1e-14, tol_is_relative=True, maxiter=400,
-                                schur_mode=O.SCHUR_SPECTRAL).solve(fs)
-        assert rel(got, ref) < 1e-10, k
+        ref = O.reference_solution(doms, conn, fs)
+        assert rel(got, ref) < 1e-12, k
 
 
 def test_single_domain_graph(ctx):
@@ -65,7 +64,7 @@ def test_single_domain_graph(ctx):
             bound[0].copy_(torch.from_numpy(np.ascontiguousarray(fields[0])).to(device()))
             info = cat.graph_launch(stream=stream)
             torch.cuda.synchronize()
-            ref, _, _ = O.Composite(doms, conn).solve(fields)
+            ref = O.reference_solution(doms, conn, fields)
             assert rel([bound[0].cpu().numpy()], ref) < 1e-12
             assert info["status"] == 0
 
@@ -175,12 +174,77 @@ def test_tshape_graph_and_multi_rhs(ctx):
         singles.append(want)
         assert info["status"] == 0 and info["rel_residual"] <= 1e-12, info
         assert rel(got, want) < 1e-13, k
-        ref, _, _ = O.Composite(doms, conn, tol=1e-14, tol_is_relative=True, maxiter=400,
-                                schur_mode=O.SCHUR_SPECTRAL).solve(fs)
-        assert rel(got, ref) < 1e-10, k
+        ref = O.reference_solution(doms, conn, fs)
+        assert rel(got, ref) < 1e-12, k
     multi = fpa.ConcatPoissonSolver2D(ctx, doms, conn, m=40, tol=1e-12, tol_is_relative=True, max_iter=100)
     ts = [to_dev(np.stack([rhs[r][d] for r in range(3)])) for d in range(len(doms))]
     multi.solve(ts)
     torch.cuda.synchronize()
     for r in range(3):
         assert rel([t[r].cpu().numpy() for t in ts], singles[r]) < 1e-12, r
+
+
+def test_graph_dropped_when_workspace_grows(ctx):
+    """A solve with more right-hand sides than were captured reallocates the
+    workspace the graph was recorded on: the graph is destroyed with it, so a
+    later replay reports that no solve is captured instead of replaying into
+    freed memory; a new capture works again."""
+    import torch
+    import focalors_poisson_amd as fpa
+    from focalors_poisson_amd.cases import cross
+    banner("graph dropped when the workspace grows")
+    doms, conn, fields, _ = cross(32, seed=21, nrhs=4)
+    cat = fpa.ConcatPoissonSolver2D(ctx, doms, conn, m=40, tol=1e-12, tol_is_relative=True, max_iter=100)
+    stream = torch.cuda.Stream()
+    bound = [to_dev(f[0]) for f in fields]
+    with torch.cuda.stream(stream):
+        cat.graph_capture(bound, stream=stream)
+    many = [to_dev(f) for f in fields]
+    cat.solve(many)
+    torch.cuda.synchronize()
+    with pytest.raises(fpa.FPError) as e:
+        cat.graph_launch(stream=stream)
+    assert e.value.status == 1
+    with torch.cuda.stream(stream):
+        cat.graph_capture(bound, stream=stream)
+        for b, f in zip(bound, fields):
+            b.copy_(torch.from_numpy(np.ascontiguousarray(f[0])).to(device()))
+        info = cat.graph_launch(stream=stream)
+    torch.cuda.synchronize()
+    assert info["status"] == 0
+    assert rel([b.cpu().numpy() for b in bound], [m[0].cpu().numpy() for m in many]) < 1e-13
+
+
+@pytest.mark.parametrize("n", [64, 256])
+def test_refined_graph_replays(ctx, n):
+    """FP_F64_REFINE captured: the FP64 solve, the FP64 composite residual,
+    one correction solve and the update in one graph; replays match the eager
+    refined solve and the converged oracle at 1e-12."""
+    import torch
+    import focalors_poisson_amd as fpa
+    from focalors_poisson_amd.cases import cross
+    banner("refined graph n=%d" % n)
+    cases = [cross(n, seed=sd) for sd in (31, 32)]
+    doms, conn = cases[0][0], cases[0][1]
+    cat = fpa.ConcatPoissonSolver2D(ctx, doms, conn, m=40, tol=1e-12, tol_is_relative=True, max_iter=100,
+                                    precision=fpa.FP_F64_REFINE)
+    stream = torch.cuda.Stream()
+    rhs = [torch.empty(f[0].shape, dtype=torch.float64, device=device()) for f in cases[0][2]]
+    out = [torch.empty_like(r) for r in rhs]
+    with torch.cuda.stream(stream):
+        cat.graph_capture(out, stream=stream, rhs=rhs)
+    for k, (_, _, fields, _) in enumerate(cases):
+        fs = [f[0] for f in fields]
+        eager = [to_dev(f) for f in fs]
+        einfo = cat.solve(eager)
+        torch.cuda.synchronize()
+        assert einfo["refine_steps"] == 2, einfo
+        with torch.cuda.stream(stream):
+            for r, f in zip(rhs, fs):
+                r.copy_(torch.from_numpy(f).to(device()))
+            info = cat.graph_launch(stream=stream)
+        torch.cuda.synchronize()
+        assert info["status"] == 0 and info["refine_steps"] == 2, info
+        got = [o.cpu().numpy() for o in out]
+        assert rel(got, [e.cpu().numpy() for e in eager]) < 1e-14, k
+        assert rel(got, O.reference_solution(doms, conn, fs)) < 1e-12, k

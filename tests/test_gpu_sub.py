@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from gpu_util import banner, device, rel, to_dev
+from gpu_util import banner, device, rel, sub_reference, to_dev
 
 pytestmark = pytest.mark.gpu
 D, N, P, A = 0, 1, 2, 3
@@ -267,9 +267,9 @@ def test_long_prime_47_length(ctx):
     s2 = fpa.PoissonSolver2D(ctx, 8, ny, h, h, (D, D, D, D))
     t = to_dev(g)
     banner("fp32 refine ny=3054")
-    s2.solve_refine(t, tol=1e-12, max_steps=10)
+    s2.solve_refine(t, tol=1e-13, max_steps=10)
     torch.cuda.synchronize()
-    assert rel(t.cpu().numpy(), O.sub_solve(g, h, h, (D, D, D, D))) < 1e-10
+    assert rel(t.cpu().numpy(), sub_reference(g, h, h, (D, D, D, D))) < 1e-12
 
 
 @pytest.mark.parametrize("form", [2, 3])
@@ -286,22 +286,23 @@ def test_odd_prime_forms_fp32_refine(ctx, monkeypatch, form):
     s = fpa.PoissonSolver2D(ctx, nx, ny, h, h, (D, D, D, D))
     t = to_dev(f)
     banner("odd form %d fp32 refine" % form)
-    s.solve_refine(t, tol=1e-12, max_steps=10)
+    s.solve_refine(t, tol=1e-13, max_steps=10)
     torch.cuda.synchronize()
-    assert rel(t.cpu().numpy(), O.sub_solve(f, h, h, (D, D, D, D))) < 1e-10
+    assert rel(t.cpu().numpy(), sub_reference(f, h, h, (D, D, D, D))) < 1e-12
 
 
 @pytest.mark.parametrize("nx,ny,bc", [(64, 64, (D, D, D, D)), (200, 131, (N, D, A, N)), (512, 1024, (D, A, N, N)),
                                       (1024, 1024, (A, A, A, A))])
 def test_fp32_refine_reaches_fp64_oracle(ctx, nx, ny, bc):
     """FP32 inner solves + FP64 refinement: every step's error vs the FP64
-    oracle is printed; the final one meets the FP64 bar of 1e-10."""
+    oracle is printed; the final one meets the FP64 bar of 1e-12 against the
+    converged oracle."""
     import torch
     import focalors_poisson_amd as fpa
     rng = np.random.default_rng(nx + 3 * ny)
     h = 1.0 / max(nx, ny)
     f = rng.standard_normal((nx, ny)) * h * h
-    ref = O.sub_solve(f, h, h, bc)
+    ref = sub_reference(f, h, h, bc)
     s = fpa.PoissonSolver2D(ctx, nx, ny, h, h, bc)
     errs = []
     for steps in (1, 2, 3, 6):
@@ -318,7 +319,7 @@ def test_fp32_refine_reaches_fp64_oracle(ctx, nx, ny, bc):
     torch.cuda.synchronize()
     err = rel(t.cpu().numpy(), ref)
     banner("refine nx=%d ny=%d converged in %d steps, rel err %.3e" % (nx, ny, info["refine_steps"], err))
-    assert err < 1e-10 and info["refine_residual"] <= 1e-13
+    assert err < 1e-12 and info["refine_residual"] <= 1e-13
 
 
 # ---- periodic y (PoissonFFT2D_PP, poisson_fft2d.cpp:264-465) ----
@@ -364,7 +365,9 @@ def test_periodic_y_singular_gauge(ctx):
     f *= h * h
     bc = (N, N, P, P)
     banner("periodic y singular")
-    assert rel(gpu_solve(ctx, f, h, h, bc), O.sub_solve(f, h, h, bc)) < 1e-11
+    err = rel(gpu_solve(ctx, f, h, h, bc), O.sub_solve(f, h, h, bc))
+    banner("periodic y singular rel-L2 %.3e" % err)
+    assert err < 1e-12
 
 
 # ---- periodic x (ChasingMethodBase::chasing_periodic_*, chasing_method_base.cpp:178-285) ----
@@ -378,7 +381,9 @@ def test_periodic_x_solve(ctx, nx, ny, bc):
     h = 1.0 / max(nx, ny)
     f = rng.standard_normal((nx, ny)) * h * h
     banner("periodic x nx=%d ny=%d bc=%s" % (nx, ny, bc))
-    assert rel(gpu_solve(ctx, f, h, h, bc), O.sub_solve(f, h, h, bc)) < 1e-11
+    err = rel(gpu_solve(ctx, f, h, h, bc), sub_reference(f, h, h, bc))
+    banner("periodic x rel-L2 %.3e" % err)
+    assert err < 1e-12
 
 
 @pytest.mark.parametrize("nx,ny,bc", [(12, 16, (P, P, P, P)), (12, 9, (P, P, N, N)), (256, 256, (P, P, P, P))])
@@ -391,7 +396,9 @@ def test_periodic_x_singular_gauge(ctx, nx, ny, bc):
     f -= f.mean()
     f *= h * h
     banner("periodic x singular nx=%d ny=%d bc=%s" % (nx, ny, bc))
-    assert rel(gpu_solve(ctx, f, h, h, bc), O.sub_solve(f, h, h, bc)) < 1e-10
+    err = rel(gpu_solve(ctx, f, h, h, bc), O.sub_solve(f, h, h, bc))
+    banner("periodic x singular rel-L2 %.3e" % err)
+    assert err < 1e-12
 
 
 @pytest.mark.parametrize("bc", [(P, P, D, N), (D, N, P, P)])
@@ -406,9 +413,9 @@ def test_periodic_fp32_refine(ctx, bc):
     banner("periodic fp32 refine bc=%s" % (bc,))
     s = fpa.PoissonSolver2D(ctx, nx, ny, h, h, bc)
     t = to_dev(f)
-    info = s.solve_refine(t, tol=1e-12, max_steps=12)
+    info = s.solve_refine(t, tol=1e-13, max_steps=12)
     torch.cuda.synchronize()
-    assert rel(t.cpu().numpy(), O.sub_solve(f, h, h, bc)) < 1e-10, info
+    assert rel(t.cpu().numpy(), sub_reference(f, h, h, bc)) < 1e-12, info
 
 
 def test_batched_sixty_with_periodic(ctx):
@@ -433,7 +440,7 @@ def test_batched_sixty_with_periodic(ctx):
     fpa.solve_batched(ctx, solvers, fields)
     torch.cuda.synchronize()
     for f, t, (h, bc) in zip(hosts, fields, specs):
-        assert rel(t.cpu().numpy(), O.sub_solve(f, h, h, bc)) < 1e-11, (f.shape, bc)
+        assert rel(t.cpu().numpy(), sub_reference(f, h, h, bc)) < 1e-12, (f.shape, bc)
 
 
 def test_batched_tables_queued_back_to_back(ctx):
@@ -466,7 +473,7 @@ def test_batched_tables_queued_back_to_back(ctx):
     torch.cuda.synchronize()
     for it in range(10):
         for t, (f, h, bc) in zip(sets[it], batches[it % 2][1]):
-            assert rel(t.cpu().numpy(), O.sub_solve(f, h, h, bc)) < 1e-11, (it, f.shape)
+            assert rel(t.cpu().numpy(), sub_reference(f, h, h, bc)) < 1e-12, (it, f.shape)
 
 
 @pytest.mark.parametrize("f32", [False, True])
@@ -480,4 +487,6 @@ def test_dirichlet_4096_long_bluestein(ctx, f32):
     f = rng.standard_normal((n, n)) * h * h
     banner("dirichlet 4096 f32=%d" % f32)
     got = gpu_solve(ctx, f, h, h, (D, D, D, D), torch.float32 if f32 else None)
-    assert rel(got, O.sub_solve(f, h, h, (D, D, D, D))) < (1e-4 if f32 else 1e-11)
+    err = rel(got, sub_reference(f, h, h, (D, D, D, D)))
+    banner("dirichlet 4096 f32=%d rel-L2 vs the converged oracle %.3e" % (f32, err))
+    assert err < (1e-4 if f32 else 1e-12)

@@ -75,6 +75,50 @@ def test_dd_is_scipy_dst1_and_nn_is_dct2():
     assert np.allclose(np.abs(got[:, ::-1]), np.abs(c), atol=1e-13)
 
 
+def closed_form_basis(kind, ny):
+    """The forward transforms' matrices written out from the trigonometric
+    sums of poisson_fft2d.cpp (no FFT): row j = coefficient j.
+      NN (:102-117): -sqrt(2/N) cos(pi (N-1-j)(2n+1) / 2N) for j <= N-2 and
+                     sqrt(1/N) (the mean) for j = N-1;
+      DN (:241-250): (-1)^j 2/sqrt(2N+1) sin(pi (n+1)(2(N-j)-1) / (2N+1));
+      ND (:180-188): (-1)^j 2/sqrt(2N+1) cos(pi (2n+1)(2(N-j)-1) / (2(2N+1)));
+      DD (:36-44):   sqrt(2/(N+1)) sin(pi (j+1)(n+1) / (N+1))."""
+    j = np.arange(ny, dtype=np.longdouble)[:, None]
+    n = np.arange(ny, dtype=np.longdouble)[None, :]
+    pi = np.longdouble(np.pi)
+    N_ = np.longdouble(ny)
+    sgn = np.where(np.arange(ny) % 2 == 0, 1.0, -1.0)[:, None]
+    if kind == "NN":
+        M = -np.sqrt(2 / N_) * np.cos(pi * (N_ - 1 - j) * (2 * n + 1) / (2 * N_))
+        M[-1, :] = np.sqrt(1 / N_)
+    elif kind == "DN":
+        M = sgn * 2 / np.sqrt(2 * N_ + 1) * np.sin(pi * (n + 1) * (2 * (N_ - j) - 1) / (2 * N_ + 1))
+    elif kind == "ND":
+        M = sgn * 2 / np.sqrt(2 * N_ + 1) * np.cos(pi * (2 * n + 1) * (2 * (N_ - j) - 1) / (2 * (2 * N_ + 1)))
+    else:
+        M = np.sqrt(2 / (N_ + 1)) * np.sin(pi * (j + 1) * (n + 1) / (N_ + 1))
+    return M.astype(np.float64)
+
+
+@pytest.mark.parametrize("kind,ny", [(k, n) for k in ["NN", "DN", "ND", "DD"]
+                                     for n in [1, 2, 3, 5, 8, 13, 64, 100, 255, 512, 1024]])
+def test_transform_signs_closed_form(kind, ny):
+    """Coefficient for coefficient, signs included: the oracle's FFT-based
+    transforms against the closed forms evaluated as plain sums (a flipped or
+    alternating sign of any mode, a reversed mode order or a wrong scale
+    fails here; the eigen-relation test above cannot see signs)."""
+    bcp = {"NN": (N, N), "DN": (D, N), "ND": (N, D), "DD": (D, D)}[kind]
+    rng = np.random.default_rng(ny + 7)
+    x = rng.standard_normal((3, ny))
+    M = closed_form_basis(kind, ny)
+    ref = x @ M.T
+    got = O.transform(bcp[0], bcp[1], x)
+    assert np.abs(got - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max()), kind
+    # the inverse (transform_transpose) is the transpose of the same matrix
+    back = O.transform(bcp[0], bcp[1], ref, inverse=True)
+    assert np.abs(back - ref @ M).max() <= 1e-12 * max(1.0, np.abs(x).max())
+
+
 @pytest.mark.parametrize("ny", [4, 10, 64])
 def test_pp_transform_closed_form(ny):
     """The periodic transform's matrix (poisson_fft2d.cpp:367-403) is
@@ -284,3 +328,82 @@ def test_gmres_mock_semantics(loc):
     x2 = _krylov_min(Bm, c, x1, 2)
     assert np.allclose(resvec[0], np.linalg.norm(c - Bm @ c))
     assert np.allclose(x, x2, rtol=1e-10, atol=1e-10 * np.abs(x2).max())
+
+
+def refined_sparse(doms, conn, fs, bufs=None, steps=3):
+    """The assembled sparse system solved by LU and refined with residuals
+    summed in long double: the discrete solution to about FP64 rounding,
+    independently of the oracle's transforms, sweeps, Schur blocks and GMRES."""
+    import scipy.sparse.linalg as spla
+    M, rhs, offs = discrete.assemble(doms, conn, fs, bufs)
+    C = M.tocoo()
+    vl = C.data.astype(np.longdouble)
+    lu = spla.splu(M.tocsc())
+    u = lu.solve(rhs)
+    for _ in range(steps):
+        y = np.zeros(len(u), dtype=np.longdouble)
+        np.add.at(y, C.row, vl * u.astype(np.longdouble)[C.col])
+        u = u + lu.solve((rhs.astype(np.longdouble) - y).astype(np.float64))
+    return [u[offs[d]:offs[d + 1]].reshape(doms[d][0], doms[d][1]) for d in range(len(doms))]
+
+
+def _cases_for_residual():
+    g, doms, conn, fs, bufs = cross32()
+    yield doms, conn, fs, bufs
+    rng = np.random.default_rng(41)
+    h = 1.0 / 24
+    tdoms = [(36, 24, h, h, (A, A, D, A)), (14, 24, h, h, (N, A, D, N)), (20, 24, h, h, (A, D, N, D)),
+             (36, 18, h, h, (D, N, A, N))]
+    tconn = [(0, 0, 1), (0, 1, 2), (0, 3, 3)]
+    yield tdoms, tconn, [rng.standard_normal((d[0], d[1])) for d in tdoms], {(1, 0): rng.standard_normal(24),
+                                                                                 (3, 3): rng.standard_normal(36)}
+    pdoms = [(16, 12, 0.5, 0.25, (P, P, D, A)), (16, 10, 0.5, 0.25, (P, P, A, N))]
+    yield pdoms, [(0, 3, 1)], [rng.standard_normal((d[0], d[1])) for d in pdoms], {(0, 2): rng.standard_normal(16)}
+
+
+@pytest.mark.parametrize("case", [0, 1, 2])
+def test_cat_residual_is_the_assembled_system(case):
+    """or_cat_residual = hx^2 F - A u of discrete.assemble (Neumann and
+    periodic faces, ghost data on Dirichlet and Neumann faces, hx != hy)."""
+    doms, conn, fs, bufs = list(_cases_for_residual())[case]
+    rng = np.random.default_rng(case)
+    u = [rng.standard_normal((d[0], d[1])) for d in doms]
+    M, rhs, offs = discrete.assemble(doms, conn, fs, bufs)
+    ref = rhs - M @ np.concatenate([x.ravel() for x in u])
+    got = O.Composite(doms, conn).residual(u, fs, bufs)
+    assert np.abs(np.concatenate([r.ravel() for r in got]) - ref).max() < 1e-13 * np.abs(ref).max()
+    # a wrong sign of the coupling or of a ghost term shows at the interface
+    # and face lines: the residual of the exact solution is zero
+    ex = refined_sparse(doms, conn, fs, bufs)
+    r0 = O.Composite(doms, conn).residual(ex, fs, bufs)
+    assert max(np.abs(r).max() for r in r0) < 1e-13 * max(1.0, np.abs(rhs).max())
+
+
+@pytest.mark.parametrize("case", [0, 1])
+def test_refined_oracle_reaches_the_discrete_solution(case):
+    """The parity reference (or_cat_solve_refined): GMRES to 1e-13 leaves
+    the FP64 floor of the composite solve, one refinement step with the
+    long-double residual removes it; it then matches the refined sparse
+    solve to about rounding, far below the 1e-12 bar."""
+    doms, conn, fs, bufs = list(_cases_for_residual())[case]
+    ex = refined_sparse(doms, conn, fs, bufs)
+    cat = O.Composite(doms, conn, m=40, tol=1e-13, tol_is_relative=True, maxiter=1000,
+                      schur_mode=O.SCHUR_SPECTRAL)
+    u2, _, _ = cat.solve(fs, bufs, refine=2)
+    assert rel(u2, ex) < 1e-15
+    assert rel(O.reference_solution(doms, conn, fs, bufs), ex) < 1e-15
+
+
+def test_refined_oracle_cross_128():
+    """At n = 128 the unrefined FP64 composite solve is about 6e-13 off the
+    discrete solution; the refined one agrees with the refined sparse solve."""
+    from focalors_poisson_amd.cases import cross
+    doms, conn, fields, _ = cross(128, seed=1)
+    fs = [f[0] for f in fields]
+    ex = refined_sparse(doms, conn, fs)
+    cat = O.Composite(doms, conn, m=40, tol=1e-13, tol_is_relative=True, maxiter=1000,
+                      schur_mode=O.SCHUR_SPECTRAL)
+    u0, _, _ = cat.solve(fs)
+    u1, _, _ = cat.solve(fs, refine=1)
+    assert rel(u0, ex) > 1e-14  # the floor is real
+    assert rel(u1, ex) < 1e-15
