@@ -101,3 +101,17 @@ def multiscale_rhs(nx, ny, hx, hy, ox, oy, seed=4, modes=32):
         px, py = rng.uniform(0, 2 * math.pi, 2)
         f += a * np.sin(kx * math.pi * X + px) * np.sin(ky * math.pi * Y + py)
     return f
+
+
+def smooth_target(doms, offsets, seed=6, nrhs=1):
+    """A seeded smooth field per domain (the adjoint's u*): per right-hand
+    side sin(a pi x + p) cos(b pi y + q) with a, b in [1, 3] and phases in
+    [0, 2 pi).  Arrays of shape (nrhs, nx, ny)."""
+    rng = np.random.default_rng(seed)
+    prm = [(rng.uniform(1, 3), rng.uniform(1, 3), rng.uniform(0, 2 * math.pi), rng.uniform(0, 2 * math.pi))
+           for _ in range(nrhs)]
+    out = []
+    for (nx, ny, hx, hy, _), (ox, oy) in zip(doms, offsets):
+        X, Y = centres(nx, ny, hx, hy, ox, oy)
+        out.append(np.stack([np.sin(a * math.pi * X + p) * np.cos(b * math.pi * Y + q) for a, b, p, q in prm]))
+    return out

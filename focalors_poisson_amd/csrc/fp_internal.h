@@ -172,11 +172,12 @@ struct SubPlan {
     int    bc[4] = {0, 0, 0, 0};
     int    kind = T_DD;
     bool   no_dirichlet = false;
-    std::vector<double>   x_diag;   // host copy, reference order
+    std::vector<double>   x_diag;   // host copy, reference order: -2 - delta
+    std::vector<double>   delta;    // per mode: -(x_diag + 2) = (hx/hy)^2 4 sin^2(.) >= 0, to full precision
     TransformPlan<double> t64;
     TransformPlan<float>  t32;
-    DevBuf<double>        xd;       // x_diag on the device
-    DevBuf<double>        pst;      // ny*tri_p: Thomas pivot preceding each row chunk (mode-major)
+    DevBuf<double>        xd;       // delta on the device
+    DevBuf<double>        pst;      // ny*tri_p: pivot excess e = 1 + 1/pivot before each row chunk (mode-major)
     int                   tri_ch = 16, tri_p = 0;        // rows per chunk of the x sweep, chunks per mode
     int                   mb_field = 4, mb_lines = 4;    // modes per workgroup of the two sweeps
     // periodic x: per mode A_D^-1 e_0 then A_D^-1 e_{nx-1} (2 nx ny, field layout)
@@ -258,8 +259,8 @@ struct LinesArgs {
     const T*  lines;    // per rhs: Xh0 (ny) | Xh1 (ny) | a (nx) | b (nx)
     long long ldl;      // 2 ny + 2 nx
     const T*  phi;      // phi0 (ny) | phi1 (ny)
-    const double* xd;   // filled by the launcher from the plan: x_diag,
-    const double* pst;  // chunk-start pivots,
+    const double* xd;   // filled by the launcher from the plan: delta,
+    const double* pst;  // chunk-start pivot excesses,
     int       ns, ne;   // Neumann x ends
     T*        xu;       // per rhs: coefficient rows 0 and nx-1 (2 ny)
     T*        colpart;  // per rhs, per workgroup: 2 x nx column partial sums

@@ -50,19 +50,31 @@ void SubPlan::build(int nx_, int ny_, double hx_, double hy_, const int bc_[4])
     FP_HIP(hipGetDevice(&dev));
     // PoissonSolver2DBase::cal_lambda + PoissonSolver2D::init
     // (poisson_solver2d_base.cpp:28-62, poisson_solver2d.cpp:43-51)
-    const double PI = 3.14159265358979323846;
+    // The reference forms x_diag = -2 + r (lambda - 2) with lambda a cosine,
+    // which cancels to a few digits for the lowest modes; here lambda - 2 =
+    // -4 sin^2(a) with the half angle a of each kind, in long double:
+    //   DD  lambda = 2 cos(pi i/(N+1))         a = pi i / (2 (N+1))
+    //   NN  lambda = -2 cos(pi i/N)            a = pi (N - i) / (2N)
+    //   DN, ND  lambda = -2 cos(2 pi i/(2N+1))  a = pi (2N+1-2i) / (2 (2N+1))
+    //   PP  lambda = -2 cos(2 pi floor(i/2)/N) a = pi (N - 2 floor(i/2)) / (2N)
+    // and x_diag = -2 - delta, delta = r 4 sin^2(a) >= 0.
+    const long double PI = 3.141592653589793238462643383279502884L;
+    const long double r  = (long double)hx * hx / ((long double)hy * hy);
     x_diag.resize(ny);
+    delta.resize(ny);
     for (int j = 0; j < ny; j++) {
-        const int i = j + 1;
-        double    lam;
-        if (kind == T_PP) lam = -2.0 * std::cos(2.0 * PI / ny * std::floor(i / 2.0));
-        else if (kind == T_NN) lam = -2.0 * std::cos(PI / ny * i);
-        else if (kind == T_DD) lam = 2.0 * std::cos(PI / (ny + 1) * i);
-        else lam = -2.0 * std::cos(2.0 * PI * i / (2 * ny + 1));
-        x_diag[j] = -2.0 + hx * hx / hy / hy * (lam - 2.0);
+        const long double i = j + 1, N = ny;
+        long double       a;
+        if (kind == T_PP) a = PI * (N - 2.0L * (long double)((j + 1) / 2)) / (2.0L * N);
+        else if (kind == T_NN) a = PI * (N - i) / (2.0L * N);
+        else if (kind == T_DD) a = PI * i / (2.0L * (N + 1.0L));
+        else a = PI * (2.0L * N + 1.0L - 2.0L * i) / (2.0L * (2.0L * N + 1.0L));
+        const long double sa = std::sin(a);
+        delta[j]             = (double)(r * 4.0L * sa * sa);
+        x_diag[j]            = -2.0 - delta[j];
     }
     xd.alloc(ny);
-    xd.upload(x_diag.data(), ny);
+    xd.upload(delta.data(), ny);
     // periodic x (ChasingMethod2D::init, chasing_method2d.cpp:21-33 with
     // chasing_periodic_precompute, chasing_method_base.cpp:232-258): the two
     // correction vectors of every mode, Dirichlet-end Thomas in long double
@@ -71,7 +83,7 @@ void SubPlan::build(int nx_, int ny_, double hx_, double hy_, const int bc_[4])
         std::vector<double>      z(2 * (size_t)nx * ny);
         std::vector<long double> cp(nx), y(nx);
         for (int j = 0; j < ny; j++) {
-            const long double d = x_diag[j];
+            const long double d = -2.0L - (long double)delta[j];
             for (int e = 0; e < 2; e++) {
                 // forward elimination of tridiag(1, d, 1) with the unit right-hand side e_0 / e_{nx-1}
                 cp[0] = 1.0L / d;

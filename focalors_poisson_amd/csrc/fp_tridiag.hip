@@ -12,6 +12,15 @@
 // Parallel Thomas: the LU factor of tridiag(1, b_i, 1) (b_i = the mode's
 // diagonal d_j, plus 1 on a Neumann end row) has the reciprocal pivots
 //   p_0 = 1 / b_0,   p_i = 1 / (b_i - p_{i-1})       (the reference's c_i).
+// With d_j = -2 - delta_j the reference's form loses the small delta_j of the
+// lowest modes to the rounding of d_j (about 1e-10 relative error of a
+// 4096-row Dirichlet solve): here the pivots are carried as excesses
+// e_i = 1 + p_i, which satisfy, with every term non-negative,
+//   q_i = delta + e_{i-1},  p_i = -1 / (1 + q_i),  e_i = -q_i p_i   (interior rows)
+//   p_i = -1 / (delta + e_{i-1}),  e_i = -(delta + (e_{i-1} - 1)) p_i  (Neumann rows)
+// from e_{-1} = 1, and delta_j = (hx/hy)^2 4 sin^2(.) comes to full precision
+// from the eigenvalue's closed form.  The same LU factor to rounding, without
+// the cancellation.
 // The forward sweep y_i = p_i (f_i - y_{i-1}) and the back substitution
 // x_i = y_i - p_i x_{i+1} are first-order affine recurrences, so each thread
 // owns a chunk of CH rows of one mode, composes its chunk's affine map, the
@@ -116,19 +125,31 @@ __device__ __forceinline__ int xcd_block(int b, int nblocks)
     return (b & 7) * (nblocks >> 3) + (b >> 3);
 }
 
-// The CH pivots of rows i0 .. i0+CH-1 of one mode from the pivot before the
-// chunk (0 before row 0), by the host's recurrence.
+// One step of the excess recurrence (the host's, bit for bit): the
+// reciprocal pivot p of the row and the excess e = 1 + p it passes on.
+__host__ __device__ __forceinline__ void pivot_step(double delta, bool neumann_row, double& e, double& p)
+{
+    if (!neumann_row) {
+        const double q = delta + e;
+        p              = -1.0 / (1.0 + q);
+        e              = -q * p;
+    } else {
+        p = -1.0 / (delta + e);
+        e = -(delta + (e - 1.0)) * p;
+    }
+}
+
+// The CH reciprocal pivots of rows i0 .. i0+CH-1 of one mode from the pivot
+// excess before the chunk (1 before row 0), by the host's recurrence.
 template <typename T, int CH>
-__device__ __forceinline__ void chunk_pivots(T (&pv)[CH], double d, double p, int i0, int nx, int ns, int ne,
+__device__ __forceinline__ void chunk_pivots(T (&pv)[CH], double delta, double e, int i0, int nx, int ns, int ne,
                                              bool act)
 {
+    double p = 0.0;
 #pragma unroll
     for (int k = 0; k < CH; k++) {
         const int i = i0 + k;
-        if (act && i < nx) {
-            const double b = ((i == 0 && ns) || (i == nx - 1 && ne)) ? d + 1.0 : d;
-            p              = 1.0 / (b - p);
-        }
+        if (act && i < nx) pivot_step(delta, (i == 0 && ns) || (i == nx - 1 && ne), e, p);
         pv[k] = (T)p;
     }
 }
@@ -138,8 +159,8 @@ template <typename T>
 struct TriProb {
     T*            data;
     long long     ld;
-    const double* xd;   // x_diag (ny)
-    const double* pst;  // chunk-start pivots, P per mode
+    const double* xd;   // delta (ny)
+    const double* pst;  // chunk-start pivot excesses, P per mode
     int           nx, ny, P, MB, singular_last, nblk, ns, ne;  // nblk: mode blocks per right-hand side
     // periodic x (cyc): z1 = A_D^-1 e_0 and z2 = A_D^-1 e_{nx-1} per mode, laid
     // out like the field (A_D: the Dirichlet-end tridiagonal of the mode)
@@ -848,22 +869,20 @@ int sweep_blocks(const SubPlan& sp)
 
 // ---- chunk schedule and its autotuning ----
 
-// Host pivots of every mode (the reference's recurrence) sampled at the chunk
-// starts of chunk length ch: pst[j*P + c] = p_{c*ch - 1} (0 for c = 0).
+// Host pivot excesses of every mode (the recurrence above) sampled at the
+// chunk starts of chunk length ch: pst[j*P + c] = e_{c*ch - 1} (1 for c = 0).
 void SubPlan::set_chunks(int ch)
 {
     tri_ch = ch;
     tri_p  = (nx + ch - 1) / ch;
-    std::vector<double> t((size_t)ny * tri_p, 0.0);
+    std::vector<double> t((size_t)ny * tri_p, 1.0);
     const bool          ns = bc[XNEG] == NEUMANN, ne = bc[XPOS] == NEUMANN;
     for (int j = 0; j < ny; j++) {
         if (no_dirichlet && j == ny - 1) continue;  // singular mode: recurrence, no pivots
-        const double d = x_diag[j];
-        double       p = 0.0;
+        double e = 1.0, p = 0.0;
         for (int i = 0; i < nx; i++) {
-            if (i % ch == 0) t[(size_t)j * tri_p + i / ch] = p;
-            const double b = ((i == 0 && ns) || (i == nx - 1 && ne)) ? d + 1.0 : d;
-            p              = 1.0 / (b - p);
+            if (i % ch == 0) t[(size_t)j * tri_p + i / ch] = e;
+            pivot_step(delta[j], (i == 0 && ns) || (i == nx - 1 && ne), e, p);
         }
     }
     pst.alloc(t.size());

@@ -976,7 +976,10 @@ static dsolver* dsolver_create(or_cat* c, int dom)
         S->loc   = loc;
         S->cn    = (loc == OR_XNEG || loc == OR_XPOS) ? c->ny[ch] : c->nx[ch];
         S->v     = (double*)malloc(sizeof(double) * (size_t)S->cn * S->cn);
-        if (c->schur_mode == OR_SCHUR_SPECTRAL && !c->solver[ch]->gmres) schur_spectral(c, S);
+        /* the spectral form solves the child's x systems with Dirichlet/Neumann
+         * ends: a periodic x axis takes the unit solves */
+        if (c->schur_mode == OR_SCHUR_SPECTRAL && !c->solver[ch]->gmres && c->bc[4 * ch + OR_XNEG] != OR_PERIODIC)
+            schur_spectral(c, S);
         else schur_unit_solves(c, S);
     }
     return d;
@@ -1239,30 +1242,31 @@ static void cat_residual_ld(const or_cat* c, double* const* u, double* const* f,
             for (int j = 0; j < ny; j++) {
                 const size_t k  = (size_t)i * ny + j;
                 long double  au = -(2.0L + 2.0L * ry) * ud[k];
-                /* x neighbours */
+                /* x neighbours; an Adjacented face without a neighbour (a lone
+                 * domain given that type) has ghost 0, as a Dirichlet face */
                 if (i > 0) au += ud[k - ny];
                 else if (t[OR_XNEG] == OR_NEUMANN) au += ud[k];
                 else if (t[OR_XNEG] == OR_PERIODIC) au += ud[(size_t)(nx - 1) * ny + j];
-                else if (t[OR_XNEG] == OR_ADJACENTED) {
+                else if (t[OR_XNEG] == OR_ADJACENTED && c->adj[4 * d + OR_XNEG] >= 0) {
                     const int nb = c->adj[4 * d + OR_XNEG];
                     au += u[nb][(size_t)(c->nx[nb] - 1) * c->ny[nb] + j];
                 }
                 if (i < nx - 1) au += ud[k + ny];
                 else if (t[OR_XPOS] == OR_NEUMANN) au += ud[k];
                 else if (t[OR_XPOS] == OR_PERIODIC) au += ud[j];
-                else if (t[OR_XPOS] == OR_ADJACENTED) au += u[c->adj[4 * d + OR_XPOS]][j];
+                else if (t[OR_XPOS] == OR_ADJACENTED && c->adj[4 * d + OR_XPOS] >= 0) au += u[c->adj[4 * d + OR_XPOS]][j];
                 /* y neighbours */
                 if (j > 0) au += (long double)ry * ud[k - 1];
                 else if (t[OR_YNEG] == OR_NEUMANN) au += (long double)ry * ud[k];
                 else if (t[OR_YNEG] == OR_PERIODIC) au += (long double)ry * ud[k + ny - 1];
-                else if (t[OR_YNEG] == OR_ADJACENTED) {
+                else if (t[OR_YNEG] == OR_ADJACENTED && c->adj[4 * d + OR_YNEG] >= 0) {
                     const int nb = c->adj[4 * d + OR_YNEG];
                     au += u[nb][(size_t)i * c->ny[nb] + c->ny[nb] - 1];
                 }
                 if (j < ny - 1) au += (long double)ry * ud[k + 1];
                 else if (t[OR_YPOS] == OR_NEUMANN) au += (long double)ry * ud[k];
                 else if (t[OR_YPOS] == OR_PERIODIC) au += (long double)ry * ud[k - (ny - 1)];
-                else if (t[OR_YPOS] == OR_ADJACENTED) {
+                else if (t[OR_YPOS] == OR_ADJACENTED && c->adj[4 * d + OR_YPOS] >= 0) {
                     const int nb = c->adj[4 * d + OR_YPOS];
                     au += u[nb][(size_t)i * c->ny[nb]];
                 }

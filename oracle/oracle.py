@@ -224,12 +224,14 @@ class Composite:
             self.h = None
 
 
-def reference_solution(domains, conn, fields, buffers=None, refine=2, tol=1e-13, m=40):
+def reference_solution(domains, conn, fields, buffers=None, refine=2, tol=1e-11, tol_refine=1e-7, m=40):
     """The converged FP64 composite solution the parity tests compare with:
     interface GMRES(m) to relative tol, then `refine` steps of refinement with
-    the long-double composite residual."""
+    the long-double composite residual, whose correction solves run to
+    relative tol_refine (each step multiplies the error by about that
+    accuracy, so two steps reach FP64 rounding from any tol <= 1e-8)."""
     cat = Composite(domains, conn, m=m, tol=tol, tol_is_relative=True, maxiter=2000, schur_mode=SCHUR_SPECTRAL)
-    return cat.solve(fields, buffers, refine=refine)[0]
+    return cat.solve(fields, buffers, refine=refine, tol_refine=tol_refine)[0]
 
 
 def gmres_mock(ainv, S, b, m, tol, maxiter):

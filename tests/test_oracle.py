@@ -359,9 +359,12 @@ def _cases_for_residual():
                                                                                  (3, 3): rng.standard_normal(36)}
     pdoms = [(16, 12, 0.5, 0.25, (P, P, D, A)), (16, 10, 0.5, 0.25, (P, P, A, N))]
     yield pdoms, [(0, 3, 1)], [rng.standard_normal((d[0], d[1])) for d in pdoms], {(0, 2): rng.standard_normal(16)}
+    # a lone domain with an Adjacented face and no neighbour: ghost 0 there
+    sdoms = [(20, 13, 0.05, 0.05, (N, D, A, N))]
+    yield sdoms, [], [rng.standard_normal((20, 13))], {(0, 3): rng.standard_normal(20)}
 
 
-@pytest.mark.parametrize("case", [0, 1, 2])
+@pytest.mark.parametrize("case", [0, 1, 2, 3])
 def test_cat_residual_is_the_assembled_system(case):
     """or_cat_residual = hx^2 F - A u of discrete.assemble (Neumann and
     periodic faces, ghost data on Dirichlet and Neumann faces, hx != hy)."""
@@ -379,7 +382,7 @@ def test_cat_residual_is_the_assembled_system(case):
     assert max(np.abs(r).max() for r in r0) < 1e-13 * max(1.0, np.abs(rhs).max())
 
 
-@pytest.mark.parametrize("case", [0, 1])
+@pytest.mark.parametrize("case", [0, 1, 2, 3])
 def test_refined_oracle_reaches_the_discrete_solution(case):
     """The parity reference (or_cat_solve_refined): GMRES to 1e-13 leaves
     the FP64 floor of the composite solve, one refinement step with the
