@@ -32,7 +32,8 @@ def parse():
     ap.add_argument("--n", type=int, default=1024, help="centre size (arms n/2)")
     ap.add_argument("--m", type=int, default=200, help="GMRES restart length on the interface")
     ap.add_argument("--tol", type=float, default=1e-12, help="relative GMRES tolerance on the interface")
-    ap.add_argument("--oracle-tol", type=float, default=1e-13, help="relative GMRES tolerance of the CPU oracle")
+    ap.add_argument("--oracle-tol", type=float, default=1e-11,
+                    help="relative GMRES tolerance of the timed CPU oracle solve")
     ap.add_argument("--refine-tol", type=float, default=1e-11,
                     help="target estimated relative error of the FP32+refine variant")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -68,16 +69,16 @@ def time_kernel(fn, reps, torch):
 
 
 def measured_traffic(kernel, n):
-    """HBM bytes per launch of `kernel` on the 1024^2 centre, from the
-    rocprofv3 FETCH_SIZE / WRITE_SIZE runs recorded under profiles/ (null for
-    another kernel or size)."""
-    path = os.path.join(ROOT, "profiles", "centre1024_dd_traffic.json")
+    """HBM bytes per launch of `kernel` at the default size, from the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs recorded under profiles/
+    (cross1024_traffic.json; null for another kernel or size)."""
+    path = os.path.join(ROOT, "profiles", "cross1024_traffic.json")
     if n != 1024 or not os.path.exists(path):
         return None
     with open(path) as fh:
         rec = json.load(fh)
-    kinds = {"ytransform_fwd", "ytransform_inv"} if rec.get("kernel", "").startswith("ytransform") else {rec.get("kernel")}
-    return rec.get("traffic_bytes") if kernel in kinds else None
+    k = rec.get("kernels", {}).get(kernel)
+    return None if k is None else k.get("traffic_bytes")
 
 
 def centre_kernels(fpa, ctx, doms, torch, reps, nrhs=1):
@@ -98,6 +99,24 @@ def centre_kernels(fpa, ctx, doms, torch, reps, nrhs=1):
         ("tridiag", lambda: sub.chasing(buf), 2 * fbytes),
     ]:
         t = time_kernel(fn, reps, torch)
+        kern[name] = {"ms": round(t, 5), "bytes": nbytes, "gbs": round(nbytes / (t * 1e-3) / 1e9, 1)}
+    return kern
+
+
+def composite_transform(fpa, ctx, doms, conn, torch, reps, nrhs=1):
+    """Average duration of the solve's dominant launch: the batched y
+    transform of all five domains in one launch (fp_cat_transform, the same
+    kernel and shape as the solve's first and last stages), forward and
+    inverse.  Algorithmic bytes: one read and one write of every field (16 B
+    per cell)."""
+    cat = fpa.ConcatPoissonSolver2D(ctx, doms, conn, m=20, tol=1e-12, tol_is_relative=True)
+    dev = torch.device("cuda:0")
+    bufs = [torch.randn(*((d[0], d[1]) if nrhs == 1 else (nrhs, d[0], d[1])), dtype=torch.float64, device=dev)
+            for d in doms]
+    nbytes = 16.0 * sum(d[0] * d[1] for d in doms) * nrhs
+    kern = {}
+    for name, inv in (("cat_ytransform_fwd", False), ("cat_ytransform_inv", True)):
+        t = time_kernel(lambda: cat.transform(bufs, inverse=inv), reps, torch)
         kern[name] = {"ms": round(t, 5), "bytes": nbytes, "gbs": round(nbytes / (t * 1e-3) / 1e9, 1)}
     return kern
 
@@ -329,6 +348,7 @@ def main():
         return
     if a.kernels_only:
         kern = centre_kernels(fpa, ctx, doms, torch, a.kernel_reps)
+        kern.update(composite_transform(fpa, ctx, doms, conn, torch, a.kernel_reps))
         src = torch.randn(n * n, dtype=torch.float64, device=dev)
         dst = torch.empty_like(src)
         t = time_kernel(lambda: dst.copy_(src), a.kernel_reps, torch)
@@ -338,17 +358,20 @@ def main():
     cells = sum(d[0] * d[1] for d in doms)
     pristine = [torch.from_numpy(f[0]).to(dev) for f in fields]
 
-    def run_variant(precision, into=False):
-        """Eager solves.  into=False: in place (fp_cat_solve), the right-hand
-        sides refilled by one device copy per step; into=True: out of place
-        (fp_cat_solve_into), the resident right-hand sides read directly."""
+    def make(precision):
         t0 = time.perf_counter()
         cat = fpa.ConcatPoissonSolver2D(ctx, doms, conn, m=a.m, tol=a.tol, tol_is_relative=True, max_iter=100,
                                         precision=precision)
         if precision == fpa.FP_F32_REFINE:
             cat.set_refinement(tol=a.refine_tol, max_steps=10)
         torch.cuda.synchronize()
-        t_build = time.perf_counter() - t0
+        return cat, time.perf_counter() - t0
+
+    def run_variant(precision, into=False):
+        """Eager solves.  into=False: in place (fp_cat_solve), the right-hand
+        sides refilled by one device copy per step; into=True: out of place
+        (fp_cat_solve_into), the resident right-hand sides read directly."""
+        cat, t_build = make(precision)
         src, dst, work = flat_fields(torch, pristine)
         rhs = pristine if into else None  # resident device tensors, read only
         info = {}
@@ -370,11 +393,11 @@ def main():
         ms = (time.perf_counter() - t0) / a.steps * 1e3
         return {"ms": ms, "build_s": t_build, "info": dict(info), "u": [w.cpu().numpy() for w in work]}
 
-    def run_graph(into=False):
-        """FP64 solve captured once in a HIP graph and replayed per step: in
+    def run_graph(into=False, precision=fpa.FP_F64):
+        """The solve captured once in a HIP graph and replayed per step: in
         place with the right-hand sides refilled by an eager copy on the same
         stream, or (into=True) out of place from the resident right-hand sides."""
-        cat = fpa.ConcatPoissonSolver2D(ctx, doms, conn, m=a.m, tol=a.tol, tol_is_relative=True, max_iter=100)
+        cat, t_build = make(precision)
         stream = torch.cuda.Stream()
         src, dst, work = flat_fields(torch, pristine)
         rhs = pristine if into else None  # resident device tensors, read only
@@ -396,45 +419,57 @@ def main():
             ms = (time.perf_counter() - t0) / a.steps * 1e3
             info = step(wait=True)  # the residual check of a replay
         torch.cuda.synchronize()
-        return {"ms": ms, "build_s": 0.0, "info": dict(info), "u": [w.cpu().numpy() for w in work]}
+        return {"ms": ms, "build_s": t_build, "info": dict(info), "u": [w.cpu().numpy() for w in work]}
 
-    v64 = run_variant(fpa.FP_F64)
-    vg = run_graph()
-    vi = run_variant(fpa.FP_F64, into=True)
-    vgi = run_graph(into=True)
-    v32 = run_variant(fpa.FP_F32_REFINE)
+    runs = {
+        "f64": run_variant(fpa.FP_F64),
+        "f64_graph": run_graph(),
+        "f64_into": run_variant(fpa.FP_F64, into=True),
+        "f64_graph_into": run_graph(into=True),
+        "f64_refine_graph_into": run_graph(into=True, precision=fpa.FP_F64_REFINE),
+        "f32_refine": run_variant(fpa.FP_F32_REFINE),
+    }
 
-    # ---- per-kernel timing on the centre subdomain ----
-    kern = centre_kernels(fpa, ctx, doms, torch, a.kernel_reps)
-    dom_name = max(kern, key=lambda k: kern[k]["ms"])
+    # ---- the dominant launch of the solve, timed alone: the batched y
+    # transform of all five domains (the solve's first and last stages) ----
+    kern = composite_transform(fpa, ctx, doms, conn, torch, a.kernel_reps)
+    kern.update(centre_kernels(fpa, ctx, doms, torch, a.kernel_reps))
+    dom_name = "cat_ytransform_inv"
     dom = kern[dom_name]
     # the same kernels on 32 right-hand sides: their throughput once launch
     # latency no longer dominates (not the workload above; context only)
-    kern32 = centre_kernels(fpa, ctx, doms, torch, max(3, a.kernel_reps // 10), nrhs=32)
+    kern32 = composite_transform(fpa, ctx, doms, conn, torch, max(3, a.kernel_reps // 10), nrhs=32)
+    kern32.update(centre_kernels(fpa, ctx, doms, torch, max(3, a.kernel_reps // 10), nrhs=32))
 
-    # ---- CPU baseline + accuracy: the oracle on the host cores ----
+    # ---- accuracy: the converged oracle; CPU baseline: the oracle's plain
+    # solve (the reference's algorithm) timed on the host cores ----
     cpu = None
-    runs = {"f64": v64, "f64_graph": vg, "f64_into": vi, "f64_graph_into": vgi, "f32_refine": v32}
     err = {k: None for k in runs}
+    cpu_err = None
     if not a.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
-        # reference semantics, GMRES(20); 1e-13 relative keeps the oracle's own
-        # error near 1e-11 at this size, so the comparison resolves 1e-10
-        oc = O.Composite(doms, conn, m=20, tol=a.oracle_tol, tol_is_relative=True, maxiter=200,
+        fs = [f[0] for f in fields]
+        oc = O.Composite(doms, conn, m=20, tol=a.oracle_tol, tol_is_relative=True, maxiter=500,
                          schur_mode=O.SCHUR_SPECTRAL)
         t0 = time.perf_counter()
-        ref, it, _ = oc.solve([f[0] for f in fields])
+        base, it, _ = oc.solve(fs)
         t_cpu = time.perf_counter() - t0
+        ref = O.reference_solution(doms, conn, fs)  # untimed: GMRES + long-double residual refinement
         y = np.concatenate([r.ravel() for r in ref])
         for key, v in runs.items():
             x = np.concatenate([u.ravel() for u in v["u"]])
             err[key] = float(np.linalg.norm(x - y) / np.linalg.norm(y))
+        xb = np.concatenate([u.ravel() for u in base])
+        cpu_err = float(np.linalg.norm(xb - y) / np.linalg.norm(y))
         cpu = {"value": round(1.0 / t_cpu, 4), "unit": "cross solves/s", "cores": O.num_threads(),
                "kind": "port",
-               "sample": "one full %dx%d-centre cross solve on the host (construction excluded), reference "
-                         "algorithm: full-field GMRES(m=20) with dense Schur blocks to relative %.0e, %d "
-                         "iterations, %.2f s" % (n, n, a.oracle_tol, it, t_cpu)}
+               "sample": "one full %dx%d-centre cross solve on the host (construction excluded) by the CPU "
+                         "restatement of the reference's algorithm: full-field GMRES(m=20) with dense Schur blocks "
+                         "to relative residual %.0e, %d iterations, %.2f s, rel-L2 %.2e vs the converged solution; "
+                         "its FFT is this repository's own Stockham/Bluestein code with O(R^2) DFT stages for odd "
+                         "prime factors (FFTW, which the reference calls, is not in the image)"
+                         % (n, n, a.oracle_tol, it, t_cpu, cpu_err)}
 
     def variant(v, key):
         i = v["info"]
@@ -444,14 +479,18 @@ def main():
                 "refine_sweeps": i.get("refine_steps"), "refine_error_estimate": i.get("refine_residual")}
 
     variants = {k: variant(v, k) for k, v in runs.items()}
-    # the headline is the fastest FP64 form whose residual check passed and
-    # (when the oracle ran) whose error meets the 1e-10 bar
-    def passed(k):
+
+    # the headline is the fastest plain FP64 form whose residual check passed
+    # and (when the oracle ran) whose error meets the metric's 1e-10 bar
+    def passed(k, bar):
         ok = runs[k]["info"].get("status", 0) == 0
-        return ok and (err[k] is None or err[k] <= 1e-10)
-    cands = [k for k in ("f64", "f64_graph", "f64_into", "f64_graph_into") if passed(k)] or ["f64"]
+        return ok and (err[k] is None or err[k] <= bar)
+    cands = [k for k in ("f64", "f64_graph", "f64_into", "f64_graph_into") if passed(k, 1e-10)] or ["f64"]
     main_key = min(cands, key=lambda k: runs[k]["ms"])
     main_v = runs[main_key]
+    # the fastest form within 1e-12 of the discrete solution
+    c12 = [k for k in runs if passed(k, 1e-12) and err[k] is not None]
+    k12 = min(c12, key=lambda k: runs[k]["ms"]) if c12 else None
     out = {
         "metric": "cross-geometry solves/s at 1e-10 rel L2: FP64 vs FP32+refine; HBM GB/s",
         "value": round(1000.0 / main_v["ms"], 3),
@@ -467,7 +506,10 @@ def main():
         "data": "synthetic",
         "config": {"workload": "cross %d: centre %dx%d, arms %dx%d / %dx%d, FP64, interface GMRES(m=%d) to "
                                "relative %.0e" % (n, n, n, n // 2, n, n, n // 2, a.m, a.tol),
-                   "form": main_key, "cells": cells, "rel_l2_vs_oracle": err[main_key], "variants": variants},
+                   "form": main_key, "cells": cells, "rel_l2_vs_oracle": err[main_key],
+                   "fastest_within_1e-12": None if k12 is None else
+                   {"form": k12, "solves_per_s": variants[k12]["solves_per_s"], "rel_l2_vs_oracle": err[k12]},
+                   "variants": variants},
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": dom["gbs"], "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(dom["gbs"] / PEAK_HBM_GBS, 4),
                      "traffic": measured_traffic(dom_name, n)},

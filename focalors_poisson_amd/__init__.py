@@ -243,6 +243,13 @@ class ConcatPoissonSolver2D:
         d["status"] = st
         return d
 
+    def transform(self, fields, inverse=False, stream=None):
+        """The y transforms of every domain in the solve's one batched launch,
+        in place (fp_cat_transform)."""
+        fp, _, nrhs = self._ptrs(fields, None)
+        _capi.check(_capi.lib().fp_cat_transform(self._h, fp, nrhs, int(inverse), _stream_ptr(stream)))
+        return fields
+
     def vjp(self, grad_u, faces=(), stream=None):
         """Adjoint of solve: returns (dL/df per domain, {(d, face): dL/dg}) for
         dL/du = grad_u (one tensor per domain, same shapes as the fields)."""

@@ -87,6 +87,7 @@ def lib():
     L.fp_cat_destroy.argtypes = [_vp]
     L.fp_cat_destroy.restype = None
     L.fp_cat_root.argtypes = [_vp]
+    L.fp_cat_transform.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, _vp]
     L.fp_cat_solve.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.c_int, _vp,
                                ctypes.POINTER(SolveInfo)]
     L.fp_cat_vjp.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.c_int, _vp,

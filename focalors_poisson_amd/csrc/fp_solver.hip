@@ -1926,6 +1926,32 @@ void fp_cat_destroy(fp_cat cat) { delete cat; }
 
 int fp_cat_root(fp_cat cat) { return cat ? cat->root : -1; }
 
+int fp_cat_transform(fp_cat cat, double* const* d_fields, int nrhs, int inverse, void* stream)
+{
+    return guarded([&] {
+        FP_REQUIRE(cat && d_fields && nrhs >= 0, "null argument");
+        for (int d = 0; d < cat->nd; d++) FP_REQUIRE(d_fields[d], "null field");
+        FP_HIP(hipSetDevice(cat->ctx->device));
+        if (nrhs == 0) return;
+        // the batched launch of the solve's first and last stages: every
+        // domain in one launch with the handle's shared shape
+        const auto                        shape = fp::batch_shape<double>(cat);
+        std::vector<fp::TransformJob<double>> jobs(cat->nd);
+        for (int d = 0; d < cat->nd; d++) {
+            fp::TransformJob<double>& j = jobs[d];
+            j       = {};
+            j.plan  = &cat->nodes[d]->plan.t64;
+            j.data  = d_fields[d];
+            j.ld    = fp::dom_n(cat, d);
+            j.nx    = cat->nx[d];
+            j.nrhs  = nrhs;
+            j.dst64 = 1;
+            fp::apply_batch_shape(j, shape);
+        }
+        fp::launch_ytransform_batch<double>(jobs.data(), cat->nd, inverse ? 1 : 0, (hipStream_t)stream);
+    });
+}
+
 static int cat_solve_entry(fp_cat cat, const double* const* d_rhs, double* const* d_fields,
                            const double* const* d_ghost, int nrhs, void* stream, fp_solve_info* info)
 {

@@ -157,6 +157,12 @@ int fp_cat_create(fp_ctx ctx, const fp_geometry_desc* g, int m, double tol, int 
 int fp_cat_set_refinement(fp_cat cat, double tol, int max_steps);
 void fp_cat_destroy(fp_cat cat);
 int  fp_cat_root(fp_cat cat);
+/* PoissonFFT2D::transform (inverse = 0) / transform_transpose (inverse = 1)
+ * of every subdomain of the composite, in place, in the one batched launch
+ * the solve uses for its first and last stages (pe/poisson/poisson_fft2d.h:8-16
+ * applied to each domain of concat_solver2d.cpp's solver_map); d_fields[d]
+ * holds nrhs fields of domain d. */
+int  fp_cat_transform(fp_cat cat, double* const* d_fields, int nrhs, int inverse, void* stream);
 /* ConcatPoissonSolver2D::solve(): d_fields[d] holds nrhs raw right-hand sides
  * of domain d (overwritten with the solution); d_ghost[4*d+face] holds nrhs
  * ghost-centre boundary data arrays of that face back to back (the

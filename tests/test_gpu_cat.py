@@ -335,3 +335,39 @@ def test_all_neumann_cross(ctx):
     with pytest.raises(fpa.FPError) as e:
         fpa.ConcatPoissonSolver2D(ctx, doms, conn, precision=fpa.FP_F64_REFINE)
     assert e.value.status == 6
+
+
+@pytest.mark.parametrize("inverse", [False, True])
+def test_composite_transform_matches_each_domain(ctx, inverse):
+    """fp_cat_transform (every domain's y transform in the solve's one batched
+    launch) equals each domain's own PoissonFFT2D::transform /
+    transform_transpose (fp_sub_transform) and the oracle's, coefficient for
+    coefficient, on two right-hand sides."""
+    import torch
+    import focalors_poisson_amd as fpa
+    from focalors_poisson_amd.cases import cross
+    doms, conn, fields, _ = cross(96, seed=13, nrhs=2)
+    cat = fpa.ConcatPoissonSolver2D(ctx, doms, conn, m=40, tol=1e-12, tol_is_relative=True)
+    ts = [to_dev(f) for f in fields]
+    banner("composite batched transform inverse=%d" % inverse)
+    cat.transform(ts, inverse=inverse)
+    torch.cuda.synchronize()
+    eff = cat_bcs(doms, conn)
+    for d, (dom, f, t) in enumerate(zip(doms, fields, ts)):
+        sub = fpa.PoissonSolver2D(ctx, dom[0], dom[1], dom[2], dom[3], eff[d])
+        one = to_dev(f)
+        sub.transform(one, inverse=inverse)
+        torch.cuda.synchronize()
+        got = t.cpu().numpy()
+        assert np.abs(got - one.cpu().numpy()).max() <= 1e-13 * np.abs(got).max(), d
+        for r in range(2):
+            ref = O.transform(eff[d][2], eff[d][3], f[r], inverse=inverse)
+            assert np.abs(got[r] - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max()), (d, r)
+
+
+def cat_bcs(doms, conn):
+    bcs = [list(d[4]) for d in doms]
+    for a, dr, b in conn:
+        bcs[a][dr] = A
+        bcs[b][dr ^ 1] = A
+    return [tuple(b) for b in bcs]

@@ -157,7 +157,9 @@ def test_config2_cross_64_rhs(ctx):
     assert info["status"] == 0, info
     one = fpa.ConcatPoissonSolver2D(ctx, doms, conn, m=200, tol=1e-12, tol_is_relative=True, max_iter=100)
     oc = O.Composite(doms, conn, schur_mode=O.SCHUR_SPECTRAL)
-    bn = lambda fs: np.sqrt(sum(float(np.sum((f * d[2] ** 2) ** 2)) for f, d in zip(fs, doms)))
+    # backward error |b - A u| / (8 |u| + |b|): every row of the scaled
+    # operator has absolute row sum at most 8
+    nrm = lambda xs: np.sqrt(sum(float(np.sum(x * x)) for x in xs))
     worst_same = worst_res = 0.0
     for r in range(64):
         fs = [f[r] for f in fields]
@@ -167,15 +169,15 @@ def test_config2_cross_64_rhs(ctx):
         torch.cuda.synchronize()
         worst_same = max(worst_same, rel(us, [s.cpu().numpy() for s in single]))
         res = oc.residual(us, fs)
-        worst_res = max(worst_res, np.sqrt(sum(float(np.sum(x * x)) for x in res)) / bn(fs))
+        worst_res = max(worst_res, nrm(res) / (8 * nrm(us) + nrm([f * d[2] ** 2 for f, d in zip(fs, doms)])))
         if r == 0:
             err = rel(us, O.reference_solution(doms, conn, fs))
             banner("config 2 cross x64 rhs 0 rel-L2 vs oracle %.3e" % err)
             assert err < PLAIN_F64_1024_BAR
-    banner("config 2 cross x64: worst rel diff vs single solves %.2e, worst relative residual %.2e"
+    banner("config 2 cross x64: worst rel diff vs single solves %.2e, worst backward error %.2e"
            % (worst_same, worst_res))
     assert worst_same < 1e-13
-    assert worst_res < 1e-12
+    assert worst_res < 1e-13
 
 
 # ---- [3] ----
