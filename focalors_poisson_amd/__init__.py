@@ -119,8 +119,10 @@ class PoissonSolver2D:
             self._h = None
 
 
-def solve_batched(ctx, solvers, fields, stream=None):
-    """Many independent subdomain solves in one call (fp_sub_solve_batched)."""
+def solve_batched(ctx, solvers, fields, stream=None, noalloc=False):
+    """Many independent subdomain solves in one call (fp_sub_solve_batched;
+    noalloc=True: fp_sub_solve_batched_noalloc, FP64 only, which allocates no
+    device memory)."""
     import torch
     k = len(solvers)
     if k == 0:
@@ -129,7 +131,12 @@ def solve_batched(ctx, solvers, fields, stream=None):
     subs = (ctypes.c_void_p * k)(*[s._h.value for s in solvers])
     ptrs = (ctypes.c_void_p * k)(*[f.data_ptr() for f in fields])
     nr = (ctypes.c_int * k)(*[_check_field(f, fields[0].dtype, s.nx * s.ny) for s, f in zip(solvers, fields)])
-    fn = _capi.lib().fp_sub_solve_batched if f64 else _capi.lib().fp_sub_solve_batched_f32
+    if noalloc:
+        if not f64:
+            raise TypeError("the no-allocation batched solve takes float64 fields")
+        fn = _capi.lib().fp_sub_solve_batched_noalloc
+    else:
+        fn = _capi.lib().fp_sub_solve_batched if f64 else _capi.lib().fp_sub_solve_batched_f32
     _capi.check(fn(ctx._h, k, subs, ptrs, nr, _stream_ptr(stream)))
 
 

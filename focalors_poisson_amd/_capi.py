@@ -81,6 +81,8 @@ def lib():
                                       ctypes.POINTER(SolveInfo)]
     L.fp_sub_solve_batched.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _ip, _vp]
     L.fp_sub_solve_batched_f32.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _ip, _vp]
+    L.fp_sub_solve_batched_noalloc.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _ip,
+                                               _vp]
     L.fp_cat_create.argtypes = [_vp, ctypes.POINTER(GeometryDesc), ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                 ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]
     L.fp_cat_set_refinement.argtypes = [_vp, ctypes.c_double, ctypes.c_int]

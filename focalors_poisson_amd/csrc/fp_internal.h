@@ -104,9 +104,22 @@ __device__ __forceinline__ const P& table_find(const JobTable<P>& t, int blk, in
 void* staging_get(size_t n);
 void  staging_put(void* p, hipStream_t s);
 
+// A caller-provided region of device memory that the descriptor tables of
+// one call are carved from instead of being allocated (the entry point that
+// allocates nothing sets it for the duration of its call, on its thread).
+struct TableArena {
+    unsigned char* base = nullptr;  // device address
+    size_t         cap  = 0;
+    size_t         used = 0;
+};
+TableArena*& table_arena();  // this thread's arena, null when none (fp_staging.cpp)
+
 // Copies n descriptors and their workgroup offsets into stream-ordered device
 // memory (released with table_release after the launch on the same stream),
 // through pinned staging so that the host does not wait for the device.
+// With an arena set: into the arena, by a copy from pageable host memory
+// (no device or pinned allocation; the runtime reads the source before the
+// call returns, so the host waits for the work queued before it).
 // Not for a stream under graph capture: the copy reads host memory.
 template <typename P>
 JobTable<P> table_upload(const std::vector<P>& p, const std::vector<int>& start, hipStream_t s, void*& mem)
@@ -117,6 +130,22 @@ JobTable<P> table_upload(const std::vector<P>& p, const std::vector<int>& start,
         throw Error(FP_E_INVALID, "batches of more than 8 subdomains cannot be captured");
     const size_t pb = (p.size() * sizeof(P) + 255) & ~(size_t)255;
     const size_t nb = pb + start.size() * sizeof(int);
+    if (TableArena* ar = table_arena()) {
+        const size_t off = (ar->used + 255) & ~(size_t)255;
+        if (off + nb > ar->cap)
+            throw Error(FP_E_INVALID, "descriptor tables exceed the scratch of the no-allocation entry point");
+        std::vector<unsigned char> h(nb);
+        std::memcpy(h.data(), p.data(), p.size() * sizeof(P));
+        std::memcpy(h.data() + pb, start.data(), start.size() * sizeof(int));
+        FP_HIP(hipMemcpyAsync(ar->base + off, h.data(), nb, hipMemcpyHostToDevice, s));
+        ar->used = off + nb;
+        mem      = nullptr;  // nothing to release
+        JobTable<P> t;
+        t.nprob = (int)p.size();
+        t.p     = reinterpret_cast<const P*>(ar->base + off);
+        t.start = reinterpret_cast<const int*>(ar->base + off + pb);
+        return t;
+    }
     FP_HIP(hipMallocAsync(&mem, nb, s));
     unsigned char* h = static_cast<unsigned char*>(staging_get(nb));
     std::memcpy(h, p.data(), p.size() * sizeof(P));

@@ -271,8 +271,17 @@ struct FastLines {
 
 }  // namespace fp
 
+// Device scratch of fp_sub_solve_batched_noalloc: the descriptor tables of
+// one call (about 100 bytes per problem and launch) are carved from this
+// array instead of being allocated.  Each GPU the library runs on holds its
+// own copy until the process ends; host code reaches it only through the
+// address hipGetSymbolAddress returns on that device (fp_ctx::scratch).
+constexpr size_t kNoAllocScratchBytes = 4u << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
 struct fp_ctx_s {
-    int device = 0;
+    int            device  = 0;
+    unsigned char* scratch = nullptr;  // this device's copy of `scratch`
 };
 
 struct fp_sub_s {
@@ -1759,7 +1768,16 @@ int fp_ctx_create(int device, fp_ctx* out)
         FP_HIP(hipSetDevice(device));
         auto* c   = new fp_ctx_s();
         c->device = device;
-        *out      = c;
+        // the device address of this GPU's scratch array (its name in host
+        // code is not a device address)
+        void*            ptr = nullptr;
+        const hipError_t e   = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) {
+            delete c;
+            FP_HIP(e);
+        }
+        c->scratch = static_cast<unsigned char*>(ptr);
+        *out       = c;
     });
 }
 
@@ -1853,6 +1871,27 @@ int fp_sub_solve_batched_f32(fp_ctx ctx, int nprob, const fp_sub* subs, float* c
                              void* stream)
 {
     return guarded([&] { solve_batched<float>(ctx, nprob, subs, d_f, nrhs, stream); });
+}
+
+int fp_sub_solve_batched_noalloc(fp_ctx ctx, int nprob, const fp_sub* subs, double* const* d_f, const int* nrhs,
+                                 void* stream)
+{
+    return guarded([&] {
+        FP_REQUIRE(ctx && ctx->scratch, "null context");
+        FP_REQUIRE(nprob <= FP_NOALLOC_MAX_PROBLEMS, "more problems than the no-allocation entry point takes");
+        fp::TableArena ar;
+        ar.base = ctx->scratch;
+        ar.cap  = kNoAllocScratchBytes;
+        fp::TableArena*& cur = fp::table_arena();
+        cur                  = &ar;
+        try {
+            solve_batched<double>(ctx, nprob, subs, d_f, nrhs, stream);
+        } catch (...) {
+            cur = nullptr;
+            throw;
+        }
+        cur = nullptr;
+    });
 }
 
 int fp_cat_create(fp_ctx ctx, const fp_geometry_desc* g, int m, double tol, int tol_is_relative, int max_iter,

@@ -46,6 +46,12 @@ void* staging_get(size_t n)
     return b.p;
 }
 
+TableArena*& table_arena()
+{
+    thread_local TableArena* a = nullptr;
+    return a;
+}
+
 void staging_put(void* p, hipStream_t s)
 {
     std::lock_guard<std::mutex> lk(g_mu);

@@ -133,6 +133,20 @@ int fp_sub_solve_batched(fp_ctx ctx, int nprob, const fp_sub* subs, double* cons
                          void* stream);
 int fp_sub_solve_batched_f32(fp_ctx ctx, int nprob, const fp_sub* subs, float* const* d_f, const int* nrhs,
                              void* stream);
+/* fp_sub_solve_batched without any allocation (no reference counterpart: for
+ * callers that cannot allocate device memory, e.g. tight serving loops or
+ * harnesses that forbid hipMalloc).  Same arguments, kernels and results; the
+ * descriptor tables of batches of more than 8 problems go to a 4 MiB device
+ * array of the library (one per GPU, held until the process ends) instead of
+ * stream-ordered allocations and pinned staging, copied from pageable host
+ * memory, so the host waits for the work queued on the stream before it.
+ * At most FP_NOALLOC_MAX_PROBLEMS problems per call (FP_E_INVALID beyond).
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * (different streams or threads at the same time) race; calls on one stream
+ * are ordered. */
+#define FP_NOALLOC_MAX_PROBLEMS 4096
+int fp_sub_solve_batched_noalloc(fp_ctx ctx, int nprob, const fp_sub* subs, double* const* d_f, const int* nrhs,
+                                 void* stream);
 
 /* ---- composite geometry: ConcatPoissonSolver2D ------------------------- */
 /* ConcatPoissonSolver2D(Variable2D*) + set_parameter(m, tol, maxIter):
