@@ -119,6 +119,37 @@ class PoissonSolver2D:
             self._h = None
 
 
+class PoissonSolver3D:
+    """fp_sub3d: the 3-D subdomain fast solver (transforms in z and y,
+    tridiagonal in x), pe/poisson/poisson_solver3d.h.  bc: six face types
+    (x-, x+, y-, y+, z-, z+)."""
+
+    def __init__(self, ctx, nx, ny, nz, hx, hy, hz, bc):
+        self.ctx, self.nx, self.ny, self.nz = ctx, nx, ny, nz
+        self.hx, self.hy, self.hz, self.bc = hx, hy, hz, tuple(bc)
+        if len(self.bc) != 6:
+            raise ValueError("a 3-D subdomain has six faces")
+        self._h = ctypes.c_void_p()
+        b = (ctypes.c_ubyte * 6)(*bc)
+        _capi.check(_capi.lib().fp_sub3d_create(ctx._h, nx, ny, nz, float(hx), float(hy), float(hz), b,
+                                                ctypes.byref(self._h)))
+
+    def solve(self, f, stream=None):
+        """In-place solve of one or more hx^2-scaled right-hand sides (float64
+        tensor of shape (..., nx, ny, nz), z contiguous)."""
+        import torch
+        n = self.nx * self.ny * self.nz
+        nrhs = _check_field(f, torch.float64, n)
+        _capi.check(_capi.lib().fp_sub3d_solve(self._h, ctypes.c_void_p(f.data_ptr()), nrhs, n, _stream_ptr(stream)))
+        return f
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and h.value:
+            _capi.lib().fp_sub3d_destroy(h)
+            self._h = None
+
+
 def solve_batched(ctx, solvers, fields, stream=None, noalloc=False):
     """Many independent subdomain solves in one call (fp_sub_solve_batched;
     noalloc=True: fp_sub_solve_batched_noalloc, FP64 only, which allocates no

@@ -83,6 +83,11 @@ def lib():
     L.fp_sub_solve_batched_f32.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _ip, _vp]
     L.fp_sub_solve_batched_noalloc.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _ip,
                                                _vp]
+    L.fp_sub3d_create.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                  ctypes.c_double, ctypes.POINTER(ctypes.c_ubyte), ctypes.POINTER(_vp)]
+    L.fp_sub3d_destroy.argtypes = [_vp]
+    L.fp_sub3d_destroy.restype = None
+    L.fp_sub3d_solve.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_longlong, _vp]
     L.fp_cat_create.argtypes = [_vp, ctypes.POINTER(GeometryDesc), ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                 ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]
     L.fp_cat_set_refinement.argtypes = [_vp, ctypes.c_double, ctypes.c_int]

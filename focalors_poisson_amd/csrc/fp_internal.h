@@ -72,6 +72,29 @@ struct DevBuf {
     void upload(const T* h, size_t count) { FP_HIP(hipMemcpy(p, h, count * sizeof(T), hipMemcpyHostToDevice)); }
 };
 
+// The message of the last failed C ABI call on this thread (fp_last_error).
+std::string& last_error_slot();
+
+// Runs fn, turning its exceptions into an fp_status and the message into
+// last_error_slot(): no exception crosses the C ABI.
+template <typename F>
+int guarded(F&& fn)
+{
+    try {
+        fn();
+        return FP_OK;
+    } catch (const Error& e) {
+        last_error_slot() = e.what();
+        return e.status;
+    } catch (const std::bad_alloc&) {
+        last_error_slot() = "host allocation failed";
+        return FP_E_NOMEM;
+    } catch (const std::exception& e) {
+        last_error_slot() = e.what();
+        return FP_E_INVALID;
+    }
+}
+
 constexpr int kMaxRadices = 24;
 // subdomain problems whose descriptors travel as kernel arguments; larger
 // batches put them in device memory (JobTable) and still take one launch
@@ -214,10 +237,14 @@ struct SubPlan {
     DevBuf<double>        zc;
     int                   dev = 0;
     void build(int nx, int ny, double hx, double hy, const int bc[4]);
+    void init_sweep(const std::vector<double>& delta);
     void ensure_f32();
     void set_chunks(int ch);
     void tune_tridiag(bool measure);
 };
+
+// 2 - lambda_i of the y (or z) transform kind, i = 1..n, to full precision
+std::vector<long double> eigen_gaps(int kind, int n);
 
 // Choices of the autotuners by key (fp_tune.cpp): process-wide, and shared
 // across processes through the file FP_TUNE_FILE names, if any.
@@ -339,3 +366,9 @@ void sub_solve_batch(SubPlan* const* plans, T* const* data, const long long* ld,
                      hipStream_t s);
 
 }  // namespace fp
+
+// A context: one HIP device (fp_ctx_create).
+struct fp_ctx_s {
+    int            device  = 0;
+    unsigned char* scratch = nullptr;  // this device's copy of the no-allocation scratch array
+};

@@ -34,45 +34,41 @@ static int transform_kind(int neg, int pos)
     return T_ND;
 }
 
-void SubPlan::build(int nx_, int ny_, double hx_, double hy_, const int bc_[4])
+// The eigenvalue gaps 2 - lambda_i (i = 1..N, reference order) of the
+// transform kind.  The reference forms x_diag = -2 + r (lambda - 2) with
+// lambda a cosine, which cancels to a few digits for the lowest modes
+// (poisson_solver2d_base.cpp:28-62, poisson_solver3d_base.cpp:30-66); here
+// 2 - lambda = 4 sin^2(a) with the half angle a of each kind, in long double:
+//   DD  lambda = 2 cos(pi i/(N+1))          a = pi i / (2 (N+1))
+//   NN  lambda = -2 cos(pi i/N)             a = pi (N - i) / (2N)
+//   DN, ND  lambda = -2 cos(2 pi i/(2N+1))  a = pi (2N+1-2i) / (2 (2N+1))
+//   PP  lambda = -2 cos(2 pi floor(i/2)/N)  a = pi (N - 2 floor(i/2)) / (2N)
+// so that x_diag = -2 - delta with delta = r 4 sin^2(a) >= 0.
+std::vector<long double> eigen_gaps(int kind, int n)
 {
-    FP_REQUIRE(nx_ >= 2 && ny_ >= 1, "subdomain needs nx >= 2 and ny >= 1");
-    FP_REQUIRE(hx_ > 0 && hy_ > 0 && std::isfinite(hx_) && std::isfinite(hy_), "spacings must be positive");
-    for (int f = 0; f < 4; f++)
-        FP_REQUIRE(bc_[f] == DIRICHLET || bc_[f] == NEUMANN || bc_[f] == ADJACENTED || bc_[f] == PERIODIC,
-                   "invalid boundary type");
-    FP_REQUIRE((bc_[XNEG] == PERIODIC) == (bc_[XPOS] == PERIODIC), "a periodic face needs a periodic partner");
-    FP_REQUIRE(bc_[YNEG] != PERIODIC || ny_ % 2 == 0, "a periodic y axis needs an even ny");
-    nx = nx_; ny = ny_; hx = hx_; hy = hy_;
-    for (int f = 0; f < 4; f++) bc[f] = bc_[f];
-    kind         = transform_kind(bc[YNEG], bc[YPOS]);
-    no_dirichlet = !(dir_like(bc[0]) || dir_like(bc[1]) || dir_like(bc[2]) || dir_like(bc[3]));
-    FP_HIP(hipGetDevice(&dev));
-    // PoissonSolver2DBase::cal_lambda + PoissonSolver2D::init
-    // (poisson_solver2d_base.cpp:28-62, poisson_solver2d.cpp:43-51)
-    // The reference forms x_diag = -2 + r (lambda - 2) with lambda a cosine,
-    // which cancels to a few digits for the lowest modes; here lambda - 2 =
-    // -4 sin^2(a) with the half angle a of each kind, in long double:
-    //   DD  lambda = 2 cos(pi i/(N+1))         a = pi i / (2 (N+1))
-    //   NN  lambda = -2 cos(pi i/N)            a = pi (N - i) / (2N)
-    //   DN, ND  lambda = -2 cos(2 pi i/(2N+1))  a = pi (2N+1-2i) / (2 (2N+1))
-    //   PP  lambda = -2 cos(2 pi floor(i/2)/N) a = pi (N - 2 floor(i/2)) / (2N)
-    // and x_diag = -2 - delta, delta = r 4 sin^2(a) >= 0.
-    const long double PI = 3.141592653589793238462643383279502884L;
-    const long double r  = (long double)hx * hx / ((long double)hy * hy);
-    x_diag.resize(ny);
-    delta.resize(ny);
-    for (int j = 0; j < ny; j++) {
-        const long double i = j + 1, N = ny;
+    const long double        PI = 3.141592653589793238462643383279502884L;
+    std::vector<long double> g(n);
+    for (int j = 0; j < n; j++) {
+        const long double i = j + 1, N = n;
         long double       a;
         if (kind == T_PP) a = PI * (N - 2.0L * (long double)((j + 1) / 2)) / (2.0L * N);
         else if (kind == T_NN) a = PI * (N - i) / (2.0L * N);
         else if (kind == T_DD) a = PI * i / (2.0L * (N + 1.0L));
         else a = PI * (2.0L * N + 1.0L - 2.0L * i) / (2.0L * (2.0L * N + 1.0L));
         const long double sa = std::sin(a);
-        delta[j]             = (double)(r * 4.0L * sa * sa);
-        x_diag[j]            = -2.0 - delta[j];
+        g[j]                 = 4.0L * sa * sa;
     }
+    return g;
+}
+
+// The x-sweep part of a plan: nx rows, ny modes (the y modes of a 2-D
+// subdomain or the (z, y) mode pairs of a 3-D one) with per-mode gap delta;
+// bc[XNEG], bc[XPOS], no_dirichlet already set.
+void SubPlan::init_sweep(const std::vector<double>& dl)
+{
+    delta = dl;
+    x_diag.resize(ny);
+    for (int j = 0; j < ny; j++) x_diag[j] = -2.0 - delta[j];
     xd.alloc(ny);
     xd.upload(delta.data(), ny);
     // periodic x (ChasingMethod2D::init, chasing_method2d.cpp:21-33 with
@@ -102,8 +98,31 @@ void SubPlan::build(int nx_, int ny_, double hx_, double hy_, const int bc_[4])
     }
     // reciprocal Thomas pivots (the reference's c vector,
     // chasing_method_base.cpp:33-46) are regenerated inside the sweep from
-    // one stored pivot per row chunk; the chunk schedule is tuned here
+    // one stored pivot excess per row chunk; the chunk schedule is tuned here
     tune_tridiag(true);
+}
+
+void SubPlan::build(int nx_, int ny_, double hx_, double hy_, const int bc_[4])
+{
+    FP_REQUIRE(nx_ >= 2 && ny_ >= 1, "subdomain needs nx >= 2 and ny >= 1");
+    FP_REQUIRE(hx_ > 0 && hy_ > 0 && std::isfinite(hx_) && std::isfinite(hy_), "spacings must be positive");
+    for (int f = 0; f < 4; f++)
+        FP_REQUIRE(bc_[f] == DIRICHLET || bc_[f] == NEUMANN || bc_[f] == ADJACENTED || bc_[f] == PERIODIC,
+                   "invalid boundary type");
+    FP_REQUIRE((bc_[XNEG] == PERIODIC) == (bc_[XPOS] == PERIODIC), "a periodic face needs a periodic partner");
+    FP_REQUIRE(bc_[YNEG] != PERIODIC || ny_ % 2 == 0, "a periodic y axis needs an even ny");
+    nx = nx_; ny = ny_; hx = hx_; hy = hy_;
+    for (int f = 0; f < 4; f++) bc[f] = bc_[f];
+    kind         = transform_kind(bc[YNEG], bc[YPOS]);
+    no_dirichlet = !(dir_like(bc[0]) || dir_like(bc[1]) || dir_like(bc[2]) || dir_like(bc[3]));
+    FP_HIP(hipGetDevice(&dev));
+    // PoissonSolver2DBase::cal_lambda + PoissonSolver2D::init
+    // (poisson_solver2d_base.cpp:28-62, poisson_solver2d.cpp:43-51)
+    const std::vector<long double> g = eigen_gaps(kind, ny);
+    const long double              r = (long double)hx * hx / ((long double)hy * hy);
+    std::vector<double>            dl(ny);
+    for (int j = 0; j < ny; j++) dl[j] = (double)(r * g[j]);
+    init_sweep(dl);
     t64.build(kind, ny);
     tune_transform(t64, nx, true);
 }
@@ -279,10 +298,6 @@ struct FastLines {
 constexpr size_t kNoAllocScratchBytes = 4u << 20;
 static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 
-struct fp_ctx_s {
-    int            device  = 0;
-    unsigned char* scratch = nullptr;  // this device's copy of `scratch`
-};
 
 struct fp_sub_s {
     fp_ctx_s*   ctx = nullptr;
@@ -1700,26 +1715,16 @@ static void cat_solve(fp_cat_s* c, double* const* f, const double* const* ghost,
 // C ABI
 
 namespace {
-thread_local std::string g_last_error;
-
-template <typename F>
-int guarded(F&& fn)
-{
-    try {
-        fn();
-        return FP_OK;
-    } catch (const fp::Error& e) {
-        g_last_error = e.what();
-        return e.status;
-    } catch (const std::bad_alloc& e) {
-        g_last_error = "host allocation failed";
-        return FP_E_NOMEM;
-    } catch (const std::exception& e) {
-        g_last_error = e.what();
-        return FP_E_INVALID;
-    }
-}
+using fp::guarded;
 }  // namespace
+
+namespace fp {
+std::string& last_error_slot()
+{
+    thread_local std::string s;
+    return s;
+}
+}  // namespace fp
 
 namespace {
 template <typename T>
@@ -1756,7 +1761,7 @@ const char* fp_status_str(int st)
     }
 }
 
-const char* fp_last_error(void) { return g_last_error.c_str(); }
+const char* fp_last_error(void) { return fp::last_error_slot().c_str(); }
 
 int fp_ctx_create(int device, fp_ctx* out)
 {

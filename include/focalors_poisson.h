@@ -44,6 +44,7 @@ extern "C" {
 typedef struct fp_ctx_s* fp_ctx;
 typedef struct fp_sub_s* fp_sub;
 typedef struct fp_cat_s* fp_cat;
+typedef struct fp_sub3d_s* fp_sub3d;
 
 typedef enum fp_status {
     FP_OK               = 0,
@@ -147,6 +148,20 @@ int fp_sub_solve_batched_f32(fp_ctx ctx, int nprob, const fp_sub* subs, float* c
 #define FP_NOALLOC_MAX_PROBLEMS 4096
 int fp_sub_solve_batched_noalloc(fp_ctx ctx, int nprob, const fp_sub* subs, double* const* d_f, const int* nrhs,
                                  void* stream);
+
+/* ---- one 3-D subdomain: PoissonSolver3D -------------------------------- */
+/* PoissonSolver3D(nx, ny, nz, hx, hy, hz, bc x6), pe/poisson/poisson_solver3d.cpp:3-85.
+ * bc: XNegative, XPositive, YNegative, YPositive, ZNegative, ZPositive
+ * (base/location_boundary.h:8-16).  A periodic y or z axis needs an even
+ * length. */
+int  fp_sub3d_create(fp_ctx ctx, int nx, int ny, int nz, double hx, double hy, double hz, const unsigned char bc[6],
+                     fp_sub3d* out);
+void fp_sub3d_destroy(fp_sub3d sub);
+/* PoissonSolver3D::solve(field3& f), pe/poisson/poisson_solver3d.cpp:87-123:
+ * d_f holds nrhs fields ld apart, each field3's layout [i][j][k] with k (z)
+ * contiguous (base/field/field3.cpp), the hx^2-scaled right-hand side on
+ * entry and the solution on exit. */
+int  fp_sub3d_solve(fp_sub3d sub, double* d_f, int nrhs, long long ld, void* stream);
 
 /* ---- composite geometry: ConcatPoissonSolver2D ------------------------- */
 /* ConcatPoissonSolver2D(Variable2D*) + set_parameter(m, tol, maxIter):

@@ -637,6 +637,83 @@ int or_sub_solve(int nx, int ny, double hx, double hy, const int* bc4, double* f
 }
 
 /* ------------------------------------------------------------------------ */
+/* 3-D subdomain solver (pe/poisson/poisson_solver3d.cpp)                     */
+/* ------------------------------------------------------------------------ */
+
+/* PoissonSolver3DBase::cal_lambda (poisson_solver3d_base.cpp:30-66): lambda - 2
+ * of mode i = 1..N. */
+static double lambda3(int kind, int i, int N)
+{
+    if (kind == T_PP) return -2.0 - 2.0 * cos(2.0 * OR_PI / N * floor(i / 2.0));
+    if (kind == T_NN) return -2.0 - 2.0 * cos(OR_PI / N * i);
+    if (kind == T_DD) return -2.0 + 2.0 * cos(OR_PI / (N + 1) * i);
+    return -2.0 - 2.0 * cos(2.0 * OR_PI * i / (2 * N + 1));
+}
+
+/* PoissonSolver3D::init + solve (poisson_solver3d.cpp:49-123) with
+ * ChasingMethod3D::chasing (chasing_method3d.cpp:57-103) on an already
+ * hx^2-scaled right-hand side f, layout [i][j][k] (k contiguous, field3).
+ * Deviation: with a periodic x axis and no Dirichlet-like face the reference
+ * applies the periodic singular recurrence to every y mode of the last z mode
+ * (chasing_method3d.cpp:66); the zero mode alone is singular, and only it
+ * takes the recurrence here (as in the non-periodic branch, :84). */
+int or_sub_solve3d(int nx, int ny, int nz, double hx, double hy, double hz, const int* bc, double* f)
+{
+    const int ky = transform_kind(bc[2], bc[3]), kz = transform_kind(bc[4], bc[5]);
+    if (ky == T_BAD || kz == T_BAD) return 1;
+    if ((ky == T_PP && ny % 2) || (kz == T_PP && nz % 2)) return 1;
+    if ((bc[0] == OR_PERIODIC) != (bc[1] == OR_PERIODIC) || nx < 2) return 1;
+    const size_t n  = (size_t)nx * ny * nz;
+    double*      t  = (double*)malloc(sizeof(double) * n);
+    double*      u  = (double*)malloc(sizeof(double) * n);
+    double*      xd = (double*)malloc(sizeof(double) * (size_t)ny * nz);
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+            xd[(size_t)k * ny + j] = -2.0 + hx * hx / hy / hy * lambda3(ky, j + 1, ny) +
+                                     hx * hx / hz / hz * lambda3(kz, k + 1, nz);
+    int no_dir = 1;
+    for (int q = 0; q < 6; q++)
+        if (is_dir_like(bc[q])) no_dir = 0;
+    or_transform(bc[4], bc[5], 0, nx * ny, nz, f, t);
+    /* field3::transpose {2, 0, 1}: (i, j, k) -> (k, i, j) */
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nx; i++)
+        for (int j = 0; j < ny; j++)
+            for (int k = 0; k < nz; k++) u[((size_t)k * nx + i) * ny + j] = t[((size_t)i * ny + j) * nz + k];
+    or_transform(bc[2], bc[3], 0, nz * nx, ny, u, t);
+    /* x lines of every (k, j) mode pair: ChasingMethod3D::chasing */
+#pragma omp parallel
+    {
+        double* in  = (double*)malloc(sizeof(double) * nx);
+        double* out = (double*)malloc(sizeof(double) * nx);
+#pragma omp for schedule(dynamic)
+        for (int kj = 0; kj < ny * nz; kj++) {
+            const int k = kj / ny, j = kj % ny;
+            for (int i = 0; i < nx; i++) in[i] = t[((size_t)k * nx + i) * ny + j];
+            const int sing = no_dir && k == nz - 1 && j == ny - 1;
+            if (bc[0] == OR_PERIODIC) {
+                if (sing) or_chasing_periodic_singular(nx, in, out);
+                else or_chasing_periodic(xd[kj], nx, in, out);
+            } else if (sing) {
+                or_chasing_singular(nx, in, out);
+            } else {
+                or_chasing(xd[kj], nx, bc[0], bc[1], in, out);
+            }
+            for (int i = 0; i < nx; i++) t[((size_t)k * nx + i) * ny + j] = out[i];
+        }
+        free(in); free(out);
+    }
+    or_transform(bc[2], bc[3], 1, nz * nx, ny, t, u);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nx; i++)
+        for (int j = 0; j < ny; j++)
+            for (int k = 0; k < nz; k++) t[((size_t)i * ny + j) * nz + k] = u[((size_t)k * nx + i) * ny + j];
+    or_transform(bc[4], bc[5], 1, nx * ny, nz, t, f);
+    free(t); free(u); free(xd);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Composite solver (pe/concat/, base/domain/geometry2d.cpp, geometry_tree.hpp) */
 /* ------------------------------------------------------------------------ */
 

@@ -1,5 +1,6 @@
 /*
- * fp_oracle.h -- CPU FP64 restatement of the Focalors_Poisson 2-D solve path.
+ * fp_oracle.h -- CPU FP64 restatement of the Focalors_Poisson solve path (2-D
+ * composite, 2-D and 3-D subdomain solvers).
  *
  * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP path is
  * compared against (tests/, __graft_entry__.smoke(), bench.py cpu_baseline leg).
@@ -45,6 +46,10 @@ void or_chasing_singular(int n, const double* f, double* x);
 
 /* PoissonSolver2D::solve on an already hx^2-scaled RHS; result written to f. */
 int or_sub_solve(int nx, int ny, double hx, double hy, const int* bc4, double* f);
+
+/* PoissonSolver3D::solve on an already hx^2-scaled RHS, field3 layout
+ * [i][j][k] (k contiguous); bc6: x-, x+, y-, y+, z-, z+.  Result written to f. */
+int or_sub_solve3d(int nx, int ny, int nz, double hx, double hy, double hz, const int* bc6, double* f);
 
 /* Composite geometry (ConcatPoissonSolver2D).  Domains 0..nd-1 with sizes and
  * spacings; bc is nd*4 face types (Adjacented faces are set by the connection

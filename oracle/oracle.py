@@ -46,6 +46,8 @@ def lib():
         L.or_chasing_periodic_singular.argtypes = [ctypes.c_int, _dp, _dp]
         L.or_chasing_periodic_singular.restype = None
         L.or_sub_solve.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _ip, _dp]
+        L.or_sub_solve3d.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_double, _ip, _dp]
         L.or_cat_create.restype = ctypes.c_void_p
         L.or_cat_create.argtypes = [ctypes.c_int, _ip, _ip, _dp, _dp, _ip, ctypes.c_int, _ip, ctypes.c_int,
                                     ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int]
@@ -136,6 +138,15 @@ def sub_solve(f, hx, hy, bc):
     f = _f64(f).copy()
     b = (ctypes.c_int * 4)(*bc)
     assert lib().or_sub_solve(f.shape[0], f.shape[1], hx, hy, b, _p(f)) == 0
+    return f
+
+
+def sub_solve3d(f, hx, hy, hz, bc):
+    """PoissonSolver3D::solve of an already hx^2-scaled RHS of shape
+    (nx, ny, nz) (returns a new array)."""
+    f = _f64(f).copy()
+    b = (ctypes.c_int * 6)(*bc)
+    assert lib().or_sub_solve3d(f.shape[0], f.shape[1], f.shape[2], hx, hy, hz, b, _p(f)) == 0
     return f
 
 
