@@ -313,3 +313,63 @@ class ConcatPoissonSolver2D:
         if h and h.value:
             _capi.lib().fp_cat_destroy(h)
             self._h = None
+
+
+class ConcatPoissonSolver3D:
+    """fp_cat3d: 3-D composite geometry solve (tree sweep + face GMRES),
+    pe/concat/concat_solver3d.h.
+
+    boxes: list of (nx, ny, nz, hx, hy, hz, (bc x-, x+, y-, y+, z-, z+))
+    conn:  list of (a, dir, b) = Geometry3D::connect(a, dir, b), dir 0..5
+    """
+
+    def __init__(self, ctx, boxes, conn, m=20, tol=1e-8, tol_is_relative=False, max_iter=100):
+        self.ctx, self.boxes, self.conn = ctx, list(boxes), list(conn)
+        nd = len(boxes)
+        ci = lambda k: (ctypes.c_int * nd)(*[b[k] for b in boxes])
+        cd = lambda k: (ctypes.c_double * nd)(*[b[k] for b in boxes])
+        self._keep = [ci(0), ci(1), ci(2), cd(3), cd(4), cd(5),
+                      (ctypes.c_ubyte * (6 * nd))(*[t for b in boxes for t in b[6]]),
+                      (ctypes.c_int * max(1, 3 * len(conn)))(*[v for c in conn for v in c])]
+        k = self._keep
+        desc = _capi.Geometry3DDesc(nd, k[0], k[1], k[2], k[3], k[4], k[5], k[6], len(conn), k[7])
+        self._h = ctypes.c_void_p()
+        _capi.check(_capi.lib().fp_cat3d_create(ctx._h, ctypes.byref(desc), int(m), float(tol), int(tol_is_relative),
+                                                int(max_iter), ctypes.byref(self._h)))
+        self.root = _capi.lib().fp_cat3d_root(self._h)
+
+    def solve(self, fields, ghost=None, stream=None, allow_not_converged=False):
+        """In-place composite solve: fields[d] (float64, shape (..., nx, ny, nz))
+        raw right-hand sides -> solutions; ghost: {(box, face): plane data}."""
+        import torch
+        nd = len(self.boxes)
+        nrhs = None
+        for d, f in enumerate(fields):
+            b = self.boxes[d]
+            r = _check_field(f, torch.float64, b[0] * b[1] * b[2])
+            if nrhs is None:
+                nrhs = r
+            elif r != nrhs:
+                raise ValueError("all boxes need the same number of right-hand sides")
+        fp = (ctypes.c_void_p * nd)(*[f.data_ptr() for f in fields])
+        gp = (ctypes.c_void_p * (6 * nd))()
+        for (d, face), g in (ghost or {}).items():
+            nx, ny, nz = self.boxes[d][:3]
+            cnt = (ny * nz, ny * nz, nx * nz, nx * nz, nx * ny, nx * ny)[face]
+            if not g.is_cuda or g.dtype != torch.float64 or not g.is_contiguous() or g.numel() != cnt * nrhs:
+                raise ValueError("ghost data of box %d face %d: %d contiguous float64 device values" %
+                                 (d, face, cnt * nrhs))
+            gp[6 * d + face] = g.data_ptr()
+        info = _capi.SolveInfo()
+        allow = (_capi.FP_E_NOT_CONVERGED,) if allow_not_converged else ()
+        st = _capi.check(_capi.lib().fp_cat3d_solve(self._h, fp, gp, nrhs, _stream_ptr(stream), ctypes.byref(info)),
+                         allow)
+        d = info.as_dict()
+        d["status"] = st
+        return d
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and h.value:
+            _capi.lib().fp_cat3d_destroy(h)
+            self._h = None

@@ -35,6 +35,11 @@ class GeometryDesc(ctypes.Structure):
                 ("bc", ctypes.POINTER(ctypes.c_ubyte)), ("nconn", ctypes.c_int), ("conn", _ip)]
 
 
+class Geometry3DDesc(ctypes.Structure):
+    _fields_ = [("nd", ctypes.c_int), ("nx", _ip), ("ny", _ip), ("nz", _ip), ("hx", _dp), ("hy", _dp), ("hz", _dp),
+                ("bc", ctypes.POINTER(ctypes.c_ubyte)), ("nconn", ctypes.c_int), ("conn", _ip)]
+
+
 class FPError(RuntimeError):
     def __init__(self, status, msg):
         super().__init__("%s (status %d): %s" % (status_str(status), status, msg))
@@ -88,6 +93,13 @@ def lib():
     L.fp_sub3d_destroy.argtypes = [_vp]
     L.fp_sub3d_destroy.restype = None
     L.fp_sub3d_solve.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_longlong, _vp]
+    L.fp_cat3d_create.argtypes = [_vp, ctypes.POINTER(Geometry3DDesc), ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                  ctypes.c_int, ctypes.POINTER(_vp)]
+    L.fp_cat3d_destroy.argtypes = [_vp]
+    L.fp_cat3d_destroy.restype = None
+    L.fp_cat3d_root.argtypes = [_vp]
+    L.fp_cat3d_solve.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.c_int, _vp,
+                                 ctypes.POINTER(SolveInfo)]
     L.fp_cat_create.argtypes = [_vp, ctypes.POINTER(GeometryDesc), ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                 ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]
     L.fp_cat_set_refinement.argtypes = [_vp, ctypes.c_double, ctypes.c_int]

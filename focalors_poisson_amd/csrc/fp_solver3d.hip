@@ -11,27 +11,12 @@
 // every row access coalesced, exactly as the 2-D solver's sweep; one
 // transpose takes it back.  Seven passes over the field instead of the
 // reference's eleven.
-#include "fp_internal.h"
+#include "fp_solver3d.h"
 
 #include <cmath>
 #include <memory>
 
 namespace fp {
-
-// Mode pairs (k, j) of the x sweep are numbered m = k ny + j; the pivot gap of
-// a pair is the sum of the y and z gaps, each to full precision:
-//   x_diag[k][j] = -2 + (hx/hy)^2 lambda_y[j] + (hx/hz)^2 lambda_z[k]
-// (poisson_solver3d.cpp:60-70, lambda = cal_lambda's values, <= 0).
-struct SubPlan3D {
-    int                   nx = 0, ny = 0, nz = 0;
-    double                hx = 0, hy = 0, hz = 0;
-    int                   bc[6] = {0, 0, 0, 0, 0, 0};
-    TransformPlan<double> tz, ty;
-    SubPlan               sweep;  // nx rows, ny nz modes
-    DevBuf<double>        work;   // the y-contiguous copy [i][k][j] of every right-hand side
-    int                   work_nrhs = 0;
-    void                  build(int nx, int ny, int nz, double hx, double hy, double hz, const int bc[6]);
-};
 
 static int kind_of(int neg, int pos)
 {
@@ -135,7 +120,7 @@ static void launch_transpose_planes(const double* src, double* dst, int a, int b
 
 // PoissonSolver3D::solve (poisson_solver3d.cpp:87-123) on nrhs hx^2-scaled
 // right-hand sides ld apart, in place.
-static void sub3d_solve(SubPlan3D& p, double* f, long long ld, int nrhs, hipStream_t s)
+void sub3d_solve(SubPlan3D& p, double* f, long long ld, int nrhs, hipStream_t s)
 {
     if (nrhs <= 0) return;
     const long long n = (long long)p.nx * p.ny * p.nz;

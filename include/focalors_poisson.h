@@ -45,6 +45,7 @@ typedef struct fp_ctx_s* fp_ctx;
 typedef struct fp_sub_s* fp_sub;
 typedef struct fp_cat_s* fp_cat;
 typedef struct fp_sub3d_s* fp_sub3d;
+typedef struct fp_cat3d_s* fp_cat3d;
 
 typedef enum fp_status {
     FP_OK               = 0,
@@ -162,6 +163,37 @@ void fp_sub3d_destroy(fp_sub3d sub);
  * contiguous (base/field/field3.cpp), the hx^2-scaled right-hand side on
  * entry and the solution on exit. */
 int  fp_sub3d_solve(fp_sub3d sub, double* d_f, int nrhs, long long ld, void* stream);
+
+/* ---- 3-D composite geometry: ConcatPoissonSolver3D --------------------- */
+/* Boxes and Geometry3D::connect calls (base/domain/geometry3d.cpp); bc holds
+ * nd*6 face types, faces named in a connection become Adjacented. */
+typedef struct fp_geometry3d_desc {
+    int                  nd;
+    const int*           nx;
+    const int*           ny;
+    const int*           nz;
+    const double*        hx;
+    const double*        hy;
+    const double*        hz;
+    const unsigned char* bc;    /* nd*6 face types */
+    int                  nconn;
+    const int*           conn;  /* nconn*3: (a, dir, b) = connect(a, dir, b), dir 0..5 */
+} fp_geometry3d_desc;
+/* ConcatPoissonSolver3D(Variable3D*) + set_parameter(m, tol, maxIter)
+ * (pe/concat/concat_solver3d.cpp:5-95): the tree, the 3-D subdomain plans
+ * and the dense Schur blocks of every non-leaf (schur_mat3d.cpp), built on
+ * the device.  Geometries without a Dirichlet face return FP_E_UNSUPPORTED. */
+int  fp_cat3d_create(fp_ctx ctx, const fp_geometry3d_desc* g, int m, double tol, int tol_is_relative, int max_iter,
+                     fp_cat3d* out);
+void fp_cat3d_destroy(fp_cat3d cat);
+int  fp_cat3d_root(fp_cat3d cat);
+/* ConcatPoissonSolver3D::solve() (concat_solver3d.cpp:97-247): d_fields[d]
+ * holds nrhs raw right-hand sides of box d (field3 layout, overwritten with
+ * the solution); d_ghost[6*d+face] holds nrhs ghost-data planes of that face
+ * back to back (face order of its cells: x faces (j, k), y faces (i, k),
+ * z faces (i, j), last index fastest), or NULL. */
+int  fp_cat3d_solve(fp_cat3d cat, double* const* d_fields, const double* const* d_ghost, int nrhs, void* stream,
+                    fp_solve_info* info);
 
 /* ---- composite geometry: ConcatPoissonSolver2D ------------------------- */
 /* ConcatPoissonSolver2D(Variable2D*) + set_parameter(m, tol, maxIter):

@@ -257,3 +257,200 @@ def gmres_mock(ainv, S, b, m, tol, maxiter):
 
 def num_threads():
     return lib().or_num_threads()
+
+
+# ---- 3-D composite (ConcatPoissonSolver3D), numpy restatement ----
+
+def _face3(loc, shape):
+    """Index tuple of face `loc` (x-, x+, y-, y+, z-, z+) of a field3 array
+    (field3::xneg_bond_add and the like, base/field/field3.cpp)."""
+    nx, ny, nz = shape
+    return {0: (0, slice(None), slice(None)), 1: (nx - 1, slice(None), slice(None)),
+            2: (slice(None), 0, slice(None)), 3: (slice(None), ny - 1, slice(None)),
+            4: (slice(None), slice(None), 0), 5: (slice(None), slice(None), nz - 1)}[loc]
+
+
+class Composite3D:
+    """ConcatPoissonSolver3D restated on the CPU (pe/concat/concat_solver3d.cpp,
+    gmres_solver3d.cpp, schur_mat3d.cpp), subdomain solves by or_sub_solve3d.
+    For small boxes: the Schur blocks are built by one child solve per face
+    cell, as the reference builds them.
+
+    boxes: list of (nx, ny, nz, hx, hy, hz, bc6); conn: (a, dir, b) as
+    Geometry3D::connect(a, dir, b), dir 0..5."""
+
+    def __init__(self, boxes, conn, m=20, tol=1e-8, tol_is_relative=False, maxiter=100):
+        self.boxes, self.conn = list(boxes), list(conn)
+        self.m, self.tol, self.rel, self.maxiter = m, tol, tol_is_relative, maxiter
+        nd = len(boxes)
+        self.bc = [list(b[6]) for b in boxes]
+        self.adj = [[-1] * 6 for _ in range(nd)]
+        for a, dr, b in conn:
+            self.adj[a][dr], self.adj[b][dr ^ 1] = b, a
+            self.bc[a][dr], self.bc[b][dr ^ 1] = ADJACENTED, ADJACENTED
+        self._tree()
+        self.S = {}
+        # construct_solver_map: deepest level first, root last (concat_solver3d.cpp:74-95)
+        for lev in reversed(self.levels):
+            for d in lev:
+                for loc, c in self.children[d]:
+                    self.S[c] = self._schur(d, c, loc)
+        self.last_iters = 0
+
+    def _tree(self):
+        """TreeUtils root choice (geometry_tree.hpp:22-95; lower index on a tie) and BFS."""
+        nd = len(self.boxes)
+        root = 0
+        if nd > 1:
+            deg = [sum(1 for x in self.adj[a] if x >= 0) for a in range(nd)]
+            q = [a for a in range(nd) if deg[a] <= 1]
+            qh, remaining = 0, nd
+            while remaining > 2 and qh < len(q):
+                cnt = len(q) - qh
+                remaining -= cnt
+                for _ in range(cnt):
+                    leaf = q[qh]
+                    qh += 1
+                    for nb in self.adj[leaf]:
+                        if nb >= 0:
+                            deg[nb] -= 1
+                            if deg[nb] == 1:
+                                q.append(nb)
+            if qh < len(q):
+                root = q[qh]
+                if qh + 1 < len(q) and q[qh + 1] < root:
+                    root = q[qh + 1]
+        self.root = root
+        self.parent, self.parent_loc = [-1] * nd, [-1] * nd
+        self.children = [[] for _ in range(nd)]
+        level = [-1] * nd
+        level[root] = 0
+        order = [root]
+        for a in order:
+            for loc in range(6):
+                b = self.adj[a][loc]
+                if b < 0 or b == self.parent[a] or level[b] >= 0:
+                    continue
+                self.children[a].append((loc, b))
+                self.parent[b], self.parent_loc[b] = a, loc ^ 1
+                level[b] = level[a] + 1
+                order.append(b)
+        self.levels = [[a for a in order if level[a] == l] for l in range(max(level) + 1)]
+
+    def _pe(self, d, f):
+        nx, ny, nz, hx, hy, hz, _ = self.boxes[d]
+        return sub_solve3d(f, hx, hy, hz, self.bc[d])
+
+    def _schur(self, d, c, loc):
+        """SchurMat3D_*::construct (schur_mat3d.cpp): column t = the child's
+        response on its face next to the parent to a unit source there."""
+        shape = self.boxes[c][:3]
+        fc = _face3(loc ^ 1, shape)
+        cells = np.arange(np.prod(shape)).reshape(shape)[fc].ravel()
+        S = np.zeros((len(cells), len(cells)))
+        for t, cell in enumerate(cells):
+            e = np.zeros(shape)
+            e.ravel()[cell] = 1.0
+            S[:, t] = self._dsolve(c, e).ravel()[cells]
+        return S
+
+    def _afun(self, d, x):
+        """GMRESSolver3D::Afun (gmres_solver3d.cpp:127-141)."""
+        ft = np.zeros_like(x)
+        for loc, c in self.children[d]:
+            fr = _face3(loc, x.shape)
+            ft[fr] += (self.S[c] @ x[fr].ravel()).reshape(x[fr].shape)
+        return x - self._pe(d, ft)
+
+    def _gmres(self, d, b):
+        """GMRESSolver3D::solve (gmres_solver3d.cpp:163-306): x0 = b, MGS,
+        Givens, breakdown 1e-12, tolerance on the residual norm."""
+        b = self._pe(d, b)
+        tol = self.tol * (np.linalg.norm(b) if self.rel else 1.0)
+        x = b.copy()
+        m = self.m
+        it = 0
+        for _ in range(self.maxiter):
+            r = b - self._afun(d, x)
+            beta = np.linalg.norm(r)
+            if beta < tol:
+                break
+            H = np.zeros((m + 1, m))
+            cs, sn, g = np.zeros(m), np.zeros(m), np.zeros(m + 1)
+            V = [r / beta]
+            g[0] = beta
+            j = 0
+            while j < m:
+                w = self._afun(d, V[j])
+                it += 1
+                for i in range(j + 1):
+                    H[i, j] = np.sum(V[i] * w)
+                    w = w - H[i, j] * V[i]
+                hj = np.linalg.norm(w)
+                if hj < 1e-12:
+                    break
+                H[j + 1, j] = hj
+                V.append(w / hj)
+                for i in range(j):
+                    t = cs[i] * H[i, j] + sn[i] * H[i + 1, j]
+                    H[i + 1, j] = -sn[i] * H[i, j] + cs[i] * H[i + 1, j]
+                    H[i, j] = t
+                den = np.hypot(H[j, j], H[j + 1, j])
+                cs[j], sn[j] = H[j, j] / den, H[j + 1, j] / den
+                H[j, j] = cs[j] * H[j, j] + sn[j] * H[j + 1, j]
+                H[j + 1, j] = 0.0
+                g[j], g[j + 1] = cs[j] * g[j], -sn[j] * g[j]
+                if abs(g[j + 1]) < tol:
+                    j += 1
+                    break
+                j += 1
+            y = np.zeros(j)
+            for i in range(j - 1, -1, -1):
+                y[i] = (g[i] - H[i, i + 1:j] @ y[i + 1:j]) / H[i, i]
+            for i in range(j):
+                x = x + y[i] * V[i]
+        if d == self.root:
+            self.last_iters = it
+        return x
+
+    def _dsolve(self, d, f):
+        return self._gmres(d, f) if self.children[d] else self._pe(d, f)
+
+    def solve(self, fields, ghost=None):
+        """ConcatPoissonSolver3D::solve (concat_solver3d.cpp:97-157) with the
+        boundary assembly (:159-247).  fields: raw right-hand sides; ghost:
+        dict (box, face) -> plane of ghost data.  Returns the solutions."""
+        f = [np.array(x, dtype=np.float64) for x in fields]
+        for d, (nx, ny, nz, hx, hy, hz, _) in enumerate(self.boxes):
+            h = (hx, hy, hz)
+            for face in range(6):
+                g = (ghost or {}).get((d, face))
+                t = self.bc[d][face]
+                if g is None or t not in (DIRICHLET, NEUMANN):
+                    continue
+                hh = h[face // 2]
+                a = -1.0 / hh ** 2 if t == DIRICHLET else (-1.0 / hh if face & 1 else 1.0 / hh)
+                fr = _face3(face, f[d].shape)
+                f[d][fr] += a * np.asarray(g).reshape(f[d][fr].shape)
+            f[d] *= hx * hx
+        temp = {}
+        for lev in reversed(self.levels[1:]):
+            for d in lev:
+                temp[d] = f[d].copy()
+                for loc, c in self.children[d]:
+                    fr, fc = _face3(loc, f[d].shape), _face3(loc ^ 1, f[c].shape)
+                    temp[d][fr] -= temp[c][fc]
+                    f[d][fr] -= temp[c][fc]
+            for d in lev:
+                temp[d] = self._dsolve(d, temp[d])
+        r = self.root
+        for loc, c in self.children[r]:
+            f[r][_face3(loc, f[r].shape)] -= temp[c][_face3(loc ^ 1, f[c].shape)]
+        f[r] = self._dsolve(r, f[r])
+        for lev in self.levels[1:]:
+            for d in lev:
+                p, loc = self.parent[d], self.parent_loc[d]
+                f[d][_face3(loc, f[d].shape)] -= f[p][_face3(loc ^ 1, f[p].shape)]
+            for d in lev:
+                f[d] = self._dsolve(d, f[d])
+        return f

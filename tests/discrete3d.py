@@ -59,3 +59,60 @@ def refined(f, hx, hy, hz, bc, solve_fn, steps=2):
         r = (f.ravel().astype(np.longdouble) - y).astype(np.float64).reshape(f.shape)
         u = u + solve_fn(r)
     return u
+
+
+def face_index(loc, nx, ny, nz):
+    """Flat indices of face `loc` (x-, x+, y-, y+, z-, z+) of a box, in the
+    order of the ghost-data planes: x faces (j, k), y faces (i, k), z faces
+    (i, j), last index fastest."""
+    idx = np.arange(nx * ny * nz).reshape(nx, ny, nz)
+    sl = {0: idx[0, :, :], 1: idx[nx - 1, :, :], 2: idx[:, 0, :], 3: idx[:, ny - 1, :], 4: idx[:, :, 0],
+          5: idx[:, :, nz - 1]}[loc]
+    return sl.ravel()
+
+
+def effective_bc(boxes, conn):
+    bcs = [list(b[6]) for b in boxes]
+    for a, dr, b in conn:
+        bcs[a][dr] = A
+        bcs[b][dr ^ 1] = A
+    return bcs
+
+
+def assemble_composite(boxes, conn, fields, ghost=None):
+    """boxes: (nx, ny, nz, hx, hy, hz, bc6); conn: (a, dir, b); fields: raw f.
+    Returns (matrix, rhs, offsets) of the hx^2-scaled global 7-point system
+    with the faces named in conn coupled with coefficient 1
+    (concat_solver3d.cpp:97-157, 159-247)."""
+    bcs = effective_bc(boxes, conn)
+    offs = np.cumsum([0] + [b[0] * b[1] * b[2] for b in boxes])
+    blocks, rhs = [], np.zeros(offs[-1])
+    ghost = ghost or {}
+    for d, (nx, ny, nz, hx, hy, hz, _) in enumerate(boxes):
+        blocks.append(assemble(nx, ny, nz, hx, hy, hz, bcs[d]))
+        f = np.asarray(fields[d], dtype=np.float64).ravel().copy()
+        h = (hx, hy, hz)
+        for face in range(6):
+            g = ghost.get((d, face))
+            if g is None or bcs[d][face] not in (D, N):
+                continue
+            hh = h[face // 2]
+            a = -1.0 / hh ** 2 if bcs[d][face] == D else (-1.0 / hh if face & 1 else 1.0 / hh)
+            f[face_index(face, nx, ny, nz)] += a * np.asarray(g).ravel()
+        rhs[offs[d]:offs[d + 1]] = hx * hx * f
+    M = sp.block_diag(blocks, format="lil")
+    for a, dr, b in conn:
+        na, nb = boxes[a][:3], boxes[b][:3]
+        ia = offs[a] + face_index(dr, *na)
+        ib = offs[b] + face_index(dr ^ 1, *nb)
+        assert len(ia) == len(ib)
+        for p, q in zip(ia, ib):
+            M[p, q] += 1.0
+            M[q, p] += 1.0
+    return M.tocsc(), rhs, offs
+
+
+def solve_composite(boxes, conn, fields, ghost=None):
+    M, rhs, offs = assemble_composite(boxes, conn, fields, ghost)
+    u = spla.spsolve(M, rhs)
+    return [u[offs[d]:offs[d + 1]].reshape(boxes[d][:3]) for d in range(len(boxes))]
