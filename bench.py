@@ -305,6 +305,18 @@ def extras(fpa, ctx, torch, np, n):
                          "cpu_oracle_solves_per_s": round(1.0 / t_cpu, 1), "cpu_threads": O.num_threads()}
     del cat, src, wk
 
+    # the case API (C++ classes over the C ABI) on the same cross: fields kept
+    # on the device between solves by field2's mirror, and with a host round
+    # trip per solve (PCIe included)
+    prog = os.path.join(ROOT, "tests", "cpp", "_build", "case_cross_timing")
+    if os.path.exists(prog):
+        import subprocess
+        r = subprocess.run([prog, str(n), "20"], capture_output=True, text=True, timeout=300)
+        w = r.stdout.split()
+        if r.returncode == 0 and "device_resident_ms" in w:
+            out["case_api_cross"] = {"device_resident_ms": float(w[w.index("device_resident_ms") + 1]),
+                                     "host_roundtrip_ms": float(w[w.index("host_roundtrip_ms") + 1])}
+
     # adjoint on the cross, batch of 32: (forward + backward) / forward time
     doms, conn, fields, _ = cases.cross(n, seed=1, nrhs=32)
     cat = fpa.ConcatPoissonSolver2D(ctx, doms, conn, m=200, tol=1e-12, tol_is_relative=True, max_iter=100)

@@ -65,27 +65,127 @@ fp_ctx focalors_context()
 
 void field2::init(int in_nx, int in_ny, const std::string& in_name)
 {
-    nx   = in_nx;
-    ny   = in_ny;
-    name = in_name;
+    release_device();
+    nx      = in_nx;
+    ny      = in_ny;
+    name    = in_name;
+    host_ok = true;
     value.assign((size_t)nx * ny, 0.0);
 }
 
-void field2::clear(double v) { std::fill(value.begin(), value.end(), v); }
+void field2::release_device()
+{
+    if (dev) (void)hipFree(dev);
+    dev    = nullptr;
+    dev_ok = false;
+}
+
+field2::~field2() { release_device(); }
+
+field2::field2(const field2& o) { *this = o; }
+
+field2& field2::operator=(const field2& o)
+{
+    if (this == &o) return *this;
+    if (nx != o.nx || ny != o.ny) {
+        release_device();
+        nx = o.nx;
+        ny = o.ny;
+        value.assign((size_t)nx * ny, 0.0);
+    }
+    name = o.name;
+    if (o.dev_ok) {
+        // device to device: the copy stays where the values are; queued on the
+        // null stream, ordered before the solves and host copies that follow
+        if (!dev) {
+            (void)focalors_context();
+            check_hip(hipMalloc(&dev, sizeof(double) * std::max<size_t>(1, value.size())), "hipMalloc");
+        }
+        check_hip(hipMemcpyAsync(dev, o.dev, sizeof(double) * value.size(), hipMemcpyDeviceToDevice, nullptr),
+                  "hipMemcpyAsync");
+        dev_ok  = true;
+        host_ok = false;
+    } else {
+        value   = o.value;
+        host_ok = true;
+        dev_ok  = false;
+    }
+    return *this;
+}
+
+field2::field2(field2&& o) noexcept { *this = std::move(o); }
+
+field2& field2::operator=(field2&& o) noexcept
+{
+    if (this == &o) return *this;
+    release_device();
+    nx      = o.nx;
+    ny      = o.ny;
+    value   = std::move(o.value);
+    name    = std::move(o.name);
+    dev     = o.dev;
+    host_ok = o.host_ok;
+    dev_ok  = o.dev_ok;
+    o.dev   = nullptr;
+    o.dev_ok = false;
+    o.nx = o.ny = 0;
+    o.value.clear();
+    o.host_ok = true;
+    return *this;
+}
+
+const double* field2::device_data() const
+{
+    if (!dev) {
+        (void)focalors_context();  // the device of the case API
+        check_hip(hipMalloc(&dev, sizeof(double) * std::max<size_t>(1, value.size())), "hipMalloc");
+        dev_ok = false;
+    }
+    if (!dev_ok) {
+        check_hip(hipMemcpy(dev, value.data(), sizeof(double) * value.size(), hipMemcpyHostToDevice), "hipMemcpy");
+        dev_ok = true;
+    }
+    return dev;
+}
+
+double* field2::device_data()
+{
+    const double* d = static_cast<const field2*>(this)->device_data();
+    host_ok         = false;
+    return const_cast<double*>(d);
+}
+
+void field2::sync_host() const
+{
+    if (host_ok) return;
+    check_hip(hipMemcpy(value.data(), dev, sizeof(double) * value.size(), hipMemcpyDeviceToHost), "hipMemcpy");
+    host_ok = true;
+}
+
+void field2::clear(double v)
+{
+    host_write();
+    std::fill(value.begin(), value.end(), v);
+}
 
 field2& field2::operator*=(double a)
 {
+    host_write();
     for (auto& x : value) x *= a;
     return *this;
 }
 
 void field2::add_affine_transform(double a, const field2& x, double b)
 {
+    host_write();
+    x.host_read();
     for (size_t q = 0; q < value.size(); q++) value[q] += a * x.value[q] + b;
 }
 
 double field2::dot(const field2& o) const
 {
+    host_read();
+    o.host_read();
     double s = 0.0;
     for (size_t q = 0; q < value.size(); q++) s += value[q] * o.value[q];
     return s;
@@ -95,6 +195,7 @@ double field2::norm() const { return std::sqrt(dot(*this)); }
 
 double field2::sum() const
 {
+    host_read();
     double s = 0.0;
     for (double v : value) s += v;
     return s;
@@ -363,22 +464,15 @@ void PoissonSolver2D::init(double hx, double hy, const std::array<PDEBoundaryTyp
     const unsigned char b[4] = {(unsigned char)bc_code(bc[0]), (unsigned char)bc_code(bc[1]),
                                 (unsigned char)bc_code(bc[2]), (unsigned char)bc_code(bc[3])};
     check_status(fp_sub_create(focalors_context(), nx, ny, hx, hy, b, &sub));
-    check_hip(hipMalloc(&d_f, sizeof(double) * (size_t)nx * ny), "hipMalloc");
 }
 
-PoissonSolver2D::~PoissonSolver2D()
-{
-    if (d_f) (void)hipFree(d_f);
-    fp_sub_destroy(sub);
-}
+PoissonSolver2D::~PoissonSolver2D() { fp_sub_destroy(sub); }
 
+// in place on the field's device copy (uploaded only if the host copy is newer)
 void PoissonSolver2D::solve(field2& f)
 {
     if (f.get_nx() != nx || f.get_ny() != ny) throw std::invalid_argument("PoissonSolver2D::solve: field size");
-    const size_t bytes = sizeof(double) * (size_t)nx * ny;
-    check_hip(hipMemcpy(d_f, f.data(), bytes, hipMemcpyHostToDevice), "hipMemcpy");
-    check_status(fp_sub_solve(sub, d_f, 1, (long long)nx * ny, nullptr));
-    check_hip(hipMemcpy(f.data(), d_f, bytes, hipMemcpyDeviceToHost), "hipMemcpy");
+    check_status(fp_sub_solve(sub, f.device_data(), 1, (long long)nx * ny, nullptr));
 }
 
 // ---- ConcatPoissonSolver2D --------------------------------------------------------
@@ -395,12 +489,12 @@ ConcatPoissonSolver2D::~ConcatPoissonSolver2D() { release(); }
 
 void ConcatPoissonSolver2D::release()
 {
-    for (auto* p : d_fields) (void)hipFree(p);
     for (auto* p : d_ghost)
         if (p) (void)hipFree(p);
     d_fields.clear();
     d_ghost.clear();
     d_ghost_c.clear();
+    ghost_sent.clear();
     if (cat) fp_cat_destroy(cat);
     cat = nullptr;
 }
@@ -456,8 +550,8 @@ void ConcatPoissonSolver2D::rebuild()
     d_fields.assign(nd, nullptr);
     d_ghost.assign(4 * nd, nullptr);
     d_ghost_c.assign(4 * nd, nullptr);
+    ghost_sent.assign(4 * nd, {});
     for (int d = 0; d < nd; d++) {
-        check_hip(hipMalloc(&d_fields[d], sizeof(double) * (size_t)nx[d] * ny[d]), "hipMalloc");
         for (int f = 0; f < 4; f++) {
             const PDEBoundaryType t = (PDEBoundaryType)bc[4 * d + f];
             if (t != PDEBoundaryType::Dirichlet && t != PDEBoundaryType::Neumann) continue;
@@ -474,6 +568,9 @@ Domain2DUniform* ConcatPoissonSolver2D::tree_root() const
     return r >= 0 && r < (int)order.size() ? order[r] : nullptr;
 }
 
+// The fields are solved in place on their device copies (uploaded only when
+// the host copy is newer) and stay there: the host copy is brought back when
+// the caller next reads it.  Ghost data are uploaded when they changed.
 void ConcatPoissonSolver2D::solve()
 {
     const int nd = (int)order.size();
@@ -482,22 +579,22 @@ void ConcatPoissonSolver2D::solve()
         auto  it = variable->field_map.find(s);
         if (it == variable->field_map.end()) throw std::runtime_error("ConcatPoissonSolver2D: domain without a field");
         field2& f = *it->second;
-        check_hip(hipMemcpy(d_fields[d], f.data(), sizeof(double) * f.get_size_n(), hipMemcpyHostToDevice), "hipMemcpy");
+        if (f.get_nx() != s->nx || f.get_ny() != s->ny)
+            throw std::runtime_error("ConcatPoissonSolver2D: field size differs from its domain");
+        d_fields[d] = f.device_data();
         for (int k = 0; k < 4; k++) {
             if (!d_ghost[4 * d + k]) continue;
-            auto& b = variable->buffer_map[s][kBoundaryLocations2D[k]];
+            auto&        b   = variable->buffer_map[s][kBoundaryLocations2D[k]];
             const size_t len = k < 2 ? s->ny : s->nx;
             std::vector<double> host(len, 0.0);
             std::copy(b.begin(), b.begin() + std::min(len, b.size()), host.begin());
+            if (host == ghost_sent[4 * d + k]) continue;
             check_hip(hipMemcpy(d_ghost[4 * d + k], host.data(), sizeof(double) * len, hipMemcpyHostToDevice),
                       "hipMemcpy");
+            ghost_sent[4 * d + k] = std::move(host);
         }
     }
     const int st = fp_cat_solve(cat, d_fields.data(), d_ghost_c.data(), 1, nullptr, &info);
     // the reference returns silently after maxIter; so does this layer
     if (st != FP_OK && st != FP_E_NOT_CONVERGED) check_status(st);
-    for (int d = 0; d < nd; d++) {
-        field2& f = *variable->field_map[order[d]];
-        check_hip(hipMemcpy(f.data(), d_fields[d], sizeof(double) * f.get_size_n(), hipMemcpyDeviceToHost), "hipMemcpy");
-    }
 }

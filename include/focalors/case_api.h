@@ -10,9 +10,8 @@
 //   DomainSolver2D     pe/concat/domain_solver.h:9-15
 //   PoissonSolver2D    pe/poisson/poisson_solver2d.h:10-34
 //   ConcatPoissonSolver2D  pe/concat/concat_solver2d.h:18-56
-// Fields live on the host as in the reference; each solve copies them to the
-// device, solves there and copies the result back (the handles keep the
-// device buffers between solves).
+// field2 keeps a device mirror: the solvers work on it in place, so a field
+// crosses PCIe only when the host reads or writes it between solves.
 #pragma once
 
 #include <array>
@@ -50,15 +49,24 @@ class field2
 public:
     field2() = default;
     field2(int in_nx, int in_ny, const std::string& in_name = "Default") { init(in_nx, in_ny, in_name); }
+    // copies take the source's current values and stay on the side where they
+    // are: a device-current source is copied device to device
+    field2(const field2& o);
+    field2& operator=(const field2& o);
+    field2(field2&& o) noexcept;
+    field2& operator=(field2&& o) noexcept;
+    ~field2();
     void init(int in_nx, int in_ny, const std::string& in_name = "Default");
 
-    double&       operator()(int i, int j) { return value[(size_t)i * ny + j]; }
-    double        operator()(int i, int j) const { return value[(size_t)i * ny + j]; }
+    // host access brings the host copy up to date first; a writable access
+    // makes the device copy stale
+    double&       operator()(int i, int j) { host_write(); return value[(size_t)i * ny + j]; }
+    double        operator()(int i, int j) const { host_read(); return value[(size_t)i * ny + j]; }
     int           get_nx() const { return nx; }
     int           get_ny() const { return ny; }
     int           get_size_n() const { return nx * ny; }
-    double*       data() { return value.data(); }
-    const double* data() const { return value.data(); }
+    double*       data() { host_write(); return value.data(); }
+    const double* data() const { host_read(); return value.data(); }
     std::string   get_name() const { return name; }
     void          clear(double v = 0.0);
     field2&       operator*=(double a);
@@ -68,10 +76,34 @@ public:
     double        sum() const;
     void          print() const;
 
+    // Device mirror (no reference counterpart; it mirrors the host storage of
+    // base/field/field2.cpp:55-72): the values are current on the host, on the
+    // device of focalors_context(), or on both, and each side is brought up to
+    // date when it is used, so that fields stay on the device across solves
+    // and cross PCIe only when the host touches them.
+    double*       device_data();        // current device copy, for writing: the host copy becomes stale
+    const double* device_data() const;  // current device copy, read only
+    void          sync_host() const;    // the host copy brought up to date
+    bool          host_current() const { return host_ok; }
+    bool          device_current() const { return dev_ok; }
+
 private:
-    int                 nx = 0, ny = 0;
-    std::vector<double> value;
-    std::string         name = "Default";
+    void host_read() const
+    {
+        if (!host_ok) sync_host();
+    }
+    void host_write()
+    {
+        host_read();
+        dev_ok = false;
+    }
+    void release_device();
+
+    int                         nx = 0, ny = 0;
+    mutable std::vector<double> value;
+    std::string                 name = "Default";
+    mutable double*             dev     = nullptr;
+    mutable bool                host_ok = true, dev_ok = false;
 };
 
 class Geometry2D;
@@ -184,7 +216,6 @@ private:
     void   init(double hx, double hy, const std::array<PDEBoundaryType, 4>& bc);
     int    nx = 0, ny = 0;
     fp_sub sub = nullptr;
-    double* d_f = nullptr;
 };
 
 class ConcatPoissonSolver2D
@@ -218,5 +249,6 @@ private:
     std::vector<double*>             d_fields;
     std::vector<const double*>       d_ghost_c;
     std::vector<double*>             d_ghost;
+    std::vector<std::vector<double>> ghost_sent;  // the ghost data last uploaded, per face
     fp_solve_info                    info{};
 };

@@ -87,3 +87,24 @@ def test_case_manufactured_second_order():
     assert errs[0] < 1e-3, errs
     for a, b in zip(errs, errs[1:]):
         assert 3.5 < a / b < 4.5, errs
+
+
+def test_case_field2_device_mirror():
+    """field2's device mirror: solves with and without host access in between
+    agree bit for bit, device-current fields copy device to device, host
+    writes after a solve reach the next one."""
+    banner("case_field2_mirror")
+    out = case_programs.run("case_field2_mirror")
+    assert "mirror ok" in out, out
+
+
+def test_case_cross_timing_small():
+    """The case-API timing program on a 128 cross: both forms (fields
+    device-resident between solves, host round trip per solve) give the same
+    solution."""
+    banner("case_cross_timing 128")
+    out = case_programs.run("case_cross_timing", 128, 3).split()
+    k = out.index("solution_norm")
+    a, b = float(out[k + 1]), float(out[k + 2])
+    assert abs(a - b) <= 1e-12 * abs(a), out
+    assert float(out[out.index("device_resident_ms") + 1]) > 0
