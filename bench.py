@@ -43,6 +43,9 @@ def parse():
                          "problems in FP64 and FP32, 4096^2 FP64 vs FP32+refine, L-shape, periodic channel, "
                          "adjoint fwd+bwd ratio); "
                          "prints one JSON line")
+    ap.add_argument("--only", default="",
+                    help="with --extras: a comma-separated subset of the workloads (cross_x64, batched, dirichlet_4096, "
+                         "lshape, periodic, single_256, case_api, adjoint, box_256, cross3d)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the per-kernel timing on the centre subdomain plus an 8 MiB device copy (the "
                          "calibration kernel of the rocprofv3 counter runs); no composite solve")
@@ -145,197 +148,262 @@ def flat_fields(torch, pristine):
     return src, dst, views
 
 
-def extras(fpa, ctx, torch, np, n):
-    """The other workloads of the solve path, one measurement each."""
+def extras(fpa, ctx, torch, np, n, only=""):
+    """The other workloads of the solve path, one measurement each (all of
+    them, or the comma-separated subset `only`)."""
     from focalors_poisson_amd import cases
     dev = torch.device("cuda:0")
     out = {}
+    sel = set(w for w in only.split(",") if w)
+    want = lambda name: not sel or name in sel
 
-    # cross with 64 right-hand sides (seeds 100..163), FP64, one call
-    doms, conn, fields, _ = cases.cross(n, seed=100, nrhs=64)
-    cat = fpa.ConcatPoissonSolver2D(ctx, doms, conn, m=200, tol=1e-12, tol_is_relative=True, max_iter=100)
-    pristine = [torch.from_numpy(f).to(dev) for f in fields]
-    work = [torch.empty_like(p) for p in pristine]
-    info = {}
-
-    def multi():
-        for w, p in zip(work, pristine):
-            w.copy_(p)
-        info.update(cat.solve(work))
-
-    t = timed(multi, torch)
-    one = [p[:1].clone() for p in pristine]
-    cat.solve(one)
-    torch.cuda.synchronize()
-    same = max(float((w[:1] - o).norm() / o.norm()) for w, o in zip(work, one))
-    out["cross_x64_rhs"] = {"rhs_solves_per_s": round(64 / t, 1), "ms_per_call": round(t * 1e3, 3),
-                            "gmres_rel_residual": info.get("rel_residual"),
-                            "rel_diff_rhs0_vs_single_rhs_solve": same}
-    del cat, pristine, work, one, fields
-
-    # 512 independent Dirichlet problems, sides log-uniform in [64, 1024] (seed 2), RHS U(-1, 1) (seed 3)
-    sizes = cases.batched_sizes(512, seed=2)
-    rng = np.random.default_rng(3)
-    t0 = time.perf_counter()
-    subs = [fpa.PoissonSolver2D(ctx, nx, ny, 1.0 / max(nx, ny), 1.0 / max(nx, ny), (0, 0, 0, 0)) for nx, ny in sizes]
-    t_plan = time.perf_counter() - t0
-    rhs = [torch.from_numpy(rng.uniform(-1, 1, (nx, ny))).to(dev) for nx, ny in sizes]
-    cells = sum(nx * ny for nx, ny in sizes)
-    for prec, dt in (("f64", torch.float64), ("f32", torch.float32)):
-        src, dst, wk = flat_fields(torch, [r.to(dt) for r in rhs])
-
-        def batch():
-            dst.copy_(src)  # one copy refills all 512 right-hand sides
-            fpa.solve_batched(ctx, subs, wk)
-
-        batch()  # first calls grow the stream-ordered and pinned staging pools
-        t = timed(batch, torch, reps=10)
-        out["batched_512_" + prec] = {"problems_per_s": round(512 / t, 1), "ms_per_batch": round(t * 1e3, 3),
-                                      "cells": cells, "plan_build_s": round(t_plan, 2)}
-    del subs, rhs
-
-    # 4096^2 Dirichlet, multiscale RHS (seed 4): FP64 vs FP32 + FP64 refinement to 1e-10
-    N4 = 4096
-    h = 1.0 / N4
-    f = cases.multiscale_rhs(N4, N4, h, h, 0.0, 0.0, seed=4) * h * h
-    sub = fpa.PoissonSolver2D(ctx, N4, N4, h, h, (0, 0, 0, 0))
-    src = torch.from_numpy(f).to(dev)
-    u64 = torch.empty_like(src)
-
-    def s64():
-        u64.copy_(src)
-        sub.solve(u64)
-
-    t64 = timed(s64, torch)
-    u32 = torch.empty_like(src)
-    rinfo = {}
-
-    def s32():
-        u32.copy_(src)
-        rinfo.update(sub.solve_refine(u32, tol=1e-10, max_steps=10))
-
-    t32 = timed(s32, torch)
-    out["dirichlet_4096"] = {"f64_ms": round(t64 * 1e3, 3), "f32_refine_ms": round(t32 * 1e3, 3),
-                             "refine_steps": rinfo.get("refine_steps"),
-                             "refine_last_correction": rinfo.get("refine_residual"),
-                             "rel_l2_f32_refine_vs_f64": float((u32 - u64).norm() / u64.norm())}
-    del sub, src, u64, u32
-
-    # L-shape of three 2048^2 Dirichlet domains: FP64 vs FP32 + FP64 refinement
-    doms, conn, fields, _ = cases.lshape(2048, seed=4)
-    src = [torch.from_numpy(f).to(dev) for f in fields]
-    res = {}
-    for prec, key in ((fpa.FP_F64, "f64"), (fpa.FP_F32_REFINE, "f32_refine")):
-        cat = fpa.ConcatPoissonSolver2D(ctx, doms, conn, m=200, tol=1e-12, tol_is_relative=True, max_iter=100,
-                                        precision=prec)
-        if prec == fpa.FP_F32_REFINE:
-            cat.set_refinement(tol=1e-12, max_steps=10)
-        wk = [torch.empty_like(x) for x in src]
+    if want('cross_x64'):
+        # cross with 64 right-hand sides (seeds 100..163), FP64, one call
+        doms, conn, fields, _ = cases.cross(n, seed=100, nrhs=64)
+        cat = fpa.ConcatPoissonSolver2D(ctx, doms, conn, m=200, tol=1e-12, tol_is_relative=True, max_iter=100)
+        pristine = [torch.from_numpy(f).to(dev) for f in fields]
+        work = [torch.empty_like(p) for p in pristine]
         info = {}
 
-        def solve_l():
-            for w, x in zip(wk, src):
-                w.copy_(x)
-            info.update(cat.solve(wk))
+        def multi():
+            for w, p in zip(work, pristine):
+                w.copy_(p)
+            info.update(cat.solve(work))
 
-        res[key] = (timed(solve_l, torch), dict(info), [w.clone() for w in wk])
-        del cat
-    d = max(float((a - b).norm() / b.norm()) for a, b in zip(res["f32_refine"][2], res["f64"][2]))
-    out["lshape_3x2048"] = {"f64_ms": round(res["f64"][0] * 1e3, 3),
-                            "f32_refine_ms": round(res["f32_refine"][0] * 1e3, 3),
-                            "refine_steps": res["f32_refine"][1].get("refine_steps"),
-                            "rel_l2_f32_refine_vs_f64": d,
-                            "gmres_rel_residual_f64": res["f64"][1].get("rel_residual")}
-    del res, src
+        t = timed(multi, torch)
+        one = [p[:1].clone() for p in pristine]
+        cat.solve(one)
+        torch.cuda.synchronize()
+        same = max(float((w[:1] - o).norm() / o.norm()) for w, o in zip(work, one))
+        out["cross_x64_rhs"] = {"rhs_solves_per_s": round(64 / t, 1), "ms_per_call": round(t * 1e3, 3),
+                                "gmres_rel_residual": info.get("rel_residual"),
+                                "rel_diff_rhs0_vs_single_rhs_solve": same}
+        del cat, pristine, work, one, fields
 
-    # periodic channel: two 1024^2 domains side by side, periodic in y,
-    # Dirichlet / Neumann x ends, coupled across x (the periodic transform on
-    # the two-level path), RHS N(0,1) (seed 9); FP64 and FP32 + refinement
-    hp = 1.0 / 1024
-    pdoms = [(1024, 1024, hp, hp, (0, 3, 2, 2)), (1024, 1024, hp, hp, (3, 1, 2, 2))]
-    pconn = [(0, 1, 1)]
-    prng = np.random.default_rng(9)
-    src = [torch.from_numpy(prng.standard_normal((d[0], d[1]))).to(dev) for d in pdoms]
-    res = {}
-    for prec, key in ((fpa.FP_F64, "f64"), (fpa.FP_F32_REFINE, "f32_refine")):
-        cat = fpa.ConcatPoissonSolver2D(ctx, pdoms, pconn, m=200, tol=1e-12, tol_is_relative=True, max_iter=100,
-                                        precision=prec)
-        if prec == fpa.FP_F32_REFINE:
-            cat.set_refinement(tol=1e-12, max_steps=10)
+    if want('batched'):
+        # 512 independent Dirichlet problems, sides log-uniform in [64, 1024] (seed 2), RHS U(-1, 1) (seed 3)
+        sizes = cases.batched_sizes(512, seed=2)
+        rng = np.random.default_rng(3)
+        t0 = time.perf_counter()
+        subs = [fpa.PoissonSolver2D(ctx, nx, ny, 1.0 / max(nx, ny), 1.0 / max(nx, ny), (0, 0, 0, 0)) for nx, ny in sizes]
+        t_plan = time.perf_counter() - t0
+        rhs = [torch.from_numpy(rng.uniform(-1, 1, (nx, ny))).to(dev) for nx, ny in sizes]
+        cells = sum(nx * ny for nx, ny in sizes)
+        for prec, dt in (("f64", torch.float64), ("f32", torch.float32)):
+            src, dst, wk = flat_fields(torch, [r.to(dt) for r in rhs])
+
+            def batch():
+                dst.copy_(src)  # one copy refills all 512 right-hand sides
+                fpa.solve_batched(ctx, subs, wk)
+
+            batch()  # first calls grow the stream-ordered and pinned staging pools
+            t = timed(batch, torch, reps=10)
+            out["batched_512_" + prec] = {"problems_per_s": round(512 / t, 1), "ms_per_batch": round(t * 1e3, 3),
+                                          "cells": cells, "plan_build_s": round(t_plan, 2)}
+        del subs, rhs
+
+    if want('dirichlet_4096'):
+        # 4096^2 Dirichlet, multiscale RHS (seed 4): FP64 vs FP32 + FP64 refinement to 1e-10
+        N4 = 4096
+        h = 1.0 / N4
+        f = cases.multiscale_rhs(N4, N4, h, h, 0.0, 0.0, seed=4) * h * h
+        sub = fpa.PoissonSolver2D(ctx, N4, N4, h, h, (0, 0, 0, 0))
+        src = torch.from_numpy(f).to(dev)
+        u64 = torch.empty_like(src)
+
+        def s64():
+            u64.copy_(src)
+            sub.solve(u64)
+
+        t64 = timed(s64, torch)
+        u32 = torch.empty_like(src)
+        rinfo = {}
+
+        def s32():
+            u32.copy_(src)
+            rinfo.update(sub.solve_refine(u32, tol=1e-10, max_steps=10))
+
+        t32 = timed(s32, torch)
+        out["dirichlet_4096"] = {"f64_ms": round(t64 * 1e3, 3), "f32_refine_ms": round(t32 * 1e3, 3),
+                                 "refine_steps": rinfo.get("refine_steps"),
+                                 "refine_last_correction": rinfo.get("refine_residual"),
+                                 "rel_l2_f32_refine_vs_f64": float((u32 - u64).norm() / u64.norm())}
+        del sub, src, u64, u32
+
+    if want('lshape'):
+        # L-shape of three 2048^2 Dirichlet domains: FP64 vs FP32 + FP64 refinement
+        doms, conn, fields, _ = cases.lshape(2048, seed=4)
+        src = [torch.from_numpy(f).to(dev) for f in fields]
+        res = {}
+        for prec, key in ((fpa.FP_F64, "f64"), (fpa.FP_F32_REFINE, "f32_refine")):
+            cat = fpa.ConcatPoissonSolver2D(ctx, doms, conn, m=200, tol=1e-12, tol_is_relative=True, max_iter=100,
+                                            precision=prec)
+            if prec == fpa.FP_F32_REFINE:
+                cat.set_refinement(tol=1e-12, max_steps=10)
+            wk = [torch.empty_like(x) for x in src]
+            info = {}
+
+            def solve_l():
+                for w, x in zip(wk, src):
+                    w.copy_(x)
+                info.update(cat.solve(wk))
+
+            res[key] = (timed(solve_l, torch), dict(info), [w.clone() for w in wk])
+            del cat
+        d = max(float((a - b).norm() / b.norm()) for a, b in zip(res["f32_refine"][2], res["f64"][2]))
+        out["lshape_3x2048"] = {"f64_ms": round(res["f64"][0] * 1e3, 3),
+                                "f32_refine_ms": round(res["f32_refine"][0] * 1e3, 3),
+                                "refine_steps": res["f32_refine"][1].get("refine_steps"),
+                                "rel_l2_f32_refine_vs_f64": d,
+                                "gmres_rel_residual_f64": res["f64"][1].get("rel_residual")}
+        del res, src
+
+    if want('periodic'):
+        # periodic channel: two 1024^2 domains side by side, periodic in y,
+        # Dirichlet / Neumann x ends, coupled across x (the periodic transform on
+        # the two-level path), RHS N(0,1) (seed 9); FP64 and FP32 + refinement
+        hp = 1.0 / 1024
+        pdoms = [(1024, 1024, hp, hp, (0, 3, 2, 2)), (1024, 1024, hp, hp, (3, 1, 2, 2))]
+        pconn = [(0, 1, 1)]
+        prng = np.random.default_rng(9)
+        src = [torch.from_numpy(prng.standard_normal((d[0], d[1]))).to(dev) for d in pdoms]
+        res = {}
+        for prec, key in ((fpa.FP_F64, "f64"), (fpa.FP_F32_REFINE, "f32_refine")):
+            cat = fpa.ConcatPoissonSolver2D(ctx, pdoms, pconn, m=200, tol=1e-12, tol_is_relative=True, max_iter=100,
+                                            precision=prec)
+            if prec == fpa.FP_F32_REFINE:
+                cat.set_refinement(tol=1e-12, max_steps=10)
+            wk = [torch.empty_like(x) for x in src]
+            info = {}
+
+            def solve_p():
+                for w, x in zip(wk, src):
+                    w.copy_(x)
+                info.update(cat.solve(wk))
+
+            res[key] = (timed(solve_p, torch), dict(info), [w.clone() for w in wk])
+            del cat
+        d = max(float((a - b).norm() / b.norm()) for a, b in zip(res["f32_refine"][2], res["f64"][2]))
+        out["periodic_channel_2x1024"] = {"f64_ms": round(res["f64"][0] * 1e3, 3),
+                                          "f32_refine_ms": round(res["f32_refine"][0] * 1e3, 3),
+                                          "refine_steps": res["f32_refine"][1].get("refine_steps"),
+                                          "rel_l2_f32_refine_vs_f64": d,
+                                          "gmres_iterations_f64": res["f64"][1].get("iterations"),
+                                          "gmres_rel_residual_f64": res["f64"][1].get("rel_residual")}
+        del res, src
+
+    if want('single_256'):
+        # single 256^2 Dirichlet domain (the reference's own CPU case): solves/s,
+        # error vs the oracle, and the oracle's own time on the host
+        doms, conn, fields, _ = cases.single(256)
+        cat = fpa.ConcatPoissonSolver2D(ctx, doms, conn, m=20, tol=1e-12, tol_is_relative=True)
+        src = [torch.from_numpy(f).to(dev) for f in fields]
         wk = [torch.empty_like(x) for x in src]
-        info = {}
 
-        def solve_p():
+        def solve_s():
             for w, x in zip(wk, src):
                 w.copy_(x)
-            info.update(cat.solve(wk))
+            cat.solve(wk)
 
-        res[key] = (timed(solve_p, torch), dict(info), [w.clone() for w in wk])
-        del cat
-    d = max(float((a - b).norm() / b.norm()) for a, b in zip(res["f32_refine"][2], res["f64"][2]))
-    out["periodic_channel_2x1024"] = {"f64_ms": round(res["f64"][0] * 1e3, 3),
-                                      "f32_refine_ms": round(res["f32_refine"][0] * 1e3, 3),
-                                      "refine_steps": res["f32_refine"][1].get("refine_steps"),
-                                      "rel_l2_f32_refine_vs_f64": d,
-                                      "gmres_iterations_f64": res["f64"][1].get("iterations"),
-                                      "gmres_rel_residual_f64": res["f64"][1].get("rel_residual")}
-    del res, src
+        ts = timed(solve_s, torch, reps=50)
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        t0 = time.perf_counter()
+        ref, _, _ = O.Composite(doms, conn).solve(fields)
+        t_cpu = time.perf_counter() - t0
+        u = wk[0].cpu().numpy()
+        out["single_256"] = {"solves_per_s": round(1.0 / ts, 1), "ms_per_solve": round(ts * 1e3, 4),
+                             "rel_l2_vs_oracle": float(np.linalg.norm(u - ref[0]) / np.linalg.norm(ref[0])),
+                             "cpu_oracle_solves_per_s": round(1.0 / t_cpu, 1), "cpu_threads": O.num_threads()}
+        del cat, src, wk
 
-    # single 256^2 Dirichlet domain (the reference's own CPU case): solves/s,
-    # error vs the oracle, and the oracle's own time on the host
-    doms, conn, fields, _ = cases.single(256)
-    cat = fpa.ConcatPoissonSolver2D(ctx, doms, conn, m=20, tol=1e-12, tol_is_relative=True)
-    src = [torch.from_numpy(f).to(dev) for f in fields]
-    wk = [torch.empty_like(x) for x in src]
+    if want('case_api'):
+        # the case API (C++ classes over the C ABI) on the same cross: fields kept
+        # on the device between solves by field2's mirror, and with a host round
+        # trip per solve (PCIe included)
+        prog = os.path.join(ROOT, "tests", "cpp", "_build", "case_cross_timing")
+        if os.path.exists(prog):
+            import subprocess
+            r = subprocess.run([prog, str(n), "20"], capture_output=True, text=True, timeout=300)
+            w = r.stdout.split()
+            if r.returncode == 0 and "device_resident_ms" in w:
+                out["case_api_cross"] = {"device_resident_ms": float(w[w.index("device_resident_ms") + 1]),
+                                         "host_roundtrip_ms": float(w[w.index("host_roundtrip_ms") + 1])}
 
-    def solve_s():
-        for w, x in zip(wk, src):
-            w.copy_(x)
-        cat.solve(wk)
+    if want('adjoint'):
+        # adjoint on the cross, batch of 32: (forward + backward) / forward time
+        doms, conn, fields, _ = cases.cross(n, seed=1, nrhs=32)
+        cat = fpa.ConcatPoissonSolver2D(ctx, doms, conn, m=200, tol=1e-12, tol_is_relative=True, max_iter=100)
+        fs = [torch.from_numpy(f).to(dev) for f in fields]
+        wk = [torch.empty_like(x) for x in fs]
 
-    ts = timed(solve_s, torch, reps=50)
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
-    t0 = time.perf_counter()
-    ref, _, _ = O.Composite(doms, conn).solve(fields)
-    t_cpu = time.perf_counter() - t0
-    u = wk[0].cpu().numpy()
-    out["single_256"] = {"solves_per_s": round(1.0 / ts, 1), "ms_per_solve": round(ts * 1e3, 4),
-                         "rel_l2_vs_oracle": float(np.linalg.norm(u - ref[0]) / np.linalg.norm(ref[0])),
-                         "cpu_oracle_solves_per_s": round(1.0 / t_cpu, 1), "cpu_threads": O.num_threads()}
-    del cat, src, wk
+        def fwd():
+            for w, x in zip(wk, fs):
+                w.copy_(x)
+            cat.solve(wk)
 
-    # the case API (C++ classes over the C ABI) on the same cross: fields kept
-    # on the device between solves by field2's mirror, and with a host round
-    # trip per solve (PCIe included)
-    prog = os.path.join(ROOT, "tests", "cpp", "_build", "case_cross_timing")
-    if os.path.exists(prog):
-        import subprocess
-        r = subprocess.run([prog, str(n), "20"], capture_output=True, text=True, timeout=300)
-        w = r.stdout.split()
-        if r.returncode == 0 and "device_resident_ms" in w:
-            out["case_api_cross"] = {"device_resident_ms": float(w[w.index("device_resident_ms") + 1]),
-                                     "host_roundtrip_ms": float(w[w.index("host_roundtrip_ms") + 1])}
+        def fwd_bwd():
+            fwd()
+            cat.vjp(wk)
 
-    # adjoint on the cross, batch of 32: (forward + backward) / forward time
-    doms, conn, fields, _ = cases.cross(n, seed=1, nrhs=32)
-    cat = fpa.ConcatPoissonSolver2D(ctx, doms, conn, m=200, tol=1e-12, tol_is_relative=True, max_iter=100)
-    fs = [torch.from_numpy(f).to(dev) for f in fields]
-    wk = [torch.empty_like(x) for x in fs]
+        tf = timed(fwd, torch)
+        tfb = timed(fwd_bwd, torch)
+        out["adjoint_cross_x32"] = {"forward_ms": round(tf * 1e3, 3), "forward_backward_ms": round(tfb * 1e3, 3),
+                                    "ratio": round(tfb / tf, 3)}
+    # one 256^3 Dirichlet box (PoissonSolver3D), FP64: solves/s and the
+    # algorithmic bandwidth (7 passes of 16 B per cell: z, y transforms both
+    # ways, two plane transposes, the x sweep)
+    if want('box_256'):
+        nb = 256
+        hb = 1.0 / nb
+        sub3 = fpa.PoissonSolver3D(ctx, nb, nb, nb, hb, hb, hb, (0, 0, 0, 0, 0, 0))
+        g = torch.Generator(device=dev)
+        g.manual_seed(5)
+        src = torch.rand((nb, nb, nb), dtype=torch.float64, device=dev, generator=g) * hb * hb
+        u3 = torch.empty_like(src)
 
-    def fwd():
-        for w, x in zip(wk, fs):
-            w.copy_(x)
-        cat.solve(wk)
+        def s3():
+            u3.copy_(src)
+            sub3.solve(u3)
 
-    def fwd_bwd():
-        fwd()
-        cat.vjp(wk)
+        t3 = timed(s3, torch, reps=10)
+        tc = timed(lambda: u3.copy_(src), torch, reps=10)
+        out["box_256_f64"] = {"ms_per_solve": round(t3 * 1e3, 3), "refill_copy_ms": round(tc * 1e3, 3),
+                              "solves_per_s": round(1.0 / t3, 1),
+                              "gbs_7_passes": round(7 * 16.0 * nb ** 3 / (t3 - tc) / 1e9, 1)}
+        del sub3, src, u3
 
-    tf = timed(fwd, torch)
-    tfb = timed(fwd_bwd, torch)
-    out["adjoint_cross_x32"] = {"forward_ms": round(tf * 1e3, 3), "forward_backward_ms": round(tfb * 1e3, 3),
-                                "ratio": round(tfb / tf, 3)}
+    # a 3-D cross: centre 64^3 with six arms 32 x 64 x 64 (Dirichlet outer
+    # faces but Neumann on the far faces of the x arms), one FP64 composite solve
+    if want('cross3d'):
+        nc, na = 64, 32
+        hc = 1.0 / nc
+        A3, D3, N3 = 3, 0, 1
+        boxes = [(nc, nc, nc, hc, hc, hc, (A3, A3, A3, A3, A3, A3)),
+                 (na, nc, nc, hc, hc, hc, (N3, A3, D3, D3, D3, D3)), (na, nc, nc, hc, hc, hc, (A3, N3, D3, D3, D3, D3)),
+                 (nc, na, nc, hc, hc, hc, (D3, D3, D3, A3, D3, D3)), (nc, na, nc, hc, hc, hc, (D3, D3, A3, D3, D3, D3)),
+                 (nc, nc, na, hc, hc, hc, (D3, D3, D3, D3, D3, A3)), (nc, nc, na, hc, hc, hc, (D3, D3, D3, D3, A3, D3))]
+        conn3 = [(0, 0, 1), (0, 1, 2), (0, 2, 3), (0, 3, 4), (0, 4, 5), (0, 5, 6)]
+        t0 = time.perf_counter()
+        c3 = fpa.ConcatPoissonSolver3D(ctx, boxes, conn3, m=60, tol=1e-12, tol_is_relative=True, max_iter=100)
+        torch.cuda.synchronize()
+        tb = time.perf_counter() - t0
+        g = torch.Generator(device=dev)
+        g.manual_seed(6)
+        srcs = [torch.randn(b[:3], dtype=torch.float64, device=dev, generator=g) for b in boxes]
+        wks = [torch.empty_like(x) for x in srcs]
+        inf3 = {}
+
+        def sc3():
+            for w, x in zip(wks, srcs):
+                w.copy_(x)
+            inf3.update(c3.solve(wks))
+
+        tt = timed(sc3, torch, reps=3)
+        out["cross3d_64"] = {"ms_per_solve": round(tt * 1e3, 3), "construct_s": round(tb, 2),
+                             "gmres_iterations": inf3.get("iterations"), "rel_residual": inf3.get("rel_residual"),
+                             "cells": sum(b[0] * b[1] * b[2] for b in boxes)}
+        del c3, srcs, wks
     return out
 
 
@@ -356,7 +424,7 @@ def main():
     doms, conn, fields, _ = cross(n, seed=1)
     ctx = fpa.Context(0)
     if a.extras:
-        print(json.dumps({"extras": extras(fpa, ctx, torch, np, n)}), flush=True)
+        print(json.dumps({"extras": extras(fpa, ctx, torch, np, n, a.only)}), flush=True)
         return
     if a.kernels_only:
         kern = centre_kernels(fpa, ctx, doms, torch, a.kernel_reps)

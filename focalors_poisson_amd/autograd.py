@@ -13,11 +13,14 @@ class _CompositeSolve(torch.autograd.Function):
         solver.solve(fields, ghost)
         ctx.solver = solver
         ctx.ghost_keys = ghost_keys
+        ctx.like = [(f.shape, f.dtype, f.device) for f in fields]
         return tuple(fields)
 
     @staticmethod
     def backward(ctx, *grad_u):
-        gu = [g if g is not None else torch.zeros(()) for g in grad_u]
+        # an output the loss does not use has no gradient: zeros of its shape
+        gu = [g if g is not None else torch.zeros(shape, dtype=dt, device=dev)
+              for g, (shape, dt, dev) in zip(grad_u, ctx.like)]
         gf, gg = ctx.solver.vjp([g.contiguous() for g in gu], faces=ctx.ghost_keys)
         return (None, None, *gf, *[gg[k].view_as(gg[k]) for k in ctx.ghost_keys])
 

@@ -2147,24 +2147,28 @@ int fp_cat_vjp(fp_cat cat, const double* const* d_gu, double* const* d_gf, doubl
 {
     int st = guarded([&] {
         FP_REQUIRE(cat && d_gu && d_gf && nrhs >= 1, "null argument");
+        for (int d = 0; d < cat->nd; d++) FP_REQUIRE(d_gu[d] && d_gf[d], "null field");
         FP_HIP(hipSetDevice(cat->ctx->device));
-        hipStream_t                     s  = (hipStream_t)stream;
-        const int                       nd = cat->nd;
-        std::vector<fp::DevBuf<double>> lam(nd);
-        std::vector<double*>            lp(nd);
-        for (int d = 0; d < nd; d++) {
-            FP_REQUIRE(d_gu[d] && d_gf[d], "null field");
-            const long long n = fp::dom_n(cat, d);
-            lam[d].alloc((size_t)n * nrhs);
-            lp[d] = lam[d].p;
-            // the solve multiplies by hx^2 after assembly: pre-divide so that b = dL/du
-            fp::launch_axpby(d_gu[d], lam[d].p, n * nrhs, 1.0 / (cat->hx[d] * cat->hx[d]), 0.0, s);
+        hipStream_t s  = (hipStream_t)stream;
+        const int   nd = cat->nd;
+        cat->last_refine_steps = 0;
+        cat->last_refine_est = cat->last_refine_corr = 0.0;
+        // lambda = A^-1 dL/du into d_gf: dL/du is the assembled, scaled
+        // right-hand side as it stands (no ghost terms, no hx^2 factor), read
+        // by the forward transforms of the two-level path directly
+        if (cat->precision == FP_F64 && fp::fast_ok(cat)) {
+            fp::cat_sweep_fast_t<double>(cat, d_gf, nullptr, nrhs, s, d_gu, true);
+        } else {
+            for (int d = 0; d < nd; d++)
+                if (d_gf[d] != d_gu[d])
+                    FP_HIP(hipMemcpyAsync(d_gf[d], d_gu[d], sizeof(double) * fp::dom_n(cat, d) * nrhs,
+                                          hipMemcpyDeviceToDevice, s));
+            if (cat->precision != FP_F64) fp::cat_refine(cat, d_gf, nrhs, s);
+            else fp::cat_sweep(cat, d_gf, nrhs, s);
         }
-        fp::cat_solve(cat, lp.data(), nullptr, nrhs, s);
         for (int d = 0; d < nd; d++) {
             const long long n   = fp::dom_n(cat, d);
             const double    hx2 = cat->hx[d] * cat->hx[d], hx = cat->hx[d], hy = cat->hy[d];
-            fp::launch_axpby(lam[d].p, d_gf[d], n * nrhs, hx2, 0.0, s);
             for (int face = 0; face < 4; face++) {
                 double* out = d_gghost ? d_gghost[4 * d + face] : nullptr;
                 if (!out) continue;
@@ -2176,15 +2180,16 @@ int fp_cat_vjp(fp_cat cat, const double* const* d_gu, double* const* d_gf, doubl
                     coef           = (face == fp::XNEG || face == fp::YNEG) ? 1.0 / h : -1.0 / h;
                 } else
                     coef = 0.0;  // Adjacented / Null faces take no ghost data
-                fp::launch_face_extract(lam[d].p, cat->nx[d], cat->ny[d], n, face, hx2 * coef, out, nrhs, s);
+                fp::launch_face_extract(d_gf[d], cat->nx[d], cat->ny[d], n, face, hx2 * coef, out, nrhs, s);
             }
+            // dL/df = hx^2 lambda, in place
+            fp::launch_axpby(d_gf[d], d_gf[d], n * nrhs, hx2, 0.0, s);
         }
-        FP_HIP(hipStreamSynchronize(s));
         if (info) {
-            info->iterations      = cat->last_steps;
-            info->restarts        = cat->last_restarts;
-            info->residual        = cat->last_res;
-            info->rel_residual    = cat->last_rel;
+            info->iterations        = cat->last_steps;
+            info->restarts          = cat->last_restarts;
+            info->residual          = cat->last_res;
+            info->rel_residual      = cat->last_rel;
             info->refine_steps      = cat->last_refine_steps;
             info->refine_residual   = cat->last_refine_est;
             info->refine_correction = cat->last_refine_corr;

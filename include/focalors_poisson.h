@@ -263,7 +263,9 @@ int fp_cat_graph_launch(fp_cat cat, void* stream, fp_solve_info* info);
  * dL/du in d_gu[d] (nrhs fields per domain), writes dL/df into d_gf[d] and
  * dL/dg for the ghost data of every face into d_gghost[4*d+face] (zero for
  * Adjacented faces; NULL entries are skipped).  d_gghost may be NULL.
- * Costs one composite solve: the assembled operator is symmetric. */
+ * Costs one composite solve (the assembled operator is symmetric), which
+ * reads d_gu directly, plus one scaling pass over d_gf; allocates nothing
+ * beyond the handle's solve workspace. */
 int fp_cat_vjp(fp_cat cat, const double* const* d_gu, double* const* d_gf, double* const* d_gghost, int nrhs,
                void* stream, fp_solve_info* info);
 

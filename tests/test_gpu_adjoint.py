@@ -68,12 +68,23 @@ def test_vjp_matches_oracle_gradient(setup):
     gu = [u - t for u, t in zip(us, target)]
     gf, gg = cat.vjp([to_dev(g) for g in gu], faces=list(ghost))
     torch.cuda.synchronize()
-    # oracle: lambda = A^-1 dL/du through the oracle's composite solve of dL/du / hx^2
-    oc = O.Composite(doms, conn, tol=1e-14, tol_is_relative=True, maxiter=400)
-    lam, _, _ = oc.solve([g / (d[2] ** 2) for g, d in zip(gu, doms)])
+    # oracle: lambda = A^-1 dL/du through the converged oracle solve of dL/du / hx^2
+    lam = O.reference_solution(doms, conn, [g / (d[2] ** 2) for g, d in zip(gu, doms)])
     ref_f = [d[2] ** 2 * l for d, l in zip(doms, lam)]
     banner("adjoint vs oracle gradient")
-    assert rel([g.cpu().numpy() for g in gf], ref_f) < 1e-10
+    assert rel([g.cpu().numpy() for g in gf], ref_f) < 1e-12
+    # dL/dg on each face with data: hx^2 c lambda on the face's line, c the
+    # ghost coefficient of the assembly (Dirichlet -1/h^2, Neumann +-1/h)
+    lines = {0: lambda x: x[0, :], 1: lambda x: x[-1, :], 2: lambda x: x[:, 0], 3: lambda x: x[:, -1]}
+    got, ref = [], []
+    for (d, face) in ghost:
+        hx, hy = doms[d][2], doms[d][3]
+        h = hx if face < 2 else hy
+        t = doms[d][4][face]
+        c = -1.0 / h ** 2 if t == D else (1.0 / h if face in (0, 2) else -1.0 / h)
+        ref.append(hx * hx * c * lines[face](lam[d]))
+        got.append(gg[(d, face)].cpu().numpy())
+    assert rel(got, ref) < 1e-12
 
 
 def test_autograd_function(setup):
@@ -94,3 +105,21 @@ def test_autograd_function(setup):
         assert torch.allclose(a.grad, b, rtol=1e-12, atol=1e-12 * float(b.abs().max()))
     for k in ghost:
         assert torch.allclose(gt[k].grad, gg[k], rtol=1e-12, atol=1e-12 * float(gg[k].abs().max()))
+
+
+def test_autograd_unused_output(setup):
+    """A loss that uses only the centre's solution: the other outputs get no
+    gradient from autograd (None), which the backward takes as zeros of their
+    shape; the gradient equals the vjp with zero dL/du on those domains."""
+    import torch
+    from focalors_poisson_amd.autograd import composite_solve
+    fpa, cat, doms, conn, fs, ghost, target = setup
+    ft = [to_dev(f).requires_grad_(True) for f in fs]
+    us = composite_solve(cat, ft)
+    loss = 0.5 * ((us[0] - to_dev(target[0])) ** 2).sum()
+    loss.backward()
+    gu = [us[0].detach() - to_dev(target[0])] + [torch.zeros_like(u) for u in us[1:]]
+    gf, _ = cat.vjp(gu)
+    banner("autograd with unused outputs")
+    for a, b in zip(ft, gf):
+        assert torch.allclose(a.grad, b, rtol=1e-12, atol=1e-12 * float(b.abs().max()))
