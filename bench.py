@@ -46,6 +46,9 @@ def parse():
     ap.add_argument("--only", default="",
                     help="with --extras: a comma-separated subset of the workloads (cross_x64, batched, dirichlet_4096, "
                          "lshape, periodic, single_256, case_api, adjoint, box_256, cross3d)")
+    ap.add_argument("--forms", default="",
+                    help="a comma-separated subset of the solve forms to time (f64, f64_graph, f64_into, "
+                         "f64_graph_into, f64_refine_graph_into, f32_refine); default all")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the per-kernel timing on the centre subdomain plus an 8 MiB device copy (the "
                          "calibration kernel of the rocprofv3 counter runs); no composite solve")
@@ -122,6 +125,16 @@ def composite_transform(fpa, ctx, doms, conn, torch, reps, nrhs=1):
         t = time_kernel(lambda: cat.transform(bufs, inverse=inv), reps, torch)
         kern[name] = {"ms": round(t, 5), "bytes": nbytes, "gbs": round(nbytes / (t * 1e-3) / 1e9, 1)}
     return kern
+
+
+def copy_kernel(torch, n, reps):
+    """A device-to-device copy of n*n doubles (8 MiB at n = 1024): the
+    calibration kernel of the rocprofv3 FETCH_SIZE / WRITE_SIZE runs, whose
+    algorithmic traffic is known exactly (one read and one write)."""
+    src = torch.randn(n * n, dtype=torch.float64, device=torch.device("cuda:0"))
+    dst = torch.empty_like(src)
+    t = time_kernel(lambda: dst.copy_(src), reps, torch)
+    return {"copy_8MiB": {"ms": round(t, 5), "bytes": 16.0 * n * n, "gbs": round(16.0 * n * n / (t * 1e-3) / 1e9, 1)}}
 
 
 def timed(fn, torch, reps=3):
@@ -429,10 +442,7 @@ def main():
     if a.kernels_only:
         kern = centre_kernels(fpa, ctx, doms, torch, a.kernel_reps)
         kern.update(composite_transform(fpa, ctx, doms, conn, torch, a.kernel_reps))
-        src = torch.randn(n * n, dtype=torch.float64, device=dev)
-        dst = torch.empty_like(src)
-        t = time_kernel(lambda: dst.copy_(src), a.kernel_reps, torch)
-        kern["copy_8MiB"] = {"ms": round(t, 5), "bytes": 16.0 * n * n, "gbs": round(16.0 * n * n / (t * 1e-3) / 1e9, 1)}
+        kern.update(copy_kernel(torch, n, a.kernel_reps))
         print(json.dumps({"kernels": kern}), flush=True)
         return
     cells = sum(d[0] * d[1] for d in doms)
@@ -501,19 +511,25 @@ def main():
         torch.cuda.synchronize()
         return {"ms": ms, "build_s": t_build, "info": dict(info), "u": [w.cpu().numpy() for w in work]}
 
-    runs = {
-        "f64": run_variant(fpa.FP_F64),
-        "f64_graph": run_graph(),
-        "f64_into": run_variant(fpa.FP_F64, into=True),
-        "f64_graph_into": run_graph(into=True),
-        "f64_refine_graph_into": run_graph(into=True, precision=fpa.FP_F64_REFINE),
-        "f32_refine": run_variant(fpa.FP_F32_REFINE),
+    forms = {
+        "f64": lambda: run_variant(fpa.FP_F64),
+        "f64_graph": lambda: run_graph(),
+        "f64_into": lambda: run_variant(fpa.FP_F64, into=True),
+        "f64_graph_into": lambda: run_graph(into=True),
+        "f64_refine_graph_into": lambda: run_graph(into=True, precision=fpa.FP_F64_REFINE),
+        "f32_refine": lambda: run_variant(fpa.FP_F32_REFINE),
     }
+    pick = [f for f in a.forms.split(",") if f] or list(forms)
+    for f in pick:
+        if f not in forms:
+            sys.exit("bench.py: unknown form %r (known: %s)" % (f, ", ".join(forms)))
+    runs = {f: forms[f]() for f in pick}
 
     # ---- the dominant launch of the solve, timed alone: the batched y
     # transform of all five domains (the solve's first and last stages) ----
     kern = composite_transform(fpa, ctx, doms, conn, torch, a.kernel_reps)
     kern.update(centre_kernels(fpa, ctx, doms, torch, a.kernel_reps))
+    kern.update(copy_kernel(torch, n, a.kernel_reps))
     dom_name = "cat_ytransform_inv"
     dom = kern[dom_name]
     # the same kernels on 32 right-hand sides: their throughput once launch
@@ -565,7 +581,8 @@ def main():
     def passed(k, bar):
         ok = runs[k]["info"].get("status", 0) == 0
         return ok and (err[k] is None or err[k] <= bar)
-    cands = [k for k in ("f64", "f64_graph", "f64_into", "f64_graph_into") if passed(k, 1e-10)] or ["f64"]
+    plain = [k for k in ("f64", "f64_graph", "f64_into", "f64_graph_into") if k in runs]
+    cands = [k for k in plain if passed(k, 1e-10)] or plain or list(runs)
     main_key = min(cands, key=lambda k: runs[k]["ms"])
     main_v = runs[main_key]
     # the fastest form within 1e-12 of the discrete solution
