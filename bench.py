@@ -194,26 +194,59 @@ def extras(fpa, ctx, torch, np, n, only=""):
         del cat, pristine, work, one, fields
 
     if want('batched'):
-        # 512 independent Dirichlet problems, sides log-uniform in [64, 1024] (seed 2), RHS U(-1, 1) (seed 3)
+        # 512 independent Dirichlet problems, sides log-uniform in [64, 1024] (seed 2), RHS U(-1, 1) (seed 3).
+        # Per-call batched solve (fp_sub_solve_batched) with timed plans, and the prepared batch
+        # (fp_batch) replayed from a graph, with timed and with estimated plans.  Algorithmic bytes: one
+        # read and one write of every field per phase (transform, sweep, inverse transform).
         sizes = cases.batched_sizes(512, seed=2)
         rng = np.random.default_rng(3)
-        t0 = time.perf_counter()
-        subs = [fpa.PoissonSolver2D(ctx, nx, ny, 1.0 / max(nx, ny), 1.0 / max(nx, ny), (0, 0, 0, 0)) for nx, ny in sizes]
-        t_plan = time.perf_counter() - t0
         rhs = [torch.from_numpy(rng.uniform(-1, 1, (nx, ny))).to(dev) for nx, ny in sizes]
         cells = sum(nx * ny for nx, ny in sizes)
+        est = fpa.Context(0)
+        est.set_planning(fpa.FP_PLAN_ESTIMATE)
+        plans = {}
+        for name, c in (("measure", ctx), ("estimate", est)):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            plans[name] = [fpa.PoissonSolver2D(c, nx, ny, 1.0 / max(nx, ny), 1.0 / max(nx, ny), (0, 0, 0, 0))
+                           for nx, ny in sizes]
+            torch.cuda.synchronize()
+            out["batched_512_plan_build_s_" + name] = round(time.perf_counter() - t0, 3)
         for prec, dt in (("f64", torch.float64), ("f32", torch.float32)):
             src, dst, wk = flat_fields(torch, [r.to(dt) for r in rhs])
+            nbytes = 3 * 2 * cells * (8 if prec == "f64" else 4)
 
             def batch():
                 dst.copy_(src)  # one copy refills all 512 right-hand sides
-                fpa.solve_batched(ctx, subs, wk)
+                fpa.solve_batched(ctx, plans["measure"], wk)
 
             batch()  # first calls grow the stream-ordered and pinned staging pools
             t = timed(batch, torch, reps=10)
-            out["batched_512_" + prec] = {"problems_per_s": round(512 / t, 1), "ms_per_batch": round(t * 1e3, 3),
-                                          "cells": cells, "plan_build_s": round(t_plan, 2)}
-        del subs, rhs
+            res = {"per_call_ms": round(t * 1e3, 3)}
+            for name in ("measure", "estimate"):
+                b = fpa.BatchSolve(ctx, plans[name], wk)
+                stream = torch.cuda.Stream()
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.stream(stream):
+                    g.capture_begin()
+                    b.solve(stream=stream)
+                    g.capture_end()
+                torch.cuda.synchronize()
+
+                def replay():
+                    dst.copy_(src)
+                    g.replay()
+
+                tg = timed(replay, torch, reps=10)
+                # the refill copy alone, to report the solve's own time
+                tc = timed(lambda: dst.copy_(src), torch, reps=10)
+                res["prepared_graph_ms_" + name] = round(tg * 1e3, 3)
+                res["prepared_graph_solve_only_ms_" + name] = round((tg - tc) * 1e3, 3)
+                res["prepared_gbs_" + name] = round(nbytes / (tg - tc) / 1e9, 1)
+                del g, b
+            res.update({"cells": cells, "algorithmic_bytes": nbytes, "refill_copy_ms": round(tc * 1e3, 3)})
+            out["batched_512_" + prec] = res
+        del plans, rhs
 
     if want('dirichlet_4096'):
         # 4096^2 Dirichlet, multiscale RHS (seed 4): FP64 vs FP32 + FP64 refinement to 1e-10

@@ -15,7 +15,8 @@ stream provider only); every computation runs in libfocalors_poisson.so.
 import ctypes
 
 from . import _capi
-from ._capi import FP_F32_REFINE, FP_F64, FP_F64_REFINE, FPError, SolveInfo  # noqa: F401
+from ._capi import (FP_F32_REFINE, FP_F64, FP_F64_REFINE, FP_PLAN_ESTIMATE, FP_PLAN_MEASURE,  # noqa: F401
+                    FPError, SolveInfo)
 
 DIRICHLET, NEUMANN, PERIODIC, ADJACENTED, NULL = 0, 1, 2, 3, 4
 XNEG, XPOS, YNEG, YPOS = 0, 1, 2, 3
@@ -35,6 +36,12 @@ class Context:
         self._h = ctypes.c_void_p()
         _capi.check(_capi.lib().fp_ctx_create(int(device), ctypes.byref(self._h)))
         self.device = device
+
+    def set_planning(self, rigor):
+        """FP_PLAN_MEASURE (timed launch shapes, the default) or
+        FP_PLAN_ESTIMATE (estimated shapes, no timing) for the plans created
+        afterwards (fp_ctx_set_planning)."""
+        _capi.check(_capi.lib().fp_ctx_set_planning(self._h, int(rigor)))
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -169,6 +176,35 @@ def solve_batched(ctx, solvers, fields, stream=None, noalloc=False):
     else:
         fn = _capi.lib().fp_sub_solve_batched if f64 else _capi.lib().fp_sub_solve_batched_f32
     _capi.check(fn(ctx._h, k, subs, ptrs, nr, _stream_ptr(stream)))
+
+
+class BatchSolve:
+    """fp_batch: a fixed batch of independent subdomain solves over bound
+    fields, prepared once; solve() queues only kernel launches and can be
+    captured into a graph (torch.cuda.graph).  The solvers and fields must
+    stay alive as long as the batch."""
+
+    def __init__(self, ctx, solvers, fields):
+        import torch
+        k = len(solvers)
+        if k == 0 or len(fields) != k:
+            raise ValueError("need one field per solver")
+        self.solvers, self.fields = list(solvers), list(fields)
+        fp32 = fields[0].dtype == torch.float32
+        subs = (ctypes.c_void_p * k)(*[s._h.value for s in solvers])
+        ptrs = (ctypes.c_void_p * k)(*[f.data_ptr() for f in fields])
+        nr = (ctypes.c_int * k)(*[_check_field(f, fields[0].dtype, s.nx * s.ny) for s, f in zip(solvers, fields)])
+        self._h = ctypes.c_void_p()
+        _capi.check(_capi.lib().fp_batch_create(ctx._h, k, subs, ptrs, nr, int(fp32), ctypes.byref(self._h)))
+
+    def solve(self, stream=None):
+        _capi.check(_capi.lib().fp_batch_solve(self._h, _stream_ptr(stream)))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and h.value:
+            _capi.lib().fp_batch_destroy(h)
+            self._h = None
 
 
 class ConcatPoissonSolver2D:

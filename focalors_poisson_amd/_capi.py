@@ -13,6 +13,7 @@ HEADER = os.path.join(os.path.dirname(PKG), "include", "focalors_poisson.h")
 
 FP_OK, FP_E_INVALID, FP_E_NOMEM, FP_E_HIP, FP_E_NOT_CONVERGED, FP_E_NUMERIC, FP_E_UNSUPPORTED = range(7)
 FP_F64, FP_F32_REFINE, FP_F64_REFINE = 0, 1, 2
+FP_PLAN_MEASURE, FP_PLAN_ESTIMATE = 0, 1
 
 _lib = None
 _vp = ctypes.c_void_p
@@ -74,6 +75,12 @@ def lib():
     L.fp_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(_vp)]
     L.fp_ctx_destroy.argtypes = [_vp]
     L.fp_ctx_destroy.restype = None
+    L.fp_ctx_set_planning.argtypes = [_vp, ctypes.c_int]
+    L.fp_batch_create.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _ip, ctypes.c_int,
+                                  ctypes.POINTER(_vp)]
+    L.fp_batch_solve.argtypes = [_vp, _vp]
+    L.fp_batch_destroy.argtypes = [_vp]
+    L.fp_batch_destroy.restype = None
     L.fp_sub_create.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                 ctypes.POINTER(ctypes.c_ubyte), ctypes.POINTER(_vp)]
     L.fp_sub_destroy.argtypes = [_vp]

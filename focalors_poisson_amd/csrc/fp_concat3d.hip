@@ -295,6 +295,7 @@ static void build_node3(Cat3* c, int d, hipStream_t s)
     nd->dom = d;
     int bc6[6];
     for (int f = 0; f < 6; f++) bc6[f] = c->bc[6 * d + f];
+    nd->plan.measure = c->ctx->measure;
     nd->plan.build(c->nx[d], c->ny[d], c->nz[d], c->hx[d], c->hy[d], c->hz[d], bc6);
     std::vector<int> cells;
     for (int loc = 0; loc < 6; loc++) {

@@ -213,7 +213,7 @@ struct TransformPlan {
     void build(int kind, int N);
 };
 // Times the plan's transform on an nx-row field and keeps the fastest
-// workgroup size (measure = false: the default).
+// workgroup size (measure = false: the estimated shape, no timing).
 template <typename T>
 void tune_transform(TransformPlan<T>& tp, int nx, bool measure);
 
@@ -236,6 +236,7 @@ struct SubPlan {
     bool                  cyc = false;
     DevBuf<double>        zc;
     int                   dev = 0;
+    bool                  measure = true;  // time the launch shapes (else the estimated ones)
     void build(int nx, int ny, double hx, double hy, const int bc[4]);
     void init_sweep(const std::vector<double>& delta);
     void ensure_f32();
@@ -309,6 +310,28 @@ struct TridiagJob {
 template <typename T>
 void launch_tridiag_batch(const TridiagJob<T>* jobs, int njobs, hipStream_t s);
 
+// A launch prepared once and replayed: its shape and its descriptor table
+// (nprob descriptors from offset 0, then nprob + 1 workgroup offsets from
+// start_off), which the owner copies to device memory; launching then reads
+// only that copy, so it queues a kernel and nothing else (graph capture
+// allowed).  Within a launch the problems are ordered by decreasing work per
+// workgroup, so that the longest workgroups are dispatched first.
+struct GroupPlan {
+    int                        engine = 1, nt = 0, odd = 0, ept = 0;  // transforms
+    int                        ch = 0;                               // sweeps: rows per chunk
+    int                        blocks = 0, threads = 0, nprob = 0;
+    size_t                     lds = 0, start_off = 0;
+    std::vector<unsigned char> table;
+};
+template <typename T>
+std::vector<GroupPlan> plan_ytransform_groups(const TransformJob<T>* jobs, int njobs, int inverse);
+template <typename T>
+void launch_ytransform_group(const GroupPlan& g, const unsigned char* dtable, hipStream_t s);
+template <typename T>
+std::vector<GroupPlan> plan_tridiag_groups(const TridiagJob<T>* jobs, int njobs);
+template <typename T>
+void launch_tridiag_group(const GroupPlan& g, const unsigned char* dtable, hipStream_t s);
+
 // Arguments of the line-sourced tridiagonal sweep (interface operator).
 template <typename T>
 struct LinesArgs {
@@ -371,4 +394,11 @@ void sub_solve_batch(SubPlan* const* plans, T* const* data, const long long* ld,
 struct fp_ctx_s {
     int            device  = 0;
     unsigned char* scratch = nullptr;  // this device's copy of the no-allocation scratch array
+    bool           measure = true;     // plans built from this context time their launch shapes
+};
+
+// A subdomain solver (fp_sub_create).
+struct fp_sub_s {
+    fp_ctx_s*   ctx = nullptr;
+    fp::SubPlan plan;
 };

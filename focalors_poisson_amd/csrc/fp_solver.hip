@@ -99,7 +99,7 @@ void SubPlan::init_sweep(const std::vector<double>& dl)
     // reciprocal Thomas pivots (the reference's c vector,
     // chasing_method_base.cpp:33-46) are regenerated inside the sweep from
     // one stored pivot excess per row chunk; the chunk schedule is tuned here
-    tune_tridiag(true);
+    tune_tridiag(measure);
 }
 
 void SubPlan::build(int nx_, int ny_, double hx_, double hy_, const int bc_[4])
@@ -124,14 +124,14 @@ void SubPlan::build(int nx_, int ny_, double hx_, double hy_, const int bc_[4])
     for (int j = 0; j < ny; j++) dl[j] = (double)(r * g[j]);
     init_sweep(dl);
     t64.build(kind, ny);
-    tune_transform(t64, nx, true);
+    tune_transform(t64, nx, measure);
 }
 
 void SubPlan::ensure_f32()
 {
     if (t32.L) return;
     t32.build(kind, ny);
-    tune_transform(t32, nx, true);
+    tune_transform(t32, nx, measure);
 }
 
 template <typename T>
@@ -298,11 +298,6 @@ struct FastLines {
 constexpr size_t kNoAllocScratchBytes = 4u << 20;
 static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 
-
-struct fp_sub_s {
-    fp_ctx_s*   ctx = nullptr;
-    fp::SubPlan plan;
-};
 
 struct fp_cat_s {
     fp_ctx_s*                              ctx = nullptr;
@@ -789,6 +784,7 @@ static void build_node(fp_cat_s* c, int d, hipStream_t s)
     nd->dom = d;
     int bc[4];
     for (int f = 0; f < 4; f++) bc[f] = c->bc[4 * d + f];
+    nd->plan.measure = c->ctx->measure;
     nd->plan.build(c->nx[d], c->ny[d], c->hx[d], c->hy[d], bc);
     std::vector<int>                 cells;
     std::map<int, std::vector<int>>  users;
@@ -1788,6 +1784,15 @@ int fp_ctx_create(int device, fp_ctx* out)
 
 void fp_ctx_destroy(fp_ctx ctx) { delete ctx; }
 
+int fp_ctx_set_planning(fp_ctx ctx, int rigor)
+{
+    return guarded([&] {
+        FP_REQUIRE(ctx, "null context");
+        FP_REQUIRE(rigor == FP_PLAN_MEASURE || rigor == FP_PLAN_ESTIMATE, "unknown planning rigor");
+        ctx->measure = rigor == FP_PLAN_MEASURE;
+    });
+}
+
 int fp_sub_create(fp_ctx ctx, int nx, int ny, double hx, double hy, const unsigned char bc[4], fp_sub* out)
 {
     return guarded([&] {
@@ -1796,6 +1801,7 @@ int fp_sub_create(fp_ctx ctx, int nx, int ny, double hx, double hy, const unsign
         auto s = std::make_unique<fp_sub_s>();
         s->ctx = ctx;
         int b[4] = {bc[0], bc[1], bc[2], bc[3]};
+        s->plan.measure = ctx->measure;
         s->plan.build(nx, ny, hx, hy, b);
         *out = s.release();
     });

@@ -18,6 +18,7 @@ struct SubPlan3D {
     SubPlan               sweep;  // nx rows, ny nz modes
     DevBuf<double>        work;   // the y-contiguous copy [i][k][j] of every right-hand side
     int                   work_nrhs = 0;
+    bool                  measure = true;  // time the launch shapes (else the estimated ones)
     void                  build(int nx, int ny, int nz, double hx, double hy, double hz, const int bc[6]);
 };
 

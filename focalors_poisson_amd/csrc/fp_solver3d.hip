@@ -66,11 +66,12 @@ void SubPlan3D::build(int nx_, int ny_, int nz_, double hx_, double hy_, double 
     for (int f = 0; f < 6; f++) any = any || dir_like(bc[f]);
     sweep.no_dirichlet = !any;
     FP_HIP(hipGetDevice(&sweep.dev));
+    sweep.measure = measure;
     sweep.init_sweep(dl);
     tz.build(kz, nz);
-    tune_transform(tz, nx * ny, true);
+    tune_transform(tz, nx * ny, measure);
     ty.build(ky, ny);
-    tune_transform(ty, nx * nz, true);
+    tune_transform(ty, nx * nz, measure);
 }
 
 // Transpose of the a x b block of every x-plane: dst[s][q][p] = src[s][p][q],
@@ -157,6 +158,7 @@ int fp_sub3d_create(fp_ctx ctx, int nx, int ny, int nz, double hx, double hy, do
         s->ctx = ctx;
         int b[6];
         for (int f = 0; f < 6; f++) b[f] = bc[f];
+        s->plan.measure = ctx->measure;
         s->plan.build(nx, ny, nz, hx, hy, hz, b);
         *out = s.release();
     });

@@ -40,6 +40,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <complex>
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <set>
@@ -1460,6 +1461,71 @@ void launch_ytransform_batch(const TransformJob<T>* jobs, int njobs, int inverse
                         inverse, s);
 }
 
+// The launches of launch_ytransform_batch, prepared: one per group of
+// (engine, threads, odd-prime form, values per thread), each group's jobs in
+// decreasing FFT length (the work of one workgroup grows with it).
+template <typename T>
+std::vector<GroupPlan> plan_ytransform_groups(const TransformJob<T>* jobs, int njobs, int inverse)
+{
+    std::map<std::array<int, 4>, std::vector<const TransformJob<T>*>> groups;
+    for (int k = 0; k < njobs; k++) {
+        if (jobs[k].nrhs <= 0 || jobs[k].nx <= 0) continue;
+        const TransformPlan<T>& tp = *jobs[k].plan;
+        groups[{tp.engine, jobs[k].nt ? jobs[k].nt : tp.nt, job_form(jobs[k]), tp.ept}].push_back(&jobs[k]);
+    }
+    std::vector<GroupPlan> out;
+    for (auto& kv : groups) {
+        std::vector<const TransformJob<T>*>& js = kv.second;
+        std::stable_sort(js.begin(), js.end(), [](const TransformJob<T>* a, const TransformJob<T>* b) {
+            return a->plan->F > b->plan->F;
+        });
+        GroupPlan g;
+        g.engine = kv.first[0];
+        g.nt     = kv.first[1];
+        g.odd    = kv.first[2];
+        g.ept    = kv.first[3];
+        g.nprob  = (int)js.size();
+        std::vector<YArgs<T>> args(js.size());
+        std::vector<int>      start(js.size() + 1);
+        long long             blocks = 0;
+        for (size_t q = 0; q < js.size(); q++) {
+            const TransformJob<T>& j = *js[q];
+            args[q]                  = yargs(j, inverse);
+            start[q]                 = (int)blocks;
+            blocks += (long long)((j.nx + 1) / 2) * j.nrhs;
+            if (blocks > (1LL << 30)) throw Error(FP_E_INVALID, "too many transform workgroups in one launch");
+            g.lds = std::max(g.lds, lds_bytes(*j.plan, g.odd));
+            if (g.engine == 1 && !sb_fits(j.plan->rad, j.plan->nrad, j.plan->F, g.nt, g.odd, j.plan->ept))
+                throw Error(FP_E_INVALID, "transform launch shape does not fit the plan");
+        }
+        start[js.size()] = (int)blocks;
+        if (g.lds > 160 * 1024) throw Error(FP_E_INVALID, "transform length exceeds the LDS of one workgroup");
+        if (g.engine == 1 && g.odd == ODD_MFMA && sizeof(T) != 8) throw Error(FP_E_INVALID, "matrix-core passes are FP64");
+        g.blocks    = (int)blocks;
+        g.threads   = g.nt;
+        g.start_off = (args.size() * sizeof(YArgs<T>) + 255) & ~(size_t)255;
+        g.table.resize(g.start_off + start.size() * sizeof(int));
+        std::memcpy(g.table.data(), args.data(), args.size() * sizeof(YArgs<T>));
+        std::memcpy(g.table.data() + g.start_off, start.data(), start.size() * sizeof(int));
+        if (g.blocks > 0) out.push_back(std::move(g));
+    }
+    return out;
+}
+
+template <typename T>
+void launch_ytransform_group(const GroupPlan& g, const unsigned char* dtable, hipStream_t s)
+{
+    JobTable<YArgs<T>> t;
+    t.nprob = g.nprob;
+    t.p     = reinterpret_cast<const YArgs<T>*>(dtable);
+    t.start = reinterpret_cast<const int*>(dtable + g.start_off);
+    launch_kernel<T>(g.engine, g.odd, g.ept, g.blocks, g.nt, g.lds, s, t);
+}
+template std::vector<GroupPlan> plan_ytransform_groups<double>(const TransformJob<double>*, int, int);
+template std::vector<GroupPlan> plan_ytransform_groups<float>(const TransformJob<float>*, int, int);
+template void launch_ytransform_group<double>(const GroupPlan&, const unsigned char*, hipStream_t);
+template void launch_ytransform_group<float>(const GroupPlan&, const unsigned char*, hipStream_t);
+
 template <typename T>
 void launch_ytransform(const TransformPlan<T>& tp, T* data, long long ld, int nx, int nrhs, int inverse,
                        hipStream_t s)
@@ -1507,7 +1573,25 @@ void tune_transform(TransformPlan<T>& tp, int nx, bool measure)
                 return;
             }
     }
-    if (!measure || cand.size() < 2 || (long long)nx * tp.N < (1LL << 16)) return;
+    if (!measure && !cand.empty()) {
+        // estimated shape: the commonest winner of the timed choices, 128
+        // threads with three pairs per odd-prime work item, where it fits
+        std::array<int, 2> e = cand[0];
+        for (int form : {(int)ODD_VALU3, (int)ODD_VALU, (int)ODD_MFMA, (int)ODD_NONE}) {
+            bool found = false;
+            for (const auto& c : cand)
+                if (c[1] == form) {
+                    e     = c;
+                    found = true;
+                    break;
+                }
+            if (found) break;
+        }
+        tp.nt = tp.nt1 = e[0];
+        tp.odd = tp.odd1 = e[1];
+        return;
+    }
+    if (cand.size() < 2 || (long long)nx * tp.N < (1LL << 16)) return;
     const std::string key = "yt " + std::to_string(tp.kind) + " " + std::to_string(tp.N) + " " +
                             std::to_string(nx) + " " + std::to_string(sizeof(T));
     {

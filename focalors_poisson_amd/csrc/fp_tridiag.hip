@@ -751,6 +751,89 @@ void launch_tridiag_batch(const TridiagJob<T>* jobs, int njobs, hipStream_t s)
     if (!g32.empty()) launch_tridiag_group<T, 32>(g32.data(), (int)g32.size(), s);
 }
 
+// The launches of launch_tridiag_batch, prepared: one per chunk length, the
+// jobs of each in decreasing number of rows (the length of a workgroup's
+// chains).
+template <typename T>
+std::vector<GroupPlan> plan_tridiag_groups(const TridiagJob<T>* jobs, int njobs)
+{
+    std::map<int, std::vector<const TridiagJob<T>*>> groups;
+    for (int k = 0; k < njobs; k++)
+        if (jobs[k].nrhs > 0) groups[jobs[k].sp->tri_ch].push_back(&jobs[k]);
+    std::vector<GroupPlan> out;
+    for (auto& kv : groups) {
+        const int                          CH = kv.first;
+        std::vector<const TridiagJob<T>*>& js = kv.second;
+        std::stable_sort(js.begin(), js.end(), [](const TridiagJob<T>* a, const TridiagJob<T>* b) {
+            return a->sp->nx > b->sp->nx;
+        });
+        GroupPlan g;
+        g.ch      = CH;
+        g.nprob   = (int)js.size();
+        g.threads = 64;
+        std::vector<TriProb<T>> probs(js.size());
+        std::vector<int>        start(js.size() + 1);
+        long long               blocks = 0;
+        for (size_t q = 0; q < js.size(); q++) {
+            const TridiagJob<T>& jb = *js[q];
+            const SubPlan&       sp = *jb.sp;
+            const int            P  = sp.tri_p;
+            const int            MB = tri_mb(P, CH, sp.mb_field);
+            if (MB * P > tri_max_threads(CH)) throw Error(FP_E_INVALID, "nx too large for the tridiagonal kernel");
+            TriProb<T>& pr   = probs[q];
+            pr.data          = jb.data;
+            pr.ld            = jb.ld;
+            pr.xd            = sp.xd.p;
+            pr.pst           = sp.pst.p;
+            pr.nx            = sp.nx;
+            pr.ny            = sp.ny;
+            pr.P             = P;
+            pr.MB            = MB;
+            pr.singular_last = sp.no_dirichlet ? 1 : 0;
+            pr.nblk          = (sp.ny + MB - 1) / MB;
+            pr.ns            = sp.bc[XNEG] == NEUMANN;
+            pr.ne            = sp.bc[XPOS] == NEUMANN;
+            pr.cyc           = sp.cyc ? 1 : 0;
+            pr.z1            = sp.cyc ? sp.zc.p : nullptr;
+            pr.z2            = sp.cyc ? sp.zc.p + (size_t)sp.nx * sp.ny : nullptr;
+            start[q]         = (int)blocks;
+            blocks += (long long)pr.nblk * jb.nrhs;
+            if (blocks > (1LL << 30)) throw Error(FP_E_INVALID, "too many tridiagonal workgroups in one launch");
+            g.threads = std::max(g.threads, (MB * P + 63) / 64 * 64);
+            g.lds     = std::max(g.lds, 3 * (size_t)P * MB * sizeof(T));
+        }
+        start[js.size()] = (int)blocks;
+        g.blocks         = (int)blocks;
+        g.start_off      = (probs.size() * sizeof(TriProb<T>) + 255) & ~(size_t)255;
+        g.table.resize(g.start_off + start.size() * sizeof(int));
+        std::memcpy(g.table.data(), probs.data(), probs.size() * sizeof(TriProb<T>));
+        std::memcpy(g.table.data() + g.start_off, start.data(), start.size() * sizeof(int));
+        if (g.blocks > 0) out.push_back(std::move(g));
+    }
+    return out;
+}
+
+template <typename T>
+void launch_tridiag_group(const GroupPlan& g, const unsigned char* dtable, hipStream_t s)
+{
+    JobTable<TriProb<T>> t;
+    t.nprob = g.nprob;
+    t.p     = reinterpret_cast<const TriProb<T>*>(dtable);
+    t.start = reinterpret_cast<const int*>(dtable + g.start_off);
+    if (g.ch == 8)
+        hipLaunchKernelGGL((k_tridiag<T, 8, JobTable<TriProb<T>>>), dim3(g.blocks), dim3(g.threads), g.lds, s, t);
+    else if (g.ch == 16)
+        hipLaunchKernelGGL((k_tridiag<T, 16, JobTable<TriProb<T>>>), dim3(g.blocks), dim3(g.threads), g.lds, s, t);
+    else if (g.ch == 32)
+        hipLaunchKernelGGL((k_tridiag<T, 32, JobTable<TriProb<T>>>), dim3(g.blocks), dim3(g.threads), g.lds, s, t);
+    else throw Error(FP_E_INVALID, "unknown chunk length");
+    FP_HIP(hipGetLastError());
+}
+template std::vector<GroupPlan> plan_tridiag_groups<double>(const TridiagJob<double>*, int);
+template std::vector<GroupPlan> plan_tridiag_groups<float>(const TridiagJob<float>*, int);
+template void launch_tridiag_group<double>(const GroupPlan&, const unsigned char*, hipStream_t);
+template void launch_tridiag_group<float>(const GroupPlan&, const unsigned char*, hipStream_t);
+
 template <typename T>
 void launch_tridiag(const SubPlan& sp, T* data, long long ld, int nrhs, hipStream_t s)
 {
@@ -899,12 +982,13 @@ static std::vector<int> chunk_candidates(int nx)
     return v;
 }
 
-// Default schedule without timing: the smallest chunk whose chain fits one
-// wavefront per 64 chunks, and 4 modes per workgroup.
+// Default schedule without timing: the shortest chunk that gives a
+// workgroup of 4 modes (the commonest choice of the timed schedule), else
+// the longest.
 static int default_chunk(int nx)
 {
-    for (int ch : {16, 32})
-        if ((nx + ch - 1) / ch <= 64) return ch;
+    for (int ch : {8, 16, 32})
+        if (4 * ((nx + ch - 1) / ch) <= tri_max_threads(ch)) return ch;
     return 32;
 }
 

@@ -95,6 +95,15 @@ typedef struct fp_geometry_desc {
 /* ---- context ----------------------------------------------------------- */
 int         fp_ctx_create(int device, fp_ctx* out);
 void        fp_ctx_destroy(fp_ctx ctx);
+/* Planning rigor of the plans created from ctx afterwards, the analogue of
+ * the FFTW_MEASURE flag the reference passes to every fftw_plan_dft_r2c_1d
+ * (pe/poisson/poisson_fft2d.cpp:34,102,177,239): FP_PLAN_MEASURE (the
+ * default) times the candidate launch shapes of each new plan's transforms
+ * and sweeps on the device; FP_PLAN_ESTIMATE takes estimated shapes without
+ * timing (plan construction in milliseconds; solves may be slower). */
+#define FP_PLAN_MEASURE 0
+#define FP_PLAN_ESTIMATE 1
+int         fp_ctx_set_planning(fp_ctx ctx, int rigor);
 const char* fp_status_str(int status);
 const char* fp_last_error(void);
 int         fp_version(void);
@@ -135,6 +144,23 @@ int fp_sub_solve_batched(fp_ctx ctx, int nprob, const fp_sub* subs, double* cons
                          void* stream);
 int fp_sub_solve_batched_f32(fp_ctx ctx, int nprob, const fp_sub* subs, float* const* d_f, const int* nrhs,
                              void* stream);
+/* A fixed batch of independent subdomain solves, prepared once (no reference
+ * counterpart; the reference loops over PoissonSolver2D objects,
+ * pe/poisson/poisson_solver2d.cpp:62-90): the problems, their fields d_f[k]
+ * (nrhs[k] fields of subs[k] back to back, FP64, or FP32 when fp32 != 0) and
+ * the launch descriptors are bound at creation and the descriptors copied to
+ * device memory the batch owns.  fp_batch_solve then queues only kernel
+ * launches on the stream (no allocation, no host copy), so it may be
+ * captured into a HIP graph and replayed.  Every problem takes 128 threads
+ * per transform workgroup where that fits, so that problems of different
+ * lengths share launches; within a launch the problems with the longest
+ * workgroups come first.  subs and the fields must outlive the batch; calls
+ * of one batch on different streams must not overlap. */
+typedef struct fp_batch_s* fp_batch;
+int  fp_batch_create(fp_ctx ctx, int nprob, const fp_sub* subs, void* const* d_f, const int* nrhs, int fp32,
+                     fp_batch* out);
+int  fp_batch_solve(fp_batch b, void* stream);
+void fp_batch_destroy(fp_batch b);
 /* fp_sub_solve_batched without any allocation (no reference counterpart: for
  * callers that cannot allocate device memory, e.g. tight serving loops or
  * harnesses that forbid hipMalloc).  Same arguments, kernels and results; the

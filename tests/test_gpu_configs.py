@@ -139,6 +139,30 @@ def test_config2_batched_512(ctx):
     e32 = max(float((a.double() - b).norm() / b.norm()) for a, b in zip(u32, u64))
     banner("config 2 FP32 worst rel-L2 vs FP64 %.3e" % e32)
     assert e32 < 1e-4
+    # the same workload as a prepared batch captured in a graph: every replay
+    # against the per-call FP64 results
+    for dt, ref, bar in ((torch.float64, u64, 1e-13), (torch.float32, u64, 1e-4)):
+        banner("config 2: prepared batch, graph replays, %s" % dt)
+        src = [to_dev(f, dt) for f in rhs]
+        work = [x.clone() for x in src]
+        b = fpa.BatchSolve(ctx, subs, work)
+        g = torch.cuda.CUDAGraph()
+        st = torch.cuda.Stream()
+        with torch.cuda.stream(st):
+            g.capture_begin()
+            b.solve(stream=st)
+            g.capture_end()
+        torch.cuda.synchronize()
+        for rep in range(2):
+            for w, x in zip(work, src):
+                w.copy_(x)
+            torch.cuda.synchronize()
+            g.replay()
+            torch.cuda.synchronize()
+            e = max(float((w.double() - r.double()).norm() / r.double().norm()) for w, r in zip(work, ref))
+            banner("config 2 prepared batch replay %d worst rel-L2 vs per-call FP64 %.3e" % (rep, e))
+            assert e < bar
+        del g, b
 
 
 def test_config2_cross_64_rhs(ctx):
