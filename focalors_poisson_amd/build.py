@@ -12,7 +12,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 OBJ = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libfocalors_poisson.so")
-SOURCES = ["fp_transform.hip", "fp_tridiag.hip", "fp_vec.hip", "fp_refine.hip", "fp_solver.hip", "fp_solver3d.hip", "fp_concat3d.hip", "fp_batch.hip",
+SOURCES = ["fp_transform.hip", "fp_tridiag.hip", "fp_vec.hip", "fp_refine.hip", "fp_solver.hip", "fp_solver3d.hip", "fp_concat3d.hip", "fp_batch.hip", "fp_gmres2d.hip",
            "fp_tune.cpp", "fp_staging.cpp", "case_api.cpp"]
 ARCH = os.environ.get("FP_OFFLOAD_ARCH", "gfx950")
 
@@ -46,7 +46,7 @@ def build(verbose=False):
                 print(w, file=sys.stderr)
     tmp = LIB + ".tmp"
     cmd = ["hipcc", "-shared", "-fPIC", "--offload-arch=" + ARCH, *objs, "-o", tmp,
-           "-L/opt/rocm/lib", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
+           "-Wl,-rpath,/opt/rocm/lib"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stderr[-8000:])

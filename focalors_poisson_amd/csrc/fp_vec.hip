@@ -3,108 +3,197 @@
 // interface unknowns.
 #include "fp_vec.h"
 
-#include <rocblas/rocblas.h>
-
+#include <algorithm>
 #include <mutex>
 
 namespace fp {
 
-// ---- dense products of many right-hand sides: rocBLAS GEMM ----------------
+// ---- dense products of many right-hand sides on the matrix cores ----------
 // With several right-hand sides the dense products (explicit inverse, Schur
-// blocks, basis maps) are matrix-matrix products; a GEMM reads each matrix
-// once instead of once per right-hand side.  One rocBLAS handle per device.
+// blocks, basis maps) are matrix-matrix products
+//   Y[r][i] = alpha sum_k A[i][k] X[r][k] + beta Y[r][i],   r < nrhs, i < m,
+// with A row-major (lda) and each right-hand side's X[r], Y[r] contiguous.
+// They run on v_mfma_f64_16x16x4f64 (FP32 operands are widened to FP64:
+// exact products, FP64 sums).  Operand layouts of the instruction (gfx950):
+// A(i, q) in lane i + 16 q, B(q, j) in lane j + 16 q, D(i, j) in lane
+// j + 16 (i % 4), element i / 4.
+//
+// A wavefront computes 16 rows by up to 16 NTL right-hand sides over one
+// run of k; the nw wavefronts of a workgroup take equal runs of k for the
+// same outputs and their partial tiles are added in a fixed order through
+// one LDS slot (no atomics and no scratch in device memory: results are
+// reproducible and the call allocates nothing).  Two row tiles per
+// wavefront (half the operand loads of X) measured slower: at 2048
+// wavefronts or fewer the loop is bound by its own latency, and the second
+// tile halves the wavefronts.  Each
+// block of 16 k is four instructions per (row tile, right-hand-side tile),
+// and lane (i, q) takes the four consecutive columns 4q .. 4q+3 of the
+// block, one per instruction, in both operands: the sum over the block is
+// the same, and every lane reads 32 contiguous bytes of each of its rows per
+// block (two 16-byte loads where rows are 16-byte aligned).  Every A load
+// serves 4 NTL products; the next block's loads are issued before this
+// block's products.
+typedef double gd4_t __attribute__((ext_vector_type(4)));
 constexpr int kGemmMinRhs = 4;
+constexpr int kGemmRowTiles = 1;  // 16-row tiles per wavefront
+constexpr int kGemmMaxWaves = 8;
 
-static rocblas_handle blas_handle()
+// v[s] = p[kk + s] (s < 4) where kk + s < ke and ok, else 0
+template <typename T, bool VEC>
+__device__ __forceinline__ void gemm_load4(const T* __restrict__ p, int kk, int ke, bool ok, double (&v)[4])
 {
-    static std::mutex     mu;
-    static rocblas_handle h[64] = {};
-    int                   dev   = 0;
-    FP_HIP(hipGetDevice(&dev));
-    FP_REQUIRE(dev >= 0 && dev < 64, "device index out of range");
-    std::lock_guard<std::mutex> lk(mu);
-    if (!h[dev] && rocblas_create_handle(&h[dev]) != rocblas_status_success)
-        throw Error(FP_E_HIP, "rocblas_create_handle failed");
-    return h[dev];
-}
-
-// Y[r] = alpha A X[r] + beta Y[r] for r < nrhs: A is m x k row-major (lda),
-// X[r] has k values (ldx apart), Y[r] m values (ldy apart).  In rocBLAS's
-// column-major terms: Y = alpha op(A^T) X + beta Y.  The outputs are few
-// (m x nrhs, e.g. 4096 x 64) against a long k, so a plain GEMM would run on a
-// few dozen workgroups: k is split into ks equal chunks, one strided-batched
-// GEMM writes ks partial products into stream-ordered scratch
-// (hipMallocAsync, returned to the pool after use) and k_splitk_sum adds them
-// in fixed order (deterministic) with alpha and beta.
-static rocblas_status gemm(rocblas_handle h, int m, int n, int k, const double* A, int lda, const double* X,
-                           int ldx, double* Y, int ldy, double alpha, double beta)
-{
-    return rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, m, n, k, &alpha, A, lda, X, ldx,
-                         &beta, Y, ldy);
-}
-static rocblas_status gemm(rocblas_handle h, int m, int n, int k, const float* A, int lda, const float* X, int ldx,
-                           float* Y, int ldy, float alpha, float beta)
-{
-    return rocblas_sgemm(h, rocblas_operation_transpose, rocblas_operation_none, m, n, k, &alpha, A, lda, X, ldx,
-                         &beta, Y, ldy);
-}
-static rocblas_status gemm_batched(rocblas_handle h, int m, int n, int k, const double* A, int lda, long long sa,
-                                   const double* X, int ldx, long long sx, double* W, int ldw, long long sw, int cnt)
-{
-    const double one = 1.0, zero = 0.0;
-    return rocblas_dgemm_strided_batched(h, rocblas_operation_transpose, rocblas_operation_none, m, n, k, &one, A,
-                                         lda, sa, X, ldx, sx, &zero, W, ldw, sw, cnt);
-}
-static rocblas_status gemm_batched(rocblas_handle h, int m, int n, int k, const float* A, int lda, long long sa,
-                                   const float* X, int ldx, long long sx, float* W, int ldw, long long sw, int cnt)
-{
-    const float one = 1.0f, zero = 0.0f;
-    return rocblas_sgemm_strided_batched(h, rocblas_operation_transpose, rocblas_operation_none, m, n, k, &one, A,
-                                         lda, sa, X, ldx, sx, &zero, W, ldw, sw, cnt);
-}
-
-template <typename T>
-__global__ void k_splitk_sum(const T* W, int ks, int m, int nrhs, T* Y, long long ldy, T alpha, T beta)
-{
-    const long long tot = (long long)m * nrhs;
-    for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < tot;
-         q += (long long)gridDim.x * blockDim.x) {
-        T acc = T(0);
-        for (int c = 0; c < ks; c++) acc += W[c * tot + q];
-        const long long r = q / m, i = q - r * m;
-        T&              y = Y[r * ldy + i];
-        y                 = alpha * acc + (beta != T(0) ? beta * y : T(0));
+    if (VEC && ok && kk + 3 < ke) {
+        if constexpr (sizeof(T) == 8) {
+            const double2 x0 = *reinterpret_cast<const double2*>(p + kk);
+            const double2 x1 = *reinterpret_cast<const double2*>(p + kk + 2);
+            v[0] = x0.x;
+            v[1] = x0.y;
+            v[2] = x1.x;
+            v[3] = x1.y;
+        } else {
+            const float4 x = *reinterpret_cast<const float4*>(p + kk);
+            v[0] = x.x;
+            v[1] = x.y;
+            v[2] = x.z;
+            v[3] = x.w;
+        }
+    } else {
+#pragma unroll
+        for (int s = 0; s < 4; s++) v[s] = (ok && kk + s < ke) ? (double)p[kk + s] : 0.0;
     }
+}
+
+template <typename T, int NTL, bool VEC>
+__global__ __launch_bounds__(64 * kGemmMaxWaves) void k_gemm_mfma(const T* __restrict__ A, int m, int k,
+                                                                 long long lda, const T* __restrict__ X,
+                                                                 long long ldx, T* Y, long long ldy, int nrhs,
+                                                                 double alpha, double beta, int kc)
+{
+    constexpr int MT = kGemmRowTiles;
+    extern __shared__ double gred[];  // one slot of 64 lanes x 4 MT NTL doubles
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int il = lane & 15, q = lane >> 4;
+    const int i0 = blockIdx.x * 16 * MT;
+    const int r0 = blockIdx.y * 16 * NTL;
+    const T*  Ar[MT];
+    bool      aok[MT];
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++) {
+        const int i = i0 + 16 * mt + il;
+        aok[mt]     = i < m;
+        Ar[mt]      = A + (long long)(aok[mt] ? i : 0) * lda;
+    }
+    const T* Xr[NTL];
+    bool     xok[NTL];
+#pragma unroll
+    for (int t = 0; t < NTL; t++) {
+        const int r = r0 + 16 * t + il;
+        xok[t]      = r < nrhs;
+        Xr[t]       = X + (long long)(xok[t] ? r : 0) * ldx;
+    }
+    gd4_t acc[MT][NTL];
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+        for (int t = 0; t < NTL; t++) acc[mt][t] = gd4_t{0.0, 0.0, 0.0, 0.0};
+    const int kb = w * kc, ke = min(k, kb + kc);
+    double    a[MT][4], b[NTL][4];
+    auto      load = [&](int k0) {
+        const int kk = k0 + 4 * q;
+#pragma unroll
+        for (int mt = 0; mt < MT; mt++) gemm_load4<T, VEC>(Ar[mt], kk, ke, aok[mt], a[mt]);
+#pragma unroll
+        for (int t = 0; t < NTL; t++) gemm_load4<T, VEC>(Xr[t], kk, ke, xok[t], b[t]);
+    };
+    if (kb < ke) load(kb);
+    for (int k0 = kb; k0 < ke; k0 += 16) {
+        double ca[MT][4], cb[NTL][4];
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+#pragma unroll
+            for (int mt = 0; mt < MT; mt++) ca[mt][s] = a[mt][s];
+#pragma unroll
+            for (int t = 0; t < NTL; t++) cb[t][s] = b[t][s];
+        }
+        if (k0 + 16 < ke) load(k0 + 16);
+#pragma unroll
+        for (int s = 0; s < 4; s++)
+#pragma unroll
+            for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+                for (int t = 0; t < NTL; t++)
+                    acc[mt][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ca[mt][s], cb[t][s], acc[mt][t], 0, 0, 0);
+    }
+    // fixed order: wavefront 0 adds the partial tiles of nw-1, nw-2, .., 1
+    for (int h = nw - 1; h >= 1; h--) {
+        if (w == h) {
+#pragma unroll
+            for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+                for (int t = 0; t < NTL; t++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++) gred[((mt * NTL + t) * 4 + v) * 64 + lane] = acc[mt][t][v];
+        }
+        __syncthreads();
+        if (w == 0) {
+#pragma unroll
+            for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+                for (int t = 0; t < NTL; t++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++) acc[mt][t][v] += gred[((mt * NTL + t) * 4 + v) * 64 + lane];
+        }
+        __syncthreads();
+    }
+    if (w != 0) return;
+#pragma unroll
+    for (int t = 0; t < NTL; t++) {
+        const int r = r0 + 16 * t + il;
+        if (r >= nrhs) continue;
+#pragma unroll
+        for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int row = i0 + 16 * mt + 4 * v + q;
+                if (row >= m) continue;
+                T&           y   = Y[(long long)r * ldy + row];
+                const double val = alpha * acc[mt][t][v] + (beta != 0.0 ? beta * (double)y : 0.0);
+                y                = (T)val;
+            }
+    }
+}
+
+template <typename T, int NTL>
+static void launch_gemm_mfma(const T* A, int m, int k, long long lda, const T* X, long long ldx, T* Y,
+                             long long ldy, int nrhs, T alpha, T beta, hipStream_t s)
+{
+    const int gx = (m + 16 * kGemmRowTiles - 1) / (16 * kGemmRowTiles), gy = (nrhs + 16 * NTL - 1) / (16 * NTL);
+    // wavefronts per workgroup: enough for about 2048 on the device, each
+    // with at least 4 blocks of k
+    int nw = 1;
+    while (nw < kGemmMaxWaves && (long long)gx * gy * nw < 2048 && k / (16 * 2 * nw) >= 4) nw *= 2;
+    const int    kc  = ((k + nw - 1) / nw + 15) / 16 * 16;
+    const size_t lds = (size_t)64 * 4 * kGemmRowTiles * NTL * sizeof(double);
+    // 16-byte loads when every row of A and X starts on a 16-byte boundary
+    constexpr long long vl = 16 / sizeof(T);
+    const bool vec = ((uintptr_t)A % 16 == 0) && ((uintptr_t)X % 16 == 0) && lda % vl == 0 && ldx % vl == 0;
+    if (vec)
+        hipLaunchKernelGGL((k_gemm_mfma<T, NTL, true>), dim3(gx, gy), dim3(64 * nw), lds, s, A, m, k, lda, X, ldx, Y,
+                           ldy, nrhs, (double)alpha, (double)beta, kc);
+    else
+        hipLaunchKernelGGL((k_gemm_mfma<T, NTL, false>), dim3(gx, gy), dim3(64 * nw), lds, s, A, m, k, lda, X, ldx,
+                           Y, ldy, nrhs, (double)alpha, (double)beta, kc);
+    FP_HIP(hipGetLastError());
 }
 
 template <typename T>
 static void dense_apply(const T* A, int m, int k, int lda, const T* X, long long ldx, T* Y, long long ldy, int nrhs,
                         T alpha, T beta, hipStream_t s)
 {
-    rocblas_handle h = blas_handle();
-    if (rocblas_set_stream(h, s) != rocblas_status_success) throw Error(FP_E_HIP, "rocblas_set_stream failed");
-    // chunks of at least 128 of k, until about 512 tiles of 64 x 64 outputs
-    int ks = 1;
-    const long long tiles = (long long)((m + 63) / 64) * ((nrhs + 63) / 64);
-    while (ks < 16 && tiles * ks * 2 <= 512 && k % (2 * ks) == 0 && k / (2 * ks) >= 128) ks *= 2;
-    rocblas_status st;
-    if (ks == 1) {
-        st = gemm(h, m, nrhs, k, A, lda, X, (int)ldx, Y, (int)ldy, alpha, beta);
-    } else {
-        const int       kc  = k / ks;
-        const long long tot = (long long)m * nrhs;
-        T*              W   = nullptr;
-        FP_HIP(hipMallocAsync((void**)&W, sizeof(T) * (size_t)(tot * ks), s));
-        st = gemm_batched(h, m, nrhs, kc, A, lda, kc, X, (int)ldx, kc, W, m, tot, ks);
-        if (st == rocblas_status_success) {
-            const int nb = (int)std::min<long long>((tot + 255) / 256, 2048);
-            hipLaunchKernelGGL(k_splitk_sum<T>, dim3(nb), dim3(256), 0, s, W, ks, m, nrhs, Y, ldy, alpha, beta);
-        }
-        const hipError_t le = hipGetLastError();
-        FP_HIP(hipFreeAsync(W, s));
-        FP_HIP(le);
-    }
-    if (st != rocblas_status_success) throw Error(FP_E_HIP, std::string("rocblas gemm: ") + rocblas_status_to_string(st));
+    if (m <= 0 || nrhs <= 0) return;
+    if (nrhs <= 16) launch_gemm_mfma<T, 1>(A, m, k, lda, X, ldx, Y, ldy, nrhs, alpha, beta, s);
+    else if (nrhs <= 32) launch_gemm_mfma<T, 2>(A, m, k, lda, X, ldx, Y, ldy, nrhs, alpha, beta, s);
+    else launch_gemm_mfma<T, 4>(A, m, k, lda, X, ldx, Y, ldy, nrhs, alpha, beta, s);
 }
 
 constexpr int VB = 256;  // threads per block of the vector kernels

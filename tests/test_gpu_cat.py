@@ -205,6 +205,28 @@ def test_cross_multi_rhs_split_gemm(ctx):
     assert rel([u[0] for u in us], ref) < BAR, info
 
 
+@pytest.mark.parametrize("nrhs", [5, 19, 37, 70])
+def test_ragged_multi_rhs_matrix_core_products(ctx, nrhs):
+    """Right-hand-side counts that take one, two and four 16-wide tiles of the
+    matrix-core products (and two tile groups at 70), on a T-shape whose
+    interface lengths (45 rows, 37 and 53 columns) are not multiples of 16:
+    every right-hand side against its own one-right-hand-side solve (the
+    matrix-vector kernels) and, for three of them, the oracle."""
+    rng = np.random.default_rng(500 + nrhs)
+    h = 1.0 / 53
+    doms = [(37, 45, h, h, (A, A, D, A)), (21, 45, h, h, (N, A, D, N)), (29, 45, h, h, (A, D, N, D)),
+            (37, 30, h, h, (D, N, A, N))]
+    conn = [(0, 0, 1), (0, 1, 2), (0, 3, 3)]
+    fs = [rng.standard_normal((nrhs, d[0], d[1])) for d in doms]
+    banner("ragged multi rhs %d" % nrhs)
+    us, info, _ = run(ctx, doms, conn, fs, tol=1e-13)
+    for r in sorted({0, nrhs // 2, nrhs - 1}):
+        one, _, _ = run(ctx, doms, conn, [f[r] for f in fs], tol=1e-13)
+        assert rel([u[r] for u in us], one) < 1e-12, r
+        ref = O.reference_solution(doms, conn, [f[r] for f in fs])
+        assert rel([u[r] for u in us], ref) < BAR, (r, info)
+
+
 P = 2
 
 
