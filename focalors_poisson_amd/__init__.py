@@ -351,6 +351,149 @@ class ConcatPoissonSolver2D:
             self._h = None
 
 
+class _Fields:
+    """Device pointers inside registered float64 tensors, as tensor views:
+    the buffers a domain-solver callback is handed all lie in tensors the
+    Python side owns (workspaces, the right-hand side being solved)."""
+
+    def __init__(self, *tensors):
+        self.tensors = list(tensors)
+
+    def view(self, ptr, nrhs, nx, ny):
+        n = nrhs * nx * ny
+        for t in self.tensors:
+            base = t.data_ptr()
+            if base <= ptr and ptr + 8 * n <= base + 8 * t.numel():
+                off = (ptr - base) // 8
+                return t.view(-1)[off:off + n].view(nrhs, nx, ny)
+        raise ValueError("the domain solver was handed a buffer outside the registered tensors")
+
+
+class _DomainSolver:
+    """A DomainSolver2D for the C ABI (fp_domain_solve_fn): a PoissonSolver2D
+    (its fp_sub through fp_sub_domain_solve, no Python in the loop), or a
+    callable solve(fields) acting in place on a (nrhs, nx, ny) float64 view
+    on the current stream."""
+
+    def __init__(self, solver, fields):
+        self.keep = solver
+        if isinstance(solver, PoissonSolver2D):
+            self.fn = ctypes.cast(_capi.lib().fp_sub_domain_solve, _capi.DOMAIN_FN)
+            self.user = solver._h
+        else:
+            def cb(user, ptr, nx, ny, nrhs, stream):
+                try:
+                    solver(fields.view(ptr, nrhs, nx, ny))
+                    return 0
+                except Exception:  # reported to the library as a failed solve
+                    import traceback
+                    traceback.print_exc()
+                    return 1
+            self.fn = _capi.DOMAIN_FN(cb)
+            self.user = None
+
+
+class SchurMat2D:
+    """fp_schur2d: SchurMat2D_{xneg,xpos,yneg,ypos} (pe/concat/schur_mat2d.h:7-69)
+    of a branch domain bnx x bny on face `loc` of the root, built by unit
+    solves of branch_solver (a PoissonSolver2D or a callable, see
+    GMRESSolver2D)."""
+
+    def __init__(self, ctx, loc, bnx, bny, branch_solver, stream=None, chunk=64):
+        import torch
+        cn = bny if loc in (XNEG, XPOS) else bnx
+        self.loc, self.cn = loc, cn
+        self.work = torch.empty(max(1, min(chunk, cn)) * bnx * bny, dtype=torch.float64,
+                                device=torch.device("cuda", ctx.device))
+        self._solver = _DomainSolver(branch_solver, _Fields(self.work))
+        self._h = ctypes.c_void_p()
+        _capi.check(_capi.lib().fp_schur2d_create(ctx._h, int(loc), int(bnx), int(bny), self._solver.fn,
+                                                  self._solver.user, self.work.data_ptr(), self.work.numel(),
+                                                  _stream_ptr(stream), ctypes.byref(self._h)))
+
+    def values(self):
+        import numpy as np
+        out = np.empty((self.cn, self.cn))
+        _capi.check(_capi.lib().fp_schur2d_values(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out
+
+    def apply(self, root, out=None, accumulate=False, stream=None):
+        import torch
+        if out is None:
+            out = torch.zeros_like(root)
+        _capi.check(_capi.lib().fp_schur2d_apply(self._h, root.data_ptr(), root.shape[-2], root.shape[-1],
+                                                 out.data_ptr(), int(accumulate), _stream_ptr(stream)))
+        return out
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and h.value:
+            _capi.lib().fp_schur2d_destroy(h)
+            self._h = None
+
+
+class GMRESSolver2D:
+    """fp_gmres2d: GMRESSolver2D (pe/concat/gmres_solver2d.h:12-57) on an
+    nx x ny root field: solves (I - A^-1 S) x = A^-1 b with A^-1 the solver of
+    set_solver and S the Schur blocks of schur_mat_construct / add_schur.
+    Solvers are PoissonSolver2D objects or callables f(fields) that solve the
+    (nrhs, nx, ny) float64 device view in place on the current stream."""
+
+    def __init__(self, ctx, nx, ny, m, tol, max_iter):
+        import torch
+        self.ctx, self.nx, self.ny = ctx, nx, ny
+        n = _capi.lib().fp_gmres2d_workspace(int(nx), int(ny), int(m))
+        self.work = torch.empty(n, dtype=torch.float64, device=torch.device("cuda", ctx.device))
+        self._fields = _Fields(self.work)
+        self._schur, self._solver, self._x0 = [], None, None
+        self._h = ctypes.c_void_p()
+        _capi.check(_capi.lib().fp_gmres2d_create(ctx._h, int(nx), int(ny), int(m), float(tol), int(max_iter),
+                                                  self.work.data_ptr(), n, ctypes.byref(self._h)))
+
+    def set_solver(self, solver):
+        self._solver = _DomainSolver(solver, self._fields)
+        _capi.check(_capi.lib().fp_gmres2d_set_solver(self._h, self._solver.fn, self._solver.user))
+
+    def add_schur(self, S):
+        _capi.check(_capi.lib().fp_gmres2d_add_schur(self._h, S._h))
+        self._schur.append(S)
+
+    def schur_mat_construct(self, adjacency, stream=None):
+        """adjacency: {loc: (bnx, bny, branch_solver)} (schur_mat_construct,
+        gmres_solver2d.cpp:39-91)."""
+        for loc, (bnx, bny, solver) in adjacency.items():
+            self.add_schur(SchurMat2D(self.ctx, loc, bnx, bny, solver, stream=stream))
+
+    def set_initial_guess(self, x0):
+        self._x0 = x0
+        _capi.check(_capi.lib().fp_gmres2d_set_initial_guess(self._h, None if x0 is None else x0.data_ptr()))
+
+    def solve(self, b, stream=None, allow_not_converged=False, max_res=1024):
+        """Overwrites b (nx x ny float64, on the device) with x; returns the
+        solve info with the restart residuals under "resvec"."""
+        import numpy as np
+        _check_field(b, b.dtype, self.nx * self.ny)
+        self._fields.tensors = [self.work, b]
+        info = _capi.SolveInfo()
+        res = np.zeros(max_res)
+        nres = ctypes.c_int(0)
+        st = _capi.lib().fp_gmres2d_solve(self._h, b.data_ptr(), _stream_ptr(stream), ctypes.byref(info),
+                                          res.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), max_res,
+                                          ctypes.byref(nres))
+        self._fields.tensors = [self.work]
+        _capi.check(st, allow=(_capi.FP_E_NOT_CONVERGED,) if allow_not_converged else ())
+        out = info.as_dict()
+        out["status"] = st
+        out["resvec"] = res[:min(nres.value, max_res)].copy()
+        return out
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and h.value:
+            _capi.lib().fp_gmres2d_destroy(h)
+            self._h = None
+
+
 class ConcatPoissonSolver3D:
     """fp_cat3d: 3-D composite geometry solve (tree sweep + face GMRES),
     pe/concat/concat_solver3d.h.

@@ -41,6 +41,10 @@ class Geometry3DDesc(ctypes.Structure):
                 ("bc", ctypes.POINTER(ctypes.c_ubyte)), ("nconn", ctypes.c_int), ("conn", _ip)]
 
 
+# fp_domain_solve_fn: int (*)(void* user, double* d_f, int nx, int ny, int nrhs, void* stream)
+DOMAIN_FN = ctypes.CFUNCTYPE(ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp)
+
+
 class FPError(RuntimeError):
     def __init__(self, status, msg):
         super().__init__("%s (status %d): %s" % (status_str(status), status, msg))
@@ -124,6 +128,24 @@ def lib():
     L.fp_cat_graph_capture_into.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                             ctypes.c_int, _vp]
     L.fp_cat_graph_launch.argtypes = [_vp, _vp, ctypes.POINTER(SolveInfo)]
+    L.fp_sub_domain_solve.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp]
+    L.fp_schur2d_create.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, DOMAIN_FN, _vp, _vp,
+                                    ctypes.c_longlong, _vp, ctypes.POINTER(_vp)]
+    L.fp_schur2d_size.argtypes = [_vp]
+    L.fp_schur2d_values.argtypes = [_vp, _dp]
+    L.fp_schur2d_apply.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, _vp]
+    L.fp_schur2d_destroy.argtypes = [_vp]
+    L.fp_schur2d_destroy.restype = None
+    L.fp_gmres2d_workspace.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.fp_gmres2d_workspace.restype = ctypes.c_longlong
+    L.fp_gmres2d_create.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                    _vp, ctypes.c_longlong, ctypes.POINTER(_vp)]
+    L.fp_gmres2d_set_solver.argtypes = [_vp, DOMAIN_FN, _vp]
+    L.fp_gmres2d_add_schur.argtypes = [_vp, _vp]
+    L.fp_gmres2d_set_initial_guess.argtypes = [_vp, _vp]
+    L.fp_gmres2d_solve.argtypes = [_vp, _vp, _vp, ctypes.POINTER(SolveInfo), _dp, ctypes.c_int, _ip]
+    L.fp_gmres2d_destroy.argtypes = [_vp]
+    L.fp_gmres2d_destroy.restype = None
     _lib = L
     return L
 

@@ -295,6 +295,54 @@ int fp_cat_graph_launch(fp_cat cat, void* stream, fp_solve_info* info);
 int fp_cat_vjp(fp_cat cat, const double* const* d_gu, double* const* d_gf, double* const* d_gghost, int nrhs,
                void* stream, fp_solve_info* info);
 
+/* ---- the composite's building blocks over caller-supplied solvers ------- */
+/* DomainSolver2D::solve(field2& f) (pe/concat/domain_solver.h:9-15) as a
+ * callback: solve the nrhs fields (nx x ny each, back to back) at d_f in
+ * place, queuing the work on stream; return 0, or nonzero to fail the call
+ * that invoked it (FP_E_INVALID).  fp_sub_domain_solve is one: user = an
+ * fp_sub, its fp_sub_solve on the fields. */
+typedef int (*fp_domain_solve_fn)(void* user, double* d_f, int nx, int ny, int nrhs, void* stream);
+int fp_sub_domain_solve(void* sub, double* d_f, int nx, int ny, int nrhs, void* stream);
+/* SchurMat2D_xneg / _xpos / _yneg / _ypos (pe/concat/schur_mat2d.h:7-69):
+ * loc = the face of the root the branch (bnx x bny) sits on; construct
+ * (schur_mat2d.cpp) runs one unit solve of the branch per interface point
+ * through the callback, at most 64 per call and as many as the caller's
+ * workspace d_work holds (work_len doubles, at least bnx*bny; nothing is
+ * allocated there beyond the block itself).  values copies value(j, i) to
+ * h_value[j*cn+i]; apply is operator* (root line -> the same line of d_out,
+ * which is zeroed first unless accumulate). */
+typedef struct fp_schur2d_s* fp_schur2d;
+int  fp_schur2d_create(fp_ctx ctx, int loc, int bnx, int bny, fp_domain_solve_fn branch, void* user, double* d_work,
+                       long long work_len, void* stream, fp_schur2d* out);
+int  fp_schur2d_size(fp_schur2d S);
+int  fp_schur2d_values(fp_schur2d S, double* h_value);
+int  fp_schur2d_apply(fp_schur2d S, const double* d_root, int rnx, int rny, double* d_out, int accumulate,
+                      void* stream);
+void fp_schur2d_destroy(fp_schur2d S);
+/* GMRESSolver2D (pe/concat/gmres_solver2d.h:12-57): solves
+ * (I - A^-1 S) x = A^-1 b on an nx x ny root field, A^-1 = the solver set by
+ * fp_gmres2d_set_solver (set_solver), S = the sum of the blocks added
+ * (schur_mat_construct), with restart length m, absolute tolerance tol and
+ * at most max_iter restarts, step by step as gmres_solver2d.cpp:123-252
+ * (x0 = A^-1 b unless set_initial_guess).  Its buffers are the caller's
+ * workspace d_work of fp_gmres2d_workspace(nx, ny, m) doubles.  solve
+ * overwrites d_b with x; info reports iterations (Arnoldi steps), restarts
+ * and residual (the last restart residual); h_resvec (capacity cap) gets the
+ * restart residuals (the reference's resVec), *nres their count.  Returns
+ * FP_E_NOT_CONVERGED (x written all the same) when max_iter restarts ended
+ * above tol, where the reference returns silently.  The blocks must outlive
+ * the solver. */
+typedef struct fp_gmres2d_s* fp_gmres2d;
+long long fp_gmres2d_workspace(int nx, int ny, int m);
+int  fp_gmres2d_create(fp_ctx ctx, int nx, int ny, int m, double tol, int max_iter, double* d_work,
+                       long long work_len, fp_gmres2d* out);
+int  fp_gmres2d_set_solver(fp_gmres2d g, fp_domain_solve_fn pe, void* user);
+int  fp_gmres2d_add_schur(fp_gmres2d g, fp_schur2d S);
+int  fp_gmres2d_set_initial_guess(fp_gmres2d g, const double* d_x0);
+int  fp_gmres2d_solve(fp_gmres2d g, double* d_b, void* stream, fp_solve_info* info, double* h_resvec, int cap,
+                      int* nres);
+void fp_gmres2d_destroy(fp_gmres2d g);
+
 #ifdef __cplusplus
 }
 #endif
