@@ -107,10 +107,18 @@ static or_plan* plan_create(int n)
     return p;
 }
 
-/* Plan cache: plans are created under a lock and never freed (like FFTW wisdom). */
-#define OR_MAX_PLANS 256
-static or_plan* g_plans[OR_MAX_PLANS];
-static int      g_nplans = 0;
+/* Plan cache: plans are created under a lock and never freed (like FFTW wisdom).
+ * The table grows without bound, so every length keeps exactly one plan for the
+ * life of the process, however many lengths a test session visits. */
+static or_plan** g_plans  = NULL;
+static int       g_nplans = 0, g_cap = 0;
+
+static void plan_free(or_plan* p)
+{
+    if (!p) return;
+    plan_free(p->sub);
+    free(p->w); free(p->chirp); free(p->bhat); free(p);
+}
 
 static or_plan* plan_get(int n)
 {
@@ -130,13 +138,26 @@ static or_plan* plan_get(int n)
         if (!found) {
             found = fresh;
             fresh = NULL;
-            if (g_nplans < OR_MAX_PLANS) g_plans[g_nplans++] = found;
+            if (g_nplans == g_cap) {
+                int       cap = g_cap ? 2 * g_cap : 256;
+                or_plan** t   = (or_plan**)realloc(g_plans, sizeof(or_plan*) * (size_t)cap);
+                if (!t) abort();
+                g_plans = t;
+                g_cap   = cap;
+            }
+            g_plans[g_nplans++] = found;
         }
     }
-    if (fresh) { /* another thread won the race; keep its plan */
-        free(fresh->w); free(fresh->chirp); free(fresh->bhat); free(fresh);
-    }
+    if (fresh) plan_free(fresh); /* another thread won the race; keep its plan */
     return found;
+}
+
+int or_plan_count(void)
+{
+    int n;
+#pragma omp critical(or_plan_cache)
+    n = g_nplans;
+    return n;
 }
 
 /* One Stockham pass of radix R over n points: combines R sub-transforms of

@@ -29,6 +29,8 @@ enum { OR_SCHUR_UNIT_SOLVES = 0, OR_SCHUR_SPECTRAL = 1 };
 
 /* Complex DFT of length n, interleaved re/im; sign -1 forward, +1 backward (unscaled). */
 int or_dft(int n, int sign, const double* in_ri, double* out_ri);
+/* Number of FFT lengths whose plan is cached (one plan per length, kept for the process). */
+int or_plan_count(void);
 /* Real-to-complex DFT of length n: n/2+1 interleaved bins (FFTW r2c layout). */
 int or_r2c(int n, const double* in, double* out_ri);
 

@@ -35,6 +35,7 @@ def lib():
         L = ctypes.CDLL(LIB_PATH)
         L.or_dft.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp]
         L.or_r2c.argtypes = [ctypes.c_int, _dp, _dp]
+        L.or_plan_count.argtypes = []
         L.or_transform.argtypes = [ctypes.c_int] * 5 + [_dp, _dp]
         L.or_x_diag.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp]
         L.or_chasing.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp]
@@ -257,6 +258,11 @@ def gmres_mock(ainv, S, b, m, tol, maxiter):
 
 def num_threads():
     return lib().or_num_threads()
+
+
+def plan_count():
+    """FFT lengths with a cached plan in the oracle (one plan per length)."""
+    return lib().or_plan_count()
 
 
 # ---- 3-D composite (ConcatPoissonSolver3D), numpy restatement ----

@@ -24,8 +24,21 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
+def _rss_gib():
+    """The process's resident memory (anonymous, file-backed, shared) and
+    its peak, for the case banners."""
+    keys = ("VmRSS:", "VmHWM:", "RssAnon:", "RssFile:", "RssShmem:")
+    try:
+        with open("/proc/self/status") as fh:
+            v = {l.split(":")[0]: int(l.split()[1]) / 2.0 ** 20 for l in fh if l.startswith(keys)}
+        return " [rss %.2f GiB (anon %.2f, file %.2f, shmem %.2f), peak %.2f GiB]" % (
+            v["VmRSS"], v["RssAnon"], v["RssFile"], v["RssShmem"], v["VmHWM"])
+    except (OSError, KeyError, ValueError):
+        return ""
+
+
 def banner(msg):
-    print("[gpu-case] " + msg, flush=True)
+    print("[gpu-case] " + msg + _rss_gib(), flush=True)
 
 
 def sub_reference(f, hx, hy, bc, refine=2):

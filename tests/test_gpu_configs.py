@@ -1,7 +1,12 @@
 """The five benchmark workloads of the solve path at their full sizes, on the
-device, against the converged CPU oracle (the reference's algorithm, GMRES
-then refinement with the long-double composite residual, which reaches the
-discrete solution to about FP64 rounding):
+device, against converged CPU references that reach the discrete solution to
+about FP64 rounding: the oracle's restatement of the reference's algorithm
+refined with the long-double composite residual for the single domains, and
+the direct solve of oracle/direct.py (closed-form bases in both directions,
+exact elimination of the interface, long-double residual refinement) for the
+full-size cross and L, where the reference's own algorithm takes minutes.
+tests/test_oracle.py pins the direct solve against that algorithm and the
+assembled sparse system:
 
   [0] one 256^2 Dirichlet domain, manufactured right-hand side;
   [1] the cross (centre 1024^2, arms 512 x 1024 / 1024 x 512), FP64, FP64 with
@@ -21,6 +26,7 @@ refinement against their own precision."""
 import numpy as np
 import pytest
 
+import direct
 import oracle as O
 from gpu_util import banner, device, rel, sub_reference, to_dev
 
@@ -70,8 +76,8 @@ def cross1024():
     from focalors_poisson_amd.cases import cross
     doms, conn, fields, _ = cross(1024, seed=1)
     fs = [f[0] for f in fields]
-    banner("config 1: oracle solution of the 1024 cross")
-    return doms, conn, fs, O.reference_solution(doms, conn, fs)
+    banner("config 1: direct reference solution of the 1024 cross")
+    return doms, conn, fs, direct.direct_solution(doms, conn, fs)
 
 
 # The plain FP64 composite solve carries the rounding floor of the interface
@@ -89,7 +95,7 @@ def test_config1_cross_1024(ctx, cross1024, precision):
     banner("config 1: cross 1024 precision=%d" % precision)
     us, info = solve_cat(ctx, doms, conn, fs, precision=precision, refine_tol=1e-13 if precision == 1 else None)
     err = rel(us, ref)
-    banner("config 1 precision=%d rel-L2 vs oracle %.3e (%s)" % (precision, err, info))
+    banner("config 1 precision=%d rel-L2 vs the direct reference %.3e (%s)" % (precision, err, info))
     assert info["status"] == 0
     assert err < (PLAIN_F64_1024_BAR if precision == 0 else BAR)
 
@@ -167,7 +173,7 @@ def test_config2_batched_512(ctx):
 
 def test_config2_cross_64_rhs(ctx):
     """The cross with 64 seeded right-hand sides (seeds 100..163) in one call:
-    right-hand side 0 against the oracle, every one against its own
+    right-hand side 0 against the direct reference, every one against its own
     one-right-hand-side solve and by its composite residual."""
     import torch
     import focalors_poisson_amd as fpa
@@ -195,8 +201,8 @@ def test_config2_cross_64_rhs(ctx):
         res = oc.residual(us, fs)
         worst_res = max(worst_res, nrm(res) / (8 * nrm(us) + nrm([f * d[2] ** 2 for f, d in zip(fs, doms)])))
         if r == 0:
-            err = rel(us, O.reference_solution(doms, conn, fs))
-            banner("config 2 cross x64 rhs 0 rel-L2 vs oracle %.3e" % err)
+            err = rel(us, direct.direct_solution(doms, conn, fs))
+            banner("config 2 cross x64 rhs 0 rel-L2 vs the direct reference %.3e" % err)
             assert err < PLAIN_F64_1024_BAR
     banner("config 2 cross x64: worst rel diff vs single solves %.2e, worst backward error %.2e"
            % (worst_same, worst_res))
@@ -246,8 +252,8 @@ def test_config3_dirichlet_4096_refine_steps(ctx):
 def lshape2048():
     from focalors_poisson_amd.cases import lshape
     doms, conn, fields, _ = lshape(2048, seed=4)
-    banner("config 3: oracle solution of the L of three 2048^2 domains")
-    return doms, conn, fields, O.reference_solution(doms, conn, fields)
+    banner("config 3: direct reference solution of the L of three 2048^2 domains")
+    return doms, conn, fields, direct.direct_solution(doms, conn, fields)
 
 
 def test_config3_lshape_refine_steps(ctx, lshape2048):
@@ -258,7 +264,7 @@ def test_config3_lshape_refine_steps(ctx, lshape2048):
         us, info = solve_cat(ctx, doms, conn, fields, precision=1, refine_tol=1e-30, max_steps=steps,
                              allow_not_converged=True)
         errs.append(rel(us, ref))
-        banner("config 3: L 3x2048^2 FP32+refine %d sweeps rel-L2 vs oracle %.3e (estimate %.2e)"
+        banner("config 3: L 3x2048^2 FP32+refine %d sweeps rel-L2 vs the direct reference %.3e (estimate %.2e)"
                % (steps, errs[-1], info["refine_residual"]))
     assert errs[1] < errs[0] and min(errs) < BAR
     us, info = solve_cat(ctx, doms, conn, fields, precision=1, refine_tol=1e-13)
@@ -272,7 +278,7 @@ def test_config3_lshape_f64(ctx, lshape2048, precision):
     doms, conn, fields, ref = lshape2048
     us, info = solve_cat(ctx, doms, conn, fields, precision=precision)
     err = rel(us, ref)
-    banner("config 3: L 3x2048^2 precision=%d rel-L2 vs oracle %.3e" % (precision, err))
+    banner("config 3: L 3x2048^2 precision=%d rel-L2 vs the direct reference %.3e" % (precision, err))
     assert err < BAR
 
 
@@ -333,7 +339,7 @@ def test_config4_adjoint_cross_1024_x32(ctx):
     # assembled operator is symmetric), dL/df = hx^2 lambda, dL/dg = hx^2
     # (-1/h^2) lambda on the Dirichlet face's line
     du = [g[0].cpu().numpy() for g in gu]
-    lam = O.reference_solution(doms, conn, [x / d[2] ** 2 for x, d in zip(du, doms)])
+    lam = direct.direct_solution(doms, conn, [x / d[2] ** 2 for x, d in zip(du, doms)])
     ref_f = [d[2] ** 2 * l for d, l in zip(doms, lam)]
     err_f = rel([g[0].cpu().numpy() for g in gf], ref_f)
     lines = {0: lambda x: x[0, :], 1: lambda x: x[-1, :], 2: lambda x: x[:, 0], 3: lambda x: x[:, -1]}
@@ -343,5 +349,5 @@ def test_config4_adjoint_cross_1024_x32(ctx):
         ref_g.append(doms[d][2] ** 2 * (-1.0 / hh ** 2) * lines[face](lam[d]))
         got_g.append(gg[(d, face)].cpu().numpy()[:glen(d, face)])
     err_g = rel(got_g, ref_g)
-    banner("config 4 rel-L2 vs the oracle gradient: dL/df %.3e, dL/dg %.3e" % (err_f, err_g))
+    banner("config 4 rel-L2 vs the reference gradient: dL/df %.3e, dL/dg %.3e" % (err_f, err_g))
     assert err_f < BAR and err_g < BAR

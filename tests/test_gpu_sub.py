@@ -487,6 +487,7 @@ def test_dirichlet_4096_long_bluestein(ctx, f32):
     f = rng.standard_normal((n, n)) * h * h
     banner("dirichlet 4096 f32=%d" % f32)
     got = gpu_solve(ctx, f, h, h, (D, D, D, D), torch.float32 if f32 else None)
+    banner("dirichlet 4096 f32=%d solved on the device" % f32)
     err = rel(got, sub_reference(f, h, h, (D, D, D, D)))
     banner("dirichlet 4096 f32=%d rel-L2 vs the converged oracle %.3e" % (f32, err))
     assert err < (1e-4 if f32 else 1e-12)

@@ -412,6 +412,92 @@ def test_refined_oracle_cross_128():
     assert rel(u1, ex) < 1e-15
 
 
+def test_plan_cache_keeps_one_plan_per_length():
+    """Every FFT length keeps one cached plan however many lengths a process
+    visits (a bounded table once created a plan per call beyond its 256th
+    length and never freed it)."""
+    rng = np.random.default_rng(5)
+    before = O.plan_count()
+    lengths = [5000 + 7 * k for k in range(300)]
+    for n in lengths:
+        x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        assert np.allclose(O.dft(x), np.fft.fft(x), rtol=0, atol=1e-10 * np.abs(x).sum())
+    grown = O.plan_count() - before
+    assert grown >= len(lengths)
+    for n in lengths[-5:]:
+        O.dft(np.ones(n, dtype=complex))
+    assert O.plan_count() - before == grown
+
+
+# ---- the direct converged reference of star-shaped composites (oracle/direct.py) ----
+
+@pytest.mark.parametrize("n", [1, 2, 3, 8, 33, 256])
+@pytest.mark.parametrize("ends", [(D, D), (N, N), (D, N), (N, D), (A, N), (N, A)])
+def test_direct_bases_are_eigenbases(n, ends):
+    """The closed-form bases of the 1-D three-point operator with each pair
+    of end conditions: orthonormal, and T V^T = V^T diag(lambda) with the
+    Neumann end row's diagonal -1 and a Dirichlet-like end's ghost 0."""
+    import direct
+    V, lam = direct._basis(n, *ends)
+    T = np.diag(np.full(n, -2.0)) + np.diag(np.ones(n - 1), 1) + np.diag(np.ones(n - 1), -1)
+    if ends[0] == N:
+        T[0, 0] += 1.0
+    if ends[1] == N:
+        T[-1, -1] += 1.0
+    assert np.abs(V @ V.T - np.eye(n)).max() < 1e-13
+    assert np.abs(T @ V.T - V.T * np.asarray(lam, dtype=np.float64)).max() < 1e-13
+
+
+@pytest.mark.parametrize("case", [0, 1, 3])
+def test_direct_reference_matches_the_discrete_solution(case):
+    """oracle/direct.py against the refined sparse solve of the assembled
+    system: the cross with ghost data on every kind of face, the T of unequal
+    domains, a lone domain with an unconnected Adjacented face; unrefined to
+    about rounding, refined to below it."""
+    import direct
+    doms, conn, fs, bufs = list(_cases_for_residual())[case]
+    ex = refined_sparse(doms, conn, fs, bufs)
+    assert rel(direct.direct_solution(doms, conn, fs, bufs, refine=0), ex) < 1e-14
+    assert rel(direct.direct_solution(doms, conn, fs, bufs), ex) < 1e-15
+
+
+def test_direct_reference_matches_the_reference_algorithm():
+    """The cross at n = 128 and the L of three 96^2 domains with Dirichlet
+    ghost data: the direct reference equals the refined restatement of the
+    reference's own algorithm (GMRES over full root fields, Schur blocks)."""
+    import direct
+    from focalors_poisson_amd.cases import cross, lshape
+    doms, conn, fields, _ = cross(128, seed=1)
+    fs = [f[0] for f in fields]
+    assert rel(direct.direct_solution(doms, conn, fs), O.reference_solution(doms, conn, fs)) < 1e-15
+    doms, conn, fs, _ = lshape(96, seed=4)
+    rng = np.random.default_rng(96)
+    bufs = {(d, f): rng.standard_normal(doms[d][1] if f < 2 else doms[d][0])
+            for d in range(3) for f in range(4) if doms[d][4][f] == D}
+    assert rel(direct.direct_solution(doms, conn, fs, bufs), O.reference_solution(doms, conn, fs, bufs)) < 1e-15
+
+
+def test_direct_reference_sees_a_planted_coupling_error():
+    """A coupling coefficient of -1 instead of +1 on one interface (planted in
+    a copy of the assembled sparse system) moves the discrete solution by far
+    more than the 1e-12 bar: the agreement above is not a tolerance artefact."""
+    import direct
+    import scipy.sparse.linalg as spla
+    doms, conn, fs, bufs = list(_cases_for_residual())[0]
+    M, rhs, offs = discrete.assemble(doms, conn, fs, bufs)
+    a, dr, b = conn[0]
+    M = M.tolil()
+    ia = offs[a] + np.arange(doms[a][0] * doms[a][1]).reshape(doms[a][0], doms[a][1])
+    ib = offs[b] + np.arange(doms[b][0] * doms[b][1]).reshape(doms[b][0], doms[b][1])
+    la = {0: ia[0, :], 1: ia[-1, :], 2: ia[:, 0], 3: ia[:, -1]}[dr]
+    lb = {0: ib[-1, :], 1: ib[0, :], 2: ib[:, -1], 3: ib[:, 0]}[dr]
+    for p, q in zip(la, lb):
+        M[p, q] = -1.0
+    u = spla.spsolve(M.tocsc(), rhs)
+    wrong = [u[offs[d]:offs[d + 1]].reshape(doms[d][0], doms[d][1]) for d in range(len(doms))]
+    assert rel(wrong, direct.direct_solution(doms, conn, fs, bufs)) > 1e-6
+
+
 # ---- 3-D subdomain solver (PoissonSolver3D) ----
 
 BC3 = [(D, D, D, D, D, D), (N, D, A, N, D, N), (D, A, N, N, N, D), (A, N, D, N, N, N), (P, P, D, N, N, D),
