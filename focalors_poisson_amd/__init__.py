@@ -351,6 +351,87 @@ class ConcatPoissonSolver2D:
             self._h = None
 
 
+class PressureProjection:
+    """fp_proj: the pressure projection of an incompressible flow step on a
+    staggered grid over the composite geometry of `cat` (a
+    ConcatPoissonSolver2D, kept alive here): u = u* - dt G p with
+    D G p = D u* / dt.  Per domain d: u[d] the x-face velocities
+    ((nx+1) x ny), v[d] the y-face velocities (nx x (ny+1)), p[d] the
+    pressure (nx x ny), all contiguous float64 device tensors; pghost: dict
+    (domain, face) -> the pressure's ghost-centre values on Dirichlet faces."""
+
+    def __init__(self, cat):
+        self.cat = cat
+        self._h = ctypes.c_void_p()
+        _capi.check(_capi.lib().fp_proj_create(cat._h, ctypes.byref(self._h)))
+
+    def _arr(self, ts, rows, cols, what):
+        import torch
+        nd = len(self.cat.domains)
+        if len(ts) != nd:
+            raise ValueError("need one %s tensor per domain" % what)
+        for d, t in enumerate(ts):
+            nx, ny = self.cat.domains[d][0], self.cat.domains[d][1]
+            _check_field(t, torch.float64, (nx + rows) * (ny + cols))
+            if t.numel() != (nx + rows) * (ny + cols):
+                raise ValueError("%s of domain %d needs %d values" % (what, d, (nx + rows) * (ny + cols)))
+        return (ctypes.c_void_p * nd)(*[t.data_ptr() for t in ts])
+
+    def _ghost(self, pghost):
+        import torch
+        nd = len(self.cat.domains)
+        gp = (ctypes.c_void_p * (4 * nd))()
+        for (d, face), g in (pghost or {}).items():
+            if not g.is_cuda or g.dtype != torch.float64 or not g.is_contiguous():
+                raise ValueError("ghost data must be contiguous float64 device tensors")
+            flen = self.cat.domains[d][1] if face < 2 else self.cat.domains[d][0]
+            if g.numel() != flen:
+                raise ValueError("ghost data of domain %d face %d needs %d values" % (d, face, flen))
+            gp[4 * d + face] = g.data_ptr()
+        return gp
+
+    def divergence(self, u, v, out, scale=1.0, stream=None):
+        """out[d] = scale D(u, v) (fp_proj_divergence)."""
+        _capi.check(_capi.lib().fp_proj_divergence(self._h, self._arr(u, 1, 0, "u"), self._arr(v, 0, 1, "v"),
+                                                   float(scale), self._arr(out, 0, 0, "out"), _stream_ptr(stream)))
+
+    def correct(self, u, v, p, pghost=None, scale=1.0, stream=None):
+        """u -= scale Gx p, v -= scale Gy p (fp_proj_correct)."""
+        _capi.check(_capi.lib().fp_proj_correct(self._h, self._arr(u, 1, 0, "u"), self._arr(v, 0, 1, "v"),
+                                                self._arr(p, 0, 0, "p"), self._ghost(pghost), float(scale),
+                                                _stream_ptr(stream)))
+
+    def step(self, u, v, p, dt, pghost=None, stream=None):
+        """One projection, in place on u and v, pressure into p
+        (fp_proj_step); returns the solve statistics."""
+        info = _capi.SolveInfo()
+        _capi.check(_capi.lib().fp_proj_step(self._h, self._arr(u, 1, 0, "u"), self._arr(v, 0, 1, "v"),
+                                             self._arr(p, 0, 0, "p"), self._ghost(pghost), float(dt),
+                                             _stream_ptr(stream), ctypes.byref(info)))
+        return info.as_dict()
+
+    def capture(self, u, v, p, dt, pghost=None, stream=None):
+        """Records step() on these tensors into a HIP graph (fp_proj_capture;
+        a non-default stream)."""
+        self._bound = (list(u), list(v), list(p), dict(pghost or {}))
+        _capi.check(_capi.lib().fp_proj_capture(self._h, self._arr(u, 1, 0, "u"), self._arr(v, 0, 1, "v"),
+                                                self._arr(p, 0, 0, "p"), self._ghost(pghost), float(dt),
+                                                _stream_ptr(stream)))
+
+    def launch(self, stream=None, wait=False):
+        """Replays the captured step (fp_proj_launch); with wait, returns the
+        solve statistics of the replay."""
+        info = _capi.SolveInfo()
+        _capi.check(_capi.lib().fp_proj_launch(self._h, _stream_ptr(stream), ctypes.byref(info) if wait else None))
+        return info.as_dict() if wait else None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and h.value:
+            _capi.lib().fp_proj_destroy(h)
+            self._h = None
+
+
 class _Fields:
     """Device pointers inside registered float64 tensors, as tensor views:
     the buffers a domain-solver callback is handed all lie in tensors the

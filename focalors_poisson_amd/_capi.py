@@ -146,6 +146,15 @@ def lib():
     L.fp_gmres2d_solve.argtypes = [_vp, _vp, _vp, ctypes.POINTER(SolveInfo), _dp, ctypes.c_int, _ip]
     L.fp_gmres2d_destroy.argtypes = [_vp]
     L.fp_gmres2d_destroy.restype = None
+    _pp = ctypes.POINTER(_vp)
+    L.fp_proj_create.argtypes = [_vp, _pp]
+    L.fp_proj_destroy.argtypes = [_vp]
+    L.fp_proj_destroy.restype = None
+    L.fp_proj_divergence.argtypes = [_vp, _pp, _pp, ctypes.c_double, _pp, _vp]
+    L.fp_proj_correct.argtypes = [_vp, _pp, _pp, _pp, _pp, ctypes.c_double, _vp]
+    L.fp_proj_step.argtypes = [_vp, _pp, _pp, _pp, _pp, ctypes.c_double, _vp, ctypes.POINTER(SolveInfo)]
+    L.fp_proj_capture.argtypes = [_vp, _pp, _pp, _pp, _pp, ctypes.c_double, _vp]
+    L.fp_proj_launch.argtypes = [_vp, _vp, ctypes.POINTER(SolveInfo)]
     _lib = L
     return L
 

@@ -12,7 +12,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 OBJ = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libfocalors_poisson.so")
-SOURCES = ["fp_transform.hip", "fp_tridiag.hip", "fp_vec.hip", "fp_refine.hip", "fp_solver.hip", "fp_solver3d.hip", "fp_concat3d.hip", "fp_batch.hip", "fp_gmres2d.hip",
+SOURCES = ["fp_transform.hip", "fp_tridiag.hip", "fp_vec.hip", "fp_refine.hip", "fp_solver.hip", "fp_solver3d.hip", "fp_concat3d.hip", "fp_batch.hip", "fp_gmres2d.hip", "fp_project.hip",
            "fp_tune.cpp", "fp_staging.cpp", "case_api.cpp"]
 ARCH = os.environ.get("FP_OFFLOAD_ARCH", "gfx950")
 

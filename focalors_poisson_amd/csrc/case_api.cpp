@@ -6,7 +6,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
+#include <cstring>
 #include <iostream>
 #include <mutex>
 #include <queue>
@@ -192,6 +194,34 @@ double field2::dot(const field2& o) const
 }
 
 double field2::norm() const { return std::sqrt(dot(*this)); }
+
+void field2::write_binary(const std::string& path) const
+{
+    host_read();
+    FILE* fh = std::fopen(path.c_str(), "wb");
+    if (!fh) throw std::runtime_error("field2::write_binary: cannot open " + path);
+    const int32_t dims[2] = {nx, ny};
+    const bool    ok      = std::fwrite("FPF2", 1, 4, fh) == 4 && std::fwrite(dims, sizeof(int32_t), 2, fh) == 2 &&
+                   std::fwrite(value.data(), sizeof(double), value.size(), fh) == value.size();
+    if (std::fclose(fh) != 0 || !ok) throw std::runtime_error("field2::write_binary: write failed for " + path);
+}
+
+void field2::read_binary(const std::string& path)
+{
+    FILE* fh = std::fopen(path.c_str(), "rb");
+    if (!fh) throw std::runtime_error("field2::read_binary: cannot open " + path);
+    char    magic[4];
+    int32_t dims[2];
+    bool    ok = std::fread(magic, 1, 4, fh) == 4 && std::memcmp(magic, "FPF2", 4) == 0 &&
+              std::fread(dims, sizeof(int32_t), 2, fh) == 2 && dims[0] > 0 && dims[1] > 0;
+    if (ok) {
+        if (dims[0] != nx || dims[1] != ny) init(dims[0], dims[1], name);
+        host_write();
+        ok = std::fread(value.data(), sizeof(double), value.size(), fh) == value.size();
+    }
+    std::fclose(fh);
+    if (!ok) throw std::runtime_error("field2::read_binary: not a field file or truncated: " + path);
+}
 
 double field2::sum() const
 {
@@ -573,6 +603,14 @@ Domain2DUniform* ConcatPoissonSolver2D::tree_root() const
 // the caller next reads it.  Ghost data are uploaded when they changed.
 void ConcatPoissonSolver2D::solve()
 {
+    prepare();
+    const int st = fp_cat_solve(cat, d_fields.data(), d_ghost_c.data(), 1, nullptr, &info);
+    // the reference returns silently after maxIter; so does this layer
+    if (st != FP_OK && st != FP_E_NOT_CONVERGED) check_status(st);
+}
+
+void ConcatPoissonSolver2D::prepare()
+{
     const int nd = (int)order.size();
     for (int d = 0; d < nd; d++) {
         auto* s  = order[d];
@@ -594,7 +632,80 @@ void ConcatPoissonSolver2D::solve()
             ghost_sent[4 * d + k] = std::move(host);
         }
     }
-    const int st = fp_cat_solve(cat, d_fields.data(), d_ghost_c.data(), 1, nullptr, &info);
-    // the reference returns silently after maxIter; so does this layer
+}
+
+// ---- pressure projection --------------------------------------------------
+
+PressureProjection2D::PressureProjection2D(ConcatPoissonSolver2D& s) : solver(&s)
+{
+    check_hip(hipStreamCreate(reinterpret_cast<hipStream_t*>(&stream)), "hipStreamCreate");
+}
+
+PressureProjection2D::~PressureProjection2D()
+{
+    if (proj) fp_proj_destroy(proj);
+    if (stream) (void)hipStreamDestroy(reinterpret_cast<hipStream_t>(stream));
+}
+
+void PressureProjection2D::set_velocity(Domain2DUniform* s, field2& u, field2& v)
+{
+    if (u.get_nx() != s->nx + 1 || u.get_ny() != s->ny || v.get_nx() != s->nx || v.get_ny() != s->ny + 1)
+        throw std::runtime_error("PressureProjection2D: u needs (nx+1) x ny values and v nx x (ny+1)");
+    vel[s]   = {&u, &v};
+    captured = false;
+}
+
+// device pointers of every field, the pressure's ghost data uploaded; a
+// rebuilt solver (new parameters) gets a new projection handle
+void PressureProjection2D::bind()
+{
+    if (proj && cat != solver->cat) {
+        fp_proj_destroy(proj);
+        proj     = nullptr;
+        captured = false;
+    }
+    if (!proj) {
+        check_status(fp_proj_create(solver->cat, &proj));
+        cat = solver->cat;
+    }
+    solver->prepare();
+    const int nd = (int)solver->order.size();
+    du.assign(nd, nullptr);
+    dv.assign(nd, nullptr);
+    for (int d = 0; d < nd; d++) {
+        auto it = vel.find(solver->order[d]);
+        if (it == vel.end()) throw std::runtime_error("PressureProjection2D: a domain without velocity fields");
+        du[d] = it->second.first->device_data();
+        dv[d] = it->second.second->device_data();
+    }
+}
+
+void PressureProjection2D::project(double dt)
+{
+    bind();
+    const int st = fp_proj_step(proj, du.data(), dv.data(), solver->d_fields.data(), solver->d_ghost_c.data(), dt,
+                                stream, &info);
+    check_hip(hipStreamSynchronize(reinterpret_cast<hipStream_t>(stream)), "hipStreamSynchronize");
+    if (st != FP_OK && st != FP_E_NOT_CONVERGED) check_status(st);
+}
+
+void PressureProjection2D::capture(double dt)
+{
+    bind();
+    check_status(fp_proj_capture(proj, du.data(), dv.data(), solver->d_fields.data(), solver->d_ghost_c.data(), dt,
+                                 stream));
+    cu       = du;
+    cv       = dv;
+    cp       = solver->d_fields;
+    captured = true;
+}
+
+void PressureProjection2D::replay()
+{
+    if (!captured) throw std::runtime_error("PressureProjection2D: nothing captured");
+    bind();
+    if (!captured || du != cu || dv != cv || solver->d_fields != cp)
+        throw std::runtime_error("PressureProjection2D: the fields moved since capture(): capture again");
+    const int st = fp_proj_launch(proj, stream, &info);
     if (st != FP_OK && st != FP_E_NOT_CONVERGED) check_status(st);
 }

@@ -9,6 +9,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <functional>
 #include <vector>
 
 #include "focalors_poisson.h"
@@ -378,6 +379,24 @@ void launch_sweep_batch(const SweepJob<T>* jobs, int njobs, hipStream_t s);
 int sweep_blocks(const SubPlan& sp);
 template <typename T>
 int lines_workgroups(const SubPlan& sp);
+
+// The geometry of a composite handle (faces named in a connection are
+// ADJACENTED in bc; adj[4 d + face] is the neighbour or -1).
+struct CatView {
+    int           nd;
+    const int*    nx;
+    const int*    ny;
+    const double* hx;
+    const double* hy;
+    const int*    bc;
+    const int*    adj;
+    int           device;
+};
+CatView cat_view(const fp_cat_s* c);
+void    cat_capture(fp_cat_s* cat, const double* const* d_rhs, double* const* d_fields, const double* const* d_ghost,
+                    int nrhs, hipStream_t s, const std::function<void(hipStream_t)>& pre,
+                    const std::function<void(hipStream_t)>& post, bool proj);
+int     cat_replay(fp_cat_s* cat, bool proj, void* stream, fp_solve_info* info);
 
 // Full subdomain solve of nrhs fields (transform, sweep, inverse transform).
 template <typename T>

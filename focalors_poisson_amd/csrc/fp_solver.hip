@@ -328,6 +328,7 @@ struct fp_cat_s {
     {
         if (h_rpart) (void)hipHostFree(h_rpart);
         if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (pexec) (void)hipGraphExecDestroy(pexec);
     }
     int                                    precond_kind = 1;  // 0: none, 1: dense inverse of the interface operator
     // launch shape {threads per workgroup, odd-prime form} shared by the
@@ -344,6 +345,9 @@ struct fp_cat_s {
     // FP64 residual decides convergence
     bool                                   no_wait    = false;
     hipGraphExec_t                         gexec      = nullptr;
+    // the same solve recorded between a pressure projection's divergence and
+    // velocity correction (fp_proj_capture); bound to the same workspace
+    hipGraphExec_t                         pexec      = nullptr;
     int                                    gnrhs      = 0;
     int                                    gsteps     = 0;  // refinement sweeps recorded in the graph
     // statistics of the last solve
@@ -412,10 +416,11 @@ static bool same_basis(const SubPlan& a, const SubPlan& b)
 // solve is captured instead of replaying into freed memory.
 static void drop_graph(fp_cat_s* c)
 {
-    if (!c->gexec) return;
+    if (!c->gexec && !c->pexec) return;
     FP_HIP(hipDeviceSynchronize());
-    (void)hipGraphExecDestroy(c->gexec);
-    c->gexec = nullptr;
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->pexec) (void)hipGraphExecDestroy(c->pexec);
+    c->gexec = c->pexec = nullptr;
     c->gnrhs = 0;
 }
 
@@ -2037,12 +2042,17 @@ int fp_cat_solve_into(fp_cat cat, const double* const* d_rhs, double* const* d_o
     return cat_solve_entry(cat, d_rhs, d_out, d_ghost, nrhs, stream, info);
 }
 
-static int graph_capture(fp_cat cat, const double* const* d_rhs, double* const* d_fields,
-                         const double* const* d_ghost, int nrhs, void* stream)
+// Records the FP64 solve of fp_cat_solve(_into) into a HIP graph, with
+// `pre` and `post` (may be empty) enqueued on the capturing stream before and
+// after it, into the handle's slot `proj` ? pexec : gexec.  Throws on error.
+extern "C++" void fp::cat_capture(fp_cat_s* cat, const double* const* d_rhs, double* const* d_fields,
+                     const double* const* d_ghost, int nrhs, hipStream_t s,
+                     const std::function<void(hipStream_t)>& pre, const std::function<void(hipStream_t)>& post,
+                     bool proj)
 {
-    return guarded([&] {
+    {
         FP_REQUIRE(cat && d_fields && nrhs >= 1, "null argument");
-        FP_REQUIRE(stream, "graph capture needs a stream other than the null stream");
+        FP_REQUIRE(s, "graph capture needs a stream other than the null stream");
         for (int d = 0; d < cat->nd; d++) FP_REQUIRE(d_fields[d] && (!d_rhs || d_rhs[d]), "null field");
         const bool refine = cat->precision == FP_F64_REFINE;
         FP_REQUIRE((cat->precision == FP_F64 || refine) && fp::fast_ok(cat),
@@ -2050,7 +2060,7 @@ static int graph_capture(fp_cat cat, const double* const* d_rhs, double* const* 
         fp::Node& R = *cat->nodes[cat->root];
         FP_REQUIRE(R.ch.empty() || R.precond, "graph capture needs the explicit-inverse interface preconditioner");
         FP_HIP(hipSetDevice(cat->ctx->device));
-        hipStream_t s = (hipStream_t)stream;
+        hipGraphExec_t& slot = proj ? cat->pexec : cat->gexec;
         // every allocation before the capture
         if (!R.ch.empty()) {
             fp::ensure_ws(cat, R, nrhs);
@@ -2058,14 +2068,15 @@ static int graph_capture(fp_cat cat, const double* const* d_rhs, double* const* 
         }
         if (refine) fp::ensure_refine_ws(cat, nrhs);
         FP_HIP(hipDeviceSynchronize());
-        if (cat->gexec) {
-            (void)hipGraphExecDestroy(cat->gexec);
-            cat->gexec = nullptr;
+        if (slot) {
+            (void)hipGraphExecDestroy(slot);
+            slot = nullptr;
         }
         hipGraph_t graph = nullptr;
         FP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         cat->graph_mode = true;
         try {
+            if (pre) pre(s);
             if (refine) {
                 if (d_rhs)
                     for (int d = 0; d < cat->nd; d++)
@@ -2076,6 +2087,7 @@ static int graph_capture(fp_cat cat, const double* const* d_rhs, double* const* 
                 fp::cat_refine(cat, d_fields, nrhs, s, true);
             } else
                 fp::cat_sweep_fast(cat, d_fields, d_ghost, nrhs, s, d_rhs);
+            if (post) post(s);
         } catch (...) {
             cat->graph_mode = false;
             (void)hipStreamEndCapture(s, &graph);
@@ -2084,33 +2096,31 @@ static int graph_capture(fp_cat cat, const double* const* d_rhs, double* const* 
         }
         cat->graph_mode = false;
         FP_HIP(hipStreamEndCapture(s, &graph));
-        hipError_t e = hipGraphInstantiate(&cat->gexec, graph, nullptr, nullptr, 0);
+        hipError_t e = hipGraphInstantiate(&slot, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
         FP_HIP(e);
         cat->gnrhs  = nrhs;
         cat->gsteps = refine ? cat->last_refine_steps : 0;
-    });
+    }
 }
 
-int fp_cat_graph_capture(fp_cat cat, double* const* d_fields, const double* const* d_ghost, int nrhs, void* stream)
+static int graph_capture(fp_cat cat, const double* const* d_rhs, double* const* d_fields,
+                         const double* const* d_ghost, int nrhs, void* stream)
 {
-    return graph_capture(cat, nullptr, d_fields, d_ghost, nrhs, stream);
+    return guarded([&] { fp::cat_capture(cat, d_rhs, d_fields, d_ghost, nrhs, (hipStream_t)stream, {}, {}, false); });
 }
 
-int fp_cat_graph_capture_into(fp_cat cat, const double* const* d_rhs, double* const* d_out,
-                              const double* const* d_ghost, int nrhs, void* stream)
-{
-    if (!d_rhs) return guarded([] { FP_REQUIRE(false, "null argument"); });
-    return graph_capture(cat, d_rhs, d_out, d_ghost, nrhs, stream);
-}
-
-int fp_cat_graph_launch(fp_cat cat, void* stream, fp_solve_info* info)
+// Replays the graph in slot `proj` ? pexec : gexec; with info, waits and
+// reports the recorded residual check.  Returns the status.
+extern "C++" int fp::cat_replay(fp_cat_s* cat, bool proj, void* stream, fp_solve_info* info)
 {
     int st = guarded([&] {
-        FP_REQUIRE(cat && cat->gexec, "no captured solve");
+        FP_REQUIRE(cat, "null argument");
+        hipGraphExec_t exec = proj ? cat->pexec : cat->gexec;
+        FP_REQUIRE(exec, proj ? "no captured projection" : "no captured solve");
         FP_HIP(hipSetDevice(cat->ctx->device));
         hipStream_t s = (hipStream_t)stream;
-        FP_HIP(hipGraphLaunch(cat->gexec, s));
+        FP_HIP(hipGraphLaunch(exec, s));
         cat->last_status = 0;
         if (!info) return;
         FP_HIP(hipStreamSynchronize(s));
@@ -2130,6 +2140,30 @@ int fp_cat_graph_launch(fp_cat cat, void* stream, fp_solve_info* info)
     });
     if (st == FP_OK && cat && cat->last_status) st = cat->last_status;
     return st;
+}
+
+int fp_cat_graph_capture(fp_cat cat, double* const* d_fields, const double* const* d_ghost, int nrhs, void* stream)
+{
+    return graph_capture(cat, nullptr, d_fields, d_ghost, nrhs, stream);
+}
+
+int fp_cat_graph_capture_into(fp_cat cat, const double* const* d_rhs, double* const* d_out,
+                              const double* const* d_ghost, int nrhs, void* stream)
+{
+    if (!d_rhs) return guarded([] { FP_REQUIRE(false, "null argument"); });
+    return graph_capture(cat, d_rhs, d_out, d_ghost, nrhs, stream);
+}
+
+int fp_cat_graph_launch(fp_cat cat, void* stream, fp_solve_info* info)
+{
+    return fp::cat_replay(cat, false, stream, info);
+}
+
+// The geometry of a composite handle, for the pressure projection.
+extern "C++" fp::CatView fp::cat_view(const fp_cat_s* c)
+{
+    return {c->nd, c->nx.data(), c->ny.data(), c->hx.data(), c->hy.data(), c->bc.data(), c->adj.data(),
+            c->ctx->device};
 }
 
 int fp_cat_set_refinement(fp_cat cat, double tol, int max_steps)

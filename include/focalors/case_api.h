@@ -87,6 +87,13 @@ public:
     bool          host_current() const { return host_ok; }
     bool          device_current() const { return dev_ok; }
 
+    // Field I/O (no reference counterpart in this tree; a time-stepping
+    // caller writes and restores its fields): a little-endian binary file,
+    // the 4 bytes "FPF2", nx and ny as 32-bit integers, then the nx*ny values
+    // row-major as IEEE doubles.  read_binary takes the stored size.
+    void write_binary(const std::string& path) const;
+    void read_binary(const std::string& path);
+
 private:
     void host_read() const
     {
@@ -238,8 +245,11 @@ public:
     Variable2D* variable = nullptr;
 
 private:
+    friend class PressureProjection2D;
     void rebuild();
     void release();
+    // device copies of the fields and the ghost data as the solve needs them
+    void prepare();
 
     int                              m = 20, maxIter = 100;
     double                           tol = 1e-8;
@@ -251,4 +261,43 @@ private:
     std::vector<double*>             d_ghost;
     std::vector<std::vector<double>> ghost_sent;  // the ghost data last uploaded, per face
     fp_solve_info                    info{};
+};
+
+// The pressure projection of an incompressible flow step on a staggered grid
+// (fp_proj_*; no reference counterpart in this tree: it is the Navier–Stokes
+// caller of ConcatPoissonSolver2D): D G p = D u* / dt, then u = u* - dt G p,
+// so that the discrete divergence of u vanishes to the solve's accuracy.
+// The pressure is the variable of `solver` (its centre fields receive p; its
+// Dirichlet faces are outlets whose buffer_map holds the pressure there, its
+// Neumann faces walls or inlets whose normal velocity stays as given).  Each
+// domain's face velocities are field2s of (nx+1) x ny (u, x-faces) and
+// nx x (ny+1) (v, y-faces); a face shared by two domains is held by both.
+// Needs hx == hy in every domain.  The fields stay on the device between
+// steps (field2's mirror): capture() records one step into a HIP graph and
+// replay() runs it each time step on the same field objects, refreshed in
+// place; project() is the eager step.
+class PressureProjection2D
+{
+public:
+    explicit PressureProjection2D(ConcatPoissonSolver2D& solver);
+    ~PressureProjection2D();
+    PressureProjection2D(const PressureProjection2D&)            = delete;
+    PressureProjection2D& operator=(const PressureProjection2D&) = delete;
+    void          set_velocity(Domain2DUniform* s, field2& u, field2& v);
+    void          project(double dt);
+    void          capture(double dt);
+    void          replay();
+    fp_solve_info last_info() const { return info; }
+
+private:
+    void bind();
+
+    ConcatPoissonSolver2D*                                          solver = nullptr;
+    fp_cat                                                          cat    = nullptr;  // the handle proj was made for
+    fp_proj                                                         proj   = nullptr;
+    void*                                                           stream = nullptr;
+    std::unordered_map<Domain2DUniform*, std::pair<field2*, field2*>> vel;
+    std::vector<double*>                                            du, dv, cu, cv, cp;  // bound / captured pointers
+    bool                                                            captured = false;
+    fp_solve_info                                                   info{};
 };

@@ -343,6 +343,43 @@ int  fp_gmres2d_solve(fp_gmres2d g, double* d_b, void* stream, fp_solve_info* in
                       int* nres);
 void fp_gmres2d_destroy(fp_gmres2d g);
 
+/* ---- the caller side: pressure projection on a staggered grid ---------- */
+/* No reference counterpart in this tree: the Focalors Navier–Stokes solver
+ * that the Poisson module serves (README_en.md:26) makes each provisional
+ * velocity divergence-free by D G p = D u* / dt, u = u* - dt G p, with p at
+ * cell centres and the velocity on the faces.  On that grid D G is the
+ * composite 5-point operator of `cat` with its face conventions (Neumann:
+ * a wall or inflow face, whose normal velocity stays as given; Dirichlet: an
+ * outlet with ghost-centre pressure values; Adjacented: the neighbour's
+ * cells), so the corrected velocity's divergence vanishes to the solve's
+ * accuracy.  Per subdomain d (nx x ny cells): d_u[d] holds the x-faces
+ * [i][j], i = 0..nx ((nx+1) x ny), d_v[d] the y-faces [i][j], j = 0..ny
+ * (nx x (ny+1)); a face shared by two subdomains is held by both and stays
+ * equal in both.  d_p[d] receives the pressure (nx x ny); d_pghost (NULL or
+ * 4*nd pointers, NULL entries allowed) holds the pressure's ghost-centre
+ * values on Dirichlet faces (ny values on x faces, nx on y faces).  Needs
+ * hx == hy in every subdomain; one right-hand side.
+ * fp_proj_step: divergence, the composite solve (fp_cat_solve_into) and the
+ * correction, eager.  fp_proj_capture records the same three into one HIP
+ * graph (for handles fp_cat_graph_capture covers, on a non-null stream);
+ * fp_proj_launch replays it (the caller refreshes d_u, d_v and d_pghost in
+ * place between replays).  The graph is the handle's: a solve with more
+ * right-hand sides than captured destroys it, as it does the captured solve.
+ * fp_proj_divergence (out = scale D(u, v)) and fp_proj_correct
+ * (u -= scale Gx p, v -= scale Gy p) are the two steps on their own. */
+typedef struct fp_proj_s* fp_proj;
+int  fp_proj_create(fp_cat cat, fp_proj* out);
+void fp_proj_destroy(fp_proj proj);
+int  fp_proj_divergence(fp_proj proj, const double* const* d_u, const double* const* d_v, double scale,
+                        double* const* d_out, void* stream);
+int  fp_proj_correct(fp_proj proj, double* const* d_u, double* const* d_v, const double* const* d_p,
+                     const double* const* d_pghost, double scale, void* stream);
+int  fp_proj_step(fp_proj proj, double* const* d_u, double* const* d_v, double* const* d_p,
+                  const double* const* d_pghost, double dt, void* stream, fp_solve_info* info);
+int  fp_proj_capture(fp_proj proj, double* const* d_u, double* const* d_v, double* const* d_p,
+                     const double* const* d_pghost, double dt, void* stream);
+int  fp_proj_launch(fp_proj proj, void* stream, fp_solve_info* info);
+
 #ifdef __cplusplus
 }
 #endif

@@ -108,3 +108,17 @@ def test_case_cross_timing_small():
     a, b = float(out[k + 1]), float(out[k + 2])
     assert abs(a - b) <= 1e-12 * abs(a), out
     assert float(out[out.index("device_resident_ms") + 1]) > 0
+
+
+def test_case_projection_time_loop(tmp_path):
+    """The Navier–Stokes caller's pattern through the case API: a pressure
+    projection (PressureProjection2D over ConcatPoissonSolver2D) captured once
+    and replayed every time step on a two-domain channel, the fields resident
+    on the device; the projected velocity divergence-free to the solve's
+    accuracy, inlet and wall velocities kept, interface copies equal, the
+    first replay equal to the eager step, and the pressure through a
+    write_binary / read_binary round trip."""
+    banner("case_projection")
+    out = case_programs.run("case_projection", tmp_path / "p.fpf2")
+    banner(out.strip().splitlines()[0])
+    assert "projection ok" in out, out

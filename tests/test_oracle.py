@@ -498,6 +498,81 @@ def test_direct_reference_sees_a_planted_coupling_error():
     assert rel(wrong, direct.direct_solution(doms, conn, fs, bufs)) > 1e-6
 
 
+# ---- pressure projection on a staggered grid (oracle/projection.py) ----
+
+def _shared_faces_equal(doms, conn, u, v):
+    worst = 0.0
+    for a, dr, b in conn:
+        ua, ub = np.asarray(u[a]), np.asarray(u[b])
+        va, vb = np.asarray(v[a]), np.asarray(v[b])
+        if dr == 1:
+            worst = max(worst, np.abs(ua[-1] - ub[0]).max())
+        elif dr == 0:
+            worst = max(worst, np.abs(ua[0] - ub[-1]).max())
+        elif dr == 3:
+            worst = max(worst, np.abs(va[:, -1] - vb[:, 0]).max())
+        else:
+            worst = max(worst, np.abs(va[:, 0] - vb[:, -1]).max())
+    return worst
+
+
+@pytest.mark.parametrize("case", [0, 1, 2, 3])
+def test_projection_operators_compose_to_the_composite_laplacian(case):
+    """D G p (gradient by the correction from zero velocities, then the
+    divergence) times hx^2 equals the assembled composite 5-point operator
+    applied to p minus the ghost terms of the Dirichlet pressure data
+    (tests/discrete.py): walls Neumann, outlets Dirichlet, interfaces with
+    coefficient 1, periodic pairs (square cells: the interface coefficient 1
+    is D G only for hx = hy).  So the projected velocity's divergence is the
+    residual of the composite solve."""
+    import projection as PR
+    rng = np.random.default_rng(100 + case)
+    if case == 2:  # periodic x pairs, coupled across y; square cells (D G is the 5-point operator only then)
+        doms = [(16, 12, 0.25, 0.25, (P, P, D, A)), (16, 10, 0.25, 0.25, (P, P, A, N))]
+        conn, bufs = [(0, 3, 1)], {(0, 2): rng.standard_normal(16)}
+    else:
+        doms, conn, _, bufs = list(_cases_for_residual())[case]
+    bcs = discrete.effective_bc(doms, conn)
+    pg = {k: g for k, g in bufs.items() if bcs[k[0]][k[1]] == D}
+    p = [rng.standard_normal((d[0], d[1])) for d in doms]
+    u0 = [np.zeros((d[0] + 1, d[1])) for d in doms]
+    v0 = [np.zeros((d[0], d[1] + 1)) for d in doms]
+    u, v = PR.correct(doms, conn, u0, v0, p, pg, scale=-1.0)
+    assert _shared_faces_equal(doms, conn, u, v) == 0.0
+    lap = PR.divergence(doms, u, v)
+    M, rhs, offs = discrete.assemble(doms, conn, [np.zeros((d[0], d[1])) for d in doms], pg)
+    ref = M @ np.concatenate([x.ravel() for x in p]) - rhs
+    got = np.concatenate([d[2] ** 2 * l.ravel() for d, l in zip(doms, lap)])
+    assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+def test_projection_reference_is_divergence_free():
+    """project(): the corrected velocity of a random provisional one on the
+    cross with ghost data has divergence at rounding level, and its shared
+    faces stay equal."""
+    import projection as PR
+    g, doms, conn, fs, bufs = cross32()
+    bcs = discrete.effective_bc(doms, conn)
+    rng = np.random.default_rng(5)
+    u = [rng.standard_normal((d[0] + 1, d[1])) for d in doms]
+    v = [rng.standard_normal((d[0], d[1] + 1)) for d in doms]
+    for a, dr, b in conn:  # a shared face holds one velocity
+        if dr == 1:
+            u[b][0] = u[a][-1]
+        elif dr == 0:
+            u[b][-1] = u[a][0]
+        elif dr == 3:
+            v[b][:, 0] = v[a][:, -1]
+        else:
+            v[b][:, -1] = v[a][:, 0]
+    pg = {k: x for k, x in bufs.items() if bcs[k[0]][k[1]] == D}
+    p, uo, vo = PR.project(doms, conn, u, v, 0.01, pg)
+    d0 = max(np.abs(x).max() for x in PR.divergence(doms, u, v))
+    d1 = max(np.abs(x).max() for x in PR.divergence(doms, uo, vo))
+    assert d1 < 1e-12 * d0
+    assert _shared_faces_equal(doms, conn, uo, vo) < 1e-15 * max(np.abs(x).max() for x in uo)
+
+
 # ---- 3-D subdomain solver (PoissonSolver3D) ----
 
 BC3 = [(D, D, D, D, D, D), (N, D, A, N, D, N), (D, A, N, N, N, D), (A, N, D, N, N, N), (P, P, D, N, N, D),
