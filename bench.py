@@ -45,7 +45,7 @@ def parse():
                          "prints one JSON line")
     ap.add_argument("--only", default="",
                     help="with --extras: a comma-separated subset of the workloads (cross_x64, batched, dirichlet_4096, "
-                         "lshape, periodic, single_256, case_api, adjoint, box_256, cross3d)")
+                         "lshape, periodic, single_256, case_api, adjoint, box_256, cross3d, projection)")
     ap.add_argument("--forms", default="",
                     help="a comma-separated subset of the solve forms to time (f64, f64_graph, f64_into, "
                          "f64_graph_into, f64_refine_graph_into, f32_refine); default all")
@@ -472,6 +472,51 @@ def extras(fpa, ctx, torch, np, n, only=""):
                              "gmres_iterations": inf3.get("iterations"), "rel_residual": inf3.get("rel_residual"),
                              "cells": sum(b[0] * b[1] * b[2] for b in boxes)}
         del c3, srcs, wks
+
+    if want('projection'):
+        # the flow solver's pressure projection on the cross (staggered grid,
+        # walls Neumann, outlets Dirichlet): divergence, composite solve and
+        # velocity correction replayed from one graph per step, the provisional
+        # velocity refilled by one device copy per step
+        doms, conn, _ = cases.cross_geometry(n)
+        cat = fpa.ConcatPoissonSolver2D(ctx, doms, conn, m=200, tol=1e-12, tol_is_relative=True, max_iter=100)
+        pj = fpa.PressureProjection(cat)
+        g = torch.Generator(device=dev)
+        g.manual_seed(11)
+        u0 = [torch.randn(d[0] + 1, d[1], dtype=torch.float64, device=dev, generator=g) for d in doms]
+        v0 = [torch.randn(d[0], d[1] + 1, dtype=torch.float64, device=dev, generator=g) for d in doms]
+        for a_, dr, b_ in conn:  # one value per shared face
+            if dr == 1:
+                u0[b_][0].copy_(u0[a_][-1])
+            elif dr == 0:
+                u0[b_][-1].copy_(u0[a_][0])
+            elif dr == 3:
+                v0[b_][:, 0].copy_(v0[a_][:, -1])
+            else:
+                v0[b_][:, -1].copy_(v0[a_][:, 0])
+        u = [x.clone() for x in u0]
+        v = [x.clone() for x in v0]
+        p = [torch.zeros(d[0], d[1], dtype=torch.float64, device=dev) for d in doms]
+        stream = torch.cuda.Stream()
+        with torch.cuda.stream(stream):
+            pj.capture(u, v, p, 0.01, stream=stream)
+
+            def step():
+                for x, y in zip(u + v, u0 + v0):
+                    x.copy_(y)
+                pj.launch(stream=stream)
+
+            tp = timed(step, torch, reps=20)
+        torch.cuda.synchronize()
+        div0 = [torch.empty(d[0], d[1], dtype=torch.float64, device=dev) for d in doms]
+        div1 = [torch.empty_like(x) for x in div0]
+        pj.divergence(u0, v0, div0)
+        pj.divergence(u, v, div1)
+        torch.cuda.synchronize()
+        out["projection_cross"] = {"ms_per_step": round(tp * 1e3, 4),
+                                   "divergence_after_rel": max(float(x.abs().max()) for x in div1)
+                                   / max(float(x.abs().max()) for x in div0)}
+        del pj, cat
     return out
 
 
