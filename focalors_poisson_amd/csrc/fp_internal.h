@@ -393,10 +393,14 @@ struct CatView {
     int           device;
 };
 CatView cat_view(const fp_cat_s* c);
+// The solve recorded into a HIP graph with work before and after it: into
+// the handle's captured-solve slot (proj = 0) or its projection slot for the
+// projection with id proj (> 0); cat_replay replays a slot (the projection
+// slot only for the projection that recorded it).
 void    cat_capture(fp_cat_s* cat, const double* const* d_rhs, double* const* d_fields, const double* const* d_ghost,
                     int nrhs, hipStream_t s, const std::function<void(hipStream_t)>& pre,
-                    const std::function<void(hipStream_t)>& post, bool proj);
-int     cat_replay(fp_cat_s* cat, bool proj, void* stream, fp_solve_info* info);
+                    const std::function<void(hipStream_t)>& post, unsigned long long proj);
+int     cat_replay(fp_cat_s* cat, unsigned long long proj, void* stream, fp_solve_info* info);
 
 // Full subdomain solve of nrhs fields (transform, sweep, inverse transform).
 template <typename T>

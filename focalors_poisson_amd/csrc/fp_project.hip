@@ -21,6 +21,7 @@
 // pressure (walls, inflow) keep their prescribed normal velocity.
 #include "fp_internal.h"
 
+#include <atomic>
 #include <cmath>
 #include <memory>
 
@@ -109,11 +110,16 @@ int blocks_of(long long n) { return (int)((n + 255) / 256); }
 }  // namespace
 
 struct fp_proj_s {
-    fp_cat                         cat = nullptr;
-    fp::CatView                    g{};
+    fp_cat                          cat = nullptr;
+    fp::CatView                     g{};
     std::vector<fp::DevBuf<double>> rhs;  // D u* / dt per subdomain, the solve's right-hand side
-    bool                           captured = false;
+    // the id under which the handle's projection slot holds this
+    // projection's graph: unique per projection, so that no other projection
+    // (one made later at the same address included) replays it
+    unsigned long long id = 0;
 };
+
+static std::atomic<unsigned long long> g_next_proj_id{1};
 
 namespace {
 
@@ -197,6 +203,7 @@ int fp_proj_create(fp_cat cat, fp_proj* out)
         FP_REQUIRE(cat && out, "null argument");
         auto pj = std::make_unique<fp_proj_s>();
         pj->cat = cat;
+        pj->id  = g_next_proj_id.fetch_add(1);
         pj->g   = fp::cat_view(cat);
         const fp::CatView& g = pj->g;
         for (int d = 0; d < g.nd; d++) {
@@ -277,15 +284,14 @@ int fp_proj_capture(fp_proj pj, double* const* d_u, double* const* d_v, double* 
             pj->cat, crhs.data(), p.data(), gh.data(), 1, (hipStream_t)stream,
             [&](hipStream_t s) { divergence(pj, u.data(), v.data(), 1.0 / dt, rhs.data(), s); },
             [&](hipStream_t s) { correct(pj, u.data(), v.data(), cp.data(), d_pghost ? pg.data() : nullptr, dt, s); },
-            true);
-        pj->captured = true;
+            pj->id);
     });
 }
 
 int fp_proj_launch(fp_proj pj, void* stream, fp_solve_info* info)
 {
     if (!pj) return guarded([] { FP_REQUIRE(false, "null argument"); });
-    return fp::cat_replay(pj->cat, true, stream, info);
+    return fp::cat_replay(pj->cat, pj->id, stream, info);
 }
 
 }  // extern "C"

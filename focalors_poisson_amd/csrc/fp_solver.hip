@@ -348,6 +348,7 @@ struct fp_cat_s {
     // the same solve recorded between a pressure projection's divergence and
     // velocity correction (fp_proj_capture); bound to the same workspace
     hipGraphExec_t                         pexec      = nullptr;
+    unsigned long long                     pexec_id   = 0;  // the projection that recorded pexec
     int                                    gnrhs      = 0;
     int                                    gsteps     = 0;  // refinement sweeps recorded in the graph
     // statistics of the last solve
@@ -2044,11 +2045,12 @@ int fp_cat_solve_into(fp_cat cat, const double* const* d_rhs, double* const* d_o
 
 // Records the FP64 solve of fp_cat_solve(_into) into a HIP graph, with
 // `pre` and `post` (may be empty) enqueued on the capturing stream before and
-// after it, into the handle's slot `proj` ? pexec : gexec.  Throws on error.
+// after it, into the handle's slot gexec (proj = 0) or pexec, recorded for
+// the projection whose id is proj.  Throws on error.
 extern "C++" void fp::cat_capture(fp_cat_s* cat, const double* const* d_rhs, double* const* d_fields,
                      const double* const* d_ghost, int nrhs, hipStream_t s,
                      const std::function<void(hipStream_t)>& pre, const std::function<void(hipStream_t)>& post,
-                     bool proj)
+                     unsigned long long proj)
 {
     {
         FP_REQUIRE(cat && d_fields && nrhs >= 1, "null argument");
@@ -2101,22 +2103,24 @@ extern "C++" void fp::cat_capture(fp_cat_s* cat, const double* const* d_rhs, dou
         FP_HIP(e);
         cat->gnrhs  = nrhs;
         cat->gsteps = refine ? cat->last_refine_steps : 0;
+        if (proj) cat->pexec_id = proj;
     }
 }
 
 static int graph_capture(fp_cat cat, const double* const* d_rhs, double* const* d_fields,
                          const double* const* d_ghost, int nrhs, void* stream)
 {
-    return guarded([&] { fp::cat_capture(cat, d_rhs, d_fields, d_ghost, nrhs, (hipStream_t)stream, {}, {}, false); });
+    return guarded([&] { fp::cat_capture(cat, d_rhs, d_fields, d_ghost, nrhs, (hipStream_t)stream, {}, {}, 0); });
 }
 
-// Replays the graph in slot `proj` ? pexec : gexec; with info, waits and
-// reports the recorded residual check.  Returns the status.
-extern "C++" int fp::cat_replay(fp_cat_s* cat, bool proj, void* stream, fp_solve_info* info)
+// Replays the graph in slot `proj` ? pexec : gexec (pexec only when the
+// projection `proj` recorded it); with info, waits and reports the recorded
+// residual check.  Returns the status.
+extern "C++" int fp::cat_replay(fp_cat_s* cat, unsigned long long proj, void* stream, fp_solve_info* info)
 {
     int st = guarded([&] {
         FP_REQUIRE(cat, "null argument");
-        hipGraphExec_t exec = proj ? cat->pexec : cat->gexec;
+        hipGraphExec_t exec = proj ? (cat->pexec_id == proj ? cat->pexec : nullptr) : cat->gexec;
         FP_REQUIRE(exec, proj ? "no captured projection" : "no captured solve");
         FP_HIP(hipSetDevice(cat->ctx->device));
         hipStream_t s = (hipStream_t)stream;
@@ -2156,7 +2160,7 @@ int fp_cat_graph_capture_into(fp_cat cat, const double* const* d_rhs, double* co
 
 int fp_cat_graph_launch(fp_cat cat, void* stream, fp_solve_info* info)
 {
-    return fp::cat_replay(cat, false, stream, info);
+    return fp::cat_replay(cat, 0, stream, info);
 }
 
 // The geometry of a composite handle, for the pressure projection.

@@ -363,8 +363,11 @@ void fp_gmres2d_destroy(fp_gmres2d g);
  * correction, eager.  fp_proj_capture records the same three into one HIP
  * graph (for handles fp_cat_graph_capture covers, on a non-null stream);
  * fp_proj_launch replays it (the caller refreshes d_u, d_v and d_pghost in
- * place between replays).  The graph is the handle's: a solve with more
- * right-hand sides than captured destroys it, as it does the captured solve.
+ * place between replays).  The graph is kept in the fp_cat handle (one per
+ * handle: the last projection that captured), so only the projection that
+ * recorded it replays it, a solve with more right-hand sides than captured
+ * destroys it (as it does the captured solve).  A projection is not used
+ * after its fp_cat is destroyed (fp_proj_destroy itself never touches it).
  * fp_proj_divergence (out = scale D(u, v)) and fp_proj_correct
  * (u -= scale Gx p, v -= scale Gy p) are the two steps on their own. */
 typedef struct fp_proj_s* fp_proj;

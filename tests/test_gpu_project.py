@@ -183,6 +183,31 @@ def test_projection_graph_replays(ctx):
         assert d1 < 4e-15 * 64 * 64 * d0
 
 
+def test_projection_graph_belongs_to_its_projection(ctx):
+    """Two projections on one solver: only the one that captured replays
+    the graph; a projection made after the capturing one is gone does not
+    inherit its graph (the graph refers to the first one's buffers)."""
+    import torch
+    import focalors_poisson_amd as fpa
+    doms, conn = geometry("cross", 32)
+    cat, pa = make(ctx, doms, conn)
+    pb = fpa.PressureProjection(cat)
+    u, v = velocity(doms, conn, 30)
+    du, dv = [to_dev(x) for x in u], [to_dev(x) for x in v]
+    dp = [torch.zeros(d[0], d[1], dtype=torch.float64, device=device()) for d in doms]
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        pa.capture(du, dv, dp, 0.01, stream=s)
+        pa.launch(stream=s, wait=True)
+        with pytest.raises(fpa.FPError):
+            pb.launch(stream=s)
+    torch.cuda.synchronize()
+    del pa
+    pc = fpa.PressureProjection(cat)
+    with pytest.raises(fpa.FPError):
+        pc.launch(stream=s)
+
+
 def test_projection_rejects_bad_input(ctx):
     import focalors_poisson_amd as fpa
     h = 1.0 / 16
