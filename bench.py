@@ -45,7 +45,7 @@ def parse():
                          "prints one JSON line")
     ap.add_argument("--only", default="",
                     help="with --extras: a comma-separated subset of the workloads (cross_x64, batched, dirichlet_4096, "
-                         "lshape, periodic, single_256, case_api, adjoint, box_256, cross3d, projection)")
+                         "lshape, periodic, single_256, case_api, adjoint, box_256, cross3d, cross_krylov, projection)")
     ap.add_argument("--forms", default="",
                     help="a comma-separated subset of the solve forms to time (f64, f64_graph, f64_into, "
                          "f64_graph_into, f64_refine_graph_into, f32_refine); default all")
@@ -472,6 +472,48 @@ def extras(fpa, ctx, torch, np, n, only=""):
                              "gmres_iterations": inf3.get("iterations"), "rel_residual": inf3.get("rel_residual"),
                              "cells": sum(b[0] * b[1] * b[2] for b in boxes)}
         del c3, srcs, wks
+
+    if want('cross_krylov'):
+        # the reference's interface method on the device: the cross solved by
+        # GMRES(m=200) over the Schur system from x0 = C, no explicit-inverse
+        # start (FP_INTERFACE_PRECOND=0 at construction), to relative 1e-12;
+        # against the explicit-inverse solve of the same right-hand side
+        doms, conn, fields, ghost = cases.cross(n, seed=1)
+        src = [torch.from_numpy(f).to(dev) for f in fields]
+        gh = {k: torch.from_numpy(g).to(dev) for k, g in ghost.items()} if ghost else None
+        res = {}
+        for key, pre in (("krylov", "0"), ("explicit_inverse", "1")):
+            old = os.environ.get("FP_INTERFACE_PRECOND")
+            os.environ["FP_INTERFACE_PRECOND"] = pre
+            try:
+                cat = fpa.ConcatPoissonSolver2D(ctx, doms, conn, m=200, tol=1e-12, tol_is_relative=True,
+                                                max_iter=100)
+            finally:
+                if old is None:
+                    os.environ.pop("FP_INTERFACE_PRECOND")
+                else:
+                    os.environ["FP_INTERFACE_PRECOND"] = old
+            wk = [torch.empty_like(x) for x in src]
+            inf = {}
+
+            def solve_k():
+                for w, x in zip(wk, src):
+                    w.copy_(x)
+                inf.update(cat.solve(wk, ghost=gh))
+
+            res[key] = (timed(solve_k, torch, reps=5), dict(inf), [w.clone() for w in wk])
+            del cat
+        tk, ik, uk = res["krylov"]
+        tp, ip, up = res["explicit_inverse"]
+        steps = ik.get("iterations") or 0
+        diff = math.sqrt(sum(float((a - b).norm()) ** 2 for a, b in zip(uk, up))
+                         / sum(float(b.norm()) ** 2 for b in up))
+        out["cross_krylov"] = {"ms_per_solve": round(tk * 1e3, 3), "gmres_iterations": steps,
+                               "restarts": ik.get("restarts"), "rel_residual": ik.get("rel_residual"),
+                               "ms_per_solve_explicit_inverse": round(tp * 1e3, 3),
+                               "ms_per_krylov_step": round((tk - tp) * 1e3 / steps, 4) if steps else None,
+                               "rel_l2_vs_explicit_inverse_solve": diff}
+        del res, src, uk, up
 
     if want('projection'):
         # the flow solver's pressure projection on the cross (staggered grid,

@@ -923,6 +923,8 @@ void launch_set_units(double* buf, long long ldb, const int* cell, int i0, int c
 // blocks per system; a dot product is one block per basis vector.
 
 constexpr int GB = 64;
+constexpr int GU = 4;  // wavefronts of the basis-vector updates (k_gm_update, k_gm_update_x)
+static_assert(GU == 4, "the updates add their four partial sums pairwise");
 
 __device__ __forceinline__ double wave_sum(double v)
 {
@@ -1025,31 +1027,43 @@ __global__ __launch_bounds__(256) void k_gm_dots(GmresDev g, int j)
     if (threadIdx.x == 0) g.part2[(long long)r * (g.m + 1) + i] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
 }
 
-// w -= sum_i h_i V_i; Hacc (= h on pass 0, += h on pass 1); partial norms
-__global__ __launch_bounds__(GB) void k_gm_update(GmresDev g, int j, int pass)
+// w -= sum_i h_i V_i; Hacc (= h on pass 0, += h on pass 1); partial norms.
+// GB entries of w per workgroup, the basis vectors dealt round-robin to its
+// GU wavefronts (a fixed order: wavefront q sums i = q, q + GU, ...; the GU
+// partial sums are added pairwise), so each wavefront's dependent chain of
+// loads is a GU-th of j + 1.
+__global__ __launch_bounds__(GB * GU) void k_gm_update(GmresDev g, int j, int pass)
 {
     __shared__ double hs[1024];
+    __shared__ double red[GU - 1][GB];
     const int r = blockIdx.y;
     if (!g.state[r * 4 + 0]) return;
     const int m = g.m;
-    for (int i = threadIdx.x; i <= j; i += GB) hs[i] = g.part2[(long long)r * (m + 1) + i];
+    for (int i = threadIdx.x; i <= j; i += GB * GU) hs[i] = g.part2[(long long)r * (m + 1) + i];
     __syncthreads();
     if (blockIdx.x == 0) {
         double* H = g.Hacc + (long long)r * (m + 1);
-        for (int i = threadIdx.x; i <= j; i += GB) H[i] = (pass == 0 ? 0.0 : H[i]) + hs[i];
+        for (int i = threadIdx.x; i <= j; i += GB * GU) H[i] = (pass == 0 ? 0.0 : H[i]) + hs[i];
     }
-    const long long K   = g.K;
-    const long long ldv = (long long)g.nrhs * K;
-    const long long k   = blockIdx.x * (long long)GB + threadIdx.x;
-    double          w   = 0.0;
+    const int       lane = threadIdx.x % GB, wv = threadIdx.x / GB;
+    const long long K    = g.K;
+    const long long ldv  = (long long)g.nrhs * K;
+    const long long k    = blockIdx.x * (long long)GB + lane;
+    double          acc  = 0.0;
     if (k < K) {
-        w                 = g.Wv[r * K + k];
-        const double* vk  = g.V + (long long)r * K + k;
-        for (int i = 0; i <= j; i++) w -= hs[i] * vk[i * ldv];
+        const double* vk = g.V + (long long)r * K + k;
+        for (int i = wv; i <= j; i += GU) acc += hs[i] * vk[i * ldv];
+    }
+    if (wv > 0) red[wv - 1][lane] = acc;
+    __syncthreads();
+    if (wv != 0) return;
+    double w = 0.0;
+    if (k < K) {
+        w               = g.Wv[r * K + k] - ((acc + red[0][lane]) + (red[1][lane] + red[2][lane]));
         g.Wv[r * K + k] = w;
     }
-    const double acc = wave_sum(w * w);
-    if (threadIdx.x == 0) g.part[(long long)r * g.nblk + blockIdx.x] = acc;
+    const double a = wave_sum(w * w);
+    if (lane == 0) g.part[(long long)r * g.nblk + blockIdx.x] = a;
 }
 
 // h_{j+1,j} = ||w||; breakdown check; V_{j+1}; Givens update of column j and
@@ -1157,22 +1171,29 @@ __global__ __launch_bounds__(256) void k_gm_backsub(GmresDev g)
     for (int i = threadIdx.x; i < m; i += 256) yo[i] = y[i];
 }
 
-// x += V(:, 0:jr) y (gmres_solver2d.cpp:238-241)
-__global__ __launch_bounds__(GB) void k_gm_update_x(GmresDev g)
+// x += V(:, 0:jr) y (gmres_solver2d.cpp:238-241), the basis vectors dealt
+// to the workgroup's wavefronts as in k_gm_update
+__global__ __launch_bounds__(GB * GU) void k_gm_update_x(GmresDev g)
 {
     __shared__ double ys[1024];
+    __shared__ double red[GU - 1][GB];
     const int r  = blockIdx.y;
     const int jr = g.state[r * 4 + 1];
-    for (int i = threadIdx.x; i < jr; i += GB) ys[i] = g.Y[(long long)r * g.m + i];
+    for (int i = threadIdx.x; i < jr; i += GB * GU) ys[i] = g.Y[(long long)r * g.m + i];
     __syncthreads();
-    const long long K   = g.K;
-    const long long ldv = (long long)g.nrhs * K;
-    const long long k   = blockIdx.x * (long long)GB + threadIdx.x;
-    if (k >= K) return;
-    double        x  = g.X[r * K + k];
-    const double* vk = g.V + (long long)r * K + k;
-    for (int i = 0; i < jr; i++) x += ys[i] * vk[i * ldv];
-    g.X[r * K + k] = x;
+    const int       lane = threadIdx.x % GB, wv = threadIdx.x / GB;
+    const long long K    = g.K;
+    const long long ldv  = (long long)g.nrhs * K;
+    const long long k    = blockIdx.x * (long long)GB + lane;
+    double          acc  = 0.0;
+    if (k < K) {
+        const double* vk = g.V + (long long)r * K + k;
+        for (int i = wv; i < jr; i += GU) acc += ys[i] * vk[i * ldv];
+    }
+    if (wv > 0) red[wv - 1][lane] = acc;
+    __syncthreads();
+    if (wv != 0 || k >= K) return;
+    g.X[r * K + k] += (acc + red[0][lane]) + (red[1][lane] + red[2][lane]);
 }
 
 // state, residuals and initial norms to host-mapped memory (read by the host
@@ -1331,7 +1352,7 @@ void gm_arnoldi(const GmresDev& g, int j, hipStream_t s)
     for (int pass = 0; pass < 2; pass++) {
         hipLaunchKernelGGL(k_gm_dots, dim3(j + 1, g.nrhs), dim3(256), 0, s, g, j);
         FP_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_gm_update, dim3(g.nblk, g.nrhs), dim3(GB), 0, s, g, j, pass);
+        hipLaunchKernelGGL(k_gm_update, dim3(g.nblk, g.nrhs), dim3(GB * GU), 0, s, g, j, pass);
         FP_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(k_gm_finish, dim3(g.nblk, g.nrhs), dim3(GB), 0, s, g, j);
@@ -1342,7 +1363,7 @@ void gm_finish_cycle(const GmresDev& g, hipStream_t s)
 {
     hipLaunchKernelGGL(k_gm_backsub, dim3(g.nrhs), dim3(256), 0, s, g);
     FP_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_gm_update_x, dim3(g.nblk, g.nrhs), dim3(GB), 0, s, g);
+    hipLaunchKernelGGL(k_gm_update_x, dim3(g.nblk, g.nrhs), dim3(GB * GU), 0, s, g);
     FP_HIP(hipGetLastError());
 }
 

@@ -87,6 +87,31 @@ def test_cross_vs_oracle(ctx, n, precision):
     assert err < BAR, info
 
 
+@pytest.mark.parametrize("n,nrhs", [(64, 1), (256, 1), (128, 3)])
+def test_cross_interface_gmres_without_explicit_inverse(ctx, monkeypatch, n, nrhs):
+    """The reference's interface method (gmres_solver2d.cpp:123-252): GMRES
+    over the Schur system from x0 = C with no explicit-inverse start
+    (FP_INTERFACE_PRECOND=0), Krylov steps by the line-sourced operator, to
+    relative 1e-13: the converged solution of every right-hand side."""
+    from focalors_poisson_amd.cases import cross
+    monkeypatch.setenv("FP_INTERFACE_PRECOND", "0")
+    doms, conn, fields, _ = cross(n, seed=21, nrhs=nrhs)
+    banner("cross interface GMRES n=%d nrhs=%d" % (n, nrhs))
+    us, info, cat = run(ctx, doms, conn, fields if nrhs > 1 else [f[0] for f in fields], m=200)
+    assert info["iterations"] > 5, info
+    for r in range(nrhs):
+        fs = [f[r] for f in fields]
+        got = [u[r] for u in us] if nrhs > 1 else us
+        err = rel(got, O.reference_solution(doms, conn, fs))
+        print("cross %d rhs %d: %d Krylov steps, rel-L2 %.2e" % (n, r, info["iterations"], err))
+        assert err < BAR, (r, info)
+    import torch
+    import focalors_poisson_amd as fpa
+    s = torch.cuda.Stream()
+    with pytest.raises(fpa.FPError, match="explicit-inverse"):  # a captured solve needs the explicit-inverse start
+        cat.graph_capture([to_dev(f[0]) for f in fields], stream=s)
+
+
 def test_cross_multi_rhs(ctx):
     from focalors_poisson_amd.cases import cross
     n, nrhs = 32, 4
