@@ -199,13 +199,16 @@ struct TransformPlan {
     int        F    = 0;  // FFT length: L, or the Bluestein length (prime factors of L above 47)
     int        nrad = 0;
     int        rad[kMaxRadices] = {};
-    int        engine = 1;  // 1: single-buffer FFT, 0: two-buffer fallback (prime factors above 47)
+    int        engine = 1;  // 1: single-buffer FFT in LDS, 0: two LDS buffers, 2: two buffers in device memory
+                            // (lengths beyond the LDS of one workgroup)
     int        ncs  = 0;  // cos/sin entries of the odd-prime passes (VALU form)
     DevBuf<double> frag;  // FP64 odd-prime passes: matrix-core operand fragments (matrix-core form)
     DevBuf<T>  W;         // twiddles: 64 of W^m, then L/64+1 of W^(64m), W = exp(-2 pi i / L), then
                           // (cos, sin)(2 pi m / R) of every odd-prime pass R
     DevBuf<T>  aux;       // 2N: NN phase exp(-i pi m / 2N) or ND (sin, cos) of 2N pi (j+1)/(2N+1)
     DevBuf<T>  blu;       // Bluestein: chirp (L), then the DFT of its conjugate / F (F)
+    DevBuf<T>  gscr;      // engine 2: 256 workgroup slots of two F-value complex buffers; the plan's
+                          // launches must be ordered (one stream at a time, as its handle)
     T          c0 = 0, c1 = 0;
     int        nt = 256;  // threads per workgroup (tuned)
     int        odd = 0;   // odd-prime pass form: 0 none, 1 matrix cores (FP64), 2 / 3 VALU pair sums with 5 / 3 pairs per item (tuned)

@@ -19,7 +19,7 @@
 //
 // One workgroup owns one row pair of one right-hand side: it loads both rows
 // (coalesced), runs the FFT in LDS and writes the coefficients back in place.
-// Two FFT engines share those load/store phases:
+// Three FFT engines share those load/store phases:
 //
 //   k_ytransform_sb  (every length whose prime factors are at most 47): a
 //     mixed-radix Stockham FFT on ONE LDS buffer of L values.  Each pass reads
@@ -30,9 +30,13 @@
 //     butterfly, the other powers by multiplication.  Odd primes 7..47 run a
 //     symmetric-pair DFT split over groups of threads (a group is a whole
 //     wavefront, so no divergence), its cosines broadcast from an LDS table.
-//   k_ytransform_gen (fallback for larger prime factors or lengths whose
-//     passes do not fit the per-thread register budget): two LDS buffers,
-//     any prime through an R-term in-LDS sum per output.
+//   k_ytransform_gen (fallback for lengths whose passes do not fit the
+//     per-thread register budget): two LDS buffers, any prime through an
+//     R-term in-LDS sum per output.
+//   k_ytransform_glob (lengths whose buffers exceed one workgroup's LDS,
+//     e.g. the FP64 quarter-wave kinds at N = 4096, L = 8193 = 3 x 2731):
+//     the two-buffer engine's passes on buffers in device memory, Bluestein
+//     for prime factors above 47, a bounded grid looping over row pairs.
 #include "fp_internal.h"
 
 #include <algorithm>
@@ -81,6 +85,7 @@ struct YArgs {
     int          rad[kMaxRadices];  // radices of F
     const C2<T>* chirp;  // Bluestein: exp(-i pi n^2 / L), n < L
     const C2<T>* bh;     // Bluestein: DFT_F of the conjugate chirp, over F
+    C2<T>*       gs;     // global-memory engine: two buffers of F values per workgroup slot
     const C2<T>* W;    // 64 + L/64 + 1 twiddles, then cos/sin tables of the odd-prime passes
     int          ncs;  // entries of those tables (staged in LDS; FP32 odd-prime passes)
     const double* frag;  // FP64 odd-prime passes: matrix-core operand fragments of the DFT blocks
@@ -597,17 +602,19 @@ __device__ void pass_generic(C2<T>* __restrict__ src, C2<T>* __restrict__ dst, c
 template <bool FWD, typename T>
 __device__ C2<T>* run_fft_gen(const YArgs<T>& a, const Tw<T> W, C2<T>* A, C2<T>* B)
 {
+    // FFT of length F (= L for the two-buffer engine; the Bluestein length
+    // in the global-memory engine)
     C2<T>* src = A;
     C2<T>* dst = B;
     int    Ns  = 1;
     for (int s = 0; s < a.nrad; s++) {
         const int R = a.rad[s];
-        if (R == 2) pass_reg<2, FWD>(src, dst, W, a.L, Ns);
-        else if (R == 3) pass_reg<3, FWD>(src, dst, W, a.L, Ns);
-        else if (R == 4) pass_reg<4, FWD>(src, dst, W, a.L, Ns);
-        else if (R == 5) pass_reg<5, FWD>(src, dst, W, a.L, Ns);
-        else if (R == 8) pass_reg<8, FWD>(src, dst, W, a.L, Ns);
-        else pass_generic<FWD>(src, dst, W, a.L, Ns, R);
+        if (R == 2) pass_reg<2, FWD>(src, dst, W, a.F, Ns);
+        else if (R == 3) pass_reg<3, FWD>(src, dst, W, a.F, Ns);
+        else if (R == 4) pass_reg<4, FWD>(src, dst, W, a.F, Ns);
+        else if (R == 5) pass_reg<5, FWD>(src, dst, W, a.F, Ns);
+        else if (R == 8) pass_reg<8, FWD>(src, dst, W, a.F, Ns);
+        else pass_generic<FWD>(src, dst, W, a.F, Ns, R);
         __syncthreads();
         C2<T>* t = src;
         src      = dst;
@@ -947,17 +954,22 @@ __device__ __forceinline__ void store_rows(const YArgs<T>& a, const RowPair<T>& 
 }
 
 template <typename T>
-__device__ __forceinline__ const YArgs<T>& find_problem(const YBatch<T>& bt, int& local)
+__device__ __forceinline__ const YArgs<T>& find_problem(const YBatch<T>& bt, int blk, int& local)
 {
     int q = 0;
-    while (q + 1 < bt.nprob && (int)blockIdx.x >= bt.start[q + 1]) q++;
-    local = (int)blockIdx.x - bt.start[q];
+    while (q + 1 < bt.nprob && blk >= bt.start[q + 1]) q++;
+    local = blk - bt.start[q];
     return bt.p[q];
 }
 template <typename T>
-__device__ __forceinline__ const YArgs<T>& find_problem(const JobTable<YArgs<T>>& bt, int& local)
+__device__ __forceinline__ const YArgs<T>& find_problem(const JobTable<YArgs<T>>& bt, int blk, int& local)
 {
-    return table_find(bt, (int)blockIdx.x, local);
+    return table_find(bt, blk, local);
+}
+template <typename T, typename BT>
+__device__ __forceinline__ const YArgs<T>& find_problem(const BT& bt, int& local)
+{
+    return find_problem<T>(bt, (int)blockIdx.x, local);
 }
 
 // threads per workgroup: chosen per plan (128, 256 or 512)
@@ -967,7 +979,7 @@ template <typename T, int OM, typename BT, int EPT>
 __global__ __launch_bounds__(kMaxNT) void k_ytransform_sb(BT bt)
 {
     int             local;
-    const YArgs<T>& a  = find_problem(bt, local);
+    const YArgs<T>& a  = find_problem<T>(bt, local);
     const RowPair<T> rp = row_pair(a, local);
     const int       NT = blockDim.x;
 
@@ -1006,7 +1018,7 @@ template <typename T, typename BT>
 __global__ __launch_bounds__(kMaxNT) void k_ytransform_gen(BT bt)
 {
     int             local;
-    const YArgs<T>& a  = find_problem(bt, local);
+    const YArgs<T>& a  = find_problem<T>(bt, local);
     const RowPair<T> rp = row_pair(a, local);
     const int       NT = blockDim.x;
 
@@ -1024,6 +1036,74 @@ __global__ __launch_bounds__(kMaxNT) void k_ytransform_gen(BT bt)
     __syncthreads();
     C2<T>* Z = const_cast<C2<T>*>(run_fft_gen<true>(a, W, A, B));
     store_rows<0>(a, rp, Z, wt);
+}
+
+// ---------------------------------------------------------------------------
+// global-memory engine: transforms whose buffers exceed one workgroup's LDS
+
+// The engine keeps the two FFT buffers of a row pair in device memory (a
+// plan-owned scratch of kGlobSlots slots of 2F values) and runs the
+// two-buffer engine's passes on them; only the twiddle tables and the scan
+// totals are in LDS.  A workgroup's global writes are visible to its other
+// threads after a barrier (one CU, one L1).  Lengths with a prime factor
+// above 47 take Bluestein over the 2,3,5-smooth F >= 2L - 1.  The grid is at
+// most kGlobSlots workgroups; workgroup g owns slot g of every job's scratch
+// and loops over the row pairs g, g + grid, ...
+constexpr int kGlobSlots = 256;
+constexpr int kGlobNT    = 512;
+
+// Bluestein in the two device buffers: A holds z (length L) on entry; the
+// returned buffer holds its DFT of sign FWD in [0, L).
+template <bool FWD, typename T>
+__device__ C2<T>* run_bluestein_glob(const YArgs<T>& a, const Tw<T> W, C2<T>* A, C2<T>* B)
+{
+    const int L = a.L, F = a.F, NT = blockDim.x;
+    for (int n = threadIdx.x; n < F; n += NT) {
+        C2<T> v{T(0), T(0)};
+        if (n < L) {
+            v = A[n];
+            if (!FWD) v.y = -v.y;
+            v = cmul(v, a.chirp[n]);
+        }
+        A[n] = v;
+    }
+    __syncthreads();
+    C2<T>* P = run_fft_gen<true>(a, W, A, B);
+    for (int k = threadIdx.x; k < F; k += NT) P[k] = cmul(P[k], a.bh[k]);
+    __syncthreads();
+    C2<T>* Q = run_fft_gen<false>(a, W, P, P == A ? B : A);
+    for (int k = threadIdx.x; k < L; k += NT) {
+        C2<T> v = cmul(Q[k], a.chirp[k]);
+        if (!FWD) v.y = -v.y;
+        Q[k] = v;
+    }
+    __syncthreads();
+    return Q;
+}
+
+template <typename T, typename BT>
+__global__ __launch_bounds__(kGlobNT) void k_ytransform_glob(BT bt, int total)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C2<T>* tlo = reinterpret_cast<C2<T>*>(smem_raw);
+    C2<T>* thi = tlo + 64;
+    const int NT = blockDim.x;
+    for (int blk = blockIdx.x; blk < total; blk += gridDim.x) {
+        int              local;
+        const YArgs<T>&  a   = find_problem<T>(bt, blk, local);
+        const RowPair<T> rp  = row_pair(a, local);
+        const int        nhi = a.F / 64 + 1;
+        C2<T>*           wt  = thi + nhi;  // DD: wavefront totals of the prefix scan
+        C2<T>*           A   = a.gs + (size_t)blockIdx.x * 2 * (size_t)a.F;
+        C2<T>*           B   = A + a.F;
+        __syncthreads();  // the previous row pair's readers of the tables and buffers are done
+        for (int q = threadIdx.x; q < 64 + nhi; q += NT) tlo[q] = a.W[q];
+        const Tw<T> W{tlo, thi};
+        load_rows<0>(a, rp, A);
+        __syncthreads();
+        C2<T>* Z = a.F != a.L ? run_bluestein_glob<true>(a, W, A, B) : run_fft_gen<true>(a, W, A, B);
+        store_rows<0>(a, rp, Z, wt);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1138,6 +1218,7 @@ static size_t lds_bytes(const TransformPlan<T>& tp, int odd)
     // wavefront totals of the DD prefix scan (which runs inside Z), one per wavefront
     const size_t sc = tp.kind == T_DD ? (size_t)kMaxNT / 64 : 0;
     if (tp.engine == 1) return (zl + 64 + nhi + ncs + sc) * sizeof(C2<T>) + nfrag * sizeof(double);
+    if (tp.engine == 2) return (64 + nhi + (size_t)kGlobNT / 64) * sizeof(C2<T>);
     return (2 * (size_t)tp.L + 64 + nhi + sc) * sizeof(C2<T>);
 }
 
@@ -1152,10 +1233,12 @@ void TransformPlan<T>::build(int k, int n)
     else L = 2 * N + 1;
     if (kind == T_PP && N % 2 != 0) throw Error(FP_E_INVALID, "the periodic transform needs an even length");
     // prime factors above 47: Bluestein over the smallest 2,3,5-smooth F >=
-    // 2L - 1 when its single buffer fits the LDS, else the fallback engine
+    // 2L - 1, in LDS when its single buffer fits there, else in the
+    // global-memory engine
     F        = L;
     ept      = kEPT;
-    bool big = false;
+    engine   = 1;
+    bool big = false, glob = false;
     try {
         factor_radices(L, rad, nrad);
         for (int s = 0; s < nrad; s++) big |= rad[s] > kMaxOddPrime;
@@ -1171,14 +1254,11 @@ void TransformPlan<T>::build(int k, int n)
         for (int e : {kEPT, kEPTLong})
             for (int t : {128, 256, 512})
                 if (!fits && sb_fits(rb, nb, Fb, t, ODD_NONE, e)) fits = e;
-        if (fits && need <= 160 * 1024) {
-            F    = Fb;
-            ept  = fits;
-            nrad = nb;
-            for (int s = 0; s < nb; s++) rad[s] = rb[s];
-        } else {
-            factor_radices(L, rad, nrad);
-        }
+        F    = Fb;
+        nrad = nb;
+        for (int s = 0; s < nb; s++) rad[s] = rb[s];
+        if (fits && need <= 160 * 1024) ept = fits;
+        else glob = true;
     }
     const long double PI  = 3.141592653589793238462643383279502884L;
     const int         nhi = F / 64 + 1;
@@ -1297,7 +1377,8 @@ void TransformPlan<T>::build(int k, int n)
     odd         = ODD_NONE;
     bool chosen = false;
     engine      = 1;  // lds_bytes below sizes the single buffer
-    for (int form : odd_forms<T>(has_odd)) {
+    for (int form : odd_forms<T>(glob ? false : has_odd)) {
+        if (glob) break;
         for (int t : {256, 512, 128})
             if (sb_fits(rad, nrad, F, t, form, ept) && lds_bytes(*this, form) <= 160 * 1024) {
                 chosen = true;
@@ -1307,7 +1388,15 @@ void TransformPlan<T>::build(int k, int n)
             }
         if (chosen) break;
     }
-    if (!chosen) engine = 0;
+    if (!chosen) {
+        engine = 0;
+        if (glob || lds_bytes(*this, ODD_NONE) > 160 * 1024) {
+            // the two buffers in device memory, one pair per workgroup slot
+            engine = 2;
+            nt     = kGlobNT;
+            gscr.alloc((size_t)kGlobSlots * 2 * 2 * (size_t)F);
+        }
+    }
 }
 
 template struct TransformPlan<double>;
@@ -1340,6 +1429,7 @@ static YArgs<T> yargs(const TransformJob<T>& j, int inverse)
     a.F       = tp.F;
     a.chirp   = tp.F != tp.L ? reinterpret_cast<const C2<T>*>(tp.blu.p) : nullptr;
     a.bh      = tp.F != tp.L ? reinterpret_cast<const C2<T>*>(tp.blu.p) + tp.L : nullptr;
+    a.gs      = tp.engine == 2 ? reinterpret_cast<C2<T>*>(tp.gscr.p) : nullptr;
     a.kind    = tp.kind;
     a.inverse = inverse;
     a.nrad    = tp.nrad;
@@ -1366,6 +1456,13 @@ template <typename T, typename B>
 static void launch_kernel(int engine, int odd, int ept, int blocks, int NT, size_t lds, hipStream_t s, const B& bt)
 {
     if (engine == 1 && ept != kEPT && odd != ODD_NONE) throw Error(FP_E_INVALID, "long transforms are 2,3,5-smooth");
+    if (engine == 2) {
+        // at most kGlobSlots workgroups, each looping over row pairs in its own scratch slot
+        hipLaunchKernelGGL((k_ytransform_glob<T, B>), dim3(std::min(blocks, kGlobSlots)), dim3(kGlobNT), lds, s, bt,
+                           blocks);
+        FP_HIP(hipGetLastError());
+        return;
+    }
     const void* fn = engine == 0         ? (const void*)k_ytransform_gen<T, B>
                      : ept != kEPT       ? (const void*)k_ytransform_sb<T, ODD_NONE, B, kEPTLong>
                      : odd == ODD_NONE   ? (const void*)k_ytransform_sb<T, ODD_NONE, B, kEPT>
@@ -1555,7 +1652,7 @@ void launch_ytransform(const TransformPlan<T>& tp, T* data, long long ld, int nx
 template <typename T>
 void tune_transform(TransformPlan<T>& tp, int nx, bool measure)
 {
-    if (tp.engine == 0) return;
+    if (tp.engine != 1) return;
     bool has_odd = false;
     for (int s = 0; s < tp.nrad; s++) has_odd |= tp.rad[s] == 7 || tp.rad[s] > 8;
     std::vector<std::array<int, 2>> cand;  // (threads, odd-prime form)

@@ -648,3 +648,18 @@ def test_composite3d_oracle_matches_sparse(case):
     cat = O.Composite3D(boxes, conn, m=40, tol=1e-13, tol_is_relative=True, maxiter=200)
     got = cat.solve(fs, ghost)
     assert rel(got, ref) < 1e-11
+
+
+@pytest.mark.parametrize("bcp,ny", [((D, D), 64), ((N, N), 37), ((D, N), 100), ((N, D), 33), ((D, N), 2048),
+                                    ((P, P), 4), ((P, P), 10), ((P, P), 64)])
+def test_blocked_closed_form_matches_oracle(bcp, ny):
+    """The blocked closed-form transforms (tests/closed_form.py, the checker
+    of the long transforms) against the oracle both ways, where the oracle's
+    own phases are exact to rounding."""
+    import closed_form
+    rng = np.random.default_rng(ny + 3)
+    x = rng.standard_normal((3, ny))
+    for inv in (False, True):
+        ref = O.transform(bcp[0], bcp[1], x, inverse=inv)
+        got = closed_form.transform(bcp[0], bcp[1], x, inverse=inv)
+        assert np.abs(got - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max()), (bcp, ny, inv)
