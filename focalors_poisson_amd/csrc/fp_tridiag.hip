@@ -116,6 +116,77 @@ __device__ void chain_chunks(const T* SA, const T* SB, T* SY, int P, int MB, int
     }
 }
 
+// Column partials of one chunk: over the workgroup's MB modes, the sums of
+// phi0(j) U(i,j) and phi1(j) U(i,j) for the chunk's CH rows, stored to
+// out[i] and out[nx + i].  The MB modes of a chunk are MB neighbouring lanes
+// (a quad for MB = 4), reduced by DPP quad permutations (xor 2, then xor 1:
+// the order of a shuffle tree, so every lane of the quad holds the same
+// bits); rows are taken MB at a time so that lane jl stores row k0 + jl and
+// one store instruction covers MB consecutive rows of every chunk.  Other MB
+// reduce through the LDS crossbar and lane 0 stores.
+template <typename T, int CH>
+__device__ __forceinline__ void store_col_partials(const T (&fv)[CH], T p0, T p1, int MB, int jl, int c, int P,
+                                                   int i0, int nx, T* out)
+{
+    if (MB == 4 || MB == 2) {
+#pragma unroll
+        for (int k0 = 0; k0 < CH; k0 += 4) {
+            T a0 = T(0), a1 = T(0), b0 = T(0), b1 = T(0);  // rows k0 + jl and, for MB = 2, k0 + 2 + jl
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int k = k0 + t;
+                if (k >= CH) break;
+                T s0 = p0 * fv[k], s1 = p1 * fv[k];
+                if (MB == 4) {
+                    s0 += dpp<0x4E, 0xf>(s0, s0);  // quad_perm [2,3,0,1]: lane ^ 2
+                    s1 += dpp<0x4E, 0xf>(s1, s1);
+                }
+                s0 += dpp<0xB1, 0xf>(s0, s0);  // quad_perm [1,0,3,2]: lane ^ 1
+                s1 += dpp<0xB1, 0xf>(s1, s1);
+                if (MB == 4) {
+                    if (t == jl) {
+                        a0 = s0;
+                        a1 = s1;
+                    }
+                } else if ((t & 1) == jl) {
+                    if (t < 2) {
+                        a0 = s0;
+                        a1 = s1;
+                    } else {
+                        b0 = s0;
+                        b1 = s1;
+                    }
+                }
+            }
+            if (c < P) {
+                const int ka = k0 + jl, kb = k0 + 2 + jl;
+                if (ka < CH && i0 + ka < nx) {
+                    out[i0 + ka]      = a0;
+                    out[nx + i0 + ka] = a1;
+                }
+                if (MB == 2 && kb < CH && i0 + kb < nx) {
+                    out[i0 + kb]      = b0;
+                    out[nx + i0 + kb] = b1;
+                }
+            }
+        }
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < CH; k++) {
+        T s0 = p0 * fv[k], s1 = p1 * fv[k];
+        for (int off = MB / 2; off > 0; off >>= 1) {
+            s0 += __shfl_xor(s0, off, 64);
+            s1 += __shfl_xor(s1, off, 64);
+        }
+        const int i = i0 + k;
+        if (jl == 0 && c < P && i < nx) {
+            out[i]      = s0;
+            out[nx + i] = s1;
+        }
+    }
+}
+
 // The dispatcher deals workgroups to the 8 XCDs round robin.  Renumbering the
 // grid so that consecutive logical blocks (neighbouring modes, which share
 // cache lines of every row) run on one XCD keeps those lines in one L2.
@@ -459,19 +530,7 @@ __global__ __launch_bounds__(tri_threads<CH>()) void k_tridiag_lines(LinesArgs<T
     if (a.ycols) {
         // reduce phi0 u, phi1 u over the MB modes of this chunk (lanes c*MB .. c*MB+MB-1)
         T* out = a.colpart + ((long long)r * gridDim.x + blockIdx.x) * 2 * nx;
-#pragma unroll
-        for (int k = 0; k < CH; k++) {
-            T s0 = p0 * fv[k], s1 = p1 * fv[k];
-            for (int off = MB / 2; off > 0; off >>= 1) {
-                s0 += __shfl_xor(s0, off, 64);
-                s1 += __shfl_xor(s1, off, 64);
-            }
-            const int i = i0 + k;
-            if (jl == 0 && c < P && i < nx) {
-                out[i]      = s0;
-                out[nx + i] = s1;
-            }
-        }
+        store_col_partials<T, CH>(fv, p0, p1, MB, jl, c, P, i0, nx, out);
     }
 }
 
@@ -643,19 +702,7 @@ __global__ __launch_bounds__(tri_threads<CH>()) void k_sweep(SweepBatch<T> bt)
     }
     if (pr.cp) {
         T* out = pr.cp + ((long long)r * pr.nblk + mblk) * 2 * nx;
-#pragma unroll
-        for (int k = 0; k < CH; k++) {
-            T s0 = p0 * fv[k], s1 = p1 * fv[k];
-            for (int off = MB / 2; off > 0; off >>= 1) {
-                s0 += __shfl_xor(s0, off, 64);
-                s1 += __shfl_xor(s1, off, 64);
-            }
-            const int i = i0 + k;
-            if (jl == 0 && c < P && i < nx) {
-                out[i]      = s0;
-                out[nx + i] = s1;
-            }
-        }
+        store_col_partials<T, CH>(fv, p0, p1, MB, jl, c, P, i0, nx, out);
     }
 }
 
