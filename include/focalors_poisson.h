@@ -110,7 +110,11 @@ int         fp_version(void);
 
 /* ---- one subdomain: PoissonSolver2D ------------------------------------ */
 /* PoissonSolver2D(nx, ny, hx, hy, bc_xneg, bc_xpos, bc_yneg, bc_ypos),
- * pe/poisson/poisson_solver2d.cpp:3-21. */
+ * pe/poisson/poisson_solver2d.cpp:3-21.  Any ny, as the reference's FFTW
+ * plans: a y transform whose FFT buffer exceeds one workgroup's LDS runs
+ * from device memory the handle allocates here (256 slots of two
+ * FFT-length complex buffers per precision, e.g. 138 MB in FP64 for a
+ * Dirichlet/Neumann pair at ny = 4096). */
 int fp_sub_create(fp_ctx ctx, int nx, int ny, double hx, double hy, const unsigned char bc[4], fp_sub* out);
 void fp_sub_destroy(fp_sub sub);
 /* PoissonSolver2D::solve(field2& f), pe/poisson/poisson_solver2d.cpp:62-90:
