@@ -132,7 +132,7 @@ static void ensure_ws3(Cat3* c, Node3& nd, int nrhs)
     nd.g.alloc((size_t)nrhs * (m + 1));
     nd.Y.alloc((size_t)nrhs * m);
     nd.part.alloc((size_t)nrhs * nd.nblk);
-    nd.part2.alloc((size_t)nrhs * (m + 1));
+    nd.part2.alloc((size_t)nrhs * (m + 1) * kDotSeg);
     nd.tolabs.alloc(nrhs);
     nd.resid.alloc(nrhs);
     nd.rnorm.alloc(nrhs);
@@ -169,8 +169,10 @@ static void node3_solve(Cat3* c, int d, double* f, int nrhs, hipStream_t s);
 // contribution, as the reference's accumulation of the S products).
 static void schur_scatter(Cat3* c, Node3& nd, const double* v, int nrhs, hipStream_t s)
 {
-    const long long n = n3(c, nd.dom);
-    for (auto& L : nd.ch) launch_schur_gemv(L.S.p, L.cn, v, nd.K, L.off, nd.Tv.p, nd.K, L.off, nrhs, s);
+    const long long         n = n3(c, nd.dom);
+    std::vector<SchurChild> ch;
+    for (auto& L : nd.ch) ch.push_back({L.S.p, L.cn, L.off, L.off, 0});
+    launch_schur_gemv_all(ch.data(), (int)ch.size(), v, nd.K, nd.Tv.p, nd.K, nrhs, s);  // every face in one launch
     launch_fill<double>(nd.G.p, n * nrhs, 0.0, s);
     for (auto& L : nd.ch) index_add(nd.G.p, n, L.pcell.p, nd.Tv.p + L.off, nd.K, nullptr, L.cn, 1.0, nrhs, s);
 }

@@ -444,7 +444,7 @@ static void ensure_ws(fp_cat_s* c, Node& nd, int nrhs)
     nd.g.alloc((size_t)nrhs * (m + 1));
     nd.Y.alloc((size_t)nrhs * m);
     nd.part.alloc((size_t)nrhs * nd.nblk);
-    nd.part2.alloc((size_t)nrhs * (m + 1));
+    nd.part2.alloc((size_t)nrhs * (m + 1) * kDotSeg);
     nd.tolabs.alloc(nrhs);
     nd.resid.alloc(nrhs);
     nd.rnorm.alloc(nrhs);
@@ -543,6 +543,14 @@ static bool full_field_matvec()
 // periodic x axis (the line-sourced operator runs Dirichlet-end sweeps only).
 static bool node_full_matvec(const Node& nd) { return full_field_matvec() || nd.plan.cyc; }
 
+// Tv = S v on every child's segment of the interface vector, one launch
+static void schur_products(Node& nd, const double* v, int nrhs, hipStream_t s)
+{
+    std::vector<SchurChild> ch;
+    for (auto& L : nd.ch) ch.push_back({L.S.p, L.cn, L.off, L.off, 0});
+    launch_schur_gemv_all(ch.data(), (int)ch.size(), v, nd.K, nd.Tv.p, nd.K, nrhs, s);
+}
+
 static void matvec(fp_cat_s* c, Node& nd, const double* v, double* out, int nrhs, hipStream_t s)
 {
     if (!node_full_matvec(nd)) {
@@ -550,7 +558,7 @@ static void matvec(fp_cat_s* c, Node& nd, const double* v, double* out, int nrhs
         return;
     }
     const long long n = dom_n(c, nd.dom);
-    for (auto& L : nd.ch) launch_schur_gemv(L.S.p, L.cn, v, nd.K, L.off, nd.Tv.p, nd.K, L.off, nrhs, s);
+    schur_products(nd, v, nrhs, s);
     launch_fill<double>(nd.G.p, n * nrhs, 0.0, s);
     launch_scatter(nd.G.p, n, nd.ucell.p, nd.uk1.p, nd.uk2.p, nd.nu, nd.Tv.p, nd.K, nrhs, s);
     sub_solve<double>(nd.plan, nd.G.p, n, nrhs, s);
@@ -735,7 +743,7 @@ static void gmres_solve(fp_cat_s* c, Node& nd, double* b, int nrhs, hipStream_t 
     gmres_iterate(c, nd, nrhs, s);
     if (nd.spectral_x) launch_ytransform<double>(nd.plan.t64, nd.X.p, K, nd.nxl, nrhs, 1, s);
     // u = z0 + A^-1 E S y
-    for (auto& L : nd.ch) launch_schur_gemv(L.S.p, L.cn, nd.X.p, K, L.off, nd.Tv.p, K, L.off, nrhs, s);
+    schur_products(nd, nd.X.p, nrhs, s);
     launch_fill<double>(nd.G.p, n * nrhs, 0.0, s);
     launch_scatter(nd.G.p, n, nd.ucell.p, nd.uk1.p, nd.uk2.p, nd.nu, nd.Tv.p, K, nrhs, s);
     dom_solve(c, nd.dom, nd.G.p, nrhs, s);

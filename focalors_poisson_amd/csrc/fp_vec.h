@@ -27,14 +27,13 @@ void launch_gather(const double* G, long long ldG, const int* cell, int K, const
                    double a, double b, int nrhs, hipStream_t s);
 void launch_scatter(double* G, long long ldG, const int* ucell, const int* k1, const int* k2, int nu, const double* w,
                     long long ldw, int nrhs, hipStream_t s);
-void launch_schur_gemv(const double* S, int cn, const double* x, long long ldx, int xoff, double* y, long long ldy,
-                       int yoff, int nrhs, hipStream_t s);
 struct SchurChild {
     const double* S;
     int           cn, voff, lboff, row0;
 };
+constexpr int kMaxSchurChildren = 8;  // children per launch of the Schur products
 struct SchurLinesArgs {
-    SchurChild    ch[4];
+    SchurChild    ch[kMaxSchurChildren];
     int           nch, total_rows;
     const double* v;
     long long     ldv;
@@ -42,6 +41,10 @@ struct SchurLinesArgs {
     long long     ldl;
 };
 void launch_schur_lines(const SchurLinesArgs& a, int nrhs, hipStream_t s);
+// y[off_c + i] = sum_j S_c(i, j) x[off_c + j] for every child c (S, cn and
+// voff = lboff = off_c of each entry), kMaxSchurChildren children per launch
+void launch_schur_gemv_all(const SchurChild* ch, int nch, const double* x, long long ldx, double* y, long long ldy,
+                           int nrhs, hipStream_t s);
 // partial column sums of a line-sourced sweep (nblk blocks per rhs) -> y-lines
 // (half = 0 / 1: only the column-0 / column-(ny-1) sums, nx values)
 void launch_colsum(const double* cp, int nblk, int nx, double* yl, long long ldy, int nrhs, hipStream_t s,
@@ -124,6 +127,10 @@ void launch_schur_collect(const double* buf, long long ldb, const int* cell, int
 void launch_set_units(double* buf, long long ldb, const int* cell, int i0, int cnt, hipStream_t s);
 
 // Device state of a batched interface GMRES (all pointers device memory).
+// Segments per Arnoldi dot product (one workgroup each, summed in order by
+// the update): at least 2048 entries per segment, at most kDotSeg.
+constexpr int kDotSeg = 16;
+inline int gm_nseg(int K) { return K <= 2048 ? 1 : (K + 2047) / 2048 < kDotSeg ? (K + 2047) / 2048 : kDotSeg; }
 struct GmresDev {
     int     K, nrhs, m, nblk;  // nblk = ceil(K / 64)
     double* V;      // (m+1) x nrhs x K
@@ -138,7 +145,7 @@ struct GmresDev {
     double* Y;      // nrhs x m
     double* rnorm;  // nrhs: norm of the right-hand side C at GMRES start
     double* part;   // nrhs x nblk partial sums
-    double* part2;  // nrhs x (m+1): dot products of the current pass
+    double* part2;  // nrhs x (m+1) x kDotSeg: segment sums of the dot products of the current pass
     double* tolabs; // nrhs
     double* resid;  // nrhs
     int*    state;  // nrhs x 4: active, valid columns, status (1 converged, 2 numeric), steps

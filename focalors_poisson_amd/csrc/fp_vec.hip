@@ -357,32 +357,11 @@ void launch_scatter(double* G, long long ldG, const int* ucell, const int* k1, c
 }
 
 // ---- dense Schur blocks --------------------------------------------------------
-// SchurMat2D_*::operator* (schur_mat2d.cpp:18-33 etc.): y_i = sum_j S(i,j) x_j,
-// one wavefront per output row; x and y are segments of interface vectors.
-__global__ __launch_bounds__(256) void k_schur_gemv(const double* __restrict__ S, int cn, const double* x,
-                                                    long long ldx, int xoff, double* y, long long ldy, int yoff)
-{
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int i    = blockIdx.x * 4 + wave;
-    if (i >= cn) return;
-    const long long r   = blockIdx.y;
-    const double*   Si  = S + (long long)i * cn;
-    const double*   xr  = x + r * ldx + xoff;
-    double          acc = 0.0;
-    for (int j = lane; j < cn; j += 64) acc += Si[j] * xr[j];
-    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-    if (lane == 0) y[r * ldy + yoff + i] = acc;
-}
-void launch_schur_gemv(const double* S, int cn, const double* x, long long ldx, int xoff, double* y, long long ldy,
-                       int yoff, int nrhs, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_schur_gemv, dim3((cn + 3) / 4, nrhs), dim3(256), 0, s, S, cn, x, ldx, xoff, y, ldy, yoff);
-    FP_HIP(hipGetLastError());
-}
-
+// SchurMat2D_*::operator* (schur_mat2d.cpp:18-33 etc.): y_i = sum_j S(i,j) x_j.
 // All children's Schur products of one root in one launch, written straight
-// into the root's line buffer (x-lines as rows 0/1, y-lines as a/b):
-// one wavefront per output row, 4 rows per block.
+// into the root's line buffer (x-lines as rows 0/1, y-lines as a/b) or, by
+// launch_schur_gemv_all, into an interface vector: one wavefront per output
+// row, 4 rows per block.
 __global__ __launch_bounds__(256) void k_schur_lines(SchurLinesArgs a)
 {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -420,6 +399,27 @@ void launch_schur_lines(const SchurLinesArgs& a, int nrhs, hipStream_t s)
     }
     hipLaunchKernelGGL(k_schur_lines, dim3((a.total_rows + 3) / 4, nrhs), dim3(256), 0, s, a);
     FP_HIP(hipGetLastError());
+}
+
+void launch_schur_gemv_all(const SchurChild* ch, int nch, const double* x, long long ldx, double* y, long long ldy,
+                           int nrhs, hipStream_t s)
+{
+    for (int c0 = 0; c0 < nch; c0 += kMaxSchurChildren) {
+        SchurLinesArgs a{};
+        int            rows = 0;
+        for (int c = c0; c < nch && c < c0 + kMaxSchurChildren; c++) {
+            a.ch[a.nch]      = ch[c];
+            a.ch[a.nch].row0 = rows;
+            rows += ch[c].cn;
+            a.nch++;
+        }
+        a.total_rows = rows;
+        a.v          = x;
+        a.ldv        = ldx;
+        a.lb         = y;
+        a.ldl        = ldy;
+        launch_schur_lines(a, nrhs, s);
+    }
 }
 
 // Reductions over the column partials of the line-sourced sweeps: kRedU
@@ -1010,21 +1010,31 @@ __global__ __launch_bounds__(GB) void k_gm_cycle_start(GmresDev g)
     }
 }
 
-// Hd[r][i] = V_i . w, one block per basis vector
-__global__ __launch_bounds__(256) void k_gm_dots(GmresDev g, int j)
+// Hd[r][i] = V_i . w: one block per (basis vector, segment of nseg), two
+// independent sums per thread; the segment sums go to part2 and are added
+// in segment order by the update
+__global__ __launch_bounds__(256) void k_gm_dots(GmresDev g, int j, int nseg)
 {
     __shared__ double sh[4];
-    const int r = blockIdx.y, i = blockIdx.x;
+    const int r = blockIdx.y, i = blockIdx.x, sg = blockIdx.z;
     if (!g.state[r * 4 + 0]) return;
     const long long K   = g.K;
-    const double*   vi  = g.V + (long long)i * g.nrhs * K + (long long)r * K;
-    const double*   w   = g.Wv + (long long)r * K;
-    double          acc = 0.0;
-    for (long long k = threadIdx.x; k < K; k += 256) acc += vi[k] * w[k];
-    acc = wave_sum(acc);
+    const long long len = ((K + nseg - 1) / nseg + 255) / 256 * 256;
+    const long long k0 = sg * len, k1 = min(K, k0 + len);
+    const double*   vi = g.V + (long long)i * g.nrhs * K + (long long)r * K;
+    const double*   w  = g.Wv + (long long)r * K;
+    double          a0 = 0.0, a1 = 0.0;
+    long long       k  = k0 + threadIdx.x;
+    for (; k + 256 < k1; k += 512) {
+        a0 += vi[k] * w[k];
+        a1 += vi[k + 256] * w[k + 256];
+    }
+    if (k < k1) a0 += vi[k] * w[k];
+    const double acc = wave_sum(a0 + a1);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
     __syncthreads();
-    if (threadIdx.x == 0) g.part2[(long long)r * (g.m + 1) + i] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+    if (threadIdx.x == 0)
+        g.part2[((long long)r * (g.m + 1) + i) * kDotSeg + sg] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
 }
 
 // w -= sum_i h_i V_i; Hacc (= h on pass 0, += h on pass 1); partial norms.
@@ -1032,14 +1042,19 @@ __global__ __launch_bounds__(256) void k_gm_dots(GmresDev g, int j)
 // GU wavefronts (a fixed order: wavefront q sums i = q, q + GU, ...; the GU
 // partial sums are added pairwise), so each wavefront's dependent chain of
 // loads is a GU-th of j + 1.
-__global__ __launch_bounds__(GB * GU) void k_gm_update(GmresDev g, int j, int pass)
+__global__ __launch_bounds__(GB * GU) void k_gm_update(GmresDev g, int j, int pass, int nseg)
 {
     __shared__ double hs[1024];
     __shared__ double red[GU - 1][GB];
     const int r = blockIdx.y;
     if (!g.state[r * 4 + 0]) return;
     const int m = g.m;
-    for (int i = threadIdx.x; i <= j; i += GB * GU) hs[i] = g.part2[(long long)r * (m + 1) + i];
+    for (int i = threadIdx.x; i <= j; i += GB * GU) {
+        const double* ps = g.part2 + ((long long)r * (m + 1) + i) * kDotSeg;
+        double        h  = ps[0];
+        for (int sg = 1; sg < nseg; sg++) h += ps[sg];
+        hs[i] = h;
+    }
     __syncthreads();
     if (blockIdx.x == 0) {
         double* H = g.Hacc + (long long)r * (m + 1);
@@ -1349,10 +1364,11 @@ bool gm_resid(const GmresDev& g, hipStream_t s)
 
 void gm_arnoldi(const GmresDev& g, int j, hipStream_t s)
 {
+    const int nseg = gm_nseg(g.K);
     for (int pass = 0; pass < 2; pass++) {
-        hipLaunchKernelGGL(k_gm_dots, dim3(j + 1, g.nrhs), dim3(256), 0, s, g, j);
+        hipLaunchKernelGGL(k_gm_dots, dim3(j + 1, g.nrhs, nseg), dim3(256), 0, s, g, j, nseg);
         FP_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_gm_update, dim3(g.nblk, g.nrhs), dim3(GB * GU), 0, s, g, j, pass);
+        hipLaunchKernelGGL(k_gm_update, dim3(g.nblk, g.nrhs), dim3(GB * GU), 0, s, g, j, pass, nseg);
         FP_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(k_gm_finish, dim3(g.nblk, g.nrhs), dim3(GB), 0, s, g, j);
