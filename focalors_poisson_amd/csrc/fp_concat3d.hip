@@ -75,6 +75,11 @@ struct Node3 {
     std::vector<Link3> ch;
     int                K = 0;
     DevBuf<int>        cell;  // the K face cells of this node, children in link order
+    // the distinct face cells (nu) and, per cell, the up to three positions
+    // in the K face values that land on it (edge and corner cells belong to
+    // two or three faces), in link order, -1 past the last
+    int                nu = 0;
+    DevBuf<int>        ucell, uk;
     // GMRES workspace for ws_nrhs systems
     int            ws_nrhs = 0, ws_m = 0, nblk = 1;
     DevBuf<double> V, Wv, Tv, X, C, H, Hacc, cs, sn, g, Y, part, part2, tolabs, resid, rnorm, G;
@@ -164,9 +169,24 @@ static GmresDev gdev3(Cat3* c, Node3& nd, int nrhs)
 
 static void node3_solve(Cat3* c, int d, double* f, int nrhs, hipStream_t s);
 
-// G = E S v scattered onto the root's face cells (each child's product added
-// in turn: edge and corner cells shared by two or three faces receive every
-// contribution, as the reference's accumulation of the S products).
+// G[ucell[t]] = the sum, in link order, of w at the up to three positions
+// uk[3t..3t+2] (one thread per distinct face cell, so no two write one cell)
+__global__ void k_face_scatter(double* G, long long ldG, const int* ucell, const int* uk, int nu, const double* w,
+                               long long ldw)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nu) return;
+    const long long r  = blockIdx.y;
+    const double*   wr = w + r * ldw;
+    double          v  = wr[uk[3 * t]];
+    if (uk[3 * t + 1] >= 0) v += wr[uk[3 * t + 1]];
+    if (uk[3 * t + 2] >= 0) v += wr[uk[3 * t + 2]];
+    G[r * ldG + ucell[t]] = v;
+}
+
+// G = E S v scattered onto the root's face cells: edge and corner cells
+// shared by two or three faces receive every contribution, in the order of
+// the reference's accumulation of the S products (child by child).
 static void schur_scatter(Cat3* c, Node3& nd, const double* v, int nrhs, hipStream_t s)
 {
     const long long         n = n3(c, nd.dom);
@@ -174,7 +194,11 @@ static void schur_scatter(Cat3* c, Node3& nd, const double* v, int nrhs, hipStre
     for (auto& L : nd.ch) ch.push_back({L.S.p, L.cn, L.off, L.off, 0});
     launch_schur_gemv_all(ch.data(), (int)ch.size(), v, nd.K, nd.Tv.p, nd.K, nrhs, s);  // every face in one launch
     launch_fill<double>(nd.G.p, n * nrhs, 0.0, s);
-    for (auto& L : nd.ch) index_add(nd.G.p, n, L.pcell.p, nd.Tv.p + L.off, nd.K, nullptr, L.cn, 1.0, nrhs, s);
+    if (nd.nu > 0) {
+        hipLaunchKernelGGL(k_face_scatter, dim3((nd.nu + 255) / 256, nrhs), dim3(256), 0, s, nd.G.p, n, nd.ucell.p,
+                           nd.uk.p, nd.nu, nd.Tv.p, (long long)nd.K);
+        FP_HIP(hipGetLastError());
+    }
 }
 
 // out = v - R A^-1 E S v  (GMRESSolver3D::Afun on face vectors)
@@ -322,6 +346,25 @@ static void build_node3(Cat3* c, int d, hipStream_t s)
     if (nd->K) {
         nd->cell.alloc(cells.size());
         nd->cell.upload(cells.data(), cells.size());
+        std::vector<int> first(n3(c, d), -1), uc, uk;
+        for (int k = 0; k < nd->K; k++) {
+            int& u = first[cells[k]];
+            if (u < 0) {
+                u = (int)uc.size();
+                uc.push_back(cells[k]);
+                uk.insert(uk.end(), {k, -1, -1});
+            } else {
+                int q = 1;
+                while (q < 3 && uk[3 * u + q] >= 0) q++;
+                FP_REQUIRE(q < 3, "a face cell on more than three faces");
+                uk[3 * u + q] = k;
+            }
+        }
+        nd->nu = (int)uc.size();
+        nd->ucell.alloc(uc.size());
+        nd->ucell.upload(uc.data(), uc.size());
+        nd->uk.alloc(uk.size());
+        nd->uk.upload(uk.data(), uk.size());
     }
     c->nodes[d] = std::move(nd);
     for (auto& L : c->nodes[d]->ch) build_schur3(c, L, s);
