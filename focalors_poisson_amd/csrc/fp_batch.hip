@@ -69,7 +69,7 @@ struct Batch : BatchBase {
             j.dst64 = sizeof(T) == 8;
             apply_batch_shape(j, shape);
             tj.push_back(j);
-            rj.push_back({&sp, static_cast<T*>(d_f[k]), (long long)sp.nx * sp.ny, nrhs[k]});
+            rj.push_back({&sp, static_cast<T*>(d_f[k]), (long long)sp.nx * sp.ny, nrhs[k], 1});
         }
         fw  = plan_ytransform_groups<T>(tj.data(), (int)tj.size(), 0);
         tri = plan_tridiag_groups<T>(rj.data(), (int)rj.size());

@@ -236,6 +236,11 @@ struct SubPlan {
     DevBuf<double>        pst;      // ny*tri_p: pivot excess e = 1 + 1/pivot before each row chunk (mode-major)
     int                   tri_ch = 16, tri_p = 0;        // rows per chunk of the x sweep, chunks per mode
     int                   mb_field = 4, mb_lines = 4;    // modes per workgroup of the two sweeps
+    // the field sweep's shape in batches of many problems (the throughput
+    // regime: longer chunks, more modes per workgroup, whole cache lines per
+    // row segment), and its chunk-start table when the chunk length differs
+    int                   tri_ch_b = 0, tri_p_b = 0, mb_b = 0;
+    DevBuf<double>        pst_b;
     // periodic x: per mode A_D^-1 e_0 then A_D^-1 e_{nx-1} (2 nx ny, field layout)
     bool                  cyc = false;
     DevBuf<double>        zc;
@@ -245,6 +250,7 @@ struct SubPlan {
     void init_sweep(const std::vector<double>& delta);
     void ensure_f32();
     void set_chunks(int ch);
+    void set_batch_chunks();
     void tune_tridiag(bool measure);
 };
 
@@ -310,6 +316,7 @@ struct TridiagJob {
     T*             data;
     long long      ld;
     int            nrhs;
+    int            thru = 0;  // 1: part of a batch of many problems (the plan's batch shape)
 };
 template <typename T>
 void launch_tridiag_batch(const TridiagJob<T>* jobs, int njobs, hipStream_t s);
@@ -412,7 +419,7 @@ void sub_solve(SubPlan& sp, T* data, long long ld, int nrhs, hipStream_t s);
 // fields of plans[k] at data[k] (ld[k] apart).
 template <typename T>
 void sub_solve_batch(SubPlan* const* plans, T* const* data, const long long* ld, const int* nrhs, int n,
-                     hipStream_t s);
+                     hipStream_t s, bool thru = false);
 
 }  // namespace fp
 

@@ -100,6 +100,7 @@ void SubPlan::init_sweep(const std::vector<double>& dl)
     // chasing_method_base.cpp:33-46) are regenerated inside the sweep from
     // one stored pivot excess per row chunk; the chunk schedule is tuned here
     tune_tridiag(measure);
+    set_batch_chunks();
 }
 
 void SubPlan::build(int nx_, int ny_, double hx_, double hy_, const int bc_[4])
@@ -155,7 +156,7 @@ template void sub_solve<float>(SubPlan&, float*, long long, int, hipStream_t);
 
 template <typename T>
 void sub_solve_batch(SubPlan* const* plans, T* const* data, const long long* ld, const int* nrhs, int n,
-                     hipStream_t s)
+                     hipStream_t s, bool thru)
 {
     std::vector<TransformJob<T>> tj;
     std::vector<TridiagJob<T>>   rj;
@@ -173,15 +174,17 @@ void sub_solve_batch(SubPlan* const* plans, T* const* data, const long long* ld,
         j.nrhs  = nrhs[k];
         j.dst64 = sizeof(T) == 8;
         tj.push_back(j);
-        rj.push_back({&sp, data[k], ld[k], nrhs[k]});
+        rj.push_back({&sp, data[k], ld[k], nrhs[k], thru ? 1 : 0});
     }
     if (tj.empty()) return;
     launch_ytransform_batch<T>(tj.data(), (int)tj.size(), 0, s);
     launch_tridiag_batch<T>(rj.data(), (int)rj.size(), s);
     launch_ytransform_batch<T>(tj.data(), (int)tj.size(), 1, s);
 }
-template void sub_solve_batch<double>(SubPlan* const*, double* const*, const long long*, const int*, int, hipStream_t);
-template void sub_solve_batch<float>(SubPlan* const*, float* const*, const long long*, const int*, int, hipStream_t);
+template void sub_solve_batch<double>(SubPlan* const*, double* const*, const long long*, const int*, int, hipStream_t,
+                                      bool);
+template void sub_solve_batch<float>(SubPlan* const*, float* const*, const long long*, const int*, int, hipStream_t,
+                                     bool);
 
 // FP32 transforms and sweeps on FP64 fields: the forward transform reads the
 // FP64 field and writes FP32 coefficients to scratch[k], the sweep runs there,
@@ -1749,7 +1752,7 @@ void solve_batched(fp_ctx ctx, int nprob, const fp_sub* subs, T* const* d_f, con
         plans[k] = &subs[k]->plan;
         ld[k]    = (long long)subs[k]->plan.nx * subs[k]->plan.ny;
     }
-    fp::sub_solve_batch<T>(plans.data(), d_f, ld.data(), nrhs, nprob, (hipStream_t)stream);
+    fp::sub_solve_batch<T>(plans.data(), d_f, ld.data(), nrhs, nprob, (hipStream_t)stream, nprob > 1);
 }
 }  // namespace
 
