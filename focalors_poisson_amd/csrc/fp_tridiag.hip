@@ -122,8 +122,11 @@ __device__ void chain_chunks(const T* SA, const T* SB, T* SY, int P, int MB, int
 // (a quad for MB = 4), reduced by DPP quad permutations (xor 2, then xor 1:
 // the order of a shuffle tree, so every lane of the quad holds the same
 // bits); rows are taken MB at a time so that lane jl stores row k0 + jl and
-// one store instruction covers MB consecutive rows of every chunk.  Other MB
-// reduce through the LDS crossbar and lane 0 stores.
+// one store instruction covers MB consecutive rows of every chunk.  MB = 8
+// takes the 8 lanes of a half-row: the half-row mirror first (lane l and
+// 7 - l), then the two quad swaps; each lane adds the same four pair sums,
+// two by two, so all 8 hold the same bits.  Other MB reduce through the LDS
+// crossbar and lane 0 stores.
 template <typename T, int CH>
 __device__ __forceinline__ void store_col_partials(const T (&fv)[CH], T p0, T p1, int MB, int jl, int c, int P,
                                                    int i0, int nx, T* out)
@@ -168,6 +171,38 @@ __device__ __forceinline__ void store_col_partials(const T (&fv)[CH], T p0, T p1
                     out[i0 + kb]      = b0;
                     out[nx + i0 + kb] = b1;
                 }
+            }
+        }
+        return;
+    }
+    if (CH >= 16 && MB == 8) {
+        // the 8 lanes of a half-row: the mirror (lane 7 - l) pairs them, the
+        // two quad swaps add the four pair sums, so every lane holds the
+        // total and lane jl stores row k0 + jl (not compiled for 8-row
+        // chunks, whose sweep it would take from 72 to 92 VGPRs)
+#pragma unroll
+        for (int k0 = 0; k0 < CH; k0 += 8) {
+            T a0 = T(0), a1 = T(0);
+#pragma unroll
+            for (int t = 0; t < 8; t++) {
+                const int k = k0 + t;
+                if (k >= CH) break;
+                T s0 = p0 * fv[k], s1 = p1 * fv[k];
+                s0 += dpp<0x141, 0xf>(s0, s0);  // row_half_mirror: lane 7 - l
+                s1 += dpp<0x141, 0xf>(s1, s1);
+                s0 += dpp<0x4E, 0xf>(s0, s0);   // lane ^ 2
+                s1 += dpp<0x4E, 0xf>(s1, s1);
+                s0 += dpp<0xB1, 0xf>(s0, s0);   // lane ^ 1
+                s1 += dpp<0xB1, 0xf>(s1, s1);
+                if (t == jl) {
+                    a0 = s0;
+                    a1 = s1;
+                }
+            }
+            const int ka = k0 + jl;
+            if (c < P && ka < CH && i0 + ka < nx) {
+                out[i0 + ka]      = a0;
+                out[nx + i0 + ka] = a1;
             }
         }
         return;

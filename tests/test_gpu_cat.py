@@ -288,7 +288,7 @@ def test_periodic_x_root_with_child(ctx):
     assert rel(us, ref) < BAR, info
 
 
-@pytest.mark.parametrize("shape", ["8,1", "8,4", "16,16", "32,8"])
+@pytest.mark.parametrize("shape", ["8,1", "8,4", "16,8", "16,16", "32,8"])
 def test_cross_sweep_shapes(ctx, monkeypatch, shape):
     """The composite sweep at forced chunk lengths and modes per workgroup
     (FP_TRI_SHAPE): the same solution as the oracle at each shape."""
